@@ -1,0 +1,239 @@
+"""Benchmark of the FE core on one MI355X.
+
+Step (the timed unit): on the P1 Poisson 2048x2048 structured triangle mesh
+(8,388,608 elements, 4,198,401 nodes, 29,372,417 CSR entries), FP64,
+kappa_e = exp(0.5 z_e) (seed 2), f ~ N(0,1) (seed 3), u ~ N(0,1) (seed 4):
+  1. assemble the CSR Jacobian and the residual K u - F   (fx_p1_assemble)
+  2. rhs = -res; Dirichlet elimination on all edges, g = 0 (fx_bc_prepare_forward)
+  3. 10 SpMVs y_k = A x_k with x_k ~ U(-1,1) (seeds 10..19)   (fx_csr_matvec)
+All inputs are resident in HBM before timing.  `value` is elements per
+second over whole steps.  `roofline` is the assembly kernel's algorithmic
+bytes per launch (coords, conn, kappa, f, u, femx's jac_map, CSR values,
+residual) over its mean duration from HIP events on its stream.  The
+`cpu_baseline` times the same step on the host with the oracle's OpenMP
+restatement of femx's CPU assembly (element loop, thread-private scratch,
+atomic scatter) plus femx's serial host elimination and matvec.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "elements/s assembled (Jac+res) & VJPs/s, %HBM roofline; batched vs plain GPU"
+
+
+def p1_alg_bytes(N, E, nnz):
+    """Algorithmic bytes of one P1 assembly launch: each input read once,
+    each output written once, plus femx's element->CSR map (jac_map)."""
+    return 2 * N * 8 + 3 * E * 4 + E * 8 + N * 8 + N * 8 + 9 * E * 4 + nnz * 8 + N * 8
+
+
+def vjp_alg_bytes(N, E):
+    """coords, conn, m, lam, u, node->element incidence + offsets, J^T lam, g."""
+    return 2 * N * 8 + 3 * E * 4 + E * 8 + N * 8 + N * 8 + 3 * E * 4 + (N + 1) * 4 + N * 8 + E * 8
+
+
+def spmv_alg_bytes(N, nnz):
+    return nnz * 8 + nnz * 4 + (N + 1) * 4 + N * 8 + N * 8
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-vjp", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
+              "the multi-GPU partition is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from femx_amd import _lib
+    from femx_amd import assembly as A
+    from femx_amd.inverse import PoissonLogKappaResidual
+    from femx_amd.linalg import CsrMatrix, MatrixHandler, VectorHandler, ptr, stream_handle
+
+    _lib.lib()
+    dev = "cuda"
+    n = args.n
+    log(f"[bench] building P1 {n}x{n} mesh and maps")
+    geom = A.structured_p1(n, dev)
+    N, E = geom.num_nodes, geom.num_elems
+    amap = A.make_assembly_map(N, N, geom.h_conn)
+    nnz = amap.pattern.nnz
+    x, y = geom.h_x, geom.h_y
+    dofs = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+    bmap = A.make_boundary_map(dofs, amap.pattern, dev)
+
+    kappa = np.exp(0.5 * np.random.default_rng(2).standard_normal(E))
+    f = np.random.default_rng(3).standard_normal(N)
+    u = np.random.default_rng(4).standard_normal(N)
+    xs = [np.random.default_rng(10 + k).uniform(-1.0, 1.0, N) for k in range(10)]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    d_kappa, d_f, d_u = t(kappa), t(f), t(u)
+    d_xs = [t(v) for v in xs]
+    d_ys = [torch.empty(N, dtype=torch.float64, device=dev) for _ in range(10)]
+    jac = CsrMatrix(amap.pattern, dev)
+    res = torch.empty(N, dtype=torch.float64, device=dev)
+    rhs = torch.empty(N, dtype=torch.float64, device=dev)
+    g0 = torch.zeros(dofs.size, dtype=torch.float64, device=dev)
+    op = A.PoissonP1(d_kappa, d_f)
+    mh, vh = MatrixHandler(), VectorHandler()
+    rp, ci = amap.pattern.device_arrays(dev)
+    inc_off, inc = amap.device("inc_off", dev), amap.device("inc", dev)
+    stream = stream_handle()
+
+    def assemble_launch():
+        _lib.call("fx_p1_assemble", N, E, nnz, amap.max_row_len, ptr(geom.x), ptr(geom.y),
+                  ptr(geom.conn), ptr(d_kappa), 0, ptr(d_f), ptr(d_u), ptr(inc_off), ptr(inc),
+                  ptr(rp), ptr(jac.vals), ptr(res), stream)
+
+    def step():
+        A.assemble(op, geom, amap, d_u, res, jac)
+        vh.axpby(-1.0, res, 0.0, rhs)
+        A.prepare_forward_solve(bmap, jac, rhs, g0)
+        for k in range(10):
+            mh.matvec(jac, d_xs[k], d_ys[k])
+
+    log("[bench] warmup")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    log(f"[bench] timing {args.steps} steps")
+    ev0.record()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = ev0.elapsed_time(ev1) / args.steps
+
+    # Dominant kernel alone: mean duration over back-to-back launches.
+    def time_kernel(fn, reps):
+        for _ in range(2):
+            fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps * 1e-3
+
+    reps = max(args.steps, 10)
+    t_asm = time_kernel(assemble_launch, reps)
+    t_spmv = time_kernel(lambda: mh.matvec(jac, d_xs[0], d_ys[0]), reps)
+    t_bc = time_kernel(lambda: A.prepare_forward_solve(bmap, jac, rhs, g0), reps)
+    b_asm = p1_alg_bytes(N, E, nnz)
+    b_spmv = spmv_alg_bytes(N, nnz)
+    kernels = {
+        "p1_assemble": {"us": t_asm * 1e6, "alg_bytes": b_asm,
+                        "GBs": b_asm / t_asm / 1e9, "elements_per_s": E / t_asm},
+        "csr_spmv": {"us": t_spmv * 1e6, "alg_bytes": b_spmv, "GBs": b_spmv / t_spmv / 1e9},
+        "bc_prepare_forward": {"us": t_bc * 1e6},
+    }
+
+    extra = {}
+    if not args.no_vjp:
+        nv = 1024
+        log(f"[bench] VJP P1 {nv}x{nv}")
+        gv = A.structured_p1(nv, dev)
+        xv, yv = gv.h_x, gv.h_y
+        vd = np.nonzero((xv == 0) | (xv == 1) | (yv == 0) | (yv == 1))[0].astype(np.int32)
+        Rv = PoissonLogKappaResidual(gv, vd)
+        m = t(0.5 * np.random.default_rng(6).standard_normal(gv.num_elems))
+        lam = t(np.random.default_rng(7).standard_normal(gv.num_nodes))
+        uv = t(np.random.default_rng(8).standard_normal(gv.num_nodes))
+        jt = torch.empty(gv.num_nodes, dtype=torch.float64, device=dev)
+        gg = torch.empty(gv.num_elems, dtype=torch.float64, device=dev)
+        t_vjp = time_kernel(lambda: Rv.vjp(uv, m, lam, jt, gg), reps)
+        b_vjp = vjp_alg_bytes(gv.num_nodes, gv.num_elems)
+        extra["vjp_p1_1024"] = {"VJPs_per_s": 1.0 / t_vjp, "us": t_vjp * 1e6,
+                                "elements_per_s": gv.num_elems / t_vjp, "alg_bytes": b_vjp,
+                                "GBs": b_vjp / t_vjp / 1e9,
+                                "frac": b_vjp / t_vjp / 1e9 / HBM_PEAK_GBS}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        log("[bench] CPU baseline (oracle OpenMP assembly + host elimination + 10 matvec)")
+        _, jm = amap.jac()
+        bc = dict(bc_rows=bmap.bc_rows, diag=bmap.diag, col_offsets=bmap.col_offsets,
+                  col_entries=bmap.col_entries, col_rows=bmap.col_rows, bc_mask=bmap.bc_mask)
+        conn = geom.h_conn.reshape(-1)
+        rp_h, ci_h = amap.pattern.row_ptr, amap.pattern.col_ind
+        yk = np.empty(N)
+
+        def cpu_step():
+            vals, r = O.p1_assemble(x, y, conn, kappa, f, u, jm, nnz, omp=True)
+            rr = -r
+            O.prepare_forward(rp_h, bc, vals, rr, np.zeros(dofs.size))
+            for k in range(10):
+                O.matvec(rp_h, ci_h, vals, xs[k], yk)
+
+        cpu_step()
+        times = []
+        t_start = time.perf_counter()
+        while len(times) < 5 and (time.perf_counter() - t_start) < 12.0:
+            a = time.perf_counter()
+            cpu_step()
+            times.append(time.perf_counter() - a)
+        tc = float(np.median(times))
+        cpu = {"value": E / tc, "unit": "elements/s", "cores": O.omp_threads(), "kind": "port",
+               "sample": f"{len(times)} full steps of the same workload (P1 {n}^2, "
+                         f"{E} elements) after 1 warm-up, median; OpenMP assembly on "
+                         f"{O.omp_threads()} threads, serial elimination and matvec",
+               "ms_per_step": tc * 1e3}
+
+    out = {
+        "metric": METRIC,
+        "value": E / (ms_step * 1e-3),
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"P1 Poisson {n}x{n} structured triangles: Jacobian+residual+load "
+                               "assembly, Dirichlet elimination, 10 SpMV",
+                   "elements": E, "nodes": N, "nnz": nnz, "spmv_per_step": 10},
+        "roofline": {"bound": "hbm", "achieved": b_asm / t_asm / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": b_asm / t_asm / 1e9 / HBM_PEAK_GBS,
+                     "traffic": None, "kernel": "k_p1_assemble"},
+        "cpu_baseline": cpu,
+        "kernels": kernels,
+        "extra": extra,
+        "wall_s_timed": wall,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,96 @@
+"""ctypes binding of libfemx_amd.so (the C-ABI in include/femx_amd.h).
+
+The library is built in-tree by femx_amd.build.  Loading fails loudly when
+it is missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfemx_amd.so")
+
+_i32 = C.c_int32
+_f64 = C.c_double
+_vp = C.c_void_p
+_pi = C.POINTER(C.c_int32)
+
+_SIGS = {
+    "fx_version": ([], C.c_int),
+    "fx_last_error": ([], C.c_char_p),
+    "fx_mesh_p1": ([_i32, _vp, _vp, _vp], C.c_int),
+    "fx_mesh_p2": ([_i32, _vp, _vp, _vp, _vp], C.c_int),
+    "fx_th_dofs": ([_i32, _vp, _vp, _vp], C.c_int),
+    "fx_map_create": ([_i32, _i32, _i32, _vp, _vp, _vp, _vp, C.POINTER(_vp)], C.c_int),
+    "fx_map_info_get": ([_vp, _vp], C.c_int),
+    "fx_map_export_pattern": ([_vp, _vp, _vp], C.c_int),
+    "fx_map_export_jac": ([_vp, _vp, _vp], C.c_int),
+    "fx_map_export_rowinc3": ([_vp, _vp, _vp], C.c_int),
+    "fx_map_destroy": ([_vp], None),
+    "fx_csr_transpose_map": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp], C.c_int),
+    "fx_bcmap_create": ([_i32, _vp, _vp, _i32, _vp, C.POINTER(_vp)], C.c_int),
+    "fx_bcmap_info_get": ([_vp, _vp], C.c_int),
+    "fx_bcmap_export": ([_vp] * 7, C.c_int),
+    "fx_bcmap_export_elim": ([_vp] * 5, C.c_int),
+    "fx_bcmap_destroy": ([_vp], None),
+    "fx_bc_replace_rows": ([_i32, _vp, _vp, _vp, _vp, _f64, _vp], C.c_int),
+    "fx_bc_replace_res": ([_i32, _vp, _vp, _vp, _vp, _vp], C.c_int),
+    "fx_bc_zero": ([_i32, _vp, _vp, _vp], C.c_int),
+    "fx_bc_prepare_forward": ([_i32, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+                              C.c_int),
+    "fx_csr_matvec": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _vp], C.c_int),
+    "fx_csr_matvec_t": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _vp], C.c_int),
+    "fx_vec_axpby": ([_i32, _f64, _vp, _f64, _vp, _vp], C.c_int),
+    "fx_dot_work_len": ([], _i32),
+    "fx_vec_dot": ([_i32, _vp, _vp, _vp, _vp, _vp], C.c_int),
+    "fx_p1_assemble": ([_i32, _i32, _i32, _i32] + [_vp] * 4 + [C.c_int] + [_vp] * 7 + [_vp],
+                       C.c_int),
+    "fx_p1_vjp": ([_i32, _i32] + [_vp] * 11 + [_vp], C.c_int),
+}
+
+
+class MapInfo(C.Structure):
+    _fields_ = [(n, _i32) for n in ("num_elems", "num_res", "num_states", "nnz", "jac_len",
+                                    "max_res", "max_state", "max_jac", "max_row_len", "num_inc")]
+
+
+class BcMapInfo(C.Structure):
+    _fields_ = [(n, _i32) for n in ("rows", "num_bcs", "num_col_entries", "num_elim_rows",
+                                    "num_elim_entries")]
+
+
+_LIB = None
+
+
+def lib():
+    """The loaded library.  Raises when it has not been built."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"femx_amd: {LIB_PATH} is missing; build it with `python -m femx_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return list(_SIGS)
+
+
+def check(rc):
+    """Raise RuntimeError with the library's message for a non-zero status,
+    as femx's require() throws std::runtime_error (femx/common/Checks.hpp:11-17)."""
+    if rc != 0:
+        msg = lib().fx_last_error().decode()
+        raise RuntimeError(msg or f"femx_amd error {rc}")
+    return rc
+
+
+def call(name, *args):
+    return check(getattr(lib(), name)(*args))

@@ -1,0 +1,517 @@
+// Host-side builders of the femx_amd C-ABI: structured meshes, the assembly
+// map (CSR pattern, femx's jac_map and the row-incidence tables of the
+// row-gather kernels), the boundary map and the CSR transpose permutation.
+//
+// The map builder produces femx's exact arrays (femx/assembly/
+// AssemblyMap.cpp:54-176) without femx's global COO sort: it buckets the
+// local rows by global row (stable in element order), then sorts and
+// deduplicates each row's columns on its own.  Rows are independent, so the
+// per-row passes run on all host threads.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "fx_common.hpp"
+
+using Index = fx_index;
+
+namespace fx
+{
+namespace
+{
+thread_local std::string g_last_error;
+}
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace
+{
+
+// Run fn(begin, end) over [0, n) on up to hardware_concurrency() threads.
+template <class F>
+void parallel_ranges(std::int64_t n, F&& fn)
+{
+  unsigned nt = std::max(1u, std::thread::hardware_concurrency());
+  nt          = static_cast<unsigned>(std::min<std::int64_t>(nt, std::max<std::int64_t>(1, n / 4096)));
+  if (nt <= 1)
+  {
+    fn(std::int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(nt);
+  for (unsigned t = 0; t < nt; ++t)
+  {
+    const std::int64_t b = n * t / nt, e = n * (t + 1) / nt;
+    pool.emplace_back([&fn, b, e] { fn(b, e); });
+  }
+  for (auto& th : pool) th.join();
+}
+
+} // namespace
+} // namespace fx
+
+struct fx_map
+{
+  Index num_elems = 0, num_res = 0, num_states = 0;
+  Index max_res = 0, max_state = 0, max_jac = 0, max_row_len = 0;
+  std::vector<Index> res_off, res_dofs, st_off, st_dofs, jac_off;
+  std::vector<Index> row_ptr, col_ind, jac_map;
+  std::vector<Index> inc_off, inc_elem, inc_local; // row incidence, ascending element per row
+};
+
+struct fx_bcmap
+{
+  Index rows = 0;
+  std::vector<Index> bc_rows, diag, col_off, col_ent, col_rows, mask;
+  std::vector<Index> elim_rows, elim_off, elim_k, elim_ib;
+};
+
+extern "C" {
+
+const char* fx_last_error(void) { return fx::g_last_error.c_str(); }
+
+int fx_version(void) { return 1; }
+
+// ---------------------------------------------------------------------------
+// Structured meshes (conventions stated in include/femx_amd.h).  Coordinates
+// are x_min + i*dx as femx's Mesh::makeStructuredQuad computes them
+// (femx/fem/Mesh.cpp:17-28).
+
+int fx_mesh_p1(Index n, double* x, double* y, Index* conn)
+{
+  return fx::guard([&] {
+    fx::require(n > 0 && x && y && conn, FX_ERR_ARG, "fx_mesh_p1: invalid arguments");
+    fx::require(static_cast<std::int64_t>(n + 1) * (n + 1) < std::numeric_limits<Index>::max()
+                    && 6LL * n * n < std::numeric_limits<Index>::max(),
+                FX_ERR_OVERFLOW, "fx_mesh_p1: mesh exceeds the int32 index range");
+    const double dx = 1.0 / static_cast<double>(n);
+    const Index  np = n + 1;
+    for (Index j = 0; j <= n; ++j)
+      for (Index i = 0; i <= n; ++i)
+      {
+        x[j * np + i] = 0.0 + static_cast<double>(i) * dx;
+        y[j * np + i] = 0.0 + static_cast<double>(j) * dx;
+      }
+    for (Index j = 0; j < n; ++j)
+      for (Index i = 0; i < n; ++i)
+      {
+        const Index a = j * np + i, b = a + 1, d = a + np, c = d + 1;
+        Index*      e = conn + 6 * (static_cast<std::int64_t>(j) * n + i);
+        e[0] = a, e[1] = b, e[2] = c;
+        e[3] = a, e[4] = c, e[5] = d;
+      }
+  });
+}
+
+int fx_mesh_p2(Index n, double* x, double* y, Index* conn6, Index* vconn3)
+{
+  return fx::guard([&] {
+    fx::require(n > 0 && x && y && conn6 && vconn3, FX_ERR_ARG, "fx_mesh_p2: invalid arguments");
+    fx::require(static_cast<std::int64_t>(2 * n + 1) * (2 * n + 1) * 2 + (n + 1) * (n + 1)
+                        < std::numeric_limits<Index>::max()
+                    && 12LL * n * n < std::numeric_limits<Index>::max(),
+                FX_ERR_OVERFLOW, "fx_mesh_p2: mesh exceeds the int32 index range");
+    const Index  m = 2 * n, mp = m + 1, np = n + 1;
+    const double dx = 1.0 / static_cast<double>(m);
+    for (Index J = 0; J <= m; ++J)
+      for (Index I = 0; I <= m; ++I)
+      {
+        x[J * mp + I] = 0.0 + static_cast<double>(I) * dx;
+        y[J * mp + I] = 0.0 + static_cast<double>(J) * dx;
+      }
+    auto L = [mp](Index I, Index J) { return J * mp + I; };
+    for (Index j = 0; j < n; ++j)
+      for (Index i = 0; i < n; ++i)
+      {
+        const Index        I = 2 * i, J = 2 * j;
+        const std::int64_t c = static_cast<std::int64_t>(j) * n + i;
+        Index*             e = conn6 + 12 * c;
+        e[0] = L(I, J), e[1] = L(I + 2, J), e[2] = L(I + 2, J + 2);
+        e[3] = L(I + 1, J), e[4] = L(I + 2, J + 1), e[5] = L(I + 1, J + 1);
+        e[6] = L(I, J), e[7] = L(I + 2, J + 2), e[8] = L(I, J + 2);
+        e[9] = L(I + 1, J + 1), e[10] = L(I + 1, J + 2), e[11] = L(I, J + 1);
+        Index*      v = vconn3 + 6 * c;
+        const Index a = j * np + i, b = a + 1, d = a + np, cc = d + 1;
+        v[0] = a, v[1] = b, v[2] = cc;
+        v[3] = a, v[4] = cc, v[5] = d;
+      }
+  });
+}
+
+int fx_th_dofs(Index n, const Index* conn6, const Index* vconn3, Index* dofs15)
+{
+  return fx::guard([&] {
+    fx::require(n > 0 && conn6 && vconn3 && dofs15, FX_ERR_ARG, "fx_th_dofs: invalid arguments");
+    const Index        nv = (2 * n + 1) * (2 * n + 1);
+    const std::int64_t E  = 2LL * n * n;
+    for (std::int64_t e = 0; e < E; ++e)
+    {
+      Index* d = dofs15 + 15 * e;
+      for (int a = 0; a < 6; ++a)
+        for (int c = 0; c < 2; ++c) d[2 * a + c] = 2 * conn6[6 * e + a] + c;
+      for (int b = 0; b < 3; ++b) d[12 + b] = 2 * nv + vconn3[3 * e + b];
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Assembly map.  Validation and messages follow femx's makeAssemblyMap
+// (femx/assembly/AssemblyMap.cpp:20-100).
+
+int fx_map_create(Index num_res, Index num_states, Index num_elems, const Index* res_offsets,
+                  const Index* res_dofs, const Index* state_offsets, const Index* state_dofs,
+                  fx_map** out)
+{
+  return fx::guard([&] {
+    fx::require(out != nullptr, FX_ERR_ARG, "fx_map_create: null output handle");
+    *out = nullptr;
+    fx::require(num_res >= 0 && num_states >= 0,
+                FX_ERR_ARG, "AssemblyMap global dimensions must be non-negative");
+    fx::require(num_elems >= 0 && res_offsets && state_offsets
+                    && (num_elems == 0 || (res_dofs && state_dofs)),
+                FX_ERR_ARG, "fx_map_create: invalid element tables");
+    auto m        = std::make_unique<fx_map>();
+    m->num_elems  = num_elems;
+    m->num_res    = num_res;
+    m->num_states = num_states;
+    m->res_off.assign(res_offsets, res_offsets + num_elems + 1);
+    m->st_off.assign(state_offsets, state_offsets + num_elems + 1);
+    fx::require(m->res_off[0] == 0 && m->st_off[0] == 0, FX_ERR_ARG,
+                "AssemblyMap element offsets must start at zero");
+    m->jac_off.assign(num_elems + 1, 0);
+    for (Index e = 0; e < num_elems; ++e)
+    {
+      const Index nr = m->res_off[e + 1] - m->res_off[e];
+      const Index nc = m->st_off[e + 1] - m->st_off[e];
+      fx::require(nr >= 0 && nc >= 0, FX_ERR_ARG,
+                  "AssemblyMap local dimensions must be non-negative");
+      const std::int64_t nloc = static_cast<std::int64_t>(nr) * nc;
+      fx::require(nloc <= std::numeric_limits<Index>::max(), FX_ERR_OVERFLOW,
+                  "AssemblyMap local Jacobian is too large");
+      const std::int64_t tot = static_cast<std::int64_t>(m->jac_off[e]) + nloc;
+      fx::require(tot <= std::numeric_limits<Index>::max(), FX_ERR_OVERFLOW,
+                  "AssemblyMap flattened data exceeds the Index range");
+      m->jac_off[e + 1] = static_cast<Index>(tot);
+      m->max_res        = std::max(m->max_res, nr);
+      m->max_state      = std::max(m->max_state, nc);
+      m->max_jac        = std::max(m->max_jac, static_cast<Index>(nloc));
+    }
+    const Index nres_tot = m->res_off[num_elems], nst_tot = m->st_off[num_elems];
+    m->res_dofs.assign(res_dofs, res_dofs + nres_tot);
+    m->st_dofs.assign(state_dofs, state_dofs + nst_tot);
+    for (Index d : m->res_dofs)
+      fx::require(d >= 0 && d < num_res, FX_ERR_ARG, "AssemblyMap residual DOF is out of range");
+    for (Index d : m->st_dofs)
+      fx::require(d >= 0 && d < num_states, FX_ERR_ARG, "AssemblyMap state DOF is out of range");
+
+    // Row incidence: bucket (element, local row) by global row, stable in
+    // element order.
+    m->inc_off.assign(num_res + 1, 0);
+    for (Index d : m->res_dofs) ++m->inc_off[d + 1];
+    for (Index r = 0; r < num_res; ++r) m->inc_off[r + 1] += m->inc_off[r];
+    m->inc_elem.resize(nres_tot);
+    m->inc_local.resize(nres_tot);
+    {
+      std::vector<Index> next(m->inc_off.begin(), m->inc_off.end() - 1);
+      for (Index e = 0; e < num_elems; ++e)
+        for (Index p = m->res_off[e]; p < m->res_off[e + 1]; ++p)
+        {
+          const Index t  = next[m->res_dofs[p]]++;
+          m->inc_elem[t]  = e;
+          m->inc_local[t] = p - m->res_off[e];
+        }
+    }
+
+    // Per-row column sets: pass 1 counts, pass 2 fills col_ind and jac_map.
+    std::vector<Index> row_len(num_res, 0);
+    auto row_cols = [&](Index r, std::vector<Index>& cols) {
+      cols.clear();
+      for (Index t = m->inc_off[r]; t < m->inc_off[r + 1]; ++t)
+      {
+        const Index e = m->inc_elem[t];
+        cols.insert(cols.end(), m->st_dofs.begin() + m->st_off[e],
+                    m->st_dofs.begin() + m->st_off[e + 1]);
+      }
+      std::sort(cols.begin(), cols.end());
+      cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    };
+    fx::parallel_ranges(num_res, [&](std::int64_t b, std::int64_t e) {
+      std::vector<Index> cols;
+      for (std::int64_t r = b; r < e; ++r)
+      {
+        row_cols(static_cast<Index>(r), cols);
+        row_len[r] = static_cast<Index>(cols.size());
+      }
+    });
+    m->row_ptr.assign(num_res + 1, 0);
+    for (Index r = 0; r < num_res; ++r)
+    {
+      m->row_ptr[r + 1] = m->row_ptr[r] + row_len[r];
+      m->max_row_len    = std::max(m->max_row_len, row_len[r]);
+    }
+    m->col_ind.resize(m->row_ptr[num_res]);
+    m->jac_map.resize(m->jac_off[num_elems]);
+    fx::parallel_ranges(num_res, [&](std::int64_t b, std::int64_t e) {
+      std::vector<Index> cols;
+      for (std::int64_t rr = b; rr < e; ++rr)
+      {
+        const Index r = static_cast<Index>(rr);
+        row_cols(r, cols);
+        const Index base = m->row_ptr[r];
+        std::copy(cols.begin(), cols.end(), m->col_ind.begin() + base);
+        for (Index t = m->inc_off[r]; t < m->inc_off[r + 1]; ++t)
+        {
+          const Index el = m->inc_elem[t], li = m->inc_local[t];
+          const Index nc = m->st_off[el + 1] - m->st_off[el];
+          Index*      jm = m->jac_map.data() + m->jac_off[el] + li * nc;
+          for (Index lj = 0; lj < nc; ++lj)
+          {
+            const Index c = m->st_dofs[m->st_off[el] + lj];
+            jm[lj] = base + static_cast<Index>(std::lower_bound(cols.begin(), cols.end(), c)
+                                               - cols.begin());
+          }
+        }
+      }
+    });
+    *out = m.release();
+  });
+}
+
+int fx_map_info_get(const fx_map* m, fx_map_info* info)
+{
+  return fx::guard([&] {
+    fx::require(m && info, FX_ERR_ARG, "fx_map_info_get: null argument");
+    info->num_elems   = m->num_elems;
+    info->num_res     = m->num_res;
+    info->num_states  = m->num_states;
+    info->nnz         = m->row_ptr.back();
+    info->jac_len     = m->jac_off.back();
+    info->max_res     = m->max_res;
+    info->max_state   = m->max_state;
+    info->max_jac     = m->max_jac;
+    info->max_row_len = m->max_row_len;
+    info->num_inc     = static_cast<Index>(m->inc_elem.size());
+  });
+}
+
+int fx_map_export_pattern(const fx_map* m, Index* row_ptr, Index* col_ind)
+{
+  return fx::guard([&] {
+    fx::require(m && row_ptr && (col_ind || m->col_ind.empty()), FX_ERR_ARG,
+                "fx_map_export_pattern: null argument");
+    std::copy(m->row_ptr.begin(), m->row_ptr.end(), row_ptr);
+    std::copy(m->col_ind.begin(), m->col_ind.end(), col_ind);
+  });
+}
+
+int fx_map_export_jac(const fx_map* m, Index* jac_offsets, Index* jac_map)
+{
+  return fx::guard([&] {
+    fx::require(m && jac_offsets && (jac_map || m->jac_map.empty()), FX_ERR_ARG,
+                "fx_map_export_jac: null argument");
+    std::copy(m->jac_off.begin(), m->jac_off.end(), jac_offsets);
+    std::copy(m->jac_map.begin(), m->jac_map.end(), jac_map);
+  });
+}
+
+int fx_map_export_rowinc3(const fx_map* m, Index* inc_off, Index* inc)
+{
+  return fx::guard([&] {
+    fx::require(m && inc_off && (inc || m->inc_elem.empty()), FX_ERR_ARG,
+                "fx_map_export_rowinc3: null argument");
+    for (Index e = 0; e < m->num_elems; ++e)
+      fx::require(m->res_off[e + 1] - m->res_off[e] == 3 && m->st_off[e + 1] - m->st_off[e] == 3,
+                  FX_ERR_UNSUPPORTED, "fx_map_export_rowinc3: every element needs 3x3 local DOFs");
+    fx::require(m->max_row_len <= 255, FX_ERR_UNSUPPORTED,
+                "fx_map_export_rowinc3: CSR rows longer than 255 entries");
+    std::copy(m->inc_off.begin(), m->inc_off.end(), inc_off);
+    for (Index r = 0; r < m->num_res; ++r)
+      for (Index t = m->inc_off[r]; t < m->inc_off[r + 1]; ++t)
+      {
+        const Index  e = m->inc_elem[t], li = m->inc_local[t];
+        const Index* jm   = m->jac_map.data() + m->jac_off[e] + 3 * li;
+        const Index  base = m->row_ptr[r];
+        std::uint32_t meta = static_cast<std::uint32_t>(li);
+        for (int j = 0; j < 3; ++j)
+          meta |= static_cast<std::uint32_t>(jm[j] - base) << (8 * (j + 1));
+        inc[2 * t]     = e;
+        inc[2 * t + 1] = static_cast<Index>(meta);
+      }
+  });
+}
+
+void fx_map_destroy(fx_map* m) { delete m; }
+
+// ---------------------------------------------------------------------------
+// CSR transpose permutation: counting sort by column, rows visited in order.
+
+int fx_csr_transpose_map(Index rows, Index cols, const Index* row_ptr, const Index* col_ind,
+                         Index* t_ptr, Index* t_row, Index* t_perm)
+{
+  return fx::guard([&] {
+    fx::require(rows >= 0 && cols >= 0 && row_ptr && t_ptr, FX_ERR_ARG,
+                "fx_csr_transpose_map: invalid arguments");
+    const Index nnz = row_ptr[rows];
+    fx::require(nnz == 0 || (col_ind && t_row && t_perm), FX_ERR_ARG,
+                "fx_csr_transpose_map: null arrays");
+    std::fill(t_ptr, t_ptr + cols + 1, 0);
+    for (Index k = 0; k < nnz; ++k)
+    {
+      fx::require(col_ind[k] >= 0 && col_ind[k] < cols, FX_ERR_ARG,
+                  "fx_csr_transpose_map: column index out of range");
+      ++t_ptr[col_ind[k] + 1];
+    }
+    for (Index c = 0; c < cols; ++c) t_ptr[c + 1] += t_ptr[c];
+    std::vector<Index> next(t_ptr, t_ptr + cols);
+    for (Index r = 0; r < rows; ++r)
+      for (Index k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+      {
+        const Index d = next[col_ind[k]]++;
+        t_row[d]      = r;
+        t_perm[d]     = k;
+      }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Boundary map.  Validation order and messages follow femx's makeBoundaryMap
+// (femx/assembly/BoundaryMap.cpp:59-139).
+
+int fx_bcmap_create(Index rows, const Index* row_ptr, const Index* col_ind, Index num_bcs,
+                    const Index* dofs, fx_bcmap** out)
+{
+  return fx::guard([&] {
+    fx::require(out != nullptr, FX_ERR_ARG, "fx_bcmap_create: null output handle");
+    *out = nullptr;
+    fx::require(rows >= 0 && num_bcs >= 0 && row_ptr && (num_bcs == 0 || dofs), FX_ERR_ARG,
+                "fx_bcmap_create: invalid arguments");
+    auto b  = std::make_unique<fx_bcmap>();
+    b->rows = rows;
+    b->bc_rows.resize(num_bcs);
+    b->diag.assign(num_bcs, -1);
+    b->col_off.assign(num_bcs + 1, 0);
+    b->mask.assign(rows, 0);
+    std::vector<Index> bc_by_col(rows, -1);
+    for (Index ib = 0; ib < num_bcs; ++ib)
+    {
+      const Index dof = dofs[ib];
+      fx::require(dof >= 0 && dof < rows, FX_ERR_ARG, "BoundaryMap constrained DOF is out of range");
+      fx::require(b->mask[dof] == 0, FX_ERR_ARG, "BoundaryMap constrained DOFs must be unique");
+      b->bc_rows[ib] = dof;
+      b->mask[dof]   = 1;
+      bc_by_col[dof] = ib;
+    }
+    for (Index r = 0; r < rows; ++r)
+      for (Index k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+      {
+        const Index c = col_ind[k];
+        fx::require(c >= 0 && c < rows, FX_ERR_ARG, "BoundaryMap requires a square CSR pattern");
+        const Index ib = bc_by_col[c];
+        if (ib >= 0)
+        {
+          ++b->col_off[ib + 1];
+          if (r == c)
+          {
+            fx::require(b->diag[ib] < 0, FX_ERR_PATTERN,
+                        "BoundaryMap constrained row has duplicate diagonal entries");
+            b->diag[ib] = k;
+          }
+        }
+      }
+    for (Index ib = 0; ib < num_bcs; ++ib)
+    {
+      fx::require(b->diag[ib] >= 0, FX_ERR_PATTERN, "BoundaryMap constrained row has no diagonal entry");
+      b->col_off[ib + 1] += b->col_off[ib];
+    }
+    const Index nce = b->col_off[num_bcs];
+    b->col_ent.resize(nce);
+    b->col_rows.resize(nce);
+    std::vector<Index> next(b->col_off.begin(), b->col_off.end() - 1);
+    for (Index r = 0; r < rows; ++r)
+      for (Index k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+      {
+        const Index ib = bc_by_col[col_ind[k]];
+        if (ib >= 0)
+        {
+          const Index d = next[ib]++;
+          b->col_ent[d]  = k;
+          b->col_rows[d] = r;
+        }
+      }
+    // Row-grouped elimination lists: entries of unconstrained rows in
+    // constrained columns, each row's entries in ascending boundary index.
+    std::vector<Index> cnt(rows + 1, 0);
+    for (Index i = 0; i < nce; ++i)
+      if (!b->mask[b->col_rows[i]]) ++cnt[b->col_rows[i] + 1];
+    for (Index r = 0; r < rows; ++r) cnt[r + 1] += cnt[r];
+    const Index ne = cnt[rows];
+    b->elim_k.resize(ne);
+    b->elim_ib.resize(ne);
+    std::vector<Index> pos(cnt.begin(), cnt.end() - 1);
+    for (Index ib = 0; ib < num_bcs; ++ib)
+      for (Index i = b->col_off[ib]; i < b->col_off[ib + 1]; ++i)
+      {
+        const Index r = b->col_rows[i];
+        if (b->mask[r]) continue;
+        const Index d = pos[r]++;
+        b->elim_k[d]  = b->col_ent[i];
+        b->elim_ib[d] = ib;
+      }
+    b->elim_off.push_back(0);
+    for (Index r = 0; r < rows; ++r)
+      if (cnt[r + 1] > cnt[r])
+      {
+        b->elim_rows.push_back(r);
+        b->elim_off.push_back(cnt[r + 1]);
+      }
+    *out = b.release();
+  });
+}
+
+int fx_bcmap_info_get(const fx_bcmap* b, fx_bcmap_info* info)
+{
+  return fx::guard([&] {
+    fx::require(b && info, FX_ERR_ARG, "fx_bcmap_info_get: null argument");
+    info->rows             = b->rows;
+    info->num_bcs          = static_cast<Index>(b->bc_rows.size());
+    info->num_col_entries  = static_cast<Index>(b->col_ent.size());
+    info->num_elim_rows    = static_cast<Index>(b->elim_rows.size());
+    info->num_elim_entries = static_cast<Index>(b->elim_k.size());
+  });
+}
+
+int fx_bcmap_export(const fx_bcmap* b, Index* bc_rows, Index* diag, Index* col_offsets,
+                    Index* col_entries, Index* col_rows, Index* bc_mask)
+{
+  return fx::guard([&] {
+    fx::require(b != nullptr, FX_ERR_ARG, "fx_bcmap_export: null map");
+    if (bc_rows) std::copy(b->bc_rows.begin(), b->bc_rows.end(), bc_rows);
+    if (diag) std::copy(b->diag.begin(), b->diag.end(), diag);
+    if (col_offsets) std::copy(b->col_off.begin(), b->col_off.end(), col_offsets);
+    if (col_entries) std::copy(b->col_ent.begin(), b->col_ent.end(), col_entries);
+    if (col_rows) std::copy(b->col_rows.begin(), b->col_rows.end(), col_rows);
+    if (bc_mask) std::copy(b->mask.begin(), b->mask.end(), bc_mask);
+  });
+}
+
+int fx_bcmap_export_elim(const fx_bcmap* b, Index* elim_rows, Index* elim_off, Index* elim_k,
+                         Index* elim_ib)
+{
+  return fx::guard([&] {
+    fx::require(b && elim_off, FX_ERR_ARG, "fx_bcmap_export_elim: null argument");
+    if (elim_rows) std::copy(b->elim_rows.begin(), b->elim_rows.end(), elim_rows);
+    std::copy(b->elim_off.begin(), b->elim_off.end(), elim_off);
+    if (elim_k) std::copy(b->elim_k.begin(), b->elim_k.end(), elim_k);
+    if (elim_ib) std::copy(b->elim_ib.begin(), b->elim_ib.end(), elim_ib);
+  });
+}
+
+void fx_bcmap_destroy(fx_bcmap* b) { delete b; }
+
+} // extern "C"

@@ -1,0 +1,158 @@
+// CSR SpMV, transpose apply and vector ops for gfx950.
+//
+// Replaces cusparseSpMV (femx/linalg/handler/MatrixHandler.cu:335-398),
+// cublasDdot and axpbyKernel (femx/linalg/handler/VectorHandler.cu:124-136,
+// 297-317).  SpMV keeps one row per lane and sums the row in CSR order, the
+// order of femx's host matvec (MatrixHandler.cpp:157-175), so results equal
+// the host's bit for bit; FE rows are short (7 entries for structured P1), so
+// a lane per row reads each wave's slice of vals/col_ind as one contiguous
+// span.  The transpose apply gathers each column through the transpose
+// permutation in ascending row order -- the order femx's host matvecT adds
+// into y (MatrixHandler.cpp:177-198) -- instead of scattering with atomics.
+#include "fx_common.hpp"
+
+namespace
+{
+constexpr int kRowBlock = 256;
+constexpr int kDotBlock = 256;
+constexpr int kDotGrid  = 1024;
+
+__global__ __launch_bounds__(kRowBlock) void k_spmv(int rows, const int* __restrict__ row_ptr,
+                                                    const int* __restrict__ col,
+                                                    const double* __restrict__ vals,
+                                                    const double* __restrict__ x,
+                                                    double* __restrict__ y, double alpha,
+                                                    double beta)
+{
+  const int r = blockIdx.x * kRowBlock + threadIdx.x;
+  if (r >= rows) return;
+  double    v  = 0.0;
+  const int k1 = row_ptr[r + 1];
+  for (int k = row_ptr[r]; k < k1; ++k) v = v + vals[k] * x[col[k]];
+  y[r] = beta == 0.0 ? alpha * v : alpha * v + beta * y[r];
+}
+
+__global__ __launch_bounds__(kRowBlock) void k_spmv_t(int cols, const int* __restrict__ t_ptr,
+                                                      const int* __restrict__ t_row,
+                                                      const int* __restrict__ t_perm,
+                                                      const double* __restrict__ vals,
+                                                      const double* __restrict__ x,
+                                                      double* __restrict__ y, double alpha,
+                                                      double beta)
+{
+  const int c = blockIdx.x * kRowBlock + threadIdx.x;
+  if (c >= cols) return;
+  double    v  = beta == 0.0 ? 0.0 : y[c] * beta;
+  const int k1 = t_ptr[c + 1];
+  for (int k = t_ptr[c]; k < k1; ++k) v = v + vals[t_perm[k]] * (alpha * x[t_row[k]]);
+  y[c] = v;
+}
+
+__global__ __launch_bounds__(kRowBlock) void k_axpby(int n, double a, const double* __restrict__ x,
+                                                     double b, double* __restrict__ y)
+{
+  const int i = blockIdx.x * kRowBlock + threadIdx.x;
+  if (i < n) y[i] = b == 0.0 ? a * x[i] : a * x[i] + b * y[i];
+}
+
+// Fixed-shape two-stage tree reduction: the grid, the stride pattern and the
+// tree are independent of timing, so the sum is reproducible bit for bit.
+__device__ __forceinline__ double block_sum(double v)
+{
+  __shared__ double s[kDotBlock];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = kDotBlock / 2; w > 0; w >>= 1)
+  {
+    if (threadIdx.x < w) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + w];
+    __syncthreads();
+  }
+  return s[0];
+}
+
+__global__ __launch_bounds__(kDotBlock) void k_dot_partial(int n, const double* __restrict__ x,
+                                                           const double* __restrict__ y,
+                                                           double* __restrict__ work)
+{
+  double v = 0.0;
+  for (int i = blockIdx.x * kDotBlock + threadIdx.x; i < n; i += kDotGrid * kDotBlock)
+    v = v + x[i] * y[i];
+  v = block_sum(v);
+  if (threadIdx.x == 0) work[blockIdx.x] = v;
+}
+
+__global__ __launch_bounds__(kDotBlock) void k_dot_final(const double* __restrict__ work,
+                                                         double* __restrict__ out)
+{
+  double v = 0.0;
+  for (int i = threadIdx.x; i < kDotGrid; i += kDotBlock) v = v + work[i];
+  v = block_sum(v);
+  if (threadIdx.x == 0) out[0] = v;
+}
+} // namespace
+
+extern "C" {
+
+int fx_csr_matvec(fx_index rows, fx_index cols, const fx_index* row_ptr, const fx_index* col_ind,
+                  const double* vals, const double* x, double* y, double alpha, double beta,
+                  void* stream)
+{
+  return fx::guard([&] {
+    fx::require(rows >= 0 && cols >= 0, FX_ERR_ARG, "fx_csr_matvec: negative size");
+    if (rows == 0) return;
+    fx::require(row_ptr && y && (cols == 0 || x), FX_ERR_ARG, "fx_csr_matvec: null array");
+    fx::require(!fx::overlaps(x, std::size_t(cols) * 8, y, std::size_t(rows) * 8),
+                FX_ERR_ALIAS, "CSR matvec does not support in-place views");
+    fx::require(vals != y, FX_ERR_ALIAS, "CSR matvec output must not alias matrix values");
+    hipLaunchKernelGGL(k_spmv, dim3(fx::blocks_for(rows, kRowBlock)), dim3(kRowBlock), 0,
+                       fx::as_stream(stream), rows, row_ptr, col_ind, vals, x, y, alpha, beta);
+    fx::check_launch("fx_csr_matvec");
+  });
+}
+
+int fx_csr_matvec_t(fx_index rows, fx_index cols, const fx_index* t_ptr, const fx_index* t_row,
+                    const fx_index* t_perm, const double* vals, const double* x, double* y,
+                    double alpha, double beta, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(rows >= 0 && cols >= 0, FX_ERR_ARG, "fx_csr_matvec_t: negative size");
+    if (cols == 0) return;
+    fx::require(t_ptr && y && (rows == 0 || x), FX_ERR_ARG, "fx_csr_matvec_t: null array");
+    fx::require(!fx::overlaps(x, std::size_t(rows) * 8, y, std::size_t(cols) * 8),
+                FX_ERR_ALIAS, "CSR matvecT does not support in-place views");
+    fx::require(vals != y, FX_ERR_ALIAS, "CSR matvecT output must not alias matrix values");
+    hipLaunchKernelGGL(k_spmv_t, dim3(fx::blocks_for(cols, kRowBlock)), dim3(kRowBlock), 0,
+                       fx::as_stream(stream), cols, t_ptr, t_row, t_perm, vals, x, y, alpha, beta);
+    fx::check_launch("fx_csr_matvec_t");
+  });
+}
+
+int fx_vec_axpby(fx_index n, double a, const double* x, double b, double* y, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(n >= 0, FX_ERR_ARG, "fx_vec_axpby: negative size");
+    if (n == 0) return;
+    fx::require(x && y, FX_ERR_ARG, "fx_vec_axpby: null array");
+    hipLaunchKernelGGL(k_axpby, dim3(fx::blocks_for(n, kRowBlock)), dim3(kRowBlock), 0,
+                       fx::as_stream(stream), n, a, x, b, y);
+    fx::check_launch("fx_vec_axpby");
+  });
+}
+
+fx_index fx_dot_work_len(void) { return kDotGrid; }
+
+int fx_vec_dot(fx_index n, const double* x, const double* y, double* work, double* out,
+               void* stream)
+{
+  return fx::guard([&] {
+    fx::require(n >= 0 && work && out && (n == 0 || (x && y)), FX_ERR_ARG,
+                "fx_vec_dot: invalid arguments");
+    hipLaunchKernelGGL(k_dot_partial, dim3(kDotGrid), dim3(kDotBlock), 0, fx::as_stream(stream), n,
+                       x, y, work);
+    fx::check_launch("fx_vec_dot");
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(kDotBlock), 0, fx::as_stream(stream), work, out);
+    fx::check_launch("fx_vec_dot");
+  });
+}
+
+} // extern "C"

@@ -1,0 +1,189 @@
+"""Device CSR storage and handlers, mirroring femx::linalg.
+
+`CsrPattern` follows femx's immutable, shared CSR pattern with a layout
+identity (femx/linalg/CsrPattern.hpp:28-32,43-217); `CsrMatrix` owns device
+values over a shared pattern (femx/linalg/CsrMatrix.hpp:17-105);
+`MatrixHandler` and `VectorHandler` mirror the CUDA handlers
+(femx/linalg/handler/MatrixHandler.hpp:193-378, VectorHandler.hpp:190-477)
+and run on the HIP kernels of libfemx_amd.so.  Device memory is torch's; the
+arithmetic is the library's.
+"""
+from __future__ import annotations
+
+import itertools
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_layout_ids = itertools.count(1)
+
+
+def ptr(a):
+    """Raw address of a torch tensor or numpy array (None stays None)."""
+    if a is None:
+        return None
+    if isinstance(a, torch.Tensor):
+        return a.data_ptr() if a.numel() > 0 else None
+    return a.ctypes.data if a.size > 0 else None
+
+
+def stream_handle(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return s.cuda_stream
+
+
+def _require(cond, msg):
+    if not cond:
+        raise RuntimeError(msg)
+
+
+def _check_dev_vec(t, n, dtype, what):
+    _require(isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous(),
+             f"{what} must be a contiguous CUDA tensor of {dtype}")
+    _require(t.numel() == n, f"{what} has size {t.numel()}, expected {n}")
+
+
+class CsrPattern:
+    """Immutable CSR structure with a layout identity shared by matrices."""
+
+    def __init__(self, rows, cols, row_ptr, col_ind, layout_id=None):
+        row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+        col_ind = np.ascontiguousarray(col_ind, np.int32)
+        _require(rows >= 0 and cols >= 0, "CsrPattern dimensions must be non-negative")
+        _require(row_ptr.size == rows + 1, "CsrPattern row_ptr size must be rows + 1")
+        _require(row_ptr[0] == 0 and np.all(np.diff(row_ptr) >= 0),
+                 "CsrPattern row_ptr must start at zero and be non-decreasing")
+        _require(row_ptr[-1] == col_ind.size, "CsrPattern nnz does not match col_ind")
+        _require(col_ind.size == 0 or (col_ind.min() >= 0 and col_ind.max() < cols),
+                 "CsrPattern column index is out of range")
+        self.rows, self.cols = int(rows), int(cols)
+        self.row_ptr, self.col_ind = row_ptr, col_ind
+        self.layout_id = next(_layout_ids) if layout_id is None else layout_id
+        self._dev = {}
+        self._tmap = {}
+
+    @property
+    def nnz(self):
+        return int(self.col_ind.size)
+
+    def device_arrays(self, device="cuda"):
+        dev = torch.device(device)
+        key = str(dev)
+        if key not in self._dev:
+            self._dev[key] = (torch.from_numpy(self.row_ptr).to(dev),
+                              torch.from_numpy(self.col_ind).to(dev))
+        return self._dev[key]
+
+    def transpose_map(self, device="cuda"):
+        """(t_ptr, t_row, t_perm) device tensors, built once per device."""
+        dev = torch.device(device)
+        key = str(dev)
+        if key not in self._tmap:
+            t_ptr = np.empty(self.cols + 1, np.int32)
+            t_row = np.empty(max(self.nnz, 1), np.int32)
+            t_perm = np.empty(max(self.nnz, 1), np.int32)
+            _lib.call("fx_csr_transpose_map", self.rows, self.cols, ptr(self.row_ptr),
+                      ptr(self.col_ind), ptr(t_ptr), ptr(t_row), ptr(t_perm))
+            self._tmap[key] = tuple(torch.from_numpy(a).to(dev)
+                                    for a in (t_ptr, t_row[: self.nnz], t_perm[: self.nnz]))
+        return self._tmap[key]
+
+
+class CsrMatrix:
+    """Device CSR matrix: shared pattern, owned FP64 values."""
+
+    def __init__(self, pattern: CsrPattern, device="cuda", vals=None):
+        self.pattern = pattern
+        self.device = torch.device(device)
+        if vals is None:
+            vals = torch.zeros(pattern.nnz, dtype=torch.float64, device=self.device)
+        _check_dev_vec(vals, pattern.nnz, torch.float64, "CsrMatrix values")
+        self.vals = vals
+
+    @property
+    def rows(self):
+        return self.pattern.rows
+
+    @property
+    def cols(self):
+        return self.pattern.cols
+
+    @property
+    def nnz(self):
+        return self.pattern.nnz
+
+    def clone(self):
+        return CsrMatrix(self.pattern, self.device, self.vals.clone())
+
+    def to_dense(self):
+        """Dense host copy (tests and small problems only)."""
+        v = self.vals.detach().cpu().numpy()
+        out = np.zeros((self.rows, self.cols))
+        rp, ci = self.pattern.row_ptr, self.pattern.col_ind
+        for r in range(self.rows):
+            out[r, ci[rp[r]:rp[r + 1]]] += v[rp[r]:rp[r + 1]]
+        return out
+
+
+class MatrixHandler:
+    """Stream-ordered CSR operations (MatrixHandler<CudaCsrBackend>)."""
+
+    def __init__(self, stream=None):
+        self.stream = stream
+
+    def zero(self, mat: CsrMatrix):
+        mat.vals.zero_()
+
+    def copy(self, src: CsrMatrix, dst: CsrMatrix):
+        _require(src.pattern.layout_id == dst.pattern.layout_id,
+                 "CSR copy requires matrices with the same layout")
+        dst.vals.copy_(src.vals)
+
+    def matvec(self, mat: CsrMatrix, x, y, alpha=1.0, beta=0.0):
+        """y = alpha*A*x + beta*y (femx MatrixHandler::matvec)."""
+        _check_dev_vec(x, mat.cols, torch.float64, "CSR matvec input")
+        _check_dev_vec(y, mat.rows, torch.float64, "CSR matvec output")
+        rp, ci = mat.pattern.device_arrays(mat.device)
+        _lib.call("fx_csr_matvec", mat.rows, mat.cols, ptr(rp), ptr(ci), ptr(mat.vals), ptr(x),
+                  ptr(y), float(alpha), float(beta), stream_handle(self.stream))
+        return y
+
+    def matvecT(self, mat: CsrMatrix, x, y, alpha=1.0, beta=0.0):
+        """y = alpha*A^T*x + beta*y (femx MatrixHandler::matvecT)."""
+        _check_dev_vec(x, mat.rows, torch.float64, "CSR matvecT input")
+        _check_dev_vec(y, mat.cols, torch.float64, "CSR matvecT output")
+        t_ptr, t_row, t_perm = mat.pattern.transpose_map(mat.device)
+        _lib.call("fx_csr_matvec_t", mat.rows, mat.cols, ptr(t_ptr), ptr(t_row), ptr(t_perm),
+                  ptr(mat.vals), ptr(x), ptr(y), float(alpha), float(beta),
+                  stream_handle(self.stream))
+        return y
+
+
+class VectorHandler:
+    """Stream-ordered vector operations (VectorHandler<CudaCsrBackend>)."""
+
+    def __init__(self, stream=None):
+        self.stream = stream
+        self._work = {}
+
+    def axpby(self, a, x, b, y):
+        """y = a*x + b*y."""
+        _check_dev_vec(x, y.numel(), torch.float64, "axpby input")
+        _lib.call("fx_vec_axpby", y.numel(), float(a), ptr(x), float(b), ptr(y),
+                  stream_handle(self.stream))
+        return y
+
+    def dot(self, x, y, out=None):
+        """Deterministic dot product into a one-element device tensor."""
+        _check_dev_vec(y, x.numel(), torch.float64, "dot input")
+        key = str(x.device)
+        if key not in self._work:
+            self._work[key] = torch.empty(_lib.lib().fx_dot_work_len(), dtype=torch.float64,
+                                          device=x.device)
+        if out is None:
+            out = torch.empty(1, dtype=torch.float64, device=x.device)
+        _lib.call("fx_vec_dot", x.numel(), ptr(x), ptr(y), ptr(self._work[key]), ptr(out),
+                  stream_handle(self.stream))
+        return out

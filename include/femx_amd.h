@@ -1,0 +1,217 @@
+/*
+ * femx_amd.h -- C-ABI of the MI355X-native femx FE core.
+ *
+ * All sizes and indices are int32 (femx `Index`, femx/common/Types.hpp:19);
+ * all values are FP64 (femx `Real`, :17).  Pointers named d_* are device
+ * pointers; h_* are host pointers.  `stream` is a hipStream_t passed as
+ * void*, as femx's CudaContext hands its stream to kernels
+ * (femx/common/Context.hpp:68-71).  Device entry points are asynchronous on
+ * `stream`; callers synchronise explicitly, as with femx's CUDA path
+ * (femx/assembly/CudaAssembly.hpp:302-318).
+ *
+ * Every function returns FX_OK (0) or an error code; fx_last_error() gives
+ * the message of the last failure on the calling thread.  No C++ exception
+ * crosses this boundary.  Outputs are replaced, never accumulated into,
+ * unless a function says otherwise (femx/assembly/Assembly.hpp:168-169).
+ *
+ * Results are deterministic: no entry point uses floating-point atomics, so
+ * repeated calls on the same inputs give bitwise equal outputs.
+ */
+#ifndef FEMX_AMD_H
+#define FEMX_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t fx_index;
+
+enum fx_status
+{
+  FX_OK              = 0,
+  FX_ERR_ARG         = 1, /* sizes, ranges or null pointers */
+  FX_ERR_ALIAS       = 2, /* outputs overlap inputs or each other */
+  FX_ERR_LAYOUT      = 3, /* map/matrix layout mismatch */
+  FX_ERR_OVERFLOW    = 4, /* int32 index range exceeded */
+  FX_ERR_HIP         = 5, /* HIP runtime error */
+  FX_ERR_PATTERN     = 6, /* CSR pattern lacks a required entry */
+  FX_ERR_UNSUPPORTED = 7, /* element/map shape not handled by a kernel */
+  FX_ERR_ALLOC       = 8, /* host allocation failed */
+  FX_ERR_INTERNAL    = 9
+};
+
+const char* fx_last_error(void);
+int fx_version(void);
+
+/* ======================================================================== */
+/* Structured triangle meshes on [0,1]^2 (host).  femx has only
+ * Mesh::makeStructuredQuad (femx/fem/Mesh.cpp:10-51); cell (i,j) is split
+ * along its (i,j)-(i+1,j+1) diagonal into two counter-clockwise triangles.
+ * P1: N=(n+1)^2 nodes, E=2n^2, h_conn[3E].  P2: N_v=(2n+1)^2 lattice nodes,
+ * h_conn6[6E] in Gmsh order (vertices, edges (0,1),(1,2),(2,0)) and
+ * h_vconn3[3E] vertex ids on the (n+1)^2 vertex lattice. */
+int fx_mesh_p1(fx_index n, double* h_x, double* h_y, fx_index* h_conn);
+int fx_mesh_p2(fx_index n, double* h_x, double* h_y, fx_index* h_conn6, fx_index* h_vconn3);
+/* Taylor-Hood element DOF table h_dofs15[15E]: interleaved velocity DOFs
+ * 2*node+c (FESpace::globalDof, femx/fem/FESpace.cpp:103-106) then pressure
+ * DOFs offset by 2*N_v (MixedFESpace::elemDofs, femx/fem/MixedFESpace.cpp:
+ * 121-137). */
+int fx_th_dofs(fx_index n, const fx_index* h_conn6, const fx_index* h_vconn3,
+               fx_index* h_dofs15);
+
+/* ======================================================================== */
+/* Assembly map (host).  Replaces femx::assembly::makeAssemblyMap
+ * (femx/assembly/AssemblyMap.hpp:220-236, AssemblyMap.cpp:54-176): the CSR
+ * pattern and jac_map are bit-identical to femx's.  It also derives the
+ * row-incidence tables that the deterministic row-gather kernels read. */
+typedef struct fx_map fx_map;
+
+typedef struct
+{
+  fx_index num_elems, num_res, num_states;
+  fx_index nnz;         /* CSR entries */
+  fx_index jac_len;     /* jac_offsets[num_elems] (sum of local matrix sizes) */
+  fx_index max_res, max_state, max_jac;
+  fx_index max_row_len; /* longest CSR row */
+  fx_index num_inc;     /* row incidences: sum over elements of local rows */
+} fx_map_info;
+
+int fx_map_create(fx_index num_res, fx_index num_states, fx_index num_elems,
+                  const fx_index* h_res_offsets, const fx_index* h_res_dofs,
+                  const fx_index* h_state_offsets, const fx_index* h_state_dofs,
+                  fx_map** out);
+int fx_map_info_get(const fx_map* map, fx_map_info* info);
+/* row_ptr[num_res+1], col_ind[nnz] (femx CsrPattern, femx/linalg/CsrPattern.hpp). */
+int fx_map_export_pattern(const fx_map* map, fx_index* h_row_ptr, fx_index* h_col_ind);
+/* femx's jac_offsets[num_elems+1] and jac_map[jac_len] (AssemblyMap.hpp:22-68). */
+int fx_map_export_jac(const fx_map* map, fx_index* h_jac_offsets, fx_index* h_jac_map);
+/* Row incidence for elements with 3 local rows and 3 local columns (P1):
+ * inc_off[num_res+1]; inc[2*num_inc] pairs (e, li | p0<<8 | p1<<16 | p2<<24)
+ * listing, for each row in ascending element order, the element e, the local
+ * row li and the positions p_j (within the row, from row_ptr[row]) of the
+ * element's three columns.  FX_ERR_UNSUPPORTED for other shapes or rows
+ * longer than 255. */
+int fx_map_export_rowinc3(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc);
+void fx_map_destroy(fx_map* map);
+
+/* CSR transpose permutation (host), for the deterministic transpose apply:
+ * t_ptr[cols+1], t_row[nnz], t_perm[nnz] list column c's entries in
+ * ascending row order with their source CSR index.  Replaces the cached
+ * transpose map of MatrixHandler<CudaCsrBackend>::transpose
+ * (femx/linalg/handler/MatrixHandler.cu:401-512). */
+int fx_csr_transpose_map(fx_index rows, fx_index cols, const fx_index* h_row_ptr,
+                         const fx_index* h_col_ind, fx_index* h_t_ptr, fx_index* h_t_row,
+                         fx_index* h_t_perm);
+
+/* ======================================================================== */
+/* Boundary map (host).  Replaces femx::assembly::makeBoundaryMap
+ * (femx/assembly/BoundaryMap.hpp:185-186, BoundaryMap.cpp:59-139), with the
+ * same validation and the same arrays, plus the row-grouped elimination
+ * lists used by fx_bc_prepare_forward in place of atomics. */
+typedef struct fx_bcmap fx_bcmap;
+
+typedef struct
+{
+  fx_index rows, num_bcs;
+  fx_index num_col_entries; /* col_offsets[num_bcs] */
+  fx_index num_elim_rows;   /* unconstrained rows touching constrained columns */
+  fx_index num_elim_entries;
+} fx_bcmap_info;
+
+int fx_bcmap_create(fx_index rows, const fx_index* h_row_ptr, const fx_index* h_col_ind,
+                    fx_index num_bcs, const fx_index* h_dofs, fx_bcmap** out);
+int fx_bcmap_info_get(const fx_bcmap* map, fx_bcmap_info* info);
+/* femx's BoundaryMapView arrays (BoundaryMap.hpp:17-62). */
+int fx_bcmap_export(const fx_bcmap* map, fx_index* h_bc_rows, fx_index* h_diag,
+                    fx_index* h_col_offsets, fx_index* h_col_entries, fx_index* h_col_rows,
+                    fx_index* h_bc_mask);
+/* elim_rows[num_elim_rows], elim_off[num_elim_rows+1], elim_k[num_elim_entries]
+ * (CSR index) and elim_ib[num_elim_entries] (boundary index), each row's
+ * entries in ascending boundary index -- the order femx's host loop applies
+ * them (BoundaryMap.cpp:227-240). */
+int fx_bcmap_export_elim(const fx_bcmap* map, fx_index* h_elim_rows, fx_index* h_elim_off,
+                         fx_index* h_elim_k, fx_index* h_elim_ib);
+void fx_bcmap_destroy(fx_bcmap* map);
+
+/* ======================================================================== */
+/* Boundary operations (device).  Replace the CUDA overloads of replaceRows,
+ * replaceRes, zeroBoundary and prepareForwardSolve (femx/assembly/
+ * BoundaryMap.hpp:220-294, BoundaryMap.cu:16-211). */
+int fx_bc_replace_rows(fx_index num_bcs, const fx_index* d_bc_rows, const fx_index* d_diag,
+                       const fx_index* d_row_ptr, double* d_vals, double diag, void* stream);
+int fx_bc_replace_res(fx_index num_bcs, const fx_index* d_bc_rows, const double* d_state,
+                      const double* d_bc_vals, double* d_res, void* stream);
+int fx_bc_zero(fx_index num_bcs, const fx_index* d_bc_rows, double* d_v, void* stream);
+int fx_bc_prepare_forward(fx_index num_bcs, const fx_index* d_bc_rows, const fx_index* d_diag,
+                          const fx_index* d_row_ptr, fx_index num_elim_rows,
+                          const fx_index* d_elim_rows, const fx_index* d_elim_off,
+                          const fx_index* d_elim_k, const fx_index* d_elim_ib, double* d_vals,
+                          double* d_rhs, const double* d_bc_vals, void* stream);
+
+/* ======================================================================== */
+/* CSR SpMV (device).  y = alpha*A*x + beta*y, sums in CSR order per row
+ * (MatrixHandler<HostCsrBackend>::matvec, femx/linalg/handler/
+ * MatrixHandler.cpp:157-175; replaces cusparseSpMV, MatrixHandler.cu:
+ * 335-398,514-527).  With beta == 0, y is not read.  x and y must not
+ * overlap each other or vals (MatrixHandler.cu:182-202). */
+int fx_csr_matvec(fx_index rows, fx_index cols, const fx_index* d_row_ptr,
+                  const fx_index* d_col_ind, const double* d_vals, const double* d_x,
+                  double* d_y, double alpha, double beta, void* stream);
+/* y = alpha*A^T*x + beta*y through the transpose map (matvecT,
+ * MatrixHandler.cpp:177-198 and MatrixHandler.cu:529-542), each column summed
+ * in ascending row order. */
+int fx_csr_matvec_t(fx_index rows, fx_index cols, const fx_index* d_t_ptr,
+                    const fx_index* d_t_row, const fx_index* d_t_perm, const double* d_vals,
+                    const double* d_x, double* d_y, double alpha, double beta, void* stream);
+
+/* Vector ops (VectorHandler<CudaCsrBackend>, femx/linalg/handler/
+ * VectorHandler.cu:124-317).  axpby: y = a*x + b*y; with b == 0, y is not
+ * read. */
+int fx_vec_axpby(fx_index n, double a, const double* d_x, double b, double* d_y, void* stream);
+/* Deterministic dot product into d_out[0]; d_work holds fx_dot_work_len()
+ * doubles. */
+fx_index fx_dot_work_len(void);
+int fx_vec_dot(fx_index n, const double* d_x, const double* d_y, double* d_work, double* d_out,
+               void* stream);
+
+/* ======================================================================== */
+/* P1 Poisson assembly (device).  Drop-in for assembly::assemble with the
+ * Poisson operator (femx/assembly/CudaAssembly.hpp:319-364 driving
+ * examples/poisson/PoissonComponents.hpp:35-64), extended with a per-element
+ * diffusion kappa[E] (or its logarithm m[E], kappa = exp(m), when
+ * kappa_is_log is nonzero) and a nodal load f[N]:
+ *   vals = CSR values of K (K^e = kappa_e * femx's Laplace element matrix),
+ *   res  = K u - F,  F_i = int f_h phi_i  (f_h the P1 interpolant of f).
+ * f or u may be NULL (zero); vals may be NULL for a residual-only pass
+ * (assembleResidual, CudaAssembly.hpp:420-472).  Geometry is SoA x[N], y[N]; conn[3E] is the
+ * element DOF table (res_dofs = state_dofs); inc_off/inc come from
+ * fx_map_export_rowinc3, row_ptr from fx_map_export_pattern, max_row_len from
+ * fx_map_info_get.  Each CSR value and residual entry is summed over its
+ * elements in ascending element order, as femx's host assemble does
+ * (Assembly.hpp:184-225). */
+int fx_p1_assemble(fx_index num_nodes, fx_index num_elems, fx_index nnz,
+                   fx_index max_row_len, const double* d_x, const double* d_y,
+                   const fx_index* d_conn, const double* d_kappa, int kappa_is_log,
+                   const double* d_f, const double* d_u, const fx_index* d_inc_off, const fx_index* d_inc,
+                   const fx_index* d_row_ptr, double* d_vals, double* d_res, void* stream);
+
+/* P1 log-kappa adjoint VJP (device), one pass over the row incidence:
+ *   jt_lam = J^T lam   (J the row-replaced Jacobian: lam is zeroed on
+ *                       constrained rows first and passes through there),
+ *   g[e]   = (dR/dm_e)^T lam = lam~_e^T K^e u_e,  kappa_e = exp(m_e).
+ * bc_mask[N] may be NULL (unconstrained).  Fills the Residual::applyParamJacT
+ * contract (femx/state/Residual.hpp:55-59) used by ReducedFunctional::gradAt
+ * (femx/inverse/ReducedFunctional.hpp:94-117), with the boundary convention
+ * of ConstrainedTimeResidual::applyJacT (femx/assembly/
+ * ConstrainedTimeResidual.cpp:379-405). */
+int fx_p1_vjp(fx_index num_nodes, fx_index num_elems, const double* d_x, const double* d_y,
+              const fx_index* d_conn, const double* d_m, const double* d_u,
+              const double* d_lam, const fx_index* d_bc_mask, const fx_index* d_inc_off,
+              const fx_index* d_inc, double* d_jt_lam, double* d_g, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,266 @@
+"""ctypes front-end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by __graft_entry__.smoke() as
+the checker and by bench.py for its cpu_baseline leg; never by the product.
+
+Also exposes the compiled femx reference (oracle/_ref/libfemx_ref.so) when it
+has been built, as `ref()`; callers must treat its absence as "not
+available" and skip.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+I32 = np.int32
+F64 = np.float64
+_pi = C.POINTER(C.c_int32)
+_pd = C.POINTER(C.c_double)
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype == np.int32:
+        return a.ctypes.data_as(_pi)
+    if a.dtype == np.float64:
+        return a.ctypes.data_as(_pd)
+    raise TypeError(a.dtype)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle oracle)")
+        _LIB = C.CDLL(path)
+        _LIB.or_quad_tri.restype = C.c_int
+        _LIB.or_map_build.restype = C.c_int
+        _LIB.or_bc_build.restype = C.c_int
+        _LIB.or_omp_threads.restype = C.c_int
+        for name in ("or_p1_assemble", "or_p1_assemble_omp", "or_p1_vjp", "or_p1_vjp_omp",
+                     "or_matvec", "or_matvecT"):
+            getattr(_LIB, name).restype = None
+        _LIB.or_th_assemble.argtypes = None
+    return _LIB
+
+
+def ref():
+    """The compiled femx reference, or None when it was not built here."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "libfemx_ref.so")
+        if not os.path.exists(path):
+            return None
+        _REF = C.CDLL(path)
+        _REF.ref_last_error.restype = C.c_char_p
+        _REF.ref_bc_num_col_entries.restype = C.c_int32
+        _REF.ref_map_build.argtypes = [C.c_int32] * 3 + [_pi] * 4 + [C.POINTER(C.c_void_p)]
+        _REF.ref_map_sizes.argtypes = [C.c_void_p, _pi, _pi, _pi]
+        _REF.ref_map_export.argtypes = [C.c_void_p, _pi, _pi, _pi, _pi]
+        _REF.ref_map_free.argtypes = [C.c_void_p]
+        _REF.ref_bc_build.argtypes = [C.c_int32, _pi, _pi, C.c_int32, _pi, C.POINTER(C.c_void_p)]
+        _REF.ref_bc_num_col_entries.argtypes = [C.c_void_p]
+        _REF.ref_bc_export.argtypes = [C.c_void_p] + [_pi] * 6
+        _REF.ref_bc_free.argtypes = [C.c_void_p]
+        _REF.ref_bc_apply.argtypes = ([C.c_int32, C.c_int32, _pi, _pi, C.c_int32, _pi]
+                                      + [_pd] * 4 + [C.c_double])
+        _REF.ref_spmv.argtypes = ([C.c_int32] * 3 + [_pi, _pi] + [_pd] * 3
+                                  + [C.c_double, C.c_double])
+        _REF.ref_poisson_p1.argtypes = ([C.c_int32, _pd, _pd, C.c_int32, _pi, _pd, C.c_int32,
+                                         _pi, _pi, _pd, _pd])
+    return _REF
+
+
+# --------------------------------------------------------------------------
+# meshes and DOF tables
+
+def mesh_p1(n):
+    N, E = (n + 1) ** 2, 2 * n * n
+    x = np.empty(N, F64)
+    y = np.empty(N, F64)
+    conn = np.empty(3 * E, I32)
+    lib().or_mesh_p1(C.c_int(n), _p(x), _p(y), _p(conn))
+    return x, y, conn
+
+
+def mesh_p2(n):
+    Nv, E = (2 * n + 1) ** 2, 2 * n * n
+    x = np.empty(Nv, F64)
+    y = np.empty(Nv, F64)
+    conn6 = np.empty(6 * E, I32)
+    vconn3 = np.empty(3 * E, I32)
+    lib().or_mesh_p2(C.c_int(n), _p(x), _p(y), _p(conn6), _p(vconn3))
+    return x, y, conn6, vconn3
+
+
+def th_dofs(n, conn6, vconn3):
+    E = conn6.size // 6
+    d = np.empty(15 * E, I32)
+    lib().or_th_dofs(C.c_int(n), C.c_int32(E), _p(conn6), _p(vconn3), _p(d))
+    return d
+
+
+def _take(ptr, n, dtype):
+    if n == 0:
+        out = np.empty(0, dtype)
+    else:
+        out = np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+    lib().or_free(ptr)
+    return out
+
+
+def map_build(num_res, num_states, res_offsets, res_dofs, state_offsets, state_dofs):
+    """makeAssemblyMap restatement -> (row_ptr, col_ind, jac_offsets, jac_map)."""
+    res_offsets = np.ascontiguousarray(res_offsets, I32)
+    res_dofs = np.ascontiguousarray(res_dofs, I32)
+    state_offsets = np.ascontiguousarray(state_offsets, I32)
+    state_dofs = np.ascontiguousarray(state_dofs, I32)
+    E = res_offsets.size - 1
+    rp, ci, jo, jm = (_pi(), _pi(), _pi(), _pi())
+    nnz = C.c_int32()
+    rc = lib().or_map_build(C.c_int32(num_res), C.c_int32(num_states), C.c_int32(E),
+                            _p(res_offsets), _p(res_dofs), _p(state_offsets), _p(state_dofs),
+                            C.byref(rp), C.byref(ci), C.byref(jo), C.byref(jm), C.byref(nnz))
+    if rc != 0:
+        raise RuntimeError(f"or_map_build failed ({rc})")
+    total = int(np.sum(np.diff(res_offsets).astype(np.int64) * np.diff(state_offsets)))
+    return (_take(rp, num_res + 1, I32), _take(ci, nnz.value, I32), _take(jo, E + 1, I32),
+            _take(jm, total, I32))
+
+
+def uniform_offsets(E, k):
+    return (np.arange(E + 1, dtype=np.int64) * k).astype(I32)
+
+
+def bc_build(row_ptr, col_ind, dofs):
+    rows = row_ptr.size - 1
+    dofs = np.ascontiguousarray(dofs, I32)
+    nb = dofs.size
+    diag = np.empty(nb, I32)
+    col_off = np.empty(nb + 1, I32)
+    mask = np.empty(rows, I32)
+    ce, cr = _pi(), _pi()
+    rc = lib().or_bc_build(C.c_int32(rows), _p(row_ptr), _p(col_ind), C.c_int32(nb), _p(dofs),
+                           _p(diag), _p(col_off), C.byref(ce), C.byref(cr), _p(mask))
+    if rc != 0:
+        raise RuntimeError(f"or_bc_build failed ({rc})")
+    nce = int(col_off[-1])
+    return dict(bc_rows=dofs.copy(), diag=diag, col_offsets=col_off,
+                col_entries=_take(ce, nce, I32), col_rows=_take(cr, nce, I32), bc_mask=mask)
+
+
+def replace_rows(row_ptr, bc, vals, diag_val):
+    lib().or_replace_rows(_p(row_ptr), C.c_int32(bc["bc_rows"].size), _p(bc["bc_rows"]),
+                          _p(bc["diag"]), _p(vals), C.c_double(diag_val))
+
+
+def replace_res(bc, state, bc_vals, res):
+    lib().or_replace_res(C.c_int32(bc["bc_rows"].size), _p(bc["bc_rows"]), _p(state),
+                         _p(bc_vals), _p(res))
+
+
+def zero_boundary(bc, v):
+    lib().or_zero_boundary(C.c_int32(bc["bc_rows"].size), _p(bc["bc_rows"]), _p(v))
+
+
+def prepare_forward(row_ptr, bc, vals, rhs, bc_vals):
+    lib().or_prepare_forward(_p(row_ptr), C.c_int32(bc["bc_rows"].size), _p(bc["bc_rows"]),
+                             _p(bc["diag"]), _p(bc["col_offsets"]), _p(bc["col_entries"]),
+                             _p(bc["col_rows"]), _p(bc["bc_mask"]), _p(vals), _p(rhs),
+                             _p(np.ascontiguousarray(bc_vals, F64)))
+
+
+def matvec(row_ptr, col_ind, vals, x, y, alpha=1.0, beta=0.0):
+    lib().or_matvec(C.c_int32(row_ptr.size - 1), _p(row_ptr), _p(col_ind), _p(vals), _p(x),
+                    _p(y), C.c_double(alpha), C.c_double(beta))
+
+
+def matvecT(row_ptr, col_ind, vals, ncols, x, y, alpha=1.0, beta=0.0):
+    lib().or_matvecT(C.c_int32(row_ptr.size - 1), C.c_int32(ncols), _p(row_ptr), _p(col_ind),
+                     _p(vals), _p(x), _p(y), C.c_double(alpha), C.c_double(beta))
+
+
+def quad_tri(order):
+    rs = np.zeros(14, F64)
+    w = np.zeros(7, F64)
+    nq = lib().or_quad_tri(C.c_int(order), _p(rs), _p(w))
+    if nq < 0:
+        raise RuntimeError("unsupported order")
+    return rs[: 2 * nq].reshape(nq, 2), w[:nq]
+
+
+def p2_shape(r, s):
+    N = np.zeros(6, F64)
+    d = np.zeros(12, F64)
+    lib().or_p2_shape(C.c_double(r), C.c_double(s), _p(N), _p(d))
+    return N, d.reshape(6, 2)
+
+
+def p1_element(xe, ye, kappa, fe, ue):
+    Ke = np.zeros(9, F64)
+    Re = np.zeros(3, F64)
+    lib().or_p1_element(_p(np.ascontiguousarray(xe, F64)), _p(np.ascontiguousarray(ye, F64)),
+                        C.c_double(kappa), _p(np.ascontiguousarray(fe, F64)),
+                        _p(np.ascontiguousarray(ue, F64)), _p(Ke), _p(Re))
+    return Ke.reshape(3, 3), Re
+
+
+def p1_assemble(x, y, conn, kappa, f, u, jac_map, nnz, omp=False):
+    N = x.size
+    E = conn.size // 3
+    vals = np.empty(nnz, F64)
+    res = np.empty(N, F64)
+    fn = lib().or_p1_assemble_omp if omp else lib().or_p1_assemble
+    fn(C.c_int32(N), _p(x), _p(y), C.c_int32(E), _p(conn), _p(kappa), _p(f), _p(u),
+       _p(jac_map), C.c_int32(nnz), _p(vals), _p(res))
+    return vals, res
+
+
+def p1_vjp(x, y, conn, m, u, lam, bc_mask=None, omp=False):
+    N = x.size
+    E = conn.size // 3
+    jt = np.empty(N, F64)
+    g = np.empty(E, F64)
+    fn = lib().or_p1_vjp_omp if omp else lib().or_p1_vjp
+    fn(C.c_int32(N), _p(x), _p(y), C.c_int32(E), _p(conn), _p(m), _p(u), _p(lam),
+       _p(bc_mask), _p(jt), _p(g))
+    return jt, g
+
+
+def th_element(xe, ye, nu, se):
+    Ke = np.zeros(225, F64)
+    Re = np.zeros(15, F64)
+    lib().or_th_element(_p(np.ascontiguousarray(xe, F64)), _p(np.ascontiguousarray(ye, F64)),
+                        C.c_double(nu), _p(np.ascontiguousarray(se, F64)), _p(Ke), _p(Re))
+    return Ke.reshape(15, 15), Re
+
+
+def th_assemble(xv, yv, vconn3, dofs15, ndof, nu, state, jac_map, nnz, omp=False):
+    E = vconn3.size // 3
+    vals = np.empty(nnz, F64)
+    res = np.empty(ndof, F64)
+    fn = lib().or_th_assemble_omp if omp else lib().or_th_assemble
+    fn(_p(xv), _p(yv), C.c_int32(E), _p(vconn3), _p(dofs15), C.c_int32(ndof), C.c_double(nu),
+       _p(state), _p(jac_map), C.c_int32(nnz), _p(vals), _p(res))
+    return vals, res
+
+
+def th_vjp(xv, yv, vconn3, dofs15, ndof, nu, state, lam, bc_mask=None):
+    E = vconn3.size // 3
+    jt = np.empty(ndof, F64)
+    dnu = C.c_double()
+    lib().or_th_vjp(_p(xv), _p(yv), C.c_int32(E), _p(vconn3), _p(dofs15), C.c_int32(ndof),
+                    C.c_double(nu), _p(state), _p(lam), _p(bc_mask), _p(jt), C.byref(dnu))
+    return jt, dnu.value
+
+
+def omp_threads():
+    return lib().or_omp_threads()

@@ -1,0 +1,249 @@
+// Checker-only driver over the compiled femx reference sources.
+//
+// This file is test infrastructure.  It is compiled together with the
+// reference's own hot-path sources (listed in oracle/Makefile) into
+// oracle/_ref/libfemx_ref.so and is used only to write golden fixtures and to
+// cross-check the C restatement in oracle/femx_oracle.c.  Nothing in the
+// product links or loads it.
+//
+// Every entry point takes plain host arrays, builds the reference's own
+// objects from them and calls the reference routine named in its comment.
+
+#include <cstdint>
+#include <cstring>
+#include <exception>
+#include <string>
+
+#include <femx/assembly/Assembly.hpp>
+#include <femx/assembly/AssemblyMap.hpp>
+#include <femx/assembly/BoundaryMap.hpp>
+#include <femx/fem/DofLayout.hpp>
+#include <femx/fem/ElementQuadratureData.hpp>
+#include <femx/fem/FESpace.hpp>
+#include <femx/fem/GaussQuadrature.hpp>
+#include <femx/fem/Geometry.hpp>
+#include <femx/fem/Mesh.hpp>
+#include <femx/fem/elements/LagrangeTriangleP1.hpp>
+#include <femx/linalg/CsrMatrix.hpp>
+#include <femx/linalg/CsrPattern.hpp>
+#include <femx/linalg/handler/MatrixHandler.hpp>
+
+#include "PoissonComponents.hpp"
+
+using femx::Array;
+using femx::Index;
+using femx::Real;
+
+namespace
+{
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& fn)
+{
+  try
+  {
+    fn();
+    return 0;
+  }
+  catch (const std::exception& e)
+  {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+struct MapHolder
+{
+  femx::assembly::HostAssemblyMap map;
+};
+
+struct BcHolder
+{
+  femx::assembly::HostBoundaryMap map;
+};
+
+femx::HostCsrPattern makePattern(Index rows, Index cols, const Index* row_ptr,
+                                 const Index* col_ind)
+{
+  femx::HostIndexVector rp(rows + 1);
+  for (Index i = 0; i <= rows; ++i) rp[i] = row_ptr[i];
+  const Index           nnz = row_ptr[rows];
+  femx::HostIndexVector ci(nnz);
+  for (Index k = 0; k < nnz; ++k) ci[k] = col_ind[k];
+  return femx::HostCsrPattern(rows, cols, std::move(rp), std::move(ci));
+}
+
+} // namespace
+
+extern "C" {
+
+const char* ref_last_error() { return g_err.c_str(); }
+
+// femx::assembly::makeAssemblyMap (femx/assembly/AssemblyMap.cpp:54-176).
+int ref_map_build(Index num_res, Index num_states, Index num_elems,
+                  const Index* res_offsets, const Index* res_dofs,
+                  const Index* state_offsets, const Index* state_dofs,
+                  void** out)
+{
+  return guarded([&] {
+    Array<Array<Index>> rows(num_elems), cols(num_elems);
+    for (Index e = 0; e < num_elems; ++e)
+    {
+      for (Index p = res_offsets[e]; p < res_offsets[e + 1]; ++p) rows[e].push_back(res_dofs[p]);
+      for (Index p = state_offsets[e]; p < state_offsets[e + 1]; ++p) cols[e].push_back(state_dofs[p]);
+    }
+    auto* h = new MapHolder{femx::assembly::makeAssemblyMap(num_res, num_states, rows, cols)};
+    *out    = h;
+  });
+}
+
+int ref_map_sizes(void* h, Index* nnz, Index* jac_len, Index* rows)
+{
+  auto& m  = static_cast<MapHolder*>(h)->map;
+  *nnz     = m.pattern().nnz();
+  *jac_len = m.view().jac_offsets[m.numElems()];
+  *rows    = m.numRes();
+  return 0;
+}
+
+int ref_map_export(void* h, Index* row_ptr, Index* col_ind, Index* jac_offsets, Index* jac_map)
+{
+  auto&       m = static_cast<MapHolder*>(h)->map;
+  const auto  v = m.view();
+  const auto& p = m.pattern();
+  std::memcpy(row_ptr, p.rowPtrData(), sizeof(Index) * (p.rows() + 1));
+  std::memcpy(col_ind, p.colIndData(), sizeof(Index) * p.nnz());
+  std::memcpy(jac_offsets, v.jac_offsets, sizeof(Index) * (m.numElems() + 1));
+  std::memcpy(jac_map, v.jac_map, sizeof(Index) * v.jac_offsets[m.numElems()]);
+  return 0;
+}
+
+void ref_map_free(void* h) { delete static_cast<MapHolder*>(h); }
+
+// femx::assembly::makeBoundaryMap (femx/assembly/BoundaryMap.cpp:59-139).
+int ref_bc_build(Index rows, const Index* row_ptr, const Index* col_ind, Index num_bcs,
+                 const Index* dofs, void** out)
+{
+  return guarded([&] {
+    auto         pat = makePattern(rows, rows, row_ptr, col_ind);
+    Array<Index> d(num_bcs);
+    for (Index i = 0; i < num_bcs; ++i) d[i] = dofs[i];
+    *out = new BcHolder{femx::assembly::makeBoundaryMap(d, pat)};
+  });
+}
+
+Index ref_bc_num_col_entries(void* h)
+{
+  const auto v = static_cast<BcHolder*>(h)->map.view();
+  return v.col_offsets[v.num_bcs];
+}
+
+int ref_bc_export(void* h, Index* bc_rows, Index* diag, Index* col_offsets, Index* col_entries,
+                  Index* col_rows, Index* bc_mask)
+{
+  const auto v = static_cast<BcHolder*>(h)->map.view();
+  std::memcpy(bc_rows, v.bc_rows, sizeof(Index) * v.num_bcs);
+  std::memcpy(diag, v.diag_entries, sizeof(Index) * v.num_bcs);
+  std::memcpy(col_offsets, v.col_offsets, sizeof(Index) * (v.num_bcs + 1));
+  const Index nce = v.col_offsets[v.num_bcs];
+  std::memcpy(col_entries, v.col_entries, sizeof(Index) * nce);
+  std::memcpy(col_rows, v.col_rows, sizeof(Index) * nce);
+  std::memcpy(bc_mask, v.bc_mask, sizeof(Index) * v.num_rows);
+  return 0;
+}
+
+void ref_bc_free(void* h) { delete static_cast<BcHolder*>(h); }
+
+// femx::assembly::replaceRows / replaceRes / zeroBoundary / prepareForwardSolve
+// (femx/assembly/BoundaryMap.cpp:172-247).  `op`: 0 replaceRows(diag),
+// 1 prepareForwardSolve, 2 replaceRes, 3 zeroBoundary.
+int ref_bc_apply(Index op, Index rows, const Index* row_ptr, const Index* col_ind, Index num_bcs,
+                 const Index* dofs, Real* vals, Real* rhs, const Real* bc_vals, const Real* state,
+                 Real diag)
+{
+  return guarded([&] {
+    auto         pat = makePattern(rows, rows, row_ptr, col_ind);
+    Array<Index> d(num_bcs);
+    for (Index i = 0; i < num_bcs; ++i) d[i] = dofs[i];
+    auto               map = femx::assembly::makeBoundaryMap(d, pat);
+    femx::HostCsrMatrix mat(pat);
+    const Index        nnz = pat.nnz();
+    for (Index k = 0; k < nnz; ++k) mat.valsData()[k] = vals ? vals[k] : 0.0;
+    femx::HostVector r(rows), g(num_bcs), s(rows);
+    for (Index i = 0; i < rows; ++i) r[i] = rhs ? rhs[i] : 0.0;
+    for (Index i = 0; i < num_bcs; ++i) g[i] = bc_vals ? bc_vals[i] : 0.0;
+    for (Index i = 0; i < rows; ++i) s[i] = state ? state[i] : 0.0;
+    if (op == 0) femx::assembly::replaceRows(map, mat, diag);
+    else if (op == 1) femx::assembly::prepareForwardSolve(map, mat, r, g);
+    else if (op == 2) femx::assembly::replaceRes(map, s, g, r);
+    else if (op == 3) femx::assembly::zeroBoundary(map, r.view());
+    if (vals)
+      for (Index k = 0; k < nnz; ++k) vals[k] = mat.valsData()[k];
+    if (rhs)
+      for (Index i = 0; i < rows; ++i) rhs[i] = r[i];
+  });
+}
+
+// femx::linalg::MatrixHandler<HostCsrBackend>::matvec / matvecT
+// (femx/linalg/handler/MatrixHandler.cpp:157-198).
+int ref_spmv(Index transpose, Index rows, Index cols, const Index* row_ptr, const Index* col_ind,
+             const Real* vals, const Real* x, Real* y, Real alpha, Real beta)
+{
+  return guarded([&] {
+    auto                pat = makePattern(rows, cols, row_ptr, col_ind);
+    femx::HostCsrMatrix mat(pat);
+    for (Index k = 0; k < pat.nnz(); ++k) mat.valsData()[k] = vals[k];
+    const Index      nin = transpose ? rows : cols;
+    const Index      nout = transpose ? cols : rows;
+    femx::HostVector xv(nin), yv(nout);
+    for (Index i = 0; i < nin; ++i) xv[i] = x[i];
+    for (Index i = 0; i < nout; ++i) yv[i] = y[i];
+    femx::CpuContext                ctx;
+    femx::linalg::HostMatrixHandler mh(ctx);
+    if (transpose) mh.matvecT(mat, xv.view(), yv.view(), alpha, beta);
+    else mh.matvec(mat, xv.view(), yv.view(), alpha, beta);
+    for (Index i = 0; i < nout; ++i) y[i] = yv[i];
+  });
+}
+
+// Poisson (Laplace) assembly on a P1 triangle mesh through the reference's own
+// pipeline: Mesh/FESpace/makeGeometry (femx/fem), makeElementQuadratureData
+// with the order-2 triangle rule, makeAssemblyMap(DofLayout), and
+// assembly::assemble with examples/poisson/PoissonComponents.hpp:35-64.
+// Writes the CSR pattern (row_ptr size N+1, col_ind size nnz_cap), the CSR
+// values and the residual K*state.
+int ref_poisson_p1(Index num_nodes, const Real* x, const Real* y, Index num_elems,
+                   const Index* conn, const Real* state, Index nnz_cap, Index* row_ptr,
+                   Index* col_ind, Real* vals, Real* res)
+{
+  return guarded([&] {
+    femx::fem::Mesh mesh(2);
+    for (Index i = 0; i < num_nodes; ++i) mesh.addNode({x[i], y[i], 0.0});
+    for (Index e = 0; e < num_elems; ++e)
+      mesh.addElem({conn[3 * e], conn[3 * e + 1], conn[3 * e + 2]},
+                   femx::fem::Element::Shape::Triangle, 2, 0, 0, {});
+    femx::fem::LagrangeTriangleP1 fe;
+    femx::fem::FESpace            space(&mesh, &fe);
+    space.setup();
+    const auto geom = femx::fem::makeGeometry(mesh);
+    const auto data = femx::fem::makeElementQuadratureData(
+        space, femx::fem::GaussQuadrature::make(fe.referenceElement(), 2));
+    const auto map = femx::assembly::makeAssemblyMap(femx::fem::DofLayout(space));
+    femx::require(map.pattern().nnz() <= nnz_cap, "nnz capacity too small");
+    femx::HostCsrMatrix jac(map.pattern());
+    femx::HostVector    st(num_nodes), r;
+    for (Index i = 0; i < num_nodes; ++i) st[i] = state[i];
+    femx::CpuContext ctx;
+    femx::assembly::assemble(
+        femx::examples::poisson::PoissonComponents<femx::MemorySpace::Host>(data.view()), geom,
+        map, st, r, jac, ctx);
+    const auto& p = map.pattern();
+    std::memcpy(row_ptr, p.rowPtrData(), sizeof(Index) * (p.rows() + 1));
+    std::memcpy(col_ind, p.colIndData(), sizeof(Index) * p.nnz());
+    std::memcpy(vals, jac.valsData(), sizeof(Real) * p.nnz());
+    for (Index i = 0; i < num_nodes; ++i) res[i] = r[i];
+  });
+}
+
+} // extern "C"

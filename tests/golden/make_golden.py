@@ -1,0 +1,153 @@
+"""Write the golden fixtures of tests/golden/ from the compiled femx reference.
+
+Runs in the development container only: it needs oracle/_ref/libfemx_ref.so,
+which `make -C oracle ref` compiles from femx's own sources.  Each fixture
+holds the inputs (structured triangle meshes built by this repository's
+conventions, seeded values) and the reference's outputs:
+
+  map_p1_n{1,2,4,8}.npz   makeAssemblyMap on P1 DOF tables
+  map_th_n{1,2}.npz       makeAssemblyMap on Taylor-Hood DOF tables
+  bc_p1_n4.npz            makeBoundaryMap on the n=4 boundary DOFs
+  bcops_p1_n4.npz         replaceRows(1 and 0), replaceRes, zeroBoundary,
+                          prepareForwardSolve on seeded values
+  poisson_p1_n{2,8}.npz   assemble(PoissonComponents) K values and K u
+  spmv_p1_n4.npz          matvec / matvecT with alpha, beta
+
+    python tests/golden/make_golden.py
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle import oracle as O  # noqa: E402
+
+I32 = np.int32
+
+
+def p(a):
+    return O._p(a)
+
+
+def ref_map(R, num_res, num_states, off, dofs):
+    E = off.size - 1
+    h = C.c_void_p()
+    assert R.ref_map_build(num_res, num_states, E, p(off), p(dofs), p(off), p(dofs), C.byref(h)) == 0
+    nnz, jl, rows = C.c_int32(), C.c_int32(), C.c_int32()
+    R.ref_map_sizes(h, C.byref(nnz), C.byref(jl), C.byref(rows))
+    rp = np.empty(num_res + 1, I32)
+    ci = np.empty(nnz.value, I32)
+    jo = np.empty(E + 1, I32)
+    jm = np.empty(jl.value, I32)
+    R.ref_map_export(h, p(rp), p(ci), p(jo), p(jm))
+    R.ref_map_free(h)
+    return rp, ci, jo, jm
+
+
+def boundary_dofs_p1(n):
+    """Marker-based DirichletBC order: ascending node id (femx/fem/
+    DirichletBC.cpp:138-145) over nodes on the unit-square boundary."""
+    x, y, _ = O.mesh_p1(n)
+    on = (x == 0.0) | (x == 1.0) | (y == 0.0) | (y == 1.0)
+    return np.nonzero(on)[0].astype(I32)
+
+
+def main():
+    R = O.ref()
+    if R is None:
+        raise SystemExit("oracle/_ref/libfemx_ref.so is not built (make -C oracle ref)")
+    rng = np.random.default_rng(20240611)
+
+    for n in (1, 2, 4, 8):
+        x, y, conn = O.mesh_p1(n)
+        N, E = x.size, conn.size // 3
+        off = O.uniform_offsets(E, 3)
+        rp, ci, jo, jm = ref_map(R, N, N, off, conn)
+        np.savez(os.path.join(HERE, f"map_p1_n{n}.npz"), n=n, conn=conn, row_ptr=rp, col_ind=ci,
+                 jac_offsets=jo, jac_map=jm)
+
+    for n in (1, 2):
+        xv, yv, c6, v3 = O.mesh_p2(n)
+        d = O.th_dofs(n, c6, v3)
+        E = 2 * n * n
+        ndof = 2 * (2 * n + 1) ** 2 + (n + 1) ** 2
+        off = O.uniform_offsets(E, 15)
+        rp, ci, jo, jm = ref_map(R, ndof, ndof, off, d)
+        np.savez(os.path.join(HERE, f"map_th_n{n}.npz"), n=n, dofs15=d, row_ptr=rp, col_ind=ci,
+                 jac_offsets=jo, jac_map=jm)
+
+    # Boundary map and boundary operations on the P1 n=4 pattern.
+    n = 4
+    x, y, conn = O.mesh_p1(n)
+    N, E = x.size, conn.size // 3
+    off = O.uniform_offsets(E, 3)
+    rp, ci, _, _ = ref_map(R, N, N, off, conn)
+    dofs = boundary_dofs_p1(n)
+    h = C.c_void_p()
+    assert R.ref_bc_build(N, p(rp), p(ci), dofs.size, p(dofs), C.byref(h)) == 0
+    nce = R.ref_bc_num_col_entries(h)
+    arrs = dict(bc_rows=np.empty(dofs.size, I32), diag=np.empty(dofs.size, I32),
+                col_offsets=np.empty(dofs.size + 1, I32), col_entries=np.empty(nce, I32),
+                col_rows=np.empty(nce, I32), bc_mask=np.empty(N, I32))
+    R.ref_bc_export(h, *(p(arrs[k]) for k in ("bc_rows", "diag", "col_offsets", "col_entries",
+                                              "col_rows", "bc_mask")))
+    R.ref_bc_free(h)
+    np.savez(os.path.join(HERE, "bc_p1_n4.npz"), row_ptr=rp, col_ind=ci, dofs=dofs, **arrs)
+
+    nnz = ci.size
+    vals0 = rng.standard_normal(nnz)
+    rhs0 = rng.standard_normal(N)
+    g = rng.standard_normal(dofs.size)
+    state = rng.standard_normal(N)
+    out = {}
+    for name, op, diag in (("rows1", 0, 1.0), ("rows0", 0, 0.0), ("forward", 1, 1.0),
+                           ("res", 2, 0.0), ("zero", 3, 0.0)):
+        v = vals0.copy()
+        r = rhs0.copy()
+        assert R.ref_bc_apply(op, N, p(rp), p(ci), dofs.size, p(dofs), p(v), p(r), p(g), p(state),
+                              diag) == 0, R.ref_last_error()
+        out[name + "_vals"], out[name + "_rhs"] = v, r
+    np.savez(os.path.join(HERE, "bcops_p1_n4.npz"), row_ptr=rp, col_ind=ci, dofs=dofs,
+             vals0=vals0, rhs0=rhs0, bc_vals=g, state=state, **out)
+
+    # Poisson (Laplace) assembly through the reference's own pipeline.
+    for n in (2, 8):
+        x, y, conn = O.mesh_p1(n)
+        N, E = x.size, conn.size // 3
+        u = rng.standard_normal(N)
+        cap = 7 * n * n + 6 * n + 1
+        rp = np.empty(N + 1, I32)
+        ci = np.empty(cap, I32)
+        vals = np.empty(cap)
+        res = np.empty(N)
+        assert R.ref_poisson_p1(N, p(x), p(y), E, p(conn), p(u), cap, p(rp), p(ci), p(vals),
+                                p(res)) == 0, R.ref_last_error()
+        np.savez(os.path.join(HERE, f"poisson_p1_n{n}.npz"), n=n, x=x, y=y, conn=conn, state=u,
+                 row_ptr=rp, col_ind=ci, vals=vals, res=res)
+
+    # SpMV / transpose apply on the P1 n=4 pattern with seeded values.
+    n = 4
+    x, y, conn = O.mesh_p1(n)
+    N, E = x.size, conn.size // 3
+    off = O.uniform_offsets(E, 3)
+    rp, ci, _, _ = ref_map(R, N, N, off, conn)
+    vals = rng.standard_normal(ci.size)
+    xin = rng.uniform(-1, 1, N)
+    y0 = rng.standard_normal(N)
+    res = {}
+    for tr in (0, 1):
+        for alpha, beta in ((1.0, 0.0), (1.5, -0.5)):
+            yy = y0.copy()
+            assert R.ref_spmv(tr, N, N, p(rp), p(ci), p(vals), p(xin), p(yy), alpha, beta) == 0
+            res[f"y_t{tr}_a{alpha}_b{beta}"] = yy
+    np.savez(os.path.join(HERE, "spmv_p1_n4.npz"), row_ptr=rp, col_ind=ci, vals=vals, x=xin,
+             y0=y0, **res)
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

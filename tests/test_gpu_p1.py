@@ -1,0 +1,228 @@
+"""HIP path vs the oracle and the compiled reference's fixtures: P1 Poisson
+assembly, Dirichlet operations, SpMV and vector ops.  Integer outputs are
+bit-exact; the FP64 outputs of these kernels are bitwise equal to the
+oracle's (same operation order, no contraction), which is stricter than the
+1e-12 mixed-norm bar of conftest.close."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import close, golden
+from femx_amd import assembly as A
+from femx_amd.linalg import CsrMatrix, CsrPattern, MatrixHandler, VectorHandler
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+RNG = np.random.default_rng(3)
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(DEV)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy()
+
+
+def p1_setup(n):
+    geom = A.structured_p1(n, DEV)
+    amap = A.make_assembly_map(geom.num_nodes, geom.num_nodes, geom.h_conn)
+    return geom, amap
+
+
+@pytest.mark.parametrize("n", [1, 2, 8, 33])
+def test_p1_assemble_bitwise_vs_oracle(n):
+    geom, amap = p1_setup(n)
+    N, E = geom.num_nodes, geom.num_elems
+    kappa = np.exp(0.5 * RNG.standard_normal(E))
+    f = RNG.standard_normal(N)
+    u = RNG.standard_normal(N)
+    jac = CsrMatrix(amap.pattern, DEV)
+    res = torch.empty(N, dtype=torch.float64, device=DEV)
+    A.assemble(A.PoissonP1(dev(kappa), dev(f)), geom, amap, dev(u), res, jac)
+    _, jm = amap.jac()
+    ov, orr = O.p1_assemble(geom.h_x, geom.h_y, geom.h_conn.reshape(-1), kappa, f, u, jm,
+                            amap.pattern.nnz)
+    assert np.array_equal(host(jac.vals), ov)
+    assert np.array_equal(host(res), orr)
+    # residual-only pass gives the same residual
+    res2 = torch.full((N,), np.nan, dtype=torch.float64, device=DEV)
+    A.assemble(A.PoissonP1(dev(kappa), dev(f)), geom, amap, dev(u), res2, None)
+    assert np.array_equal(host(res2), orr)
+    # log-kappa input
+    res3 = torch.empty(N, dtype=torch.float64, device=DEV)
+    jac3 = CsrMatrix(amap.pattern, DEV)
+    A.assemble(A.PoissonP1(f=dev(f), log_kappa=dev(np.log(kappa))), geom, amap, dev(u), res3, jac3)
+    assert close(host(jac3.vals), ov) and close(host(res3), orr)
+
+
+@pytest.mark.parametrize("n", [2, 8])
+def test_p1_laplace_bitwise_vs_reference_fixture(n):
+    """kappa = 1, f = 0: femx's own PoissonComponents assembly."""
+    g = golden(f"poisson_p1_n{n}.npz")
+    geom = A.Geometry(g["x"], g["y"], g["conn"].reshape(-1, 3), DEV)
+    amap = A.make_assembly_map(geom.num_nodes, geom.num_nodes, geom.h_conn)
+    assert np.array_equal(amap.pattern.row_ptr, g["row_ptr"])
+    jac = CsrMatrix(amap.pattern, DEV)
+    res = torch.empty(geom.num_nodes, dtype=torch.float64, device=DEV)
+    A.assemble(A.PoissonP1(), geom, amap, dev(g["state"]), res, jac)
+    assert np.array_equal(host(jac.vals), g["vals"])
+    assert np.array_equal(host(res), g["res"])
+
+
+def test_p1_full_size_bitwise_and_deterministic():
+    """Config 2 size (2048^2 cells, 8.4M elements): bitwise equal to the
+    oracle and to a second launch."""
+    n = 2048
+    geom, amap = p1_setup(n)
+    N, E = geom.num_nodes, geom.num_elems
+    assert amap.pattern.nnz == 7 * n * n + 6 * n + 1
+    rng = np.random.default_rng(2)
+    kappa = np.exp(0.5 * rng.standard_normal(E))
+    f = rng.standard_normal(N)
+    u = rng.standard_normal(N)
+    op = A.PoissonP1(dev(kappa), dev(f))
+    jac = CsrMatrix(amap.pattern, DEV)
+    res = torch.empty(N, dtype=torch.float64, device=DEV)
+    A.assemble(op, geom, amap, dev(u), res, jac)
+    v1, r1 = host(jac.vals), host(res)
+    A.assemble(op, geom, amap, dev(u), res, jac)
+    assert np.array_equal(host(jac.vals), v1) and np.array_equal(host(res), r1)
+    _, jm = amap.jac()
+    ov, orr = O.p1_assemble(geom.h_x, geom.h_y, geom.h_conn.reshape(-1), kappa, f, u, jm,
+                            amap.pattern.nnz)
+    assert np.array_equal(v1, ov) and np.array_equal(r1, orr)
+
+
+def test_bc_ops_bitwise_vs_reference_fixture():
+    g = golden("bcops_p1_n4.npz")
+    N = g["row_ptr"].size - 1
+    pat = CsrPattern(N, N, g["row_ptr"], g["col_ind"])
+    b = A.make_boundary_map(g["dofs"], pat, DEV)
+    for name, diag in (("rows1", 1.0), ("rows0", 0.0)):
+        m = CsrMatrix(pat, DEV, dev(g["vals0"]))
+        A.replace_rows(b, m, diag)
+        assert np.array_equal(host(m.vals), g[name + "_vals"])
+    m = CsrMatrix(pat, DEV, dev(g["vals0"]))
+    rhs = dev(g["rhs0"])
+    A.prepare_forward_solve(b, m, rhs, dev(g["bc_vals"]))
+    assert np.array_equal(host(m.vals), g["forward_vals"])
+    assert np.array_equal(host(rhs), g["forward_rhs"])
+    r = dev(g["rhs0"])
+    A.replace_res(b, dev(g["state"]), dev(g["bc_vals"]), r)
+    assert np.array_equal(host(r), g["res_rhs"])
+    r = dev(g["rhs0"])
+    A.zero_boundary(b, r)
+    assert np.array_equal(host(r), g["zero_rhs"])
+
+
+def test_bc_kat_and_layout_check():
+    # tests/unit/backend/runBackendStateTests.cpp:80-114
+    pat = CsrPattern(3, 3, [0, 3, 6, 9], [0, 1, 2] * 3)
+    b = A.make_boundary_map([1], pat, DEV)
+    m = CsrMatrix(pat, DEV, dev(np.array([4.0, 1.0, 2.0, 3.0, 5.0, 6.0, 7.0, 8.0, 9.0])))
+    rhs = dev(np.array([10.0, 20.0, 30.0]))
+    A.prepare_forward_solve(b, m, rhs, dev(np.array([2.0])))
+    assert host(m.vals).tolist() == [4.0, 0.0, 2.0, 0.0, 1.0, 0.0, 7.0, 0.0, 9.0]
+    assert host(rhs).tolist() == [8.0, 2.0, 14.0]
+    other = CsrMatrix(CsrPattern(3, 3, [0, 3, 6, 9], [0, 1, 2] * 3), DEV)
+    with pytest.raises(RuntimeError, match="layout"):
+        A.replace_rows(b, other, 1.0)
+
+
+def test_spmv_bitwise_vs_reference_fixture():
+    g = golden("spmv_p1_n4.npz")
+    N = g["row_ptr"].size - 1
+    pat = CsrPattern(N, N, g["row_ptr"], g["col_ind"])
+    m = CsrMatrix(pat, DEV, dev(g["vals"]))
+    mh = MatrixHandler()
+    for tr in (0, 1):
+        for alpha, beta in ((1.0, 0.0), (1.5, -0.5)):
+            y = dev(g["y0"])
+            (mh.matvecT if tr else mh.matvec)(m, dev(g["x"]), y, alpha, beta)
+            assert np.array_equal(host(y), g[f"y_t{tr}_a{alpha}_b{beta}"])
+
+
+def test_linalg_kats():
+    # tests/unit/backend/runCudaLinalgTests.cu:70-181
+    pat = CsrPattern(3, 4, [0, 2, 4, 7], [0, 2, 1, 3, 0, 2, 3])
+    m = CsrMatrix(pat, DEV, dev(np.array([2.0, -1.0, 3.0, 4.0, -2.0, 5.0, 1.0])))
+    mh, vh = MatrixHandler(), VectorHandler()
+    x = dev(np.array([1.0, 2.0, 3.0, 4.0]))
+    y = torch.empty(3, dtype=torch.float64, device=DEV)
+    mh.matvec(m, x, y)
+    assert host(y).tolist() == [-1.0, 22.0, 17.0]
+    assert host(vh.dot(x, x)).tolist() == [30.0]
+    vh.axpby(-2.0, dev(np.array([1.0, 2.0, 3.0])), 0.5, y)
+    assert host(y).tolist() == [-2.5, 7.0, 2.5]
+    t = torch.empty(4, dtype=torch.float64, device=DEV)
+    mh.matvecT(m, dev(np.array([2.0, -1.0, 0.5])), t)
+    ref = np.zeros(4)
+    O.matvecT(pat.row_ptr, pat.col_ind, np.array([2.0, -1.0, 3.0, 4.0, -2.0, 5.0, 1.0]), 4,
+              np.array([2.0, -1.0, 0.5]), ref)
+    assert np.array_equal(host(t), ref)
+    buf = torch.zeros(9, dtype=torch.float64, device=DEV)
+    with pytest.raises(RuntimeError, match="in-place"):
+        from femx_amd import _lib
+        from femx_amd.linalg import ptr, stream_handle
+        rp, ci = pat.device_arrays(DEV)
+        _lib.call("fx_csr_matvec", 3, 4, ptr(rp), ptr(ci), ptr(m.vals), buf.data_ptr() + 24,
+                  buf.data_ptr() + 32, 1.0, 0.0, stream_handle())
+
+
+def test_dot_is_deterministic_and_accurate():
+    vh = VectorHandler()
+    x = dev(RNG.standard_normal(3_000_001))
+    y = dev(RNG.standard_normal(3_000_001))
+    a = host(vh.dot(x, y))[0]
+    b = host(vh.dot(x, y))[0]
+    assert a == b
+    ref = float(np.dot(host(x), host(y)))
+    assert abs(a - ref) <= 1e-12 * np.sqrt(3e6) * 2
+
+
+def test_config1_pipeline_bitwise():
+    """Config 1: 32x32 P1, kappa = 1, f = 2 pi^2 sin sin, homogeneous
+    Dirichlet on all edges; assemble, apply BCs, one SpMV."""
+    n = 32
+    geom, amap = p1_setup(n)
+    N = geom.num_nodes
+    x, y = geom.h_x, geom.h_y
+    f = 2 * np.pi ** 2 * np.sin(np.pi * x) * np.sin(np.pi * y)
+    dofs = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+    assert dofs.size == 128
+    b = A.make_boundary_map(dofs, amap.pattern, DEV)
+    jac = CsrMatrix(amap.pattern, DEV)
+    res = torch.empty(N, dtype=torch.float64, device=DEV)
+    A.assemble(A.PoissonP1(None, dev(f)), geom, amap, None, res, jac)
+    rhs = -res
+    A.prepare_forward_solve(b, jac, rhs, torch.zeros(dofs.size, dtype=torch.float64, device=DEV))
+    xs = dev(np.sin(np.pi * x) * np.sin(np.pi * y))
+    out = torch.empty(N, dtype=torch.float64, device=DEV)
+    MatrixHandler().matvec(jac, xs, out)
+    _, jm = amap.jac()
+    ov, orr = O.p1_assemble(x, y, geom.h_conn.reshape(-1), np.ones(geom.num_elems), f,
+                            np.zeros(N), jm, amap.pattern.nnz)
+    orhs = -orr
+    bc = O.bc_build(amap.pattern.row_ptr, amap.pattern.col_ind, dofs)
+    O.prepare_forward(amap.pattern.row_ptr, bc, ov, orhs, np.zeros(dofs.size))
+    oy = np.zeros(N)
+    O.matvec(amap.pattern.row_ptr, amap.pattern.col_ind, ov, np.sin(np.pi * x) * np.sin(np.pi * y),
+             oy)
+    assert np.array_equal(host(jac.vals), ov)
+    assert np.array_equal(host(rhs), orhs)
+    assert np.array_equal(host(out), oy)
+    # A u_exact approximates the right-hand side (consistency, O(h^2))
+    assert np.abs(host(out) - orhs).max() < 0.05 * np.abs(orhs).max()
+
+
+def test_empty_inputs():
+    pat = CsrPattern(0, 0, [0], [])
+    m = CsrMatrix(pat, DEV)
+    y = torch.empty(0, dtype=torch.float64, device=DEV)
+    MatrixHandler().matvec(m, torch.empty(0, dtype=torch.float64, device=DEV), y)
+    b = A.make_boundary_map([], CsrPattern(2, 2, [0, 1, 2], [0, 1]), DEV)
+    A.zero_boundary(b, torch.ones(2, dtype=torch.float64, device=DEV))
+    torch.cuda.synchronize()
