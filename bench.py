@@ -60,6 +60,10 @@ def main():
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vjp", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="for rocprofv3 counter runs: a 1 GiB device copy (counter "
+                         "calibration), then 5 launches each of assembly, SpMV, elimination "
+                         "and VJP; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
@@ -114,6 +118,30 @@ def main():
         A.prepare_forward_solve(bmap, jac, rhs, g0)
         for k in range(10):
             mh.matvec(jac, d_xs[k], d_ys[k])
+
+    if args.kernels_only:
+        src = torch.empty(2 ** 27, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        for _ in range(5):
+            assemble_launch()
+        for _ in range(5):
+            mh.matvec(jac, d_xs[0], d_ys[0])
+        for _ in range(5):
+            A.prepare_forward_solve(bmap, jac, rhs, g0)
+        gv = A.structured_p1(1024, dev)
+        xv, yv = gv.h_x, gv.h_y
+        vd = np.nonzero((xv == 0) | (xv == 1) | (yv == 0) | (yv == 1))[0].astype(np.int32)
+        Rv = PoissonLogKappaResidual(gv, vd)
+        m = t(0.5 * np.random.default_rng(6).standard_normal(gv.num_elems))
+        lam = t(np.random.default_rng(7).standard_normal(gv.num_nodes))
+        uv = t(np.random.default_rng(8).standard_normal(gv.num_nodes))
+        for _ in range(5):
+            Rv.vjp(uv, m, lam)
+        torch.cuda.synchronize()
+        log("[bench] kernels-only done")
+        return
 
     log("[bench] warmup")
     for _ in range(args.warmup):

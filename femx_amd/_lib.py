@@ -27,6 +27,8 @@ _SIGS = {
     "fx_map_export_pattern": ([_vp, _vp, _vp], C.c_int),
     "fx_map_export_jac": ([_vp, _vp, _vp], C.c_int),
     "fx_map_export_rowinc3": ([_vp, _vp, _vp], C.c_int),
+    "fx_map_export_rowinc15": ([_vp, _vp, _vp, _vp], C.c_int),
+    "fx_map_th_blocks": ([_vp, _i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_map_destroy": ([_vp], None),
     "fx_csr_transpose_map": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "fx_bcmap_create": ([_i32, _vp, _vp, _i32, _vp, C.POINTER(_vp)], C.c_int),
@@ -47,6 +49,8 @@ _SIGS = {
     "fx_p1_assemble": ([_i32, _i32, _i32, _i32] + [_vp] * 4 + [C.c_int] + [_vp] * 7 + [_vp],
                        C.c_int),
     "fx_p1_vjp": ([_i32, _i32] + [_vp] * 11 + [_vp], C.c_int),
+    "fx_th_assemble": ([_i32] * 4 + [_vp] * 5 + [_f64] + [_vp] * 9 + [_vp], C.c_int),
+    "fx_th_vjp": ([_i32] * 3 + [_vp] * 5 + [_f64] + [_vp] * 10 + [_vp], C.c_int),
 }
 
 

@@ -54,6 +54,7 @@ class AssemblyMap:
         _lib.call("fx_map_create", int(num_res), int(num_states), E, ptr(self.res_offsets),
                   ptr(self.res_dofs), ptr(self.state_offsets), ptr(self.state_dofs), C.byref(h))
         self._h = h
+        self._destroy = _lib.lib().fx_map_destroy
         info = _lib.MapInfo()
         _lib.call("fx_map_info_get", h, C.byref(info))
         self.info = info
@@ -71,7 +72,7 @@ class AssemblyMap:
     def __del__(self):
         h = getattr(self, "_h", None)
         if h:
-            _lib.lib().fx_map_destroy(h)
+            self._destroy(h)
             self._h = None
 
     def jac(self):

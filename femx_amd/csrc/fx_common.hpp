@@ -71,6 +71,19 @@ inline bool overlaps(const void* a, std::size_t na, const void* b, std::size_t n
   return pa < pb + nb && pb < pa + na;
 }
 
+// Blocks of consecutive rows land on the same XCD in order: hardware deals
+// workgroups to the 8 XCDs round-robin, so remap the dispatch index to keep
+// neighbouring row blocks (which share elements and nodes) in one L2.
+__device__ __forceinline__ int xcd_block(int nblocks)
+{
+  const int bid = blockIdx.x;
+  const int per = nblocks / 8, rem = nblocks % 8;
+  const int xcd = bid % 8, k = bid / 8;
+  // XCD x owns logical blocks [x*per + min(x,rem), ...) of length per + (x<rem),
+  // and receives exactly that many dispatch indices (those with bid % 8 == x).
+  return xcd * per + min(xcd, rem) + k;
+}
+
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 inline unsigned blocks_for(std::int64_t n, int block)

@@ -346,6 +346,91 @@ int fx_map_export_rowinc3(const fx_map* m, Index* inc_off, Index* inc)
   });
 }
 
+
+// Row incidence for 15x15 elements (Taylor-Hood): inc_elem[num_inc] and
+// inc_meta[4*num_inc] (16 bytes per incidence: positions of the element's 15
+// columns within the row, from row_ptr[row], then the local row).
+int fx_map_export_rowinc15(const fx_map* m, Index* inc_off, Index* inc_elem, Index* inc_meta)
+{
+  return fx::guard([&] {
+    fx::require(m && inc_off && (m->inc_elem.empty() || (inc_elem && inc_meta)), FX_ERR_ARG,
+                "fx_map_export_rowinc15: null argument");
+    for (Index e = 0; e < m->num_elems; ++e)
+      fx::require(m->res_off[e + 1] - m->res_off[e] == 15 && m->st_off[e + 1] - m->st_off[e] == 15,
+                  FX_ERR_UNSUPPORTED, "fx_map_export_rowinc15: every element needs 15x15 local DOFs");
+    fx::require(m->max_row_len <= 255, FX_ERR_UNSUPPORTED,
+                "fx_map_export_rowinc15: CSR rows longer than 255 entries");
+    std::copy(m->inc_off.begin(), m->inc_off.end(), inc_off);
+    for (Index r = 0; r < m->num_res; ++r)
+      for (Index t = m->inc_off[r]; t < m->inc_off[r + 1]; ++t)
+      {
+        const Index  e = m->inc_elem[t], li = m->inc_local[t];
+        const Index* jm   = m->jac_map.data() + m->jac_off[e] + 15 * li;
+        std::uint8_t b[16];
+        for (int j = 0; j < 15; ++j) b[j] = static_cast<std::uint8_t>(jm[j] - m->row_ptr[r]);
+        b[15] = static_cast<std::uint8_t>(li);
+        inc_elem[t] = e;
+        std::memcpy(inc_meta + 4 * t, b, 16);
+      }
+  });
+}
+
+// Work blocks of the Taylor-Hood row kernels.  Rows are cut into blocks of
+// at most `lanes` work items; a work item is a pair of velocity rows (r, r+1)
+// when the two rows are interchangeable for the kernel -- same incident
+// elements in the same order, local rows li and li+1 with li even and < 12,
+// identical column positions -- and a single row otherwise.  Blocks hold
+// contiguous rows.  blk_row[num_blocks+1] are block row starts, blk_flags
+// the item kind (1 = pairs, 0 = single rows); max_block_nnz bounds the LDS
+// row tile.  Call with null arrays to get the counts.
+int fx_map_th_blocks(const fx_map* m, Index lanes, Index* num_blocks, Index* max_block_nnz,
+                     Index* blk_row, Index* blk_flags)
+{
+  return fx::guard([&] {
+    fx::require(m && num_blocks && max_block_nnz && lanes > 0, FX_ERR_ARG,
+                "fx_map_th_blocks: invalid arguments");
+    auto pairable = [&](Index r) {
+      if (r + 1 >= m->num_res) return false;
+      const Index a0 = m->inc_off[r], a1 = m->inc_off[r + 1], b1 = m->inc_off[r + 2];
+      if (a1 - a0 != b1 - a1 || a1 == a0) return false;
+      for (Index k = 0; k < a1 - a0; ++k)
+      {
+        const Index t = a0 + k, u = a1 + k;
+        const Index e = m->inc_elem[t], li = m->inc_local[t];
+        if (m->inc_elem[u] != e || m->inc_local[u] != li + 1 || (li & 1) || li >= 12) return false;
+        if (m->res_off[e + 1] - m->res_off[e] != 15 || m->st_off[e + 1] - m->st_off[e] != 15)
+          return false;
+        const Index* ja = m->jac_map.data() + m->jac_off[e] + 15 * li;
+        const Index* jb = ja + 15;
+        for (int j = 0; j < 15; ++j)
+          if (ja[j] - m->row_ptr[r] != jb[j] - m->row_ptr[r + 1]) return false;
+      }
+      return true;
+    };
+    std::vector<Index> starts, flags;
+    Index              r = 0, mx = 0;
+    while (r < m->num_res)
+    {
+      const bool pair  = pairable(r);
+      const Index start = r;
+      Index       items = 0;
+      while (r < m->num_res && items < lanes && pairable(r) == pair)
+      {
+        r += pair ? 2 : 1;
+        ++items;
+      }
+      starts.push_back(start);
+      flags.push_back(pair ? 1 : 0);
+      mx = std::max(mx, m->row_ptr[r] - m->row_ptr[start]);
+    }
+    starts.push_back(m->num_res);
+    *num_blocks    = static_cast<Index>(flags.size());
+    *max_block_nnz = mx;
+    if (blk_row) std::copy(starts.begin(), starts.end(), blk_row);
+    if (blk_flags) std::copy(flags.begin(), flags.end(), blk_flags);
+  });
+}
+
 void fx_map_destroy(fx_map* m) { delete m; }
 
 // ---------------------------------------------------------------------------

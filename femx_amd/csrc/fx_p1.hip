@@ -5,8 +5,11 @@
 // recomputes the one element-matrix row it needs from the three vertex
 // coordinates in registers (affine triangles: constant gradients), so no
 // per-element quadrature data is streamed from HBM.  A workgroup owns 256
-// consecutive rows; their CSR values are one contiguous range, accumulated in
-// LDS and written back with coalesced stores.  Every CSR value and residual
+// consecutive rows: their incidence entries are one contiguous range, staged
+// in LDS with coalesced loads, and their CSR values are one contiguous range,
+// accumulated in LDS and written back with coalesced stores.  Streams read or
+// written once (incidences, outputs) use non-temporal accesses so that the
+// L2 keeps the node and element data that neighbouring rows share.  Every CSR value and residual
 // entry is a sum over its elements in ascending element order with no
 // atomics: results are deterministic and equal, operation for operation, to
 // femx's host assemble loop (femx/assembly/Assembly.hpp:184-225) evaluating
@@ -20,7 +23,7 @@ namespace
 {
 
 constexpr int kP1Block = 256;
-constexpr int kP1MaxRowLen = 32; // LDS bound: 256 rows x 32 x 8 B = 64 KiB
+constexpr int kP1MaxRowLen = 32; // LDS bound: 256 rows x 32 x 8 B = 64 KiB (+16 KiB staging)
 
 // Physical P1 gradients and |det J|*wt of an affine triangle, evaluated as
 // ElementValues::calcPhysicalValues/invJacobian do with the P1 reference
@@ -87,17 +90,19 @@ __device__ __forceinline__ double p1_load(const P1Geom& g, int r, double f0, dou
   return F;
 }
 
-// Blocks of consecutive rows land on the same XCD in order: hardware deals
-// workgroups to the 8 XCDs round-robin, so remap the dispatch index to keep
-// neighbouring row blocks (which share elements and nodes) in one L2.
-__device__ __forceinline__ int xcd_block(int nblocks)
+// Incidences of one block staged in LDS when they fit (coalesced loads of a
+// contiguous range); longer blocks read them straight from global memory.
+constexpr int kIncStage = 2048;
+
+__device__ __forceinline__ int2 inc_at(const long long* sinc, const int2* inc, bool staged, int t,
+                                       int ib0)
 {
-  const int bid = blockIdx.x;
-  const int per = nblocks / 8, rem = nblocks % 8;
-  const int xcd = bid % 8, k = bid / 8;
-  // XCD x owns logical blocks [x*per + min(x,rem), ...) of length per + (x<rem),
-  // and receives exactly that many dispatch indices (those with bid % 8 == x).
-  return xcd * per + min(xcd, rem) + k;
+  if (staged)
+  {
+    const long long v = sinc[t - ib0];
+    return make_int2(static_cast<int>(v & 0xffffffffll), static_cast<int>(v >> 32));
+  }
+  return inc[t];
 }
 
 __global__ __launch_bounds__(kP1Block) void k_p1_assemble(
@@ -105,16 +110,21 @@ __global__ __launch_bounds__(kP1Block) void k_p1_assemble(
     const int* __restrict__ conn, const double* __restrict__ kappa, int kappa_is_log,
     const double* __restrict__ f, const double* __restrict__ u, const int* __restrict__ inc_off,
     const int2* __restrict__ inc, const int* __restrict__ row_ptr, double* __restrict__ vals,
-    double* __restrict__ res)
+    double* __restrict__ res, int acc_cap)
 {
-  extern __shared__ double acc[];
-  const int nblocks = gridDim.x;
-  const int blk     = xcd_block(nblocks);
-  const int r0      = blk * kP1Block;
-  const int r1      = min(r0 + kP1Block, num_rows);
-  const int base    = row_ptr[r0];
-  const int bn      = vals ? row_ptr[r1] - base : 0;
+  extern __shared__ double smem[];
+  double*     acc  = smem;
+  long long*  sinc = reinterpret_cast<long long*>(smem + acc_cap);
+  const int   r0   = blockIdx.x * kP1Block;
+  const int   r1   = min(r0 + kP1Block, num_rows);
+  const int   base = row_ptr[r0];
+  const int   bn   = vals ? row_ptr[r1] - base : 0;
+  const int   ib0 = inc_off[r0], ib1 = inc_off[r1];
+  const bool  staged = ib1 - ib0 <= kIncStage;
   for (int i = threadIdx.x; i < bn; i += kP1Block) acc[i] = 0.0;
+  if (staged)
+    for (int i = threadIdx.x; i < ib1 - ib0; i += kP1Block)
+      sinc[i] = __builtin_nontemporal_load(reinterpret_cast<const long long*>(inc) + ib0 + i);
   __syncthreads();
 
   const int r = r0 + threadIdx.x;
@@ -125,7 +135,7 @@ __global__ __launch_bounds__(kP1Block) void k_p1_assemble(
     const int t1   = inc_off[r + 1];
     for (int t = inc_off[r]; t < t1; ++t)
     {
-      const int2     it   = inc[t];
+      const int2     it   = inc_at(sinc, inc, staged, t, ib0);
       const int      e    = it.x;
       const unsigned meta = static_cast<unsigned>(it.y);
       const int      li   = meta & 0xff;
@@ -145,10 +155,11 @@ __global__ __launch_bounds__(kP1Block) void k_p1_assemble(
         racc[meta >> 24]          = racc[meta >> 24] + k2;
       }
     }
-    res[r] = rsum;
+    __builtin_nontemporal_store(rsum, res + r);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < bn; i += kP1Block) vals[base + i] = acc[i];
+  for (int i = threadIdx.x; i < bn; i += kP1Block)
+    __builtin_nontemporal_store(acc[i], vals + base + i);
 }
 
 // One pass: each lane (node j) sums column j of its elements' matrices against
@@ -161,13 +172,21 @@ __global__ __launch_bounds__(kP1Block) void k_p1_vjp(
     const int* __restrict__ inc_off, const int2* __restrict__ inc, double* __restrict__ jt,
     double* __restrict__ gout)
 {
-  const int j = xcd_block(gridDim.x) * kP1Block + threadIdx.x;
+  __shared__ long long sinc[kIncStage];
+  const int  r0 = blockIdx.x * kP1Block, r1 = min(r0 + kP1Block, num_rows);
+  const int  ib0 = inc_off[r0], ib1 = inc_off[r1];
+  const bool staged = ib1 - ib0 <= kIncStage;
+  if (staged)
+    for (int i = threadIdx.x; i < ib1 - ib0; i += kP1Block)
+      sinc[i] = __builtin_nontemporal_load(reinterpret_cast<const long long*>(inc) + ib0 + i);
+  __syncthreads();
+  const int j = r0 + threadIdx.x;
   if (j >= num_rows) return;
   double    acc = 0.0;
   const int t1  = inc_off[j + 1];
   for (int t = inc_off[j]; t < t1; ++t)
   {
-    const int2   it = inc[t];
+    const int2   it = inc_at(sinc, inc, staged, t, ib0);
     const int    e  = it.x;
     const int    li = static_cast<unsigned>(it.y) & 0xff;
     const int    n[3] = {conn[3 * e], conn[3 * e + 1], conn[3 * e + 2]};
@@ -192,7 +211,7 @@ __global__ __launch_bounds__(kP1Block) void k_p1_vjp(
     }
   }
   if (bc_mask && bc_mask[j]) acc = acc + lam[j];
-  jt[j] = acc;
+  __builtin_nontemporal_store(acc, jt + j);
 }
 
 } // namespace
@@ -217,11 +236,12 @@ int fx_p1_assemble(fx_index num_nodes, fx_index num_elems, fx_index nnz, fx_inde
                     && !(u && (fx::overlaps(u, N * 8, res, N * 8)
                                || fx::overlaps(u, N * 8, vals, std::size_t(nnz) * 8))),
                 FX_ERR_ALIAS, "Assembly state, residual, and matrix values must not alias");
-    const unsigned nb  = fx::blocks_for(num_nodes, kP1Block);
-    const size_t   lds = static_cast<size_t>(kP1Block) * (max_row_len > 0 ? max_row_len : 1) * 8;
+    const unsigned nb      = fx::blocks_for(num_nodes, kP1Block);
+    const int      acc_cap = kP1Block * (max_row_len > 0 ? max_row_len : 1);
+    const size_t   lds     = static_cast<size_t>(acc_cap) * 8 + kIncStage * 8;
     hipLaunchKernelGGL(k_p1_assemble, dim3(nb), dim3(kP1Block), lds, fx::as_stream(stream),
                        num_nodes, x, y, conn, kappa, kappa_is_log, f, u, inc_off,
-                       reinterpret_cast<const int2*>(inc), row_ptr, vals, res);
+                       reinterpret_cast<const int2*>(inc), row_ptr, vals, res, acc_cap);
     fx::check_launch("fx_p1_assemble");
   });
 }

@@ -5,8 +5,7 @@
 // 297-317).  SpMV keeps one row per lane and sums the row in CSR order, the
 // order of femx's host matvec (MatrixHandler.cpp:157-175), so results equal
 // the host's bit for bit; FE rows are short (7 entries for structured P1), so
-// a lane per row reads each wave's slice of vals/col_ind as one contiguous
-// span.  The transpose apply gathers each column through the transpose
+// a block's rows are staged through LDS with coalesced loads.  The transpose apply gathers each column through the transpose
 // permutation in ascending row order -- the order femx's host matvecT adds
 // into y (MatrixHandler.cpp:177-198) -- instead of scattering with atomics.
 #include "fx_common.hpp"
@@ -17,6 +16,12 @@ constexpr int kRowBlock = 256;
 constexpr int kDotBlock = 256;
 constexpr int kDotGrid  = 1024;
 
+constexpr int kStage = 2048; // CSR entries staged per block (24 KiB of LDS)
+
+// CSR-stream SpMV: a block's 256 rows own one contiguous range of vals and
+// col_ind, loaded into LDS with coalesced non-temporal loads (they are read
+// once); each lane then sums its row from LDS in CSR order.  Blocks whose
+// range exceeds the stage read their rows directly.
 __global__ __launch_bounds__(kRowBlock) void k_spmv(int rows, const int* __restrict__ row_ptr,
                                                     const int* __restrict__ col,
                                                     const double* __restrict__ vals,
@@ -24,11 +29,33 @@ __global__ __launch_bounds__(kRowBlock) void k_spmv(int rows, const int* __restr
                                                     double* __restrict__ y, double alpha,
                                                     double beta)
 {
-  const int r = blockIdx.x * kRowBlock + threadIdx.x;
+  __shared__ double sv[kStage];
+  __shared__ int    sc[kStage];
+  const int  r0 = fx::xcd_block(gridDim.x) * kRowBlock, r1 = min(r0 + kRowBlock, rows);
+  const int  k0 = row_ptr[r0], kn = row_ptr[r1] - k0;
+  const bool staged = kn <= kStage;
+  if (staged)
+  {
+    for (int i = threadIdx.x; i < kn; i += kRowBlock)
+    {
+      sv[i] = __builtin_nontemporal_load(vals + k0 + i);
+      sc[i] = __builtin_nontemporal_load(col + k0 + i);
+    }
+    __syncthreads();
+  }
+  const int r = r0 + threadIdx.x;
   if (r >= rows) return;
-  double    v  = 0.0;
-  const int k1 = row_ptr[r + 1];
-  for (int k = row_ptr[r]; k < k1; ++k) v = v + vals[k] * x[col[k]];
+  double v = 0.0;
+  if (staged)
+  {
+    const int a = row_ptr[r] - k0, b = row_ptr[r + 1] - k0;
+    for (int k = a; k < b; ++k) v = v + sv[k] * x[sc[k]];
+  }
+  else
+  {
+    const int k1 = row_ptr[r + 1];
+    for (int k = row_ptr[r]; k < k1; ++k) v = v + vals[k] * x[col[k]];
+  }
   y[r] = beta == 0.0 ? alpha * v : alpha * v + beta * y[r];
 }
 

@@ -1,0 +1,256 @@
+"""Taylor-Hood P2/P1 steady Navier-Stokes on the HIP kernels.
+
+`THGeometry` holds the vertex coordinates (the geometry of the affine P2
+element), the vertex and P2 connectivity, and the element DOF table in the
+FESpace/MixedFESpace numbering (velocity DOFs 2*node + c, then pressure DOFs
+offset by 2*N_v; femx/fem/FESpace.cpp:96-106, femx/fem/MixedFESpace.cpp:
+121-137).  `assemble_ns` drops in for assembly::assemble with this operator
+(femx/assembly/CudaAssembly.hpp:319-364); `NavierStokesResidual` fills the
+stationary Residual contract (femx/state/Residual.hpp:18-60) with the
+per-element viscosity as control, and `NSResidualVJP` exposes it to torch
+autograd.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .assembly import AssemblyMap, make_assembly_map, make_boundary_map, replace_res, replace_rows
+from .linalg import CsrMatrix, _check_dev_vec, _require, ptr, stream_handle
+
+TH_LANES = 64
+
+
+class THGeometry:
+    """Structured or user-supplied Taylor-Hood mesh data (device copies)."""
+
+    def __init__(self, xv, yv, vconn3, conn6, dofs15, num_vel_nodes, device="cuda"):
+        self.device = torch.device(device)
+        self.h_xv = np.ascontiguousarray(xv, np.float64)
+        self.h_yv = np.ascontiguousarray(yv, np.float64)
+        self.h_vconn3 = np.ascontiguousarray(vconn3, np.int32).reshape(-1, 3)
+        self.h_conn6 = np.ascontiguousarray(conn6, np.int32).reshape(-1, 6)
+        self.h_dofs15 = np.ascontiguousarray(dofs15, np.int32).reshape(-1, 15)
+        self.num_elems = self.h_dofs15.shape[0]
+        self.num_vertices = self.h_xv.size
+        self.num_vel_nodes = int(num_vel_nodes)
+        self.ndof = 2 * self.num_vel_nodes + self.num_vertices
+        _require(self.h_vconn3.shape[0] == self.num_elems == self.h_conn6.shape[0],
+                 "Taylor-Hood tables have different element counts")
+        t = lambda a: torch.from_numpy(a.reshape(-1)).to(self.device)  # noqa: E731
+        self.xv, self.yv = t(self.h_xv), t(self.h_yv)
+        self.vconn3, self.dofs15 = t(self.h_vconn3), t(self.h_dofs15)
+
+
+def structured_th(n, device="cuda"):
+    """Taylor-Hood mesh of [0,1]^2 with n x n cells (fx_mesh_p2, fx_th_dofs)."""
+    Nv, E = (2 * n + 1) ** 2, 2 * n * n
+    x = np.empty(Nv)
+    y = np.empty(Nv)
+    c6 = np.empty(6 * E, np.int32)
+    v3 = np.empty(3 * E, np.int32)
+    _lib.call("fx_mesh_p2", n, ptr(x), ptr(y), ptr(c6), ptr(v3))
+    d = np.empty(15 * E, np.int32)
+    _lib.call("fx_th_dofs", n, ptr(c6), ptr(v3), ptr(d))
+    # vertex coordinates: lattice nodes (2i, 2j) of the P2 lattice
+    m = 2 * n + 1
+    idx = (np.arange(n + 1)[:, None] * 2 * m + np.arange(n + 1)[None, :] * 2).reshape(-1)
+    geom = THGeometry(x[idx], y[idx], v3, c6, d, Nv, device)
+    geom.h_x2, geom.h_y2 = x, y  # P2 node coordinates (for boundary marking, states)
+    geom.n = n
+    return geom
+
+
+class THPlan:
+    """Row-incidence tables and work blocks of the Taylor-Hood kernels for one
+    assembly map, on one device."""
+
+    def __init__(self, amap: AssemblyMap, device="cuda"):
+        dev = torch.device(device)
+        info = amap.info
+        off = np.empty(amap.num_res + 1, np.int32)
+        elem = np.empty(max(info.num_inc, 1), np.int32)
+        meta = np.empty(max(4 * info.num_inc, 4), np.int32)
+        _lib.call("fx_map_export_rowinc15", amap._h, ptr(off), ptr(elem), ptr(meta))
+        nb, mx = C.c_int32(), C.c_int32()
+        _lib.call("fx_map_th_blocks", amap._h, TH_LANES, C.byref(nb), C.byref(mx), None, None)
+        blk_row = np.empty(nb.value + 1, np.int32)
+        blk_flags = np.empty(max(nb.value, 1), np.int32)
+        _lib.call("fx_map_th_blocks", amap._h, TH_LANES, C.byref(nb), C.byref(mx), ptr(blk_row),
+                  ptr(blk_flags))
+        self.num_blocks, self.max_block_nnz = nb.value, mx.value
+        self.h_blk_row, self.h_blk_flags = blk_row, blk_flags[: nb.value]
+        t = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
+        self.inc_off, self.inc_elem = t(off), t(elem[: info.num_inc])
+        self.inc_meta = t(meta[: 4 * info.num_inc])
+        self.blk_row, self.blk_flags = t(blk_row), t(self.h_blk_flags)
+
+
+class TaylorHoodNS:
+    """Steady incompressible NS operator: viscosity nu (scalar) or nu_e[E]."""
+
+    def __init__(self, nu, nu_e=None):
+        self.nu, self.nu_e = float(nu), nu_e
+
+
+def _plan(amap, device):
+    key = "_th_plan_" + str(torch.device(device))
+    p = getattr(amap, key, None)
+    if p is None:
+        p = THPlan(amap, device)
+        setattr(amap, key, p)
+    return p
+
+
+def _check(geom: THGeometry, amap: AssemblyMap):
+    if getattr(amap, "_th_checked", None) is geom:
+        return
+    _require(geom.num_elems == amap.num_elems,
+             "Geometry and AssemblyMap have different element counts")
+    _require(amap.num_res == geom.ndof and amap.num_states == geom.ndof
+             and amap.max_res == 15 and amap.max_state == 15,
+             "TaylorHoodNS needs a 15x15 map over the Taylor-Hood DOFs")
+    _require(np.array_equal(amap.res_dofs, geom.h_dofs15.reshape(-1))
+             and np.array_equal(amap.state_dofs, geom.h_dofs15.reshape(-1)),
+             "TaylorHoodNS needs element DOFs equal to the geometry's DOF table")
+    amap._th_checked = geom
+
+
+def assemble_ns(op: TaylorHoodNS, geom: THGeometry, amap: AssemblyMap, state, res,
+                jac: CsrMatrix = None, stream=None):
+    """Residual into `res` and, when `jac` is given, the Newton Jacobian into
+    `jac.vals`; outputs replaced.  Returns `res`."""
+    _check(geom, amap)
+    _check_dev_vec(state, geom.ndof, torch.float64, "Assembly state")
+    _check_dev_vec(res, geom.ndof, torch.float64, "Assembly residual")
+    if jac is not None:
+        _require(jac.pattern.layout_id == amap.pattern.layout_id,
+                 "Assembly matrix must use the AssemblyMap CSR layout")
+        _require(len({state.data_ptr(), res.data_ptr(), jac.vals.data_ptr()}) == 3,
+                 "Assembly state, residual, and matrix values must not alias")
+    if op.nu_e is not None:
+        _check_dev_vec(op.nu_e, geom.num_elems, torch.float64, "Per-element viscosity")
+    p = _plan(amap, geom.device)
+    rp, _ = amap.pattern.device_arrays(geom.device)
+    _lib.call("fx_th_assemble", geom.ndof, geom.num_elems, p.num_blocks, p.max_block_nnz,
+              ptr(geom.xv), ptr(geom.yv), ptr(geom.vconn3), ptr(geom.dofs15), ptr(state), op.nu,
+              ptr(op.nu_e), ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row),
+              ptr(p.blk_flags), ptr(rp), ptr(jac.vals) if jac is not None else None, ptr(res),
+              stream_handle(stream))
+    return res
+
+
+def ns_vjp(op: TaylorHoodNS, geom: THGeometry, amap: AssemblyMap, state, lam, bc_mask=None,
+           jt_out=None, g_out=None, stream=None):
+    """(J^T lam, per-element dR/dnu_e^T lam) over the row-replaced Jacobian."""
+    _check(geom, amap)
+    dev = geom.device
+    _check_dev_vec(state, geom.ndof, torch.float64, "VJP state")
+    _check_dev_vec(lam, geom.ndof, torch.float64, "VJP adjoint")
+    if jt_out is None:
+        jt_out = torch.empty(geom.ndof, dtype=torch.float64, device=dev)
+    if g_out is None:
+        g_out = torch.empty(geom.num_elems, dtype=torch.float64, device=dev)
+    p = _plan(amap, dev)
+    _lib.call("fx_th_vjp", geom.ndof, geom.num_elems, p.num_blocks, ptr(geom.xv), ptr(geom.yv),
+              ptr(geom.vconn3), ptr(geom.dofs15), ptr(state), op.nu, ptr(op.nu_e), ptr(lam),
+              ptr(bc_mask), ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row),
+              ptr(p.blk_flags), ptr(jt_out), ptr(g_out), stream_handle(stream))
+    return jt_out, g_out
+
+
+def cavity_bc(geom: THGeometry, lid=1.0):
+    """Lid-driven cavity velocity Dirichlet DOFs in ascending node order
+    (DirichletBC::addBoundary, femx/fem/DirichletBC.cpp:138-145): both
+    components on every boundary P2 node; u_x = lid on y = 1 (corners
+    included), 0 elsewhere."""
+    x, y = geom.h_x2, geom.h_y2
+    nodes = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0]
+    dofs = np.stack([2 * nodes, 2 * nodes + 1], axis=1).reshape(-1).astype(np.int32)
+    vals = np.zeros(dofs.size)
+    vals[0::2] = np.where(y[nodes] == 1.0, lid, 0.0)
+    return dofs, vals
+
+
+def fourier_state(geom: THGeometry, seed, lid=1.0):
+    """Config-3 state: each of u_x, u_y, p = sum over the 16 modes
+    (kx, ky) in {1..4}^2 of a_k/|k|^2 sin(pi kx x) sin(pi ky y), a_k ~ N(0,1)
+    (numpy Philox seeded with `seed`, drawn in the order u_x, u_y, p), then
+    u_x = lid on y = 1."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    kx, ky = np.meshgrid(np.arange(1, 5), np.arange(1, 5), indexing="ij")
+    kx, ky = kx.reshape(-1), ky.reshape(-1)
+    amp = rng.standard_normal((3, 16)) / (kx ** 2 + ky ** 2)
+
+    def field(a, x, y):
+        return (a[None, :] * np.sin(np.pi * kx[None, :] * x[:, None])
+                * np.sin(np.pi * ky[None, :] * y[:, None])).sum(axis=1)
+
+    x2, y2 = geom.h_x2, geom.h_y2
+    s = np.empty(geom.ndof)
+    s[0:2 * geom.num_vel_nodes:2] = field(amp[0], x2, y2)
+    s[1:2 * geom.num_vel_nodes:2] = field(amp[1], x2, y2)
+    s[2 * geom.num_vel_nodes:] = field(amp[2], geom.h_xv, geom.h_yv)
+    top = np.nonzero(y2 == 1.0)[0]
+    s[2 * top] = lid
+    return s
+
+
+class NavierStokesResidual:
+    """Residual contract for steady NS with Dirichlet rows replaced and the
+    per-element viscosity nu_e as the control."""
+
+    def __init__(self, geom: THGeometry, bc_dofs=None, bc_vals=None, amap=None):
+        self.geom = geom
+        self.map = amap if amap is not None else make_assembly_map(geom.ndof, geom.ndof,
+                                                                   geom.h_dofs15)
+        self.num_states = self.num_res = geom.ndof
+        self.num_params = geom.num_elems
+        self.bmap = (make_boundary_map(bc_dofs, self.map.pattern, geom.device)
+                     if bc_dofs is not None else None)
+        if self.bmap is not None:
+            self.bc_vals = (torch.zeros(self.bmap.num_bcs, dtype=torch.float64, device=geom.device)
+                            if bc_vals is None else bc_vals)
+
+    def res(self, state, nu_e, out, stream=None):
+        assemble_ns(TaylorHoodNS(0.0, nu_e), self.geom, self.map, state, out, None, stream)
+        if self.bmap is not None:
+            replace_res(self.bmap, state, self.bc_vals, out, stream)
+        return out
+
+    def assemble_state_jac(self, state, nu_e, out: CsrMatrix, stream=None):
+        tmp = torch.empty(self.num_res, dtype=torch.float64, device=self.geom.device)
+        assemble_ns(TaylorHoodNS(0.0, nu_e), self.geom, self.map, state, tmp, out, stream)
+        if self.bmap is not None:
+            replace_rows(self.bmap, out, 1.0, stream)
+        return out
+
+    def vjp(self, state, nu_e, adj, jt_out=None, g_out=None, stream=None):
+        mask = self.bmap.d_mask if self.bmap is not None else None
+        return ns_vjp(TaylorHoodNS(0.0, nu_e), self.geom, self.map, state, adj, mask, jt_out,
+                      g_out, stream)
+
+    def apply_param_jac_t(self, state, nu_e, adj, out=None, stream=None):
+        return self.vjp(state, nu_e, adj, g_out=out, stream=stream)[1]
+
+
+class NSResidualVJP(torch.autograd.Function):
+    """R = residual(u, nu_e) with backward lam -> (J^T lam, (dR/dnu_e)^T lam)."""
+
+    @staticmethod
+    def forward(ctx, u, nu_e, residual):
+        u, nu_e = u.contiguous(), nu_e.contiguous()
+        out = torch.empty(residual.num_res, dtype=torch.float64, device=u.device)
+        residual.res(u, nu_e, out)
+        ctx.residual = residual
+        ctx.save_for_backward(u, nu_e)
+        return out
+
+    @staticmethod
+    def backward(ctx, lam):
+        u, nu_e = ctx.saved_tensors
+        jt, g = ctx.residual.vjp(u, nu_e, lam.contiguous())
+        return jt, g, None

@@ -94,6 +94,21 @@ int fx_map_export_jac(const fx_map* map, fx_index* h_jac_offsets, fx_index* h_ja
  * element's three columns.  FX_ERR_UNSUPPORTED for other shapes or rows
  * longer than 255. */
 int fx_map_export_rowinc3(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc);
+/* Row incidence for elements with 15 local rows and columns (Taylor-Hood):
+ * inc_off[num_res+1], inc_elem[num_inc] and inc_meta[4*num_inc] -- 16 bytes
+ * per incidence: the positions (within the row, from row_ptr[row]) of the
+ * element's 15 columns, then the local row.  Rows in ascending element
+ * order.  FX_ERR_UNSUPPORTED for other shapes or rows longer than 255. */
+int fx_map_export_rowinc15(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc_elem,
+                           fx_index* h_inc_meta);
+/* Work blocks of the Taylor-Hood kernels: rows cut into contiguous blocks of
+ * at most `lanes` work items, an item being a pair of interchangeable
+ * velocity rows (r, r+1) -- same elements, local rows li and li+1 (li even,
+ * < 12), same column positions -- or a single row.  blk_row[num_blocks+1],
+ * blk_flags[num_blocks] (1 = pairs); max_block_nnz sizes the LDS row tile.
+ * Pass null arrays to query num_blocks and max_block_nnz. */
+int fx_map_th_blocks(const fx_map* map, fx_index lanes, fx_index* num_blocks,
+                     fx_index* max_block_nnz, fx_index* h_blk_row, fx_index* h_blk_flags);
 void fx_map_destroy(fx_map* map);
 
 /* CSR transpose permutation (host), for the deterministic transpose apply:
@@ -210,6 +225,36 @@ int fx_p1_vjp(fx_index num_nodes, fx_index num_elems, const double* d_x, const d
               const fx_index* d_conn, const double* d_m, const double* d_u,
               const double* d_lam, const fx_index* d_bc_mask, const fx_index* d_inc_off,
               const fx_index* d_inc, double* d_jt_lam, double* d_g, void* stream);
+
+/* ======================================================================== */
+/* Taylor-Hood P2/P1 steady incompressible Navier-Stokes (device).  Not in
+ * femx (its NS operator is GLS P1/P1, femx/model/ns/Components.hpp:400-577);
+ * drops in for assembly::assemble with an element operator
+ * (femx/assembly/CudaAssembly.hpp:319-364).  Galerkin residual and exact
+ * Newton Jacobian on the 7-point degree-5 rule, viscosity nu (or nu_e[E] per
+ * element when non-null), vertex coordinates xv/yv, vconn3[3E] vertex ids,
+ * dofs15[15E] element DOFs (fx_th_dofs order), tables from
+ * fx_map_export_rowinc15 and fx_map_th_blocks (64 lanes).  vals may be NULL
+ * (residual only).  Sums over elements in ascending element order, no
+ * atomics; arithmetic uses fused multiply-adds. */
+int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks,
+                   fx_index max_block_nnz, const double* d_xv, const double* d_yv,
+                   const fx_index* d_vconn3, const fx_index* d_dofs15, const double* d_state,
+                   double nu, const double* d_nu_e, const fx_index* d_inc_off,
+                   const fx_index* d_inc_elem, const fx_index* d_inc_meta,
+                   const fx_index* d_blk_row, const fx_index* d_blk_flags,
+                   const fx_index* d_row_ptr, double* d_vals, double* d_res, void* stream);
+/* Navier-Stokes adjoint VJP: jt = J^T lam over the row-replaced Newton
+ * Jacobian (lam zeroed on constrained rows, passed through there; bc_mask may
+ * be NULL) and, when g is non-null, g[e] = lam~_e^T dR_e/dnu_e, the
+ * per-element viscosity sensitivity.  Same tables as fx_th_assemble; the
+ * contract of Residual::applyParamJacT (femx/state/Residual.hpp:55-59). */
+int fx_th_vjp(fx_index ndof, fx_index num_elems, fx_index num_blocks, const double* d_xv,
+              const double* d_yv, const fx_index* d_vconn3, const fx_index* d_dofs15,
+              const double* d_state, double nu, const double* d_nu_e, const double* d_lam,
+              const fx_index* d_bc_mask, const fx_index* d_inc_off, const fx_index* d_inc_elem,
+              const fx_index* d_inc_meta, const fx_index* d_blk_row, const fx_index* d_blk_flags,
+              double* d_jt, double* d_g, void* stream);
 
 #ifdef __cplusplus
 }

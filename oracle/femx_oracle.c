@@ -688,8 +688,8 @@ static void th_gather(const double* xv, const double* yv, const or_index* v3,
 /* assembly::assemble host driver order (femx/assembly/Assembly.hpp:184-225). */
 void or_th_assemble(const double* xv, const double* yv, or_index num_elems,
                     const or_index* vconn3, const or_index* dofs15, or_index ndof,
-                    double nu, const double* state, const or_index* jac_map, or_index nnz,
-                    double* vals, double* res)
+                    double nu, const double* nu_e, const double* state, const or_index* jac_map,
+                    or_index nnz, double* vals, double* res)
 {
   memset(res, 0, sizeof(double) * ndof);
   memset(vals, 0, sizeof(double) * nnz);
@@ -697,7 +697,7 @@ void or_th_assemble(const double* xv, const double* yv, or_index num_elems,
   {
     double xe[3], ye[3], se[15], Ke[225], Re[15];
     th_gather(xv, yv, vconn3 + 3 * e, dofs15 + 15 * e, state, xe, ye, se);
-    or_th_element(xe, ye, nu, se, Ke, Re);
+    or_th_element(xe, ye, nu_e ? nu_e[e] : nu, se, Ke, Re);
     for (int r = 0; r < 15; ++r) res[dofs15[15 * e + r]] += Re[r];
     for (int i = 0; i < 225; ++i) vals[jac_map[225 * (int64_t)e + i]] += Ke[i];
   }
@@ -705,8 +705,8 @@ void or_th_assemble(const double* xv, const double* yv, or_index num_elems,
 
 void or_th_assemble_omp(const double* xv, const double* yv, or_index num_elems,
                         const or_index* vconn3, const or_index* dofs15, or_index ndof,
-                        double nu, const double* state, const or_index* jac_map,
-                        or_index nnz, double* vals, double* res)
+                        double nu, const double* nu_e, const double* state,
+                        const or_index* jac_map, or_index nnz, double* vals, double* res)
 {
 #pragma omp parallel for schedule(static)
   for (or_index i = 0; i < ndof; ++i) res[i] = 0.0;
@@ -719,7 +719,7 @@ void or_th_assemble_omp(const double* xv, const double* yv, or_index num_elems,
     for (or_index e = 0; e < num_elems; ++e)
     {
       th_gather(xv, yv, vconn3 + 3 * e, dofs15 + 15 * e, state, xe, ye, se);
-      or_th_element(xe, ye, nu, se, Ke, Re);
+      or_th_element(xe, ye, nu_e ? nu_e[e] : nu, se, Ke, Re);
       for (int r = 0; r < 15; ++r)
       {
 #pragma omp atomic update
@@ -761,19 +761,19 @@ static void th_visc_res(const double* xv, const double* yv, const double* se, do
 
 /* Navier-Stokes adjoint VJP (not in femx): J^T lam~ + mask*lam over the
  * row-replaced Newton Jacobian (same boundary convention as or_p1_vjp), and
- * dnu = lam~^T dR/dnu, the viscous part of the residual. */
+ * g[e] = lam~_e^T dR_e/dnu_e, the viscous part of the element residual
+ * (nu_e[e] per element when nu_e is non-null, else nu). */
 void or_th_vjp(const double* xv, const double* yv, or_index num_elems,
                const or_index* vconn3, const or_index* dofs15, or_index ndof, double nu,
-               const double* state, const double* lam, const or_index* bc_mask,
-               double* jt_lam, double* dnu)
+               const double* nu_e, const double* state, const double* lam,
+               const or_index* bc_mask, double* jt_lam, double* g)
 {
   memset(jt_lam, 0, sizeof(double) * ndof);
-  double acc = 0.0;
   for (or_index e = 0; e < num_elems; ++e)
   {
     double xe[3], ye[3], se[15], Ke[225], Re[15], Rv[15], le[15];
     th_gather(xv, yv, vconn3 + 3 * e, dofs15 + 15 * e, state, xe, ye, se);
-    or_th_element(xe, ye, nu, se, Ke, Re);
+    or_th_element(xe, ye, nu_e ? nu_e[e] : nu, se, Ke, Re);
     th_visc_res(xe, ye, se, Rv);
     for (int r = 0; r < 15; ++r)
     {
@@ -782,7 +782,7 @@ void or_th_vjp(const double* xv, const double* yv, or_index num_elems,
     }
     double ge = 0.0;
     for (int r = 0; r < 15; ++r) ge += le[r] * Rv[r];
-    acc += ge;
+    g[e] = ge;
     for (int c = 0; c < 15; ++c)
     {
       double v = 0.0;
@@ -793,5 +793,4 @@ void or_th_vjp(const double* xv, const double* yv, or_index num_elems,
   if (bc_mask)
     for (or_index i = 0; i < ndof; ++i)
       if (bc_mask[i]) jt_lam[i] += lam[i];
-  *dnu = acc;
 }

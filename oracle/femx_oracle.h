@@ -92,16 +92,16 @@ void or_th_element(const double* xv, const double* yv, double nu, const double* 
                    double* Ke, double* Re);
 void or_th_assemble(const double* xv, const double* yv, or_index num_elems,
                     const or_index* vconn3, const or_index* dofs15, or_index ndof,
-                    double nu, const double* state, const or_index* jac_map, or_index nnz,
-                    double* vals, double* res);
+                    double nu, const double* nu_e, const double* state, const or_index* jac_map,
+                    or_index nnz, double* vals, double* res);
 void or_th_assemble_omp(const double* xv, const double* yv, or_index num_elems,
                         const or_index* vconn3, const or_index* dofs15, or_index ndof,
-                        double nu, const double* state, const or_index* jac_map,
-                        or_index nnz, double* vals, double* res);
+                        double nu, const double* nu_e, const double* state,
+                        const or_index* jac_map, or_index nnz, double* vals, double* res);
 void or_th_vjp(const double* xv, const double* yv, or_index num_elems,
                const or_index* vconn3, const or_index* dofs15, or_index ndof, double nu,
-               const double* state, const double* lam, const or_index* bc_mask,
-               double* jt_lam, double* dnu);
+               const double* nu_e, const double* state, const double* lam,
+               const or_index* bc_mask, double* jt_lam, double* g);
 
 int or_omp_threads(void);
 

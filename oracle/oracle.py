@@ -243,23 +243,24 @@ def th_element(xe, ye, nu, se):
     return Ke.reshape(15, 15), Re
 
 
-def th_assemble(xv, yv, vconn3, dofs15, ndof, nu, state, jac_map, nnz, omp=False):
+def th_assemble(xv, yv, vconn3, dofs15, ndof, nu, state, jac_map, nnz, omp=False, nu_e=None):
     E = vconn3.size // 3
     vals = np.empty(nnz, F64)
     res = np.empty(ndof, F64)
     fn = lib().or_th_assemble_omp if omp else lib().or_th_assemble
     fn(_p(xv), _p(yv), C.c_int32(E), _p(vconn3), _p(dofs15), C.c_int32(ndof), C.c_double(nu),
-       _p(state), _p(jac_map), C.c_int32(nnz), _p(vals), _p(res))
+       _p(nu_e), _p(state), _p(jac_map), C.c_int32(nnz), _p(vals), _p(res))
     return vals, res
 
 
-def th_vjp(xv, yv, vconn3, dofs15, ndof, nu, state, lam, bc_mask=None):
+def th_vjp(xv, yv, vconn3, dofs15, ndof, nu, state, lam, bc_mask=None, nu_e=None):
+    """(J^T lam, g[E]) with g_e = lam~_e^T dR_e/dnu_e."""
     E = vconn3.size // 3
     jt = np.empty(ndof, F64)
-    dnu = C.c_double()
+    g = np.empty(E, F64)
     lib().or_th_vjp(_p(xv), _p(yv), C.c_int32(E), _p(vconn3), _p(dofs15), C.c_int32(ndof),
-                    C.c_double(nu), _p(state), _p(lam), _p(bc_mask), _p(jt), C.byref(dnu))
-    return jt, dnu.value
+                    C.c_double(nu), _p(nu_e), _p(state), _p(lam), _p(bc_mask), _p(jt), _p(g))
+    return jt, g
 
 
 def omp_threads():

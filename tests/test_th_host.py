@@ -1,0 +1,77 @@
+"""Taylor-Hood host tables on the CPU: mesh/DOF conventions, the 15x15
+row-incidence export and the work-block plan."""
+import numpy as np
+import pytest
+
+from femx_amd import ns
+from femx_amd.assembly import make_assembly_map
+from oracle import oracle as O
+
+
+@pytest.mark.parametrize("n", [1, 2, 5])
+def test_structured_th_tables(n):
+    g = ns.structured_th(n, "cpu")
+    x2, y2, c6, v3 = O.mesh_p2(n)
+    assert np.array_equal(g.h_conn6.reshape(-1), c6) and np.array_equal(g.h_vconn3.reshape(-1), v3)
+    assert np.array_equal(g.h_dofs15.reshape(-1), O.th_dofs(n, c6, v3))
+    x1, y1, _ = O.mesh_p1(n)
+    assert np.array_equal(g.h_xv, x1) and np.array_equal(g.h_yv, y1)
+    # vertex k of each element sits at P2 node k of that element
+    c6 = c6.reshape(-1, 6)
+    v3 = v3.reshape(-1, 3)
+    assert np.array_equal(x2[c6[:, :3]], x1[v3]) and np.array_equal(y2[c6[:, :3]], y1[v3])
+    assert g.ndof == 2 * (2 * n + 1) ** 2 + (n + 1) ** 2
+
+
+@pytest.mark.parametrize("n", [1, 3])
+def test_rowinc15_and_blocks(n):
+    g = ns.structured_th(n, "cpu")
+    amap = make_assembly_map(g.ndof, g.ndof, g.h_dofs15)
+    plan = ns.THPlan(amap, "cpu")
+    off = plan.inc_off.numpy()
+    elem = plan.inc_elem.numpy()
+    meta = plan.inc_meta.numpy().view(np.uint8).reshape(-1, 16)
+    jo, jm = amap.jac()
+    rp = amap.pattern.row_ptr
+    d = g.h_dofs15
+    for r in range(g.ndof):
+        es = elem[off[r]:off[r + 1]]
+        assert np.all(np.diff(es) > 0)
+        for t in range(off[r], off[r + 1]):
+            e, li = elem[t], meta[t, 15]
+            assert d[e, li] == r
+            assert np.array_equal(rp[r] + meta[t, :15].astype(np.int64), jm[225 * e + 15 * li:
+                                                                           225 * e + 15 * li + 15])
+    br, bf = plan.h_blk_row, plan.h_blk_flags
+    assert br[0] == 0 and br[-1] == g.ndof and np.all(np.diff(br) > 0)
+    nv2 = 2 * g.num_vel_nodes
+    for b in range(plan.num_blocks):
+        items = (br[b + 1] - br[b]) // (2 if bf[b] else 1)
+        assert items <= ns.TH_LANES
+        if bf[b]:
+            assert br[b + 1] <= nv2 and br[b] % 2 == 0
+        assert rp[br[b + 1]] - rp[br[b]] <= plan.max_block_nnz
+    # every velocity row is paired, every pressure row single
+    paired = np.zeros(g.ndof, bool)
+    for b in range(plan.num_blocks):
+        paired[br[b]:br[b + 1]] = bool(bf[b])
+    assert paired[:nv2].all() and not paired[nv2:].any()
+
+
+def test_rowinc15_rejects_other_shapes():
+    amap = make_assembly_map(3, 3, [[0, 1, 2]])
+    with pytest.raises(RuntimeError, match="15x15"):
+        ns.THPlan(amap, "cpu")
+
+
+def test_fourier_state_and_cavity_bc():
+    g = ns.structured_th(4, "cpu")
+    s = ns.fourier_state(g, 5)
+    assert s.shape == (g.ndof,)
+    top = np.nonzero(g.h_y2 == 1.0)[0]
+    assert np.all(s[2 * top] == 1.0)
+    s2 = ns.fourier_state(g, 5)
+    assert np.array_equal(s, s2)
+    dofs, vals = ns.cavity_bc(g)
+    assert np.all(np.diff(dofs) > 0) and dofs.size == 2 * 4 * 8
+    assert vals.sum() == (2 * 4 + 1)
