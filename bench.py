@@ -44,8 +44,44 @@ def vjp_alg_bytes(N, E):
     return 2 * N * 8 + 3 * E * 4 + E * 8 + N * 8 + N * 8 + 3 * E * 4 + (N + 1) * 4 + N * 8 + E * 8
 
 
+def th_alg_bytes(Np, E, ndof, nnz):
+    """vertex coords, element DOF table, state, femx's jac_map, CSR values, residual."""
+    return 2 * Np * 8 + 15 * E * 4 + ndof * 8 + 225 * E * 4 + nnz * 8 + ndof * 8
+
+
+def th_vjp_alg_bytes(Np, E, ndof):
+    """vertex coords, DOF table, state, adjoint, incidence + offsets, J^T lam, g."""
+    return 2 * Np * 8 + 15 * E * 4 + 2 * ndof * 8 + 15 * E * 4 + (ndof + 1) * 4 + ndof * 8 + E * 8
+
+
 def spmv_alg_bytes(N, nnz):
     return nnz * 8 + nnz * 4 + (N + 1) * 4 + N * 8 + N * 8
+
+
+def build_th(n, dev, t):
+    """Config-3 workload: Taylor-Hood n x n lid-driven cavity, Re = 100,
+    Fourier state (seed 5) with lid u = 1; returns launch closures."""
+    import torch
+    from femx_amd import ns
+    from femx_amd.assembly import make_assembly_map
+    from femx_amd.linalg import CsrMatrix
+    g = ns.structured_th(n, dev)
+    amap = make_assembly_map(g.ndof, g.ndof, g.h_dofs15)
+    s = t(ns.fourier_state(g, 5))
+    lam = t(np.random.default_rng(7).standard_normal(g.ndof))
+    dofs, vals = ns.cavity_bc(g)
+    R = ns.NavierStokesResidual(g, dofs, t(vals), amap=amap)
+    jac = CsrMatrix(amap.pattern, dev)
+    res = torch.empty(g.ndof, dtype=torch.float64, device=dev)
+    jt = torch.empty(g.ndof, dtype=torch.float64, device=dev)
+    ge = torch.empty(g.num_elems, dtype=torch.float64, device=dev)
+    nu_e = torch.full((g.num_elems,), 0.01, dtype=torch.float64, device=dev)
+    op = ns.TaylorHoodNS(0.01)
+    return {"assemble": lambda: ns.assemble_ns(op, g, amap, s, res, jac),
+            "vjp": lambda: R.vjp(s, nu_e, lam, jt, ge),
+            "E": g.num_elems, "ndof": g.ndof, "nnz": amap.pattern.nnz,
+            "alg_bytes": th_alg_bytes(g.num_vertices, g.num_elems, g.ndof, amap.pattern.nnz),
+            "vjp_alg_bytes": th_vjp_alg_bytes(g.num_vertices, g.num_elems, g.ndof)}
 
 
 def log(msg):
@@ -60,6 +96,8 @@ def main():
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-vjp", action="store_true")
+    ap.add_argument("--no-th", action="store_true")
+    ap.add_argument("--th-n", type=int, default=512)
     ap.add_argument("--kernels-only", action="store_true",
                     help="for rocprofv3 counter runs: a 1 GiB device copy (counter "
                          "calibration), then 5 launches each of assembly, SpMV, elimination "
@@ -140,6 +178,13 @@ def main():
         for _ in range(5):
             Rv.vjp(uv, m, lam)
         torch.cuda.synchronize()
+        if not args.no_th:
+            th = build_th(args.th_n, dev, t)
+            for _ in range(5):
+                th["assemble"]()
+            for _ in range(5):
+                th["vjp"]()
+            torch.cuda.synchronize()
         log("[bench] kernels-only done")
         return
 
@@ -203,6 +248,19 @@ def main():
                                 "elements_per_s": gv.num_elems / t_vjp, "alg_bytes": b_vjp,
                                 "GBs": b_vjp / t_vjp / 1e9,
                                 "frac": b_vjp / t_vjp / 1e9 / HBM_PEAK_GBS}
+
+    if not args.no_th:
+        log(f"[bench] Taylor-Hood {args.th_n}x{args.th_n}")
+        th = build_th(args.th_n, dev, t)
+        t_th = time_kernel(th["assemble"], reps)
+        t_thv = time_kernel(th["vjp"], reps)
+        b_th, b_thv = th["alg_bytes"], th["vjp_alg_bytes"]
+        extra["th_ns_%d" % args.th_n] = {
+            "elements": th["E"], "ndof": th["ndof"], "nnz": th["nnz"],
+            "assemble_us": t_th * 1e6, "elements_per_s": th["E"] / t_th, "alg_bytes": b_th,
+            "GBs": b_th / t_th / 1e9, "frac": b_th / t_th / 1e9 / HBM_PEAK_GBS,
+            "vjp_us": t_thv * 1e6, "VJPs_per_s": 1.0 / t_thv, "vjp_alg_bytes": b_thv,
+            "vjp_GBs": b_thv / t_thv / 1e9, "vjp_frac": b_thv / t_thv / 1e9 / HBM_PEAK_GBS}
 
     cpu = None
     if not args.no_cpu:

@@ -61,7 +61,23 @@ def build(verbose=False):
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)
+    build_cpp_tests()
     return OUT
+
+
+CPP_TEST = os.path.join(OBJ, "test_femx_hip")
+
+
+def build_cpp_tests():
+    """The C++ interface tests (tests/cpp), linked against the library."""
+    src = os.path.join(ROOT, "tests", "cpp", "test_femx_hip.cpp")
+    cmd = [hipcc(), src, "-o", CPP_TEST, "-O2", "-std=c++17", "-I" + os.path.join(HERE, "include"),
+           "-I" + os.path.join(ROOT, "include"), "-L" + HERE, "-lfemx_amd",
+           "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ test build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return CPP_TEST
 
 
 if __name__ == "__main__":

@@ -1,0 +1,593 @@
+// femx_hip.hpp -- femx-shaped C++ host interface over the femx_amd C-ABI.
+//
+// Header-only.  Mirrors the reference's host types and free functions for
+// the FE hot path so that a femx caller swaps the CUDA overloads for these:
+//
+//   femx::HipContext                 femx/common/Context.hpp:48-90 (CudaContext)
+//   femx::DeviceArray<T>             femx/linalg/Vector.hpp:322-400 (device Vector)
+//   femx::DeviceCsrPattern/Matrix    femx/linalg/CsrPattern.hpp:43-217, CsrMatrix.hpp:17-105
+//   assembly::HostAssemblyMap,       femx/assembly/AssemblyMap.hpp:75-246
+//     makeAssemblyMap, copy
+//   assembly::HostBoundaryMap,       femx/assembly/BoundaryMap.hpp:71-294
+//     makeBoundaryMap, copy, replaceRows, replaceRes, zeroBoundary,
+//     prepareForwardSolve
+//   assembly::assemble               femx/assembly/CudaAssembly.hpp:319-364
+//   linalg::HipMatrixHandler         femx/linalg/handler/MatrixHandler.hpp:193-378
+//   linalg::HipVectorHandler         femx/linalg/handler/VectorHandler.hpp:190-477
+//   state::Residual                  femx/state/Residual.hpp:18-60
+//
+// Errors surface as std::runtime_error carrying the library's message, as
+// femx's require() does (femx/common/Checks.hpp:11-17).  Device work is
+// enqueued on the context's stream and is asynchronous, like femx's CUDA
+// path; call HipContext::sync() before reading results on the host.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../../include/femx_amd.h"
+
+namespace femx
+{
+
+using Real  = double;
+using Index = std::int32_t;
+
+inline void check(int status)
+{
+  if (status != FX_OK) throw std::runtime_error(fx_last_error());
+}
+
+inline void checkHip(hipError_t st, const char* what)
+{
+  if (st != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(st));
+}
+
+inline void require(bool cond, const std::string& msg)
+{
+  if (!cond) throw std::runtime_error(msg);
+}
+
+/** Owns one HIP stream (default priority), like femx's CudaContext. */
+class HipContext
+{
+public:
+  HipContext() { checkHip(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate"); }
+  ~HipContext()
+  {
+    if (stream_) (void)hipStreamDestroy(stream_);
+  }
+  HipContext(const HipContext&)            = delete;
+  HipContext& operator=(const HipContext&) = delete;
+
+  static bool available()
+  {
+    int n = 0;
+    return hipGetDeviceCount(&n) == hipSuccess && n > 0;
+  }
+  hipStream_t stream() const noexcept { return stream_; }
+  void*       raw() const noexcept { return stream_; }
+  void        sync() const { checkHip(hipStreamSynchronize(stream_), "hipStreamSynchronize"); }
+
+private:
+  hipStream_t stream_ = nullptr;
+};
+
+/** Move-only device array (femx's Vector<Device, T>). */
+template <class T>
+class DeviceArray
+{
+public:
+  DeviceArray() = default;
+  explicit DeviceArray(Index n) { resize(n); }
+  DeviceArray(const std::vector<T>& host, HipContext& ctx) { assign(host, ctx); }
+  ~DeviceArray() { release(); }
+  DeviceArray(DeviceArray&& o) noexcept : data_(std::exchange(o.data_, nullptr)), size_(std::exchange(o.size_, 0)) {}
+  DeviceArray& operator=(DeviceArray&& o) noexcept
+  {
+    if (this != &o)
+    {
+      release();
+      data_ = std::exchange(o.data_, nullptr);
+      size_ = std::exchange(o.size_, 0);
+    }
+    return *this;
+  }
+  DeviceArray(const DeviceArray&)            = delete;
+  DeviceArray& operator=(const DeviceArray&) = delete;
+
+  void resize(Index n)
+  {
+    require(n >= 0, "DeviceArray size must be non-negative");
+    if (n == size_) return;
+    release();
+    if (n > 0) checkHip(hipMalloc(&data_, sizeof(T) * static_cast<std::size_t>(n)), "hipMalloc");
+    size_ = n;
+  }
+  void assign(const std::vector<T>& host, HipContext& ctx)
+  {
+    resize(static_cast<Index>(host.size()));
+    if (size_ > 0)
+      checkHip(hipMemcpyAsync(data_, host.data(), sizeof(T) * host.size(), hipMemcpyHostToDevice,
+                              ctx.stream()),
+               "hipMemcpyAsync");
+  }
+  std::vector<T> toHost(HipContext& ctx) const
+  {
+    std::vector<T> out(size_);
+    if (size_ > 0)
+      checkHip(hipMemcpyAsync(out.data(), data_, sizeof(T) * out.size(), hipMemcpyDeviceToHost,
+                              ctx.stream()),
+               "hipMemcpyAsync");
+    ctx.sync();
+    return out;
+  }
+  T*       data() noexcept { return data_; }
+  const T* data() const noexcept { return data_; }
+  Index    size() const noexcept { return size_; }
+
+private:
+  void release()
+  {
+    if (data_) (void)hipFree(data_);
+    data_ = nullptr;
+    size_ = 0;
+  }
+  T*    data_ = nullptr;
+  Index size_ = 0;
+};
+
+using DeviceVector      = DeviceArray<Real>;
+using DeviceIndexVector = DeviceArray<Index>;
+
+namespace detail
+{
+inline std::uint64_t newLayoutId()
+{
+  static std::atomic<std::uint64_t> next{1};
+  return next++;
+}
+} // namespace detail
+
+/** Immutable host CSR pattern with a layout identity. */
+struct HostCsrPattern
+{
+  Index              rows = 0, cols = 0;
+  std::vector<Index> row_ptr{0}, col_ind;
+  std::uint64_t      layout_id = detail::newLayoutId();
+  Index              nnz() const { return static_cast<Index>(col_ind.size()); }
+};
+
+/** Device CSR pattern sharing the host pattern's layout identity. */
+class DeviceCsrPattern
+{
+public:
+  DeviceCsrPattern() = default;
+  DeviceCsrPattern(const HostCsrPattern& h, HipContext& ctx)
+    : rows_(h.rows), cols_(h.cols), layout_id_(h.layout_id), host_(h)
+  {
+    row_ptr_.assign(h.row_ptr, ctx);
+    col_ind_.assign(h.col_ind, ctx);
+  }
+  Index                 rows() const noexcept { return rows_; }
+  Index                 cols() const noexcept { return cols_; }
+  Index                 nnz() const noexcept { return col_ind_.size(); }
+  std::uint64_t         layoutId() const noexcept { return layout_id_; }
+  const Index*          rowPtrData() const noexcept { return row_ptr_.data(); }
+  const Index*          colIndData() const noexcept { return col_ind_.data(); }
+  const HostCsrPattern& host() const noexcept { return host_; }
+
+  struct TransposeMap
+  {
+    DeviceIndexVector t_ptr, t_row, t_perm;
+  };
+
+  /** Cached transpose permutation (t_ptr, t_row, t_perm) on the device. */
+  const TransposeMap& transposeMap(HipContext& ctx) const
+  {
+    if (!tmap_)
+    {
+      std::vector<Index> tp(cols_ + 1), tr(nnz()), tq(nnz());
+      check(fx_csr_transpose_map(rows_, cols_, host_.row_ptr.data(), host_.col_ind.data(), tp.data(),
+                                 tr.data(), tq.data()));
+      auto m = std::make_shared<TransposeMap>();
+      m->t_ptr.assign(tp, ctx);
+      m->t_row.assign(tr, ctx);
+      m->t_perm.assign(tq, ctx);
+      tmap_ = std::move(m);
+    }
+    return *tmap_;
+  }
+
+private:
+  Index                                      rows_ = 0, cols_ = 0;
+  std::uint64_t                              layout_id_ = 0;
+  HostCsrPattern                             host_;
+  DeviceIndexVector                          row_ptr_, col_ind_;
+  mutable std::shared_ptr<TransposeMap>      tmap_;
+};
+
+/** Device CSR matrix: shared pattern, owned values. */
+class DeviceCsrMatrix
+{
+public:
+  DeviceCsrMatrix() = default;
+  explicit DeviceCsrMatrix(std::shared_ptr<const DeviceCsrPattern> p) : pattern_(std::move(p))
+  {
+    vals_.resize(pattern_->nnz());
+  }
+  const DeviceCsrPattern& pattern() const { return *pattern_; }
+  std::shared_ptr<const DeviceCsrPattern> sharedPattern() const { return pattern_; }
+  Index        rows() const { return pattern_->rows(); }
+  Index        cols() const { return pattern_->cols(); }
+  Index        nnz() const { return pattern_->nnz(); }
+  Real*        valsData() { return vals_.data(); }
+  const Real*  valsData() const { return vals_.data(); }
+  DeviceVector& vals() { return vals_; }
+
+private:
+  std::shared_ptr<const DeviceCsrPattern> pattern_;
+  DeviceVector                            vals_;
+};
+
+namespace assembly
+{
+
+/** Host assembly map built by fx_map_create (femx makeAssemblyMap). */
+class HostAssemblyMap
+{
+public:
+  HostAssemblyMap() = default;
+  HostAssemblyMap(Index num_res, Index num_states, const std::vector<std::vector<Index>>& res_dofs,
+                  const std::vector<std::vector<Index>>& state_dofs)
+  {
+    require(res_dofs.size() == state_dofs.size(), "AssemblyMap residual/state element counts differ");
+    std::vector<Index> ro{0}, rd, so{0}, sd;
+    for (std::size_t e = 0; e < res_dofs.size(); ++e)
+    {
+      rd.insert(rd.end(), res_dofs[e].begin(), res_dofs[e].end());
+      sd.insert(sd.end(), state_dofs[e].begin(), state_dofs[e].end());
+      ro.push_back(static_cast<Index>(rd.size()));
+      so.push_back(static_cast<Index>(sd.size()));
+    }
+    fx_map* raw = nullptr;
+    check(fx_map_create(num_res, num_states, static_cast<Index>(res_dofs.size()), ro.data(),
+                        rd.data(), so.data(), sd.data(), &raw));
+    h_.reset(raw, fx_map_destroy);
+    check(fx_map_info_get(raw, &info_));
+    auto p     = std::make_shared<HostCsrPattern>();
+    p->rows    = info_.num_res;
+    p->cols    = info_.num_states;
+    p->row_ptr.resize(info_.num_res + 1);
+    p->col_ind.resize(info_.nnz);
+    check(fx_map_export_pattern(raw, p->row_ptr.data(), p->col_ind.data()));
+    pattern_ = std::move(p);
+  }
+  Index                 numElems() const { return info_.num_elems; }
+  Index                 numRes() const { return info_.num_res; }
+  Index                 numStates() const { return info_.num_states; }
+  Index                 maxRes() const { return info_.max_res; }
+  Index                 maxState() const { return info_.max_state; }
+  Index                 maxJac() const { return info_.max_jac; }
+  const fx_map_info&    info() const { return info_; }
+  const HostCsrPattern& pattern() const { return *pattern_; }
+  const fx_map*         handle() const { return h_.get(); }
+
+  /** femx's flattened jac_offsets / jac_map. */
+  void jacMap(std::vector<Index>& offsets, std::vector<Index>& map) const
+  {
+    offsets.resize(info_.num_elems + 1);
+    map.resize(info_.jac_len);
+    check(fx_map_export_jac(h_.get(), offsets.data(), map.data()));
+  }
+
+private:
+  std::shared_ptr<fx_map>               h_;
+  fx_map_info                           info_{};
+  std::shared_ptr<const HostCsrPattern> pattern_;
+};
+
+inline HostAssemblyMap makeAssemblyMap(Index num_res, Index num_states,
+                                       const std::vector<std::vector<Index>>& res_dofs,
+                                       const std::vector<std::vector<Index>>& state_dofs)
+{
+  return HostAssemblyMap(num_res, num_states, res_dofs, state_dofs);
+}
+
+/** Device map: pattern plus the row-gather tables of the element kernels. */
+struct DeviceAssemblyMap
+{
+  fx_map_info                             info{};
+  std::shared_ptr<const DeviceCsrPattern> pattern;
+  DeviceIndexVector                       inc_off, inc, inc_elem, inc_meta, blk_row, blk_flags;
+  Index                                   num_blocks = 0, max_block_nnz = 0;
+};
+
+/** Host -> device copy (femx copy(map, map, ctx)); builds the row-incidence
+ * tables the element shape needs (3x3: P1, 15x15: Taylor-Hood). */
+inline void copy(const HostAssemblyMap& src, DeviceAssemblyMap& dst, HipContext& ctx)
+{
+  dst.info    = src.info();
+  dst.pattern = std::make_shared<DeviceCsrPattern>(src.pattern(), ctx);
+  std::vector<Index> off(src.numRes() + 1);
+  if (src.maxRes() == 3 && src.maxState() == 3)
+  {
+    std::vector<Index> inc(2 * static_cast<std::size_t>(src.info().num_inc));
+    check(fx_map_export_rowinc3(src.handle(), off.data(), inc.data()));
+    dst.inc_off.assign(off, ctx);
+    dst.inc.assign(inc, ctx);
+  }
+  else if (src.maxRes() == 15 && src.maxState() == 15)
+  {
+    std::vector<Index> el(src.info().num_inc), me(4 * static_cast<std::size_t>(src.info().num_inc));
+    check(fx_map_export_rowinc15(src.handle(), off.data(), el.data(), me.data()));
+    Index nb = 0, mx = 0;
+    check(fx_map_th_blocks(src.handle(), 64, &nb, &mx, nullptr, nullptr));
+    std::vector<Index> br(nb + 1), bf(nb);
+    check(fx_map_th_blocks(src.handle(), 64, &nb, &mx, br.data(), bf.data()));
+    dst.inc_off.assign(off, ctx);
+    dst.inc_elem.assign(el, ctx);
+    dst.inc_meta.assign(me, ctx);
+    dst.blk_row.assign(br, ctx);
+    dst.blk_flags.assign(bf, ctx);
+    dst.num_blocks    = nb;
+    dst.max_block_nnz = mx;
+  }
+}
+
+/** Host boundary map (femx makeBoundaryMap) with its device copy. */
+struct HostBoundaryMap
+{
+  Index              rows = 0;
+  std::uint64_t      layout_id = 0;
+  std::vector<Index> bc_rows, diag, col_offsets, col_entries, col_rows, bc_mask;
+  std::vector<Index> elim_rows, elim_off, elim_k, elim_ib;
+  Index              numBcs() const { return static_cast<Index>(bc_rows.size()); }
+};
+
+inline HostBoundaryMap makeBoundaryMap(const std::vector<Index>& dofs, const HostCsrPattern& pattern)
+{
+  require(pattern.rows == pattern.cols, "BoundaryMap requires a square CSR pattern");
+  fx_bcmap* raw = nullptr;
+  check(fx_bcmap_create(pattern.rows, pattern.row_ptr.data(), pattern.col_ind.data(),
+                        static_cast<Index>(dofs.size()), dofs.data(), &raw));
+  std::unique_ptr<fx_bcmap, void (*)(fx_bcmap*)> h(raw, fx_bcmap_destroy);
+  fx_bcmap_info info{};
+  check(fx_bcmap_info_get(raw, &info));
+  HostBoundaryMap m;
+  m.rows      = pattern.rows;
+  m.layout_id = pattern.layout_id;
+  m.bc_rows.resize(info.num_bcs);
+  m.diag.resize(info.num_bcs);
+  m.col_offsets.resize(info.num_bcs + 1);
+  m.col_entries.resize(info.num_col_entries);
+  m.col_rows.resize(info.num_col_entries);
+  m.bc_mask.resize(pattern.rows);
+  check(fx_bcmap_export(raw, m.bc_rows.data(), m.diag.data(), m.col_offsets.data(),
+                        m.col_entries.data(), m.col_rows.data(), m.bc_mask.data()));
+  m.elim_rows.resize(info.num_elim_rows);
+  m.elim_off.resize(info.num_elim_rows + 1);
+  m.elim_k.resize(info.num_elim_entries);
+  m.elim_ib.resize(info.num_elim_entries);
+  check(fx_bcmap_export_elim(raw, m.elim_rows.data(), m.elim_off.data(), m.elim_k.data(),
+                             m.elim_ib.data()));
+  return m;
+}
+
+struct DeviceBoundaryMap
+{
+  Index             rows = 0, num_bcs = 0, num_elim_rows = 0;
+  std::uint64_t     layout_id = 0;
+  DeviceIndexVector bc_rows, diag, bc_mask, elim_rows, elim_off, elim_k, elim_ib;
+};
+
+inline void copy(const HostBoundaryMap& src, DeviceBoundaryMap& dst, HipContext& ctx)
+{
+  dst.rows          = src.rows;
+  dst.num_bcs       = src.numBcs();
+  dst.num_elim_rows = static_cast<Index>(src.elim_rows.size());
+  dst.layout_id     = src.layout_id;
+  dst.bc_rows.assign(src.bc_rows, ctx);
+  dst.diag.assign(src.diag, ctx);
+  dst.bc_mask.assign(src.bc_mask, ctx);
+  dst.elim_rows.assign(src.elim_rows, ctx);
+  dst.elim_off.assign(src.elim_off, ctx);
+  dst.elim_k.assign(src.elim_k, ctx);
+  dst.elim_ib.assign(src.elim_ib, ctx);
+}
+
+inline void checkLayout(const DeviceBoundaryMap& map, const DeviceCsrMatrix& mat)
+{
+  require(mat.pattern().layoutId() == map.layout_id,
+          "BoundaryMap matrix does not match the mapped CSR layout");
+}
+
+inline void replaceRows(const DeviceBoundaryMap& map, DeviceCsrMatrix& jac, Real diag, HipContext& ctx)
+{
+  checkLayout(map, jac);
+  check(fx_bc_replace_rows(map.num_bcs, map.bc_rows.data(), map.diag.data(),
+                           jac.pattern().rowPtrData(), jac.valsData(), diag, ctx.raw()));
+}
+
+inline void replaceRes(const DeviceBoundaryMap& map, const DeviceVector& state,
+                       const DeviceVector& bc_vals, DeviceVector& res, HipContext& ctx)
+{
+  require(state.size() == map.rows && res.size() == map.rows && bc_vals.size() == map.num_bcs,
+          "BoundaryMap residual vectors have incompatible sizes");
+  check(fx_bc_replace_res(map.num_bcs, map.bc_rows.data(), state.data(), bc_vals.data(), res.data(),
+                          ctx.raw()));
+}
+
+inline void zeroBoundary(const DeviceBoundaryMap& map, DeviceVector& vals, HipContext& ctx)
+{
+  require(vals.size() == map.rows, "BoundaryMap vector has incompatible size");
+  check(fx_bc_zero(map.num_bcs, map.bc_rows.data(), vals.data(), ctx.raw()));
+}
+
+inline void prepareForwardSolve(const DeviceBoundaryMap& map, DeviceCsrMatrix& mat, DeviceVector& rhs,
+                                const DeviceVector& bc_vals, HipContext& ctx)
+{
+  checkLayout(map, mat);
+  require(rhs.size() == map.rows && bc_vals.size() == map.num_bcs,
+          "BoundaryMap forward vectors have incompatible sizes");
+  check(fx_bc_prepare_forward(map.num_bcs, map.bc_rows.data(), map.diag.data(),
+                              mat.pattern().rowPtrData(), map.num_elim_rows, map.elim_rows.data(),
+                              map.elim_off.data(), map.elim_k.data(), map.elim_ib.data(),
+                              mat.valsData(), rhs.data(), bc_vals.data(), ctx.raw()));
+}
+
+/** Structure-of-arrays 2-D geometry on the device (femx Geometry). */
+struct DeviceGeometry
+{
+  Index             num_nodes = 0, num_elems = 0, nodes_per_elem = 0;
+  DeviceVector      x, y;
+  DeviceIndexVector conn;
+};
+
+/** P1 Poisson operator: kappa[E] (or its log) and nodal load f[N]; null = 1 / 0. */
+struct PoissonP1
+{
+  const Real* kappa     = nullptr;
+  bool        log_kappa = false;
+  const Real* f         = nullptr;
+};
+
+/** Taylor-Hood NS operator: scalar nu or per-element nu_e[E]. */
+struct TaylorHoodNS
+{
+  Real        nu   = 0.01;
+  const Real* nu_e = nullptr;
+};
+
+inline void checkAssembly(const DeviceAssemblyMap& map, const DeviceVector* state,
+                          const DeviceVector& res, const DeviceCsrMatrix* jac)
+{
+  if (state) require(state->size() == map.info.num_states, "Assembly state size does not match AssemblyMap");
+  require(res.size() == map.info.num_res, "Assembly residual size does not match AssemblyMap");
+  if (jac)
+  {
+    require(jac->pattern().layoutId() == map.pattern->layoutId(),
+            "Assembly matrix must use the AssemblyMap CSR layout");
+    require(jac->valsData() != res.data() && (!state || state->data() != res.data()),
+            "Assembly state, residual, and matrix values must not alias");
+  }
+}
+
+/** assemble(op, geom, map, state, res, jac, ctx): residual and CSR Jacobian
+ * values, both replaced (CudaAssembly.hpp:319-364).  jac may be null. */
+inline void assemble(const PoissonP1& op, const DeviceGeometry& geom, const DeviceAssemblyMap& map,
+                     const DeviceVector* state, DeviceVector& res, DeviceCsrMatrix* jac,
+                     HipContext& ctx)
+{
+  checkAssembly(map, state, res, jac);
+  require(geom.num_elems == map.info.num_elems, "Geometry and AssemblyMap have different element counts");
+  check(fx_p1_assemble(geom.num_nodes, geom.num_elems, map.info.nnz, map.info.max_row_len,
+                       geom.x.data(), geom.y.data(), geom.conn.data(), op.kappa, op.log_kappa ? 1 : 0,
+                       op.f, state ? state->data() : nullptr, map.inc_off.data(), map.inc.data(),
+                       map.pattern->rowPtrData(), jac ? jac->valsData() : nullptr, res.data(),
+                       ctx.raw()));
+}
+
+/** Taylor-Hood geometry: vertex coordinates, vertex ids and DOF table. */
+struct DeviceTHGeometry
+{
+  Index             num_elems = 0, ndof = 0;
+  DeviceVector      xv, yv;
+  DeviceIndexVector vconn3, dofs15;
+};
+
+inline void assemble(const TaylorHoodNS& op, const DeviceTHGeometry& geom, const DeviceAssemblyMap& map,
+                     const DeviceVector& state, DeviceVector& res, DeviceCsrMatrix* jac,
+                     HipContext& ctx)
+{
+  checkAssembly(map, &state, res, jac);
+  require(geom.num_elems == map.info.num_elems, "Geometry and AssemblyMap have different element counts");
+  check(fx_th_assemble(geom.ndof, geom.num_elems, map.num_blocks, map.max_block_nnz, geom.xv.data(),
+                       geom.yv.data(), geom.vconn3.data(), geom.dofs15.data(), state.data(), op.nu,
+                       op.nu_e, map.inc_off.data(), map.inc_elem.data(), map.inc_meta.data(),
+                       map.blk_row.data(), map.blk_flags.data(), map.pattern->rowPtrData(),
+                       jac ? jac->valsData() : nullptr, res.data(), ctx.raw()));
+}
+
+} // namespace assembly
+
+namespace linalg
+{
+
+/** MatrixHandler<HipCsrBackend>: matvec / matvecT / zero / copy. */
+class HipMatrixHandler
+{
+public:
+  explicit HipMatrixHandler(HipContext& ctx) : ctx_(ctx) {}
+  void matvec(const DeviceCsrMatrix& A, const Real* x, Index nx, Real* y, Index ny, Real alpha = 1.0,
+              Real beta = 0.0) const
+  {
+    require(nx == A.cols() && ny == A.rows(), "CSR matvec vector size mismatch");
+    check(fx_csr_matvec(A.rows(), A.cols(), A.pattern().rowPtrData(), A.pattern().colIndData(),
+                        A.valsData(), x, y, alpha, beta, ctx_.raw()));
+  }
+  void matvecT(const DeviceCsrMatrix& A, const Real* x, Index nx, Real* y, Index ny,
+               Real alpha = 1.0, Real beta = 0.0) const
+  {
+    require(nx == A.rows() && ny == A.cols(), "CSR matvecT vector size mismatch");
+    const auto& t = A.pattern().transposeMap(ctx_);
+    check(fx_csr_matvec_t(A.rows(), A.cols(), t.t_ptr.data(), t.t_row.data(), t.t_perm.data(),
+                          A.valsData(), x, y, alpha, beta, ctx_.raw()));
+  }
+  void zero(DeviceCsrMatrix& A) const
+  {
+    if (A.nnz() > 0)
+      checkHip(hipMemsetAsync(A.valsData(), 0, sizeof(Real) * A.nnz(), ctx_.stream()), "hipMemsetAsync");
+  }
+
+private:
+  HipContext& ctx_;
+};
+
+/** VectorHandler<HipCsrBackend>: axpby and deterministic dot. */
+class HipVectorHandler
+{
+public:
+  explicit HipVectorHandler(HipContext& ctx) : ctx_(ctx), work_(fx_dot_work_len()) {}
+  void axpby(Real a, const DeviceVector& x, Real b, DeviceVector& y) const
+  {
+    require(x.size() == y.size(), "axpby size mismatch");
+    check(fx_vec_axpby(y.size(), a, x.data(), b, y.data(), ctx_.raw()));
+  }
+  void dot(const DeviceVector& x, const DeviceVector& y, DeviceVector& out)
+  {
+    require(x.size() == y.size() && out.size() == 1, "dot size mismatch");
+    check(fx_vec_dot(x.size(), x.data(), y.data(), work_.data(), out.data(), ctx_.raw()));
+  }
+
+private:
+  HipContext&  ctx_;
+  DeviceVector work_;
+};
+
+} // namespace linalg
+
+namespace state
+{
+/** Stationary residual contract over device storage (femx Residual<Backend>). */
+class Residual
+{
+public:
+  virtual ~Residual()                                                                       = default;
+  virtual void res(const DeviceVector& state, const DeviceVector& prm, DeviceVector& out,
+                   HipContext& ctx) const                                                   = 0;
+  virtual void assembleStateJac(const DeviceVector& state, const DeviceVector& prm,
+                                DeviceCsrMatrix& out, HipContext& ctx) const                = 0;
+  virtual void applyParamJacT(const DeviceVector& state, const DeviceVector& prm,
+                              const DeviceVector& adj, DeviceVector& out, HipContext& ctx) const = 0;
+};
+} // namespace state
+
+} // namespace femx
