@@ -27,6 +27,7 @@ _SIGS = {
     "fx_map_export_pattern": ([_vp, _vp, _vp], C.c_int),
     "fx_map_export_jac": ([_vp, _vp, _vp], C.c_int),
     "fx_map_export_rowinc3": ([_vp, _vp, _vp], C.c_int),
+    "fx_map_export_rowinc": ([_vp, _vp, _vp], C.c_int),
     "fx_map_export_rowinc15": ([_vp, _vp, _vp, _vp], C.c_int),
     "fx_map_th_blocks": ([_vp, _i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_map_destroy": ([_vp], None),
@@ -50,8 +51,29 @@ _SIGS = {
                        C.c_int),
     "fx_p1_vjp": ([_i32, _i32] + [_vp] * 11 + [_vp], C.c_int),
     "fx_th_assemble": ([_i32] * 4 + [_vp] * 5 + [_f64] + [_vp] * 9 + [_vp], C.c_int),
-    "fx_th_vjp": ([_i32] * 3 + [_vp] * 5 + [_f64] + [_vp] * 10 + [_vp], C.c_int),
+    "fx_th_vjp": ([_i32] * 2 + [_vp] * 5 + [_f64] + [_vp] * 8 + [_vp], C.c_int),
+    "fx_batch_create": ([_i32, _vp, _i32, _vp, _i32, C.POINTER(_vp)], C.c_int),
+    "fx_batch_work": ([_vp, _vp, _vp], C.c_int),
+    "fx_batch_run": ([_vp, C.c_int, _vp], C.c_int),
+    "fx_batch_capture": ([_vp, C.c_int], C.c_int),
+    "fx_batch_launch": ([_vp, _vp], C.c_int),
+    "fx_batch_destroy": ([_vp], None),
 }
+
+
+class P1Instance(C.Structure):
+    _fields_ = ([(n, _i32) for n in ("num_nodes", "num_elems", "nnz", "max_row_len")]
+                + [(n, _vp) for n in ("x", "y", "conn", "m", "f", "u", "inc_off", "inc", "row_ptr",
+                                      "vals", "res", "lam", "bc_mask", "jt", "g")])
+
+
+class THInstance(C.Structure):
+    _fields_ = ([(n, _i32) for n in ("ndof", "num_elems", "num_blocks", "max_block_nnz")]
+                + [(n, _vp) for n in ("xv", "yv", "vconn3", "dofs15", "state")]
+                + [("nu", _f64)]
+                + [(n, _vp) for n in ("nu_e", "inc_off", "inc_elem", "inc_meta", "blk_row",
+                                      "blk_flags", "row_ptr", "vals", "res", "inc_pos", "lam",
+                                      "bc_mask", "jt", "g", "work")])
 
 
 class MapInfo(C.Structure):

@@ -75,12 +75,23 @@ class AssemblyMap:
             self._destroy(h)
             self._h = None
 
+    def release_host(self):
+        """Free the host builder (jac_map and incidence tables) once every
+        device table has been exported; the pattern stays."""
+        if getattr(self, "_h", None):
+            self._destroy(self._h)
+            self._h = None
+
+    def _handle(self):
+        _require(self._h is not None, "AssemblyMap host tables were released")
+        return self._h
+
     def jac(self):
         """femx's (jac_offsets, jac_map) host arrays."""
         if self._jac is None:
             jo = np.empty(self.num_elems + 1, np.int32)
             jm = np.empty(max(self.info.jac_len, 1), np.int32)
-            _lib.call("fx_map_export_jac", self._h, ptr(jo), ptr(jm))
+            _lib.call("fx_map_export_jac", self._handle(), ptr(jo), ptr(jm))
             self._jac = (jo, jm[: self.info.jac_len])
         return self._jac
 
@@ -89,7 +100,7 @@ class AssemblyMap:
         if self._rowinc3 is None:
             off = np.empty(self.num_res + 1, np.int32)
             inc = np.empty(max(2 * self.info.num_inc, 2), np.int32)
-            _lib.call("fx_map_export_rowinc3", self._h, ptr(off), ptr(inc))
+            _lib.call("fx_map_export_rowinc3", self._handle(), ptr(off), ptr(inc))
             self._rowinc3 = (off, inc[: 2 * self.info.num_inc])
         return self._rowinc3
 

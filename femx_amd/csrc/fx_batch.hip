@@ -1,0 +1,314 @@
+// Batched assembly and adjoint VJP of many P1 Poisson and Taylor-Hood
+// Navier-Stokes instances of uneven size.
+//
+// femx assembles one problem per call (femx/assembly/CudaAssembly.hpp:
+// 319-364), each with its own resize, zero, attribute query and launch.  Here
+// a batch owns device descriptors of all instances and one work list per
+// phase -- (instance, block) pairs covering every row block of every
+// instance.  Each phase is one persistent launch: a grid sized to the GPU
+// (compute units x a tunable multiplier) whose workgroups walk the work list
+// with a fixed stride, running the same per-block device code as the
+// single-instance kernels, so each instance's results are bitwise equal to
+// its plain launch.  The five phases (P1 assembly, Taylor-Hood assembly, P1
+// VJP, Taylor-Hood VJP element and gather phases) are captured once into a
+// hipGraph and replayed with a single hipGraphLaunch.
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "fx_p1_dev.hpp"
+#include "fx_th_dev.hpp"
+
+struct fx_batch
+{
+  int         num_p1 = 0, num_th = 0;
+  int         grid   = 0;
+  std::size_t p1_lds = 0, th_lds = 0;
+  // device memory
+  fxk::P1AsmArgs*  d_p1_asm  = nullptr;
+  fxk::P1VjpArgs*  d_p1_vjp  = nullptr;
+  fxk::ThAsmArgs*  d_th_asm  = nullptr;
+  fxk::ThVjpArgs*  d_th_vjp  = nullptr;
+  fxk::GatherArgs* d_th_gat  = nullptr;
+  int2*            d_work[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int              nwork[5]  = {0, 0, 0, 0, 0};
+  hipStream_t      cap_stream = nullptr;
+  hipGraph_t       graph      = nullptr;
+  hipGraphExec_t   exec       = nullptr;
+  int              graph_what = 0;
+  int              device     = 0;
+
+  ~fx_batch()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    for (void* p : {static_cast<void*>(d_p1_asm), static_cast<void*>(d_p1_vjp),
+                    static_cast<void*>(d_th_asm), static_cast<void*>(d_th_vjp),
+                    static_cast<void*>(d_th_gat)})
+      if (p) (void)hipFree(p);
+    for (auto* w : d_work)
+      if (w) (void)hipFree(w);
+  }
+};
+
+namespace
+{
+using namespace fxk;
+
+enum Phase
+{
+  kP1Asm = 0,
+  kThAsm = 1,
+  kP1Vjp = 2,
+  kThVjpElem = 3,
+  kThVjpGather = 4
+};
+
+__global__ __launch_bounds__(kP1Block) void k_batch_p1_assemble(const P1AsmArgs* __restrict__ inst,
+                                                               const int2* __restrict__ work,
+                                                               int nwork)
+{
+  extern __shared__ double smem[];
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+  {
+    const int2 it = work[w];
+    p1_assemble_block(it.y, inst[it.x], smem);
+  }
+}
+
+__global__ __launch_bounds__(kP1Block) void k_batch_p1_vjp(const P1VjpArgs* __restrict__ inst,
+                                                          const int2* __restrict__ work, int nwork)
+{
+  __shared__ long long sinc[kIncStage];
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+  {
+    const int2 it = work[w];
+    p1_vjp_block(it.y, inst[it.x], sinc);
+  }
+}
+
+__global__ __launch_bounds__(kThLanes) void k_batch_th_assemble(const ThAsmArgs* __restrict__ inst,
+                                                               const int2* __restrict__ work,
+                                                               int nwork)
+{
+  extern __shared__ double smem[];
+  ThTables& T = *reinterpret_cast<ThTables*>(smem);
+  th_build_tables(T);
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+  {
+    const int2 it = work[w];
+    th_assemble_block(it.y, inst[it.x], T, smem + sizeof(ThTables) / sizeof(double));
+  }
+}
+
+__global__ __launch_bounds__(kThLanes) void k_batch_th_vjp_elem(const ThVjpArgs* __restrict__ inst,
+                                                               const int2* __restrict__ work,
+                                                               int nwork)
+{
+  __shared__ ThTables T;
+  th_build_tables(T);
+  __syncthreads();
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+  {
+    const int2 it = work[w];
+    th_vjp_elem_block(it.y, inst[it.x], T);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_vjp_gather(const GatherArgs* __restrict__ inst,
+                                                         const int2* __restrict__ work, int nwork)
+{
+  __shared__ int spos[kGatherStage];
+  for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+  {
+    const int2 it = work[w];
+    vjp_gather_block(it.y, inst[it.x], spos);
+  }
+}
+
+template <class T>
+T* upload(const std::vector<T>& h)
+{
+  if (h.empty()) return nullptr;
+  T* d = nullptr;
+  fx::check_hip(hipMalloc(&d, sizeof(T) * h.size()), "fx_batch_create: hipMalloc");
+  fx::check_hip(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice),
+                "fx_batch_create: hipMemcpy");
+  return d;
+}
+
+void launch_phases(fx_batch* b, int what, hipStream_t s)
+{
+  const dim3 g(b->grid);
+  if (what & FX_BATCH_ASSEMBLE)
+  {
+    if (b->nwork[kP1Asm])
+      hipLaunchKernelGGL(k_batch_p1_assemble, g, dim3(kP1Block), b->p1_lds, s, b->d_p1_asm,
+                         b->d_work[kP1Asm], b->nwork[kP1Asm]);
+    if (b->nwork[kThAsm])
+      hipLaunchKernelGGL(k_batch_th_assemble, g, dim3(kThLanes), b->th_lds, s, b->d_th_asm,
+                         b->d_work[kThAsm], b->nwork[kThAsm]);
+    fx::check_launch("fx_batch: assembly");
+  }
+  if (what & FX_BATCH_VJP)
+  {
+    if (b->nwork[kP1Vjp])
+      hipLaunchKernelGGL(k_batch_p1_vjp, g, dim3(kP1Block), 0, s, b->d_p1_vjp, b->d_work[kP1Vjp],
+                         b->nwork[kP1Vjp]);
+    if (b->nwork[kThVjpElem])
+      hipLaunchKernelGGL(k_batch_th_vjp_elem, g, dim3(kThLanes), 0, s, b->d_th_vjp,
+                         b->d_work[kThVjpElem], b->nwork[kThVjpElem]);
+    if (b->nwork[kThVjpGather])
+      hipLaunchKernelGGL(k_batch_vjp_gather, g, dim3(256), 0, s, b->d_th_gat,
+                         b->d_work[kThVjpGather], b->nwork[kThVjpGather]);
+    fx::check_launch("fx_batch: VJP");
+  }
+}
+
+} // namespace
+
+extern "C" {
+
+int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
+                    const fx_th_instance* th, fx_index grid_per_cu, fx_batch** out)
+{
+  return fx::guard([&] {
+    fx::require(out != nullptr && num_p1 >= 0 && num_th >= 0 && grid_per_cu > 0
+                    && (num_p1 == 0 || p1) && (num_th == 0 || th),
+                FX_ERR_ARG, "fx_batch_create: invalid arguments");
+    *out   = nullptr;
+    auto b = std::make_unique<fx_batch>();
+    fx::check_hip(hipGetDevice(&b->device), "hipGetDevice");
+    int cus = 0;
+    fx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device),
+                  "hipDeviceGetAttribute");
+    b->grid   = std::max(1, cus * grid_per_cu);
+    b->num_p1 = num_p1;
+    b->num_th = num_th;
+
+    std::vector<P1AsmArgs>  pa;
+    std::vector<P1VjpArgs>  pv;
+    std::vector<ThAsmArgs>  ta;
+    std::vector<ThVjpArgs>  tv;
+    std::vector<GatherArgs> tg;
+    std::vector<int2>       w[5];
+    int                     max_row_len = 1;
+    fx_index                max_blk_nnz = 0;
+    for (int i = 0; i < num_p1; ++i)
+    {
+      const fx_p1_instance& q = p1[i];
+      fx::require(q.num_nodes > 0 && q.num_elems > 0 && q.max_row_len > 0 && q.max_row_len <= 32,
+                  FX_ERR_ARG, "fx_batch_create: invalid P1 instance sizes");
+      fx::require(q.x && q.y && q.conn && q.inc_off && q.inc && q.row_ptr && q.res && q.vals,
+                  FX_ERR_ARG, "fx_batch_create: P1 instance lacks an array");
+      max_row_len = std::max(max_row_len, static_cast<int>(q.max_row_len));
+      pa.push_back({q.num_nodes, q.x, q.y, q.conn, q.m, 1, q.f, q.u, q.inc_off,
+                    reinterpret_cast<const int2*>(q.inc), q.row_ptr, q.vals, q.res, 0});
+      pv.push_back({q.num_nodes, q.x, q.y, q.conn, q.m, q.u, q.lam, q.bc_mask, q.inc_off,
+                    reinterpret_cast<const int2*>(q.inc), q.jt, q.g});
+      const int nb = static_cast<int>(fx::blocks_for(q.num_nodes, kP1Block));
+      for (int k = 0; k < nb; ++k)
+      {
+        w[kP1Asm].push_back(make_int2(i, k));
+        if (q.lam && q.jt && q.g && q.m && q.u) w[kP1Vjp].push_back(make_int2(i, k));
+      }
+    }
+    const int acc_cap = kP1Block * max_row_len;
+    for (auto& a : pa) a.acc_cap = acc_cap;
+    b->p1_lds = static_cast<std::size_t>(acc_cap) * 8 + kIncStage * 8;
+    for (int i = 0; i < num_th; ++i)
+    {
+      const fx_th_instance& q = th[i];
+      fx::require(q.ndof > 0 && q.num_elems > 0 && q.num_blocks > 0 && q.max_block_nnz >= 0,
+                  FX_ERR_ARG, "fx_batch_create: invalid Taylor-Hood instance sizes");
+      fx::require(q.xv && q.yv && q.vconn3 && q.dofs15 && q.state && q.inc_off && q.inc_elem
+                      && q.inc_meta && q.blk_row && q.blk_flags && q.row_ptr && q.vals && q.res,
+                  FX_ERR_ARG, "fx_batch_create: Taylor-Hood instance lacks an array");
+      max_blk_nnz = std::max(max_blk_nnz, q.max_block_nnz);
+      ta.push_back({q.xv, q.yv, q.vconn3, q.dofs15, q.state, q.nu, q.nu_e, q.inc_off, q.inc_elem,
+                    reinterpret_cast<const uint4*>(q.inc_meta), q.blk_row, q.blk_flags, q.row_ptr,
+                    q.vals, q.res});
+      for (int k = 0; k < q.num_blocks; ++k) w[kThAsm].push_back(make_int2(i, k));
+      const bool vjp = q.lam && q.jt && q.work && q.inc_pos;
+      tv.push_back({q.num_elems, q.xv, q.yv, q.vconn3, q.dofs15, q.state, q.nu, q.nu_e, q.lam,
+                    q.bc_mask, q.work, q.g});
+      tg.push_back({q.ndof, q.inc_off, q.inc_pos, q.work, q.lam, q.bc_mask, q.jt});
+      if (vjp)
+      {
+        const int ne = static_cast<int>(fx::blocks_for(q.num_elems, kThLanes));
+        const int nd = static_cast<int>(fx::blocks_for(q.ndof, 256));
+        for (int k = 0; k < ne; ++k) w[kThVjpElem].push_back(make_int2(i, k));
+        for (int k = 0; k < nd; ++k) w[kThVjpGather].push_back(make_int2(i, k));
+      }
+    }
+    b->th_lds = sizeof(ThTables) + static_cast<std::size_t>(max_blk_nnz) * 8;
+    fx::require(b->p1_lds <= 160 * 1024 && b->th_lds <= 160 * 1024, FX_ERR_UNSUPPORTED,
+                "fx_batch_create: LDS tile exceeds the workgroup limit");
+    b->d_p1_asm = upload(pa);
+    b->d_p1_vjp = upload(pv);
+    b->d_th_asm = upload(ta);
+    b->d_th_vjp = upload(tv);
+    b->d_th_gat = upload(tg);
+    for (int p = 0; p < 5; ++p)
+    {
+      b->nwork[p]  = static_cast<int>(w[p].size());
+      b->d_work[p] = upload(w[p]);
+    }
+    *out = b.release();
+  });
+}
+
+int fx_batch_work(const fx_batch* b, fx_index* counts5, fx_index* grid)
+{
+  return fx::guard([&] {
+    fx::require(b && counts5, FX_ERR_ARG, "fx_batch_work: null argument");
+    for (int p = 0; p < 5; ++p) counts5[p] = b->nwork[p];
+    if (grid) *grid = b->grid;
+  });
+}
+
+int fx_batch_run(fx_batch* b, int what, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(b != nullptr, FX_ERR_ARG, "fx_batch_run: null batch");
+    launch_phases(b, what, fx::as_stream(stream));
+  });
+}
+
+int fx_batch_capture(fx_batch* b, int what)
+{
+  return fx::guard([&] {
+    fx::require(b != nullptr, FX_ERR_ARG, "fx_batch_capture: null batch");
+    if (b->exec)
+    {
+      (void)hipGraphExecDestroy(b->exec);
+      b->exec = nullptr;
+    }
+    if (b->graph)
+    {
+      (void)hipGraphDestroy(b->graph);
+      b->graph = nullptr;
+    }
+    if (!b->cap_stream)
+      fx::check_hip(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
+    fx::check_hip(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal),
+                  "hipStreamBeginCapture");
+    launch_phases(b, what, b->cap_stream);
+    fx::check_hip(hipStreamEndCapture(b->cap_stream, &b->graph), "hipStreamEndCapture");
+    fx::check_hip(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    b->graph_what = what;
+  });
+}
+
+int fx_batch_launch(fx_batch* b, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(b && b->exec, FX_ERR_ARG, "fx_batch_launch: capture the batch first");
+    fx::check_hip(hipGraphLaunch(b->exec, fx::as_stream(stream)), "hipGraphLaunch");
+  });
+}
+
+void fx_batch_destroy(fx_batch* b) { delete b; }
+
+} // extern "C"

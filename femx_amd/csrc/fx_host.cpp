@@ -347,6 +347,20 @@ int fx_map_export_rowinc3(const fx_map* m, Index* inc_off, Index* inc)
 }
 
 
+// Generic row incidence: inc_off[num_res+1] and inc_pos[num_inc], the
+// position res_offsets[e] + li of each (element, local row) in the flattened
+// element DOF table, rows in ascending element order.
+int fx_map_export_rowinc(const fx_map* m, Index* inc_off, Index* inc_pos)
+{
+  return fx::guard([&] {
+    fx::require(m && inc_off && (m->inc_elem.empty() || inc_pos), FX_ERR_ARG,
+                "fx_map_export_rowinc: null argument");
+    std::copy(m->inc_off.begin(), m->inc_off.end(), inc_off);
+    for (std::size_t t = 0; t < m->inc_elem.size(); ++t)
+      inc_pos[t] = m->res_off[m->inc_elem[t]] + m->inc_local[t];
+  });
+}
+
 // Row incidence for 15x15 elements (Taylor-Hood): inc_elem[num_inc] and
 // inc_meta[4*num_inc] (16 bytes per incidence: positions of the element's 15
 // columns within the row, from row_ptr[row], then the local row).

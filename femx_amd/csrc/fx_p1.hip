@@ -1,217 +1,22 @@
-// P1 Poisson assembly and log-kappa VJP for gfx950.
-//
-// Design: one lane per global row (node), gathering over the row's incident
-// elements in ascending element order (fx_map_export_rowinc3).  Each lane
-// recomputes the one element-matrix row it needs from the three vertex
-// coordinates in registers (affine triangles: constant gradients), so no
-// per-element quadrature data is streamed from HBM.  A workgroup owns 256
-// consecutive rows: their incidence entries are one contiguous range, staged
-// in LDS with coalesced loads, and their CSR values are one contiguous range,
-// accumulated in LDS and written back with coalesced stores.  Streams read or
-// written once (incidences, outputs) use non-temporal accesses so that the
-// L2 keeps the node and element data that neighbouring rows share.  Every CSR value and residual
-// entry is a sum over its elements in ascending element order with no
-// atomics: results are deterministic and equal, operation for operation, to
-// femx's host assemble loop (femx/assembly/Assembly.hpp:184-225) evaluating
-// PoissonComponents::evalRow (examples/poisson/PoissonComponents.hpp:35-64)
-// on ElementValues' physical gradients (femx/fem/ElementValues.cpp:138-181).
-// The file is compiled without floating-point contraction so that each
-// product is rounded as in femx's host code.
-#include "fx_common.hpp"
+// P1 Poisson assembly and log-kappa VJP: single-instance kernels and their
+// C-ABI entry points.  The device code and its design notes are in
+// fx_p1_dev.hpp.
+#include "fx_p1_dev.hpp"
 
 namespace
 {
+using namespace fxk;
 
-constexpr int kP1Block = 256;
-constexpr int kP1MaxRowLen = 32; // LDS bound: 256 rows x 32 x 8 B = 64 KiB (+16 KiB staging)
-
-// Physical P1 gradients and |det J|*wt of an affine triangle, evaluated as
-// ElementValues::calcPhysicalValues/invJacobian do with the P1 reference
-// gradients (LagrangeTriangleP1::calcdNdr) and weight 1/6 of the 3-point rule
-// (GaussQuadrature::triangle(2)).
-struct P1Geom
-{
-  double gx[3], gy[3];
-  double jxw;
-};
-
-__device__ __forceinline__ P1Geom p1_geom(double x0, double y0, double x1, double y1, double x2,
-                                          double y2)
-{
-  // J = sum_in x(in) dNdr(in,:) with dNdr = {(-1,-1),(1,0),(0,1)}.
-  const double j00 = x1 - x0, j01 = x2 - x0, j10 = y1 - y0, j11 = y2 - y0;
-  const double det     = j00 * j11 - j01 * j10;
-  const double inv_det = 1.0 / det;
-  const double i00 = j11 * inv_det, i01 = -j01 * inv_det, i10 = -j10 * inv_det,
-               i11 = j00 * inv_det;
-  P1Geom g;
-  g.gx[0] = -i00 - i10;
-  g.gy[0] = -i01 - i11;
-  g.gx[1] = i00;
-  g.gy[1] = i01;
-  g.gx[2] = i10;
-  g.gy[2] = i11;
-  g.jxw   = fabs(det) * (1.0 / 6.0);
-  return g;
-}
-
-// Stiffness entry K_rc = kappa * sum_q (grad phi_r . grad phi_c) JxW_q over
-// the three (identical) quadrature points, in PoissonComponents' order.
-// Select v[i] without dynamic register-array indexing (which would spill).
-__device__ __forceinline__ double sel3(const double (&v)[3], int i)
-{
-  return i == 0 ? v[0] : (i == 1 ? v[1] : v[2]);
-}
-
-__device__ __forceinline__ double p1_k(const P1Geom& g, int r, int c, double kappa)
-{
-  const double gd = sel3(g.gx, r) * sel3(g.gx, c) + sel3(g.gy, r) * sel3(g.gy, c);
-  double       s  = gd * g.jxw;
-  s               = s + gd * g.jxw;
-  s               = s + gd * g.jxw;
-  return kappa * s;
-}
-
-// Consistent load F_r = sum_q N_r(q) f_h(q) JxW_q on the 3-point rule.
-__device__ __forceinline__ double p1_load(const P1Geom& g, int r, double f0, double f1,
-                                          double f2)
-{
-  const double a = 1.0 / 6.0, b = 2.0 / 3.0;
-  const double qr[3] = {a, b, a}, qs[3] = {a, a, b};
-  double       F     = 0.0;
-#pragma unroll
-  for (int q = 0; q < 3; ++q)
-  {
-    const double n0 = 1.0 - qr[q] - qs[q], n1 = qr[q], n2 = qs[q];
-    const double fq = n0 * f0 + n1 * f1 + n2 * f2;
-    const double nr = r == 0 ? n0 : (r == 1 ? n1 : n2);
-    F               = F + nr * fq * g.jxw;
-  }
-  return F;
-}
-
-// Incidences of one block staged in LDS when they fit (coalesced loads of a
-// contiguous range); longer blocks read them straight from global memory.
-constexpr int kIncStage = 2048;
-
-__device__ __forceinline__ int2 inc_at(const long long* sinc, const int2* inc, bool staged, int t,
-                                       int ib0)
-{
-  if (staged)
-  {
-    const long long v = sinc[t - ib0];
-    return make_int2(static_cast<int>(v & 0xffffffffll), static_cast<int>(v >> 32));
-  }
-  return inc[t];
-}
-
-__global__ __launch_bounds__(kP1Block) void k_p1_assemble(
-    int num_rows, const double* __restrict__ x, const double* __restrict__ y,
-    const int* __restrict__ conn, const double* __restrict__ kappa, int kappa_is_log,
-    const double* __restrict__ f, const double* __restrict__ u, const int* __restrict__ inc_off,
-    const int2* __restrict__ inc, const int* __restrict__ row_ptr, double* __restrict__ vals,
-    double* __restrict__ res, int acc_cap)
+__global__ __launch_bounds__(kP1Block) void k_p1_assemble(P1AsmArgs a)
 {
   extern __shared__ double smem[];
-  double*     acc  = smem;
-  long long*  sinc = reinterpret_cast<long long*>(smem + acc_cap);
-  const int   r0   = blockIdx.x * kP1Block;
-  const int   r1   = min(r0 + kP1Block, num_rows);
-  const int   base = row_ptr[r0];
-  const int   bn   = vals ? row_ptr[r1] - base : 0;
-  const int   ib0 = inc_off[r0], ib1 = inc_off[r1];
-  const bool  staged = ib1 - ib0 <= kIncStage;
-  for (int i = threadIdx.x; i < bn; i += kP1Block) acc[i] = 0.0;
-  if (staged)
-    for (int i = threadIdx.x; i < ib1 - ib0; i += kP1Block)
-      sinc[i] = __builtin_nontemporal_load(reinterpret_cast<const long long*>(inc) + ib0 + i);
-  __syncthreads();
-
-  const int r = r0 + threadIdx.x;
-  if (r < num_rows)
-  {
-    double*   racc = acc + (row_ptr[r] - base);
-    double    rsum = 0.0;
-    const int t1   = inc_off[r + 1];
-    for (int t = inc_off[r]; t < t1; ++t)
-    {
-      const int2     it   = inc_at(sinc, inc, staged, t, ib0);
-      const int      e    = it.x;
-      const unsigned meta = static_cast<unsigned>(it.y);
-      const int      li   = meta & 0xff;
-      const int      n0 = conn[3 * e], n1 = conn[3 * e + 1], n2 = conn[3 * e + 2];
-      const P1Geom   g  = p1_geom(x[n0], y[n0], x[n1], y[n1], x[n2], y[n2]);
-      const double   ke = kappa ? (kappa_is_log ? exp(kappa[e]) : kappa[e]) : 1.0;
-      const double   k0 = p1_k(g, li, 0, ke), k1 = p1_k(g, li, 1, ke), k2 = p1_k(g, li, 2, ke);
-      double         re = 0.0;
-      if (u) re = k0 * u[n0] + k1 * u[n1] + k2 * u[n2];
-      if (f) re = re - p1_load(g, li, f[n0], f[n1], f[n2]);
-      else re = re - 0.0;
-      rsum = rsum + re;
-      if (vals)
-      {
-        racc[(meta >> 8) & 0xff]  = racc[(meta >> 8) & 0xff] + k0;
-        racc[(meta >> 16) & 0xff] = racc[(meta >> 16) & 0xff] + k1;
-        racc[meta >> 24]          = racc[meta >> 24] + k2;
-      }
-    }
-    __builtin_nontemporal_store(rsum, res + r);
-  }
-  __syncthreads();
-  for (int i = threadIdx.x; i < bn; i += kP1Block)
-    __builtin_nontemporal_store(acc[i], vals + base + i);
+  p1_assemble_block(blockIdx.x, a, smem);
 }
 
-// One pass: each lane (node j) sums column j of its elements' matrices against
-// the boundary-masked adjoint; the lane holding an element's local row 0
-// also writes that element's control sensitivity g_e = lam~_e^T K^e u_e.
-__global__ __launch_bounds__(kP1Block) void k_p1_vjp(
-    int num_rows, const double* __restrict__ x, const double* __restrict__ y,
-    const int* __restrict__ conn, const double* __restrict__ m, const double* __restrict__ u,
-    const double* __restrict__ lam, const int* __restrict__ bc_mask,
-    const int* __restrict__ inc_off, const int2* __restrict__ inc, double* __restrict__ jt,
-    double* __restrict__ gout)
+__global__ __launch_bounds__(kP1Block) void k_p1_vjp(P1VjpArgs a)
 {
   __shared__ long long sinc[kIncStage];
-  const int  r0 = blockIdx.x * kP1Block, r1 = min(r0 + kP1Block, num_rows);
-  const int  ib0 = inc_off[r0], ib1 = inc_off[r1];
-  const bool staged = ib1 - ib0 <= kIncStage;
-  if (staged)
-    for (int i = threadIdx.x; i < ib1 - ib0; i += kP1Block)
-      sinc[i] = __builtin_nontemporal_load(reinterpret_cast<const long long*>(inc) + ib0 + i);
-  __syncthreads();
-  const int j = r0 + threadIdx.x;
-  if (j >= num_rows) return;
-  double    acc = 0.0;
-  const int t1  = inc_off[j + 1];
-  for (int t = inc_off[j]; t < t1; ++t)
-  {
-    const int2   it = inc_at(sinc, inc, staged, t, ib0);
-    const int    e  = it.x;
-    const int    li = static_cast<unsigned>(it.y) & 0xff;
-    const int    n[3] = {conn[3 * e], conn[3 * e + 1], conn[3 * e + 2]};
-    const P1Geom g    = p1_geom(x[n[0]], y[n[0]], x[n[1]], y[n[1]], x[n[2]], y[n[2]]);
-    const double ke   = exp(m[e]);
-    double       le[3];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) le[r] = (bc_mask && bc_mask[n[r]]) ? 0.0 : lam[n[r]];
-    const double c0 = p1_k(g, 0, li, ke), c1 = p1_k(g, 1, li, ke), c2 = p1_k(g, 2, li, ke);
-    acc = acc + (c0 * le[0] + c1 * le[1] + c2 * le[2]);
-    if (li == 0)
-    {
-      const double u0 = u[n[0]], u1 = u[n[1]], u2 = u[n[2]];
-      double       ge = 0.0;
-#pragma unroll
-      for (int r = 0; r < 3; ++r)
-      {
-        const double re = p1_k(g, r, 0, ke) * u0 + p1_k(g, r, 1, ke) * u1 + p1_k(g, r, 2, ke) * u2;
-        ge              = ge + le[r] * (re - 0.0);
-      }
-      gout[e] = ge;
-    }
-  }
-  if (bc_mask && bc_mask[j]) acc = acc + lam[j];
-  __builtin_nontemporal_store(acc, jt + j);
+  p1_vjp_block(blockIdx.x, a, sinc);
 }
 
 } // namespace
@@ -239,9 +44,9 @@ int fx_p1_assemble(fx_index num_nodes, fx_index num_elems, fx_index nnz, fx_inde
     const unsigned nb      = fx::blocks_for(num_nodes, kP1Block);
     const int      acc_cap = kP1Block * (max_row_len > 0 ? max_row_len : 1);
     const size_t   lds     = static_cast<size_t>(acc_cap) * 8 + kIncStage * 8;
-    hipLaunchKernelGGL(k_p1_assemble, dim3(nb), dim3(kP1Block), lds, fx::as_stream(stream),
-                       num_nodes, x, y, conn, kappa, kappa_is_log, f, u, inc_off,
-                       reinterpret_cast<const int2*>(inc), row_ptr, vals, res, acc_cap);
+    const P1AsmArgs a{num_nodes, x,  y,       conn, kappa, kappa_is_log, f, u, inc_off,
+                      reinterpret_cast<const int2*>(inc), row_ptr, vals, res, acc_cap};
+    hipLaunchKernelGGL(k_p1_assemble, dim3(nb), dim3(kP1Block), lds, fx::as_stream(stream), a);
     fx::check_launch("fx_p1_assemble");
   });
 }
@@ -263,9 +68,9 @@ int fx_p1_vjp(fx_index num_nodes, fx_index num_elems, const double* x, const dou
                     && !fx::overlaps(g, E * 8, m, E * 8),
                 FX_ERR_ALIAS, "fx_p1_vjp: outputs must not alias inputs or each other");
     const unsigned nb = fx::blocks_for(num_nodes, kP1Block);
-    hipLaunchKernelGGL(k_p1_vjp, dim3(nb), dim3(kP1Block), 0, fx::as_stream(stream), num_nodes,
-                       x, y, conn, m, u, lam, bc_mask, inc_off,
-                       reinterpret_cast<const int2*>(inc), jt_lam, g);
+    const P1VjpArgs a{num_nodes, x,  y,   conn, m, u, lam, bc_mask, inc_off,
+                      reinterpret_cast<const int2*>(inc), jt_lam, g};
+    hipLaunchKernelGGL(k_p1_vjp, dim3(nb), dim3(kP1Block), 0, fx::as_stream(stream), a);
     fx::check_launch("fx_p1_vjp");
   });
 }

@@ -21,7 +21,7 @@ from . import _lib
 from .assembly import AssemblyMap, make_assembly_map, make_boundary_map, replace_res, replace_rows
 from .linalg import CsrMatrix, _check_dev_vec, _require, ptr, stream_handle
 
-TH_LANES = 64
+TH_ITEMS = 32  # work items per block (FX_TH_BLOCK_ITEMS)
 
 
 class THGeometry:
@@ -65,27 +65,34 @@ def structured_th(n, device="cuda"):
 
 
 class THPlan:
-    """Row-incidence tables and work blocks of the Taylor-Hood kernels for one
-    assembly map, on one device."""
+    """Row-incidence tables, work blocks and the VJP workspace of the
+    Taylor-Hood kernels for one assembly map, on one device.  Calls that share
+    a plan share its workspace: VJPs of one map on different streams must not
+    overlap."""
 
-    def __init__(self, amap: AssemblyMap, device="cuda"):
+    def __init__(self, amap: AssemblyMap, device="cuda", items=TH_ITEMS):
         dev = torch.device(device)
         info = amap.info
+        self.items = int(items)
         off = np.empty(amap.num_res + 1, np.int32)
         elem = np.empty(max(info.num_inc, 1), np.int32)
         meta = np.empty(max(4 * info.num_inc, 4), np.int32)
-        _lib.call("fx_map_export_rowinc15", amap._h, ptr(off), ptr(elem), ptr(meta))
+        _lib.call("fx_map_export_rowinc15", amap._handle(), ptr(off), ptr(elem), ptr(meta))
         nb, mx = C.c_int32(), C.c_int32()
-        _lib.call("fx_map_th_blocks", amap._h, TH_LANES, C.byref(nb), C.byref(mx), None, None)
+        _lib.call("fx_map_th_blocks", amap._handle(), self.items, C.byref(nb), C.byref(mx), None, None)
         blk_row = np.empty(nb.value + 1, np.int32)
         blk_flags = np.empty(max(nb.value, 1), np.int32)
-        _lib.call("fx_map_th_blocks", amap._h, TH_LANES, C.byref(nb), C.byref(mx), ptr(blk_row),
+        _lib.call("fx_map_th_blocks", amap._handle(), self.items, C.byref(nb), C.byref(mx), ptr(blk_row),
                   ptr(blk_flags))
         self.num_blocks, self.max_block_nnz = nb.value, mx.value
         self.h_blk_row, self.h_blk_flags = blk_row, blk_flags[: nb.value]
         t = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
         self.inc_off, self.inc_elem = t(off), t(elem[: info.num_inc])
         self.inc_meta = t(meta[: 4 * info.num_inc])
+        pos = np.empty(max(info.num_inc, 1), np.int32)
+        _lib.call("fx_map_export_rowinc", amap._handle(), ptr(off), ptr(pos))
+        self.inc_pos = t(pos[: info.num_inc])
+        self.work = torch.empty(15 * amap.num_elems, dtype=torch.float64, device=dev)
         self.blk_row, self.blk_flags = t(blk_row), t(self.h_blk_flags)
 
 
@@ -155,10 +162,10 @@ def ns_vjp(op: TaylorHoodNS, geom: THGeometry, amap: AssemblyMap, state, lam, bc
     if g_out is None:
         g_out = torch.empty(geom.num_elems, dtype=torch.float64, device=dev)
     p = _plan(amap, dev)
-    _lib.call("fx_th_vjp", geom.ndof, geom.num_elems, p.num_blocks, ptr(geom.xv), ptr(geom.yv),
+    _lib.call("fx_th_vjp", geom.ndof, geom.num_elems, ptr(geom.xv), ptr(geom.yv),
               ptr(geom.vconn3), ptr(geom.dofs15), ptr(state), op.nu, ptr(op.nu_e), ptr(lam),
-              ptr(bc_mask), ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row),
-              ptr(p.blk_flags), ptr(jt_out), ptr(g_out), stream_handle(stream))
+              ptr(bc_mask), ptr(p.inc_off), ptr(p.inc_pos), ptr(jt_out), ptr(g_out), ptr(p.work),
+              stream_handle(stream))
     return jt_out, g_out
 
 

@@ -94,6 +94,10 @@ int fx_map_export_jac(const fx_map* map, fx_index* h_jac_offsets, fx_index* h_ja
  * element's three columns.  FX_ERR_UNSUPPORTED for other shapes or rows
  * longer than 255. */
 int fx_map_export_rowinc3(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc);
+/* Generic row incidence: inc_off[num_res+1] and inc_pos[num_inc] -- for
+ * each row, in ascending element order, the position res_offsets[e] + li of
+ * its (element, local row) in the flattened element DOF table. */
+int fx_map_export_rowinc(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc_pos);
 /* Row incidence for elements with 15 local rows and columns (Taylor-Hood):
  * inc_off[num_res+1], inc_elem[num_inc] and inc_meta[4*num_inc] -- 16 bytes
  * per incidence: the positions (within the row, from row_ptr[row]) of the
@@ -107,6 +111,7 @@ int fx_map_export_rowinc15(const fx_map* map, fx_index* h_inc_off, fx_index* h_i
  * < 12), same column positions -- or a single row.  blk_row[num_blocks+1],
  * blk_flags[num_blocks] (1 = pairs); max_block_nnz sizes the LDS row tile.
  * Pass null arrays to query num_blocks and max_block_nnz. */
+#define FX_TH_BLOCK_ITEMS 32 /* work items per block the kernels are tuned for */
 int fx_map_th_blocks(const fx_map* map, fx_index lanes, fx_index* num_blocks,
                      fx_index* max_block_nnz, fx_index* h_blk_row, fx_index* h_blk_flags);
 void fx_map_destroy(fx_map* map);
@@ -234,7 +239,8 @@ int fx_p1_vjp(fx_index num_nodes, fx_index num_elems, const double* d_x, const d
  * Newton Jacobian on the 7-point degree-5 rule, viscosity nu (or nu_e[E] per
  * element when non-null), vertex coordinates xv/yv, vconn3[3E] vertex ids,
  * dofs15[15E] element DOFs (fx_th_dofs order), tables from
- * fx_map_export_rowinc15 and fx_map_th_blocks (64 lanes).  vals may be NULL
+ * fx_map_export_rowinc15 and fx_map_th_blocks (FX_TH_BLOCK_ITEMS items per
+ * block; any block size is correct, larger blocks need a larger LDS tile).  vals may be NULL
  * (residual only).  Sums over elements in ascending element order, no
  * atomics; arithmetic uses fused multiply-adds. */
 int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks,
@@ -247,14 +253,92 @@ int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks,
 /* Navier-Stokes adjoint VJP: jt = J^T lam over the row-replaced Newton
  * Jacobian (lam zeroed on constrained rows, passed through there; bc_mask may
  * be NULL) and, when g is non-null, g[e] = lam~_e^T dR_e/dnu_e, the
- * per-element viscosity sensitivity.  Same tables as fx_th_assemble; the
- * contract of Residual::applyParamJacT (femx/state/Residual.hpp:55-59). */
-int fx_th_vjp(fx_index ndof, fx_index num_elems, fx_index num_blocks, const double* d_xv,
-              const double* d_yv, const fx_index* d_vconn3, const fx_index* d_dofs15,
-              const double* d_state, double nu, const double* d_nu_e, const double* d_lam,
-              const fx_index* d_bc_mask, const fx_index* d_inc_off, const fx_index* d_inc_elem,
-              const fx_index* d_inc_meta, const fx_index* d_blk_row, const fx_index* d_blk_flags,
-              double* d_jt, double* d_g, void* stream);
+ * per-element viscosity sensitivity.  inc_off/inc_pos from
+ * fx_map_export_rowinc; work holds 15*num_elems doubles (element column
+ * contributions, summed per DOF in ascending element order).  The contract
+ * of Residual::applyParamJacT (femx/state/Residual.hpp:55-59). */
+int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* d_xv, const double* d_yv,
+              const fx_index* d_vconn3, const fx_index* d_dofs15, const double* d_state,
+              double nu, const double* d_nu_e, const double* d_lam, const fx_index* d_bc_mask,
+              const fx_index* d_inc_off, const fx_index* d_inc_pos, double* d_jt, double* d_g,
+              double* d_work, void* stream);
+
+/* ======================================================================== */
+/* Batches (device).  Many P1 Poisson and Taylor-Hood instances of uneven size
+ * assembled and differentiated by persistent launches over one work list
+ * per phase, optionally captured in a hipGraph and replayed with one launch.
+ * femx has no batching; each instance's results are bitwise equal to its
+ * single-instance calls (fx_p1_assemble with kappa_is_log = 1 and fx_p1_vjp;
+ * fx_th_assemble and fx_th_vjp).  Instance arrays are caller-owned device
+ * memory and must outlive the batch.  VJP pointers (lam, jt, g, work,
+ * inc_pos) may be NULL to batch assembly only. */
+typedef struct
+{
+  fx_index      num_nodes, num_elems, nnz, max_row_len;
+  const double* x;
+  const double* y;
+  const fx_index* conn;
+  const double* m; /* log kappa per element */
+  const double* f;
+  const double* u;
+  const fx_index* inc_off;
+  const fx_index* inc; /* fx_map_export_rowinc3 */
+  const fx_index* row_ptr;
+  double*       vals;
+  double*       res;
+  const double* lam;
+  const fx_index* bc_mask;
+  double*       jt;
+  double*       g;
+} fx_p1_instance;
+
+typedef struct
+{
+  fx_index      ndof, num_elems, num_blocks, max_block_nnz;
+  const double* xv;
+  const double* yv;
+  const fx_index* vconn3;
+  const fx_index* dofs15;
+  const double* state;
+  double        nu;
+  const double* nu_e;
+  const fx_index* inc_off;
+  const fx_index* inc_elem;
+  const fx_index* inc_meta;
+  const fx_index* blk_row;
+  const fx_index* blk_flags;
+  const fx_index* row_ptr;
+  double*       vals;
+  double*       res;
+  const fx_index* inc_pos; /* fx_map_export_rowinc */
+  const double* lam;
+  const fx_index* bc_mask;
+  double*       jt;
+  double*       g;
+  double*       work; /* 15 * num_elems doubles */
+} fx_th_instance;
+
+typedef struct fx_batch fx_batch;
+enum
+{
+  FX_BATCH_ASSEMBLE = 1,
+  FX_BATCH_VJP      = 2
+};
+
+/* grid_per_cu: persistent workgroups per compute unit (the launch grid is
+ * compute units x grid_per_cu).  Copies the descriptors to the device. */
+int fx_batch_create(fx_index num_p1, const fx_p1_instance* h_p1, fx_index num_th,
+                    const fx_th_instance* h_th, fx_index grid_per_cu, fx_batch** out);
+/* Work items per phase (P1 assembly, TH assembly, P1 VJP, TH VJP element,
+ * TH VJP gather) and the persistent grid size. */
+int fx_batch_work(const fx_batch* batch, fx_index* h_counts5, fx_index* h_grid);
+/* Launch the phases in `what` (FX_BATCH_ASSEMBLE | FX_BATCH_VJP) on stream. */
+int fx_batch_run(fx_batch* batch, int what, void* stream);
+/* Capture the phases in `what` into a hipGraph (on a private stream). */
+int fx_batch_capture(fx_batch* batch, int what);
+/* Replay the captured graph on stream (one launch). */
+int fx_batch_launch(fx_batch* batch, void* stream);
+void fx_batch_destroy(fx_batch* batch);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,73 @@
+"""Batched persistent launches and their hipGraph replay against the plain
+per-instance calls: every instance's outputs must be bitwise equal, on every
+replay, with eager work between replays."""
+import numpy as np
+import pytest
+import torch
+
+from femx_amd import batch as B
+
+pytestmark = pytest.mark.gpu
+
+
+def snapshot(cases):
+    torch.cuda.synchronize()
+    return [[t.detach().cpu().numpy().copy() for t in c.outputs()] for c in cases]
+
+
+def clear(cases):
+    for c in cases:
+        for t in c.outputs():
+            t.fill_(float("nan"))
+
+
+def same(a, b):
+    return all(np.array_equal(x, y, equal_nan=False) for xa, ya in zip(a, b) for x, y in zip(xa, ya))
+
+
+@pytest.mark.parametrize("grid_per_cu", [1, 3])
+def test_batch_matches_plain_bitwise(grid_per_cu):
+    specs = B.instance_specs(count=10, seed=9, n_min=3, n_max=40)
+    cases = B.build_cases(specs, "cuda")
+    B.plain(cases, 3)
+    ref = snapshot(cases)
+    assert all(np.isfinite(x).all() for xa in ref for x in xa)
+    b = B.Batch(cases, grid_per_cu)
+    counts, grid = b.work()
+    assert grid > 0 and all(c > 0 for c in counts)
+    clear(cases)
+    b.run(3)
+    assert same(snapshot(cases), ref)
+    b.capture(3)
+    for k in range(3):
+        clear(cases)
+        b.launch()
+        assert same(snapshot(cases), ref), f"graph replay {k}"
+        B.plain(cases[:2], 1)  # eager work between replays
+
+
+def test_batch_assembly_only_and_autotune():
+    specs = B.instance_specs(count=6, seed=3, n_min=4, n_max=20)
+    cases = B.build_cases(specs, "cuda")
+    B.plain(cases, 1)
+    ref = [[t.detach().cpu().numpy().copy() for t in c.outputs()[:2]] for c in cases]
+    b = B.Batch(cases, 2)
+    for c in cases:
+        for t in c.outputs()[:2]:
+            t.fill_(float("nan"))
+    b.capture(1)
+    b.launch()
+    torch.cuda.synchronize()
+    got = [[t.detach().cpu().numpy() for t in c.outputs()[:2]] for c in cases]
+    assert same(got, ref)
+    best, timings = B.autotune(cases, (1, 2), reps=2)
+    assert best in (1, 2) and all(t > 0 for t in timings.values())
+
+
+def test_instance_specs_follow_config5():
+    s = B.instance_specs()
+    assert len(s) == 256
+    assert all(x["kind"] == ("p1" if i % 2 == 0 else "th") for i, x in enumerate(s))
+    ns = np.array([x["n"] for x in s])
+    assert ns.min() >= 16 and ns.max() <= 512
+    assert all(10.0 <= x["Re"] <= 400.0 for x in s if x["kind"] == "th")

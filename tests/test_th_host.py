@@ -47,7 +47,7 @@ def test_rowinc15_and_blocks(n):
     nv2 = 2 * g.num_vel_nodes
     for b in range(plan.num_blocks):
         items = (br[b + 1] - br[b]) // (2 if bf[b] else 1)
-        assert items <= ns.TH_LANES
+        assert items <= ns.TH_ITEMS
         if bf[b]:
             assert br[b + 1] <= nv2 and br[b] % 2 == 0
         assert rp[br[b + 1]] - rp[br[b]] <= plan.max_block_nnz
