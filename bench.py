@@ -84,6 +84,45 @@ def build_th(n, dev, t):
             "vjp_alg_bytes": th_vjp_alg_bytes(g.num_vertices, g.num_elems, g.ndof)}
 
 
+def run_batched(args, log):
+    """Config 5: 256 instances, n ~ logU[16,512] (seed 9), even P1 Poisson,
+    odd Taylor-Hood NS (Re ~ U[10,400]); assembly + VJP of every instance.
+    Plain = one assembly and one VJP call per instance (femx's pattern);
+    batched = the autotuned persistent launches replayed as one hipGraph."""
+    from femx_amd import batch as B
+    t0 = time.perf_counter()
+    specs = B.instance_specs(count=args.batch_count, seed=9)
+    cases = B.build_cases(specs, "cuda", log=log)
+    log(f"[bench] batch built in {time.perf_counter() - t0:.1f} s")
+    E = sum(c.elements for c in cases)
+    b_alg = 0
+    for c in cases:
+        if isinstance(c, B.P1Case):
+            g = c.geom
+            b_alg += p1_alg_bytes(g.num_nodes, g.num_elems, c.map.pattern.nnz)
+            b_alg += vjp_alg_bytes(g.num_nodes, g.num_elems)
+        else:
+            g = c.geom
+            b_alg += th_alg_bytes(g.num_vertices, g.num_elems, g.ndof, c.map.pattern.nnz)
+            b_alg += th_vjp_alg_bytes(g.num_vertices, g.num_elems, g.ndof)
+    best, tune = B.autotune(cases, (1, 2, 4, 8), reps=3)
+    log(f"[bench] autotuned grid_per_cu={best} {tune}")
+    batch = B.Batch(cases, best)
+    batch.capture(3)
+    t_graph, _ = B.time_fn(batch.launch, 10)
+    t_run, _ = B.time_fn(lambda: batch.run(3), 10)
+    t_plain, wall_plain = B.time_fn(lambda: B.plain(cases, 3), 3)
+    counts, grid = batch.work()
+    return {"instances": len(cases), "elements": E, "alg_bytes": b_alg,
+            "graph_ms": t_graph * 1e3, "persistent_ms": t_run * 1e3, "plain_ms": t_plain * 1e3,
+            "plain_wall_ms": wall_plain * 1e3,
+            "elements_per_s_graph": E / t_graph, "elements_per_s_plain": E / t_plain,
+            "speedup_graph_vs_plain": t_plain / t_graph,
+            "GBs_graph": b_alg / t_graph / 1e9, "frac_graph": b_alg / t_graph / 1e9 / HBM_PEAK_GBS,
+            "grid_per_cu": best, "autotune_ms": {str(k): v * 1e3 for k, v in tune.items()},
+            "work_items": counts, "grid": grid}
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -98,6 +137,8 @@ def main():
     ap.add_argument("--no-vjp", action="store_true")
     ap.add_argument("--no-th", action="store_true")
     ap.add_argument("--th-n", type=int, default=512)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--kernels-only", action="store_true",
                     help="for rocprofv3 counter runs: a 1 GiB device copy (counter "
                          "calibration), then 5 launches each of assembly, SpMV, elimination "
@@ -261,6 +302,9 @@ def main():
             "GBs": b_th / t_th / 1e9, "frac": b_th / t_th / 1e9 / HBM_PEAK_GBS,
             "vjp_us": t_thv * 1e6, "VJPs_per_s": 1.0 / t_thv, "vjp_alg_bytes": b_thv,
             "vjp_GBs": b_thv / t_thv / 1e9, "vjp_frac": b_thv / t_thv / 1e9 / HBM_PEAK_GBS}
+
+    if not args.no_batch:
+        extra["batched_%d" % args.batch_count] = run_batched(args, log)
 
     cpu = None
     if not args.no_cpu:
