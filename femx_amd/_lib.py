@@ -29,7 +29,7 @@ _SIGS = {
     "fx_map_export_rowinc3": ([_vp, _vp, _vp], C.c_int),
     "fx_map_export_rowinc": ([_vp, _vp, _vp], C.c_int),
     "fx_map_export_rowinc15": ([_vp, _vp, _vp, _vp], C.c_int),
-    "fx_map_th_blocks": ([_vp, _i32, _vp, _vp, _vp, _vp], C.c_int),
+    "fx_map_th_blocks": ([_vp, _i32, _i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_map_destroy": ([_vp], None),
     "fx_csr_transpose_map": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "fx_bcmap_create": ([_i32, _vp, _vp, _i32, _vp, C.POINTER(_vp)], C.c_int),

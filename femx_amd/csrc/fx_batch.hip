@@ -24,6 +24,7 @@ struct fx_batch
   int         num_p1 = 0, num_th = 0;
   int         grid   = 0;
   std::size_t p1_lds = 0, th_lds = 0;
+  int         th_tile = 0; // row tile per wave of the Taylor-Hood assembly (doubles)
   // device memory
   fxk::P1AsmArgs*  d_p1_asm  = nullptr;
   fxk::P1VjpArgs*  d_p1_vjp  = nullptr;
@@ -88,17 +89,20 @@ __global__ __launch_bounds__(kP1Block) void k_batch_p1_vjp(const P1VjpArgs* __re
   }
 }
 
-__global__ __launch_bounds__(kThLanes) void k_batch_th_assemble(const ThAsmArgs* __restrict__ inst,
-                                                               const int2* __restrict__ work,
-                                                               int nwork)
+// Each wave walks the work list on its own (stride: all waves of the grid)
+// with its own row tile of `tile` doubles; the workgroup shares the tables.
+__global__ __launch_bounds__(kThLanes* kThWaves) void k_batch_th_assemble(
+    const ThAsmArgs* __restrict__ inst, const int2* __restrict__ work, int nwork, int tile)
 {
   extern __shared__ double smem[];
-  ThTables& T = *reinterpret_cast<ThTables*>(smem);
-  th_build_tables(T);
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+  ThTensor& T = *reinterpret_cast<ThTensor*>(smem);
+  th_stage_tensor(T, kThLanes * kThWaves);
+  __syncthreads();
+  const int w = threadIdx.x / kThLanes;
+  for (int i = blockIdx.x * kThWaves + w; i < nwork; i += gridDim.x * kThWaves)
   {
-    const int2 it = work[w];
-    th_assemble_block(it.y, inst[it.x], T, smem + sizeof(ThTables) / sizeof(double));
+    const int2 it = work[i];
+    th_assemble_block(it.y, inst[it.x], T, smem + kThTensorLen + w * tile);
   }
 }
 
@@ -147,8 +151,8 @@ void launch_phases(fx_batch* b, int what, hipStream_t s)
       hipLaunchKernelGGL(k_batch_p1_assemble, g, dim3(kP1Block), b->p1_lds, s, b->d_p1_asm,
                          b->d_work[kP1Asm], b->nwork[kP1Asm]);
     if (b->nwork[kThAsm])
-      hipLaunchKernelGGL(k_batch_th_assemble, g, dim3(kThLanes), b->th_lds, s, b->d_th_asm,
-                         b->d_work[kThAsm], b->nwork[kThAsm]);
+      hipLaunchKernelGGL(k_batch_th_assemble, g, dim3(kThLanes * kThWaves), b->th_lds, s,
+                         b->d_th_asm, b->d_work[kThAsm], b->nwork[kThAsm], b->th_tile);
     fx::check_launch("fx_batch: assembly");
   }
   if (what & FX_BATCH_VJP)
@@ -242,7 +246,8 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
         for (int k = 0; k < nd; ++k) w[kThVjpGather].push_back(make_int2(i, k));
       }
     }
-    b->th_lds = sizeof(ThTables) + static_cast<std::size_t>(max_blk_nnz) * 8;
+    b->th_tile = static_cast<int>(max_blk_nnz);
+    b->th_lds  = sizeof(ThTensor) + static_cast<std::size_t>(max_blk_nnz) * 8 * kThWaves;
     fx::require(b->p1_lds <= 160 * 1024 && b->th_lds <= 160 * 1024, FX_ERR_UNSUPPORTED,
                 "fx_batch_create: LDS tile exceeds the workgroup limit");
     b->d_p1_asm = upload(pa);

@@ -394,14 +394,16 @@ int fx_map_export_rowinc15(const fx_map* m, Index* inc_off, Index* inc_elem, Ind
 // when the two rows are interchangeable for the kernel -- same incident
 // elements in the same order, local rows li and li+1 with li even and < 12,
 // identical column positions -- and a single row otherwise.  Blocks hold
-// contiguous rows.  blk_row[num_blocks+1] are block row starts, blk_flags
-// the item kind (1 = pairs, 0 = single rows); max_block_nnz bounds the LDS
-// row tile.  Call with null arrays to get the counts.
-int fx_map_th_blocks(const fx_map* m, Index lanes, Index* num_blocks, Index* max_block_nnz,
-                     Index* blk_row, Index* blk_flags)
+// contiguous rows and, when max_nnz > 0, at most max_nnz CSR entries (a block
+// always takes at least one item).  blk_row[num_blocks+1] are block row
+// starts, blk_flags the item kind (1 = pairs, 0 = single rows);
+// max_block_nnz bounds the LDS row tile.  Call with null arrays to get the
+// counts.
+int fx_map_th_blocks(const fx_map* m, Index lanes, Index max_nnz, Index* num_blocks,
+                     Index* max_block_nnz, Index* blk_row, Index* blk_flags)
 {
   return fx::guard([&] {
-    fx::require(m && num_blocks && max_block_nnz && lanes > 0, FX_ERR_ARG,
+    fx::require(m && num_blocks && max_block_nnz && lanes > 0 && max_nnz >= 0, FX_ERR_ARG,
                 "fx_map_th_blocks: invalid arguments");
     auto pairable = [&](Index r) {
       if (r + 1 >= m->num_res) return false;
@@ -425,12 +427,14 @@ int fx_map_th_blocks(const fx_map* m, Index lanes, Index* num_blocks, Index* max
     Index              r = 0, mx = 0;
     while (r < m->num_res)
     {
-      const bool pair  = pairable(r);
+      const bool  pair  = pairable(r);
       const Index start = r;
       Index       items = 0;
       while (r < m->num_res && items < lanes && pairable(r) == pair)
       {
-        r += pair ? 2 : 1;
+        const Index next = r + (pair ? 2 : 1);
+        if (items > 0 && max_nnz > 0 && m->row_ptr[next] - m->row_ptr[start] > max_nnz) break;
+        r = next;
         ++items;
       }
       starts.push_back(start);

@@ -168,9 +168,12 @@ __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, d
       rsum = rsum + re;
       if (a.vals)
       {
-        racc[(meta >> 8) & 0xff]  = racc[(meta >> 8) & 0xff] + k0;
-        racc[(meta >> 16) & 0xff] = racc[(meta >> 16) & 0xff] + k1;
-        racc[meta >> 24]          = racc[meta >> 24] + k2;
+        // distinct columns: read all three, then write (no dependent round trips)
+        const int    p0 = (meta >> 8) & 0xff, p1 = (meta >> 16) & 0xff, p2 = meta >> 24;
+        const double o0 = racc[p0], o1 = racc[p1], o2 = racc[p2];
+        racc[p0]        = o0 + k0;
+        racc[p1]        = o1 + k1;
+        racc[p2]        = o2 + k2;
       }
     }
     __builtin_nontemporal_store(rsum, a.res + r);

@@ -6,12 +6,18 @@ namespace
 {
 using namespace fxk;
 
-__global__ __launch_bounds__(kThLanes) void k_th_assemble(ThAsmArgs a)
+// kThWaves waves per workgroup, one row block each, consecutive row blocks
+// per XCD; each wave has a row tile of `tile` doubles after the tables.
+__global__ __launch_bounds__(kThLanes* kThWaves) void k_th_assemble(ThAsmArgs a, int num_blocks,
+                                                                    int tile)
 {
   extern __shared__ double smem[];
-  ThTables& T = *reinterpret_cast<ThTables*>(smem);
-  th_build_tables(T);
-  th_assemble_block(blockIdx.x, a, T, smem + sizeof(ThTables) / sizeof(double));
+  ThTensor& T = *reinterpret_cast<ThTensor*>(smem);
+  th_stage_tensor(T, kThLanes * kThWaves);
+  __syncthreads();
+  const int w = threadIdx.x / kThLanes;
+  const int b = fx::xcd_block(gridDim.x) * kThWaves + w;
+  if (b < num_blocks) th_assemble_block(b, a, T, smem + kThTensorLen + w * tile);
 }
 
 __global__ __launch_bounds__(kThLanes) void k_th_vjp_elem(ThVjpArgs a)
@@ -49,12 +55,14 @@ int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks, fx_in
     fx::require(!fx::overlaps(res, std::size_t(ndof) * 8, state, std::size_t(ndof) * 8)
                     && !(vals && fx::overlaps(vals, 8, res, std::size_t(ndof) * 8)),
                 FX_ERR_ALIAS, "Assembly state, residual, and matrix values must not alias");
-    const size_t lds = sizeof(ThTables) + static_cast<size_t>(vals ? max_block_nnz : 0) * 8;
+    const int    tile = vals ? max_block_nnz : 0;
+    const size_t lds  = sizeof(ThTensor) + static_cast<size_t>(tile) * 8 * kThWaves;
     fx::require(lds <= 160 * 1024, FX_ERR_UNSUPPORTED, "fx_th_assemble: row tile exceeds LDS");
     const ThAsmArgs a{xv,      yv,       vconn3,   dofs15,   state,
                       nu,      nu_e,     inc_off,  inc_elem, reinterpret_cast<const uint4*>(inc_meta),
                       blk_row, blk_flags, row_ptr, vals,     res};
-    hipLaunchKernelGGL(k_th_assemble, dim3(num_blocks), dim3(kThLanes), lds, fx::as_stream(stream), a);
+    hipLaunchKernelGGL(k_th_assemble, dim3(fx::blocks_for(num_blocks, kThWaves)),
+                       dim3(kThLanes * kThWaves), lds, fx::as_stream(stream), a, num_blocks, tile);
     fx::check_launch("fx_th_assemble");
   });
 }

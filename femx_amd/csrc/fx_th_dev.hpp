@@ -38,6 +38,17 @@ namespace fxk
 
 
 constexpr int kThLanes = 64;
+constexpr int kThWaves = 4; // waves per assembly workgroup, sharing one copy of the tables
+
+// Order the LDS accesses of one wave before and after this point (a wave's
+// LDS instructions execute in issue order; this keeps the compiler from
+// moving them across).
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 struct ThTables
 {
@@ -141,92 +152,264 @@ __device__ __forceinline__ void th_grads(const ThTables& T, int q, const ThGeom&
   }
 }
 
-// Velocity rows (a,0) and (a,1) of the element Jacobian and residual.
-__device__ __forceinline__ void th_velocity_rows(const ThTables& T, const ThElem& E, int a, double* Kx,
-                                 double* Ky, double& Rx, double& Ry)
+// ---------------------------------------------------------------------------
+// Assembly in reference-tensor form.  On an affine triangle every integrand
+// of the Jacobian is a polynomial times products of the constant inverse
+// map, so the 7-point sums factor into reference tensors (the rule applied
+// on the reference triangle, once, at compile time) contracted with a few
+// per-element coefficients.  With H = |det J| J^-1 (the signed adjugate):
+//   stiffness  int grad phi_a . grad phi_b = sum_rs G_rs A^rs_ab,
+//              G = H H^T / |det J|
+//   convection int phi_a (u . grad phi_b)  = sum_rc V^r_c T^r_acb,
+//              V^r_c = sum_d u_(c,d) H_rd (reference-frame nodal velocity)
+//   reaction   int phi_a phi_b d_d u_c     = sum_v W^cd_v Q_abv,
+//              W^cd_v = |det J| (d_d u_c)(vertex v) (grad u is linear)
+//   pressure   -int psi_b' d_d phi_a       = -sum_r H_rd P^r_b'a
+// with A^rs_ab = sum_q w_q dN_a/dr_r dN_b/dr_s, T^r_acb = sum_q w_q N_a N_c
+// dN_b/dr_r, Q_abv = sum_q w_q N_a N_b lambda_v, P^r_b'a = sum_q w_q psi_b'
+// dN_a/dr_r.  This is the same 7-point quadrature, summed in another order.
+
+// Row a of the (a, b) tensors is read with a differing per lane; its stride
+// of 122 doubles (244 dwords) puts the six rows' reads in distinct LDS banks.
+constexpr int kThRowStride = 122;
+
+struct ThTensor
 {
+  // Row a, block b at [20 b ..]: T^r_acb at [6 r + c] (12), Q_abv at [12 + v]
+  // (3), pad, A^00_ab, A^11_ab, A^01_ab + A^10_ab at [16..18], pad.
+  double ab[6][kThRowStride];
+  double pa[6][6]; // P^r_b'a at [a][2 b' + r]
+};
+constexpr int kThTensorLen = static_cast<int>(sizeof(ThTensor) / sizeof(double));
+
+constexpr double kSqrt15 = 3.872983346207417; // nearest double to sqrt(15)
+
+constexpr double p2_shape(int a, double l0, double l1, double l2)
+{
+  switch (a)
+  {
+  case 0: return l0 * (2.0 * l0 - 1.0);
+  case 1: return l1 * (2.0 * l1 - 1.0);
+  case 2: return l2 * (2.0 * l2 - 1.0);
+  case 3: return 4.0 * l0 * l1;
+  case 4: return 4.0 * l1 * l2;
+  default: return 4.0 * l2 * l0;
+  }
+}
+
+// d N_a / d r (dir 0) or d N_a / d s (dir 1), with l0 = 1 - r - s, l1 = r, l2 = s.
+constexpr double p2_dshape(int a, int dir, double l0, double l1, double l2)
+{
+  switch (a)
+  {
+  case 0: return -(4.0 * l0 - 1.0);
+  case 1: return dir == 0 ? 4.0 * l1 - 1.0 : 0.0;
+  case 2: return dir == 0 ? 0.0 : 4.0 * l2 - 1.0;
+  case 3: return dir == 0 ? 4.0 * (l0 - l1) : -4.0 * l1;
+  case 4: return dir == 0 ? 4.0 * l2 : 4.0 * l1;
+  default: return dir == 0 ? -4.0 * l2 : 4.0 * (l0 - l2);
+  }
+}
+
+constexpr ThTensor make_th_tensor()
+{
+  const double a1 = (6.0 - kSqrt15) / 21.0, b1 = (9.0 + 2.0 * kSqrt15) / 21.0;
+  const double a2 = (6.0 + kSqrt15) / 21.0, b2 = (9.0 - 2.0 * kSqrt15) / 21.0;
+  const double w1 = (155.0 - kSqrt15) / 2400.0, w2 = (155.0 + kSqrt15) / 2400.0;
+  const double qr[7] = {1.0 / 3.0, a1, b1, a1, a2, b2, a2};
+  const double qs[7] = {1.0 / 3.0, a1, a1, b1, a2, a2, b2};
+  const double qw[7] = {9.0 / 80.0, w1, w1, w1, w2, w2, w2};
+  ThTensor t{};
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b)
+    {
+      double* e = t.ab[a] + 20 * b;
+      for (int q = 0; q < 7; ++q)
+      {
+        const double l1 = qr[q], l2 = qs[q], l0 = 1.0 - l1 - l2, w = qw[q];
+        const double lam[3] = {l0, l1, l2};
+        const double na = p2_shape(a, l0, l1, l2), nb = p2_shape(b, l0, l1, l2);
+        const double ar = p2_dshape(a, 0, l0, l1, l2), as = p2_dshape(a, 1, l0, l1, l2);
+        const double br = p2_dshape(b, 0, l0, l1, l2), bs = p2_dshape(b, 1, l0, l1, l2);
+        for (int c = 0; c < 6; ++c)
+        {
+          const double wnn = w * na * p2_shape(c, l0, l1, l2);
+          e[c]             = e[c] + wnn * br;
+          e[6 + c]         = e[6 + c] + wnn * bs;
+        }
+        for (int v = 0; v < 3; ++v) e[12 + v] = e[12 + v] + w * na * nb * lam[v];
+        e[16] = e[16] + w * ar * br;
+        e[17] = e[17] + w * as * bs;
+        e[18] = e[18] + (w * ar * bs + w * as * br);
+      }
+    }
+  for (int a = 0; a < 6; ++a)
+    for (int q = 0; q < 7; ++q)
+    {
+      const double l1 = qr[q], l2 = qs[q], l0 = 1.0 - l1 - l2, w = qw[q];
+      const double lam[3] = {l0, l1, l2};
+      for (int bp = 0; bp < 3; ++bp)
+        for (int r = 0; r < 2; ++r)
+          t.pa[a][2 * bp + r] = t.pa[a][2 * bp + r] + w * lam[bp] * p2_dshape(a, r, l0, l1, l2);
+    }
+  return t;
+}
+
+__device__ const ThTensor kThTensor = make_th_tensor();
+
+// Stage the reference tensors into LDS (all lanes; caller synchronises).
+__device__ __forceinline__ void th_stage_tensor(ThTensor& T, int nthreads)
+{
+  const double* src = &kThTensor.ab[0][0];
+  double*       dst = &T.ab[0][0];
+  for (int i = threadIdx.x; i < kThTensorLen; i += nthreads) dst[i] = src[i];
+}
+
+// Element data for assembly: the 15 local state values, H = |det J| J^-1,
+// |det J| and the element viscosity.
+struct ThAff
+{
+  double h00, h01, h10, h11, adet;
+};
+
+struct ThElemT
+{
+  double s[15];
+  ThAff  g;
+  double nu;
+};
+
+__device__ __forceinline__ void th_gather_t(ThElemT& E, int e, const double* __restrict__ xv,
+                                            const double* __restrict__ yv,
+                                            const int* __restrict__ vconn,
+                                            const int* __restrict__ dofs,
+                                            const double* __restrict__ state, double nu,
+                                            const double* __restrict__ nu_e)
+{
+  const int* d = dofs + 15 * static_cast<long>(e);
 #pragma unroll
-  for (int j = 0; j < 15; ++j) Kx[j] = 0.0, Ky[j] = 0.0;
+  for (int k = 0; k < 15; ++k) E.s[k] = state[d[k]];
+  const int*   v  = vconn + 3 * static_cast<long>(e);
+  const double x0 = xv[v[0]], y0 = yv[v[0]];
+  const double j00 = xv[v[1]] - x0, j01 = xv[v[2]] - x0, j10 = yv[v[1]] - y0, j11 = yv[v[2]] - y0;
+  const double det = j00 * j11 - j01 * j10;
+  const double sg  = det < 0.0 ? -1.0 : 1.0;
+  E.g              = {sg * j11, -sg * j01, -sg * j10, sg * j00, fabs(det)};
+  E.nu             = nu_e ? nu_e[e] : nu;
+}
+
+// Velocity rows (a,0) and (a,1) of the element Jacobian and residual:
+//   K_(a,c),(b,d) = delta_cd (nu A_ab + C_ab) + M^cd_ab,
+//   K_(a,c),b'    = -int psi_b' d_c phi_a,
+//   R_(a,c)       = sum_b (nu A_ab + C_ab) u_(b,c) + sum_b' K_(a,c),b' p_b'.
+__device__ __forceinline__ void th_velocity_rows(const ThTensor& T, const ThElemT& E, int a,
+                                                 double* Kx, double* Ky, double& Rx, double& Ry)
+{
+  const ThAff&  g   = E.g;
+  const double* s   = E.s;
+  const double  nui = E.nu / g.adet;
+  const double  G00 = nui * fma(g.h00, g.h00, g.h01 * g.h01);
+  const double  G11 = nui * fma(g.h10, g.h10, g.h11 * g.h11);
+  const double  G01 = nui * fma(g.h00, g.h10, g.h01 * g.h11);
+  double        V[12];
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+  {
+    V[c]     = fma(s[2 * c], g.h00, s[2 * c + 1] * g.h01);
+    V[6 + c] = fma(s[2 * c], g.h10, s[2 * c + 1] * g.h11);
+  }
+  // Reference gradients of u_c at the vertices (exact for P2), then W^cd_v.
+  double W[4][3];
+#pragma unroll
+  for (int c = 0; c < 2; ++c)
+  {
+    const double u0 = s[c], u1 = s[2 + c], u2 = s[4 + c], u3 = s[6 + c], u4 = s[8 + c],
+                 u5 = s[10 + c];
+    const double er[3] = {4.0 * u3 - 3.0 * u0 - u1, (u0 + 3.0 * u1) - 4.0 * u3,
+                          (u0 - u1) + 4.0 * (u4 - u5)};
+    const double es[3] = {4.0 * u5 - 3.0 * u0 - u2, (u0 - u2) + 4.0 * (u4 - u3),
+                          (u0 + 3.0 * u2) - 4.0 * u5};
+#pragma unroll
+    for (int v = 0; v < 3; ++v)
+    {
+      W[2 * c][v]     = fma(er[v], g.h00, es[v] * g.h10);
+      W[2 * c + 1][v] = fma(er[v], g.h01, es[v] * g.h11);
+    }
+  }
   Rx = 0.0, Ry = 0.0;
-  const double nu = E.nu;
-#pragma unroll 1
-  for (int q = 0; q < 7; ++q)
+  const double* ta = T.ab[a];
+#pragma unroll
+  for (int b = 0; b < 6; ++b)
   {
-    double dNx[6], dNy[6];
-    th_grads(T, q, E.g, dNx, dNy);
-    const double wq = E.g.adet * T.w[q];
-    double       u0 = 0.0, u1 = 0.0, d00 = 0.0, d01 = 0.0, d10 = 0.0, d11 = 0.0, p = 0.0;
+    const double* e = ta + 20 * b;
+    double        L = fma(G00, e[16], fma(G11, e[17], G01 * e[18]));
 #pragma unroll
-    for (int b = 0; b < 6; ++b)
-    {
-      const double nb = T.N[q][b], sx = E.s[2 * b], sy = E.s[2 * b + 1];
-      u0 = fma(nb, sx, u0), u1 = fma(nb, sy, u1);
-      d00 = fma(dNx[b], sx, d00), d01 = fma(dNy[b], sx, d01);
-      d10 = fma(dNx[b], sy, d10), d11 = fma(dNy[b], sy, d11);
-    }
+    for (int k = 0; k < 12; ++k) L = fma(V[k], e[k], L);
+    double M[4];
 #pragma unroll
-    for (int b = 0; b < 3; ++b) p = fma(T.psi[q][b], E.s[12 + b], p);
-    const double na  = T.N[q][a];
-    const double gax = fma(T.dNdr[q][2 * a], E.g.i00, T.dNdr[q][2 * a + 1] * E.g.i10);
-    const double gay = fma(T.dNdr[q][2 * a], E.g.i01, T.dNdr[q][2 * a + 1] * E.g.i11);
-    const double nwq = na * wq;
-    Rx = fma(fma(nu, fma(gax, d00, gay * d01), na * fma(u0, d00, u1 * d01)) - p * gax, wq, Rx);
-    Ry = fma(fma(nu, fma(gax, d10, gay * d11), na * fma(u0, d10, u1 * d11)) - p * gay, wq, Ry);
-    const double nugx = nu * gax * wq, nugy = nu * gay * wq;
+    for (int cd = 0; cd < 4; ++cd) M[cd] = fma(W[cd][0], e[12], fma(W[cd][1], e[13], W[cd][2] * e[14]));
+    Kx[2 * b]     = L + M[0];
+    Kx[2 * b + 1] = M[1];
+    Ky[2 * b]     = M[2];
+    Ky[2 * b + 1] = L + M[3];
+    Rx            = fma(L, s[2 * b], Rx);
+    Ry            = fma(L, s[2 * b + 1], Ry);
+  }
+  const double* pa = T.pa[a];
 #pragma unroll
-    for (int b = 0; b < 6; ++b)
-    {
-      const double la  = fma(nugx, dNx[b], fma(nugy, dNy[b], nwq * fma(u0, dNx[b], u1 * dNy[b])));
-      const double nn  = nwq * T.N[q][b];
-      Kx[2 * b]        = fma(nn, d00, Kx[2 * b] + la);
-      Kx[2 * b + 1]    = fma(nn, d01, Kx[2 * b + 1]);
-      Ky[2 * b]        = fma(nn, d10, Ky[2 * b]);
-      Ky[2 * b + 1]    = fma(nn, d11, Ky[2 * b + 1] + la);
-    }
-#pragma unroll
-    for (int b = 0; b < 3; ++b)
-    {
-      const double pw = T.psi[q][b] * wq;
-      Kx[12 + b]      = fma(-pw, gax, Kx[12 + b]);
-      Ky[12 + b]      = fma(-pw, gay, Ky[12 + b]);
-    }
+  for (int bp = 0; bp < 3; ++bp)
+  {
+    Kx[12 + bp] = -fma(g.h00, pa[2 * bp], g.h10 * pa[2 * bp + 1]);
+    Ky[12 + bp] = -fma(g.h01, pa[2 * bp], g.h11 * pa[2 * bp + 1]);
+    Rx          = fma(Kx[12 + bp], s[12 + bp], Rx);
+    Ry          = fma(Ky[12 + bp], s[12 + bp], Ry);
   }
 }
 
-// Pressure row b' of the element Jacobian and residual.
-__device__ __forceinline__ void th_pressure_row(const ThTables& T, const ThElem& E, int bp, double* Kp, double& Rp)
+// Pressure row b': K_b',(b,d) = -int psi_b' d_d phi_b, K_b',b'' = 0,
+// R_b' = -int psi_b' div u = sum_(b,d) K_b',(b,d) u_(b,d).
+__device__ __forceinline__ void th_pressure_row(const ThTensor& T, const ThElemT& E, int bp,
+                                                double* Kp, double& Rp)
 {
+  const ThAff& g = E.g;
+  Rp             = 0.0;
 #pragma unroll
-  for (int j = 0; j < 15; ++j) Kp[j] = 0.0;
-  Rp = 0.0;
-#pragma unroll 1
-  for (int q = 0; q < 7; ++q)
+  for (int b = 0; b < 6; ++b)
   {
-    double dNx[6], dNy[6];
-    th_grads(T, q, E.g, dNx, dNy);
-    const double wq = E.g.adet * T.w[q];
-    double       div = 0.0;
-#pragma unroll
-    for (int b = 0; b < 6; ++b) div = fma(dNx[b], E.s[2 * b], fma(dNy[b], E.s[2 * b + 1], div));
-    const double pw = T.psi[q][bp] * wq;
-    Rp              = fma(-pw, div, Rp);
-#pragma unroll
-    for (int b = 0; b < 6; ++b)
-    {
-      Kp[2 * b]     = fma(-pw, dNx[b], Kp[2 * b]);
-      Kp[2 * b + 1] = fma(-pw, dNy[b], Kp[2 * b + 1]);
-    }
+    const double pr = T.pa[b][2 * bp], ps = T.pa[b][2 * bp + 1];
+    Kp[2 * b]       = -fma(g.h00, pr, g.h10 * ps);
+    Kp[2 * b + 1]   = -fma(g.h01, pr, g.h11 * ps);
+    Rp              = fma(Kp[2 * b], E.s[2 * b], fma(Kp[2 * b + 1], E.s[2 * b + 1], Rp));
   }
+#pragma unroll
+  for (int j = 12; j < 15; ++j) Kp[j] = 0.0;
 }
 
-__device__ __forceinline__ void th_add_row(double* acc, uint4 meta, const double* K)
+// Add one element row -- and, for a velocity pair, the partner row, which
+// has the same column positions -- into the LDS tile.  The 15 positions of an
+// element row are distinct CSR entries, so every old value is read before any
+// is written back: the reads issue back to back instead of as dependent
+// read-modify-write round trips, and each entry still gets old + K.
+__device__ __forceinline__ void th_add_rows(double* ax, double* ay, uint4 meta, double* Kx,
+                                            double* Ky, bool pairs)
 {
   const unsigned w[4] = {meta.x, meta.y, meta.z, meta.w};
+  int            pos[15];
 #pragma unroll
-  for (int j = 0; j < 15; ++j)
+  for (int j = 0; j < 15; ++j) pos[j] = (w[j >> 2] >> (8 * (j & 3))) & 0xff;
+#pragma unroll
+  for (int j = 0; j < 15; ++j) Kx[j] = ax[pos[j]] + Kx[j];
+  if (pairs)
   {
-    const int pos = (w[j >> 2] >> (8 * (j & 3))) & 0xff;
-    acc[pos]      = acc[pos] + K[j];
+#pragma unroll
+    for (int j = 0; j < 15; ++j) Ky[j] = ay[pos[j]] + Ky[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 15; ++j) ax[pos[j]] = Kx[j];
+  if (pairs)
+  {
+#pragma unroll
+    for (int j = 0; j < 15; ++j) ay[pos[j]] = Ky[j];
   }
 }
 
@@ -251,24 +434,27 @@ struct ThAsmArgs
   double*       res;
 };
 
-// Two lanes per work item: lane role 0 takes the item's incidences t0, t0+2,
-// ..., role 1 takes t0+1, t0+3, ...; after each round role 0 applies its own
-// contribution and then its partner's (fetched by a cross-lane swap), so every
-// CSR value still receives its element contributions in ascending element
-// order, from one thread, in program order.
-__device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, const ThTables& T,
+// One row block per wave.  Two lanes per work item: lane role 0 takes the
+// item's incidences t0, t0+2, ..., role 1 takes t0+1, t0+3, ...; after each
+// round role 0 applies its own contribution and then its partner's (fetched
+// by a cross-lane swap), so every CSR value still receives its element
+// contributions in ascending element order, from one thread, in program
+// order.  `acc` is the wave's own row tile; the waves of a workgroup share
+// only the tables.
+__device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, const ThTensor& T,
                                                   double* acc)
 {
-  const int rs     = a.blk_row[b], re = a.blk_row[b + 1];
+  const int  lane  = threadIdx.x & (kThLanes - 1);
+  const int  rs    = a.blk_row[b], re = a.blk_row[b + 1];
   const bool pairs = a.blk_flags[b] != 0;
-  const int base   = a.row_ptr[rs];
-  const int bn     = a.vals ? a.row_ptr[re] - base : 0;
-  for (int i = threadIdx.x; i < bn; i += kThLanes) acc[i] = 0.0;
-  __syncthreads();
+  const int  base  = a.row_ptr[rs];
+  const int  bn    = a.vals ? a.row_ptr[re] - base : 0;
+  for (int i = lane; i < bn; i += kThLanes) acc[i] = 0.0;
+  wave_sync();
 
-  const int role   = threadIdx.x & 1;
+  const int role   = lane & 1;
   const int nitems = pairs ? (re - rs) >> 1 : re - rs;
-  for (int item = threadIdx.x >> 1; item < nitems; item += kThLanes / 2)
+  for (int item = lane >> 1; item < nitems; item += kThLanes / 2)
   {
     const int r  = rs + (pairs ? 2 * item : item);
     double*   ax = acc + (a.row_ptr[r] - base);
@@ -286,8 +472,8 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
         const int e = a.inc_elem[tm];
         meta        = a.inc_meta[tm];
         const int li = meta.w >> 24;
-        ThElem    E;
-        th_gather(E, e, a.xv, a.yv, a.vconn, a.dofs, a.state, a.nu, a.nu_e);
+        ThElemT   E;
+        th_gather_t(E, e, a.xv, a.yv, a.vconn, a.dofs, a.state, a.nu, a.nu_e);
         if (li < 12)
         {
           th_velocity_rows(T, E, li >> 1, K0, K1, R0, R1);
@@ -307,26 +493,16 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
         for (int j = 0; j < 15; ++j) K0[j] = 0.0, K1[j] = 0.0;
       }
       // Apply in element order: role 0 (incidence t) first, then role 1
-      // (incidence t+1).  The wave barrier keeps the compiler from merging or
-      // reordering the two predicated LDS updates; a wave executes them in
-      // program order.
+      // (incidence t+1), with the wave's LDS accesses kept in program order.
       if (role == 0 && have)
       {
         sx = sx + R0;
         if (pairs) sy = sy + R1;
-        if (a.vals)
-        {
-          th_add_row(ax, meta, K0);
-          if (pairs) th_add_row(ay, meta, K1);
-        }
+        if (a.vals) th_add_rows(ax, ay, meta, K0, K1, pairs);
       }
-      __builtin_amdgcn_wave_barrier();
-      if (role == 1 && have && a.vals)
-      {
-        th_add_row(ax, meta, K0);
-        if (pairs) th_add_row(ay, meta, K1);
-      }
-      __builtin_amdgcn_wave_barrier();
+      wave_sync();
+      if (role == 1 && have && a.vals) th_add_rows(ax, ay, meta, K0, K1, pairs);
+      wave_sync();
       const double pR0 = shfl1(R0), pR1 = shfl1(R1);
       if (role == 0 && t + 1 < t1)
       {
@@ -340,9 +516,9 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
       if (pairs) __builtin_nontemporal_store(sy, a.res + r + 1);
     }
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < bn; i += kThLanes) __builtin_nontemporal_store(acc[i], a.vals + base + i);
-  __syncthreads();
+  wave_sync();
+  for (int i = lane; i < bn; i += kThLanes) __builtin_nontemporal_store(acc[i], a.vals + base + i);
+  wave_sync();
 }
 
 struct ThVjpArgs

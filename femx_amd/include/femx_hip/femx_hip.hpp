@@ -329,9 +329,11 @@ inline void copy(const HostAssemblyMap& src, DeviceAssemblyMap& dst, HipContext&
     std::vector<Index> el(src.info().num_inc), me(4 * static_cast<std::size_t>(src.info().num_inc));
     check(fx_map_export_rowinc15(src.handle(), off.data(), el.data(), me.data()));
     Index nb = 0, mx = 0;
-    check(fx_map_th_blocks(src.handle(), FX_TH_BLOCK_ITEMS, &nb, &mx, nullptr, nullptr));
+    check(fx_map_th_blocks(src.handle(), FX_TH_BLOCK_ITEMS, FX_TH_TILE_NNZ, &nb, &mx, nullptr,
+                           nullptr));
     std::vector<Index> br(nb + 1), bf(nb);
-    check(fx_map_th_blocks(src.handle(), FX_TH_BLOCK_ITEMS, &nb, &mx, br.data(), bf.data()));
+    check(fx_map_th_blocks(src.handle(), FX_TH_BLOCK_ITEMS, FX_TH_TILE_NNZ, &nb, &mx, br.data(),
+                           bf.data()));
     dst.inc_off.assign(off, ctx);
     dst.inc_elem.assign(el, ctx);
     dst.inc_meta.assign(me, ctx);

@@ -22,6 +22,7 @@ from .assembly import AssemblyMap, make_assembly_map, make_boundary_map, replace
 from .linalg import CsrMatrix, _check_dev_vec, _require, ptr, stream_handle
 
 TH_ITEMS = 32  # work items per block (FX_TH_BLOCK_ITEMS)
+TH_TILE_NNZ = 0  # CSR entries per block row tile, 0 = bounded by items (FX_TH_TILE_NNZ)
 
 
 class THGeometry:
@@ -70,20 +71,20 @@ class THPlan:
     a plan share its workspace: VJPs of one map on different streams must not
     overlap."""
 
-    def __init__(self, amap: AssemblyMap, device="cuda", items=TH_ITEMS):
+    def __init__(self, amap: AssemblyMap, device="cuda", items=TH_ITEMS, tile_nnz=TH_TILE_NNZ):
         dev = torch.device(device)
         info = amap.info
-        self.items = int(items)
+        self.items, self.tile_nnz = int(items), int(tile_nnz)
         off = np.empty(amap.num_res + 1, np.int32)
         elem = np.empty(max(info.num_inc, 1), np.int32)
         meta = np.empty(max(4 * info.num_inc, 4), np.int32)
         _lib.call("fx_map_export_rowinc15", amap._handle(), ptr(off), ptr(elem), ptr(meta))
         nb, mx = C.c_int32(), C.c_int32()
-        _lib.call("fx_map_th_blocks", amap._handle(), self.items, C.byref(nb), C.byref(mx), None, None)
+        args = (amap._handle(), self.items, self.tile_nnz)
+        _lib.call("fx_map_th_blocks", *args, C.byref(nb), C.byref(mx), None, None)
         blk_row = np.empty(nb.value + 1, np.int32)
         blk_flags = np.empty(max(nb.value, 1), np.int32)
-        _lib.call("fx_map_th_blocks", amap._handle(), self.items, C.byref(nb), C.byref(mx), ptr(blk_row),
-                  ptr(blk_flags))
+        _lib.call("fx_map_th_blocks", *args, C.byref(nb), C.byref(mx), ptr(blk_row), ptr(blk_flags))
         self.num_blocks, self.max_block_nnz = nb.value, mx.value
         self.h_blk_row, self.h_blk_flags = blk_row, blk_flags[: nb.value]
         t = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731

@@ -106,13 +106,16 @@ int fx_map_export_rowinc(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc
 int fx_map_export_rowinc15(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc_elem,
                            fx_index* h_inc_meta);
 /* Work blocks of the Taylor-Hood kernels: rows cut into contiguous blocks of
- * at most `lanes` work items, an item being a pair of interchangeable
- * velocity rows (r, r+1) -- same elements, local rows li and li+1 (li even,
- * < 12), same column positions -- or a single row.  blk_row[num_blocks+1],
- * blk_flags[num_blocks] (1 = pairs); max_block_nnz sizes the LDS row tile.
- * Pass null arrays to query num_blocks and max_block_nnz. */
+ * at most `lanes` work items and, when max_nnz > 0, at most max_nnz CSR
+ * entries (a block always holds at least one item).  An item is a pair of
+ * interchangeable velocity rows (r, r+1) -- same elements, local rows li and
+ * li+1 (li even, < 12), same column positions -- or a single row.
+ * blk_row[num_blocks+1], blk_flags[num_blocks] (1 = pairs); max_block_nnz
+ * sizes the per-wave LDS row tile.  Pass null arrays to query num_blocks and
+ * max_block_nnz. */
 #define FX_TH_BLOCK_ITEMS 32 /* work items per block the kernels are tuned for */
-int fx_map_th_blocks(const fx_map* map, fx_index lanes, fx_index* num_blocks,
+#define FX_TH_TILE_NNZ 0     /* default tile bound (0: none; items bound the tile) */
+int fx_map_th_blocks(const fx_map* map, fx_index lanes, fx_index max_nnz, fx_index* num_blocks,
                      fx_index* max_block_nnz, fx_index* h_blk_row, fx_index* h_blk_flags);
 void fx_map_destroy(fx_map* map);
 
@@ -239,10 +242,10 @@ int fx_p1_vjp(fx_index num_nodes, fx_index num_elems, const double* d_x, const d
  * Newton Jacobian on the 7-point degree-5 rule, viscosity nu (or nu_e[E] per
  * element when non-null), vertex coordinates xv/yv, vconn3[3E] vertex ids,
  * dofs15[15E] element DOFs (fx_th_dofs order), tables from
- * fx_map_export_rowinc15 and fx_map_th_blocks (FX_TH_BLOCK_ITEMS items per
- * block; any block size is correct, larger blocks need a larger LDS tile).  vals may be NULL
- * (residual only).  Sums over elements in ascending element order, no
- * atomics; arithmetic uses fused multiply-adds. */
+ * fx_map_export_rowinc15 and fx_map_th_blocks (at most FX_TH_BLOCK_ITEMS
+ * items per block).  vals may be NULL (residual only).
+ * Sums over elements in ascending element order, no atomics; arithmetic uses
+ * fused multiply-adds. */
 int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks,
                    fx_index max_block_nnz, const double* d_xv, const double* d_yv,
                    const fx_index* d_vconn3, const fx_index* d_dofs15, const double* d_state,

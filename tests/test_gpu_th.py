@@ -9,7 +9,7 @@ import torch
 from conftest import close
 from femx_amd import ns
 from femx_amd.assembly import make_assembly_map
-from femx_amd.linalg import CsrMatrix
+from femx_amd.linalg import CsrMatrix, MatrixHandler
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -69,6 +69,55 @@ def test_th_full_size_matches_oracle_and_is_deterministic():
     assert close(v1, ov) and close(r1, orr)
 
 
+def perturbed(n, seed, relabel):
+    """Structured mesh with interior vertices moved by up to 0.3 h; with
+    `relabel`, every other element is listed clockwise (vertices 0,2,1, edges
+    (0,2),(2,1),(1,0)), so its det J is negative."""
+    g0 = ns.structured_th(n, DEV)
+    rng = np.random.default_rng(seed)
+    xv, yv = g0.h_xv.copy(), g0.h_yv.copy()
+    inner = (xv > 0) & (xv < 1) & (yv > 0) & (yv < 1)
+    xv[inner] += rng.uniform(-0.3, 0.3, inner.sum()) / n
+    yv[inner] += rng.uniform(-0.3, 0.3, inner.sum()) / n
+    v3, c6, d = g0.h_vconn3.copy(), g0.h_conn6.copy(), g0.h_dofs15.copy()
+    if relabel:
+        f = np.arange(g0.num_elems) % 2 == 1
+        P = np.array([0, 2, 1, 5, 4, 3])
+        v3[f] = v3[f][:, [0, 2, 1]]
+        c6[f] = c6[f][:, P]
+        perm = np.concatenate([np.stack([2 * P, 2 * P + 1], 1).reshape(-1), 12 + np.array([0, 2, 1])])
+        d[f] = d[f][:, perm]
+    return ns.THGeometry(xv, yv, v3, c6, d, g0.num_vel_nodes, DEV)
+
+
+def test_th_general_affine_and_relabelling_invariance():
+    """Skewed elements of both orientations against the oracle, and the
+    assembled operator independent of the elements' local numbering."""
+    n, nu = 5, 0.02
+    ga, gb = perturbed(n, 4, False), perturbed(n, 4, True)
+    s = RNG.standard_normal(ga.ndof)
+    out = {}
+    for key, g in (("a", ga), ("b", gb)):
+        amap = make_assembly_map(g.ndof, g.ndof, g.h_dofs15)
+        jac = CsrMatrix(amap.pattern, DEV)
+        res = torch.empty(g.ndof, dtype=torch.float64, device=DEV)
+        ns.assemble_ns(ns.TaylorHoodNS(nu), g, amap, dev(s), res, jac)
+        _, jm = amap.jac()
+        ov, orr = O.th_assemble(g.h_xv, g.h_yv, g.h_vconn3.reshape(-1), g.h_dofs15.reshape(-1),
+                                g.ndof, nu, s, jm, amap.pattern.nnz)
+        assert close(host(jac.vals), ov) and close(host(res), orr)
+        out[key] = (amap.pattern, host(jac.vals), host(res))
+    (pa, va, ra), (pb, vb, rb) = out["a"], out["b"]
+    assert np.array_equal(pa.row_ptr, pb.row_ptr) and np.array_equal(pa.col_ind, pb.col_ind)
+    assert close(vb, va) and close(rb, ra)
+    lam = RNG.standard_normal(gb.ndof)
+    R = ns.NavierStokesResidual(gb, None)
+    jt, ge = R.vjp(dev(s), dev(np.full(gb.num_elems, nu)), dev(lam))
+    ojt, og = O.th_vjp(gb.h_xv, gb.h_yv, gb.h_vconn3.reshape(-1), gb.h_dofs15.reshape(-1), gb.ndof,
+                       nu, s, lam, None, nu_e=np.full(gb.num_elems, nu))
+    assert close(host(jt), ojt) and close(host(ge), og)
+
+
 @pytest.mark.parametrize("n,with_bc", [(2, False), (6, True)])
 def test_th_vjp_matches_oracle(n, with_bc):
     g, amap = setup(n)
@@ -112,8 +161,6 @@ def test_th_residual_finite_differences():
     jac = CsrMatrix(amap.pattern, DEV)
     R.assemble_state_jac(s, nu_e, jac)
     dirv = dev(RNG.standard_normal(g.ndof))
-    Jd = host(jac.vals.new_empty(0)) if False else None
-    from femx_amd.linalg import MatrixHandler
     y = torch.empty(g.ndof, dtype=torch.float64, device=DEV)
     MatrixHandler().matvec(jac, dirv, y)
     rp, rm = torch.empty_like(out), torch.empty_like(out)

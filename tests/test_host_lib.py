@@ -197,5 +197,5 @@ def test_meshes_match_oracle(n):
 
 def test_mesh_overflow_is_reported():
     with pytest.raises(RuntimeError, match="int32"):
-        _lib.call("fx_mesh_p1", 40000, None, None, None) if False else \
-            _lib.call("fx_mesh_p1", 40000, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8))
+        # sizes are checked before any output is written
+        _lib.call("fx_mesh_p1", 40000, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8))

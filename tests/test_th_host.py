@@ -51,6 +51,7 @@ def test_rowinc15_and_blocks(n):
         if bf[b]:
             assert br[b + 1] <= nv2 and br[b] % 2 == 0
         assert rp[br[b + 1]] - rp[br[b]] <= plan.max_block_nnz
+        assert plan.tile_nnz == 0 or items == 1 or rp[br[b + 1]] - rp[br[b]] <= plan.tile_nnz
     # every velocity row is paired, every pressure row single
     paired = np.zeros(g.ndof, bool)
     for b in range(plan.num_blocks):
