@@ -7,7 +7,8 @@
 // phase -- (instance, block) pairs covering every row block of every
 // instance.  Each phase is one persistent launch: a grid sized to the GPU
 // (compute units x a tunable multiplier) whose workgroups walk the work list
-// with a fixed stride, running the same per-block device code as the
+// -- each XCD's workgroups one contiguous eighth of it, so neighbouring
+// blocks share an L2 -- running the same per-block device code as the
 // single-instance kernels, so each instance's results are bitwise equal to
 // its plain launch.  The five phases (P1 assembly, Taylor-Hood assembly, P1
 // VJP, Taylor-Hood VJP element and gather phases) are captured once into a
@@ -71,7 +72,8 @@ __global__ __launch_bounds__(kP1Block) void k_batch_p1_assemble(const P1AsmArgs*
                                                                int nwork)
 {
   extern __shared__ double smem[];
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+  const fx::XcdWalk wk = fx::xcd_walk(nwork);
+  for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
     p1_assemble_block(it.y, inst[it.x], smem);
@@ -82,15 +84,17 @@ __global__ __launch_bounds__(kP1Block) void k_batch_p1_vjp(const P1VjpArgs* __re
                                                           const int2* __restrict__ work, int nwork)
 {
   __shared__ long long sinc[kIncStage];
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+  const fx::XcdWalk wk = fx::xcd_walk(nwork);
+  for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
     p1_vjp_block(it.y, inst[it.x], sinc);
   }
 }
 
-// Each wave walks the work list on its own (stride: all waves of the grid)
-// with its own row tile of `tile` doubles; the workgroup shares the tables.
+// Each wave walks the work list on its own (xcd_walk with one walker per
+// wave) with its own row tile of `tile` doubles; the workgroup shares the
+// tables.
 __global__ __launch_bounds__(kThLanes* kThWaves) void k_batch_th_assemble(
     const ThAsmArgs* __restrict__ inst, const int2* __restrict__ work, int nwork, int tile)
 {
@@ -99,7 +103,8 @@ __global__ __launch_bounds__(kThLanes* kThWaves) void k_batch_th_assemble(
   th_stage_tensor(T, kThLanes * kThWaves);
   __syncthreads();
   const int w = threadIdx.x / kThLanes;
-  for (int i = blockIdx.x * kThWaves + w; i < nwork; i += gridDim.x * kThWaves)
+  const fx::XcdWalk wk = fx::xcd_walk(nwork, kThWaves, w);
+  for (int i = wk.first; i < wk.end; i += wk.stride)
   {
     const int2 it = work[i];
     th_assemble_block(it.y, inst[it.x], T, smem + kThTensorLen + w * tile);
@@ -113,7 +118,8 @@ __global__ __launch_bounds__(kThLanes) void k_batch_th_vjp_elem(const ThVjpArgs*
   __shared__ ThTables T;
   th_build_tables(T);
   __syncthreads();
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+  const fx::XcdWalk wk = fx::xcd_walk(nwork);
+  for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
     th_vjp_elem_block(it.y, inst[it.x], T);
@@ -124,7 +130,8 @@ __global__ __launch_bounds__(256) void k_batch_vjp_gather(const GatherArgs* __re
                                                          const int2* __restrict__ work, int nwork)
 {
   __shared__ int spos[kGatherStage];
-  for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+  const fx::XcdWalk wk = fx::xcd_walk(nwork);
+  for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
     vjp_gather_block(it.y, inst[it.x], spos);
@@ -187,7 +194,7 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     int cus = 0;
     fx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device),
                   "hipDeviceGetAttribute");
-    b->grid   = std::max(1, cus * grid_per_cu);
+    b->grid   = (std::max(1, cus * grid_per_cu) + 7) / 8 * 8; // xcd_walk needs a multiple of 8
     b->num_p1 = num_p1;
     b->num_th = num_th;
 

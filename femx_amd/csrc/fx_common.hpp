@@ -84,6 +84,23 @@ __device__ __forceinline__ int xcd_block(int nblocks)
   return xcd * per + min(xcd, rem) + k;
 }
 
+// Persistent walk over n work items by `sub` walkers per workgroup (walker
+// j): the walkers on each XCD take one contiguous eighth of the list, so
+// neighbouring items -- which share mesh data -- meet in one L2.  Requires
+// gridDim.x to be a multiple of 8 (the batch rounds its grid up to one).
+struct XcdWalk
+{
+  int first, end, stride;
+};
+
+__device__ __forceinline__ XcdWalk xcd_walk(int n, int sub = 1, int j = 0)
+{
+  const int xcd = blockIdx.x % 8, k = blockIdx.x / 8, per = gridDim.x / 8;
+  const int chunk = (n + 7) / 8;
+  const int begin = xcd * chunk;
+  return {begin + k * sub + j, min(n, begin + chunk), per * sub};
+}
+
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 inline unsigned blocks_for(std::int64_t n, int block)

@@ -10,13 +10,13 @@ using namespace fxk;
 __global__ __launch_bounds__(kP1Block) void k_p1_assemble(P1AsmArgs a)
 {
   extern __shared__ double smem[];
-  p1_assemble_block(blockIdx.x, a, smem);
+  p1_assemble_block(fx::xcd_block(gridDim.x), a, smem);
 }
 
 __global__ __launch_bounds__(kP1Block) void k_p1_vjp(P1VjpArgs a)
 {
   __shared__ long long sinc[kIncStage];
-  p1_vjp_block(blockIdx.x, a, sinc);
+  p1_vjp_block(fx::xcd_block(gridDim.x), a, sinc);
 }
 
 } // namespace

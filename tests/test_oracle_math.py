@@ -226,6 +226,14 @@ def test_th_residual_closed_forms():
     s[0:12:2] = px
     _, R = O.th_element(xe, ye, 0.3, s)
     assert np.allclose(R[12:], -A / 3, atol=1e-14)
+    # viscous scale: at u = 0 the Jacobian's velocity block is nu * stiffness,
+    # and v^T K w for the P2-exact v = x, w = x^2 is nu int grad x . grad x^2
+    # = 2 nu A xc (a dropped or doubled nu fails this; FD and symmetry do not)
+    nu = 0.3
+    K, _ = O.th_element(xe, ye, nu, np.zeros(15))
+    assert abs(px @ K[0:12:2, 0:12:2] @ px ** 2 - 2 * nu * A * xc) < 1e-13
+    assert abs(py @ K[1:12:2, 1:12:2] @ py ** 2 - 2 * nu * A * yc) < 1e-13
+    assert np.abs(K[0:12:2, 1:12:2]).max() == 0.0
 
 
 def test_th_vjp_oracle_finite_differences():
