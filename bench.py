@@ -127,6 +127,24 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_p1_2048_th_512.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` (read + written) on the profiled
+    2048^2 workload, from the committed rocprofv3 counter summary (calibrated
+    FETCH_SIZE and WRITE_SIZE medians); None when the summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                f = line.split()
+                if len(f) == 4 and f[0] == "traffic" and f[1] == kernel:
+                    return int(f[2]) + int(f[3])
+    except OSError:
+        return None
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -299,6 +317,7 @@ def main():
         extra["th_ns_%d" % args.th_n] = {
             "elements": th["E"], "ndof": th["ndof"], "nnz": th["nnz"],
             "assemble_us": t_th * 1e6, "elements_per_s": th["E"] / t_th, "alg_bytes": b_th,
+            "traffic": profiled_traffic("k_th_assemble") if args.th_n == 512 else None,
             "GBs": b_th / t_th / 1e9, "frac": b_th / t_th / 1e9 / HBM_PEAK_GBS,
             "vjp_us": t_thv * 1e6, "VJPs_per_s": 1.0 / t_thv, "vjp_alg_bytes": b_thv,
             "vjp_GBs": b_thv / t_thv / 1e9, "vjp_frac": b_thv / t_thv / 1e9 / HBM_PEAK_GBS}
@@ -356,7 +375,9 @@ def main():
                    "elements": E, "nodes": N, "nnz": nnz, "spmv_per_step": 10},
         "roofline": {"bound": "hbm", "achieved": b_asm / t_asm / 1e9, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": b_asm / t_asm / 1e9 / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "k_p1_assemble"},
+                     "traffic": profiled_traffic("k_p1_assemble") if n == 2048 else None,
+                     "kernel": "k_p1_assemble",
+                     "traffic_source": "profiles/hbm_traffic_p1_2048_th_512.md"},
         "cpu_baseline": cpu,
         "kernels": kernels,
         "extra": extra,
