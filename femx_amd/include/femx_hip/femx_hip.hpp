@@ -15,6 +15,7 @@
 //   linalg::HipMatrixHandler         femx/linalg/handler/MatrixHandler.hpp:193-378
 //   linalg::HipVectorHandler         femx/linalg/handler/VectorHandler.hpp:190-477
 //   state::Residual                  femx/state/Residual.hpp:18-60
+//   state::PoissonLogKappaResidual   a Residual for the P1 log-kappa inverse problem
 //
 // Errors surface as std::runtime_error carrying the library's message, as
 // femx's require() does (femx/common/Checks.hpp:11-17).  Device work is
@@ -589,6 +590,91 @@ public:
                                 DeviceCsrMatrix& out, HipContext& ctx) const                = 0;
   virtual void applyParamJacT(const DeviceVector& state, const DeviceVector& prm,
                               const DeviceVector& adj, DeviceVector& out, HipContext& ctx) const = 0;
+};
+
+/** P1 Poisson with a distributed control m_e = log kappa_e and Dirichlet
+ * rows replaced (femx's PoissonOpt residual, examples/poisson/PoissonOpt.cpp:
+ * 125-161, with the control moved from the boundary into the coefficient):
+ *   R_i = (K(exp m) u - F)_i on free rows,  R_r = u_r - g_r on constrained rows.
+ * Its state Jacobian is the row-replaced stiffness (diagonal one); the
+ * transposed products zero the adjoint on constrained rows first, as
+ * ConstrainedTimeResidual::applyJacT does (femx/assembly/
+ * ConstrainedTimeResidual.cpp:379-405).  The geometry, maps, load and
+ * boundary values must outlive the residual. */
+class PoissonLogKappaResidual : public Residual
+{
+public:
+  PoissonLogKappaResidual(const assembly::DeviceGeometry& geom, const assembly::DeviceAssemblyMap& map,
+                          const assembly::DeviceBoundaryMap* bc, const DeviceVector* bc_vals,
+                          const Real* f)
+      : geom_(geom), map_(map), bc_(bc), bc_vals_(bc_vals), f_(f)
+  {
+    require(!bc || (bc_vals && bc_vals->size() == bc->num_bcs),
+            "PoissonLogKappaResidual needs one boundary value per constrained row");
+  }
+
+  void res(const DeviceVector& state, const DeviceVector& prm, DeviceVector& out,
+           HipContext& ctx) const override
+  {
+    checkPrm(prm);
+    assembly::assemble(assembly::PoissonP1{prm.data(), true, f_}, geom_, map_, &state, out, nullptr,
+                       ctx);
+    if (bc_) assembly::replaceRes(*bc_, state, *bc_vals_, out, ctx);
+  }
+
+  void assembleStateJac(const DeviceVector& state, const DeviceVector& prm, DeviceCsrMatrix& out,
+                        HipContext& ctx) const override
+  {
+    checkPrm(prm);
+    DeviceVector tmp(map_.info.num_res);
+    assembly::assemble(assembly::PoissonP1{prm.data(), true, f_}, geom_, map_, &state, tmp, &out, ctx);
+    if (bc_) assembly::replaceRows(*bc_, out, 1.0, ctx);
+    ctx.sync(); // tmp is released on return
+  }
+
+  /** out[e] = ((dR/dm)^T adj)_e (Residual::applyParamJacT, Residual.hpp:55-59). */
+  void applyParamJacT(const DeviceVector& state, const DeviceVector& prm, const DeviceVector& adj,
+                      DeviceVector& out, HipContext& ctx) const override
+  {
+    DeviceVector jt(map_.info.num_states);
+    vjp(state, prm, adj, jt, out, ctx);
+    ctx.sync();
+  }
+
+  /** out = J^T adj over the row-replaced state Jacobian. */
+  void applyStateJacT(const DeviceVector& state, const DeviceVector& prm, const DeviceVector& adj,
+                      DeviceVector& out, HipContext& ctx) const
+  {
+    DeviceVector g(geom_.num_elems);
+    vjp(state, prm, adj, out, g, ctx);
+    ctx.sync();
+  }
+
+  /** Both products in one launch (fx_p1_vjp). */
+  void vjp(const DeviceVector& state, const DeviceVector& prm, const DeviceVector& adj,
+           DeviceVector& jt, DeviceVector& g, HipContext& ctx) const
+  {
+    checkPrm(prm);
+    require(state.size() == map_.info.num_states && adj.size() == map_.info.num_res
+                && jt.size() == map_.info.num_states && g.size() == geom_.num_elems,
+            "PoissonLogKappaResidual VJP vector sizes do not match");
+    check(fx_p1_vjp(geom_.num_nodes, geom_.num_elems, geom_.x.data(), geom_.y.data(),
+                    geom_.conn.data(), prm.data(), state.data(), adj.data(),
+                    bc_ ? bc_->bc_mask.data() : nullptr, map_.inc_off.data(), map_.inc.data(),
+                    jt.data(), g.data(), ctx.raw()));
+  }
+
+private:
+  void checkPrm(const DeviceVector& prm) const
+  {
+    require(prm.size() == geom_.num_elems, "PoissonLogKappaResidual control size must be num_elems");
+  }
+
+  const assembly::DeviceGeometry&    geom_;
+  const assembly::DeviceAssemblyMap& map_;
+  const assembly::DeviceBoundaryMap* bc_;
+  const DeviceVector*                bc_vals_;
+  const Real*                        f_;
 };
 } // namespace state
 

@@ -263,6 +263,64 @@ int main()
     expect(r1.toHost(ctx) == r2.toHost(ctx) && r1.toHost(ctx) == r3.toHost(ctx), "bitwise equal residuals");
   });
 
+  // the finite-difference procedure of tests/unit/inverse/runInverseTests.cpp:192-234
+  // on the Residual contract, and the adjoint identity against the assembled Jacobian
+  run("poissonLogKappaResidualGradients", [] {
+    needDevice();
+    HipContext ctx;
+    const int  n = 4, N = (n + 1) * (n + 1), E = 2 * n * n;
+    RV         x(N), y(N);
+    IV         conn(3 * E);
+    check(fx_mesh_p1(n, x.data(), y.data(), conn.data()));
+    std::vector<IV> dofs(E);
+    for (int e = 0; e < E; ++e) dofs[e] = {conn[3 * e], conn[3 * e + 1], conn[3 * e + 2]};
+    auto hmap = assembly::makeAssemblyMap(N, N, dofs, dofs);
+    assembly::DeviceAssemblyMap map;
+    assembly::copy(hmap, map, ctx);
+    IV bcd;
+    for (int i = 0; i < N; ++i)
+      if (x[i] == 0.0 || y[i] == 0.0) bcd.push_back(i);
+    auto hb = assembly::makeBoundaryMap(bcd, hmap.pattern());
+    assembly::DeviceBoundaryMap bmap;
+    assembly::copy(hb, bmap, ctx);
+    assembly::DeviceGeometry geom;
+    geom.num_nodes = N, geom.num_elems = E, geom.nodes_per_elem = 3;
+    geom.x.assign(x, ctx), geom.y.assign(y, ctx), geom.conn.assign(conn, ctx);
+    RV hm(E), hu(N), hl(N), hf(N);
+    for (int e = 0; e < E; ++e) hm[e] = 0.4 * std::sin(1.3 * e);
+    for (int i = 0; i < N; ++i) hu[i] = std::cos(0.7 * i), hl[i] = std::sin(0.9 * i + 0.2), hf[i] = 1.0 + 0.1 * i;
+    DeviceVector m(hm, ctx), u(hu, ctx), lam(hl, ctx), f(hf, ctx), g(RV(bcd.size(), 0.5), ctx);
+    state::PoissonLogKappaResidual R(geom, map, &bmap, &g, f.data());
+    DeviceVector grad(E), jt(N), r(N);
+    R.applyParamJacT(u, m, lam, grad, ctx);
+    R.applyStateJacT(u, m, lam, jt, ctx);
+    const RV hg = grad.toHost(ctx), hjt = jt.toHost(ctx);
+    auto lamDotR = [&](const RV& mm) {
+      DeviceVector dm(mm, ctx);
+      R.res(u, dm, r, ctx);
+      const RV hr = r.toHost(ctx);
+      double s = 0.0;
+      for (int i = 0; i < N; ++i) s += hl[i] * hr[i];
+      return s;
+    };
+    const double h = 1e-6;
+    for (int e : {0, 5, 17, E - 1})
+    {
+      RV mp = hm, mm = hm;
+      mp[e] += h, mm[e] -= h;
+      const double fd = (lamDotR(mp) - lamDotR(mm)) / (2 * h);
+      expect(std::abs(hg[e] - fd) <= 1e-6 * std::max(1.0, std::abs(fd)), "d(lam.R)/dm_e by central differences");
+    }
+    DeviceCsrMatrix J(map.pattern);
+    R.assembleStateJac(u, m, J, ctx);
+    DeviceVector ref(N);
+    linalg::HipMatrixHandler(ctx).matvecT(J, lam.data(), N, ref.data(), N);
+    const RV href = ref.toHost(ctx);
+    double err = 0.0, nrm = 0.0;
+    for (int i = 0; i < N; ++i) err = std::max(err, std::abs(hjt[i] - href[i])), nrm = std::max(nrm, std::abs(href[i]));
+    expect(err <= 1e-12 * nrm, "J^T lam equals the assembled row-replaced Jacobian's transpose apply");
+  });
+
   std::printf("%d passed, %d failed, %d skipped\n", g_pass, g_fail, g_skip);
   return g_fail;
 }
