@@ -27,7 +27,7 @@ _SIGS = {
     "fx_map_export_pattern": ([_vp, _vp, _vp], C.c_int),
     "fx_map_export_jac": ([_vp, _vp, _vp], C.c_int),
     "fx_map_export_rowinc3": ([_vp, _vp, _vp], C.c_int),
-    "fx_map_export_rowinc": ([_vp, _vp, _vp], C.c_int),
+    "fx_map_export_rowinc": ([_vp, _vp, _vp, _vp], C.c_int),
     "fx_map_export_rowinc15": ([_vp, _vp, _vp, _vp], C.c_int),
     "fx_map_th_blocks": ([_vp, _i32, _i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_map_destroy": ([_vp], None),
@@ -72,7 +72,7 @@ class THInstance(C.Structure):
                 + [(n, _vp) for n in ("xv", "yv", "vconn3", "dofs15", "state")]
                 + [("nu", _f64)]
                 + [(n, _vp) for n in ("nu_e", "inc_off", "inc_elem", "inc_meta", "blk_row",
-                                      "blk_flags", "row_ptr", "vals", "res", "inc_pos", "lam",
+                                      "blk_flags", "row_ptr", "vals", "res", "elem_inc", "lam",
                                       "bc_mask", "jt", "g", "work")])
 
 

@@ -114,7 +114,7 @@ class THCase:
             g.ndof, g.num_elems, p.num_blocks, p.max_block_nnz, ptr(g.xv), ptr(g.yv),
             ptr(g.vconn3), ptr(g.dofs15), ptr(self.state), self.nu, None, ptr(p.inc_off),
             ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row), ptr(p.blk_flags), ptr(rp),
-            ptr(self.jac.vals), ptr(self.res), ptr(p.inc_pos), ptr(self.lam), ptr(self.bmap.d_mask),
+            ptr(self.jac.vals), ptr(self.res), ptr(p.elem_inc), ptr(self.lam), ptr(self.bmap.d_mask),
             ptr(self.jt), ptr(self.g), ptr(p.work))
 
     def plain(self, what):

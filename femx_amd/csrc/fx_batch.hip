@@ -129,12 +129,12 @@ __global__ __launch_bounds__(kThLanes) void k_batch_th_vjp_elem(const ThVjpArgs*
 __global__ __launch_bounds__(256) void k_batch_vjp_gather(const GatherArgs* __restrict__ inst,
                                                          const int2* __restrict__ work, int nwork)
 {
-  __shared__ int spos[kGatherStage];
+  __shared__ double sval[kGatherStage];
   const fx::XcdWalk wk = fx::xcd_walk(nwork);
   for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
-    vjp_gather_block(it.y, inst[it.x], spos);
+    vjp_gather_block(it.y, inst[it.x], sval);
   }
 }
 
@@ -241,10 +241,10 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
                     reinterpret_cast<const uint4*>(q.inc_meta), q.blk_row, q.blk_flags, q.row_ptr,
                     q.vals, q.res});
       for (int k = 0; k < q.num_blocks; ++k) w[kThAsm].push_back(make_int2(i, k));
-      const bool vjp = q.lam && q.jt && q.work && q.inc_pos;
+      const bool vjp = q.lam && q.jt && q.work && q.elem_inc;
       tv.push_back({q.num_elems, q.xv, q.yv, q.vconn3, q.dofs15, q.state, q.nu, q.nu_e, q.lam,
-                    q.bc_mask, q.work, q.g});
-      tg.push_back({q.ndof, q.inc_off, q.inc_pos, q.work, q.lam, q.bc_mask, q.jt});
+                    q.bc_mask, q.elem_inc, q.work, q.g});
+      tg.push_back({q.ndof, q.inc_off, q.work, q.lam, q.bc_mask, q.jt});
       if (vjp)
       {
         const int ne = static_cast<int>(fx::blocks_for(q.num_elems, kThLanes));

@@ -350,14 +350,17 @@ int fx_map_export_rowinc3(const fx_map* m, Index* inc_off, Index* inc)
 // Generic row incidence: inc_off[num_res+1] and inc_pos[num_inc], the
 // position res_offsets[e] + li of each (element, local row) in the flattened
 // element DOF table, rows in ascending element order.
-int fx_map_export_rowinc(const fx_map* m, Index* inc_off, Index* inc_pos)
+int fx_map_export_rowinc(const fx_map* m, Index* inc_off, Index* inc_pos, Index* elem_inc)
 {
   return fx::guard([&] {
-    fx::require(m && inc_off && (m->inc_elem.empty() || inc_pos), FX_ERR_ARG,
-                "fx_map_export_rowinc: null argument");
+    fx::require(m && inc_off, FX_ERR_ARG, "fx_map_export_rowinc: null argument");
     std::copy(m->inc_off.begin(), m->inc_off.end(), inc_off);
     for (std::size_t t = 0; t < m->inc_elem.size(); ++t)
-      inc_pos[t] = m->res_off[m->inc_elem[t]] + m->inc_local[t];
+    {
+      const Index p = m->res_off[m->inc_elem[t]] + m->inc_local[t];
+      if (inc_pos) inc_pos[t] = p;
+      if (elem_inc) elem_inc[p] = static_cast<Index>(t);
+    }
   });
 }
 

@@ -30,8 +30,8 @@ __global__ __launch_bounds__(kThLanes) void k_th_vjp_elem(ThVjpArgs a)
 
 __global__ __launch_bounds__(256) void k_vjp_gather(GatherArgs a)
 {
-  __shared__ int spos[kGatherStage];
-  vjp_gather_block(fx::xcd_block(gridDim.x), a, spos);
+  __shared__ double sval[kGatherStage];
+  vjp_gather_block(fx::xcd_block(gridDim.x), a, sval);
 }
 
 } // namespace
@@ -70,14 +70,14 @@ int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks, fx_in
 int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* xv, const double* yv,
               const fx_index* vconn3, const fx_index* dofs15, const double* state, double nu,
               const double* nu_e, const double* lam, const fx_index* bc_mask,
-              const fx_index* inc_off, const fx_index* inc_pos, double* jt, double* g,
+              const fx_index* inc_off, const fx_index* elem_inc, double* jt, double* g,
               double* work, void* stream)
 {
   return fx::guard([&] {
     fx::require(ndof >= 0 && num_elems >= 0, FX_ERR_ARG, "fx_th_vjp: negative size");
     if (ndof == 0) return;
     fx::require(xv && yv && state && lam && inc_off && jt
-                    && (num_elems == 0 || (vconn3 && dofs15 && inc_pos && work)),
+                    && (num_elems == 0 || (vconn3 && dofs15 && elem_inc && work)),
                 FX_ERR_ARG, "fx_th_vjp: null array");
     const std::size_t nd = std::size_t(ndof) * 8, ne = std::size_t(num_elems) * 8;
     fx::require(!fx::overlaps(jt, nd, lam, nd) && !fx::overlaps(jt, nd, state, nd)
@@ -86,12 +86,13 @@ int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* xv, const double*
                 FX_ERR_ALIAS, "fx_th_vjp: outputs must not alias inputs or each other");
     if (num_elems > 0)
     {
-      const ThVjpArgs a{num_elems, xv, yv, vconn3, dofs15, state, nu, nu_e, lam, bc_mask, work, g};
+      const ThVjpArgs a{num_elems, xv,      yv,       vconn3, dofs15, state, nu,
+                        nu_e,      lam,     bc_mask,  elem_inc, work, g};
       hipLaunchKernelGGL(k_th_vjp_elem, dim3(fx::blocks_for(num_elems, kThLanes)), dim3(kThLanes), 0,
                          fx::as_stream(stream), a);
       fx::check_launch("fx_th_vjp");
     }
-    const GatherArgs ga{ndof, inc_off, inc_pos, work, lam, bc_mask, jt};
+    const GatherArgs ga{ndof, inc_off, work, lam, bc_mask, jt};
     hipLaunchKernelGGL(k_vjp_gather, dim3(fx::blocks_for(ndof, 256)), dim3(256), 0,
                        fx::as_stream(stream), ga);
     fx::check_launch("fx_th_vjp");

@@ -27,9 +27,10 @@
 // its element matrix with the boundary-masked adjoint through the adjoint's
 // quadrature-point interpolant (lambda_h, grad lambda_h, pi_h) -- the
 // quadrature-point state is formed once per element and J is never formed --
-// into a 15-per-element workspace; then one lane per DOF sums its column's
-// element contributions in ascending element order (a segmented reduction
-// over the row incidence) instead of scattering with atomics.
+// and writes each column's value to its slot in the row-incidence order;
+// then one lane per DOF sums its contiguous run of slots in ascending element
+// order (a segmented reduction over a stream) instead of scattering with
+// atomics.
 #pragma once
 #include "fx_common.hpp"
 
@@ -533,6 +534,7 @@ struct ThVjpArgs
   const double* nu_e;
   const double* lam;
   const int*    bc_mask;
+  const int*    elem_inc; // (e, li) -> row-incidence index (fx_map_export_rowinc)
   double*       cbuf;
   double*       gout;
 };
@@ -596,9 +598,11 @@ __device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, c
     for (int b = 0; b < 3; ++b) c[12 + b] = fma(-T.psi[q][b] * (g00 + g11), wq, c[12 + b]);
     ge = fma(fma(g00, d00, g01 * d01) + fma(g10, d10, g11 * d11), wq, ge);
   }
-  double* out = a.cbuf + 15 * static_cast<long>(e);
+  // each column's value goes to its slot in the row-incidence order, where
+  // phase 2 finds a DOF's contributions contiguous
+  const int* slot = a.elem_inc + 15 * static_cast<long>(e);
 #pragma unroll
-  for (int k = 0; k < 15; ++k) out[k] = c[k];
+  for (int k = 0; k < 15; ++k) a.cbuf[slot[k]] = c[k];
   if (a.gout) __builtin_nontemporal_store(ge, a.gout + e);
 }
 
@@ -609,25 +613,25 @@ struct GatherArgs
 {
   int           ndof;
   const int*    inc_off;
-  const int*    inc_pos;
-  const double* cbuf;
+  const double* cbuf; // element contributions in row-incidence order
   const double* lam;
   const int*    bc_mask;
   double*       jt;
 };
 
-// Phase 2 (one lane per DOF): sum the element contributions of the DOF's
-// column in ascending element order, then pass lam through on constrained
-// DOFs.  The block's incidence range is contiguous and is staged in LDS with
-// coalesced loads when it fits.
-__device__ __forceinline__ void vjp_gather_block(int blk, const GatherArgs& a, int* spos)
+// Phase 2 (one lane per DOF): sum the DOF's element contributions -- a
+// contiguous run of cbuf, in ascending element order -- then pass lam through
+// on constrained DOFs.  A block's 256 runs are one contiguous range, staged in
+// LDS with coalesced non-temporal loads when it fits (a segmented sum over a
+// stream).
+__device__ __forceinline__ void vjp_gather_block(int blk, const GatherArgs& a, double* sval)
 {
   const int  c0 = blk * 256, c1 = min(c0 + 256, a.ndof);
   const int  i0 = a.inc_off[c0], ni = a.inc_off[c1] - i0;
   const bool staged = ni <= kGatherStage;
   if (staged)
   {
-    for (int i = threadIdx.x; i < ni; i += 256) spos[i] = a.inc_pos[i0 + i];
+    for (int i = threadIdx.x; i < ni; i += 256) sval[i] = __builtin_nontemporal_load(a.cbuf + i0 + i);
     __syncthreads();
   }
   const int c = c0 + threadIdx.x;
@@ -635,7 +639,7 @@ __device__ __forceinline__ void vjp_gather_block(int blk, const GatherArgs& a, i
   {
     double    v  = 0.0;
     const int t1 = a.inc_off[c + 1];
-    for (int t = a.inc_off[c]; t < t1; ++t) v = v + a.cbuf[staged ? spos[t - i0] : a.inc_pos[t]];
+    for (int t = a.inc_off[c]; t < t1; ++t) v = v + (staged ? sval[t - i0] : a.cbuf[t]);
     if (a.bc_mask && a.bc_mask[c]) v = v + a.lam[c];
     __builtin_nontemporal_store(v, a.jt + c);
   }

@@ -90,9 +90,9 @@ class THPlan:
         t = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
         self.inc_off, self.inc_elem = t(off), t(elem[: info.num_inc])
         self.inc_meta = t(meta[: 4 * info.num_inc])
-        pos = np.empty(max(info.num_inc, 1), np.int32)
-        _lib.call("fx_map_export_rowinc", amap._handle(), ptr(off), ptr(pos))
-        self.inc_pos = t(pos[: info.num_inc])
+        slots = np.empty(max(15 * amap.num_elems, 1), np.int32)
+        _lib.call("fx_map_export_rowinc", amap._handle(), ptr(off), None, ptr(slots))
+        self.elem_inc = t(slots[: 15 * amap.num_elems])
         self.work = torch.empty(15 * amap.num_elems, dtype=torch.float64, device=dev)
         self.blk_row, self.blk_flags = t(blk_row), t(self.h_blk_flags)
 
@@ -165,7 +165,7 @@ def ns_vjp(op: TaylorHoodNS, geom: THGeometry, amap: AssemblyMap, state, lam, bc
     p = _plan(amap, dev)
     _lib.call("fx_th_vjp", geom.ndof, geom.num_elems, ptr(geom.xv), ptr(geom.yv),
               ptr(geom.vconn3), ptr(geom.dofs15), ptr(state), op.nu, ptr(op.nu_e), ptr(lam),
-              ptr(bc_mask), ptr(p.inc_off), ptr(p.inc_pos), ptr(jt_out), ptr(g_out), ptr(p.work),
+              ptr(bc_mask), ptr(p.inc_off), ptr(p.elem_inc), ptr(jt_out), ptr(g_out), ptr(p.work),
               stream_handle(stream))
     return jt_out, g_out
 

@@ -94,10 +94,13 @@ int fx_map_export_jac(const fx_map* map, fx_index* h_jac_offsets, fx_index* h_ja
  * element's three columns.  FX_ERR_UNSUPPORTED for other shapes or rows
  * longer than 255. */
 int fx_map_export_rowinc3(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc);
-/* Generic row incidence: inc_off[num_res+1] and inc_pos[num_inc] -- for
- * each row, in ascending element order, the position res_offsets[e] + li of
- * its (element, local row) in the flattened element DOF table. */
-int fx_map_export_rowinc(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc_pos);
+/* Generic row incidence: inc_off[num_res+1]; inc_pos[num_inc] -- for each
+ * row, in ascending element order, the position res_offsets[e] + li of its
+ * (element, local row) in the flattened element DOF table; elem_inc, the
+ * inverse (one entry per flattened element DOF): the incidence index of
+ * (e, li).  inc_pos and elem_inc may be NULL. */
+int fx_map_export_rowinc(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc_pos,
+                         fx_index* h_elem_inc);
 /* Row incidence for elements with 15 local rows and columns (Taylor-Hood):
  * inc_off[num_res+1], inc_elem[num_inc] and inc_meta[4*num_inc] -- 16 bytes
  * per incidence: the positions (within the row, from row_ptr[row]) of the
@@ -256,14 +259,15 @@ int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks,
 /* Navier-Stokes adjoint VJP: jt = J^T lam over the row-replaced Newton
  * Jacobian (lam zeroed on constrained rows, passed through there; bc_mask may
  * be NULL) and, when g is non-null, g[e] = lam~_e^T dR_e/dnu_e, the
- * per-element viscosity sensitivity.  inc_off/inc_pos from
- * fx_map_export_rowinc; work holds 15*num_elems doubles (element column
- * contributions, summed per DOF in ascending element order).  The contract
- * of Residual::applyParamJacT (femx/state/Residual.hpp:55-59). */
+ * per-element viscosity sensitivity.  inc_off/elem_inc from
+ * fx_map_export_rowinc; work holds 15*num_elems doubles: the element column
+ * contributions in row-incidence order, summed per DOF in ascending element
+ * order.  The contract of Residual::applyParamJacT (femx/state/
+ * Residual.hpp:55-59). */
 int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* d_xv, const double* d_yv,
               const fx_index* d_vconn3, const fx_index* d_dofs15, const double* d_state,
               double nu, const double* d_nu_e, const double* d_lam, const fx_index* d_bc_mask,
-              const fx_index* d_inc_off, const fx_index* d_inc_pos, double* d_jt, double* d_g,
+              const fx_index* d_inc_off, const fx_index* d_elem_inc, double* d_jt, double* d_g,
               double* d_work, void* stream);
 
 /* ======================================================================== */
@@ -274,7 +278,7 @@ int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* d_xv, const doubl
  * single-instance calls (fx_p1_assemble with kappa_is_log = 1 and fx_p1_vjp;
  * fx_th_assemble and fx_th_vjp).  Instance arrays are caller-owned device
  * memory and must outlive the batch.  VJP pointers (lam, jt, g, work,
- * inc_pos) may be NULL to batch assembly only. */
+ * elem_inc) may be NULL to batch assembly only. */
 typedef struct
 {
   fx_index      num_nodes, num_elems, nnz, max_row_len;
@@ -313,7 +317,7 @@ typedef struct
   const fx_index* row_ptr;
   double*       vals;
   double*       res;
-  const fx_index* inc_pos; /* fx_map_export_rowinc */
+  const fx_index* elem_inc; /* fx_map_export_rowinc */
   const double* lam;
   const fx_index* bc_mask;
   double*       jt;

@@ -59,6 +59,25 @@ def test_rowinc15_and_blocks(n):
     assert paired[:nv2].all() and not paired[nv2:].any()
 
 
+def test_elem_inc_inverts_the_row_incidence():
+    """elem_inc[15 e + li] is the incidence index of (e, li): reading a row's
+    run of incidences back through it gives the row's elements, ascending."""
+    g = ns.structured_th(3, "cpu")
+    amap = make_assembly_map(g.ndof, g.ndof, g.h_dofs15)
+    plan = ns.THPlan(amap, "cpu")
+    off = plan.inc_off.numpy()
+    slots = plan.elem_inc.numpy()
+    assert np.array_equal(np.sort(slots), np.arange(off[-1]))  # a permutation of the incidences
+    d = g.h_dofs15.reshape(-1)
+    row_of_slot = np.empty(off[-1], np.int64)
+    row_of_slot[slots] = d  # the DOF (row) each (e, li) belongs to
+    assert np.array_equal(row_of_slot, np.repeat(np.arange(g.ndof), np.diff(off)))
+    elem_of_slot = np.empty(off[-1], np.int64)
+    elem_of_slot[slots] = np.repeat(np.arange(g.num_elems), 15)
+    for r in range(g.ndof):
+        assert np.all(np.diff(elem_of_slot[off[r]:off[r + 1]]) > 0)
+
+
 def test_rowinc15_rejects_other_shapes():
     amap = make_assembly_map(3, 3, [[0, 1, 2]])
     with pytest.raises(RuntimeError, match="15x15"):
