@@ -102,7 +102,9 @@ __global__ __launch_bounds__(kThLanes* kThWaves) void k_batch_th_assemble(
   ThTensor& T = *reinterpret_cast<ThTensor*>(smem);
   th_stage_tensor(T, kThLanes * kThWaves);
   __syncthreads();
-  const int w = threadIdx.x / kThLanes;
+  // the wave index is uniform across the wave: say so, so that the work item
+  // and the instance's arguments are loaded once per wave into scalar registers
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kThLanes);
   const fx::XcdWalk wk = fx::xcd_walk(nwork, kThWaves, w);
   for (int i = wk.first; i < wk.end; i += wk.stride)
   {

@@ -15,7 +15,7 @@ __global__ __launch_bounds__(kThLanes* kThWaves) void k_th_assemble(ThAsmArgs a,
   ThTensor& T = *reinterpret_cast<ThTensor*>(smem);
   th_stage_tensor(T, kThLanes * kThWaves);
   __syncthreads();
-  const int w = threadIdx.x / kThLanes;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kThLanes); // uniform per wave
   const int b = fx::xcd_block(gridDim.x) * kThWaves + w;
   if (b < num_blocks) th_assemble_block(b, a, T, smem + kThTensorLen + w * tile);
 }
