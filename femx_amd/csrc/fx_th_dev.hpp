@@ -299,6 +299,70 @@ __device__ __forceinline__ void th_gather_t(ThElemT& E, int e, const double* __r
   E.nu             = nu_e ? nu_e[e] : nu;
 }
 
+// The same element data loaded in three dependent stages -- incidence,
+// element tables (DOFs, vertices), values -- so that a row's next rounds can
+// be in flight while the current one computes.
+struct ThIdx
+{
+  int   e;
+  uint4 meta;
+};
+
+struct ThDofs
+{
+  int d[15];
+  int v[3];
+};
+
+struct ThRaw
+{
+  double s[15];
+  double x[3], y[3];
+  double nu;
+};
+
+__device__ __forceinline__ ThIdx th_load_idx(const int* __restrict__ inc_elem,
+                                             const uint4* __restrict__ inc_meta, int t, int t1)
+{
+  ThIdx i{0, make_uint4(0, 0, 0, 0)};
+  if (t < t1)
+  {
+    i.e    = inc_elem[t];
+    i.meta = inc_meta[t];
+  }
+  return i;
+}
+
+__device__ __forceinline__ ThDofs th_load_dofs(const int* __restrict__ dofs,
+                                               const int* __restrict__ vconn, int e, bool have)
+{
+  ThDofs D;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) D.d[k] = 0;
+  D.v[0] = D.v[1] = D.v[2] = 0;
+  if (have)
+  {
+    const int* d = dofs + 15 * static_cast<long>(e);
+#pragma unroll
+    for (int k = 0; k < 15; ++k) D.d[k] = d[k];
+    const int* v = vconn + 3 * static_cast<long>(e);
+    D.v[0] = v[0], D.v[1] = v[1], D.v[2] = v[2];
+  }
+  return D;
+}
+
+__device__ __forceinline__ void th_make_elem(ThElemT& E, const ThRaw& R)
+{
+#pragma unroll
+  for (int k = 0; k < 15; ++k) E.s[k] = R.s[k];
+  const double x0 = R.x[0], y0 = R.y[0];
+  const double j00 = R.x[1] - x0, j01 = R.x[2] - x0, j10 = R.y[1] - y0, j11 = R.y[2] - y0;
+  const double det = j00 * j11 - j01 * j10;
+  const double sg  = det < 0.0 ? -1.0 : 1.0;
+  E.g              = {sg * j11, -sg * j01, -sg * j10, sg * j00, fabs(det)};
+  E.nu             = R.nu;
+}
+
 // Velocity rows (a,0) and (a,1) of the element Jacobian and residual:
 //   K_(a,c),(b,d) = delta_cd (nu A_ab + C_ab) + M^cd_ab,
 //   K_(a,c),b'    = -int psi_b' d_c phi_a,
@@ -435,6 +499,24 @@ struct ThAsmArgs
   double*       res;
 };
 
+__device__ __forceinline__ ThRaw th_load_raw(const ThAsmArgs& a, const ThDofs& D, int e, bool have)
+{
+  ThRaw R;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) R.s[k] = 0.0;
+  R.x[0] = R.x[1] = R.x[2] = R.y[0] = R.y[1] = R.y[2] = 0.0;
+  R.nu = a.nu;
+  if (have)
+  {
+#pragma unroll
+    for (int k = 0; k < 15; ++k) R.s[k] = a.state[D.d[k]];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) R.x[k] = a.xv[D.v[k]], R.y[k] = a.yv[D.v[k]];
+    if (a.nu_e) R.nu = a.nu_e[e];
+  }
+  return R;
+}
+
 // One row block per wave.  Two lanes per work item: lane role 0 takes the
 // item's incidences t0, t0+2, ..., role 1 takes t0+1, t0+3, ...; after each
 // round role 0 applies its own contribution and then its partner's (fetched
@@ -462,19 +544,26 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
     double*   ay = pairs ? acc + (a.row_ptr[r + 1] - base) : nullptr;
     double    sx = 0.0, sy = 0.0;
     const int t0 = a.inc_off[r], t1 = a.inc_off[r + 1];
+    // software pipeline over this lane's incidences t0+role, t0+role+2, ...:
+    // values of the next round and tables of the one after are in flight
+    // while the current round computes
+    ThIdx  ic = th_load_idx(a.inc_elem, a.inc_meta, t0 + role, t1);
+    ThIdx  in = th_load_idx(a.inc_elem, a.inc_meta, t0 + role + 2, t1);
+    ThRaw  rc = th_load_raw(a, th_load_dofs(a.dofs, a.vconn, ic.e, t0 + role < t1), ic.e, t0 + role < t1);
+    ThDofs dn = th_load_dofs(a.dofs, a.vconn, in.e, t0 + role + 2 < t1);
     for (int t = t0; t < t1; t += 2)
     {
       const int  tm   = t + role;
       const bool have = tm < t1;
-      double     K0[15], K1[15], R0 = 0.0, R1 = 0.0;
-      uint4      meta = make_uint4(0, 0, 0, 0);
+      const ThIdx in2 = th_load_idx(a.inc_elem, a.inc_meta, tm + 4, t1);
+      const ThRaw rn  = th_load_raw(a, dn, in.e, tm + 2 < t1);
+      double      K0[15], K1[15], R0 = 0.0, R1 = 0.0;
+      const uint4 meta = ic.meta;
       if (have)
       {
-        const int e = a.inc_elem[tm];
-        meta        = a.inc_meta[tm];
         const int li = meta.w >> 24;
         ThElemT   E;
-        th_gather_t(E, e, a.xv, a.yv, a.vconn, a.dofs, a.state, a.nu, a.nu_e);
+        th_make_elem(E, rc);
         if (li < 12)
         {
           th_velocity_rows(T, E, li >> 1, K0, K1, R0, R1);
@@ -510,6 +599,10 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
         sx = sx + pR0;
         if (pairs) sy = sy + pR1;
       }
+      dn = th_load_dofs(a.dofs, a.vconn, in2.e, tm + 4 < t1);
+      ic = in;
+      in = in2;
+      rc = rn;
     }
     if (role == 0)
     {
