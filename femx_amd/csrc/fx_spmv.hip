@@ -59,6 +59,45 @@ __global__ __launch_bounds__(kRowBlock) void k_spmv(int rows, const int* __restr
   y[r] = beta == 0.0 ? alpha * v : alpha * v + beta * y[r];
 }
 
+// k_spmv with 16-bit column offsets (col = row + off): the staged index
+// stream is half as wide; the products and their order are unchanged.
+__global__ __launch_bounds__(kRowBlock) void k_spmv16(int rows, const int* __restrict__ row_ptr,
+                                                      const short* __restrict__ off,
+                                                      const double* __restrict__ vals,
+                                                      const double* __restrict__ x,
+                                                      double* __restrict__ y, double alpha,
+                                                      double beta)
+{
+  __shared__ double sv[kStage];
+  __shared__ short  so[kStage];
+  const int  r0 = fx::xcd_block(gridDim.x) * kRowBlock, r1 = min(r0 + kRowBlock, rows);
+  const int  k0 = row_ptr[r0], kn = row_ptr[r1] - k0;
+  const bool staged = kn <= kStage;
+  if (staged)
+  {
+    for (int i = threadIdx.x; i < kn; i += kRowBlock)
+    {
+      sv[i] = __builtin_nontemporal_load(vals + k0 + i);
+      so[i] = __builtin_nontemporal_load(off + k0 + i);
+    }
+    __syncthreads();
+  }
+  const int r = r0 + threadIdx.x;
+  if (r >= rows) return;
+  double v = 0.0;
+  if (staged)
+  {
+    const int a = row_ptr[r] - k0, b = row_ptr[r + 1] - k0;
+    for (int k = a; k < b; ++k) v = v + sv[k] * x[r + so[k]];
+  }
+  else
+  {
+    const int k1 = row_ptr[r + 1];
+    for (int k = row_ptr[r]; k < k1; ++k) v = v + vals[k] * x[r + off[k]];
+  }
+  y[r] = beta == 0.0 ? alpha * v : alpha * v + beta * y[r];
+}
+
 __global__ __launch_bounds__(kRowBlock) void k_spmv_t(int cols, const int* __restrict__ t_ptr,
                                                       const int* __restrict__ t_row,
                                                       const int* __restrict__ t_perm,
@@ -134,6 +173,42 @@ int fx_csr_matvec(fx_index rows, fx_index cols, const fx_index* row_ptr, const f
     hipLaunchKernelGGL(k_spmv, dim3(fx::blocks_for(rows, kRowBlock)), dim3(kRowBlock), 0,
                        fx::as_stream(stream), rows, row_ptr, col_ind, vals, x, y, alpha, beta);
     fx::check_launch("fx_csr_matvec");
+  });
+}
+
+int fx_csr_matvec16(fx_index rows, fx_index cols, const fx_index* row_ptr, const int16_t* col_off,
+                    const double* vals, const double* x, double* y, double alpha, double beta,
+                    void* stream)
+{
+  return fx::guard([&] {
+    fx::require(rows >= 0 && cols >= 0, FX_ERR_ARG, "fx_csr_matvec16: negative size");
+    if (rows == 0) return;
+    fx::require(row_ptr && y && (cols == 0 || x), FX_ERR_ARG, "fx_csr_matvec16: null array");
+    fx::require(!fx::overlaps(x, std::size_t(cols) * 8, y, std::size_t(rows) * 8),
+                FX_ERR_ALIAS, "CSR matvec does not support in-place views");
+    fx::require(vals != y, FX_ERR_ALIAS, "CSR matvec output must not alias matrix values");
+    hipLaunchKernelGGL(k_spmv16, dim3(fx::blocks_for(rows, kRowBlock)), dim3(kRowBlock), 0,
+                       fx::as_stream(stream), rows, row_ptr, reinterpret_cast<const short*>(col_off),
+                       vals, x, y, alpha, beta);
+    fx::check_launch("fx_csr_matvec16");
+  });
+}
+
+int fx_csr_col_off16(fx_index rows, const fx_index* row_ptr, const fx_index* col_ind,
+                     int16_t* col_off)
+{
+  return fx::guard([&] {
+    fx::require(rows >= 0 && row_ptr, FX_ERR_ARG, "fx_csr_col_off16: invalid arguments");
+    fx::require(row_ptr[rows] == 0 || (col_ind && col_off), FX_ERR_ARG,
+                "fx_csr_col_off16: null array");
+    for (fx_index r = 0; r < rows; ++r)
+      for (fx_index k = row_ptr[r]; k < row_ptr[r + 1]; ++k)
+      {
+        const long d = static_cast<long>(col_ind[k]) - r;
+        fx::require(d >= -32768 && d <= 32767, FX_ERR_UNSUPPORTED,
+                    "fx_csr_col_off16: a column offset does not fit 16 bits");
+        col_off[k] = static_cast<int16_t>(d);
+      }
   });
 }
 

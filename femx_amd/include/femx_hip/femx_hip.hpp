@@ -207,12 +207,32 @@ public:
     return *tmap_;
   }
 
+  /** Cached 16-bit column offsets (col_ind - row) on the device, or null when
+   * the pattern's bandwidth does not fit 16 bits (fx_csr_col_off16). */
+  const DeviceArray<std::int16_t>* colOff16(HipContext& ctx) const
+  {
+    if (!off16_tried_)
+    {
+      off16_tried_ = true;
+      std::vector<std::int16_t> off(nnz());
+      if (fx_csr_col_off16(rows_, host_.row_ptr.data(), host_.col_ind.data(), off.data()) == FX_OK)
+      {
+        auto d = std::make_shared<DeviceArray<std::int16_t>>();
+        d->assign(off, ctx);
+        off16_ = std::move(d);
+      }
+    }
+    return off16_.get();
+  }
+
 private:
-  Index                                      rows_ = 0, cols_ = 0;
-  std::uint64_t                              layout_id_ = 0;
-  HostCsrPattern                             host_;
-  DeviceIndexVector                          row_ptr_, col_ind_;
-  mutable std::shared_ptr<TransposeMap>      tmap_;
+  Index                                             rows_ = 0, cols_ = 0;
+  std::uint64_t                                     layout_id_ = 0;
+  HostCsrPattern                                    host_;
+  DeviceIndexVector                                 row_ptr_, col_ind_;
+  mutable std::shared_ptr<TransposeMap>             tmap_;
+  mutable std::shared_ptr<DeviceArray<std::int16_t>> off16_;
+  mutable bool                                      off16_tried_ = false;
 };
 
 /** Device CSR matrix: shared pattern, owned values. */
@@ -533,8 +553,12 @@ public:
               Real beta = 0.0) const
   {
     require(nx == A.cols() && ny == A.rows(), "CSR matvec vector size mismatch");
-    check(fx_csr_matvec(A.rows(), A.cols(), A.pattern().rowPtrData(), A.pattern().colIndData(),
-                        A.valsData(), x, y, alpha, beta, ctx_.raw()));
+    if (const auto* off = A.pattern().colOff16(ctx_)) // narrower index stream, same results
+      check(fx_csr_matvec16(A.rows(), A.cols(), A.pattern().rowPtrData(), off->data(), A.valsData(),
+                            x, y, alpha, beta, ctx_.raw()));
+    else
+      check(fx_csr_matvec(A.rows(), A.cols(), A.pattern().rowPtrData(), A.pattern().colIndData(),
+                          A.valsData(), x, y, alpha, beta, ctx_.raw()));
   }
   void matvecT(const DeviceCsrMatrix& A, const Real* x, Index nx, Real* y, Index ny,
                Real alpha = 1.0, Real beta = 0.0) const

@@ -63,6 +63,7 @@ class CsrPattern:
         self.layout_id = next(_layout_ids) if layout_id is None else layout_id
         self._dev = {}
         self._tmap = {}
+        self._off16 = {}
 
     @property
     def nnz(self):
@@ -75,6 +76,17 @@ class CsrPattern:
             self._dev[key] = (torch.from_numpy(self.row_ptr).to(dev),
                               torch.from_numpy(self.col_ind).to(dev))
         return self._dev[key]
+
+    def col_off16(self, device="cuda"):
+        """16-bit column offsets (col_ind - row) on the device, built once per
+        device; None when the pattern's bandwidth does not fit 16 bits."""
+        dev = torch.device(device)
+        key = str(dev)
+        if key not in self._off16:
+            off = np.empty(max(self.nnz, 1), np.int16)
+            rc = _lib.lib().fx_csr_col_off16(self.rows, ptr(self.row_ptr), ptr(self.col_ind), ptr(off))
+            self._off16[key] = torch.from_numpy(off[: self.nnz]).to(dev) if rc == 0 else None
+        return self._off16[key]
 
     def transpose_map(self, device="cuda"):
         """(t_ptr, t_row, t_perm) device tensors, built once per device."""
@@ -146,8 +158,13 @@ class MatrixHandler:
         _check_dev_vec(x, mat.cols, torch.float64, "CSR matvec input")
         _check_dev_vec(y, mat.rows, torch.float64, "CSR matvec output")
         rp, ci = mat.pattern.device_arrays(mat.device)
-        _lib.call("fx_csr_matvec", mat.rows, mat.cols, ptr(rp), ptr(ci), ptr(mat.vals), ptr(x),
-                  ptr(y), float(alpha), float(beta), stream_handle(self.stream))
+        off = mat.pattern.col_off16(mat.device)
+        if off is not None:  # the pattern's column offsets fit 16 bits: narrower index stream
+            _lib.call("fx_csr_matvec16", mat.rows, mat.cols, ptr(rp), ptr(off), ptr(mat.vals), ptr(x),
+                      ptr(y), float(alpha), float(beta), stream_handle(self.stream))
+        else:
+            _lib.call("fx_csr_matvec", mat.rows, mat.cols, ptr(rp), ptr(ci), ptr(mat.vals), ptr(x),
+                      ptr(y), float(alpha), float(beta), stream_handle(self.stream))
         return y
 
     def matvecT(self, mat: CsrMatrix, x, y, alpha=1.0, beta=0.0):

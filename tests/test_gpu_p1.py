@@ -172,6 +172,37 @@ def test_linalg_kats():
                   buf.data_ptr() + 32, 1.0, 0.0, stream_handle())
 
 
+@pytest.mark.parametrize("bandwidth", [50, 80000])
+def test_spmv_16bit_offsets_match_int32_path(bandwidth):
+    """MatrixHandler.matvec takes the 16-bit column-offset path when the
+    pattern allows it; both paths give bitwise equal results (random
+    rectangular pattern with empty rows, negative and positive offsets)."""
+    from femx_amd import _lib
+    from femx_amd.linalg import ptr, stream_handle
+    rng = np.random.default_rng(11)
+    rows, cols = 3000, 3000 + bandwidth
+    counts = rng.integers(0, 9, rows)
+    counts[::17] = 0
+    rp = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(np.arange(max(0, r - bandwidth // 2),
+                                                      min(cols, r + bandwidth // 2)),
+                                            c, replace=False)) for r, c in enumerate(counts)])
+    pat = CsrPattern(rows, cols, rp, ci.astype(np.int32))
+    assert (pat.col_off16(DEV) is None) == (bandwidth > 32767 * 2)
+    m = CsrMatrix(pat, DEV, dev(rng.standard_normal(pat.nnz)))
+    x, y0 = dev(rng.standard_normal(cols)), dev(rng.standard_normal(rows))
+    for alpha, beta in ((1.0, 0.0), (-0.7, 1.3)):
+        y1, y2 = y0.clone(), y0.clone()
+        MatrixHandler().matvec(m, x, y1, alpha, beta)
+        d_rp, d_ci = pat.device_arrays(DEV)
+        _lib.call("fx_csr_matvec", rows, cols, ptr(d_rp), ptr(d_ci), ptr(m.vals), ptr(x), ptr(y2),
+                  alpha, beta, stream_handle())
+        assert torch.equal(y1, y2)
+        ref = np.zeros(rows)
+        O.matvec(rp, pat.col_ind, host(m.vals), host(x), ref)
+        assert np.array_equal(host(y1), alpha * ref + beta * host(y0) if beta else alpha * ref)
+
+
 def test_dot_is_deterministic_and_accurate():
     vh = VectorHandler()
     x = dev(RNG.standard_normal(3_000_001))

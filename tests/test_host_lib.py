@@ -177,6 +177,21 @@ def test_transpose_map():
     assert vals[t_perm].tolist() == [2.0, -2.0, 3.0, -1.0, 5.0, 4.0, 1.0]
 
 
+def test_col_off16():
+    pat = CsrPattern(3, 4, [0, 2, 4, 7], [0, 2, 1, 3, 0, 2, 3])
+    off = np.empty(7, np.int16)
+    _lib.call("fx_csr_col_off16", 3, ptr(pat.row_ptr), ptr(pat.col_ind), ptr(off))
+    assert off.tolist() == [0, 2, 0, 2, -2, 0, 1]
+    assert pat.col_off16("cpu").tolist() == off.tolist()
+    # an offset of 32768 does not fit: the builder refuses, the pattern keeps int32 only
+    wide = CsrPattern(2, 40000, [0, 1, 2], [32768, 1])
+    with pytest.raises(RuntimeError, match="16 bits"):
+        _lib.call("fx_csr_col_off16", 2, ptr(wide.row_ptr), ptr(wide.col_ind), ptr(off))
+    assert wide.col_off16("cpu") is None
+    edge = CsrPattern(2, 40000, [0, 1, 2], [32767, 1 - 32768 + 32768])
+    assert edge.col_off16("cpu").tolist() == [32767, 0]
+
+
 @pytest.mark.parametrize("n", [1, 3, 16])
 def test_meshes_match_oracle(n):
     N, E = (n + 1) ** 2, 2 * n * n
