@@ -1,0 +1,83 @@
+// GLS P1/P1 Navier-Stokes time-step assembly: kernel and C-ABI entry point.
+// The device code and its design notes are in fx_gls_dev.hpp.
+#include <cmath>
+
+#include "fx_gls_dev.hpp"
+
+namespace
+{
+using namespace fxk;
+
+constexpr int kGlsMaxTile = 8192; // doubles of LDS row tile per wave (64 KiB)
+
+__global__ __launch_bounds__(kGlsBlock) void k_gls_assemble(GlsAsmArgs a)
+{
+  extern __shared__ double tile[];
+  gls_assemble_block(fx::xcd_block(gridDim.x), a, tile);
+}
+
+} // namespace
+
+extern "C" {
+
+int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems, fx_index nnz,
+                    fx_index max_tile, const double* x, const double* y, const fx_index* conn3,
+                    const fx_index* node_rows, double rho, double mu, double dt, fx_index step,
+                    fx_index num_hist, const double* hist, const double* nxt,
+                    const fx_index* inc_off, const fx_index* inc_elem, const fx_index* inc_meta,
+                    const fx_index* blk_runs, const fx_index* node_tile, double* vals,
+                    double* res, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(num_nodes >= 0 && num_states >= 0 && num_elems >= 0 && nnz >= 0 && max_tile >= 0,
+                FX_ERR_ARG, "fx_gls_assemble: negative size");
+    // NavierStokesModel's parameter checks (femx/model/ns/NavierStokesModel.cpp:35-46)
+    fx::require(dt > 0.0 && std::isfinite(dt), FX_ERR_ARG,
+                "NavierStokesModel requires a positive finite time step");
+    fx::require(std::isfinite(rho) && rho > 0.0, FX_ERR_ARG,
+                "NavierStokesModel requires positive finite density");
+    fx::require(std::isfinite(mu) && mu > 0.0, FX_ERR_ARG,
+                "NavierStokesModel requires positive finite viscosity");
+    // checkTimeAssemblyInputs (femx/assembly/Assembly.hpp:86-104)
+    fx::require(num_hist > 0, FX_ERR_ARG,
+                "Time assembly state dimensions do not match AssemblyMap");
+    if (num_states == 0) return;
+    fx::require(num_nodes > 0 && x && y && node_rows && hist && nxt && inc_off && blk_runs
+                    && node_tile && res && (num_elems == 0 || (conn3 && inc_elem && inc_meta)),
+                FX_ERR_ARG, "fx_gls_assemble: null array");
+    const std::size_t S = num_states, H = S * num_hist * 8, V = std::size_t(nnz) * 8;
+    fx::require(!fx::overlaps(res, S * 8, hist, H) && !fx::overlaps(res, S * 8, nxt, S * 8)
+                    && !fx::overlaps(res, S * 8, vals, V) && !fx::overlaps(vals, V, hist, H)
+                    && !fx::overlaps(vals, V, nxt, S * 8),
+                FX_ERR_ALIAS, "Time assembly outputs must not alias inputs or each other");
+    fx::require(!vals || max_tile <= kGlsMaxTile, FX_ERR_UNSUPPORTED,
+                "fx_gls_assemble: a block's rows exceed the 64 KiB LDS tile");
+    const unsigned  nb  = fx::blocks_for(num_nodes, kGlsBlock);
+    const size_t    lds = vals ? static_cast<size_t>(max_tile > 0 ? max_tile : 1) * 8 : 0;
+    const GlsAsmArgs a{num_nodes,
+                       num_states,
+                       x,
+                       y,
+                       conn3,
+                       node_rows,
+                       step,
+                       num_hist,
+                       hist,
+                       nxt,
+                       rho,
+                       mu,
+                       dt,
+                       inc_off,
+                       inc_elem,
+                       reinterpret_cast<const uint4*>(inc_meta),
+                       reinterpret_cast<const int4*>(blk_runs),
+                       node_tile,
+                       vals,
+                       res,
+                       max_tile};
+    hipLaunchKernelGGL(k_gls_assemble, dim3(nb), dim3(kGlsBlock), lds, fx::as_stream(stream), a);
+    fx::check_launch("fx_gls_assemble");
+  });
+}
+
+} // extern "C"

@@ -452,6 +452,134 @@ int fx_map_th_blocks(const fx_map* m, Index lanes, Index max_nnz, Index* num_blo
   });
 }
 
+// Node plan of the GLS P1/P1 kernel.  A node's three rows are read off the
+// element DOF tables (local rows 2k, 2k+1 and 6+k of local node k); every
+// element must agree, and every row must belong to exactly one node.  Blocks
+// of FX_GLS_BLOCK_NODES consecutive nodes list their rows as runs of
+// consecutive rows, whose CSR ranges are laid end to end in the LDS tile.
+int fx_map_gls_plan(const fx_map* m, Index num_nodes, const Index* conn3, Index* node_rows,
+                    Index* inc_off, Index* inc_elem, Index* inc_meta, Index* blk_runs,
+                    Index* node_tile, Index* max_tile)
+{
+  return fx::guard([&] {
+    fx::require(m && num_nodes >= 0 && node_rows && inc_off && blk_runs && node_tile && max_tile
+                    && (m->num_elems == 0 || (conn3 && inc_elem && inc_meta)),
+                FX_ERR_ARG, "fx_map_gls_plan: invalid arguments");
+    const Index E = m->num_elems, N = num_nodes;
+    fx::require(m->num_res == m->num_states, FX_ERR_UNSUPPORTED,
+                "fx_map_gls_plan: residual and state spaces differ");
+    for (Index e = 0; e < E; ++e)
+    {
+      fx::require(m->res_off[e + 1] - m->res_off[e] == 9 && m->st_off[e + 1] - m->st_off[e] == 9,
+                  FX_ERR_UNSUPPORTED, "fx_map_gls_plan: every element needs 9x9 local DOFs");
+      for (int k = 0; k < 9; ++k)
+        fx::require(m->res_dofs[m->res_off[e] + k] == m->st_dofs[m->st_off[e] + k],
+                    FX_ERR_UNSUPPORTED, "fx_map_gls_plan: residual and state DOF tables differ");
+    }
+    fx::require(m->max_row_len <= 255, FX_ERR_UNSUPPORTED,
+                "fx_map_gls_plan: CSR rows longer than 255 entries");
+    std::fill(node_rows, node_rows + 3 * std::size_t(N), Index(-1));
+    std::vector<Index> cnt(std::size_t(N) + 1, 0);
+    for (Index e = 0; e < E; ++e)
+      for (int k = 0; k < 3; ++k)
+      {
+        const Index v = conn3[3 * e + k];
+        fx::require(v >= 0 && v < N, FX_ERR_ARG, "fx_map_gls_plan: node id out of range");
+        const Index* d    = m->res_dofs.data() + m->res_off[e];
+        const Index  r[3] = {d[2 * k], d[2 * k + 1], d[6 + k]};
+        Index*       nr   = node_rows + 3 * std::size_t(v);
+        if (nr[0] < 0)
+          std::copy(r, r + 3, nr);
+        else
+          fx::require(nr[0] == r[0] && nr[1] == r[1] && nr[2] == r[2], FX_ERR_UNSUPPORTED,
+                      "fx_map_gls_plan: elements disagree on a node's rows");
+        ++cnt[v + 1];
+      }
+    std::vector<Index> owner(m->num_res, -1);
+    for (Index n = 0; n < N; ++n)
+      for (int k = 0; k < 3 && node_rows[3 * n] >= 0; ++k)
+      {
+        const Index r = node_rows[3 * n + k];
+        fx::require(owner[r] < 0, FX_ERR_UNSUPPORTED,
+                    "fx_map_gls_plan: a row belongs to two nodes or twice to one");
+        owner[r] = n;
+      }
+    for (Index r = 0; r < m->num_res; ++r)
+      fx::require(owner[r] >= 0, FX_ERR_UNSUPPORTED, "fx_map_gls_plan: a row belongs to no node");
+    for (Index n = 0; n < N; ++n) cnt[n + 1] += cnt[n];
+    std::copy(cnt.begin(), cnt.end(), inc_off);
+    std::vector<Index> li_of(3 * std::size_t(E));
+    {
+      std::vector<Index> fill(cnt.begin(), cnt.end() - 1);
+      for (Index e = 0; e < E; ++e)
+        for (int k = 0; k < 3; ++k)
+        {
+          const Index t = fill[conn3[3 * e + k]]++;
+          inc_elem[t]   = e;
+          li_of[t]      = k;
+        }
+    }
+    std::vector<std::string> errs(1);
+    fx::parallel_ranges(N, [&](std::int64_t n0, std::int64_t n1) {
+      for (std::int64_t n = n0; n < n1; ++n)
+        for (Index t = cnt[n]; t < cnt[n + 1]; ++t)
+        {
+          const Index  e = inc_elem[t], ln = li_of[t];
+          std::uint8_t b[32] = {};
+          for (int kk = 0; kk < 3; ++kk)
+          {
+            const Index  lr   = kk < 2 ? 2 * ln + kk : 6 + ln;
+            const Index* jm   = m->jac_map.data() + m->jac_off[e] + 9 * lr;
+            const Index  base = m->row_ptr[node_rows[3 * n + kk]];
+            for (int j = 0; j < 9; ++j) b[9 * kk + j] = static_cast<std::uint8_t>(jm[j] - base);
+          }
+          b[27] = static_cast<std::uint8_t>(ln);
+          std::memcpy(inc_meta + 8 * std::size_t(t), b, 32);
+        }
+    });
+    const Index nb = (N + FX_GLS_BLOCK_NODES - 1) / FX_GLS_BLOCK_NODES;
+    std::fill(blk_runs, blk_runs + std::size_t(nb) * 4 * FX_GLS_MAX_RUNS, Index(0));
+    Index mx = 0;
+    for (Index b = 0; b < nb; ++b)
+    {
+      const Index        n0 = b * FX_GLS_BLOCK_NODES, n1 = std::min(N, n0 + FX_GLS_BLOCK_NODES);
+      std::vector<Index> rows;
+      for (Index n = n0; n < n1; ++n)
+        if (node_rows[3 * n] >= 0) rows.insert(rows.end(), node_rows + 3 * n, node_rows + 3 * n + 3);
+      std::sort(rows.begin(), rows.end());
+      Index* runs = blk_runs + std::size_t(b) * 4 * FX_GLS_MAX_RUNS;
+      Index  run_row[FX_GLS_MAX_RUNS + 1];
+      int    nr   = 0;
+      Index  tile = 0;
+      for (std::size_t i = 0; i < rows.size();)
+      {
+        std::size_t j = i + 1;
+        while (j < rows.size() && rows[j] == rows[j - 1] + 1) ++j;
+        fx::require(nr < FX_GLS_MAX_RUNS, FX_ERR_UNSUPPORTED,
+                    "fx_map_gls_plan: a block's rows need more than FX_GLS_MAX_RUNS runs");
+        const Index c0 = m->row_ptr[rows[i]], c1 = m->row_ptr[rows[j - 1] + 1];
+        runs[4 * nr]     = c0;
+        runs[4 * nr + 1] = c1 - c0;
+        runs[4 * nr + 2] = tile;
+        run_row[nr]      = rows[i];
+        ++nr;
+        tile += c1 - c0;
+        i = j;
+      }
+      mx = std::max(mx, tile);
+      for (Index n = n0; n < n1; ++n)
+        for (int k = 0; k < 3; ++k)
+        {
+          const Index r = node_rows[3 * n + k];
+          int         q = nr - 1;
+          while (q > 0 && run_row[q] > r) --q; // the run holding row r
+          node_tile[3 * n + k] = r < 0 ? 0 : runs[4 * q + 2] + m->row_ptr[r] - runs[4 * q];
+        }
+    }
+    *max_tile = mx;
+  });
+}
+
 void fx_map_destroy(fx_map* m) { delete m; }
 
 // ---------------------------------------------------------------------------

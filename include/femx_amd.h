@@ -120,6 +120,30 @@ int fx_map_export_rowinc15(const fx_map* map, fx_index* h_inc_off, fx_index* h_i
 #define FX_TH_TILE_NNZ 0     /* default tile bound (0: none; items bound the tile) */
 int fx_map_th_blocks(const fx_map* map, fx_index lanes, fx_index max_nnz, fx_index* num_blocks,
                      fx_index* max_block_nnz, fx_index* h_blk_row, fx_index* h_blk_flags);
+/* Node plan of the GLS P1/P1 Navier-Stokes kernel (fx_gls_assemble) over a
+ * map whose elements have 9 local DOFs in NavierOperator order -- u0 v0 u1
+ * v1 u2 v2 p0 p1 p2 (femx/model/ns/Components.hpp:247-256) -- for the mesh
+ * conn3[3E] of num_nodes nodes.  Outputs (host):
+ *   node_rows[3N]  velocity x, velocity y and pressure row of each node
+ *                  (-1 for a node of no element);
+ *   inc_off[N+1], inc_elem[3E]  each node's incident elements, ascending;
+ *   inc_meta[8 per incidence]   32 bytes: byte 9k+j is the position of local
+ *                  column j in the node's row k (from row_ptr[row]), byte 27
+ *                  the node's local index;
+ *   blk_runs[4*FX_GLS_MAX_RUNS per block of FX_GLS_BLOCK_NODES nodes]: the
+ *                  block's rows as runs of consecutive rows -- CSR begin,
+ *                  CSR length, LDS tile offset, 0 (unused runs all zero);
+ *   node_tile[3N]  tile offset of each node row's first entry;
+ *   max_tile       the largest block tile, in doubles.
+ * FX_ERR_UNSUPPORTED when the element tables disagree on a node's rows, a
+ * row belongs to no node, a block's rows need more than FX_GLS_MAX_RUNS runs
+ * or a row is longer than 255 entries. */
+#define FX_GLS_BLOCK_NODES 64
+#define FX_GLS_MAX_RUNS 4
+int fx_map_gls_plan(const fx_map* map, fx_index num_nodes, const fx_index* h_conn3,
+                    fx_index* h_node_rows, fx_index* h_inc_off, fx_index* h_inc_elem,
+                    fx_index* h_inc_meta, fx_index* h_blk_runs, fx_index* h_node_tile,
+                    fx_index* max_tile);
 void fx_map_destroy(fx_map* map);
 
 /* CSR transpose permutation (host), for the deterministic transpose apply:
@@ -281,6 +305,26 @@ int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* d_xv, const doubl
               double nu, const double* d_nu_e, const double* d_lam, const fx_index* d_bc_mask,
               const fx_index* d_inc_off, const fx_index* d_elem_inc, double* d_jt, double* d_g,
               double* d_work, void* stream);
+
+/* ======================================================================== */
+/* GLS-stabilised P1/P1 Navier-Stokes time step (device): femx's
+ * NavierOperator (femx/model/ns/Components.hpp:400-577) assembled with
+ * respect to the next state, as assembly::assemble with a time operator does
+ * (femx/assembly/CudaAssembly.hpp:366-417 on the device, Assembly.hpp:235-297
+ * on the host).  hist is lag-major (num_hist x num_states; lags 0 and 1 are
+ * read), nxt the next state; rho, mu, dt as NavierStokesModel's FluidParams
+ * and time step; tables from fx_map_gls_plan.  Writes res[num_states] =
+ * -F + K nxt and, when vals is non-null, the CSR values of K (vals NULL is
+ * assembleResidual, CudaAssembly.hpp:419-472).  Outputs are replaced.  Sums
+ * in ascending element order with femx's operation order and no
+ * contraction: equal to femx's host assembly bit for bit. */
+int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems, fx_index nnz,
+                    fx_index max_tile, const double* d_x, const double* d_y,
+                    const fx_index* d_conn3, const fx_index* d_node_rows, double rho, double mu,
+                    double dt, fx_index step, fx_index num_hist, const double* d_hist,
+                    const double* d_nxt, const fx_index* d_inc_off, const fx_index* d_inc_elem,
+                    const fx_index* d_inc_meta, const fx_index* d_blk_runs,
+                    const fx_index* d_node_tile, double* d_vals, double* d_res, void* stream);
 
 /* ======================================================================== */
 /* Batches (device).  Many P1 Poisson and Taylor-Hood instances of uneven size

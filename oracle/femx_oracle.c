@@ -794,3 +794,235 @@ void or_th_vjp(const double* xv, const double* yv, or_index num_elems,
     for (or_index i = 0; i < ndof; ++i)
       if (bc_mask[i]) jt_lam[i] += lam[i];
 }
+
+/* ------------------------------------------------------------------------ */
+/* GLS-stabilised P1/P1 Navier-Stokes time step on triangles: the femx
+ * NavierOperator (femx/model/ns/Components.hpp:230-577) restated for dim 2,
+ * three nodes and the order-2 rule NavierStokesModel uses
+ * (femx/model/ns/NavierStokesModel.cpp:33), with element data as
+ * makeNavierData stores it (femx/model/ns/Components.cpp:19-62).  Local DOF
+ * order: velocity interleaved by node (vdof = 2*in + c), then pressure
+ * (pdof = 6 + in) (Components.hpp:247-256).  Every expression keeps femx's
+ * operation order so that, built without contraction, the results equal the
+ * compiled reference's bit for bit. */
+typedef struct
+{
+  double u[2], adv[2], grad[2][2], adv_grad[2], tau;
+} gls_qp;
+
+/* elemLength (Components.hpp:286-327). */
+static double gls_elem_length(const double* dNdx, const gls_qp* q)
+{
+  double speed2 = 0.0;
+  for (int d = 0; d < 2; ++d) speed2 += q->u[d] * q->u[d];
+  const double speed = sqrt(speed2);
+  double       dir[2];
+  if (speed > 1.0e-10)
+    for (int d = 0; d < 2; ++d) dir[d] = q->u[d] / speed;
+  else
+  {
+    const double val = 1.0 / sqrt(2.0);
+    for (int d = 0; d < 2; ++d) dir[d] = val;
+  }
+  double sum = 0.0;
+  for (int in = 0; in < 3; ++in)
+  {
+    double g = 0.0;
+    for (int d = 0; d < 2; ++d) g += dir[d] * dNdx[in * 2 + d];
+    sum += g >= 0.0 ? g : -g;
+  }
+  return sum > 1.0e-14 ? 2.0 / sum : 0.0;
+}
+
+/* evalQp (Components.hpp:329-396): cur = history lag 0, prev = lag 1. */
+static void gls_eval_qp(const double* N, const double* dNdx, int step, int num_hist,
+                        const double* cur, const double* prev, double rho, double mu, double dt,
+                        gls_qp* q)
+{
+  memset(q, 0, sizeof(*q));
+  for (int in = 0; in < 3; ++in)
+    for (int c = 0; c < 2; ++c)
+    {
+      const int    id  = 2 * in + c;
+      const double val = cur[id];
+      q->u[c] += N[in] * val;
+      for (int d = 0; d < 2; ++d) q->grad[c][d] += dNdx[in * 2 + d] * val;
+      double adv = val;
+      if (step > 0 && num_hist > 1) adv = 1.5 * val - 0.5 * prev[id];
+      q->adv[c] += N[in] * adv;
+    }
+  double speed2 = 0.0;
+  for (int c = 0; c < 2; ++c)
+  {
+    speed2 += q->u[c] * q->u[c];
+    for (int d = 0; d < 2; ++d) q->adv_grad[c] += q->grad[c][d] * q->adv[d];
+  }
+  const double speed = sqrt(speed2);
+  const double h     = gls_elem_length(dNdx, q);
+  const double nu    = mu / rho;
+  const double time  = (2.0 / dt) * (2.0 / dt);
+  double       flow = 0.0, diff = 0.0;
+  if (h > 0.0)
+  {
+    flow = (2.0 * speed / h) * (2.0 * speed / h);
+    diff = (4.0 * nu / (h * h)) * (4.0 * nu / (h * h));
+  }
+  q->tau = 1.0 / sqrt(time + flow + diff);
+}
+
+/* Element matrix (row-major 9x9) and residual of NavierOperator::evalRow with
+ * respect to the next state (Components.hpp:412-428,473-577):
+ * Ke = dR/d(next), Re = -F + Ke * nxt_e.  hist_e is lag-major (num_hist x 9). */
+void or_gls_element(const double* xe, const double* ye, double rho, double mu, double dt,
+                    int step, int num_hist, const double* hist_e, const double* nxt_e,
+                    double* Ke, double* Re)
+{
+  double rs[6], w[3], invJ[4], jxw[3], N[3][3], dNdx[3][6];
+  static const double dNdr[6] = {-1.0, -1.0, 1.0, 0.0, 0.0, 1.0};
+  or_quad_tri(2, rs, w);
+  for (int q = 0; q < 3; ++q)
+  {
+    jxw[q] = tri_geometry(xe, ye, w[q], invJ);
+    phys_grad(3, dNdr, invJ, dNdx[q]);
+    N[q][0] = 1.0 - rs[2 * q] - rs[2 * q + 1];
+    N[q][1] = rs[2 * q];
+    N[q][2] = rs[2 * q + 1];
+  }
+  const double* cur  = hist_e;
+  const double* prev = num_hist > 1 ? hist_e + 9 : hist_e;
+  for (int row = 0; row < 9; ++row)
+  {
+    double*      K      = Ke + 9 * row;
+    const int    vel    = row < 6;
+    const int    i      = vel ? row / 2 : row - 6;
+    const int    c      = vel ? row - i * 2 : 0;
+    double       F      = 0.0;
+    for (int col = 0; col < 9; ++col) K[col] = 0.0;
+    for (int iq = 0; iq < 3; ++iq)
+    {
+      gls_qp qp;
+      gls_eval_qp(N[iq], dNdx[iq], step, num_hist, cur, prev, rho, mu, dt, &qp);
+      const double* g   = dNdx[iq];
+      const double  Jw  = jxw[iq];
+      const double  Ni  = N[iq][i];
+      const double  dvi = 0.0 + g[2 * i] * qp.adv[0] + g[2 * i + 1] * qp.adv[1];
+      if (vel)
+      {
+        const double grad_u = 0.0 + g[2 * i] * qp.grad[c][0] + g[2 * i + 1] * qp.grad[c][1];
+        F += (rho / dt * Ni * qp.u[c] - 0.5 * rho * Ni * qp.adv_grad[c] - 0.5 * mu * grad_u
+              + qp.tau * rho / dt * dvi * qp.u[c] - 0.5 * qp.tau * rho * dvi * qp.adv_grad[c])
+             * Jw;
+        for (int j = 0; j < 3; ++j)
+        {
+          const double Nj  = N[iq][j];
+          const double dvj = 0.0 + g[2 * j] * qp.adv[0] + g[2 * j + 1] * qp.adv[1];
+          const double gd  = 0.0 + g[2 * i] * g[2 * j] + g[2 * i + 1] * g[2 * j + 1];
+          K[2 * j + c] += (rho / dt * Ni * Nj + 0.5 * rho * Ni * dvj + 0.5 * mu * gd
+                           + qp.tau * rho / dt * dvi * Nj + 0.5 * qp.tau * rho * dvi * dvj)
+                          * Jw;
+          K[6 + j] += (-g[2 * i + c] * Nj + qp.tau * dvi * g[2 * j + c]) * Jw;
+        }
+      }
+      else
+      {
+        double div_u = 0.0, div_adv = 0.0;
+        for (int d = 0; d < 2; ++d)
+        {
+          div_u += g[2 * i + d] * qp.u[d];
+          div_adv += g[2 * i + d] * qp.adv_grad[d];
+        }
+        F += (qp.tau / dt * div_u - 0.5 * qp.tau * div_adv) * Jw;
+        for (int j = 0; j < 3; ++j)
+        {
+          const double Nj  = N[iq][j];
+          const double dvj = 0.0 + g[2 * j] * qp.adv[0] + g[2 * j + 1] * qp.adv[1];
+          for (int d = 0; d < 2; ++d)
+          {
+            const double gi = g[2 * i + d];
+            K[2 * j + d] += (Ni * g[2 * j + d] + qp.tau / dt * gi * Nj + 0.5 * qp.tau * gi * dvj)
+                            * Jw;
+          }
+          const double gd = 0.0 + g[2 * i] * g[2 * j] + g[2 * i + 1] * g[2 * j + 1];
+          K[6 + j] += qp.tau / rho * gd * Jw;
+        }
+      }
+    }
+    double out = -F;
+    for (int col = 0; col < 9; ++col) out += K[col] * nxt_e[col];
+    Re[row] = out;
+  }
+}
+
+static void gls_gather(const double* x, const double* y, const or_index* c,
+                       const or_index* d9, or_index num_states, int num_hist,
+                       const double* hist, const double* nxt, double* xe, double* ye,
+                       double* he, double* ne)
+{
+  for (int k = 0; k < 3; ++k)
+  {
+    xe[k] = x[c[k]];
+    ye[k] = y[c[k]];
+  }
+  for (int lag = 0; lag < num_hist; ++lag)
+    for (int col = 0; col < 9; ++col)
+      he[lag * 9 + col] = hist[(size_t)lag * num_states + d9[col]];
+  for (int col = 0; col < 9; ++col) ne[col] = nxt[d9[col]];
+}
+
+/* Host time assemble (femx/assembly/Assembly.hpp:235-297) of the GLS
+ * operator: outputs replaced, elements in ascending order, res[resDof] +=
+ * and vals[jacIndex] += per element.  num_hist <= 8.  vals may be NULL
+ * (residual only, as assembleResidual). */
+void or_gls_assemble(or_index num_states, const double* x, const double* y, or_index num_elems,
+                     const or_index* conn, const or_index* dofs9, double rho, double mu,
+                     double dt, int step, int num_hist, const double* hist, const double* nxt,
+                     const or_index* jac_map, or_index nnz, double* vals, double* res)
+{
+  memset(res, 0, sizeof(double) * num_states);
+  if (vals) memset(vals, 0, sizeof(double) * nnz);
+  const int nh = num_hist < 2 ? num_hist : 2; /* only lags 0 and 1 are read */
+  for (or_index e = 0; e < num_elems; ++e)
+  {
+    double xe[3], ye[3], he[18], ne[9], Ke[81], Re[9];
+    gls_gather(x, y, conn + 3 * e, dofs9 + 9 * e, num_states, nh, hist, nxt, xe, ye, he, ne);
+    or_gls_element(xe, ye, rho, mu, dt, step, num_hist, he, ne, Ke, Re);
+    for (int r = 0; r < 9; ++r) res[dofs9[9 * e + r]] += Re[r];
+    if (vals)
+      for (int i = 0; i < 81; ++i) vals[jac_map[81 * e + i]] += Ke[i];
+  }
+}
+
+/* The same with femx's OpenMP shape (NavierStokesModel.cpp:269-318): the CPU
+ * baseline.  Atomic scatter, so the sums are not in element order. */
+void or_gls_assemble_omp(or_index num_states, const double* x, const double* y,
+                         or_index num_elems, const or_index* conn, const or_index* dofs9,
+                         double rho, double mu, double dt, int step, int num_hist,
+                         const double* hist, const double* nxt, const or_index* jac_map,
+                         or_index nnz, double* vals, double* res)
+{
+  const int nh = num_hist < 2 ? num_hist : 2;
+#pragma omp parallel for schedule(static)
+  for (or_index i = 0; i < num_states; ++i) res[i] = 0.0;
+#pragma omp parallel for schedule(static)
+  for (or_index k = 0; k < nnz; ++k) vals[k] = 0.0;
+#pragma omp parallel
+  {
+    double xe[3], ye[3], he[18], ne[9], Ke[81], Re[9];
+#pragma omp for schedule(static)
+    for (or_index e = 0; e < num_elems; ++e)
+    {
+      gls_gather(x, y, conn + 3 * e, dofs9 + 9 * e, num_states, nh, hist, nxt, xe, ye, he, ne);
+      or_gls_element(xe, ye, rho, mu, dt, step, num_hist, he, ne, Ke, Re);
+      for (int r = 0; r < 9; ++r)
+      {
+#pragma omp atomic update
+        res[dofs9[9 * e + r]] += Re[r];
+      }
+      for (int i = 0; i < 81; ++i)
+      {
+#pragma omp atomic update
+        vals[jac_map[81 * e + i]] += Ke[i];
+      }
+    }
+  }
+}

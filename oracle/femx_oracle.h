@@ -103,6 +103,20 @@ void or_th_vjp(const double* xv, const double* yv, or_index num_elems,
                const double* nu_e, const double* state, const double* lam,
                const or_index* bc_mask, double* jt_lam, double* g);
 
+/* ---- GLS P1/P1 Navier-Stokes time step (femx/model/ns/Components.hpp) -- */
+void or_gls_element(const double* xe, const double* ye, double rho, double mu, double dt,
+                    int step, int num_hist, const double* hist_e, const double* nxt_e,
+                    double* Ke, double* Re);
+void or_gls_assemble(or_index num_states, const double* x, const double* y, or_index num_elems,
+                     const or_index* conn, const or_index* dofs9, double rho, double mu,
+                     double dt, int step, int num_hist, const double* hist, const double* nxt,
+                     const or_index* jac_map, or_index nnz, double* vals, double* res);
+void or_gls_assemble_omp(or_index num_states, const double* x, const double* y,
+                         or_index num_elems, const or_index* conn, const or_index* dofs9,
+                         double rho, double mu, double dt, int step, int num_hist,
+                         const double* hist, const double* nxt, const or_index* jac_map,
+                         or_index nnz, double* vals, double* res);
+
 int or_omp_threads(void);
 
 #ifdef __cplusplus

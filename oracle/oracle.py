@@ -49,6 +49,8 @@ def lib():
                      "or_matvec", "or_matvecT"):
             getattr(_LIB, name).restype = None
         _LIB.or_th_assemble.argtypes = None
+        for name in ("or_gls_element", "or_gls_assemble", "or_gls_assemble_omp"):
+            getattr(_LIB, name).restype = None
     return _LIB
 
 
@@ -76,6 +78,9 @@ def ref():
                                   + [C.c_double, C.c_double])
         _REF.ref_poisson_p1.argtypes = ([C.c_int32, _pd, _pd, C.c_int32, _pi, _pd, C.c_int32,
                                          _pi, _pi, _pd, _pd])
+        _REF.ref_ns_gls.argtypes = ([C.c_int32, _pd, _pd, C.c_int32, _pi] + [C.c_double] * 3
+                                    + [C.c_int32, C.c_int32, _pd, _pd, C.c_int32]
+                                    + [_pi, _pi, _pi, _pd, _pd])
     return _REF
 
 
@@ -261,6 +266,62 @@ def th_vjp(xv, yv, vconn3, dofs15, ndof, nu, state, lam, bc_mask=None, nu_e=None
     lib().or_th_vjp(_p(xv), _p(yv), C.c_int32(E), _p(vconn3), _p(dofs15), C.c_int32(ndof),
                     C.c_double(nu), _p(nu_e), _p(state), _p(lam), _p(bc_mask), _p(jt), _p(g))
     return jt, g
+
+
+def gls_dofs(num_nodes, conn):
+    """Element DOF table (9 per element) of femx's mixed P1/P1 space
+    (femx/model/ns/Helper.cpp:40-50, femx/fem/MixedFESpace.cpp:57-84): the
+    vector velocity field first (global 2*node + c), then pressure
+    (2*num_nodes + node); local order u0 v0 u1 v1 u2 v2 p0 p1 p2."""
+    c = conn.reshape(-1, 3).astype(np.int64)
+    d = np.empty((c.shape[0], 9), np.int64)
+    d[:, 0:6:2] = 2 * c
+    d[:, 1:6:2] = 2 * c + 1
+    d[:, 6:9] = 2 * num_nodes + c
+    return d.reshape(-1).astype(I32)
+
+
+def gls_element(xe, ye, rho, mu, dt, step, num_hist, hist_e, nxt_e):
+    Ke = np.empty(81, F64)
+    Re = np.empty(9, F64)
+    lib().or_gls_element(_p(np.ascontiguousarray(xe, F64)), _p(np.ascontiguousarray(ye, F64)),
+                         C.c_double(rho), C.c_double(mu), C.c_double(dt), C.c_int(step),
+                         C.c_int(num_hist), _p(np.ascontiguousarray(hist_e, F64)),
+                         _p(np.ascontiguousarray(nxt_e, F64)), _p(Ke), _p(Re))
+    return Ke.reshape(9, 9), Re
+
+
+def gls_assemble(x, y, conn, dofs9, num_states, rho, mu, dt, step, num_hist, hist, nxt,
+                 jac_map, nnz, omp=False, jac=True):
+    """GLS P1/P1 time assemble; hist is lag-major (num_hist x num_states)."""
+    E = conn.size // 3
+    vals = np.empty(nnz, F64) if jac else None
+    res = np.empty(num_states, F64)
+    fn = lib().or_gls_assemble_omp if omp else lib().or_gls_assemble
+    fn(C.c_int32(num_states), _p(x), _p(y), C.c_int32(E), _p(conn), _p(dofs9), C.c_double(rho),
+       C.c_double(mu), C.c_double(dt), C.c_int(step), C.c_int(num_hist), _p(hist), _p(nxt),
+       _p(jac_map), C.c_int32(nnz), _p(vals), _p(res))
+    return vals, res
+
+
+def ref_ns_gls(x, y, conn, rho, mu, dt, step, num_hist, hist, nxt):
+    """femx's own GLS time assembly (oracle/_ref); returns (dofs9, row_ptr,
+    col_ind, vals, res)."""
+    R = ref()
+    N, E = x.size, conn.size // 3
+    nd = 3 * N
+    cap = 81 * E
+    d9 = np.empty(9 * E, I32)
+    rp = np.empty(nd + 1, I32)
+    ci = np.empty(cap, I32)
+    vals = np.empty(cap, F64)
+    res = np.empty(nd, F64)
+    rc = R.ref_ns_gls(N, _p(x), _p(y), E, _p(conn), rho, mu, dt, step, num_hist,
+                      _p(np.ascontiguousarray(hist, F64)), _p(np.ascontiguousarray(nxt, F64)),
+                      cap, _p(d9), _p(rp), _p(ci), _p(vals), _p(res))
+    if rc != 0:
+        raise RuntimeError(R.ref_last_error().decode())
+    return d9, rp, ci[:rp[-1]].copy(), vals[:rp[-1]].copy(), res
 
 
 def omp_threads():

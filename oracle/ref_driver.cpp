@@ -27,6 +27,9 @@
 #include <femx/linalg/CsrMatrix.hpp>
 #include <femx/linalg/CsrPattern.hpp>
 #include <femx/linalg/handler/MatrixHandler.hpp>
+#include <femx/fem/MixedFESpace.hpp>
+#include <femx/model/ns/Components.hpp>
+#include <femx/state/TimeResidual.hpp>
 
 #include "PoissonComponents.hpp"
 
@@ -243,6 +246,63 @@ int ref_poisson_p1(Index num_nodes, const Real* x, const Real* y, Index num_elem
     std::memcpy(col_ind, p.colIndData(), sizeof(Index) * p.nnz());
     std::memcpy(vals, jac.valsData(), sizeof(Real) * p.nnz());
     for (Index i = 0; i < num_nodes; ++i) res[i] = r[i];
+  });
+}
+
+
+// GLS-stabilised P1/P1 Navier-Stokes time assembly on a triangle mesh through
+// the reference's own objects: the mixed space of NavierStokes' makeSpace
+// (femx/model/ns/Helper.cpp:40-50: vector P1 velocity, then scalar P1
+// pressure), makeNavierData with the order-2 rule
+// (femx/model/ns/Components.cpp:19-62, NavierStokesModel.cpp:33), the map of
+// DofLayout(space) (NavierStokesModel.cpp constructor) and the host time
+// assemble (femx/assembly/Assembly.hpp:235-297) of NavierOperator<Host>
+// (femx/model/ns/Components.hpp:400-577) with respect to the next state.
+// hist is lag-major (num_hist x num_states).  Writes the element DOF table
+// (9 per element, local order), the CSR pattern and values, and the residual.
+int ref_ns_gls(Index num_nodes, const Real* x, const Real* y, Index num_elems, const Index* conn,
+               Real rho, Real mu, Real dt, Index step, Index num_hist, const Real* hist,
+               const Real* nxt, Index nnz_cap, Index* elem_dofs, Index* row_ptr, Index* col_ind,
+               Real* vals, Real* res)
+{
+  return guarded([&] {
+    femx::fem::Mesh mesh(2);
+    for (Index i = 0; i < num_nodes; ++i) mesh.addNode({x[i], y[i], 0.0});
+    for (Index e = 0; e < num_elems; ++e)
+      mesh.addElem({conn[3 * e], conn[3 * e + 1], conn[3 * e + 2]},
+                   femx::fem::Element::Shape::Triangle, 2, 0, 0, {});
+    femx::fem::LagrangeTriangleP1 fe;
+    femx::fem::FESpace            u_space(&mesh, &fe, mesh.dim());
+    femx::fem::FESpace            p_space(&mesh, &fe);
+    femx::fem::MixedFESpace       space;
+    space.addField(u_space);
+    space.addField(p_space);
+    space.setup();
+    const auto quad = femx::fem::GaussQuadrature::make(
+        space.field(0).space().finiteElement().referenceElement(), 2);
+    const auto data = femx::model::ns::makeNavierData(space.field(0).space(), quad);
+    const auto map  = femx::assembly::makeAssemblyMap(femx::fem::DofLayout(space));
+    femx::require(map.pattern().nnz() <= nnz_cap, "nnz capacity too small");
+    const Index ns = map.numStates();
+    femx::HostVector h(num_hist * ns), nx(ns), r;
+    for (Index i = 0; i < num_hist * ns; ++i) h[i] = hist[i];
+    for (Index i = 0; i < ns; ++i) nx[i] = nxt[i];
+    femx::HostCsrMatrix jac(map.pattern());
+    femx::CpuContext    ctx;
+    const femx::model::ns::NavierOperator<femx::MemorySpace::Host> op(data.view(), {rho, mu}, dt);
+    femx::assembly::assemble(op, step, num_hist, femx::state::VariableBlock::NextState, map, h,
+                             nx, r, jac, ctx);
+    const auto mv = map.view();
+    for (Index e = 0; e < num_elems; ++e)
+    {
+      femx::require(mv.numStateDofs(e) == 9, "expected 9 DOFs per element");
+      for (Index k = 0; k < 9; ++k) elem_dofs[9 * e + k] = mv.stateDof(e, k);
+    }
+    const auto& p = map.pattern();
+    std::memcpy(row_ptr, p.rowPtrData(), sizeof(Index) * (p.rows() + 1));
+    std::memcpy(col_ind, p.colIndData(), sizeof(Index) * p.nnz());
+    std::memcpy(vals, jac.valsData(), sizeof(Real) * p.nnz());
+    for (Index i = 0; i < ns; ++i) res[i] = r[i];
   });
 }
 

@@ -12,6 +12,9 @@ conventions, seeded values) and the reference's outputs:
                           prepareForwardSolve on seeded values
   poisson_p1_n{2,8}.npz   assemble(PoissonComponents) K values and K u
   spmv_p1_n4.npz          matvec / matvecT with alpha, beta
+  gls_tri1.npz, gls_p1_n4_perturbed.npz, gls_p1_n6.npz
+                          NavierOperator (GLS P1/P1) time assemble: CSR
+                          values and residual per (step, num_hist, state)
 
     python tests/golden/make_golden.py
 """
@@ -146,6 +149,50 @@ def main():
             res[f"y_t{tr}_a{alpha}_b{beta}"] = yy
     np.savez(os.path.join(HERE, "spmv_p1_n4.npz"), row_ptr=rp, col_ind=ci, vals=vals, x=xin,
              y0=y0, **res)
+    # GLS P1/P1 Navier-Stokes time assembly (femx's NavierOperator through
+    # the host time assemble).  The single triangle and the state pattern are
+    # those of tests/unit/model/ns/runNavierOperatorTests.cpp:142-178.
+    def gls_case(x, y, conn, rho, mu, dt, cases, name):
+        out = dict(x=x, y=y, conn=conn, rho=rho, mu=mu, dt=dt, num_cases=len(cases))
+        for k, (step, nh, hist, nxt) in enumerate(cases):
+            d9, rp, ci, vals, res = O.ref_ns_gls(x, y, conn, rho, mu, dt, step, nh, hist, nxt)
+            out.update({f"step{k}": step, f"num_hist{k}": nh, f"hist{k}": hist, f"nxt{k}": nxt,
+                        f"vals{k}": vals, f"res{k}": res})
+        out.update(dofs9=d9, row_ptr=rp, col_ind=ci)
+        np.savez_compressed(os.path.join(HERE, name), **out)
+
+    x = np.array([0.0, 1.0, 0.0])
+    y = np.array([0.0, 0.0, 1.0])
+    conn = np.array([0, 1, 2], I32)
+    ns_ = 9
+    i = np.arange(ns_)
+    hist = np.concatenate([0.01 * (i + 1), -0.005 * (i + 1)])
+    nxt = 0.02 - 0.003 * i
+    gls_case(x, y, conn, 1.2, 0.03, 0.05,
+             [(1, 2, hist, nxt), (0, 1, hist[:ns_], nxt), (1, 2, hist, np.zeros(ns_))],
+             "gls_tri1.npz")
+
+    n = 4
+    x, y, conn = O.mesh_p1(n)
+    inner = (x > 0) & (x < 1) & (y > 0) & (y < 1)
+    x[inner] += 0.2 / n * rng.uniform(-1, 1, inner.sum())
+    y[inner] += 0.2 / n * rng.uniform(-1, 1, inner.sum())
+    nd = 3 * x.size
+    still = rng.standard_normal(2 * nd)
+    still[:2 * x.size] = 0.0  # zero velocity in both lags: elemLength's fallback direction
+    gls_case(x, y, conn, 1.2, 0.03, 0.05,
+             [(0, 1, rng.standard_normal(nd), rng.standard_normal(nd)),
+              (1, 2, rng.standard_normal(2 * nd), rng.standard_normal(nd)),
+              (2, 3, rng.standard_normal(3 * nd), rng.standard_normal(nd)),
+              (1, 2, still, rng.standard_normal(nd))],
+             "gls_p1_n4_perturbed.npz")
+
+    n = 6
+    x, y, conn = O.mesh_p1(n)
+    nd = 3 * x.size
+    gls_case(x, y, conn, 1.0, 1.0e-3, 0.01,
+             [(1, 2, rng.standard_normal(2 * nd), rng.standard_normal(nd))], "gls_p1_n6.npz")
+
     print("fixtures written to", HERE)
 
 
