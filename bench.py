@@ -58,6 +58,51 @@ def spmv_alg_bytes(N, nnz):
     return nnz * 8 + nnz * 4 + (N + 1) * 4 + N * 8 + N * 8
 
 
+def gls_alg_bytes(N, E, nnz):
+    """GLS P1/P1 time step: coords, conn, femx's element DOF table, the
+    velocity of both history lags, the next state, femx's jac_map, CSR values,
+    residual."""
+    return (2 * N * 8 + 3 * E * 4 + 9 * E * 4 + 2 * (2 * N) * 8 + 3 * N * 8 + 81 * E * 4
+            + nnz * 8 + 3 * N * 8)
+
+
+# FP64 operations (add, mul, div, sqrt each 1) of one node-incidence of the
+# GLS kernel: geometry 15, extrapolation 18, velocity gradient 16, per
+# quadrature point 242 (point state 55, row factors 11, F terms 41, 3 columns
+# x 45), residual dot products 42.
+GLS_FLOPS_PER_INCIDENCE = 15 + 18 + 16 + 3 * 242 + 42
+FP64_PEAK_TFLOPS = 78.6
+
+
+def build_gls(n, dev, t):
+    """GLS P1/P1 Navier-Stokes time step on the P1 n x n mesh: rho 1, mu 1e-3,
+    dt 0.01, step 1 with two history lags ~ N(0,1) (seed 12) and next state
+    ~ N(0,1) (seed 13); returns launch closures."""
+    import torch
+    from femx_amd import gls
+    from femx_amd.linalg import CsrMatrix, ptr
+    x = np.empty((n + 1) ** 2)
+    y = np.empty_like(x)
+    conn = np.empty(6 * n * n, np.int32)
+    from femx_amd import _lib
+    _lib.call("fx_mesh_p1", n, ptr(x), ptr(y), ptr(conn))
+    geom = gls.GLSGeometry(x, y, conn, dev)
+    amap = gls.make_navier_map(geom)
+    nd = geom.num_states
+    hist = np.random.default_rng(12).standard_normal(2 * nd)
+    nxt = np.random.default_rng(13).standard_normal(nd)
+    d_hist, d_nxt = t(hist), t(nxt)
+    op = gls.NavierOperator(geom, gls.FluidParams(1.0, 1e-3), 0.01)
+    jac = CsrMatrix(amap.pattern, dev)
+    res = torch.empty(nd, dtype=torch.float64, device=dev)
+    gls.assemble(op, 1, 2, gls.NEXT_STATE, amap, d_hist, d_nxt, res, jac)
+    return {"assemble": lambda: gls.assemble(op, 1, 2, gls.NEXT_STATE, amap, d_hist, d_nxt, res,
+                                             jac),
+            "residual": lambda: gls.assemble_residual(op, 1, 2, amap, d_hist, d_nxt, res),
+            "E": geom.num_elems, "N": geom.num_nodes, "states": nd, "nnz": amap.pattern.nnz,
+            "alg_bytes": gls_alg_bytes(geom.num_nodes, geom.num_elems, amap.pattern.nnz)}
+
+
 def build_th(n, dev, t):
     """Config-3 workload: Taylor-Hood n x n lid-driven cavity, Re = 100,
     Fourier state (seed 5) with lid u = 1; returns launch closures."""
@@ -156,6 +201,7 @@ def main():
     ap.add_argument("--no-th", action="store_true")
     ap.add_argument("--th-n", type=int, default=512)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-gls", action="store_true")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--kernels-only", action="store_true",
                     help="for rocprofv3 counter runs: a 1 GiB device copy (counter "
@@ -244,6 +290,11 @@ def main():
             for _ in range(5):
                 th["vjp"]()
             torch.cuda.synchronize()
+        if not args.no_gls:
+            gl = build_gls(n, dev, t)
+            for _ in range(5):
+                gl["assemble"]()
+            torch.cuda.synchronize()
         log("[bench] kernels-only done")
         return
 
@@ -321,6 +372,52 @@ def main():
             "GBs": b_th / t_th / 1e9, "frac": b_th / t_th / 1e9 / HBM_PEAK_GBS,
             "vjp_us": t_thv * 1e6, "VJPs_per_s": 1.0 / t_thv, "vjp_alg_bytes": b_thv,
             "vjp_GBs": b_thv / t_thv / 1e9, "vjp_frac": b_thv / t_thv / 1e9 / HBM_PEAK_GBS}
+
+    if not args.no_gls:
+        log(f"[bench] GLS Navier-Stokes time step {n}x{n}")
+        gl = build_gls(n, dev, t)
+        t_gl = time_kernel(gl["assemble"], reps)
+        t_glr = time_kernel(gl["residual"], reps)
+        inc = 3 * gl["E"]
+        fl = GLS_FLOPS_PER_INCIDENCE * inc
+        extra["gls_ns_%d" % n] = {
+            "elements": gl["E"], "states": gl["states"], "nnz": gl["nnz"],
+            "assemble_us": t_gl * 1e6, "elements_per_s": gl["E"] / t_gl,
+            "residual_us": t_glr * 1e6, "residual_elements_per_s": gl["E"] / t_glr,
+            "alg_bytes": gl["alg_bytes"], "GBs": gl["alg_bytes"] / t_gl / 1e9,
+            "frac_hbm": gl["alg_bytes"] / t_gl / 1e9 / HBM_PEAK_GBS,
+            "flops": fl, "TFLOPs": fl / t_gl / 1e12,
+            "frac_fp64": fl / t_gl / 1e12 / FP64_PEAK_TFLOPS}
+        if not args.no_cpu:
+            from oracle import oracle as O
+            ns_ = 256  # bounded sample: the 256^2 mesh of the same problem
+            xs_ = np.empty((ns_ + 1) ** 2)
+            ys_ = np.empty_like(xs_)
+            cs_ = np.empty(6 * ns_ * ns_, np.int32)
+            _lib.call("fx_mesh_p1", ns_, ptr(xs_), ptr(ys_), ptr(cs_))
+            from femx_amd import gls as G
+            gs_ = G.GLSGeometry(xs_, ys_, cs_, "cpu")
+            ms_ = G.make_navier_map(gs_)
+            _, jms = ms_.jac()
+            nds = gs_.num_states
+            hs_ = np.random.default_rng(12).standard_normal(2 * nds)
+            nxs = np.random.default_rng(13).standard_normal(nds)
+            args_ = (xs_, ys_, cs_, gs_.h_dofs9.reshape(-1), nds, 1.0, 1e-3, 0.01, 1, 2, hs_, nxs,
+                     jms, ms_.pattern.nnz)
+            O.gls_assemble(*args_, omp=True)
+            ts = []
+            t_start = time.perf_counter()
+            while len(ts) < 5 and time.perf_counter() - t_start < 8.0:
+                a = time.perf_counter()
+                O.gls_assemble(*args_, omp=True)
+                ts.append(time.perf_counter() - a)
+            tcg = float(np.median(ts))
+            extra["gls_ns_%d" % n]["cpu_baseline"] = {
+                "value": gs_.num_elems / tcg, "unit": "elements/s", "cores": O.omp_threads(),
+                "kind": "port",
+                "sample": f"{len(ts)} assemblies of the same problem on the {ns_}^2 mesh "
+                          f"({gs_.num_elems} elements), median; OpenMP element loop with "
+                          "atomic scatter as femx's NavierStokesModel"}
 
     if not args.no_batch:
         extra["batched_%d" % args.batch_count] = run_batched(args, log)

@@ -10,7 +10,7 @@ using namespace fxk;
 
 constexpr int kGlsMaxTile = 8192; // doubles of LDS row tile per wave (64 KiB)
 
-__global__ __launch_bounds__(kGlsBlock) void k_gls_assemble(GlsAsmArgs a)
+__global__ __launch_bounds__(64) void k_gls_assemble(GlsAsmArgs a)
 {
   extern __shared__ double tile[];
   gls_assemble_block(fx::xcd_block(gridDim.x), a, tile);
@@ -22,7 +22,7 @@ extern "C" {
 
 int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems, fx_index nnz,
                     fx_index max_tile, const double* x, const double* y, const fx_index* conn3,
-                    const fx_index* node_rows, double rho, double mu, double dt, fx_index step,
+                    const fx_index* dofs9, const fx_index* node_rows, double rho, double mu, double dt, fx_index step,
                     fx_index num_hist, const double* hist, const double* nxt,
                     const fx_index* inc_off, const fx_index* inc_elem, const fx_index* inc_meta,
                     const fx_index* blk_runs, const fx_index* node_tile, double* vals,
@@ -43,7 +43,7 @@ int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                 "Time assembly state dimensions do not match AssemblyMap");
     if (num_states == 0) return;
     fx::require(num_nodes > 0 && x && y && node_rows && hist && nxt && inc_off && blk_runs
-                    && node_tile && res && (num_elems == 0 || (conn3 && inc_elem && inc_meta)),
+                    && node_tile && res && (num_elems == 0 || (conn3 && dofs9 && inc_elem && inc_meta)),
                 FX_ERR_ARG, "fx_gls_assemble: null array");
     const std::size_t S = num_states, H = S * num_hist * 8, V = std::size_t(nnz) * 8;
     fx::require(!fx::overlaps(res, S * 8, hist, H) && !fx::overlaps(res, S * 8, nxt, S * 8)
@@ -59,14 +59,13 @@ int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                        x,
                        y,
                        conn3,
+                       dofs9,
                        node_rows,
                        step,
                        num_hist,
                        hist,
                        nxt,
-                       rho,
-                       mu,
-                       dt,
+                       gls_const(rho, mu, dt),
                        inc_off,
                        inc_elem,
                        reinterpret_cast<const uint4*>(inc_meta),
@@ -75,7 +74,7 @@ int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                        vals,
                        res,
                        max_tile};
-    hipLaunchKernelGGL(k_gls_assemble, dim3(nb), dim3(kGlsBlock), lds, fx::as_stream(stream), a);
+    hipLaunchKernelGGL(k_gls_assemble, dim3(nb), dim3(64), lds, fx::as_stream(stream), a);
     fx::check_launch("fx_gls_assemble");
   });
 }

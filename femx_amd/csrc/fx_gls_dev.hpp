@@ -20,14 +20,19 @@
 // gradients are constant on an affine triangle, so no per-element quadrature
 // data is read from HBM.
 //
-// A wave owns 64 consecutive nodes.  Their rows form at most kGlsRuns runs of
+// A wave owns 32 consecutive nodes, two lanes per node.  Their rows form at most kGlsRuns runs of
 // consecutive rows (two for femx's field-major mixed layout: the velocity
 // rows 2n, 2n+1 and the pressure rows 2N+n), whose CSR ranges are contiguous;
 // the wave accumulates them in an LDS tile and writes each run back with
 // coalesced stores.  Each CSR value and residual entry is a sum over its
 // elements in ascending element order, with every element term computed in
 // femx's operation order and without contraction: the results equal femx's
-// host assembly bit for bit.
+// host assembly bit for bit.  femx starts each inner sum at 0.0; here a sum
+// starts at its first term.  That changes at most the sign of an exact-zero
+// partial sum, and nothing observes it: no such sum is divided by or square-
+// rooted (the divisions and roots act on squares, guarded lengths and the
+// positive tau argument), the comparisons are sign-blind at zero, and the
+// outputs are accumulated onto +0.0 as in femx.
 #pragma once
 #include "fx_common.hpp"
 
@@ -37,6 +42,29 @@ namespace fxk
 constexpr int kGlsBlock = FX_GLS_BLOCK_NODES; // nodes per wave
 constexpr int kGlsRuns  = FX_GLS_MAX_RUNS;    // row runs per block
 
+// Loop-invariant factors of NavierOperator's expressions, formed once (on the
+// host, passed as kernel arguments) with the same operations femx applies at
+// every use, so the values are equal.
+struct GlsConst
+{
+  double rho, mu, dt, nu, time, rdt, h5r, h5m;
+};
+
+__host__ __device__ inline GlsConst gls_const(double rho, double mu, double dt)
+{
+  GlsConst c;
+  c.rho          = rho;
+  c.mu           = mu;
+  c.dt           = dt;
+  c.nu           = mu / rho;
+  const double t = 2.0 / dt;
+  c.time         = t * t;
+  c.rdt          = rho / dt;
+  c.h5r          = 0.5 * rho;
+  c.h5m          = 0.5 * mu;
+  return c;
+}
+
 struct GlsAsmArgs
 {
   int           num_nodes;
@@ -44,12 +72,13 @@ struct GlsAsmArgs
   const double* x;
   const double* y;
   const int*    conn;
+  const int*    dofs;      // [9 E]: element DOF table (local order u0 v0 u1 v1 u2 v2 p0 p1 p2)
   const int*    node_rows; // [3 N]: velocity x, velocity y, pressure row of each node
   int           step;
   int           num_hist;
   const double* hist; // lag-major, num_hist x num_states
   const double* nxt;
-  double        rho, mu, dt;
+  GlsConst      c;
   const int*    inc_off;   // [N+1]
   const int*    inc_elem;  // [3E], ascending element per node
   const uint4*  inc_meta;  // [2 per incidence]: 27 CSR offsets + local node
@@ -83,10 +112,11 @@ __device__ __forceinline__ GlsQp gls_qp(const double (&N)[3], const double (&gx)
                                         const double (&gy)[3], const double (&cur)[6],
                                         const double (&adv)[6], double nu, double time)
 {
-  GlsQp q;
-  double u0 = 0.0, u1 = 0.0, g00 = 0.0, g01 = 0.0, g10 = 0.0, g11 = 0.0, a0 = 0.0, a1 = 0.0;
+  GlsQp  q;
+  double u0 = N[0] * cur[0], g00 = gx[0] * cur[0], g01 = gy[0] * cur[0], a0 = N[0] * adv[0];
+  double u1 = N[0] * cur[1], g10 = gx[0] * cur[1], g11 = gy[0] * cur[1], a1 = N[0] * adv[1];
 #pragma unroll
-  for (int in = 0; in < 3; ++in)
+  for (int in = 1; in < 3; ++in)
   {
     const double vx = cur[2 * in], vy = cur[2 * in + 1];
     u0  = u0 + N[in] * vx;
@@ -98,12 +128,9 @@ __device__ __forceinline__ GlsQp gls_qp(const double (&N)[3], const double (&gx)
     g11 = g11 + gy[in] * vy;
     a1  = a1 + N[in] * adv[2 * in + 1];
   }
-  double ag0 = 0.0, ag1 = 0.0;
-  ag0                 = ag0 + g00 * a0;
-  ag0                 = ag0 + g01 * a1;
-  ag1                 = ag1 + g10 * a0;
-  ag1                 = ag1 + g11 * a1;
-  const double speed2 = (0.0 + u0 * u0) + u1 * u1;
+  const double ag0    = g00 * a0 + g01 * a1;
+  const double ag1    = g10 * a0 + g11 * a1;
+  const double speed2 = u0 * u0 + u1 * u1;
   const double speed  = sqrt(speed2);
   // elemLength (Components.hpp:286-327)
   double dx, dy;
@@ -116,12 +143,13 @@ __device__ __forceinline__ GlsQp gls_qp(const double (&N)[3], const double (&gx)
   {
     dx = dy = 1.0 / sqrt(2.0);
   }
-  double sum = 0.0;
+  double sum;
 #pragma unroll
   for (int in = 0; in < 3; ++in)
   {
-    const double g = (0.0 + dx * gx[in]) + dy * gy[in];
-    sum            = sum + (g >= 0.0 ? g : -g);
+    const double g = dx * gx[in] + dy * gy[in];
+    const double m = g >= 0.0 ? g : -g;
+    sum            = in == 0 ? m : sum + m;
   }
   const double h    = sum > 1.0e-14 ? 2.0 / sum : 0.0;
   double       flow = 0.0, diff = 0.0;
@@ -157,102 +185,210 @@ __device__ __forceinline__ void gls_shape(int q, double (&N)[3])
   N[2] = s;
 }
 
-// The three rows of local node li of one element: velocity x (Kx: columns
-// u0 u1 u2 then p0 p1 p2), velocity y (Ky, same), pressure (Kp: columns
-// u0 v0 u1 v1 u2 v2 p0 p1 p2), and their residuals -F + K nxt.
+// The three rows of local node li of one element.  Velocity rows: the
+// same-component velocity columns (Kuu, equal for both components) and the
+// pressure columns (Kxp, Kyp); the other component's columns are zero.
+// Pressure row: all nine columns, u0 v0 u1 v1 u2 v2 p0 p1 p2 (Kp).  r* are
+// the row residuals -F + K nxt.
 struct GlsRows
 {
-  double Kx[6], Ky[6], Kp[9];
+  double Kuu[3], Kxp[3], Kyp[3], Kp[9];
   double rx, ry, rp;
 };
 
-__device__ __forceinline__ GlsRows gls_rows(int li, const double (&gx)[3], const double (&gy)[3],
-                                            double jw, const double (&cur)[6],
-                                            const double (&adv)[6], const double (&nx)[9],
-                                            double rho, double mu, double dt)
+__device__ __forceinline__ void gls_rows(GlsRows& o, int li, const double (&gx)[3],
+                                         const double (&gy)[3], double jw,
+                                         const double (&cur)[6], const double (&adv)[6],
+                                         const GlsConst& c)
 {
-  GlsRows o;
-#pragma unroll
-  for (int k = 0; k < 6; ++k) o.Kx[k] = o.Ky[k] = 0.0;
-#pragma unroll
-  for (int k = 0; k < 9; ++k) o.Kp[k] = 0.0;
-  double       Fx = 0.0, Fy = 0.0, Fp = 0.0;
-  const double nu   = mu / rho;
-  const double time = (2.0 / dt) * (2.0 / dt);
-  const double rdt  = rho / dt;
-  const double h5r  = 0.5 * rho;
-  const double h5m  = 0.5 * mu;
-  const double gix  = gsel3(gx[0], gx[1], gx[2], li);
-  const double giy  = gsel3(gy[0], gy[1], gy[2], li);
+  double       Fx, Fy, Fp;
+  const double gix = gsel3(gx[0], gx[1], gx[2], li);
+  const double giy = gsel3(gy[0], gy[1], gy[2], li);
 #pragma unroll
   for (int iq = 0; iq < 3; ++iq)
   {
     double N[3];
     gls_shape(iq, N);
-    const GlsQp  q    = gls_qp(N, gx, gy, cur, adv, nu, time);
+    const GlsQp  q    = gls_qp(N, gx, gy, cur, adv, c.nu, c.time);
     const double Ni   = gsel3(N[0], N[1], N[2], li);
-    const double dvi  = (0.0 + gix * q.a0) + giy * q.a1;
-    const double trd  = q.tau * rho / dt;
-    const double h5tr = 0.5 * q.tau * rho;
-    const double tdt  = q.tau / dt;
+    const double dvi  = gix * q.a0 + giy * q.a1;
+    const double trd  = q.tau * c.rho / c.dt;
+    const double h5tr = 0.5 * q.tau * c.rho;
+    const double tdt  = q.tau / c.dt;
     const double h5t  = 0.5 * q.tau;
-    const double tr   = q.tau / rho;
+    const double tr   = q.tau / c.rho;
     const double tdv  = q.tau * dvi;
     // velocity rows (Components.hpp:507-536)
-    const double gux = (0.0 + gix * q.g00) + giy * q.g01;
-    const double guy = (0.0 + gix * q.g10) + giy * q.g11;
-    Fx = Fx + ((((rdt * Ni * q.u0 - h5r * Ni * q.ag0) - h5m * gux) + trd * dvi * q.u0)
+    const double gux = gix * q.g00 + giy * q.g01;
+    const double guy = gix * q.g10 + giy * q.g11;
+    const double fx = ((((c.rdt * Ni * q.u0 - c.h5r * Ni * q.ag0) - c.h5m * gux) + trd * dvi * q.u0)
                - h5tr * dvi * q.ag0)
                   * jw;
-    Fy = Fy + ((((rdt * Ni * q.u1 - h5r * Ni * q.ag1) - h5m * guy) + trd * dvi * q.u1)
+    const double fy = ((((c.rdt * Ni * q.u1 - c.h5r * Ni * q.ag1) - c.h5m * guy) + trd * dvi * q.u1)
                - h5tr * dvi * q.ag1)
                   * jw;
     // pressure row (Components.hpp:537-570)
-    const double div_u   = (0.0 + gix * q.u0) + giy * q.u1;
-    const double div_adv = (0.0 + gix * q.ag0) + giy * q.ag1;
-    Fp                   = Fp + (tdt * div_u - h5t * div_adv) * jw;
+    const double div_u   = gix * q.u0 + giy * q.u1;
+    const double div_adv = gix * q.ag0 + giy * q.ag1;
+    const double fp      = (tdt * div_u - h5t * div_adv) * jw;
+    Fx                   = iq == 0 ? fx : Fx + fx;
+    Fy                   = iq == 0 ? fy : Fy + fy;
+    Fp                   = iq == 0 ? fp : Fp + fp;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
     {
       const double Nj  = N[j];
-      const double dvj = (0.0 + gx[j] * q.a0) + gy[j] * q.a1;
-      const double gd  = (0.0 + gix * gx[j]) + giy * gy[j];
-      const double kuu =
-          (((rdt * Ni * Nj + h5r * Ni * dvj) + h5m * gd) + trd * dvi * Nj) + h5tr * dvi * dvj;
-      o.Kx[j]     = o.Kx[j] + kuu * jw;
-      o.Ky[j]     = o.Ky[j] + kuu * jw;
-      o.Kx[3 + j] = o.Kx[3 + j] + (-gix * Nj + tdv * gx[j]) * jw;
-      o.Ky[3 + j] = o.Ky[3 + j] + (-giy * Nj + tdv * gy[j]) * jw;
-      o.Kp[2 * j] = o.Kp[2 * j] + ((Ni * gx[j] + tdt * gix * Nj) + h5t * gix * dvj) * jw;
-      o.Kp[2 * j + 1] =
-          o.Kp[2 * j + 1] + ((Ni * gy[j] + tdt * giy * Nj) + h5t * giy * dvj) * jw;
-      o.Kp[6 + j] = o.Kp[6 + j] + tr * gd * jw;
+      const double dvj = gx[j] * q.a0 + gy[j] * q.a1;
+      const double gd  = gix * gx[j] + giy * gy[j];
+      const double kuu = (((c.rdt * Ni * Nj + c.h5r * Ni * dvj) + c.h5m * gd) + trd * dvi * Nj)
+                         + h5tr * dvi * dvj;
+      const double t[6] = {kuu * jw,
+                           (-gix * Nj + tdv * gx[j]) * jw,
+                           (-giy * Nj + tdv * gy[j]) * jw,
+                           ((Ni * gx[j] + tdt * gix * Nj) + h5t * gix * dvj) * jw,
+                           ((Ni * gy[j] + tdt * giy * Nj) + h5t * giy * dvj) * jw,
+                           tr * gd * jw};
+      o.Kuu[j]        = iq == 0 ? t[0] : o.Kuu[j] + t[0];
+      o.Kxp[j]        = iq == 0 ? t[1] : o.Kxp[j] + t[1];
+      o.Kyp[j]        = iq == 0 ? t[2] : o.Kyp[j] + t[2];
+      o.Kp[2 * j]     = iq == 0 ? t[3] : o.Kp[2 * j] + t[3];
+      o.Kp[2 * j + 1] = iq == 0 ? t[4] : o.Kp[2 * j + 1] + t[4];
+      o.Kp[6 + j]     = iq == 0 ? t[5] : o.Kp[6 + j] + t[5];
     }
   }
-  // out = -F + sum_col K[col] nxt[col] over the row's columns in local order;
-  // the structurally zero columns (the other velocity component) add nothing.
-  double rx = -Fx, ry = -Fy, rp = -Fp;
+  o.rx = -Fx;
+  o.ry = -Fy;
+  o.rp = -Fp;
+}
+
+// out = -F + sum_col K[col] nxt[col] over the row's columns in local order;
+// the structurally zero columns (the other velocity component) add nothing.
+__device__ __forceinline__ void gls_residuals(GlsRows& o, const double (&nx)[9])
+{
+  double rx = o.rx, ry = o.ry, rp = o.rp;
 #pragma unroll
   for (int j = 0; j < 3; ++j)
   {
-    rx = rx + o.Kx[j] * nx[2 * j];
-    ry = ry + o.Ky[j] * nx[2 * j + 1];
+    rx = rx + o.Kuu[j] * nx[2 * j];
+    ry = ry + o.Kuu[j] * nx[2 * j + 1];
   }
 #pragma unroll
   for (int j = 0; j < 3; ++j)
   {
-    rx = rx + o.Kx[3 + j] * nx[6 + j];
-    ry = ry + o.Ky[3 + j] * nx[6 + j];
+    rx = rx + o.Kxp[j] * nx[6 + j];
+    ry = ry + o.Kyp[j] * nx[6 + j];
   }
 #pragma unroll
   for (int k = 0; k < 9; ++k) rp = rp + o.Kp[k] * nx[k];
   o.rx = rx;
   o.ry = ry;
   o.rp = rp;
-  return o;
 }
 
-// One block of 64 nodes, run by one wave; tile holds tile_cap doubles.
+// Adds one element's rows into the node's three tile rows; meta byte 9k+j is
+// the position of local column j in row k.  Row by row: read the row's
+// entries, then write them (positions within a row are distinct).
+__device__ __forceinline__ void gls_add_rows(double* tile, int tx, int ty, int tp, const uint4& m0,
+                                             const uint4& m1, const GlsRows& o)
+{
+  double* px = tile + tx;
+  double* py = tile + ty;
+  double* pp = tile + tp;
+  double  v[9];
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+  {
+    v[j]     = px[meta_byte(m0, m1, 2 * j)];
+    v[3 + j] = px[meta_byte(m0, m1, 6 + j)];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+  {
+    px[meta_byte(m0, m1, 2 * j)] = v[j] + o.Kuu[j];
+    px[meta_byte(m0, m1, 6 + j)] = v[3 + j] + o.Kxp[j];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+  {
+    v[j]     = py[meta_byte(m0, m1, 9 + 2 * j + 1)];
+    v[3 + j] = py[meta_byte(m0, m1, 9 + 6 + j)];
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+  {
+    py[meta_byte(m0, m1, 9 + 2 * j + 1)] = v[j] + o.Kuu[j];
+    py[meta_byte(m0, m1, 9 + 6 + j)]     = v[3 + j] + o.Kyp[j];
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) v[k] = pp[meta_byte(m0, m1, 18 + k)];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) pp[meta_byte(m0, m1, 18 + k)] = v[k] + o.Kp[k];
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Element inputs of one incidence: geometry and the local history and next
+// states gathered through the element DOF table.
+struct GlsElem
+{
+  double gx[3], gy[3], jw;
+  double cur[6], adv[6], nx[9];
+};
+
+__device__ __forceinline__ void gls_load_elem(GlsElem& E, const GlsAsmArgs& a, int e, bool extrap)
+{
+  int    v[3], d[9];
+  double xe[3], ye[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) v[k] = a.conn[3 * e + k];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) d[k] = a.dofs[9 * e + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    xe[k] = a.x[v[k]];
+    ye[k] = a.y[v[k]];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) E.cur[k] = a.hist[d[k]];
+  if (extrap)
+  {
+    const double* prev = a.hist + a.num_states;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) E.adv[k] = 1.5 * E.cur[k] - 0.5 * prev[d[k]];
+  }
+  else
+  {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) E.adv[k] = E.cur[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) E.nx[k] = a.nxt[d[k]];
+  // ElementValues on the affine triangle (femx/fem/ElementValues.cpp:138-181)
+  const double j00 = xe[1] - xe[0], j01 = xe[2] - xe[0], j10 = ye[1] - ye[0], j11 = ye[2] - ye[0];
+  const double det = j00 * j11 - j01 * j10, inv_det = 1.0 / det;
+  const double i00 = j11 * inv_det, i01 = -j01 * inv_det, i10 = -j10 * inv_det,
+               i11 = j00 * inv_det;
+  E.gx[0] = -i00 - i10;
+  E.gx[1] = i00;
+  E.gx[2] = i10;
+  E.gy[0] = -i01 - i11;
+  E.gy[1] = i01;
+  E.gy[2] = i11;
+  E.jw    = fabs(det) * (1.0 / 6.0);
+}
+
+// One block of kGlsBlock nodes, run by one wave with two lanes per node: role
+// 0 takes the node's incidences t0, t0+2, ..., role 1 takes t0+1, t0+3, ...
+// After each round role 0 adds its element rows into the tile, then role 1
+// adds its own, and the residuals are summed in the same order (role 1's
+// fetched by a cross-lane swap): every CSR value and residual entry still
+// receives its element terms in ascending element order.
 __device__ __forceinline__ void gls_assemble_block(int blk, const GlsAsmArgs& a, double* tile)
 {
   const int lane = threadIdx.x & 63;
@@ -266,99 +402,78 @@ __device__ __forceinline__ void gls_assemble_block(int blk, const GlsAsmArgs& a,
   }
   if (a.vals)
     for (int i = lane; i < tile_len; i += 64) tile[i] = 0.0;
-  __syncthreads();
+  wave_sync();
 
+  const GlsConst& c     = a.c;
+  const bool     extrap = a.step > 0 && a.num_hist > 1;
+  const int      role   = lane & 1;
   // a node of no element owns no rows (node_rows -1) and is skipped
-  const int n = blk * kGlsBlock + lane;
-  if (n < a.num_nodes && a.node_rows[3 * n] >= 0)
+  const int  n    = blk * kGlsBlock + (lane >> 1);
+  const bool live = n < a.num_nodes && a.node_rows[3 * n] >= 0;
+  int        t0 = 0, t1 = 0, tx = 0, ty = 0, tp = 0;
+  if (live)
   {
-    const int tx = a.node_tile[3 * n], ty = a.node_tile[3 * n + 1], tp = a.node_tile[3 * n + 2];
-    double    sx = 0.0, sy = 0.0, sp = 0.0;
-    const bool extrap = a.step > 0 && a.num_hist > 1;
-    const int  t1     = a.inc_off[n + 1];
-    for (int t = a.inc_off[n]; t < t1; ++t)
+    t0 = a.inc_off[n];
+    t1 = a.inc_off[n + 1];
+    tx = a.node_tile[3 * n];
+    ty = a.node_tile[3 * n + 1];
+    tp = a.node_tile[3 * n + 2];
+  }
+  double sx = 0.0, sy = 0.0, sp = 0.0;
+  // the wave runs until its longest node is done; lanes past their node's
+  // incidences idle through the rounds
+  int rounds = (t1 - t0 + 1) >> 1;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) rounds = max(rounds, __shfl_xor(rounds, off));
+  for (int r = 0; r < rounds; ++r)
+  {
+    const int  t    = t0 + 2 * r + role;
+    const bool have = t < t1;
+    GlsRows    o;
+    uint4      m0 = make_uint4(0, 0, 0, 0), m1 = m0;
+    if (have)
     {
-      const int   e  = a.inc_elem[t];
-      const uint4 m0 = a.inc_meta[2 * t], m1 = a.inc_meta[2 * t + 1];
-      const int   li = static_cast<int>(meta_byte(m0, m1, 27));
-      int         v[3], rows[9];
-      double      xe[3], ye[3];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
+      const int e = a.inc_elem[t];
+      m0          = a.inc_meta[2 * t];
+      m1          = a.inc_meta[2 * t + 1];
+      const int li = static_cast<int>(meta_byte(m0, m1, 27));
+      GlsElem   E;
+      gls_load_elem(E, a, e, extrap);
+      gls_rows(o, li, E.gx, E.gy, E.jw, E.cur, E.adv, c);
+      gls_residuals(o, E.nx);
+    }
+    else
+    {
+      o.rx = o.ry = o.rp = 0.0;
+    }
+    if (a.vals)
+    {
+      if (role == 0 && have) gls_add_rows(tile, tx, ty, tp, m0, m1, o);
+      wave_sync();
+      if (role == 1 && have) gls_add_rows(tile, tx, ty, tp, m0, m1, o);
+      wave_sync();
+    }
+    const double px = __shfl_xor(o.rx, 1), py = __shfl_xor(o.ry, 1), pz = __shfl_xor(o.rp, 1);
+    if (role == 0 && have)
+    {
+      sx = sx + o.rx;
+      sy = sy + o.ry;
+      sp = sp + o.rp;
+      if (t + 1 < t1)
       {
-        v[k]           = a.conn[3 * e + k];
-        xe[k]          = a.x[v[k]];
-        ye[k]          = a.y[v[k]];
-        rows[2 * k]     = a.node_rows[3 * v[k]];
-        rows[2 * k + 1] = a.node_rows[3 * v[k] + 1];
-        rows[6 + k]     = a.node_rows[3 * v[k] + 2];
-      }
-      double cur[6], adv[6], nx[9];
-#pragma unroll
-      for (int k = 0; k < 6; ++k) cur[k] = a.hist[rows[k]];
-      if (extrap)
-      {
-        const double* prev = a.hist + a.num_states;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) adv[k] = 1.5 * cur[k] - 0.5 * prev[rows[k]];
-      }
-      else
-      {
-#pragma unroll
-        for (int k = 0; k < 6; ++k) adv[k] = cur[k];
-      }
-#pragma unroll
-      for (int k = 0; k < 9; ++k) nx[k] = a.nxt[rows[k]];
-
-      // ElementValues on the affine triangle (femx/fem/ElementValues.cpp:138-181)
-      const double j00 = xe[1] - xe[0], j01 = xe[2] - xe[0], j10 = ye[1] - ye[0],
-                   j11 = ye[2] - ye[0];
-      const double det = j00 * j11 - j01 * j10, inv_det = 1.0 / det;
-      const double i00 = j11 * inv_det, i01 = -j01 * inv_det, i10 = -j10 * inv_det,
-                   i11 = j00 * inv_det;
-      const double gx[3] = {-i00 - i10, i00, i10};
-      const double gy[3] = {-i01 - i11, i01, i11};
-      const double jw    = fabs(det) * (1.0 / 6.0);
-
-      const GlsRows o = gls_rows(li, gx, gy, jw, cur, adv, nx, a.rho, a.mu, a.dt);
-      sx              = sx + o.rx;
-      sy              = sy + o.ry;
-      sp              = sp + o.rp;
-      if (a.vals)
-      {
-        // meta byte 9k + j: position of local column j in the node's row k
-        double* px = tile + tx;
-        double* py = tile + ty;
-        double* pp = tile + tp;
-        // read all, then write: the positions within one row are distinct
-        double ox[6], oy[6], op[9];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-        {
-          ox[j]     = px[meta_byte(m0, m1, 2 * j)];
-          ox[3 + j] = px[meta_byte(m0, m1, 6 + j)];
-          oy[j]     = py[meta_byte(m0, m1, 9 + 2 * j + 1)];
-          oy[3 + j] = py[meta_byte(m0, m1, 9 + 6 + j)];
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) op[k] = pp[meta_byte(m0, m1, 18 + k)];
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-        {
-          px[meta_byte(m0, m1, 2 * j)]          = ox[j] + o.Kx[j];
-          px[meta_byte(m0, m1, 6 + j)]          = ox[3 + j] + o.Kx[3 + j];
-          py[meta_byte(m0, m1, 9 + 2 * j + 1)]  = oy[j] + o.Ky[j];
-          py[meta_byte(m0, m1, 9 + 6 + j)]      = oy[3 + j] + o.Ky[3 + j];
-        }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) pp[meta_byte(m0, m1, 18 + k)] = op[k] + o.Kp[k];
+        sx = sx + px;
+        sy = sy + py;
+        sp = sp + pz;
       }
     }
+  }
+  if (live && role == 0)
+  {
     __builtin_nontemporal_store(sx, a.res + a.node_rows[3 * n]);
     __builtin_nontemporal_store(sy, a.res + a.node_rows[3 * n + 1]);
     __builtin_nontemporal_store(sp, a.res + a.node_rows[3 * n + 2]);
   }
-  __syncthreads();
+  wave_sync();
   if (a.vals)
   {
 #pragma unroll

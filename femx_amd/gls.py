@@ -29,7 +29,7 @@ from . import _lib
 from .assembly import AssemblyMap, make_assembly_map
 from .linalg import CsrMatrix, _check_dev_vec, _require, ptr, stream_handle
 
-BLOCK_NODES = 64  # FX_GLS_BLOCK_NODES
+BLOCK_NODES = 32  # FX_GLS_BLOCK_NODES
 MAX_RUNS = 4  # FX_GLS_MAX_RUNS
 
 #: state::VariableBlock::NextState (femx/state/TimeResidual.hpp:96-165)
@@ -86,6 +86,7 @@ class GLSGeometry:
         self.h_dofs9 = navier_dofs(self.num_nodes, self.h_conn)
         t = lambda a: torch.from_numpy(a.reshape(-1)).to(self.device)  # noqa: E731
         self.x, self.y, self.conn = t(self.h_x), t(self.h_y), t(self.h_conn)
+        self.dofs9 = t(self.h_dofs9)
 
 
 def make_navier_map(geom: GLSGeometry) -> AssemblyMap:
@@ -174,7 +175,7 @@ def assemble(op: NavierOperator, step, num_hist, wrt, amap: AssemblyMap, hist, n
     p = _plan(amap, geom)
     _lib.call("fx_gls_assemble", geom.num_nodes, geom.num_states, geom.num_elems,
               amap.pattern.nnz, p.max_tile, ptr(geom.x), ptr(geom.y), ptr(geom.conn),
-              ptr(p.node_rows), op.fluid.rho, op.fluid.mu, op.dt, int(step), int(num_hist),
+              ptr(geom.dofs9), ptr(p.node_rows), op.fluid.rho, op.fluid.mu, op.dt, int(step), int(num_hist),
               ptr(hist), ptr(nxt), ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta),
               ptr(p.blk_runs), ptr(p.node_tile), ptr(vals) if kind == "next" else None,
               ptr(res), stream_handle(stream))

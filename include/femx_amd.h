@@ -138,7 +138,7 @@ int fx_map_th_blocks(const fx_map* map, fx_index lanes, fx_index max_nnz, fx_ind
  * FX_ERR_UNSUPPORTED when the element tables disagree on a node's rows, a
  * row belongs to no node, a block's rows need more than FX_GLS_MAX_RUNS runs
  * or a row is longer than 255 entries. */
-#define FX_GLS_BLOCK_NODES 64
+#define FX_GLS_BLOCK_NODES 32
 #define FX_GLS_MAX_RUNS 4
 int fx_map_gls_plan(const fx_map* map, fx_index num_nodes, const fx_index* h_conn3,
                     fx_index* h_node_rows, fx_index* h_inc_off, fx_index* h_inc_elem,
@@ -313,14 +313,16 @@ int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* d_xv, const doubl
  * (femx/assembly/CudaAssembly.hpp:366-417 on the device, Assembly.hpp:235-297
  * on the host).  hist is lag-major (num_hist x num_states; lags 0 and 1 are
  * read), nxt the next state; rho, mu, dt as NavierStokesModel's FluidParams
- * and time step; tables from fx_map_gls_plan.  Writes res[num_states] =
+ * and time step; dofs9[9E] the map's element DOF table (NavierOperator
+ * local order), the other tables from fx_map_gls_plan.  Writes res[num_states] =
  * -F + K nxt and, when vals is non-null, the CSR values of K (vals NULL is
  * assembleResidual, CudaAssembly.hpp:419-472).  Outputs are replaced.  Sums
  * in ascending element order with femx's operation order and no
  * contraction: equal to femx's host assembly bit for bit. */
 int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems, fx_index nnz,
                     fx_index max_tile, const double* d_x, const double* d_y,
-                    const fx_index* d_conn3, const fx_index* d_node_rows, double rho, double mu,
+                    const fx_index* d_conn3, const fx_index* d_dofs9, const fx_index* d_node_rows,
+                    double rho, double mu,
                     double dt, fx_index step, fx_index num_hist, const double* d_hist,
                     const double* d_nxt, const fx_index* d_inc_off, const fx_index* d_inc_elem,
                     const fx_index* d_inc_meta, const fx_index* d_blk_runs,

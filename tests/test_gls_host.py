@@ -164,7 +164,7 @@ def test_plan_tables(n):
     # runs: two per block (velocity rows, pressure rows) laid end to end
     runs = h["blk_runs"].reshape(p.num_blocks, gls.MAX_RUNS, 4)
     for b in range(p.num_blocks):
-        n0, n1 = 64 * b, min(N, 64 * b + 64)
+        n0, n1 = gls.BLOCK_NODES * b, min(N, gls.BLOCK_NODES * (b + 1))
         r = runs[b]
         if n0 == 0 and n1 == N:  # velocity rows run straight into the pressure rows
             assert r[0].tolist() == [0, rp[3 * N], 0, 0] and not r[1:].any()
@@ -191,7 +191,7 @@ def _emulate(geom, amap, p, rho, mu, dt, step, nh, hist, nxt):
         tl = int((runs[b, :, 2] + runs[b, :, 1]).max())
         assert tl <= p.max_tile
         tile = np.zeros(tl)
-        for v in range(64 * b, min(N, 64 * b + 64)):
+        for v in range(gls.BLOCK_NODES * b, min(N, gls.BLOCK_NODES * (b + 1))):
             rows = h["node_rows"][3 * v: 3 * v + 3]
             s = [0.0, 0.0, 0.0]
             for t in range(h["inc_off"][v], h["inc_off"][v + 1]):
