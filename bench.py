@@ -172,7 +172,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_p1_2048_th_512.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_p1_2048_th_512_gls_2048.md")
 
 
 def profiled_traffic(kernel):
@@ -474,7 +474,7 @@ def main():
                      "unit": "GB/s", "frac": b_asm / t_asm / 1e9 / HBM_PEAK_GBS,
                      "traffic": profiled_traffic("k_p1_assemble") if n == 2048 else None,
                      "kernel": "k_p1_assemble",
-                     "traffic_source": "profiles/hbm_traffic_p1_2048_th_512.md"},
+                     "traffic_source": "profiles/hbm_traffic_p1_2048_th_512_gls_2048.md"},
         "cpu_baseline": cpu,
         "kernels": kernels,
         "extra": extra,
