@@ -66,6 +66,12 @@ def gls_alg_bytes(N, E, nnz):
             + nnz * 8 + 3 * N * 8)
 
 
+def gls_vjp_alg_bytes(N, E):
+    """GLS history VJP: coords, conn, element DOF table, the velocity of both
+    history lags, next state, adjoint, output."""
+    return 2 * N * 8 + 3 * E * 4 + 9 * E * 4 + 2 * (2 * N) * 8 + 3 * N * 8 * 3
+
+
 # FP64 operations (add, mul, div, sqrt each 1) of one node-incidence of the
 # GLS kernel: geometry 15, extrapolation 18, velocity gradient 16, per
 # quadrature point 242 (point state 55, row factors 11, F terms 41, 3 columns
@@ -96,7 +102,10 @@ def build_gls(n, dev, t):
     jac = CsrMatrix(amap.pattern, dev)
     res = torch.empty(nd, dtype=torch.float64, device=dev)
     gls.assemble(op, 1, 2, gls.NEXT_STATE, amap, d_hist, d_nxt, res, jac)
-    return {"assemble": lambda: gls.assemble(op, 1, 2, gls.NEXT_STATE, amap, d_hist, d_nxt, res,
+    d_adj = t(np.random.default_rng(14).standard_normal(nd))
+    vjp_out = torch.empty(nd, dtype=torch.float64, device=dev)
+    return {"hist_vjp": lambda: gls.hist_vjp(op, 1, 2, 0, amap, d_hist, d_nxt, d_adj, vjp_out),
+            "vjp_alg_bytes": gls_vjp_alg_bytes(geom.num_nodes, geom.num_elems),"assemble": lambda: gls.assemble(op, 1, 2, gls.NEXT_STATE, amap, d_hist, d_nxt, res,
                                              jac),
             "residual": lambda: gls.assemble_residual(op, 1, 2, amap, d_hist, d_nxt, res),
             "E": geom.num_elems, "N": geom.num_nodes, "states": nd, "nnz": amap.pattern.nnz,
@@ -294,6 +303,8 @@ def main():
             gl = build_gls(n, dev, t)
             for _ in range(5):
                 gl["assemble"]()
+            for _ in range(5):
+                gl["hist_vjp"]()
             torch.cuda.synchronize()
         log("[bench] kernels-only done")
         return
@@ -378,6 +389,7 @@ def main():
         gl = build_gls(n, dev, t)
         t_gl = time_kernel(gl["assemble"], reps)
         t_glr = time_kernel(gl["residual"], reps)
+        t_glv = time_kernel(gl["hist_vjp"], reps)
         inc = 3 * gl["E"]
         fl = GLS_FLOPS_PER_INCIDENCE * inc
         extra["gls_ns_%d" % n] = {
@@ -387,7 +399,10 @@ def main():
             "alg_bytes": gl["alg_bytes"], "GBs": gl["alg_bytes"] / t_gl / 1e9,
             "frac_hbm": gl["alg_bytes"] / t_gl / 1e9 / HBM_PEAK_GBS,
             "flops": fl, "TFLOPs": fl / t_gl / 1e12,
-            "frac_fp64": fl / t_gl / 1e12 / FP64_PEAK_TFLOPS}
+            "frac_fp64": fl / t_gl / 1e12 / FP64_PEAK_TFLOPS,
+            "hist_vjp_us": t_glv * 1e6, "hist_vjp_elements_per_s": gl["E"] / t_glv,
+            "hist_vjp_alg_bytes": gl["vjp_alg_bytes"],
+            "hist_vjp_GBs": gl["vjp_alg_bytes"] / t_glv / 1e9}
         if not args.no_cpu:
             from oracle import oracle as O
             ns_ = 256  # bounded sample: the 256^2 mesh of the same problem

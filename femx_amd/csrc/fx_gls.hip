@@ -16,6 +16,11 @@ __global__ __launch_bounds__(64) void k_gls_assemble(GlsAsmArgs a)
   gls_assemble_block(fx::xcd_block(gridDim.x), a, tile);
 }
 
+__global__ __launch_bounds__(64) void k_gls_hist_vjp(GlsVjpArgs a)
+{
+  gls_hist_vjp_block(fx::xcd_block(gridDim.x), a);
+}
+
 } // namespace
 
 extern "C" {
@@ -76,6 +81,43 @@ int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                        max_tile};
     hipLaunchKernelGGL(k_gls_assemble, dim3(nb), dim3(64), lds, fx::as_stream(stream), a);
     fx::check_launch("fx_gls_assemble");
+  });
+}
+
+int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems, const double* x,
+                    const double* y, const fx_index* conn3, const fx_index* dofs9,
+                    const fx_index* node_rows, double rho, double mu, double dt, fx_index step,
+                    fx_index num_hist, fx_index lag, const double* hist, const double* nxt,
+                    const double* adj, const fx_index* inc_off, const fx_index* inc_elem,
+                    const fx_index* inc_meta, double* out, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(num_nodes >= 0 && num_states >= 0 && num_elems >= 0, FX_ERR_ARG,
+                "fx_gls_hist_vjp: negative size");
+    fx::require(dt > 0.0 && std::isfinite(dt), FX_ERR_ARG,
+                "NavierStokesModel requires a positive finite time step");
+    fx::require(std::isfinite(rho) && rho > 0.0, FX_ERR_ARG,
+                "NavierStokesModel requires positive finite density");
+    fx::require(std::isfinite(mu) && mu > 0.0, FX_ERR_ARG,
+                "NavierStokesModel requires positive finite viscosity");
+    // checkTimeAssemblyInputs' variable-block rule (femx/assembly/Assembly.hpp:100-103)
+    fx::require(num_hist > 0 && lag >= 0 && lag < num_hist, FX_ERR_ARG,
+                "Time assembly variable block is invalid");
+    if (num_states == 0) return;
+    fx::require(num_nodes > 0 && x && y && node_rows && hist && nxt && adj && inc_off && out
+                    && (num_elems == 0 || (conn3 && dofs9 && inc_elem && inc_meta)),
+                FX_ERR_ARG, "fx_gls_hist_vjp: null array");
+    const std::size_t S = std::size_t(num_states) * 8, H = S * num_hist;
+    fx::require(!fx::overlaps(out, S, hist, H) && !fx::overlaps(out, S, nxt, S)
+                    && !fx::overlaps(out, S, adj, S),
+                FX_ERR_ALIAS, "Navier history VJP output must not alias its inputs");
+    const unsigned   nb = fx::blocks_for(num_nodes, kGlsBlock);
+    const GlsVjpArgs a{num_nodes, num_states, x,   y,   conn3, dofs9,
+                       node_rows, step,       num_hist, lag, hist,  nxt,
+                       adj,       gls_const(rho, mu, dt), inc_off, inc_elem,
+                       reinterpret_cast<const uint4*>(inc_meta), out};
+    hipLaunchKernelGGL(k_gls_hist_vjp, dim3(nb), dim3(64), 0, fx::as_stream(stream), a);
+    fx::check_launch("fx_gls_hist_vjp");
   });
 }
 

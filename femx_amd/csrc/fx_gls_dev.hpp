@@ -483,4 +483,276 @@ __device__ __forceinline__ void gls_assemble_block(int blk, const GlsAsmArgs& a,
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// History VJP: out = (dR/dhist_lag)^T adj, the derivative femx takes with
+// Enzyme (applyNavierHistJacT, femx/model/ns/NavierStokesModel.cpp:321-363;
+// device femx/model/ns/Components.cu:175-238).  Written out by hand: with the
+// adjoint and next-state interpolants at a quadrature point,
+//   adj^T R_e = sum_q Jw l(u, G, adv, tau(u)),
+// and the column of node k is N_k dl/du + grad phi_k . dl/dG + alpha N_k dl/dadv
+// for the current state (lag 0) and beta N_k dl/dadv for the previous one,
+// with (alpha, beta) = (1.5, -0.5) when the advection velocity is
+// extrapolated and (1, 0) otherwise; pressure columns are zero.  tau depends on
+// u through |u| and elemLength's streamline direction.  One lane per node
+// incidence forms the node's two velocity columns for both lags.
+
+struct GlsVjpArgs
+{
+  int           num_nodes;
+  int           num_states;
+  const double* x;
+  const double* y;
+  const int*    conn;
+  const int*    dofs;
+  const int*    node_rows;
+  int           step;
+  int           num_hist;
+  int           lag;
+  const double* hist;
+  const double* nxt;
+  const double* adj;
+  GlsConst      c;
+  const int*    inc_off;
+  const int*    inc_elem;
+  const uint4*  inc_meta;
+  double*       out;
+};
+
+// Columns (k, 0..1) of node k = li for lag 0 (o0, o1) and lag 1 (p0, p1).
+__device__ __forceinline__ void gls_hvjp_cols(int li, const double (&gx)[3], const double (&gy)[3],
+                                              double jw, const double (&cur)[6],
+                                              const double (&prev)[6], bool extrap,
+                                              const double (&nx)[9], const double (&aj)[9],
+                                              const GlsConst& C, double& o0, double& o1,
+                                              double& p0, double& p1)
+{
+  const double al = extrap ? 1.5 : 1.0, be = extrap ? -0.5 : 0.0;
+  const double g[6] = {gx[0], gy[0], gx[1], gy[1], gx[2], gy[2]};
+  double       Dl[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, Dv[2][2] = {{0.0, 0.0}, {0.0, 0.0}};
+  double       G[2][2] = {{0.0, 0.0}, {0.0, 0.0}}, Dpi[2] = {0.0, 0.0}, Dq[2] = {0.0, 0.0};
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+    {
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+      {
+        Dl[c][d] += g[2 * k + d] * aj[2 * k + c];
+        Dv[c][d] += g[2 * k + d] * nx[2 * k + c];
+        G[c][d] += g[2 * k + d] * cur[2 * k + c];
+      }
+      Dpi[d] += g[2 * k + d] * aj[6 + k];
+      Dq[d] += g[2 * k + d] * nx[6 + k];
+    }
+  const double gkx = gsel3(gx[0], gx[1], gx[2], li), gky = gsel3(gy[0], gy[1], gy[2], li);
+  o0 = o1 = p0 = p1 = 0.0;
+#pragma unroll
+  for (int iq = 0; iq < 3; ++iq)
+  {
+    double Nq[3];
+    gls_shape(iq, Nq);
+    double u[2] = {0.0, 0.0}, av[2] = {0.0, 0.0}, lam[2] = {0.0, 0.0}, v[2] = {0.0, 0.0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+      {
+        u[c] += Nq[k] * cur[2 * k + c];
+        const double a = extrap ? 1.5 * cur[2 * k + c] - 0.5 * prev[2 * k + c] : cur[2 * k + c];
+        av[c] += Nq[k] * a;
+        lam[c] += Nq[k] * aj[2 * k + c];
+        v[c] += Nq[k] * nx[2 * k + c];
+      }
+    double ag[2], wv[2], z[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+    {
+      ag[c] = G[c][0] * av[0] + G[c][1] * av[1];
+      wv[c] = av[0] * Dl[c][0] + av[1] * Dl[c][1];
+      z[c]  = av[0] * Dv[c][0] + av[1] * Dv[c][1];
+    }
+    const double speed   = sqrt(u[0] * u[0] + u[1] * u[1]);
+    double       dtau[2] = {0.0, 0.0}, tau;
+    {
+      double     dir[2], sgn[3], sum = 0.0;
+      const bool moving = speed > 1.0e-10;
+      if (moving)
+      {
+        dir[0] = u[0] / speed;
+        dir[1] = u[1] / speed;
+      }
+      else
+        dir[0] = dir[1] = 1.0 / sqrt(2.0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+      {
+        const double gk = dir[0] * g[2 * k] + dir[1] * g[2 * k + 1];
+        sgn[k]          = gk >= 0.0 ? 1.0 : -1.0;
+        sum += sgn[k] * gk;
+      }
+      double flow = 0.0, diff = 0.0, dflow[2] = {0.0, 0.0}, ddiff[2] = {0.0, 0.0};
+      if (sum > 1.0e-14)
+      {
+        const double h = 2.0 / sum;
+        flow           = (2.0 * speed / h) * (2.0 * speed / h);
+        diff           = (4.0 * C.nu / (h * h)) * (4.0 * C.nu / (h * h));
+        if (moving)
+        {
+          const double S = speed * sum;
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+          {
+            double dS = 0.0;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) dS += sgn[k] * g[2 * k + c];
+            dflow[c]        = 2.0 * S * dS;
+            const double s4 = speed * speed * speed * speed;
+            ddiff[c]        = C.nu * C.nu
+                       * (4.0 * S * S * S * dS / s4
+                          - 4.0 * S * S * S * S * u[c] / (s4 * speed * speed));
+          }
+        }
+        else
+        {
+#pragma unroll
+          for (int c = 0; c < 2; ++c) dflow[c] = 8.0 / (h * h) * u[c];
+        }
+      }
+      tau = 1.0 / sqrt(C.time + flow + diff);
+#pragma unroll
+      for (int c = 0; c < 2; ++c) dtau[c] = -0.5 * tau * tau * tau * (dflow[c] + ddiff[c]);
+    }
+    double B[2], l1 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+    {
+      B[c] = -C.rdt * u[c] + 0.5 * C.rho * ag[c] + C.rdt * v[c] + 0.5 * C.rho * z[c] + Dq[c];
+      l1 += wv[c] * B[c];
+    }
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+      l1 += Dpi[d] * (-u[d] / C.dt + 0.5 * ag[d] + v[d] / C.dt + 0.5 * z[d] + Dq[d] / C.rho);
+    double Pu[2], Pag[2], Padv[2], PG[2][2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+    {
+      Pu[c]  = -C.rdt * lam[c] + tau * (-C.rdt * wv[c] - Dpi[c] / C.dt) + l1 * dtau[c];
+      Pag[c] = 0.5 * C.rho * lam[c] + tau * (0.5 * C.rho * wv[c] + 0.5 * Dpi[c]);
+    }
+#pragma unroll
+    for (int d = 0; d < 2; ++d)
+    {
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+      {
+        s += 0.5 * C.rho * lam[c] * Dv[c][d];
+        s += tau * (Dl[c][d] * B[c] + 0.5 * C.rho * wv[c] * Dv[c][d] + 0.5 * Dpi[c] * Dv[c][d]);
+        s += Pag[c] * G[c][d];
+      }
+      Padv[d] = s;
+    }
+#pragma unroll
+    for (int c = 0; c < 2; ++c)
+#pragma unroll
+      for (int d = 0; d < 2; ++d) PG[c][d] = 0.5 * C.mu * Dl[c][d] + Pag[c] * av[d];
+    const double Nk = gsel3(Nq[0], Nq[1], Nq[2], li);
+    const double d0 = Nk * Pu[0] + gkx * PG[0][0] + gky * PG[0][1] + al * Nk * Padv[0];
+    const double d1 = Nk * Pu[1] + gkx * PG[1][0] + gky * PG[1][1] + al * Nk * Padv[1];
+    o0 += jw * d0;
+    o1 += jw * d1;
+    p0 += jw * be * Nk * Padv[0];
+    p1 += jw * be * Nk * Padv[1];
+  }
+}
+
+// One block of kGlsBlock nodes, two lanes per node taking alternate
+// incidences; the node's two velocity entries are summed in ascending element
+// order (role 0's term, then role 1's, per round), its pressure entry is 0.
+__device__ __forceinline__ void gls_hist_vjp_block(int blk, const GlsVjpArgs& a)
+{
+  const int  lane   = threadIdx.x & 63;
+  const int  role   = lane & 1;
+  const int  n      = blk * kGlsBlock + (lane >> 1);
+  const bool live   = n < a.num_nodes && a.node_rows[3 * n] >= 0;
+  const bool extrap = a.step > 0 && a.num_hist > 1;
+  int        t0 = 0, t1 = 0;
+  if (live)
+  {
+    t0 = a.inc_off[n];
+    t1 = a.inc_off[n + 1];
+  }
+  double s0 = 0.0, s1 = 0.0;
+  int    rounds = (t1 - t0 + 1) >> 1;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) rounds = max(rounds, __shfl_xor(rounds, off));
+  for (int r = 0; r < rounds; ++r)
+  {
+    const int  t    = t0 + 2 * r + role;
+    const bool have = t < t1;
+    double     c0 = 0.0, c1 = 0.0;
+    if (have && a.lag < 2)
+    {
+      const int   e  = a.inc_elem[t];
+      const uint4 m1 = a.inc_meta[2 * t + 1];
+      const int   li = static_cast<int>((m1.z >> 24) & 0xffu); // byte 27
+      int         vv[3], d[9];
+      double      xe[3], ye[3], cur[6], prev[6], nx[9], aj[9];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) vv[k] = a.conn[3 * e + k];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) d[k] = a.dofs[9 * e + k];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+      {
+        xe[k] = a.x[vv[k]];
+        ye[k] = a.y[vv[k]];
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+      {
+        cur[k]  = a.hist[d[k]];
+        prev[k] = extrap ? a.hist[a.num_states + d[k]] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 9; ++k)
+      {
+        nx[k] = a.nxt[d[k]];
+        aj[k] = a.adj[d[k]];
+      }
+      const double j00 = xe[1] - xe[0], j01 = xe[2] - xe[0], j10 = ye[1] - ye[0],
+                   j11 = ye[2] - ye[0];
+      const double det = j00 * j11 - j01 * j10, inv_det = 1.0 / det;
+      const double i00 = j11 * inv_det, i01 = -j01 * inv_det, i10 = -j10 * inv_det,
+                   i11 = j00 * inv_det;
+      const double gx[3] = {-i00 - i10, i00, i10};
+      const double gy[3] = {-i01 - i11, i01, i11};
+      const double jw    = fabs(det) * (1.0 / 6.0);
+      double       o0, o1, p0, p1;
+      gls_hvjp_cols(li, gx, gy, jw, cur, prev, extrap, nx, aj, a.c, o0, o1, p0, p1);
+      c0 = a.lag == 0 ? o0 : p0;
+      c1 = a.lag == 0 ? o1 : p1;
+    }
+    const double q0 = __shfl_xor(c0, 1), q1 = __shfl_xor(c1, 1);
+    if (role == 0 && have)
+    {
+      s0 += c0;
+      s1 += c1;
+      if (t + 1 < t1)
+      {
+        s0 += q0;
+        s1 += q1;
+      }
+    }
+  }
+  if (live && role == 0)
+  {
+    __builtin_nontemporal_store(s0, a.out + a.node_rows[3 * n]);
+    __builtin_nontemporal_store(s1, a.out + a.node_rows[3 * n + 1]);
+    __builtin_nontemporal_store(0.0, a.out + a.node_rows[3 * n + 2]);
+  }
+}
+
 } // namespace fxk

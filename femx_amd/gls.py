@@ -184,6 +184,28 @@ def assemble(op: NavierOperator, step, num_hist, wrt, amap: AssemblyMap, hist, n
     return res
 
 
+def hist_vjp(op: NavierOperator, step, num_hist, lag, amap: AssemblyMap, hist, nxt, adj,
+             out=None, stream=None):
+    """(dR/dhist_lag)^T adj: TimeResidual::applyJacT with respect to a history
+    block (femx/model/ns/NavierStokesModel.cpp:321-363), the product femx
+    takes with Enzyme, here hand-derived.  Returns out (allocated when None)."""
+    geom = op.geom
+    _require(num_hist > 0 and 0 <= lag < num_hist, "Time assembly variable block is invalid")
+    _check_dev_vec(hist, num_hist * geom.num_states, torch.float64, "History VJP history")
+    _check_dev_vec(nxt, geom.num_states, torch.float64, "History VJP next state")
+    _check_dev_vec(adj, geom.num_states, torch.float64, "History VJP adjoint")
+    if out is None:
+        out = torch.empty(geom.num_states, dtype=torch.float64, device=geom.device)
+    _check_dev_vec(out, geom.num_states, torch.float64, "History VJP output")
+    p = _plan(amap, geom)
+    _lib.call("fx_gls_hist_vjp", geom.num_nodes, geom.num_states, geom.num_elems, ptr(geom.x),
+              ptr(geom.y), ptr(geom.conn), ptr(geom.dofs9), ptr(p.node_rows), op.fluid.rho,
+              op.fluid.mu, op.dt, int(step), int(num_hist), int(lag), ptr(hist), ptr(nxt),
+              ptr(adj), ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta), ptr(out),
+              stream_handle(stream))
+    return out
+
+
 def assemble_residual(op: NavierOperator, step, num_hist, amap: AssemblyMap, hist, nxt, res,
                       stream=None):
     """assembleResidual (femx/assembly/CudaAssembly.hpp:419-472): the time

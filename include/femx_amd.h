@@ -328,6 +328,22 @@ int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                     const fx_index* d_inc_meta, const fx_index* d_blk_runs,
                     const fx_index* d_node_tile, double* d_vals, double* d_res, void* stream);
 
+/* History VJP of the same operator (device): out = (dR/dhist_lag)^T adj for
+ * one history lag, the product femx takes with Enzyme
+ * (applyNavierHistJacT, femx/model/ns/NavierStokesModel.cpp:321-363;
+ * device femx/model/ns/Components.cu:175-238).  Hand-derived reverse mode of
+ * NavierOperator's residual through u, grad u, the extrapolated advection
+ * velocity and tau(u); pressure entries are zero.  adj has num_states
+ * entries (the residual space).  Per state DOF, a sum over its elements in
+ * ascending element order.  Tables from fx_map_gls_plan. */
+int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
+                    const double* d_x, const double* d_y, const fx_index* d_conn3,
+                    const fx_index* d_dofs9, const fx_index* d_node_rows, double rho, double mu,
+                    double dt, fx_index step, fx_index num_hist, fx_index lag,
+                    const double* d_hist, const double* d_nxt, const double* d_adj,
+                    const fx_index* d_inc_off, const fx_index* d_inc_elem,
+                    const fx_index* d_inc_meta, double* d_out, void* stream);
+
 /* ======================================================================== */
 /* Batches (device).  Many P1 Poisson and Taylor-Hood instances of uneven size
  * assembled and differentiated by persistent launches over one work list

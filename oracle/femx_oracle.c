@@ -1026,3 +1026,365 @@ void or_gls_assemble_omp(or_index num_states, const double* x, const double* y,
     }
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* History derivatives of the GLS residual.  femx obtains them with Enzyme
+ * (femx/model/ns/Components.cpp:65-121 histVjp over evalNavierAdj,
+ * Components.hpp:604-643; device Components.cu:175-238), which is not
+ * available here.  Two independent restatements:
+ *   or_gls_element_jvp   forward mode: the residual code of or_gls_element
+ *                        re-evaluated in dual numbers (value, tangent);
+ *   or_gls_element_hvjp  reverse mode written out by hand from the
+ *                        residual's closed form (see below).
+ * They are pinned against each other by the adjoint identity
+ * dir^T (dR/dh)^T adj = adj^T (dR/dh) dir, and the forward mode against
+ * central differences of the residual. */
+typedef struct
+{
+  double v, d;
+} dual;
+
+static dual dc(double v) { dual r = {v, 0.0}; return r; }
+static dual dadd(dual a, dual b) { dual r = {a.v + b.v, a.d + b.d}; return r; }
+static dual dsub(dual a, dual b) { dual r = {a.v - b.v, a.d - b.d}; return r; }
+static dual dmul(dual a, dual b) { dual r = {a.v * b.v, a.d * b.v + a.v * b.d}; return r; }
+static dual ddiv(dual a, dual b)
+{
+  dual r = {a.v / b.v, (a.d * b.v - a.v * b.d) / (b.v * b.v)};
+  return r;
+}
+static dual dsqrt(dual a)
+{
+  const double s = sqrt(a.v);
+  dual         r = {s, s > 0.0 ? a.d / (2.0 * s) : 0.0};
+  return r;
+}
+static dual dabs(dual a) { return a.v >= 0.0 ? a : dsub(dc(0.0), a); }
+static dual dscale(double s, dual a) { dual r = {s * a.v, s * a.d}; return r; }
+
+/* Residual rows of one element (NavierOperator::evalResidualRow,
+ * Components.hpp:448-455,473-577) in dual numbers: hist_e and dhist_e are
+ * lag-major (num_hist x 9), the tangent acting on the history only. */
+void or_gls_element_jvp(const double* xe, const double* ye, double rho, double mu, double dt,
+                        int step, int num_hist, const double* hist_e, const double* dhist_e,
+                        const double* nxt_e, double* Re, double* dRe)
+{
+  double rs[6], w[3], invJ[4], jxw[3], N[3][3], g[3][6];
+  static const double dNdr[6] = {-1.0, -1.0, 1.0, 0.0, 0.0, 1.0};
+  or_quad_tri(2, rs, w);
+  for (int q = 0; q < 3; ++q)
+  {
+    jxw[q] = tri_geometry(xe, ye, w[q], invJ);
+    phys_grad(3, dNdr, invJ, g[q]);
+    N[q][0] = 1.0 - rs[2 * q] - rs[2 * q + 1];
+    N[q][1] = rs[2 * q];
+    N[q][2] = rs[2 * q + 1];
+  }
+  const int extrap = step > 0 && num_hist > 1;
+  dual      cur[6], adv[6];
+  for (int k = 0; k < 6; ++k)
+  {
+    cur[k].v = hist_e[k];
+    cur[k].d = dhist_e[k];
+    adv[k]   = cur[k];
+    if (extrap)
+    {
+      dual prev = {hist_e[9 + k], dhist_e[9 + k]};
+      adv[k]    = dsub(dscale(1.5, cur[k]), dscale(0.5, prev));
+    }
+  }
+  const double nu = mu / rho, time = (2.0 / dt) * (2.0 / dt);
+  for (int row = 0; row < 9; ++row)
+  {
+    const int vel = row < 6;
+    const int i   = vel ? row / 2 : row - 6;
+    const int c   = vel ? row - 2 * i : 0;
+    dual      F = dc(0.0), K[9];
+    for (int col = 0; col < 9; ++col) K[col] = dc(0.0);
+    for (int iq = 0; iq < 3; ++iq)
+    {
+      const double* gq = g[iq];
+      dual          u[2] = {dc(0.0), dc(0.0)}, av[2] = {dc(0.0), dc(0.0)}, G[2][2], ag[2];
+      for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) G[a][b] = dc(0.0);
+      for (int in = 0; in < 3; ++in)
+        for (int cc = 0; cc < 2; ++cc)
+        {
+          u[cc]  = dadd(u[cc], dscale(N[iq][in], cur[2 * in + cc]));
+          av[cc] = dadd(av[cc], dscale(N[iq][in], adv[2 * in + cc]));
+          for (int d = 0; d < 2; ++d)
+            G[cc][d] = dadd(G[cc][d], dscale(gq[2 * in + d], cur[2 * in + cc]));
+        }
+      for (int cc = 0; cc < 2; ++cc)
+        ag[cc] = dadd(dmul(G[cc][0], av[0]), dmul(G[cc][1], av[1]));
+      const dual speed = dsqrt(dadd(dmul(u[0], u[0]), dmul(u[1], u[1])));
+      dual       dir[2];
+      if (speed.v > 1.0e-10)
+      {
+        dir[0] = ddiv(u[0], speed);
+        dir[1] = ddiv(u[1], speed);
+      }
+      else
+        dir[0] = dir[1] = dc(1.0 / sqrt(2.0));
+      dual sum = dc(0.0);
+      for (int in = 0; in < 3; ++in)
+        sum = dadd(sum, dabs(dadd(dscale(gq[2 * in], dir[0]), dscale(gq[2 * in + 1], dir[1]))));
+      dual flow = dc(0.0), diff = dc(0.0);
+      if (sum.v > 1.0e-14)
+      {
+        const dual h  = ddiv(dc(2.0), sum);
+        const dual f  = ddiv(dscale(2.0, speed), h);
+        const dual vv = ddiv(dc(4.0 * nu), dmul(h, h));
+        flow          = dmul(f, f);
+        diff          = dmul(vv, vv);
+      }
+      const dual   tau = ddiv(dc(1.0), dsqrt(dadd(dadd(dc(time), flow), diff)));
+      const double Jw  = jxw[iq], Ni = N[iq][i];
+      const dual   dvi = dadd(dscale(gq[2 * i], av[0]), dscale(gq[2 * i + 1], av[1]));
+      if (vel)
+      {
+        const dual gu = dadd(dscale(gq[2 * i], G[c][0]), dscale(gq[2 * i + 1], G[c][1]));
+        dual t = dscale(rho / dt * Ni, u[c]);
+        t      = dsub(t, dscale(0.5 * rho * Ni, ag[c]));
+        t      = dsub(t, dscale(0.5 * mu, gu));
+        t      = dadd(t, dmul(dscale(rho / dt, tau), dmul(dvi, u[c])));
+        t      = dsub(t, dmul(dscale(0.5 * rho, tau), dmul(dvi, ag[c])));
+        F      = dadd(F, dscale(Jw, t));
+        for (int j = 0; j < 3; ++j)
+        {
+          const double Nj  = N[iq][j];
+          const dual   dvj = dadd(dscale(gq[2 * j], av[0]), dscale(gq[2 * j + 1], av[1]));
+          const double gd  = gq[2 * i] * gq[2 * j] + gq[2 * i + 1] * gq[2 * j + 1];
+          dual k = dc(rho / dt * Ni * Nj + 0.5 * mu * gd);
+          k      = dadd(k, dscale(0.5 * rho * Ni, dvj));
+          k      = dadd(k, dmul(dscale(rho / dt * Nj, tau), dvi));
+          k      = dadd(k, dmul(dscale(0.5 * rho, tau), dmul(dvi, dvj)));
+          K[2 * j + c] = dadd(K[2 * j + c], dscale(Jw, k));
+          const dual kp = dadd(dc(-gq[2 * i + c] * Nj), dmul(tau, dscale(gq[2 * j + c], dvi)));
+          K[6 + j]      = dadd(K[6 + j], dscale(Jw, kp));
+        }
+      }
+      else
+      {
+        dual div_u = dc(0.0), div_adv = dc(0.0);
+        for (int d = 0; d < 2; ++d)
+        {
+          div_u   = dadd(div_u, dscale(gq[2 * i + d], u[d]));
+          div_adv = dadd(div_adv, dscale(gq[2 * i + d], ag[d]));
+        }
+        F = dadd(F, dscale(Jw, dsub(dmul(dscale(1.0 / dt, tau), div_u),
+                                    dmul(dscale(0.5, tau), div_adv))));
+        for (int j = 0; j < 3; ++j)
+        {
+          const double Nj  = N[iq][j];
+          const dual   dvj = dadd(dscale(gq[2 * j], av[0]), dscale(gq[2 * j + 1], av[1]));
+          for (int d = 0; d < 2; ++d)
+          {
+            const double gi = gq[2 * i + d];
+            dual k = dc(Ni * gq[2 * j + d]);
+            k      = dadd(k, dscale(gi * Nj / dt, tau));
+            k      = dadd(k, dmul(dscale(0.5 * gi, tau), dvj));
+            K[2 * j + d] = dadd(K[2 * j + d], dscale(Jw, k));
+          }
+          const double gd = gq[2 * i] * gq[2 * j] + gq[2 * i + 1] * gq[2 * j + 1];
+          K[6 + j]        = dadd(K[6 + j], dscale(gd / rho * Jw, tau));
+        }
+      }
+    }
+    dual out = dsub(dc(0.0), F);
+    for (int col = 0; col < 9; ++col) out = dadd(out, dscale(nxt_e[col], K[col]));
+    Re[row]  = out.v;
+    dRe[row] = out.d;
+  }
+}
+
+/* Reverse mode by hand.  With adjoint a (9 per element) and next state n,
+ * adj^T R_e = sum_q Jw_q l_q, where at a quadrature point, writing lam_c =
+ * sum_i N_i a(i,c), Dlam_cd = sum_i dphi_i/dx_d a(i,c), pi = sum_i N_i a_p(i),
+ * Dpi_d, v_c, Dv_cd, q, Dq_d likewise from n, w_c = adv . Dlam_c,
+ * z_c = adv . Dv_c, ag = G adv with G the velocity gradient:
+ *   l = -rho/dt lam.u + rho/2 lam.ag + mu/2 Dlam:G + rho/dt lam.v
+ *       + rho/2 lam.z + mu/2 Dlam:Dv - q div lam + pi div v + tau l1,
+ *   l1 = sum_c w_c B_c - Dpi.u/dt + Dpi.ag/2 + Dpi.v/dt + Dpi.z/2 + Dpi.Dq/rho,
+ *   B_c = -rho/dt u_c + rho/2 ag_c + rho/dt v_c + rho/2 z_c + dq/dx_c.
+ * u and G depend on lag 0, adv = 1.5 lag0 - 0.5 lag1 (step > 0, two lags) or
+ * lag 0, tau on u through |u| and elemLength.  out is (9 per lag) the
+ * element columns for both lags; pressure columns are zero. */
+void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double mu, double dt,
+                         int step, int num_hist, const double* hist_e, const double* nxt_e,
+                         const double* adj_e, double* out)
+{
+  double rs[6], w[3], invJ[4], jxw[3], N[3][3], g[6];
+  static const double dNdr[6] = {-1.0, -1.0, 1.0, 0.0, 0.0, 1.0};
+  or_quad_tri(2, rs, w);
+  for (int q = 0; q < 3; ++q)
+  {
+    jxw[q] = tri_geometry(xe, ye, w[q], invJ);
+    N[q][0] = 1.0 - rs[2 * q] - rs[2 * q + 1];
+    N[q][1] = rs[2 * q];
+    N[q][2] = rs[2 * q + 1];
+  }
+  phys_grad(3, dNdr, invJ, g);
+  const int    extrap = step > 0 && num_hist > 1;
+  const double al = extrap ? 1.5 : 1.0, be = extrap ? -0.5 : 0.0;
+  const double nu = mu / rho, time = (2.0 / dt) * (2.0 / dt), rdt = rho / dt;
+  double       Dl[2][2] = {{0, 0}, {0, 0}}, Dv[2][2] = {{0, 0}, {0, 0}}, G[2][2] = {{0, 0}, {0, 0}};
+  double       Dpi[2] = {0, 0}, Dq[2] = {0, 0};
+  for (int k = 0; k < 3; ++k)
+    for (int d = 0; d < 2; ++d)
+    {
+      for (int c = 0; c < 2; ++c)
+      {
+        Dl[c][d] += g[2 * k + d] * adj_e[2 * k + c];
+        Dv[c][d] += g[2 * k + d] * nxt_e[2 * k + c];
+        G[c][d] += g[2 * k + d] * hist_e[2 * k + c];
+      }
+      Dpi[d] += g[2 * k + d] * adj_e[6 + k];
+      Dq[d] += g[2 * k + d] * nxt_e[6 + k];
+    }
+  for (int k = 0; k < 18; ++k) out[k] = 0.0;
+  for (int iq = 0; iq < 3; ++iq)
+  {
+    const double* Nq = N[iq];
+    double        u[2] = {0, 0}, av[2] = {0, 0}, lam[2] = {0, 0}, v[2] = {0, 0};
+    for (int k = 0; k < 3; ++k)
+      for (int c = 0; c < 2; ++c)
+      {
+        u[c] += Nq[k] * hist_e[2 * k + c];
+        const double a = extrap ? 1.5 * hist_e[2 * k + c] - 0.5 * hist_e[9 + 2 * k + c]
+                                : hist_e[2 * k + c];
+        av[c] += Nq[k] * a;
+        lam[c] += Nq[k] * adj_e[2 * k + c];
+        v[c] += Nq[k] * nxt_e[2 * k + c];
+      }
+    double ag[2], wv[2], z[2];
+    for (int c = 0; c < 2; ++c)
+    {
+      ag[c] = G[c][0] * av[0] + G[c][1] * av[1];
+      wv[c] = av[0] * Dl[c][0] + av[1] * Dl[c][1];
+      z[c]  = av[0] * Dv[c][0] + av[1] * Dv[c][1];
+    }
+    /* tau and its gradient in u */
+    const double speed = sqrt(u[0] * u[0] + u[1] * u[1]);
+    double       dtau[2] = {0.0, 0.0}, tau;
+    {
+      double dir[2], sgn[3], sum = 0.0;
+      const int moving = speed > 1.0e-10;
+      if (moving)
+      {
+        dir[0] = u[0] / speed;
+        dir[1] = u[1] / speed;
+      }
+      else
+        dir[0] = dir[1] = 1.0 / sqrt(2.0);
+      for (int k = 0; k < 3; ++k)
+      {
+        const double gk = dir[0] * g[2 * k] + dir[1] * g[2 * k + 1];
+        sgn[k]          = gk >= 0.0 ? 1.0 : -1.0;
+        sum += sgn[k] * gk;
+      }
+      double flow = 0.0, diff = 0.0, dflow[2] = {0, 0}, ddiff[2] = {0, 0};
+      if (sum > 1.0e-14)
+      {
+        const double h = 2.0 / sum;
+        flow           = (2.0 * speed / h) * (2.0 * speed / h);
+        diff           = (4.0 * nu / (h * h)) * (4.0 * nu / (h * h));
+        if (moving)
+        {
+          /* S = sum_k |u.g_k| = speed*sum: flow = S^2, diff = nu^2 S^4 / speed^4 */
+          const double S = speed * sum;
+          for (int c = 0; c < 2; ++c)
+          {
+            double dS = 0.0;
+            for (int k = 0; k < 3; ++k) dS += sgn[k] * g[2 * k + c];
+            dflow[c] = 2.0 * S * dS;
+            const double s4 = speed * speed * speed * speed;
+            ddiff[c] = nu * nu * (4.0 * S * S * S * dS / s4 - 4.0 * S * S * S * S * u[c] / (s4 * speed * speed));
+          }
+        }
+        else
+          for (int c = 0; c < 2; ++c) dflow[c] = 8.0 / (h * h) * u[c];
+      }
+      tau = 1.0 / sqrt(time + flow + diff);
+      for (int c = 0; c < 2; ++c) dtau[c] = -0.5 * tau * tau * tau * (dflow[c] + ddiff[c]);
+    }
+    double B[2], l1 = 0.0;
+    for (int c = 0; c < 2; ++c)
+    {
+      B[c] = -rdt * u[c] + 0.5 * rho * ag[c] + rdt * v[c] + 0.5 * rho * z[c] + Dq[c];
+      l1 += wv[c] * B[c];
+    }
+    for (int d = 0; d < 2; ++d)
+      l1 += Dpi[d] * (-u[d] / dt + 0.5 * ag[d] + v[d] / dt + 0.5 * z[d] + Dq[d] / rho);
+    double Pu[2], Pag[2], Padv[2], PG[2][2];
+    for (int c = 0; c < 2; ++c)
+    {
+      Pu[c]  = -rdt * lam[c] + tau * (-rdt * wv[c] - Dpi[c] / dt) + l1 * dtau[c];
+      Pag[c] = 0.5 * rho * lam[c] + tau * (0.5 * rho * wv[c] + 0.5 * Dpi[c]);
+    }
+    for (int d = 0; d < 2; ++d)
+    {
+      double s = 0.0;
+      for (int c = 0; c < 2; ++c)
+      {
+        s += 0.5 * rho * lam[c] * Dv[c][d];
+        s += tau * (Dl[c][d] * B[c] + 0.5 * rho * wv[c] * Dv[c][d] + 0.5 * Dpi[c] * Dv[c][d]);
+        s += Pag[c] * G[c][d];
+      }
+      Padv[d] = s;
+    }
+    for (int c = 0; c < 2; ++c)
+      for (int d = 0; d < 2; ++d) PG[c][d] = 0.5 * mu * Dl[c][d] + Pag[c] * av[d];
+    const double Jw = jxw[iq];
+    for (int k = 0; k < 3; ++k)
+      for (int c = 0; c < 2; ++c)
+      {
+        const double d0 = Nq[k] * Pu[c] + g[2 * k] * PG[c][0] + g[2 * k + 1] * PG[c][1]
+                          + al * Nq[k] * Padv[c];
+        out[2 * k + c] += Jw * d0;
+        out[9 + 2 * k + c] += Jw * be * Nq[k] * Padv[c];
+      }
+  }
+}
+
+/* Operator-level history VJP (NavierStokesModel.cpp:321-363
+ * applyNavierHistJacT): out = (dR/dhist_lag)^T adj over all elements,
+ * summed in ascending element order. */
+void or_gls_hist_vjp(or_index num_states, const double* x, const double* y, or_index num_elems,
+                     const or_index* conn, const or_index* dofs9, double rho, double mu,
+                     double dt, int step, int num_hist, int lag, const double* hist,
+                     const double* nxt, const double* adj, double* out)
+{
+  const int nh = num_hist < 2 ? num_hist : 2;
+  memset(out, 0, sizeof(double) * num_states);
+  for (or_index e = 0; e < num_elems; ++e)
+  {
+    double xe[3], ye[3], he[18] = {0}, ne[9], ae[9], ve[18];
+    gls_gather(x, y, conn + 3 * e, dofs9 + 9 * e, num_states, nh, hist, nxt, xe, ye, he, ne);
+    for (int k = 0; k < 9; ++k) ae[k] = adj[dofs9[9 * e + k]];
+    or_gls_element_hvjp(xe, ye, rho, mu, dt, step, num_hist, he, ne, ae, ve);
+    if (lag < 2)
+      for (int k = 0; k < 9; ++k) out[dofs9[9 * e + k]] += ve[9 * lag + k];
+  }
+}
+
+/* Operator-level forward mode: out = (dR/dhist_lag) dir, assembled from the
+ * dual-number element JVP in ascending element order (the independent check
+ * of or_gls_hist_vjp through the adjoint identity). */
+void or_gls_hist_jvp(or_index num_states, const double* x, const double* y, or_index num_elems,
+                     const or_index* conn, const or_index* dofs9, double rho, double mu,
+                     double dt, int step, int num_hist, int lag, const double* hist,
+                     const double* nxt, const double* dir, double* out)
+{
+  const int nh = num_hist < 2 ? num_hist : 2;
+  memset(out, 0, sizeof(double) * num_states);
+  for (or_index e = 0; e < num_elems; ++e)
+  {
+    double xe[3], ye[3], he[18] = {0}, de[18] = {0}, ne[9], Re[9], dRe[9];
+    gls_gather(x, y, conn + 3 * e, dofs9 + 9 * e, num_states, nh, hist, nxt, xe, ye, he, ne);
+    if (lag < 2)
+      for (int k = 0; k < 9; ++k) de[9 * lag + k] = dir[dofs9[9 * e + k]];
+    or_gls_element_jvp(xe, ye, rho, mu, dt, step, num_hist, he, de, ne, Re, dRe);
+    for (int k = 0; k < 9; ++k) out[dofs9[9 * e + k]] += dRe[k];
+  }
+}

@@ -117,6 +117,24 @@ void or_gls_assemble_omp(or_index num_states, const double* x, const double* y,
                          const double* hist, const double* nxt, const or_index* jac_map,
                          or_index nnz, double* vals, double* res);
 
+/* History derivatives (femx computes them with Enzyme): forward-mode JVP
+ * of the element residual in dual numbers, hand-written element VJP (both
+ * lags, 18 values), and the operator-level VJP for one lag. */
+void or_gls_element_jvp(const double* xe, const double* ye, double rho, double mu, double dt,
+                        int step, int num_hist, const double* hist_e, const double* dhist_e,
+                        const double* nxt_e, double* Re, double* dRe);
+void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double mu, double dt,
+                         int step, int num_hist, const double* hist_e, const double* nxt_e,
+                         const double* adj_e, double* out);
+void or_gls_hist_vjp(or_index num_states, const double* x, const double* y, or_index num_elems,
+                     const or_index* conn, const or_index* dofs9, double rho, double mu,
+                     double dt, int step, int num_hist, int lag, const double* hist,
+                     const double* nxt, const double* adj, double* out);
+void or_gls_hist_jvp(or_index num_states, const double* x, const double* y, or_index num_elems,
+                     const or_index* conn, const or_index* dofs9, double rho, double mu,
+                     double dt, int step, int num_hist, int lag, const double* hist,
+                     const double* nxt, const double* dir, double* out);
+
 int or_omp_threads(void);
 
 #ifdef __cplusplus

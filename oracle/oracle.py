@@ -49,7 +49,9 @@ def lib():
                      "or_matvec", "or_matvecT"):
             getattr(_LIB, name).restype = None
         _LIB.or_th_assemble.argtypes = None
-        for name in ("or_gls_element", "or_gls_assemble", "or_gls_assemble_omp"):
+        for name in ("or_gls_element", "or_gls_assemble", "or_gls_assemble_omp",
+                     "or_gls_element_jvp", "or_gls_element_hvjp", "or_gls_hist_vjp",
+                     "or_gls_hist_jvp"):
             getattr(_LIB, name).restype = None
     return _LIB
 
@@ -302,6 +304,52 @@ def gls_assemble(x, y, conn, dofs9, num_states, rho, mu, dt, step, num_hist, his
        C.c_double(mu), C.c_double(dt), C.c_int(step), C.c_int(num_hist), _p(hist), _p(nxt),
        _p(jac_map), C.c_int32(nnz), _p(vals), _p(res))
     return vals, res
+
+
+def gls_element_jvp(xe, ye, rho, mu, dt, step, num_hist, hist_e, dhist_e, nxt_e):
+    """(R_e, dR_e/dhist . dhist) by dual numbers; hist arrays lag-major x 9."""
+    Re = np.empty(9, F64)
+    dRe = np.empty(9, F64)
+    f = lambda a: _p(np.ascontiguousarray(a, F64))  # noqa: E731
+    lib().or_gls_element_jvp(f(xe), f(ye), C.c_double(rho), C.c_double(mu), C.c_double(dt),
+                             C.c_int(step), C.c_int(num_hist), f(hist_e), f(dhist_e), f(nxt_e),
+                             _p(Re), _p(dRe))
+    return Re, dRe
+
+
+def gls_element_hvjp(xe, ye, rho, mu, dt, step, num_hist, hist_e, nxt_e, adj_e):
+    """(dR_e/dhist)^T adj for both lags: array [2, 9]."""
+    out = np.empty(18, F64)
+    h = np.zeros(18, F64)
+    h[:np.asarray(hist_e).size] = hist_e
+    f = lambda a: _p(np.ascontiguousarray(a, F64))  # noqa: E731
+    lib().or_gls_element_hvjp(f(xe), f(ye), C.c_double(rho), C.c_double(mu), C.c_double(dt),
+                              C.c_int(step), C.c_int(num_hist), _p(h), f(nxt_e), f(adj_e),
+                              _p(out))
+    return out.reshape(2, 9)
+
+
+def gls_hist_vjp(x, y, conn, dofs9, num_states, rho, mu, dt, step, num_hist, lag, hist, nxt,
+                 adj):
+    out = np.empty(num_states, F64)
+    E = conn.size // 3
+    f = lambda a: _p(np.ascontiguousarray(a, F64))  # noqa: E731
+    lib().or_gls_hist_vjp(C.c_int32(num_states), f(x), f(y), C.c_int32(E), _p(conn), _p(dofs9),
+                          C.c_double(rho), C.c_double(mu), C.c_double(dt), C.c_int(step),
+                          C.c_int(num_hist), C.c_int(lag), f(hist), f(nxt), f(adj), _p(out))
+    return out
+
+
+def gls_hist_jvp(x, y, conn, dofs9, num_states, rho, mu, dt, step, num_hist, lag, hist, nxt,
+                 dirv):
+    """(dR/dhist_lag) dir assembled from the dual-number element JVP."""
+    out = np.empty(num_states, F64)
+    E = conn.size // 3
+    f = lambda a: _p(np.ascontiguousarray(a, F64))  # noqa: E731
+    lib().or_gls_hist_jvp(C.c_int32(num_states), f(x), f(y), C.c_int32(E), _p(conn), _p(dofs9),
+                          C.c_double(rho), C.c_double(mu), C.c_double(dt), C.c_int(step),
+                          C.c_int(num_hist), C.c_int(lag), f(hist), f(nxt), f(dirv), _p(out))
+    return out
 
 
 def ref_ns_gls(x, y, conn, rho, mu, dt, step, num_hist, hist, nxt):

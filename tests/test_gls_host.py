@@ -295,3 +295,115 @@ def test_operator_parameter_checks():
         gls.NavierOperator(geom, gls.FluidParams(-1.0, 1.0), 0.1)
     with pytest.raises(RuntimeError, match="viscosity"):
         gls.NavierOperator(geom, gls.FluidParams(1.0, float("nan")), 0.1)
+
+
+# ---------------------------------------------------------------------------
+# history VJP (femx takes it with Enzyme; here hand-derived, checked against
+# an independent dual-number forward mode and against finite differences)
+
+def _near(a, b, tol):
+    """femx's near() (runNavierOperatorTests.cpp:44-48)."""
+    return abs(a - b) <= tol * max(1.0, abs(a), abs(b))
+
+
+ELEM = (np.array([0.1, 1.2, 0.3]), np.array([-0.1, 0.2, 0.9]))
+
+
+@pytest.mark.parametrize("step,nh", [(1, 2), (0, 1), (0, 2), (3, 2), (2, 3)])
+@pytest.mark.parametrize("still", [False, True])
+def test_hist_vjp_element_adjoint_identity(step, nh, still):
+    """dir^T (dR/dh)^T a from the hand-written reverse mode equals a^T (dR/dh) dir
+    from the dual-number forward mode (1e-13), for both lags at once; the
+    dual value part equals the residual; pressure columns are zero."""
+    rng = np.random.default_rng(10 * step + nh + (100 if still else 0))
+    h = rng.standard_normal(27)
+    if still:  # zero velocity in both lags: elemLength's fallback direction
+        h[:6] = 0.0
+        h[9:15] = 0.0
+    n, a = rng.standard_normal(9), rng.standard_normal(9)
+    d = rng.standard_normal(18)
+    args = ELEM + (1.2, 0.03, 0.05, step, nh)
+    nhe = min(nh, 2)
+    R, dR = O.gls_element_jvp(*args, h[:18], np.r_[d, np.zeros(0)], n)
+    _, R0 = O.gls_element(*args, h[:9 * nhe], n)
+    assert np.allclose(R, R0, rtol=1e-14, atol=1e-14 * np.abs(R0).max())
+    V = O.gls_element_hvjp(*args, h[:9 * nhe], n, a)
+    assert not V[:, 6:].any()
+    if not (step > 0 and nh > 1):
+        assert not V[1].any()  # no extrapolation: the previous state does not enter
+    lhs = a @ dR
+    rhs = V[0] @ d[:9] + (V[1] @ d[9:] if nh > 1 else 0.0)
+    assert abs(lhs - rhs) <= 1e-13 * max(1.0, abs(lhs))
+
+
+def test_hist_vjp_element_finite_difference():
+    """Central differences of the residual (h = 1e-6) against the forward mode."""
+    rng = np.random.default_rng(4)
+    h, n, a, d = (rng.standard_normal(k) for k in (18, 9, 9, 18))
+    args = ELEM + (1.2, 0.03, 0.05, 1, 2)
+    _, dR = O.gls_element_jvp(*args, h, d, n)
+    eps = 1e-6
+    _, Rp = O.gls_element(*args, h + eps * d, n)
+    _, Rm = O.gls_element(*args, h - eps * d, n)
+    fd = (Rp - Rm) / (2 * eps)
+    assert np.allclose(dR, fd, rtol=1e-7, atol=1e-7 * np.abs(fd).max())
+
+
+@pytest.mark.parametrize("lag", [0, 1])
+def test_hist_vjp_femx_finite_difference_test(lag):
+    """runNavierOperatorTests.cpp:292-382 restated on a triangle mesh: rho
+    1.2, mu 0.03, dt 0.05, step 1, two lags filled as fillStates does, dir_i =
+    0.2 - 0.01 i, adj_i = -0.15 + 0.013 i, eps 1e-6, near(vjp, fd, 2e-6)."""
+    x, y, conn = O.mesh_p1(2)
+    x = 2.0 * x  # the reference test's 2 x 1 domain
+    N = x.size
+    ns_ = 3 * N
+    d9 = O.gls_dofs(N, conn)
+    off = O.uniform_offsets(conn.size // 3, 9)
+    _, ci, _, jm = O.map_build(ns_, ns_, off, d9, off, d9)
+    i = np.arange(ns_)
+    hist = np.concatenate([0.01 * (i + 1), -0.005 * (i + 1)])
+    nxt = 0.02 - 0.003 * i
+    dirv = 0.2 - 0.01 * i
+    adj = -0.15 + 0.013 * i
+    args = (x, y, conn, d9, ns_, 1.2, 0.03, 0.05, 1, 2)
+    eps = 1e-6
+    hp, hm = hist.copy(), hist.copy()
+    hp[lag * ns_:(lag + 1) * ns_] += eps * dirv
+    hm[lag * ns_:(lag + 1) * ns_] -= eps * dirv
+    _, rp = O.gls_assemble(*args, hp, nxt, jm, ci.size, jac=False)
+    _, rm = O.gls_assemble(*args, hm, nxt, jm, ci.size, jac=False)
+    fd = (rp @ adj - rm @ adj) / (2 * eps)
+    jtw = O.gls_hist_vjp(x, y, conn, d9, ns_, 1.2, 0.03, 0.05, 1, 2, lag, hist, nxt, adj)
+    assert _near(dirv @ jtw, fd, 2e-6)
+    assert not jtw[2 * N:].any()
+
+
+def test_hist_vjp_operator_adjoint_identity():
+    """Assembled over a perturbed mesh: dir^T VJP_lag(adj) = adj^T JVP(dir in
+    lag), the JVP assembled element by element from the dual-number mode."""
+    x, y, conn = O.mesh_p1(4)
+    rng = np.random.default_rng(6)
+    inner = (x > 0) & (x < 1) & (y > 0) & (y < 1)
+    x[inner] += 0.05 * rng.uniform(-1, 1, inner.sum())
+    N = x.size
+    ns_ = 3 * N
+    d9 = O.gls_dofs(N, conn).reshape(-1, 9)
+    hist = rng.standard_normal(2 * ns_)
+    nxt, adj = rng.standard_normal(ns_), rng.standard_normal(ns_)
+    for lag in (0, 1):
+        dirv = rng.standard_normal(ns_)
+        jtw = O.gls_hist_vjp(x, y, conn, d9.reshape(-1), ns_, 1.0, 0.01, 0.02, 2, 2, lag, hist,
+                             nxt, adj)
+        jv = np.zeros(ns_)
+        c3 = conn.reshape(-1, 3)
+        for e in range(c3.shape[0]):
+            he = np.concatenate([hist[d9[e]], hist[ns_ + d9[e]]])
+            de = np.zeros(18)
+            de[9 * lag:9 * lag + 9] = dirv[d9[e]]
+            _, dR = O.gls_element_jvp(x[c3[e]], y[c3[e]], 1.0, 0.01, 0.02, 2, 2, he, de, nxt[d9[e]])
+            np.add.at(jv, d9[e], dR)
+        assert abs(dirv @ jtw - adj @ jv) <= 1e-12 * max(1.0, abs(adj @ jv))
+        jc = O.gls_hist_jvp(x, y, conn, d9.reshape(-1), ns_, 1.0, 0.01, 0.02, 2, 2, lag, hist, nxt,
+                            dirv)
+        assert np.allclose(jc, jv, rtol=1e-14, atol=1e-14 * np.abs(jv).max())

@@ -144,3 +144,79 @@ def test_gls_1024_matches_oracle_and_invariants():
     kx = torch.empty_like(res)
     MatrixHandler().matvec(A, n_d, kx)
     assert close(host(res) - host(r0), host(kx))
+
+
+@pytest.mark.parametrize("n,renumber", [(3, False), (40, True)])
+@pytest.mark.parametrize("step,nh", [(1, 2), (0, 1), (2, 3)])
+def test_gls_hist_vjp_matches_oracle_bitwise(n, renumber, step, nh):
+    """The history VJP (femx's Enzyme product, hand-derived here) against the
+    oracle's restatement: same operation order, bitwise equal, every lag."""
+    print("case", n, renumber, step, nh, flush=True)
+    x, y, conn = perturbed(n, 3 + n, renumber)
+    geom = gls.GLSGeometry(x, y, conn, DEV)
+    amap = gls.make_navier_map(geom)
+    nd = geom.num_states
+    rng = np.random.default_rng(20 + step)
+    hist = rng.standard_normal(nh * nd)
+    hist[: 2 * geom.num_nodes: 7] = 0.0  # a few nodes at rest
+    nxt, adj = rng.standard_normal(nd), rng.standard_normal(nd)
+    op = gls.NavierOperator(geom, gls.FluidParams(1.2, 0.03), 0.05)
+    for lag in range(nh):
+        out = host(gls.hist_vjp(op, step, nh, lag, amap, dev(hist), dev(nxt), dev(adj)))
+        ref = O.gls_hist_vjp(x, y, conn, geom.h_dofs9.reshape(-1), nd, 1.2, 0.03, 0.05, step, nh,
+                             lag, hist, nxt, adj)
+        assert np.array_equal(out, ref), lag
+        assert not out[2 * geom.num_nodes:].any()
+
+
+def test_gls_hist_vjp_1024_adjoint_identity():
+    """1024^2: bitwise equal to the oracle and run to run; and the adjoint
+    identity dir^T VJP_lag(adj) = adj^T JVP_lag(dir) against the oracle's
+    dual-number forward mode, an independent restatement (1e-12)."""
+    n = 1024
+    x, y, conn = perturbed(n, 2)
+    geom = gls.GLSGeometry(x, y, conn, DEV)
+    amap = gls.make_navier_map(geom)
+    nd = geom.num_states
+    rng = np.random.default_rng(13)
+    hist = rng.standard_normal(2 * nd)
+    nxt, adj, dirv = (rng.standard_normal(nd) for _ in range(3))
+    op = gls.NavierOperator(geom, gls.FluidParams(1.0, 1e-3), 0.01)
+    h_d, n_d, a_d = dev(hist), dev(nxt), dev(adj)
+    d9 = geom.h_dofs9.reshape(-1)
+    for lag in (0, 1):
+        out = host(gls.hist_vjp(op, 1, 2, lag, amap, h_d, n_d, a_d))
+        assert np.array_equal(out, host(gls.hist_vjp(op, 1, 2, lag, amap, h_d, n_d, a_d)))
+        ref = O.gls_hist_vjp(x, y, conn, d9, nd, 1.0, 1e-3, 0.01, 1, 2, lag, hist, nxt, adj)
+        assert np.array_equal(out, ref)
+        jv = O.gls_hist_jvp(x, y, conn, d9, nd, 1.0, 1e-3, 0.01, 1, 2, lag, hist, nxt, dirv)
+        lhs, rhs = dirv @ out, adj @ jv
+        assert abs(lhs - rhs) <= 1e-12 * max(1.0, np.abs(dirv * out).sum()), (lag, lhs, rhs)
+
+
+@pytest.mark.parametrize("lag", [0, 1])
+def test_gls_hist_vjp_finite_difference(lag):
+    """femx's method (runNavierOperatorTests.cpp:292-382) on the GPU: the
+    reference test's state pattern on a 32^2 mesh, central differences of
+    the GPU residual with eps 1e-6, near(vjp, fd, 2e-6)."""
+    x, y, conn = O.mesh_p1(32)
+    geom = gls.GLSGeometry(2.0 * x, y, conn, DEV)
+    amap = gls.make_navier_map(geom)
+    nd = geom.num_states
+    i = np.arange(nd)
+    hist = np.concatenate([0.01 * (i + 1), -0.005 * (i + 1)]) / nd
+    nxt = (0.02 - 0.003 * i) / nd
+    dirv = 0.2 - 0.01 * i / nd
+    adj = -0.15 + 0.013 * i / nd
+    op = gls.NavierOperator(geom, gls.FluidParams(1.2, 0.03), 0.05)
+    out = host(gls.hist_vjp(op, 1, 2, lag, amap, dev(hist), dev(nxt), dev(adj)))
+    res = torch.empty(nd, dtype=torch.float64, device=DEV)
+    eps, vals = 1e-6, []
+    for sgn in (1.0, -1.0):
+        hp = hist.copy()
+        hp[lag * nd:(lag + 1) * nd] += sgn * eps * dirv
+        gls.assemble_residual(op, 1, 2, amap, dev(hp), dev(nxt), res)
+        vals.append(host(res) @ adj)
+    fd = (vals[0] - vals[1]) / (2 * eps)
+    vjp = dirv @ out
+    assert abs(vjp - fd) <= 2e-6 * max(1.0, abs(vjp), abs(fd)), (vjp, fd)
