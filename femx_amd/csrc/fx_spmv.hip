@@ -121,6 +121,22 @@ __global__ __launch_bounds__(kRowBlock) void k_axpby(int n, double a, const doub
   if (i < n) y[i] = b == 0.0 ? a * x[i] : a * x[i] + b * y[i];
 }
 
+__global__ __launch_bounds__(kRowBlock) void k_gather(int n, const int* __restrict__ idx,
+                                                      const double* __restrict__ src,
+                                                      double* __restrict__ dst)
+{
+  const int i = blockIdx.x * kRowBlock + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+__global__ __launch_bounds__(kRowBlock) void k_scatter(int n, const int* __restrict__ idx,
+                                                       const double* __restrict__ src,
+                                                       double* __restrict__ dst)
+{
+  const int i = blockIdx.x * kRowBlock + threadIdx.x;
+  if (i < n) dst[idx[i]] = src[i];
+}
+
 // Fixed-shape two-stage tree reduction: the grid, the stride pattern and the
 // tree are independent of timing, so the sum is reproducible bit for bit.
 __device__ __forceinline__ double block_sum(double v)
@@ -238,6 +254,36 @@ int fx_vec_axpby(fx_index n, double a, const double* x, double b, double* y, voi
     hipLaunchKernelGGL(k_axpby, dim3(fx::blocks_for(n, kRowBlock)), dim3(kRowBlock), 0,
                        fx::as_stream(stream), n, a, x, b, y);
     fx::check_launch("fx_vec_axpby");
+  });
+}
+
+int fx_vec_gather(fx_index n, const fx_index* idx, const double* src, fx_index src_len,
+                  double* dst, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(n >= 0 && src_len >= 0, FX_ERR_ARG, "fx_vec_gather: negative size");
+    if (n == 0) return;
+    fx::require(idx && src && dst, FX_ERR_ARG, "fx_vec_gather: null array");
+    fx::require(!fx::overlaps(src, std::size_t(src_len) * 8, dst, std::size_t(n) * 8),
+                FX_ERR_ALIAS, "Host gather does not support aliased vectors");
+    hipLaunchKernelGGL(k_gather, dim3(fx::blocks_for(n, kRowBlock)), dim3(kRowBlock), 0,
+                       fx::as_stream(stream), n, idx, src, dst);
+    fx::check_launch("fx_vec_gather");
+  });
+}
+
+int fx_vec_scatter(fx_index n, const fx_index* idx, const double* src, double* dst,
+                   fx_index dst_len, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(n >= 0 && dst_len >= 0, FX_ERR_ARG, "fx_vec_scatter: negative size");
+    if (n == 0) return;
+    fx::require(idx && src && dst, FX_ERR_ARG, "fx_vec_scatter: null array");
+    fx::require(!fx::overlaps(src, std::size_t(n) * 8, dst, std::size_t(dst_len) * 8),
+                FX_ERR_ALIAS, "Host scatter does not support aliased vectors");
+    hipLaunchKernelGGL(k_scatter, dim3(fx::blocks_for(n, kRowBlock)), dim3(kRowBlock), 0,
+                       fx::as_stream(stream), n, idx, src, dst);
+    fx::check_launch("fx_vec_scatter");
   });
 }
 

@@ -211,3 +211,65 @@ def assemble_residual(op: NavierOperator, step, num_hist, amap: AssemblyMap, his
     """assembleResidual (femx/assembly/CudaAssembly.hpp:419-472): the time
     residual only."""
     return assemble(op, step, num_hist, NEXT_STATE, amap, hist, nxt, res, None, stream)
+
+
+class NavierResidual:
+    """The parameter-free time residual of femx's Navier-Stokes model
+    (NavierResidual, femx/model/ns/NavierStokesModel.cpp:447-545) on this
+    mesh: two history lags, res / assemble_next through the GLS kernel,
+    apply_jac_t with respect to a history lag through the history VJP.
+    `time` is a femx_amd.control.TimeContext."""
+
+    NUM_HIST = 2
+
+    def __init__(self, geom: GLSGeometry, fluid: FluidParams, dt: float, num_steps: int,
+                 amap: AssemblyMap = None):
+        _require(num_steps > 0, "NavierStokesModel requires a positive number of time steps")
+        self.geom, self.num_steps = geom, int(num_steps)
+        self.op = NavierOperator(geom, fluid, dt)
+        self.map = amap if amap is not None else make_navier_map(geom)
+
+    def dims(self):
+        ns_ = self.geom.num_states
+        return dict(num_steps=self.num_steps, num_states=ns_, num_param=0, num_res=ns_,
+                    num_hist=self.NUM_HIST)
+
+    @property
+    def pattern(self):
+        return self.map.pattern
+
+    def _check(self, time):
+        _require(0 <= time.step < self.num_steps, "Navier residual step is out of range")
+        ns_ = self.geom.num_states
+        _require(time.num_hist >= self.NUM_HIST and time.hist.numel() == time.num_hist * ns_
+                 and time.nxt.numel() == ns_ and time.prm.numel() == 0,
+                 "Navier residual vector size mismatch")
+
+    def _hist(self, time):
+        return time.hist[: self.NUM_HIST * self.geom.num_states]
+
+    def res(self, time, out, stream=None):
+        self._check(time)
+        return assemble(self.op, time.step, self.NUM_HIST, NEXT_STATE, self.map, self._hist(time),
+                        time.nxt, out, None, stream)
+
+    def assemble_next(self, time, res, jac: CsrMatrix, stream=None):
+        self._check(time)
+        return assemble(self.op, time.step, self.NUM_HIST, NEXT_STATE, self.map, self._hist(time),
+                        time.nxt, res, jac, stream)
+
+    def apply_jac_t(self, time, wrt, adj, out, stream=None):
+        self._check(time)
+        kind, lag = wrt
+        _require(kind != "next",
+                 "Navier transpose apply supports only history and parameter blocks")
+        _require(adj.numel() == self.geom.num_states, "Navier residual adjoint size mismatch")
+        if kind == "param":
+            return out[:0]
+        _require(0 <= lag < self.NUM_HIST, "Navier residual history lag is out of range")
+        return hist_vjp(self.op, time.step, self.NUM_HIST, lag, self.map, self._hist(time),
+                        time.nxt, adj, out, stream)
+
+    def prepare_linear_solve(self, time, jac, rhs, stream=None):
+        """TimeResidual's default: nothing to do (TimeResidual.hpp:229-238)."""
+        return None

@@ -232,6 +232,14 @@ int fx_csr_matvec_t(fx_index rows, fx_index cols, const fx_index* d_t_ptr,
  * VectorHandler.cu:124-317).  axpby: y = a*x + b*y; with b == 0, y is not
  * read. */
 int fx_vec_axpby(fx_index n, double a, const double* d_x, double b, double* d_y, void* stream);
+/* gather: dst[i] = src[idx[i]]; scatter: dst[idx[i]] = src[i], i < n
+ * (VectorHandler::gather/scatter, femx/linalg/handler/VectorHandler.cpp:
+ * 46-80).  Indices must be in range (the host-side builders check them);
+ * scatter indices must be distinct. */
+int fx_vec_gather(fx_index n, const fx_index* d_idx, const double* d_src, fx_index src_len,
+                  double* d_dst, void* stream);
+int fx_vec_scatter(fx_index n, const fx_index* d_idx, const double* d_src, double* d_dst,
+                   fx_index dst_len, void* stream);
 /* Deterministic dot product into d_out[0]; d_work holds fx_dot_work_len()
  * doubles. */
 fx_index fx_dot_work_len(void);

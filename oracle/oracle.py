@@ -83,6 +83,10 @@ def ref():
         _REF.ref_ns_gls.argtypes = ([C.c_int32, _pd, _pd, C.c_int32, _pi] + [C.c_double] * 3
                                     + [C.c_int32, C.c_int32, _pd, _pd, C.c_int32]
                                     + [_pi, _pi, _pi, _pd, _pd])
+        _REF.ref_ns_constrained.argtypes = (
+            [C.c_int, C.c_int32, _pd, _pd, C.c_int32, _pi, C.c_int32] + [C.c_double] * 3
+            + [C.c_int32, _pi, C.c_int32, C.c_int32, _pi, _pi, _pd, C.c_int32, _pi, _pd, _pi, _pi,
+               _pd, C.c_int32, C.c_int32, C.c_int32] + [_pd] * 4 + [_pi] + [_pd] * 6)
     return _REF
 
 
@@ -352,6 +356,40 @@ def gls_hist_jvp(x, y, conn, dofs9, num_states, rho, mu, dt, step, num_hist, lag
     return out
 
 
+def ref_ns_constrained(x, y, conn, nstep, dt, rho, mu, ctr_dofs, num_ctr_params, entries,
+                       fixed_dofs, fixed_vals, time, ctr_off, num_prm, step, hist, nxt, prm, adj):
+    """femx's NavierStokesModel residual under ConstrainedTimeResidual at one
+    step (oracle/_ref).  entries: (row, col, weight) of the DirichletControl
+    map; time: per-step (lower, upper, upper_weight).  Returns a dict."""
+    R = ref()
+    i32 = lambda a: np.ascontiguousarray(a, I32)  # noqa: E731
+    f64 = lambda a: np.ascontiguousarray(a, F64)  # noqa: E731
+    ent = np.asarray(entries, dtype=object).reshape(-1, 3)
+    er, ec, ew = i32(ent[:, 0].astype(int)), i32(ent[:, 1].astype(int)), f64(ent[:, 2].astype(float))
+    tl, tu, tw = (i32([t[0] for t in time]), i32([t[1] for t in time]),
+                  f64([t[2] for t in time]))
+    cd, fd, fv = i32(ctr_dofs), i32(fixed_dofs), f64(fixed_vals)
+    dims = np.zeros(4, I32)
+    z = np.zeros(1)
+    args = [x.size, _p(f64(x)), _p(f64(y)), conn.size // 3, _p(i32(conn)), nstep, dt, rho, mu,
+            cd.size, _p(cd), num_ctr_params, er.size, _p(er), _p(ec), _p(ew), fd.size, _p(fd),
+            _p(fv), _p(tl), _p(tu), _p(tw), ctr_off, num_prm, step]
+    rc = R.ref_ns_constrained(1, *args, _p(z), _p(z), _p(z), _p(z), _p(dims), *([_p(z)] * 6))
+    if rc != 0:
+        raise RuntimeError(R.ref_last_error().decode())
+    ns_, nh, npar, nnz = (int(v) for v in dims)
+    out = {k: np.empty(n, F64) for k, n in (("res", ns_), ("asm_res", ns_), ("asm_vals", nnz),
+                                             ("grad", npar), ("prep_vals", nnz),
+                                             ("prep_rhs", ns_))}
+    rc = R.ref_ns_constrained(0, *args, _p(f64(hist)), _p(f64(nxt)), _p(f64(prm)), _p(f64(adj)),
+                              _p(dims), _p(out["res"]), _p(out["asm_res"]), _p(out["asm_vals"]),
+                              _p(out["grad"]), _p(out["prep_vals"]), _p(out["prep_rhs"]))
+    if rc != 0:
+        raise RuntimeError(R.ref_last_error().decode())
+    out.update(num_states=ns_, num_hist=nh, num_param=npar, nnz=nnz)
+    return out
+
+
 def ref_ns_gls(x, y, conn, rho, mu, dt, step, num_hist, hist, nxt):
     """femx's own GLS time assembly (oracle/_ref); returns (dofs9, row_ptr,
     col_ind, vals, res)."""
@@ -370,6 +408,64 @@ def ref_ns_gls(x, y, conn, rho, mu, dt, step, num_hist, hist, nxt):
     if rc != 0:
         raise RuntimeError(R.ref_last_error().decode())
     return d9, rp, ci[:rp[-1]].copy(), vals[:rp[-1]].copy(), res
+
+
+# --------------------------------------------------------------------------
+# Dirichlet control (femx/fem/DirichletControl.cpp:32-73, ControlMap.cpp:
+# 36-342) and the constrained time residual (femx/assembly/
+# ConstrainedTimeResidual.cpp:318-405), composed from the restated CSR and
+# boundary operations above.
+
+def control_matrix(rows, cols, entries):
+    """makeControlMatrix: CSR with each row's entries in input order."""
+    rp = np.zeros(rows + 1, np.int64)
+    for r, _, _ in entries:
+        rp[int(r) + 1] += 1
+    rp = np.cumsum(rp)
+    nxt = rp[:-1].copy()
+    ci = np.empty(len(entries), I32)
+    v = np.empty(len(entries), F64)
+    for r, c, w in entries:
+        k = nxt[int(r)]
+        nxt[int(r)] += 1
+        ci[k], v[k] = int(c), w
+    return rp.astype(I32), ci, v
+
+
+def control_vals(cm, step, prm):
+    """controlVals (ControlMap.cpp:165-199)."""
+    rp, ci, v = cm["P"]
+    lo, hi, w = cm["time"][step]
+    lo, hi = int(lo), int(hi)
+    b = cm["num_ctr_params"]
+    off = cm["ctr_off"]
+    out = np.empty(cm["ctr_dofs"].size + cm["fixed_dofs"].size)
+    ctl = np.empty(cm["ctr_dofs"].size)
+    matvec(rp, ci, v, np.ascontiguousarray(prm[off + lo * b: off + lo * b + b]), ctl, 1.0 - w, 0.0)
+    if hi != lo and w != 0.0:
+        matvec(rp, ci, v, np.ascontiguousarray(prm[off + hi * b: off + hi * b + b]), ctl, w, 1.0)
+    nf = cm["fixed_dofs"].size
+    out[:ctl.size] = ctl
+    out[ctl.size:] = cm["fixed_vals"][step * nf:(step + 1) * nf]
+    return out
+
+
+def add_control_jac_t(cm, step, adj, grad):
+    """addControlJacT (ControlMap.cpp:309-342): grad += -P^T (adj on the
+    controlled DOFs) at the step's levels."""
+    rp, ci, v = cm["P"]
+    lo, hi, w = cm["time"][step]
+    lo, hi = int(lo), int(hi)
+    b = cm["num_ctr_params"]
+    off = cm["ctr_off"]
+    compact = np.ascontiguousarray(adj[cm["ctr_dofs"]])
+    for lev, wt in ((lo, 1.0 - w), (hi, w)):
+        if lev == hi and (hi == lo or w == 0.0):
+            continue
+        g = np.ascontiguousarray(grad[off + lev * b: off + lev * b + b])
+        matvecT(rp, ci, v, b, compact, g, -wt, 1.0)
+        grad[off + lev * b: off + lev * b + b] = g
+    return grad
 
 
 def omp_threads():

@@ -29,6 +29,10 @@
 #include <femx/linalg/handler/MatrixHandler.hpp>
 #include <femx/fem/MixedFESpace.hpp>
 #include <femx/model/ns/Components.hpp>
+#include <femx/model/ns/NavierStokesModel.hpp>
+#include <femx/assembly/ConstrainedTimeResidual.hpp>
+#include <femx/fem/ControlMap.hpp>
+#include <femx/fem/DirichletControl.hpp>
 #include <femx/state/TimeResidual.hpp>
 
 #include "PoissonComponents.hpp"
@@ -303,6 +307,90 @@ int ref_ns_gls(Index num_nodes, const Real* x, const Real* y, Index num_elems, c
     std::memcpy(col_ind, p.colIndData(), sizeof(Index) * p.nnz());
     std::memcpy(vals, jac.valsData(), sizeof(Real) * p.nnz());
     for (Index i = 0; i < ns; ++i) res[i] = r[i];
+  });
+}
+
+
+// femx's Navier-Stokes model (femx/model/ns/NavierStokesModel.cpp) on a
+// triangle mesh, decorated with Dirichlet constraints by
+// assembly::ConstrainedTimeResidual (femx/assembly/ConstrainedTimeResidual.cpp)
+// over a ControlMap (femx/fem/ControlMap.cpp:36-137) built from a
+// DirichletControl (dofs, P entries), fixed DOFs with per-step values and a
+// per-step linear time interpolation of the control levels.  At one step it
+// evaluates res, assembleNext, applyJacT with respect to the parameters, and
+// the pre-solve sequence of TimeIntegrator::solveStep
+// (femx/state/TimeIntegrator.hpp:272-289): rhs = J nxt - res, then
+// prepareLinearSolve.  Writes the dims (num_states, num_hist, num_param,
+// nnz) first; arrays are sized by the caller from a first call with
+// query = 1.
+int ref_ns_constrained(int query, Index num_nodes, const Real* x, const Real* y, Index num_elems,
+                       const Index* conn, Index nstep, Real dt, Real rho, Real mu,
+                       Index num_ctr_dofs, const Index* ctr_dofs, Index num_ctr_params,
+                       Index num_entries, const Index* ent_row, const Index* ent_col,
+                       const Real* ent_wt, Index num_fixed, const Index* fixed_dofs,
+                       const Real* fixed_vals, const Index* t_lower, const Index* t_upper,
+                       const Real* t_upper_wt, Index ctr_off, Index num_prm, Index step,
+                       const Real* hist, const Real* nxt, const Real* prm, const Real* adj,
+                       Index* dims, Real* res, Real* asm_res, Real* asm_vals, Real* grad,
+                       Real* prep_vals, Real* prep_rhs)
+{
+  return guarded([&] {
+    femx::fem::Mesh mesh(2);
+    for (Index i = 0; i < num_nodes; ++i) mesh.addNode({x[i], y[i], 0.0});
+    for (Index e = 0; e < num_elems; ++e)
+      mesh.addElem({conn[3 * e], conn[3 * e + 1], conn[3 * e + 2]},
+                   femx::fem::Element::Shape::Triangle, 2, 0, 0, {});
+    femx::model::ns::FluidParams fluid;
+    fluid.rho = rho;
+    fluid.mu  = mu;
+    femx::model::ns::NavierStokesModel model(std::move(mesh), nstep, dt, fluid);
+    Array<Index> cd(num_ctr_dofs);
+    for (Index k = 0; k < num_ctr_dofs; ++k) cd[k] = ctr_dofs[k];
+    Array<femx::fem::DirichletControlMapEntry> ents;
+    for (Index k = 0; k < num_entries; ++k) ents.push_back({ent_row[k], ent_col[k], ent_wt[k]});
+    femx::fem::DirichletControl ctr(cd, num_ctr_params, ents);
+    Array<Index>                fd(num_fixed);
+    for (Index k = 0; k < num_fixed; ++k) fd[k] = fixed_dofs[k];
+    femx::HostVector            fv(nstep * num_fixed);
+    for (Index k = 0; k < nstep * num_fixed; ++k) fv[k] = fixed_vals[k];
+    Array<femx::LinearInterpolation> tm(nstep);
+    for (Index s = 0; s < nstep; ++s) tm[s] = {t_lower[s], t_upper[s], t_upper_wt[s]};
+    auto control = femx::fem::makeControlMap(nstep, model.numStates(), ctr, fd, std::move(fv),
+                                             tm, ctr_off, num_prm);
+    femx::assembly::HostConstrainedTimeResidual cres(model.residual(), std::move(control));
+    const auto dm = cres.dims();
+    dims[0]       = dm.num_states;
+    dims[1]       = dm.num_hist;
+    dims[2]       = dm.num_param;
+    dims[3]       = model.map().pattern().nnz();
+    if (query) return;
+    const Index ns = dm.num_states;
+    femx::HostVector h(dm.num_hist * ns), nx(ns), pr(dm.num_param), aj(ns);
+    for (Index i = 0; i < dm.num_hist * ns; ++i) h[i] = hist[i];
+    for (Index i = 0; i < ns; ++i) nx[i] = nxt[i], aj[i] = adj[i];
+    for (Index i = 0; i < dm.num_param; ++i) pr[i] = prm[i];
+    const femx::state::HostTimeContext time{
+        step, nx.view(), pr.view(), {h.data(), dm.num_hist, ns}};
+    femx::CpuContext ctx;
+    femx::HostVector out;
+    cres.res(time, out, ctx);
+    for (Index i = 0; i < ns; ++i) res[i] = out[i];
+    femx::HostCsrMatrix jac(model.map().pattern());
+    femx::HostVector    r2;
+    cres.assembleNext(time, r2, jac, ctx);
+    for (Index i = 0; i < ns; ++i) asm_res[i] = r2[i];
+    std::memcpy(asm_vals, jac.valsData(), sizeof(Real) * dims[3]);
+    femx::HostVector g;
+    cres.applyJacT(time, femx::state::VariableBlock::Param, aj.view(), g, ctx);
+    for (Index i = 0; i < dm.num_param; ++i) grad[i] = g[i];
+    femx::HostVector rhs(ns);
+    femx::linalg::HostMatrixHandler mh(ctx);
+    femx::linalg::HostVectorHandler vh(ctx);
+    mh.matvec(jac, nx.view(), rhs);
+    vh.axpby(-1.0, r2.view(), 1.0, rhs.view());
+    cres.prepareLinearSolve(time, jac, rhs, ctx);
+    std::memcpy(prep_vals, jac.valsData(), sizeof(Real) * dims[3]);
+    for (Index i = 0; i < ns; ++i) prep_rhs[i] = rhs[i];
   });
 }
 

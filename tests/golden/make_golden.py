@@ -15,6 +15,9 @@ conventions, seeded values) and the reference's outputs:
   gls_tri1.npz, gls_p1_n4_perturbed.npz, gls_p1_n6.npz
                           NavierOperator (GLS P1/P1) time assemble: CSR
                           values and residual per (step, num_hist, state)
+  gls_constrained_n4.npz  NavierStokesModel residual under ConstrainedTimeResidual:
+                          res, assembleNext, applyJacT(param), and
+                          TimeIntegrator::solveStep's rhs/prepareLinearSolve
 
     python tests/golden/make_golden.py
 """
@@ -192,6 +195,45 @@ def main():
     nd = 3 * x.size
     gls_case(x, y, conn, 1.0, 1.0e-3, 0.01,
              [(1, 2, rng.standard_normal(2 * nd), rng.standard_normal(nd))], "gls_p1_n6.npz")
+
+    # femx's NavierStokesModel residual under ConstrainedTimeResidual: an
+    # inflow control with two parameters per level (profile weights y(1-y) and
+    # y, each row's entries given column 1 first), walls and a pressure gauge
+    # fixed with per-step values, four steps interpolating three control
+    # levels, and one leading parameter outside the control (ctr_off = 1).
+    n = 4
+    x, y, conn = O.mesh_p1(n)
+    inner = (x > 0) & (x < 1) & (y > 0) & (y < 1)
+    x[inner] += 0.15 / n * rng.uniform(-1, 1, inner.sum())
+    y[inner] += 0.15 / n * rng.uniform(-1, 1, inner.sum())
+    N = x.size
+    left = [i for i in range(N) if x[i] == 0.0 and 0.0 < y[i] < 1.0]
+    ctr_dofs = np.array([2 * i for i in left], I32)
+    ents = []
+    for r, i in enumerate(left):
+        ents += [(r, 1, y[i]), (r, 0, y[i] * (1.0 - y[i]))]
+    ent = np.array(ents, dtype=np.float64)
+    wall = [i for i in range(N) if (x[i] in (0.0, 1.0) or y[i] in (0.0, 1.0)) and i not in left]
+    fixed = np.array(sorted([2 * i + c for i in wall for c in (0, 1)] + [2 * i + 1 for i in left]
+                            + [2 * N]), I32)
+    nstep = 4
+    fvals = rng.standard_normal(nstep * fixed.size)
+    time = np.array([(s // 2, s // 2 + 1, 0.5 * (s % 2)) for s in range(nstep)], np.float64)
+    ns_ = 3 * N
+    out = dict(x=x, y=y, conn=conn, nstep=nstep, dt=0.05, rho=1.2, mu=0.03, ctr_dofs=ctr_dofs,
+               num_ctr_params=2, entries=ent, fixed_dofs=fixed, fixed_vals=fvals, time=time,
+               ctr_off=1, num_prm=8, num_cases=2)
+    for k, step in enumerate((1, 2)):
+        hist, nxt = rng.standard_normal(2 * ns_), rng.standard_normal(ns_)
+        prm, adj = rng.standard_normal(8), rng.standard_normal(ns_)
+        r = O.ref_ns_constrained(x, y, conn, nstep, 0.05, 1.2, 0.03, ctr_dofs, 2,
+                                 [(int(a), int(b), w) for a, b, w in ents], fixed, fvals,
+                                 [tuple(t) for t in time], 1, 8, step, hist, nxt, prm, adj)
+        out.update({f"step{k}": step, f"hist{k}": hist, f"nxt{k}": nxt, f"prm{k}": prm,
+                    f"adj{k}": adj})
+        for key in ("res", "asm_res", "asm_vals", "grad", "prep_vals", "prep_rhs"):
+            out[f"{key}{k}"] = r[key]
+    np.savez_compressed(os.path.join(HERE, "gls_constrained_n4.npz"), **out)
 
     print("fixtures written to", HERE)
 

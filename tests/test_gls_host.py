@@ -407,3 +407,74 @@ def test_hist_vjp_operator_adjoint_identity():
         jc = O.gls_hist_jvp(x, y, conn, d9.reshape(-1), ns_, 1.0, 0.01, 0.02, 2, 2, lag, hist, nxt,
                             dirv)
         assert np.allclose(jc, jv, rtol=1e-14, atol=1e-14 * np.abs(jv).max())
+
+
+# ---------------------------------------------------------------------------
+# constrained time residual (femx's NavierStokesModel under
+# ConstrainedTimeResidual; fixture from the compiled reference)
+
+def constrained_spec(g):
+    ent = [(int(r), int(c), float(w)) for r, c, w in g["entries"]]
+    nc = g["ctr_dofs"].size
+    return dict(P=O.control_matrix(nc, int(g["num_ctr_params"]), ent), ctr_dofs=g["ctr_dofs"],
+                fixed_dofs=g["fixed_dofs"], fixed_vals=g["fixed_vals"], time=g["time"],
+                num_ctr_params=int(g["num_ctr_params"]), ctr_off=int(g["ctr_off"]))
+
+
+def test_constrained_residual_oracle_equals_reference_bitwise():
+    """The oracle's composition -- GLS assembly, controlVals, replaceRes,
+    replaceRows, addControlJacT, J nxt - res, prepareForwardSolve -- against
+    femx's ConstrainedTimeResidual and TimeIntegrator::solveStep sequence."""
+    g = golden("gls_constrained_n4.npz")
+    x, y, conn = g["x"], g["y"], g["conn"]
+    N = x.size
+    ns_ = 3 * N
+    d9 = O.gls_dofs(N, conn)
+    off = O.uniform_offsets(conn.size // 3, 9)
+    rp, ci, _, jm = O.map_build(ns_, ns_, off, d9, off, d9)
+    cm = constrained_spec(g)
+    dofs = np.concatenate([g["ctr_dofs"], g["fixed_dofs"]]).astype(np.int32)
+    bc = O.bc_build(rp, ci, dofs)
+    args = (x, y, conn, d9, ns_, float(g["rho"]), float(g["mu"]), float(g["dt"]))
+    for k in range(int(g["num_cases"])):
+        step = int(g[f"step{k}"])
+        hist, nxt, prm, adj = (g[f"{a}{k}"] for a in ("hist", "nxt", "prm", "adj"))
+        vals, res = O.gls_assemble(*args, step, 2, hist, nxt, jm, ci.size)
+        bv = O.control_vals(cm, step, prm)
+        O.replace_res(bc, nxt, bv, res)
+        assert np.array_equal(res, g[f"res{k}"]) and np.array_equal(res, g[f"asm_res{k}"])
+        O.replace_rows(rp, bc, vals, 1.0)
+        assert np.array_equal(vals, g[f"asm_vals{k}"])
+        grad = O.add_control_jac_t(cm, step, adj, np.zeros(int(g["num_prm"])))
+        assert np.array_equal(grad, g[f"grad{k}"])
+        rhs = np.empty(ns_)
+        O.matvec(rp, ci, vals, nxt, rhs)
+        rhs = -1.0 * res + 1.0 * rhs
+        O.prepare_forward(rp, bc, vals, rhs, bv)
+        assert np.array_equal(vals, g[f"prep_vals{k}"]) and np.array_equal(rhs, g[f"prep_rhs{k}"])
+
+
+def test_control_map_builders_follow_reference():
+    from femx_amd import control as Ct
+    with pytest.raises(RuntimeError, match="duplicate map entry"):
+        Ct.DirichletControl([1, 2], 2, [(0, 1, 1.0), (0, 1, 2.0)])
+    with pytest.raises(RuntimeError, match="duplicate state id"):
+        Ct.DirichletControl([1, 1])
+    with pytest.raises(RuntimeError, match="control column is out of range"):
+        Ct.DirichletControl([1], 1, [(0, 3, 1.0)])
+    ctr = Ct.DirichletControl([4, 2], 2, [(1, 0, 0.5), (0, 1, 2.0), (0, 0, 3.0)])
+    assert ctr.pattern.row_ptr.tolist() == [0, 2, 3]
+    assert ctr.pattern.col_ind.tolist() == [1, 0, 0] and ctr.vals.tolist() == [2.0, 3.0, 0.5]
+    kw = dict(device="cpu")
+    with pytest.raises(RuntimeError, match="controlled DOF is invalid"):
+        Ct.ControlMap(3, 4, Ct.DirichletControl([4]), **kw)
+    with pytest.raises(RuntimeError, match="fixed DOF is invalid"):
+        Ct.ControlMap(3, 6, Ct.DirichletControl([4]), fixed_dofs=[4], **kw)
+    with pytest.raises(RuntimeError, match="time stencil is invalid"):
+        Ct.ControlMap(2, 6, ctr, time=[(0, 0, 0.5), (0, 1, 0.5)], **kw)
+    with pytest.raises(RuntimeError, match="parameter count is too small"):
+        Ct.ControlMap(2, 6, ctr, time=[(0, 1, 0.5), (1, 1, 0.0)], num_prm=3, **kw)
+    cm = Ct.ControlMap(2, 6, ctr, fixed_dofs=[0, 5], fixed_vals=[1.0, 2.0],
+                       time=[(0, 1, 0.5), (1, 1, 0.0)], ctr_off=1, **kw)
+    assert cm.num_prm == 1 + 2 * 2 and cm.dofs.tolist() == [4, 2, 0, 5]
+    assert cm.fixed_vals.tolist() == [1.0, 2.0, 1.0, 2.0]

@@ -1,0 +1,119 @@
+"""The constrained Navier-Stokes time residual on the GPU against femx's own
+NavierStokesModel under ConstrainedTimeResidual (fixture from the compiled
+reference): res, assembleNext, the parameter transpose and the linear system
+of TimeIntegrator::solveStep, bitwise; the history transpose against the
+oracle; the control Jacobian and its transpose against each other."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import golden
+from femx_amd import control as Ct
+from femx_amd import gls
+from femx_amd.linalg import CsrMatrix
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).to(DEV)
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def build(g):
+    geom = gls.GLSGeometry(g["x"], g["y"], g["conn"], DEV)
+    base = gls.NavierResidual(geom, gls.FluidParams(float(g["rho"]), float(g["mu"])),
+                              float(g["dt"]), int(g["nstep"]))
+    ents = [(int(r), int(c), float(w)) for r, c, w in g["entries"]]
+    ctr = Ct.DirichletControl(g["ctr_dofs"], int(g["num_ctr_params"]), ents)
+    cm = Ct.ControlMap(int(g["nstep"]), geom.num_states, ctr, g["fixed_dofs"], g["fixed_vals"],
+                       [tuple(t) for t in g["time"]], int(g["ctr_off"]), int(g["num_prm"]), DEV)
+    return geom, base, cm, Ct.ConstrainedTimeResidual(base, cm)
+
+
+def test_constrained_residual_equals_reference_bitwise():
+    g = golden("gls_constrained_n4.npz")
+    geom, base, cm, R = build(g)
+    ns_ = geom.num_states
+    assert R.dims()["num_param"] == int(g["num_prm"]) and R.dims()["num_hist"] == 2
+    for k in range(int(g["num_cases"])):
+        print("case", k, flush=True)
+        time = Ct.TimeContext(int(g[f"step{k}"]), dev(g[f"nxt{k}"]), dev(g[f"prm{k}"]),
+                              dev(g[f"hist{k}"]), 2)
+        res = torch.full((ns_,), np.nan, dtype=torch.float64, device=DEV)
+        R.res(time, res)
+        assert np.array_equal(host(res), g[f"res{k}"])
+        jac = CsrMatrix(base.pattern, DEV)
+        jac.vals.fill_(np.nan)
+        R.assemble_next(time, res, jac)
+        assert np.array_equal(host(res), g[f"asm_res{k}"])
+        assert np.array_equal(host(jac.vals), g[f"asm_vals{k}"])
+        grad = torch.full((int(g["num_prm"]),), np.nan, dtype=torch.float64, device=DEV)
+        R.apply_jac_t(time, ("param", 0), dev(g[f"adj{k}"]), grad)
+        assert np.array_equal(host(grad), g[f"grad{k}"])
+        rhs = torch.empty(ns_, dtype=torch.float64, device=DEV)
+        Ct.step_system(R, time, res, jac, rhs)
+        assert np.array_equal(host(jac.vals), g[f"prep_vals{k}"])
+        assert np.array_equal(host(rhs), g[f"prep_rhs{k}"])
+
+
+def test_constrained_history_transpose_masks_the_adjoint():
+    """applyJacT with respect to a history lag: the adjoint is zeroed on the
+    constrained rows, then the history VJP (ConstrainedTimeResidual.cpp:
+    379-405); against the oracle, bitwise."""
+    g = golden("gls_constrained_n4.npz")
+    geom, base, cm, R = build(g)
+    ns_ = geom.num_states
+    time = Ct.TimeContext(1, dev(g["nxt0"]), dev(g["prm0"]), dev(g["hist0"]), 2)
+    adj = g["adj0"].copy()
+    masked = adj.copy()
+    masked[cm.dofs] = 0.0
+    for lag in (0, 1):
+        out = torch.empty(ns_, dtype=torch.float64, device=DEV)
+        R.apply_jac_t(time, ("history", lag), dev(adj), out)
+        ref = O.gls_hist_vjp(g["x"], g["y"], g["conn"], geom.h_dofs9.reshape(-1), ns_,
+                             float(g["rho"]), float(g["mu"]), float(g["dt"]), 1, 2, lag,
+                             g["hist0"], g["nxt0"], masked)
+        assert np.array_equal(host(out), ref)
+    with pytest.raises(RuntimeError, match="only history and parameter blocks"):
+        R.apply_jac_t(time, ("next", 0), dev(adj), out)
+
+
+def test_control_jacobian_and_transpose_agree():
+    """dir^T (dres/dq)^T adj = adj^T (dres/dq) dir for controlJac and
+    addControlJacT, at a step with two interpolated levels; the control
+    Jacobian is zero off the controlled DOFs."""
+    g = golden("gls_constrained_n4.npz")
+    geom, base, cm, R = build(g)
+    rng = np.random.default_rng(3)
+    adj = rng.standard_normal(geom.num_states)
+    dirv = rng.standard_normal(cm.num_prm)
+    jd = host(Ct.control_jac(cm, 1, dev(dirv), torch.empty(geom.num_states, dtype=torch.float64,
+                                                            device=DEV)))
+    gt = host(Ct.add_control_jac_t(cm, 1, dev(adj), torch.zeros(cm.num_prm, dtype=torch.float64,
+                                                                 device=DEV)))
+    assert abs(adj @ jd - dirv @ gt) <= 1e-14 * max(1.0, np.abs(adj * jd).sum())
+    off = np.ones(geom.num_states, bool)
+    off[cm.dofs[:cm.num_controlled]] = False
+    assert not jd[off].any()
+
+
+def test_gather_scatter():
+    from femx_amd import _lib
+    from femx_amd.linalg import ptr
+    src = dev(np.arange(10.0))
+    idx = torch.tensor([7, 0, 3], dtype=torch.int32, device=DEV)
+    dst = torch.full((3,), -1.0, dtype=torch.float64, device=DEV)
+    _lib.call("fx_vec_gather", 3, ptr(idx), ptr(src), 10, ptr(dst), None)
+    assert host(dst).tolist() == [7.0, 0.0, 3.0]
+    out = torch.zeros(10, dtype=torch.float64, device=DEV)
+    _lib.call("fx_vec_scatter", 3, ptr(idx), ptr(dst), ptr(out), 10, None)
+    assert host(out).tolist() == [0.0, 0, 0, 3.0, 0, 0, 0, 7.0, 0, 0]
+    with pytest.raises(RuntimeError, match="aliased"):
+        _lib.call("fx_vec_gather", 3, ptr(idx), ptr(src), 10, ptr(src), None)
