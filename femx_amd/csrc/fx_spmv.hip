@@ -129,6 +129,13 @@ __global__ __launch_bounds__(kRowBlock) void k_gather(int n, const int* __restri
   if (i < n) dst[i] = src[idx[i]];
 }
 
+__global__ __launch_bounds__(kRowBlock) void k_div(int n, const double* x, const double* d,
+                                                   double* z)
+{
+  const int i = blockIdx.x * kRowBlock + threadIdx.x;
+  if (i < n) z[i] = x[i] / d[i];
+}
+
 __global__ __launch_bounds__(kRowBlock) void k_scatter(int n, const int* __restrict__ idx,
                                                        const double* __restrict__ src,
                                                        double* __restrict__ dst)
@@ -284,6 +291,18 @@ int fx_vec_scatter(fx_index n, const fx_index* idx, const double* src, double* d
     hipLaunchKernelGGL(k_scatter, dim3(fx::blocks_for(n, kRowBlock)), dim3(kRowBlock), 0,
                        fx::as_stream(stream), n, idx, src, dst);
     fx::check_launch("fx_vec_scatter");
+  });
+}
+
+int fx_vec_div(fx_index n, const double* x, const double* d, double* z, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(n >= 0, FX_ERR_ARG, "fx_vec_div: negative size");
+    if (n == 0) return;
+    fx::require(x && d && z, FX_ERR_ARG, "fx_vec_div: null array");
+    hipLaunchKernelGGL(k_div, dim3(fx::blocks_for(n, kRowBlock)), dim3(kRowBlock), 0,
+                       fx::as_stream(stream), n, x, d, z);
+    fx::check_launch("fx_vec_div");
   });
 }
 

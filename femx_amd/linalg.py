@@ -88,19 +88,32 @@ class CsrPattern:
             self._off16[key] = torch.from_numpy(off[: self.nnz]).to(dev) if rc == 0 else None
         return self._off16[key]
 
-    def transpose_map(self, device="cuda"):
-        """(t_ptr, t_row, t_perm) device tensors, built once per device."""
-        dev = torch.device(device)
-        key = str(dev)
-        if key not in self._tmap:
+    def _host_transpose(self):
+        if "host" not in self._tmap:
             t_ptr = np.empty(self.cols + 1, np.int32)
             t_row = np.empty(max(self.nnz, 1), np.int32)
             t_perm = np.empty(max(self.nnz, 1), np.int32)
             _lib.call("fx_csr_transpose_map", self.rows, self.cols, ptr(self.row_ptr),
                       ptr(self.col_ind), ptr(t_ptr), ptr(t_row), ptr(t_perm))
-            self._tmap[key] = tuple(torch.from_numpy(a).to(dev)
-                                    for a in (t_ptr, t_row[: self.nnz], t_perm[: self.nnz]))
+            self._tmap["host"] = (t_ptr, t_row[: self.nnz], t_perm[: self.nnz])
+        return self._tmap["host"]
+
+    def transpose_map(self, device="cuda"):
+        """(t_ptr, t_row, t_perm) device tensors, built once per device."""
+        dev = torch.device(device)
+        key = str(dev)
+        if key not in self._tmap:
+            self._tmap[key] = tuple(torch.from_numpy(a).to(dev) for a in self._host_transpose())
         return self._tmap[key]
+
+    def transposed(self):
+        """The transpose's pattern (its own layout), built once: rows are this
+        pattern's columns, each listing its source rows in ascending order, as
+        MatrixHandler::transpose lays it out (MatrixHandler.cpp:37-70)."""
+        if "pattern" not in self._tmap:
+            t_ptr, t_row, _ = self._host_transpose()
+            self._tmap["pattern"] = CsrPattern(self.cols, self.rows, t_ptr, t_row)
+        return self._tmap["pattern"]
 
 
 class CsrMatrix:
@@ -166,6 +179,20 @@ class MatrixHandler:
             _lib.call("fx_csr_matvec", mat.rows, mat.cols, ptr(rp), ptr(ci), ptr(mat.vals), ptr(x),
                       ptr(y), float(alpha), float(beta), stream_handle(self.stream))
         return y
+
+    def transpose(self, src: CsrMatrix, dst: CsrMatrix = None):
+        """dst = src^T (MatrixHandler::transpose, MatrixHandler.cu:401-500):
+        the transposed pattern is built once per source layout and cached;
+        values move through the cached permutation (one gather).  Returns dst,
+        allocated on the transposed layout when None or of another layout."""
+        _require(dst is None or dst is not src, "CSR transpose does not support in-place output")
+        tp = src.pattern.transposed()
+        if dst is None or dst.pattern.layout_id != tp.layout_id:
+            dst = CsrMatrix(tp, src.device)
+        _, _, t_perm = src.pattern.transpose_map(src.device)
+        _lib.call("fx_vec_gather", src.nnz, ptr(t_perm), ptr(src.vals), src.nnz, ptr(dst.vals),
+                  stream_handle(self.stream))
+        return dst
 
     def matvecT(self, mat: CsrMatrix, x, y, alpha=1.0, beta=0.0):
         """y = alpha*A^T*x + beta*y (femx MatrixHandler::matvecT)."""

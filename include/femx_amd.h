@@ -240,6 +240,9 @@ int fx_vec_gather(fx_index n, const fx_index* d_idx, const double* d_src, fx_ind
                   double* d_dst, void* stream);
 int fx_vec_scatter(fx_index n, const fx_index* d_idx, const double* d_src, double* d_dst,
                    fx_index dst_len, void* stream);
+/* z = x / d elementwise (the Jacobi step of the device iterative solver;
+ * no counterpart in femx's VectorHandler).  z may alias x. */
+int fx_vec_div(fx_index n, const double* d_x, const double* d_d, double* d_z, void* stream);
 /* Deterministic dot product into d_out[0]; d_work holds fx_dot_work_len()
  * doubles. */
 fx_index fx_dot_work_len(void);

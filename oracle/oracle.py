@@ -83,6 +83,7 @@ def ref():
         _REF.ref_ns_gls.argtypes = ([C.c_int32, _pd, _pd, C.c_int32, _pi] + [C.c_double] * 3
                                     + [C.c_int32, C.c_int32, _pd, _pd, C.c_int32]
                                     + [_pi, _pi, _pi, _pd, _pd])
+        _REF.ref_transpose.argtypes = [C.c_int32, C.c_int32, _pi, _pi, _pd, _pi, _pi, _pd]
         _REF.ref_ns_constrained.argtypes = (
             [C.c_int, C.c_int32, _pd, _pd, C.c_int32, _pi, C.c_int32] + [C.c_double] * 3
             + [C.c_int32, _pi, C.c_int32, C.c_int32, _pi, _pi, _pd, C.c_int32, _pi, _pd, _pi, _pi,

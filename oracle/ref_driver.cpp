@@ -214,6 +214,26 @@ int ref_spmv(Index transpose, Index rows, Index cols, const Index* row_ptr, cons
   });
 }
 
+// Explicit transpose through femx's host MatrixHandler::transpose
+// (femx/linalg/handler/MatrixHandler.cpp:37-70 counting sort, values through
+// the cached source-to-transpose map).  Writes t_row_ptr[cols+1],
+// t_col_ind[nnz], t_vals[nnz].
+int ref_transpose(Index rows, Index cols, const Index* row_ptr, const Index* col_ind,
+                  const Real* vals, Index* t_row_ptr, Index* t_col_ind, Real* t_vals)
+{
+  return guarded([&] {
+    auto                pat = makePattern(rows, cols, row_ptr, col_ind);
+    femx::HostCsrMatrix mat(pat), out;
+    for (Index k = 0; k < pat.nnz(); ++k) mat.valsData()[k] = vals[k];
+    femx::CpuContext                ctx;
+    femx::linalg::HostMatrixHandler mh(ctx);
+    mh.transpose(mat, out);
+    std::memcpy(t_row_ptr, out.pattern().rowPtrData(), sizeof(Index) * (cols + 1));
+    std::memcpy(t_col_ind, out.pattern().colIndData(), sizeof(Index) * pat.nnz());
+    std::memcpy(t_vals, out.valsData(), sizeof(Real) * pat.nnz());
+  });
+}
+
 // Poisson (Laplace) assembly on a P1 triangle mesh through the reference's own
 // pipeline: Mesh/FESpace/makeGeometry (femx/fem), makeElementQuadratureData
 // with the order-2 triangle rule, makeAssemblyMap(DofLayout), and

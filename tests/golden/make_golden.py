@@ -15,6 +15,8 @@ conventions, seeded values) and the reference's outputs:
   gls_tri1.npz, gls_p1_n4_perturbed.npz, gls_p1_n6.npz
                           NavierOperator (GLS P1/P1) time assemble: CSR
                           values and residual per (step, num_hist, state)
+  transpose.npz           MatrixHandler::transpose on a rectangular unsorted
+                          matrix and the P1 n=3 pattern
   gls_constrained_n4.npz  NavierStokesModel residual under ConstrainedTimeResidual:
                           res, assembleNext, applyJacT(param), and
                           TimeIntegrator::solveStep's rhs/prepareLinearSolve
@@ -234,6 +236,28 @@ def main():
         for key in ("res", "asm_res", "asm_vals", "grad", "prep_vals", "prep_rhs"):
             out[f"{key}{k}"] = r[key]
     np.savez_compressed(os.path.join(HERE, "gls_constrained_n4.npz"), **out)
+
+    # Explicit CSR transpose (MatrixHandler::transpose): a rectangular matrix
+    # with unsorted columns and an empty row, and the P1 n=3 pattern.
+    rp = np.array([0, 3, 3, 5, 9], I32)
+    ci = np.array([4, 0, 2, 1, 4, 3, 0, 4, 1], I32)
+    cases = {"rect": (4, 5, rp, ci)}
+    x, y, conn = O.mesh_p1(3)
+    N = x.size
+    off = O.uniform_offsets(conn.size // 3, 3)
+    prp, pci, _, _ = ref_map(R, N, N, off, conn)
+    cases["p1_n3"] = (N, N, prp, pci)
+    tout = {}
+    for name, (nr, nc, rp, ci) in cases.items():
+        v = rng.standard_normal(ci.size)
+        trp = np.empty(nc + 1, I32)
+        tci = np.empty(ci.size, I32)
+        tv = np.empty(ci.size)
+        assert R.ref_transpose(nr, nc, p(rp), p(ci), p(v), p(trp), p(tci), p(tv)) == 0
+        tout.update({f"{name}_shape": np.array([nr, nc]), f"{name}_row_ptr": rp,
+                     f"{name}_col_ind": ci, f"{name}_vals": v, f"{name}_t_row_ptr": trp,
+                     f"{name}_t_col_ind": tci, f"{name}_t_vals": tv})
+    np.savez_compressed(os.path.join(HERE, "transpose.npz"), **tout)
 
     print("fixtures written to", HERE)
 

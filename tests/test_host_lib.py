@@ -214,3 +214,19 @@ def test_mesh_overflow_is_reported():
     with pytest.raises(RuntimeError, match="int32"):
         # sizes are checked before any output is written
         _lib.call("fx_mesh_p1", 40000, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8))
+
+
+@pytest.mark.parametrize("name", ["rect", "p1_n3"])
+def test_transpose_pattern_matches_reference(name):
+    """MatrixHandler::transpose (compiled femx): the transposed pattern from the
+    cached transpose map, bit-exact, and values through its permutation."""
+    g = golden("transpose.npz")
+    nr, nc = (int(v) for v in g[f"{name}_shape"])
+    pat = CsrPattern(nr, nc, g[f"{name}_row_ptr"], g[f"{name}_col_ind"])
+    tp = pat.transposed()
+    assert (tp.rows, tp.cols) == (nc, nr)
+    assert np.array_equal(tp.row_ptr, g[f"{name}_t_row_ptr"])
+    assert np.array_equal(tp.col_ind, g[f"{name}_t_col_ind"])
+    _, _, perm = pat._host_transpose()
+    assert np.array_equal(g[f"{name}_vals"][perm], g[f"{name}_t_vals"])
+    assert pat.transposed() is tp  # cached
