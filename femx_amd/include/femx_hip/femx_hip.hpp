@@ -539,6 +539,173 @@ inline void assemble(const TaylorHoodNS& op, const DeviceTHGeometry& geom, const
                        jac ? jac->valsData() : nullptr, res.data(), ctx.raw()));
 }
 
+namespace detail
+{
+/** The GLS node plan (fx_map_gls_plan) on the device. */
+struct DeviceNavierPlan
+{
+  DeviceIndexVector node_rows, inc_off, inc_elem, inc_meta, blk_runs, node_tile;
+  Index             max_tile = 0;
+};
+} // namespace detail
+
+/** Mesh of femx's Navier-Stokes model: node coordinates, 3 node ids per
+ * element, and the map's element DOF table (9 per element, NavierOperator
+ * local order u0 v0 u1 v1 u2 v2 p0 p1 p2), with the node plan built from the
+ * host map. */
+struct DeviceNavierGeometry
+{
+  Index                    num_nodes = 0, num_elems = 0, num_states = 0;
+  DeviceVector             x, y;
+  DeviceIndexVector        conn3, dofs9;
+  detail::DeviceNavierPlan plan;
+};
+
+inline void makeNavierGeometry(const std::vector<Real>& x, const std::vector<Real>& y,
+                               const std::vector<Index>& conn3, const HostAssemblyMap& map,
+                               DeviceNavierGeometry& out, HipContext& ctx)
+{
+  require(x.size() == y.size() && conn3.size() % 3 == 0,
+          "Navier geometry arrays have inconsistent sizes");
+  const Index N = static_cast<Index>(x.size()), E = static_cast<Index>(conn3.size() / 3);
+  require(map.numElems() == E, "Geometry and AssemblyMap have different element counts");
+  out.num_nodes  = N;
+  out.num_elems  = E;
+  out.num_states = map.numStates();
+  out.x.assign(x, ctx);
+  out.y.assign(y, ctx);
+  out.conn3.assign(conn3, ctx);
+  // the map's element DOF table, read back through the row incidence export
+  // (incidence t is element DOF slot inc_pos[t] of row r)
+  std::vector<Index> d9(9 * static_cast<std::size_t>(E));
+  std::vector<Index> inc_off(map.numRes() + 1), inc_pos(map.info().num_inc);
+  check(fx_map_export_rowinc(map.handle(), inc_off.data(), inc_pos.data(), nullptr));
+  for (Index r = 0; r < map.numRes(); ++r)
+    for (Index t = inc_off[r]; t < inc_off[r + 1]; ++t) d9[inc_pos[t]] = r;
+  out.dofs9.assign(d9, ctx);
+  const std::size_t nb = (N + FX_GLS_BLOCK_NODES - 1) / FX_GLS_BLOCK_NODES;
+  std::vector<Index> nr(3 * std::size_t(N)), io(N + 1), ie(3 * std::size_t(E)),
+      im(24 * std::size_t(E)), br(4 * FX_GLS_MAX_RUNS * nb), nt(3 * std::size_t(N));
+  Index mx = 0;
+  check(fx_map_gls_plan(map.handle(), N, conn3.data(), nr.data(), io.data(), ie.data(), im.data(),
+                        br.data(), nt.data(), &mx));
+  auto& p = out.plan;
+  p.node_rows.assign(nr, ctx);
+  p.inc_off.assign(io, ctx);
+  p.inc_elem.assign(ie, ctx);
+  p.inc_meta.assign(im, ctx);
+  p.blk_runs.assign(br, ctx);
+  p.node_tile.assign(nt, ctx);
+  p.max_tile = mx;
+}
+
+} // namespace assembly
+
+namespace state
+{
+/** Variable block of a time-residual Jacobian (femx/state/TimeResidual.hpp:
+ * 94-165): the next state, one history lag, or the parameters. */
+class VariableBlock
+{
+public:
+  enum class Kind
+  {
+    HistoryState,
+    NextState,
+    Param
+  };
+  constexpr VariableBlock(Kind k = Kind::HistoryState, Index lag = 0) : kind_(k), lag_(lag) {}
+  static constexpr VariableBlock hist(Index lag) { return {Kind::HistoryState, lag}; }
+  constexpr bool  isNextState() const { return kind_ == Kind::NextState; }
+  constexpr bool  isHistoryState() const { return kind_ == Kind::HistoryState; }
+  constexpr bool  isParam() const { return kind_ == Kind::Param; }
+  constexpr Index historyLag() const { return lag_; }
+  static const VariableBlock NextState;
+  static const VariableBlock Param;
+
+private:
+  Kind  kind_;
+  Index lag_;
+};
+inline constexpr VariableBlock VariableBlock::NextState{VariableBlock::Kind::NextState};
+inline constexpr VariableBlock VariableBlock::Param{VariableBlock::Kind::Param};
+} // namespace state
+
+namespace model::ns
+{
+/** femx::model::ns::FluidParams (femx/model/ns/Config.hpp:10-14). */
+struct FluidParams
+{
+  Real rho = 1.0;
+  Real mu  = 1.0;
+};
+
+/** NavierOperator(data, fluid, dt) (femx/model/ns/Components.hpp:400-577):
+ * the GLS P1/P1 row operator of one time step; here it refers to the mesh
+ * (which replaces femx's NavierData: element values are recomputed on the
+ * device) and carries the fluid and the step. */
+struct NavierOperator
+{
+  const assembly::DeviceNavierGeometry* geom = nullptr;
+  FluidParams                           fluid;
+  Real                                  dt = 0.0;
+};
+
+/** applyNavierHistJacT (femx/model/ns/NavierStokesModel.cpp:321-363):
+ * out = (dR/dhist_lag)^T adj, hand-derived where femx uses Enzyme. */
+inline void applyNavierHistJacT(const NavierOperator& op, Index step, Index num_hist, Index lag,
+                                const DeviceVector& hist, const DeviceVector& nxt,
+                                const DeviceVector& adj, DeviceVector& out, HipContext& ctx)
+{
+  const auto& g = *op.geom;
+  require(hist.size() == num_hist * g.num_states && nxt.size() == g.num_states
+              && adj.size() == g.num_states,
+          "Navier history VJP dimensions do not match");
+  if (out.size() != g.num_states) out.resize(g.num_states);
+  check(fx_gls_hist_vjp(g.num_nodes, g.num_states, g.num_elems, g.x.data(), g.y.data(),
+                        g.conn3.data(), g.dofs9.data(), g.plan.node_rows.data(), op.fluid.rho,
+                        op.fluid.mu, op.dt, step, num_hist, lag, hist.data(), nxt.data(),
+                        adj.data(), g.plan.inc_off.data(), g.plan.inc_elem.data(),
+                        g.plan.inc_meta.data(), out.data(), ctx.raw()));
+}
+} // namespace model::ns
+
+namespace assembly
+{
+/** Time assembly of the GLS operator, femx's signature (host
+ * Assembly.hpp:235-297, device CudaAssembly.hpp:366-417): res and, for the
+ * next-state block, the CSR Jacobian (jac may be null: assembleResidual);
+ * with respect to a history block the Jacobian is zero, as evalRow leaves
+ * it.  Outputs replaced. */
+inline void assemble(const model::ns::NavierOperator& op, Index step, Index num_hist,
+                     state::VariableBlock wrt, const DeviceAssemblyMap& map,
+                     const DeviceVector& hist, const DeviceVector& nxt, DeviceVector& res,
+                     DeviceCsrMatrix* jac, HipContext& ctx)
+{
+  const auto& g = *op.geom;
+  require(num_hist > 0 && hist.size() == num_hist * map.info.num_states
+              && nxt.size() == map.info.num_states,
+          "Time assembly state dimensions do not match AssemblyMap");
+  require(!wrt.isParam() && (!wrt.isHistoryState()
+                             || (wrt.historyLag() >= 0 && wrt.historyLag() < num_hist)),
+          "Time assembly variable block is invalid");
+  require(g.num_elems == map.info.num_elems, "Geometry and AssemblyMap have different element counts");
+  if (res.size() != map.info.num_res) res.resize(map.info.num_res);
+  if (jac)
+    require(jac->pattern().layoutId() == map.pattern->layoutId(),
+            "Time assembly matrix must use the AssemblyMap CSR layout");
+  const bool next = wrt.isNextState();
+  check(fx_gls_assemble(g.num_nodes, g.num_states, g.num_elems, map.info.nnz, g.plan.max_tile,
+                        g.x.data(), g.y.data(), g.conn3.data(), g.dofs9.data(),
+                        g.plan.node_rows.data(), op.fluid.rho, op.fluid.mu, op.dt, step, num_hist,
+                        hist.data(), nxt.data(), g.plan.inc_off.data(), g.plan.inc_elem.data(),
+                        g.plan.inc_meta.data(), g.plan.blk_runs.data(), g.plan.node_tile.data(),
+                        jac && next ? jac->valsData() : nullptr, res.data(), ctx.raw()));
+  if (jac && !next && map.info.nnz > 0)
+    checkHip(hipMemsetAsync(jac->valsData(), 0, sizeof(Real) * map.info.nnz, ctx.stream()),
+             "hipMemsetAsync");
+}
+
 } // namespace assembly
 
 namespace linalg

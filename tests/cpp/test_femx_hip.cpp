@@ -263,6 +263,108 @@ int main()
     expect(r1.toHost(ctx) == r2.toHost(ctx) && r1.toHost(ctx) == r3.toHost(ctx), "bitwise equal residuals");
   });
 
+  // tests/unit/model/ns/runNavierOperatorTests.cpp:223-290 (row assembly
+  // invariants) and :292-382 (history VJP against central differences) on
+  // the GLS operator, with the reference tests' state pattern (fillStates)
+  run("navierRowAssemblyAndHistoryVjp", [] {
+    needDevice();
+    HipContext  ctx;
+    const int   nx = 4, ny = 2, N = (nx + 1) * (ny + 1), E = 2 * nx * ny;
+    const Index ns = 3 * N;
+    RV          x(N), y(N);
+    IV          conn;
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i)
+      {
+        x[j * (nx + 1) + i] = 2.0 * i / nx;
+        y[j * (nx + 1) + i] = 1.0 * j / ny;
+      }
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i)
+      {
+        const int a = j * (nx + 1) + i, b = a + 1, c = b + nx + 1, d = a + nx + 1;
+        conn.insert(conn.end(), {a, b, c, a, c, d});
+      }
+    std::vector<IV> dofs(E);
+    for (int e = 0; e < E; ++e)
+    {
+      const int* v = &conn[3 * e];
+      dofs[e]      = {2 * v[0], 2 * v[0] + 1, 2 * v[1], 2 * v[1] + 1, 2 * v[2], 2 * v[2] + 1,
+                      2 * N + v[0], 2 * N + v[1], 2 * N + v[2]};
+    }
+    auto hmap = assembly::makeAssemblyMap(ns, ns, dofs, dofs);
+    assembly::DeviceAssemblyMap map;
+    assembly::copy(hmap, map, ctx);
+    assembly::DeviceNavierGeometry geom;
+    assembly::makeNavierGeometry(x, y, conn, hmap, geom, ctx);
+    model::ns::NavierOperator op{&geom, {1.2, 0.03}, 0.05};
+    RV h(2 * ns), nx_(ns), zero(ns, 0.0);
+    for (Index i = 0; i < ns; ++i)
+    {
+      h[i]      = 0.01 * (i + 1);
+      h[ns + i] = -0.005 * (i + 1);
+      nx_[i]    = 0.02 - 0.003 * i;
+    }
+    DeviceVector    hist(h, ctx), nxt(nx_, ctx), nz(zero, ctx), res(ns), res0(ns), resh(ns);
+    DeviceCsrMatrix jac(map.pattern), jac0(map.pattern), jach(map.pattern);
+    const auto      next = state::VariableBlock::NextState;
+    assembly::assemble(op, 1, 2, next, map, hist, nxt, res, &jac, ctx);
+    assembly::assemble(op, 1, 2, next, map, hist, nz, res0, &jac0, ctx);
+    assembly::assemble(op, 1, 2, state::VariableBlock::hist(1), map, hist, nxt, resh, &jach, ctx);
+    const RV K = jac.vals().toHost(ctx), r = res.toHost(ctx), r0 = res0.toHost(ctx);
+    expect(K == jac0.vals().toHost(ctx), "the Jacobian does not depend on the next state");
+    RV Kx(ns, 0.0);
+    const auto& pat = hmap.pattern();
+    for (Index i = 0; i < ns; ++i)
+      for (Index k = pat.row_ptr[i]; k < pat.row_ptr[i + 1]; ++k) Kx[i] += K[k] * nx_[pat.col_ind[k]];
+    RV delta(ns);
+    for (Index i = 0; i < ns; ++i) delta[i] = Kx[i] + r0[i];
+    expect(near(r, delta, 1e-11), "res(nxt) = K nxt + res(0)");
+    const RV zh = jach.vals().toHost(ctx);
+    expect(resh.toHost(ctx) == r, "history block: same residual");
+    for (Real v : zh) expect(v == 0.0, "history block: zero Jacobian");
+    bool threw = false;
+    try
+    {
+      assembly::assemble(op, 1, 2, state::VariableBlock::Param, map, hist, nxt, res, nullptr, ctx);
+    }
+    catch (const std::runtime_error&)
+    {
+      threw = true;
+    }
+    expect(threw, "the parameter block is rejected");
+    RV dir(ns), adj(ns);
+    for (Index i = 0; i < ns; ++i)
+    {
+      dir[i] = 0.2 - 0.01 * i;
+      adj[i] = -0.15 + 0.013 * i;
+    }
+    DeviceVector a(adj, ctx), jt;
+    for (Index lag = 0; lag < 2; ++lag)
+    {
+      const Real eps = 1.0e-6;
+      Real       fdv[2];
+      for (int sgn = 0; sgn < 2; ++sgn)
+      {
+        RV hp = h;
+        for (Index i = 0; i < ns; ++i) hp[lag * ns + i] += (sgn == 0 ? eps : -eps) * dir[i];
+        DeviceVector hd(hp, ctx), rr(ns);
+        assembly::assemble(op, 1, 2, next, map, hd, nxt, rr, nullptr, ctx);
+        const RV rv = rr.toHost(ctx);
+        Real     d  = 0.0;
+        for (Index i = 0; i < ns; ++i) d += rv[i] * adj[i];
+        fdv[sgn] = d;
+      }
+      const Real fd = (fdv[0] - fdv[1]) / (2.0 * eps);
+      model::ns::applyNavierHistJacT(op, 1, 2, lag, hist, nxt, a, jt, ctx);
+      const RV jv  = jt.toHost(ctx);
+      Real     vjp = 0.0;
+      for (Index i = 0; i < ns; ++i) vjp += dir[i] * jv[i];
+      expect(std::abs(vjp - fd) <= 2.0e-6 * std::max({1.0, std::abs(vjp), std::abs(fd)}),
+             "history VJP against central differences");
+    }
+  });
+
   // the finite-difference procedure of tests/unit/inverse/runInverseTests.cpp:192-234
   // on the Residual contract, and the adjoint identity against the assembled Jacobian
   run("poissonLogKappaResidualGradients", [] {
