@@ -54,6 +54,8 @@ _SIGS = {
                        C.c_int),
     "fx_p1_vjp": ([_i32, _i32] + [_vp] * 11 + [_vp], C.c_int),
     "fx_th_assemble": ([_i32] * 4 + [_vp] * 5 + [_f64] + [_vp] * 9 + [_vp], C.c_int),
+    "fx_csr_matvec_multi": ([_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32, _f64,
+                             _f64, _vp], C.c_int),
     "fx_vec_div": ([_i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_vec_gather": ([_i32, _vp, _vp, _i32, _vp, _vp], C.c_int),
     "fx_vec_scatter": ([_i32, _vp, _vp, _vp, _i32, _vp], C.c_int),

@@ -98,6 +98,98 @@ __global__ __launch_bounds__(kRowBlock) void k_spmv16(int rows, const int* __res
   y[r] = beta == 0.0 ? alpha * v : alpha * v + beta * y[r];
 }
 
+// Several products with one matrix, y_j = alpha A x_j + beta y_j for
+// j < nv (nv <= kMaxVec): the block's CSR range is staged once and each lane
+// sums its row for every vector, each sum in CSR order exactly as k_spmv /
+// k_spmv16 form it, so every y_j equals a single-vector product bit for bit.
+// The matrix streams from HBM once instead of nv times.  off (16-bit column
+// offsets) or col (int32 columns) gives the columns; the other is null.
+constexpr int kMaxVec = 16;
+
+template <int NV>
+__global__ __launch_bounds__(kRowBlock) void k_spmv_multi(
+    int rows, const int* __restrict__ row_ptr, const int* __restrict__ col,
+    const short* __restrict__ off, const double* __restrict__ vals,
+    const double* __restrict__ X, long ldx, double* __restrict__ Y, long ldy, double alpha,
+    double beta)
+{
+  __shared__ double sv[kStage];
+  __shared__ int    sc[kStage];
+  const int  r0 = fx::xcd_block(gridDim.x) * kRowBlock, r1 = min(r0 + kRowBlock, rows);
+  const int  k0 = row_ptr[r0], kn = row_ptr[r1] - k0;
+  const bool staged = kn <= kStage;
+  const int  rr     = r0 + threadIdx.x;
+  if (staged)
+  {
+    // stage the column index of each entry (rows of this block only)
+    for (int i = threadIdx.x; i < kn; i += kRowBlock)
+    {
+      sv[i] = __builtin_nontemporal_load(vals + k0 + i);
+      sc[i] = col ? __builtin_nontemporal_load(col + k0 + i)
+                  : static_cast<int>(__builtin_nontemporal_load(off + k0 + i));
+    }
+    __syncthreads();
+  }
+  if (rr >= rows) return;
+  const int r = rr;
+  double    acc[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) acc[j] = 0.0;
+  const int a = row_ptr[r], b = row_ptr[r + 1];
+  for (int k = a; k < b; ++k)
+  {
+    double v;
+    int    c;
+    if (staged)
+    {
+      v = sv[k - k0];
+      c = sc[k - k0];
+    }
+    else
+    {
+      v = vals[k];
+      c = col ? col[k] : static_cast<int>(off[k]);
+    }
+    if (!col) c += r;
+    double xv[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) xv[j] = X[j * ldx + c];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = acc[j] + v * xv[j];
+  }
+#pragma unroll
+  for (int j = 0; j < NV; ++j)
+  {
+    double* y = Y + j * ldy + r;
+    __builtin_nontemporal_store(beta == 0.0 ? alpha * acc[j] : alpha * acc[j] + beta * *y, y);
+  }
+}
+
+template <int NV>
+void launch_multi(int rows, const int* row_ptr, const int* col, const short* off, const double* vals,
+                  const double* X, long ldx, double* Y, long ldy, double alpha, double beta,
+                  hipStream_t st)
+{
+  hipLaunchKernelGGL(k_spmv_multi<NV>, dim3(fx::blocks_for(rows, kRowBlock)), dim3(kRowBlock), 0,
+                     st, rows, row_ptr, col, off, vals, X, ldx, Y, ldy, alpha, beta);
+}
+
+// Dispatch on the vector count (compile-time, so each entry's loads issue
+// together).
+template <int NV = kMaxVec>
+void launch_multi_n(int n, int rows, const int* row_ptr, const int* col, const short* off,
+                    const double* vals, const double* X, long ldx, double* Y, long ldy,
+                    double alpha, double beta, hipStream_t st)
+{
+  if constexpr (NV >= 1)
+  {
+    if (n == NV)
+      launch_multi<NV>(rows, row_ptr, col, off, vals, X, ldx, Y, ldy, alpha, beta, st);
+    else
+      launch_multi_n<NV - 1>(n, rows, row_ptr, col, off, vals, X, ldx, Y, ldy, alpha, beta, st);
+  }
+}
+
 __global__ __launch_bounds__(kRowBlock) void k_spmv_t(int cols, const int* __restrict__ t_ptr,
                                                       const int* __restrict__ t_row,
                                                       const int* __restrict__ t_perm,
@@ -261,6 +353,32 @@ int fx_vec_axpby(fx_index n, double a, const double* x, double b, double* y, voi
     hipLaunchKernelGGL(k_axpby, dim3(fx::blocks_for(n, kRowBlock)), dim3(kRowBlock), 0,
                        fx::as_stream(stream), n, a, x, b, y);
     fx::check_launch("fx_vec_axpby");
+  });
+}
+
+int fx_csr_matvec_multi(fx_index rows, fx_index cols, const fx_index* row_ptr,
+                        const fx_index* col_ind, const int16_t* col_off, const double* vals,
+                        fx_index nv, const double* X, fx_index ldx, double* Y, fx_index ldy,
+                        double alpha, double beta, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(rows >= 0 && cols >= 0 && nv >= 0, FX_ERR_ARG, "fx_csr_matvec_multi: negative size");
+    if (rows == 0 || nv == 0) return;
+    fx::require(row_ptr && vals && X && Y && ((col_ind != nullptr) != (col_off != nullptr)),
+                FX_ERR_ARG, "fx_csr_matvec_multi: null array (give col_ind or col_off, not both)");
+    fx::require(ldx >= cols && ldy >= rows, FX_ERR_ARG,
+                "fx_csr_matvec_multi: leading dimension smaller than the vector length");
+    const std::size_t xs = std::size_t(ldx) * (nv - 1) + cols, ys = std::size_t(ldy) * (nv - 1) + rows;
+    fx::require(!fx::overlaps(X, xs * 8, Y, ys * 8), FX_ERR_ALIAS,
+                "CSR matvec input and output must not alias");
+    for (fx_index j0 = 0; j0 < nv; j0 += kMaxVec)
+    {
+      const int n = static_cast<int>(std::min<fx_index>(kMaxVec, nv - j0));
+      launch_multi_n(n, rows, row_ptr, col_ind, reinterpret_cast<const short*>(col_off), vals,
+                     X + j0 * ldx, static_cast<long>(ldx), Y + j0 * ldy, static_cast<long>(ldy),
+                     alpha, beta, fx::as_stream(stream));
+      fx::check_launch("fx_csr_matvec_multi");
+    }
   });
 }
 

@@ -180,6 +180,23 @@ class MatrixHandler:
                       ptr(y), float(alpha), float(beta), stream_handle(self.stream))
         return y
 
+    def matvec_multi(self, mat: CsrMatrix, X, Y, alpha=1.0, beta=0.0):
+        """Y[j] = alpha*A*X[j] + beta*Y[j] for each row j of the 2-D tensors
+        X [nv, cols] and Y [nv, rows]: nv products that read the matrix once
+        (per 16 vectors); each equals matvec's result bit for bit."""
+        _require(X.dim() == 2 and Y.dim() == 2 and X.shape[0] == Y.shape[0]
+                 and X.shape[1] == mat.cols and Y.shape[1] == mat.rows
+                 and X.is_contiguous() and Y.is_contiguous() and X.dtype == torch.float64
+                 and Y.dtype == torch.float64 and X.is_cuda and Y.is_cuda,
+                 "CSR multi-vector product needs contiguous CUDA float64 [nv, n] tensors")
+        rp, ci = mat.pattern.device_arrays(mat.device)
+        off = mat.pattern.col_off16(mat.device)
+        _lib.call("fx_csr_matvec_multi", mat.rows, mat.cols, ptr(rp),
+                  None if off is not None else ptr(ci), ptr(off) if off is not None else None,
+                  ptr(mat.vals), X.shape[0], ptr(X), mat.cols, ptr(Y), mat.rows, float(alpha),
+                  float(beta), stream_handle(self.stream))
+        return Y
+
     def transpose(self, src: CsrMatrix, dst: CsrMatrix = None):
         """dst = src^T (MatrixHandler::transpose, MatrixHandler.cu:401-500):
         the transposed pattern is built once per source layout and cached;

@@ -232,6 +232,15 @@ int fx_csr_matvec_t(fx_index rows, fx_index cols, const fx_index* d_t_ptr,
  * VectorHandler.cu:124-317).  axpby: y = a*x + b*y; with b == 0, y is not
  * read. */
 int fx_vec_axpby(fx_index n, double a, const double* d_x, double b, double* d_y, void* stream);
+/* nv products with one matrix: Y[j*ldy + :] = alpha A X[j*ldx + :] + beta
+ * Y[j*ldy + :], j < nv, the columns given as col_ind or as 16-bit offsets
+ * col_off (fx_csr_col_off16); exactly one of them non-null.  Each product
+ * equals fx_csr_matvec's bit for bit; the matrix is read once per 16
+ * vectors.  (cusparseSpMM's role; femx calls matvec per vector.) */
+int fx_csr_matvec_multi(fx_index rows, fx_index cols, const fx_index* d_row_ptr,
+                        const fx_index* d_col_ind, const int16_t* d_col_off, const double* d_vals,
+                        fx_index nv, const double* d_X, fx_index ldx, double* d_Y, fx_index ldy,
+                        double alpha, double beta, void* stream);
 /* gather: dst[i] = src[idx[i]]; scatter: dst[idx[i]] = src[i], i < n
  * (VectorHandler::gather/scatter, femx/linalg/handler/VectorHandler.cpp:
  * 46-80).  Indices must be in range (the host-side builders check them);

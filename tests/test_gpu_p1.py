@@ -203,6 +203,34 @@ def test_spmv_16bit_offsets_match_int32_path(bandwidth):
         assert np.array_equal(host(y1), alpha * ref + beta * host(y0) if beta else alpha * ref)
 
 
+@pytest.mark.parametrize("bandwidth,nv", [(50, 10), (80000, 3), (50, 17)])
+def test_multi_vector_spmv_equals_single_products(bandwidth, nv):
+    """matvec_multi: nv products with one matrix, each bitwise equal to its
+    single-vector matvec (16-bit and int32 column paths, two 16-vector chunks
+    at nv = 17, alpha/beta), rows with more entries than the LDS stage."""
+    rng = np.random.default_rng(12 + nv)
+    rows, cols = 2500, 2500 + bandwidth
+    counts = rng.integers(0, 9, rows)
+    counts[::13] = 0
+    counts[100:110] = 400  # one block over the 2048-entry stage: the direct-read path
+    rp = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    ci = np.concatenate([np.sort(rng.choice(np.arange(max(0, r - max(bandwidth, 800) // 2),
+                                                      min(cols, r + max(bandwidth, 800) // 2)),
+                                            c, replace=False)) for r, c in enumerate(counts)])
+    pat = CsrPattern(rows, cols, rp, ci.astype(np.int32))
+    m = CsrMatrix(pat, DEV, dev(rng.standard_normal(pat.nnz)))
+    X = dev(rng.standard_normal((nv, cols)))
+    Y0 = dev(rng.standard_normal((nv, rows)))
+    mh = MatrixHandler()
+    for alpha, beta in ((1.0, 0.0), (-0.7, 1.3)):
+        Y = Y0.clone()
+        mh.matvec_multi(m, X, Y, alpha, beta)
+        for j in range(nv):
+            yj = Y0[j].clone()
+            mh.matvec(m, X[j], yj, alpha, beta)
+            assert torch.equal(Y[j], yj), (j, alpha)
+
+
 def test_dot_is_deterministic_and_accurate():
     vh = VectorHandler()
     x = dev(RNG.standard_normal(3_000_001))
