@@ -59,7 +59,7 @@ _SIGS = {
     "fx_vec_div": ([_i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_vec_gather": ([_i32, _vp, _vp, _i32, _vp, _vp], C.c_int),
     "fx_vec_scatter": ([_i32, _vp, _vp, _vp, _i32, _vp], C.c_int),
-    "fx_gls_hist_vjp": ([_i32] * 3 + [_vp] * 5 + [_f64] * 3 + [_i32] * 3 + [_vp] * 7 + [_vp],
+    "fx_gls_hist_vjp": ([_i32] * 3 + [_vp] * 5 + [_f64] * 3 + [_i32] * 3 + [_vp] * 6 + [_vp],
                         C.c_int),
     "fx_gls_assemble": ([_i32] * 5 + [_vp] * 5 + [_f64] * 3 + [_i32] * 2 + [_vp] * 9 + [_vp],
                         C.c_int),

@@ -89,7 +89,7 @@ int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                     const fx_index* node_rows, double rho, double mu, double dt, fx_index step,
                     fx_index num_hist, fx_index lag, const double* hist, const double* nxt,
                     const double* adj, const fx_index* inc_off, const fx_index* inc_elem,
-                    const fx_index* inc_meta, double* out, void* stream)
+                    double* out, void* stream)
 {
   return fx::guard([&] {
     fx::require(num_nodes >= 0 && num_states >= 0 && num_elems >= 0, FX_ERR_ARG,
@@ -105,7 +105,7 @@ int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                 "Time assembly variable block is invalid");
     if (num_states == 0) return;
     fx::require(num_nodes > 0 && x && y && node_rows && hist && nxt && adj && inc_off && out
-                    && (num_elems == 0 || (conn3 && dofs9 && inc_elem && inc_meta)),
+                    && (num_elems == 0 || (conn3 && dofs9 && inc_elem)),
                 FX_ERR_ARG, "fx_gls_hist_vjp: null array");
     const std::size_t S = std::size_t(num_states) * 8, H = S * num_hist;
     fx::require(!fx::overlaps(out, S, hist, H) && !fx::overlaps(out, S, nxt, S)
@@ -114,8 +114,7 @@ int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
     const unsigned   nb = fx::blocks_for(num_nodes, kGlsBlock);
     const GlsVjpArgs a{num_nodes, num_states, x,   y,   conn3, dofs9,
                        node_rows, step,       num_hist, lag, hist,  nxt,
-                       adj,       gls_const(rho, mu, dt), inc_off, inc_elem,
-                       reinterpret_cast<const uint4*>(inc_meta), out};
+                       adj,       gls_const(rho, mu, dt), inc_off, inc_elem, out};
     hipLaunchKernelGGL(k_gls_hist_vjp, dim3(nb), dim3(64), 0, fx::as_stream(stream), a);
     fx::check_launch("fx_gls_hist_vjp");
   });

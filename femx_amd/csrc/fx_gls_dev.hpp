@@ -515,7 +515,6 @@ struct GlsVjpArgs
   GlsConst      c;
   const int*    inc_off;
   const int*    inc_elem;
-  const uint4*  inc_meta;
   double*       out;
 };
 
@@ -695,13 +694,13 @@ __device__ __forceinline__ void gls_hist_vjp_block(int blk, const GlsVjpArgs& a)
     double     c0 = 0.0, c1 = 0.0;
     if (have && a.lag < 2)
     {
-      const int   e  = a.inc_elem[t];
-      const uint4 m1 = a.inc_meta[2 * t + 1];
-      const int   li = static_cast<int>((m1.z >> 24) & 0xffu); // byte 27
-      int         vv[3], d[9];
-      double      xe[3], ye[3], cur[6], prev[6], nx[9], aj[9];
+      const int e = a.inc_elem[t];
+      int       vv[3], d[9];
+      double    xe[3], ye[3], cur[6], prev[6], nx[9], aj[9];
 #pragma unroll
       for (int k = 0; k < 3; ++k) vv[k] = a.conn[3 * e + k];
+      // the node's local index, read off the connectivity (no metadata load)
+      const int li = vv[0] == n ? 0 : (vv[1] == n ? 1 : 2);
 #pragma unroll
       for (int k = 0; k < 9; ++k) d[k] = a.dofs[9 * e + k];
 #pragma unroll

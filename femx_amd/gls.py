@@ -201,7 +201,7 @@ def hist_vjp(op: NavierOperator, step, num_hist, lag, amap: AssemblyMap, hist, n
     _lib.call("fx_gls_hist_vjp", geom.num_nodes, geom.num_states, geom.num_elems, ptr(geom.x),
               ptr(geom.y), ptr(geom.conn), ptr(geom.dofs9), ptr(p.node_rows), op.fluid.rho,
               op.fluid.mu, op.dt, int(step), int(num_hist), int(lag), ptr(hist), ptr(nxt),
-              ptr(adj), ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta), ptr(out),
+              ptr(adj), ptr(p.inc_off), ptr(p.inc_elem), ptr(out),
               stream_handle(stream))
     return out
 

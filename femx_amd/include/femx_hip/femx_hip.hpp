@@ -665,8 +665,8 @@ inline void applyNavierHistJacT(const NavierOperator& op, Index step, Index num_
   check(fx_gls_hist_vjp(g.num_nodes, g.num_states, g.num_elems, g.x.data(), g.y.data(),
                         g.conn3.data(), g.dofs9.data(), g.plan.node_rows.data(), op.fluid.rho,
                         op.fluid.mu, op.dt, step, num_hist, lag, hist.data(), nxt.data(),
-                        adj.data(), g.plan.inc_off.data(), g.plan.inc_elem.data(),
-                        g.plan.inc_meta.data(), out.data(), ctx.raw()));
+                        adj.data(), g.plan.inc_off.data(), g.plan.inc_elem.data(), out.data(),
+                        ctx.raw()));
 }
 } // namespace model::ns
 

@@ -355,14 +355,15 @@ int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems,
  * NavierOperator's residual through u, grad u, the extrapolated advection
  * velocity and tau(u); pressure entries are zero.  adj has num_states
  * entries (the residual space).  Per state DOF, a sum over its elements in
- * ascending element order.  Tables from fx_map_gls_plan. */
+ * ascending element order.  node_rows, inc_off and inc_elem from
+ * fx_map_gls_plan. */
 int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                     const double* d_x, const double* d_y, const fx_index* d_conn3,
                     const fx_index* d_dofs9, const fx_index* d_node_rows, double rho, double mu,
                     double dt, fx_index step, fx_index num_hist, fx_index lag,
                     const double* d_hist, const double* d_nxt, const double* d_adj,
-                    const fx_index* d_inc_off, const fx_index* d_inc_elem,
-                    const fx_index* d_inc_meta, double* d_out, void* stream);
+                    const fx_index* d_inc_off, const fx_index* d_inc_elem, double* d_out,
+                    void* stream);
 
 /* ======================================================================== */
 /* Batches (device).  Many P1 Poisson and Taylor-Hood instances of uneven size
