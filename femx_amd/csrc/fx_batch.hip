@@ -213,6 +213,8 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
       const fx_p1_instance& q = p1[i];
       fx::require(q.num_nodes > 0 && q.num_elems > 0 && q.max_row_len > 0 && q.max_row_len <= 32,
                   FX_ERR_ARG, "fx_batch_create: invalid P1 instance sizes");
+      fx::require(fx::fits32(q.num_nodes, 8) && fx::fits32(q.num_elems, 8), FX_ERR_OVERFLOW,
+                  "fx_batch_create: P1 instance arrays beyond 4 GiB");
       fx::require(q.x && q.y && q.conn && q.inc_off && q.inc && q.row_ptr && q.res && q.vals,
                   FX_ERR_ARG, "fx_batch_create: P1 instance lacks an array");
       max_row_len = std::max(max_row_len, static_cast<int>(q.max_row_len));
@@ -235,6 +237,9 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
       const fx_th_instance& q = th[i];
       fx::require(q.ndof > 0 && q.num_elems > 0 && q.num_blocks > 0 && q.max_block_nnz >= 0,
                   FX_ERR_ARG, "fx_batch_create: invalid Taylor-Hood instance sizes");
+      fx::require(fx::fits32(q.ndof, 8) && fx::fits32(q.num_elems, 8)
+                      && fx::fits32(15 * static_cast<std::int64_t>(q.num_elems), 16),
+                  FX_ERR_OVERFLOW, "fx_batch_create: Taylor-Hood instance arrays beyond 4 GiB");
       fx::require(q.xv && q.yv && q.vconn3 && q.dofs15 && q.state && q.inc_off && q.inc_elem
                       && q.inc_meta && q.blk_row && q.blk_flags && q.row_ptr && q.vals && q.res,
                   FX_ERR_ARG, "fx_batch_create: Taylor-Hood instance lacks an array");

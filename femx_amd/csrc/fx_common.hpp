@@ -101,6 +101,10 @@ __device__ __forceinline__ XcdWalk xcd_walk(int n, int sub = 1, int j = 0)
   return {begin + k * sub + j, min(n, begin + chunk), per * sub};
 }
 
+// n items of `bytes` each fit under 4 GiB: the kernels gather such arrays
+// through 32-bit byte offsets.
+inline bool fits32(std::int64_t n, int bytes) { return n >= 0 && n * bytes <= 0xffffffffLL; }
+
 inline hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 inline unsigned blocks_for(std::int64_t n, int block)

@@ -34,6 +34,8 @@ int fx_p1_assemble(fx_index num_nodes, fx_index num_elems, fx_index nnz, fx_inde
     if (num_nodes == 0) return;
     fx::require(x && y && inc_off && row_ptr && res && (num_elems == 0 || (conn && inc)),
                 FX_ERR_ARG, "fx_p1_assemble: null array");
+    fx::require(fx::fits32(num_nodes, 8) && fx::fits32(num_elems, 8), FX_ERR_OVERFLOW,
+                "fx_p1_assemble: node or element arrays beyond 4 GiB");
     fx::require(max_row_len >= 0 && max_row_len <= kP1MaxRowLen, FX_ERR_UNSUPPORTED,
                 "fx_p1_assemble: CSR rows longer than 32 entries");
     const std::size_t N = num_nodes;
@@ -62,6 +64,8 @@ int fx_p1_vjp(fx_index num_nodes, fx_index num_elems, const double* x, const dou
     fx::require(x && y && u && lam && inc_off && jt_lam
                     && (num_elems == 0 || (conn && inc && m && g)),
                 FX_ERR_ARG, "fx_p1_vjp: null array");
+    fx::require(fx::fits32(num_nodes, 8) && fx::fits32(num_elems, 8), FX_ERR_OVERFLOW,
+                "fx_p1_vjp: node or element arrays beyond 4 GiB");
     const std::size_t N = num_nodes, E = num_elems;
     fx::require(!fx::overlaps(jt_lam, N * 8, g, E * 8) && !fx::overlaps(jt_lam, N * 8, lam, N * 8)
                     && !fx::overlaps(jt_lam, N * 8, u, N * 8)

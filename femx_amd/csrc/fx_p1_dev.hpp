@@ -11,16 +11,18 @@
 // in LDS with coalesced loads, and their CSR values are one contiguous range,
 // accumulated in LDS and written back with coalesced stores.  Streams read or
 // written once (incidences, outputs) use non-temporal accesses so that the
-// L2 keeps the node and element data that neighbouring rows share.  Every CSR value and residual
-// entry is a sum over its elements in ascending element order with no
-// atomics: results are deterministic and equal, operation for operation, to
+// L2 keeps the node and element data that neighbouring rows share.  Every
+// CSR value and residual entry is a sum over its elements in ascending
+// element order, added by the one lane that owns the row (tile updates are
+// LDS atomic adds only to skip the read round trip; no two lanes ever add to
+// one entry): results are deterministic and equal, operation for operation, to
 // femx's host assemble loop (femx/assembly/Assembly.hpp:184-225) evaluating
 // PoissonComponents::evalRow (examples/poisson/PoissonComponents.hpp:35-64)
 // on ElementValues' physical gradients (femx/fem/ElementValues.cpp:138-181).
 // The file is compiled without floating-point contraction so that each
 // product is rounded as in femx's host code.
 #pragma once
-#include "fx_common.hpp"
+#include "fx_dev.hpp"
 
 namespace fxk
 {
@@ -127,8 +129,73 @@ struct P1AsmArgs
   int           acc_cap;
 };
 
+// The data one incidence reads: its element's three vertex coordinates,
+// state and load values, and the element's diffusion value.
+struct P1Raw
+{
+  double x[3], y[3], u[3], f[3], kv;
+};
+
+__device__ __forceinline__ int3 p1_conn(const int* __restrict__ conn, int e)
+{
+  const int* c = conn + 3 * static_cast<long>(e);
+  return make_int3(c[0], c[1], c[2]);
+}
+
+// The row's own node is vertex li of every element it meets, so its values
+// (o) are loaded once per row and only the element's two other vertices are
+// gathered per incidence; the three are then put back in element order.
+__device__ __forceinline__ double p1_place(int li, int k, double own, double oa, double ob)
+{
+  // vertices other than li, in element order: li=0 -> (1,2), 1 -> (0,2), 2 -> (0,1)
+  return k == 0 ? (li == 0 ? own : oa) : (k == 1 ? (li == 0 ? oa : (li == 1 ? own : ob)) : (li == 2 ? own : ob));
+}
+
+struct P1Own
+{
+  double x, y, u, f;
+};
+
+// The element's two vertices other than li, in element order:
+// li=0 -> (1,2), 1 -> (0,2), 2 -> (0,1).
+typedef int fx_i3 __attribute__((ext_vector_type(3), aligned(4)));
+
+__device__ __forceinline__ int2 p1_others(const int* __restrict__ conn, int e, int li)
+{
+  const fx_i3 c = *reinterpret_cast<const fx_i3*>(conn + 3 * static_cast<long>(e));
+  return make_int2(li == 0 ? c.y : c.x, li == 2 ? c.y : c.z);
+}
+
+__device__ __forceinline__ P1Raw p1_load_raw(const P1AsmArgs& a, int e, int2 n, int li, const P1Own& o)
+{
+  const int ia = n.x, ib = n.y;
+  const double xa = gat32(a.x, ia), xb = gat32(a.x, ib), ya = gat32(a.y, ia), yb = gat32(a.y, ib);
+  const double ua = a.u ? gat32(a.u, ia) : 0.0, ub = a.u ? gat32(a.u, ib) : 0.0;
+  const double fa = a.f ? gat32(a.f, ia) : 0.0, fb = a.f ? gat32(a.f, ib) : 0.0;
+  P1Raw R;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    R.x[k] = p1_place(li, k, o.x, xa, xb);
+    R.y[k] = p1_place(li, k, o.y, ya, yb);
+    R.u[k] = p1_place(li, k, o.u, ua, ub);
+    R.f[k] = p1_place(li, k, o.f, fa, fb);
+  }
+  R.kv = a.kappa ? gat32(a.kappa, e) : 1.0;
+  return R;
+}
+
 // One row block of the P1 assembly (rows [blk*256, blk*256+256)); smem holds
 // acc_cap doubles of row tile followed by the incidence stage.
+//
+// Each lane walks its row's incidences with the next incidence's vertex
+// indices in flight while the current one computes (the lookahead past the
+// row's last incidence re-reads that incidence, always valid, and is
+// discarded).  The row's own node is loaded once; per incidence only the
+// element's two other vertices and its diffusion value are gathered.  Tile
+// updates are LDS atomic adds: the three positions of an incidence are
+// distinct entries of the lane's own row, so each becomes old + K in program
+// order, as the host loop adds them.
 __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, double* smem)
 {
   double*     acc  = smem;
@@ -150,30 +217,34 @@ __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, d
   {
     double*   racc = acc + (a.row_ptr[r] - base);
     double    rsum = 0.0;
-    const int t1   = a.inc_off[r + 1];
-    for (int t = a.inc_off[r]; t < t1; ++t)
+    const int ta = a.inc_off[r], t1 = a.inc_off[r + 1], tl = t1 - 1;
+    if (ta < t1)
     {
-      const int2     it   = inc_at(sinc, a.inc, staged, t, ib0);
-      const int      e    = it.x;
-      const unsigned meta = static_cast<unsigned>(it.y);
-      const int      li   = meta & 0xff;
-      const int      n0 = a.conn[3 * e], n1 = a.conn[3 * e + 1], n2 = a.conn[3 * e + 2];
-      const P1Geom   g  = p1_geom(a.x[n0], a.y[n0], a.x[n1], a.y[n1], a.x[n2], a.y[n2]);
-      const double   ke = a.kappa ? (a.kappa_is_log ? exp(a.kappa[e]) : a.kappa[e]) : 1.0;
-      const double   k0 = p1_k(g, li, 0, ke), k1 = p1_k(g, li, 1, ke), k2 = p1_k(g, li, 2, ke);
-      double         re = 0.0;
-      if (a.u) re = k0 * a.u[n0] + k1 * a.u[n1] + k2 * a.u[n2];
-      if (a.f) re = re - p1_load(g, li, a.f[n0], a.f[n1], a.f[n2]);
-      else re = re - 0.0;
-      rsum = rsum + re;
-      if (a.vals)
+      const P1Own o{a.x[r], a.y[r], a.u ? a.u[r] : 0.0, a.f ? a.f[r] : 0.0};
+      int2        ic = inc_at(sinc, a.inc, staged, ta, ib0);
+      int2        cc = p1_others(a.conn, ic.x, ic.y & 0xff);
+      for (int t = ta; t < t1; ++t)
       {
-        // distinct columns: read all three, then write (no dependent round trips)
-        const int    p0 = (meta >> 8) & 0xff, p1 = (meta >> 16) & 0xff, p2 = meta >> 24;
-        const double o0 = racc[p0], o1 = racc[p1], o2 = racc[p2];
-        racc[p0]        = o0 + k0;
-        racc[p1]        = o1 + k1;
-        racc[p2]        = o2 + k2;
+        const int2     in   = inc_at(sinc, a.inc, staged, min(t + 1, tl), ib0);
+        const unsigned meta = static_cast<unsigned>(ic.y);
+        const int      li   = meta & 0xff;
+        const P1Raw    R    = p1_load_raw(a, ic.x, cc, li, o);
+        cc                  = p1_others(a.conn, in.x, in.y & 0xff);
+        const P1Geom   g    = p1_geom(R.x[0], R.y[0], R.x[1], R.y[1], R.x[2], R.y[2]);
+        const double   ke   = (a.kappa && a.kappa_is_log) ? exp(R.kv) : R.kv;
+        const double   k0 = p1_k(g, li, 0, ke), k1 = p1_k(g, li, 1, ke), k2 = p1_k(g, li, 2, ke);
+        double         re = 0.0;
+        if (a.u) re = k0 * R.u[0] + k1 * R.u[1] + k2 * R.u[2];
+        if (a.f) re = re - p1_load(g, li, R.f[0], R.f[1], R.f[2]);
+        else re = re - 0.0;
+        rsum = rsum + re;
+        if (a.vals)
+        {
+          lds_add(racc + ((meta >> 8) & 0xff), k0);
+          lds_add(racc + ((meta >> 16) & 0xff), k1);
+          lds_add(racc + (meta >> 24), k2);
+        }
+        ic = in;
       }
     }
     __builtin_nontemporal_store(rsum, a.res + r);
@@ -216,31 +287,49 @@ __device__ __forceinline__ void p1_vjp_block(int blk, const P1VjpArgs& a, long l
   if (j < a.num_rows)
   {
     double    acc = 0.0;
-    const int t1  = a.inc_off[j + 1];
-    for (int t = a.inc_off[j]; t < t1; ++t)
+    const int ta = a.inc_off[j], t1 = a.inc_off[j + 1], tl = t1 - 1;
+    if (ta < t1)
     {
-      const int2   it = inc_at(sinc, a.inc, staged, t, ib0);
-      const int    e  = it.x;
-      const int    li = static_cast<unsigned>(it.y) & 0xff;
-      const int    n[3] = {a.conn[3 * e], a.conn[3 * e + 1], a.conn[3 * e + 2]};
-      const P1Geom g    = p1_geom(a.x[n[0]], a.y[n[0]], a.x[n[1]], a.y[n[1]], a.x[n[2]], a.y[n[2]]);
-      const double ke   = exp(a.m[e]);
-      double       le[3];
-#pragma unroll
-      for (int r = 0; r < 3; ++r) le[r] = (a.bc_mask && a.bc_mask[n[r]]) ? 0.0 : a.lam[n[r]];
-      const double c0 = p1_k(g, 0, li, ke), c1 = p1_k(g, 1, li, ke), c2 = p1_k(g, 2, li, ke);
-      acc = acc + (c0 * le[0] + c1 * le[1] + c2 * le[2]);
-      if (li == 0)
+      // node j is vertex li of each of its elements: its values are loaded
+      // once, the element's two other vertices are gathered per incidence
+      // (their indices one incidence ahead)
+      auto masked = [&](int n) { return (a.bc_mask && gat32(a.bc_mask, n)) ? 0.0 : gat32(a.lam, n); };
+      const double xo = a.x[j], yo = a.y[j], lo = masked(j), uo = a.u[j];
+      int2         ic = inc_at(sinc, a.inc, staged, ta, ib0);
+      int2         cc = p1_others(a.conn, ic.x, ic.y & 0xff);
+      for (int t = ta; t < t1; ++t)
       {
-        const double u0 = a.u[n[0]], u1 = a.u[n[1]], u2 = a.u[n[2]];
-        double       ge = 0.0;
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
+        const int2   in = inc_at(sinc, a.inc, staged, min(t + 1, tl), ib0);
+        const int    e  = ic.x;
+        const int    li = static_cast<unsigned>(ic.y) & 0xff;
+        const double xa = gat32(a.x, cc.x), xb = gat32(a.x, cc.y);
+        const double ya = gat32(a.y, cc.x), yb = gat32(a.y, cc.y);
+        const double la = masked(cc.x), lb = masked(cc.y);
+        const double me = gat32(a.m, e);
+        const int2   nx = cc;
+        cc              = p1_others(a.conn, in.x, in.y & 0xff);
+        const P1Geom g  = p1_geom(p1_place(li, 0, xo, xa, xb), p1_place(li, 0, yo, ya, yb),
+                                  p1_place(li, 1, xo, xa, xb), p1_place(li, 1, yo, ya, yb),
+                                  p1_place(li, 2, xo, xa, xb), p1_place(li, 2, yo, ya, yb));
+        const double ke = exp(me);
+        const double le[3] = {p1_place(li, 0, lo, la, lb), p1_place(li, 1, lo, la, lb),
+                              p1_place(li, 2, lo, la, lb)};
+        const double c0 = p1_k(g, 0, li, ke), c1 = p1_k(g, 1, li, ke), c2 = p1_k(g, 2, li, ke);
+        acc = acc + (c0 * le[0] + c1 * le[1] + c2 * le[2]);
+        if (li == 0)
         {
-          const double re = p1_k(g, r, 0, ke) * u0 + p1_k(g, r, 1, ke) * u1 + p1_k(g, r, 2, ke) * u2;
-          ge              = ge + le[r] * (re - 0.0);
+          // the element's vertex 0 is node j; vertices 1 and 2 are its others
+          const double u0 = uo, u1 = gat32(a.u, nx.x), u2 = gat32(a.u, nx.y);
+          double       ge = 0.0;
+#pragma unroll
+          for (int r = 0; r < 3; ++r)
+          {
+            const double re = p1_k(g, r, 0, ke) * u0 + p1_k(g, r, 1, ke) * u1 + p1_k(g, r, 2, ke) * u2;
+            ge              = ge + le[r] * (re - 0.0);
+          }
+          a.gout[e] = ge;
         }
-        a.gout[e] = ge;
+        ic = in;
       }
     }
     if (a.bc_mask && a.bc_mask[j]) acc = acc + a.lam[j];

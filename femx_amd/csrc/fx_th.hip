@@ -49,6 +49,9 @@ int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks, fx_in
     fx::require(ndof >= 0 && num_elems >= 0 && num_blocks >= 0 && max_block_nnz >= 0, FX_ERR_ARG,
                 "fx_th_assemble: negative size");
     if (ndof == 0 || num_blocks == 0) return;
+    fx::require(fx::fits32(ndof, 8) && fx::fits32(num_elems, 8)
+                    && fx::fits32(15 * static_cast<std::int64_t>(num_elems), 16),
+                FX_ERR_OVERFLOW, "fx_th_assemble: state, element or incidence arrays beyond 4 GiB");
     fx::require(xv && yv && state && inc_off && blk_row && blk_flags && row_ptr && res
                     && (num_elems == 0 || (vconn3 && dofs15 && inc_elem && inc_meta)),
                 FX_ERR_ARG, "fx_th_assemble: null array");

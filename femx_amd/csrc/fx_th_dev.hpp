@@ -32,7 +32,7 @@
 // order (a segmented reduction over a stream) instead of scattering with
 // atomics.
 #pragma once
-#include "fx_common.hpp"
+#include "fx_dev.hpp"
 
 namespace fxk
 {
@@ -321,33 +321,31 @@ struct ThRaw
   double nu;
 };
 
+// Incidence t of the row-incidence tables; t must be a valid index (callers
+// clamp to the row's last incidence and discard what they do not own).
 __device__ __forceinline__ ThIdx th_load_idx(const int* __restrict__ inc_elem,
-                                             const uint4* __restrict__ inc_meta, int t, int t1)
+                                             const uint4* __restrict__ inc_meta, int t)
 {
-  ThIdx i{0, make_uint4(0, 0, 0, 0)};
-  if (t < t1)
-  {
-    i.e    = inc_elem[t];
-    i.meta = inc_meta[t];
-  }
-  return i;
+  return {gat32(inc_elem, t), gat32(inc_meta, t)};
 }
 
 __device__ __forceinline__ ThDofs th_load_dofs(const int* __restrict__ dofs,
-                                               const int* __restrict__ vconn, int e, bool have)
+                                               const int* __restrict__ vconn, int e)
 {
-  ThDofs D;
+  typedef int i4 __attribute__((ext_vector_type(4), aligned(4)));
+  typedef int i3 __attribute__((ext_vector_type(3), aligned(4)));
+  ThDofs     D;
+  const int* d = dofs + 15 * static_cast<long>(e);
 #pragma unroll
-  for (int k = 0; k < 15; ++k) D.d[k] = 0;
-  D.v[0] = D.v[1] = D.v[2] = 0;
-  if (have)
+  for (int q = 0; q < 3; ++q)
   {
-    const int* d = dofs + 15 * static_cast<long>(e);
-#pragma unroll
-    for (int k = 0; k < 15; ++k) D.d[k] = d[k];
-    const int* v = vconn + 3 * static_cast<long>(e);
-    D.v[0] = v[0], D.v[1] = v[1], D.v[2] = v[2];
+    const i4 w = *reinterpret_cast<const i4*>(d + 4 * q);
+    D.d[4 * q] = w.x, D.d[4 * q + 1] = w.y, D.d[4 * q + 2] = w.z, D.d[4 * q + 3] = w.w;
   }
+  const i3 t = *reinterpret_cast<const i3*>(d + 12);
+  D.d[12] = t.x, D.d[13] = t.y, D.d[14] = t.z;
+  const i3 v = *reinterpret_cast<const i3*>(vconn + 3 * static_cast<long>(e));
+  D.v[0] = v.x, D.v[1] = v.y, D.v[2] = v.z;
   return D;
 }
 
@@ -451,30 +449,26 @@ __device__ __forceinline__ void th_pressure_row(const ThTensor& T, const ThElemT
 }
 
 // Add one element row -- and, for a velocity pair, the partner row, which
-// has the same column positions -- into the LDS tile.  The 15 positions of an
-// element row are distinct CSR entries, so every old value is read before any
-// is written back: the reads issue back to back instead of as dependent
-// read-modify-write round trips, and each entry still gets old + K.
-__device__ __forceinline__ void th_add_rows(double* ax, double* ay, uint4 meta, double* Kx,
-                                            double* Ky, bool pairs)
+// has the same column positions -- into the LDS tile.  Each update is an
+// LDS floating-point atomic add issued without waiting for a result: the 15
+// positions of an element row are distinct CSR entries and the lanes that
+// add at the same time own distinct rows, so no two updates in one
+// instruction meet, and one wave's LDS operations execute in issue order.
+// Every entry therefore still becomes old + K, in program order, with one
+// LDS instruction instead of a read, a wait, an add and a write.
+__device__ __forceinline__ void th_add_rows(double* ax, double* ay, uint4 meta, const double* Kx,
+                                            const double* Ky, bool pairs)
 {
   const unsigned w[4] = {meta.x, meta.y, meta.z, meta.w};
   int            pos[15];
 #pragma unroll
   for (int j = 0; j < 15; ++j) pos[j] = (w[j >> 2] >> (8 * (j & 3))) & 0xff;
 #pragma unroll
-  for (int j = 0; j < 15; ++j) Kx[j] = ax[pos[j]] + Kx[j];
+  for (int j = 0; j < 15; ++j) lds_add(ax + pos[j], Kx[j]);
   if (pairs)
   {
 #pragma unroll
-    for (int j = 0; j < 15; ++j) Ky[j] = ay[pos[j]] + Ky[j];
-  }
-#pragma unroll
-  for (int j = 0; j < 15; ++j) ax[pos[j]] = Kx[j];
-  if (pairs)
-  {
-#pragma unroll
-    for (int j = 0; j < 15; ++j) ay[pos[j]] = Ky[j];
+    for (int j = 0; j < 15; ++j) lds_add(ay + pos[j], Ky[j]);
   }
 }
 
@@ -499,31 +493,28 @@ struct ThAsmArgs
   double*       res;
 };
 
-__device__ __forceinline__ ThRaw th_load_raw(const ThAsmArgs& a, const ThDofs& D, int e, bool have)
+__device__ __forceinline__ ThRaw th_load_raw(const ThAsmArgs& a, const ThDofs& D, int e)
 {
   ThRaw R;
 #pragma unroll
-  for (int k = 0; k < 15; ++k) R.s[k] = 0.0;
-  R.x[0] = R.x[1] = R.x[2] = R.y[0] = R.y[1] = R.y[2] = 0.0;
-  R.nu = a.nu;
-  if (have)
-  {
+  for (int k = 0; k < 15; ++k) R.s[k] = gat32(a.state, D.d[k]);
 #pragma unroll
-    for (int k = 0; k < 15; ++k) R.s[k] = a.state[D.d[k]];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) R.x[k] = a.xv[D.v[k]], R.y[k] = a.yv[D.v[k]];
-    if (a.nu_e) R.nu = a.nu_e[e];
-  }
+  for (int k = 0; k < 3; ++k) R.x[k] = gat32(a.xv, D.v[k]), R.y[k] = gat32(a.yv, D.v[k]);
+  R.nu = a.nu_e ? gat32(a.nu_e, e) : a.nu;
   return R;
 }
 
 // One row block per wave.  Two lanes per work item: lane role 0 takes the
-// item's incidences t0, t0+2, ..., role 1 takes t0+1, t0+3, ...; after each
-// round role 0 applies its own contribution and then its partner's (fetched
-// by a cross-lane swap), so every CSR value still receives its element
-// contributions in ascending element order, from one thread, in program
-// order.  `acc` is the wave's own row tile; the waves of a workgroup share
-// only the tables.
+// item's incidences t0, t0+2, ..., role 1 takes t0+1, t0+3, ...; each round
+// role 0 adds its element rows into the tile, then role 1 adds its own, so
+// every CSR value receives its element contributions in ascending element
+// order, in program order, with no atomics.  The residual sums take the
+// partner's value by a cross-lane swap.  `acc` is the wave's own row tile;
+// the waves of a workgroup share only the tables.
+//
+// Loads are unconditional: a lane whose next incidence lies past the row's
+// end loads the row's last incidence instead (always valid) and discards the
+// result, so no load is predicated and nothing is zero-filled.
 __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, const ThTensor& T,
                                                   double* acc)
 {
@@ -543,66 +534,71 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
     double*   ax = acc + (a.row_ptr[r] - base);
     double*   ay = pairs ? acc + (a.row_ptr[r + 1] - base) : nullptr;
     double    sx = 0.0, sy = 0.0;
-    const int t0 = a.inc_off[r], t1 = a.inc_off[r + 1];
-    // software pipeline over this lane's incidences t0+role, t0+role+2, ...:
-    // values of the next round and tables of the one after are in flight
-    // while the current round computes
-    ThIdx  ic = th_load_idx(a.inc_elem, a.inc_meta, t0 + role, t1);
-    ThIdx  in = th_load_idx(a.inc_elem, a.inc_meta, t0 + role + 2, t1);
-    ThRaw  rc = th_load_raw(a, th_load_dofs(a.dofs, a.vconn, ic.e, t0 + role < t1), ic.e, t0 + role < t1);
-    ThDofs dn = th_load_dofs(a.dofs, a.vconn, in.e, t0 + role + 2 < t1);
-    for (int t = t0; t < t1; t += 2)
+    const int t0 = a.inc_off[r], t1 = a.inc_off[r + 1], tl = t1 - 1;
+    if (t0 < t1)
     {
-      const int  tm   = t + role;
-      const bool have = tm < t1;
-      const ThIdx in2 = th_load_idx(a.inc_elem, a.inc_meta, tm + 4, t1);
-      const ThRaw rn  = th_load_raw(a, dn, in.e, tm + 2 < t1);
-      double      K0[15], K1[15], R0 = 0.0, R1 = 0.0;
-      const uint4 meta = ic.meta;
-      if (have)
-      {
-        const int li = meta.w >> 24;
-        ThElemT   E;
-        th_make_elem(E, rc);
-        if (li < 12)
+      // software pipeline over this lane's incidences t0+role, t0+role+2, ...:
+      // values of the next round and tables of the one after are in flight
+      // while the current round computes
+      ThIdx  ic = th_load_idx(a.inc_elem, a.inc_meta, min(t0 + role, tl));
+      ThIdx  in = th_load_idx(a.inc_elem, a.inc_meta, min(t0 + role + 2, tl));
+      ThRaw  ra = th_load_raw(a, th_load_dofs(a.dofs, a.vconn, ic.e), ic.e), rb;
+      ThDofs dn = th_load_dofs(a.dofs, a.vconn, in.e);
+      // one round: compute and apply incidence t+role from `cur` while the
+      // next round's values land in `nxt` (the loop alternates the two
+      // buffers, so no values are copied between rounds)
+      auto round = [&](int t, const ThRaw& cur, ThRaw& nxt) {
+        const int   tm   = t + role;
+        const bool  have = tm < t1;
+        const ThIdx in2  = th_load_idx(a.inc_elem, a.inc_meta, min(tm + 4, tl));
+        nxt              = th_load_raw(a, dn, in.e);
+        double      K0[15], K1[15], R0, R1 = 0.0;
+        const uint4 meta = ic.meta;
         {
-          th_velocity_rows(T, E, li >> 1, K0, K1, R0, R1);
-          if (!pairs && (li & 1))
+          const int li = meta.w >> 24;
+          ThElemT   E;
+          th_make_elem(E, cur);
+          if (li < 12)
           {
+            th_velocity_rows(T, E, li >> 1, K0, K1, R0, R1);
+            if (!pairs && (li & 1))
+            {
 #pragma unroll
-            for (int j = 0; j < 15; ++j) K0[j] = K1[j];
-            R0 = R1;
+              for (int j = 0; j < 15; ++j) K0[j] = K1[j];
+              R0 = R1;
+            }
           }
+          else
+            th_pressure_row(T, E, li - 12, K0, R0);
         }
-        else
-          th_pressure_row(T, E, li - 12, K0, R0);
-      }
-      else
+        // Apply in element order: role 0 (incidence t) first, then role 1
+        // (incidence t+1), with the wave's LDS accesses kept in program order.
+        if (role == 0)
+        {
+          sx = sx + R0;
+          if (pairs) sy = sy + R1;
+          if (a.vals) th_add_rows(ax, ay, meta, K0, K1, pairs);
+        }
+        wave_sync();
+        if (role == 1 && have && a.vals) th_add_rows(ax, ay, meta, K0, K1, pairs);
+        wave_sync();
+        const double pR0 = shfl1(R0), pR1 = shfl1(R1);
+        if (role == 0 && t + 1 < t1)
+        {
+          sx = sx + pR0;
+          if (pairs) sy = sy + pR1;
+        }
+        dn = th_load_dofs(a.dofs, a.vconn, in2.e);
+        ic = in;
+        in = in2;
+      };
+      for (int t = t0;;)
       {
-#pragma unroll
-        for (int j = 0; j < 15; ++j) K0[j] = 0.0, K1[j] = 0.0;
+        round(t, ra, rb);
+        if ((t += 2) >= t1) break;
+        round(t, rb, ra);
+        if ((t += 2) >= t1) break;
       }
-      // Apply in element order: role 0 (incidence t) first, then role 1
-      // (incidence t+1), with the wave's LDS accesses kept in program order.
-      if (role == 0 && have)
-      {
-        sx = sx + R0;
-        if (pairs) sy = sy + R1;
-        if (a.vals) th_add_rows(ax, ay, meta, K0, K1, pairs);
-      }
-      wave_sync();
-      if (role == 1 && have && a.vals) th_add_rows(ax, ay, meta, K0, K1, pairs);
-      wave_sync();
-      const double pR0 = shfl1(R0), pR1 = shfl1(R1);
-      if (role == 0 && t + 1 < t1)
-      {
-        sx = sx + pR0;
-        if (pairs) sy = sy + pR1;
-      }
-      dn = th_load_dofs(a.dofs, a.vconn, in2.e, tm + 4 < t1);
-      ic = in;
-      in = in2;
-      rc = rn;
     }
     if (role == 0)
     {
