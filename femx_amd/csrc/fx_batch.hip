@@ -117,14 +117,11 @@ __global__ __launch_bounds__(kThLanes) void k_batch_th_vjp_elem(const ThVjpArgs*
                                                                const int2* __restrict__ work,
                                                                int nwork)
 {
-  __shared__ ThTables T;
-  th_build_tables(T);
-  __syncthreads();
   const fx::XcdWalk wk = fx::xcd_walk(nwork);
   for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
-    th_vjp_elem_block(it.y, inst[it.x], T);
+    th_vjp_elem_block(it.y, inst[it.x], kThTables);
   }
 }
 

@@ -22,10 +22,7 @@ __global__ __launch_bounds__(kThLanes* kThWaves) void k_th_assemble(ThAsmArgs a,
 
 __global__ __launch_bounds__(kThLanes) void k_th_vjp_elem(ThVjpArgs a)
 {
-  __shared__ ThTables T;
-  th_build_tables(T);
-  __syncthreads();
-  th_vjp_elem_block(fx::xcd_block(gridDim.x), a, T);
+  th_vjp_elem_block(fx::xcd_block(gridDim.x), a, kThTables);
 }
 
 __global__ __launch_bounds__(256) void k_vjp_gather(GatherArgs a)

@@ -59,47 +59,49 @@ struct ThTables
   double w[7];        // weights on the reference triangle (sum 1/2)
 };
 
+constexpr double kSqrt15 = 3.872983346207417; // nearest double to sqrt(15)
+
 // 7-point rule: centroid and two 3-point orbits with barycentric
 // a = (6 -+ sqrt 15)/21, weights 9/80 and (155 -+ sqrt 15)/2400 (Radon;
 // Stroud T2:5-1).  P2 shapes l_i(2 l_i - 1) and 4 l_i l_j (edges 01, 12, 20).
-__device__ void th_build_tables(ThTables& T)
+constexpr ThTables make_th_tables()
 {
-  const int q = threadIdx.x;
-  if (q >= 7) return;
-  const double s15 = sqrt(15.0);
+  const double s15 = kSqrt15;
   const double a1 = (6.0 - s15) / 21.0, b1 = (9.0 + 2.0 * s15) / 21.0;
   const double a2 = (6.0 + s15) / 21.0, b2 = (9.0 - 2.0 * s15) / 21.0;
   const double w1 = (155.0 - s15) / 2400.0, w2 = (155.0 + s15) / 2400.0;
-  double       r, s, w;
-  switch (q)
+  const double qr[7] = {1.0 / 3.0, a1, b1, a1, a2, b2, a2};
+  const double qs[7] = {1.0 / 3.0, a1, a1, b1, a2, a2, b2};
+  const double qw[7] = {9.0 / 80.0, w1, w1, w1, w2, w2, w2};
+  ThTables     T{};
+  for (int q = 0; q < 7; ++q)
   {
-  case 0: r = 1.0 / 3.0, s = 1.0 / 3.0, w = 9.0 / 80.0; break;
-  case 1: r = a1, s = a1, w = w1; break;
-  case 2: r = b1, s = a1, w = w1; break;
-  case 3: r = a1, s = b1, w = w1; break;
-  case 4: r = a2, s = a2, w = w2; break;
-  case 5: r = b2, s = a2, w = w2; break;
-  default: r = a2, s = b2, w = w2; break;
+    const double l0 = 1.0 - qr[q] - qs[q], l1 = qr[q], l2 = qs[q];
+    T.w[q]    = qw[q];
+    T.N[q][0] = l0 * (2.0 * l0 - 1.0);
+    T.N[q][1] = l1 * (2.0 * l1 - 1.0);
+    T.N[q][2] = l2 * (2.0 * l2 - 1.0);
+    T.N[q][3] = 4.0 * l0 * l1;
+    T.N[q][4] = 4.0 * l1 * l2;
+    T.N[q][5] = 4.0 * l2 * l0;
+    double* d = T.dNdr[q];
+    d[0] = -(4.0 * l0 - 1.0), d[1] = -(4.0 * l0 - 1.0);
+    d[2] = 4.0 * l1 - 1.0, d[3] = 0.0;
+    d[4] = 0.0, d[5] = 4.0 * l2 - 1.0;
+    d[6] = 4.0 * (l0 - l1), d[7] = -4.0 * l1;
+    d[8] = 4.0 * l2, d[9] = 4.0 * l1;
+    d[10] = -4.0 * l2, d[11] = 4.0 * (l0 - l2);
+    T.psi[q][0] = l0;
+    T.psi[q][1] = l1;
+    T.psi[q][2] = l2;
   }
-  const double l0 = 1.0 - r - s, l1 = r, l2 = s;
-  T.w[q]       = w;
-  T.N[q][0]    = l0 * (2.0 * l0 - 1.0);
-  T.N[q][1]    = l1 * (2.0 * l1 - 1.0);
-  T.N[q][2]    = l2 * (2.0 * l2 - 1.0);
-  T.N[q][3]    = 4.0 * l0 * l1;
-  T.N[q][4]    = 4.0 * l1 * l2;
-  T.N[q][5]    = 4.0 * l2 * l0;
-  double* d    = T.dNdr[q];
-  d[0]         = -(4.0 * l0 - 1.0), d[1] = -(4.0 * l0 - 1.0);
-  d[2]         = 4.0 * l1 - 1.0, d[3] = 0.0;
-  d[4]         = 0.0, d[5] = 4.0 * l2 - 1.0;
-  d[6]         = 4.0 * (l0 - l1), d[7] = -4.0 * l1;
-  d[8]         = 4.0 * l2, d[9] = 4.0 * l1;
-  d[10]        = -4.0 * l2, d[11] = 4.0 * (l0 - l2);
-  T.psi[q][0]  = l0;
-  T.psi[q][1]  = l1;
-  T.psi[q][2]  = l2;
+  return T;
 }
+
+// The tables are compile-time constants in device memory.  The VJP reads
+// them at a quadrature point that is the same for every lane of a wave, so
+// the reads are scalar loads: no LDS copy and no workgroup barrier.
+__device__ const ThTables kThTables = make_th_tables();
 
 // Affine P1 geometry: inverse Jacobian (row-major) and |det J| as
 // ElementValues computes them (femx/fem/ElementValues.cpp:138-215).
@@ -124,21 +126,6 @@ struct ThElem
   ThGeom g;
   double nu;
 };
-
-__device__ __forceinline__ void th_gather(ThElem& E, int e, const double* __restrict__ xv,
-                                          const double* __restrict__ yv,
-                                          const int* __restrict__ vconn,
-                                          const int* __restrict__ dofs,
-                                          const double* __restrict__ state, double nu,
-                                          const double* __restrict__ nu_e)
-{
-  const int* d = dofs + 15 * static_cast<long>(e);
-#pragma unroll
-  for (int k = 0; k < 15; ++k) E.s[k] = state[d[k]];
-  const int* v = vconn + 3 * static_cast<long>(e);
-  E.g          = th_geom(xv, yv, v[0], v[1], v[2]);
-  E.nu         = nu_e ? nu_e[e] : nu;
-}
 
 // Physical P2 gradients at one quadrature point.
 __device__ __forceinline__ void th_grads(const ThTables& T, int q, const ThGeom& g, double* dNx,
@@ -182,8 +169,6 @@ struct ThTensor
   double pa[6][6]; // P^r_b'a at [a][2 b' + r]
 };
 constexpr int kThTensorLen = static_cast<int>(sizeof(ThTensor) / sizeof(double));
-
-constexpr double kSqrt15 = 3.872983346207417; // nearest double to sqrt(15)
 
 constexpr double p2_shape(int a, double l0, double l1, double l2)
 {
@@ -639,15 +624,20 @@ __device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, c
 {
   const int e = blk * kThLanes + threadIdx.x;
   if (e >= a.num_elems) return;
-  ThElem E;
-  th_gather(E, e, a.xv, a.yv, a.vconn, a.dofs, a.state, a.nu, a.nu_e);
-  double     l[15], c[15];
-  const int* d = a.dofs + 15 * static_cast<long>(e);
+  // the DOF row and vertices in vector loads, then one 32-bit-offset gather
+  // per value (the pressure state does not enter J^T lam or dR/dnu)
+  const ThDofs D = th_load_dofs(a.dofs, a.vconn, e);
+  ThElem       E;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) E.s[k] = gat32(a.state, D.d[k]);
+  E.g  = th_geom(a.xv, a.yv, D.v[0], D.v[1], D.v[2]);
+  E.nu = a.nu_e ? a.nu_e[e] : a.nu;
+  double l[15], c[15];
 #pragma unroll
   for (int k = 0; k < 15; ++k)
   {
-    const int dk = d[k];
-    l[k]         = (a.bc_mask && a.bc_mask[dk]) ? 0.0 : a.lam[dk];
+    const int dk = D.d[k];
+    l[k]         = (a.bc_mask && gat32(a.bc_mask, dk)) ? 0.0 : gat32(a.lam, dk);
     c[k]         = 0.0;
   }
   double ge = 0.0;
