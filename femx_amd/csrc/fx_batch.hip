@@ -6,7 +6,8 @@
 // a batch owns device descriptors of all instances and one work list per
 // phase -- (instance, block) pairs covering every row block of every
 // instance.  Each phase is one persistent launch: a grid sized to the GPU
-// (compute units x a tunable multiplier) whose workgroups walk the work list
+// (compute units x the larger of a tunable multiplier and the phase kernel's
+// resident workgroups per compute unit) whose workgroups walk the work list
 // -- each XCD's workgroups one contiguous eighth of it, so neighbouring
 // blocks share an L2 -- running the same per-block device code as the
 // single-instance kernels, so each instance's results are bitwise equal to
@@ -23,7 +24,9 @@
 struct fx_batch
 {
   int         num_p1 = 0, num_th = 0;
-  int         grid   = 0;
+  int         grid   = 0;       // compute units x the tunable multiplier
+  int         pgrid[5] = {0, 0, 0, 0, 0}; // per phase: at least one full residency
+  int         cus    = 0;
   std::size_t p1_lds = 0, th_lds = 0;
   int         th_tile = 0; // row tile per wave of the Taylor-Hood assembly (doubles)
   // device memory
@@ -150,27 +153,27 @@ T* upload(const std::vector<T>& h)
 
 void launch_phases(fx_batch* b, int what, hipStream_t s)
 {
-  const dim3 g(b->grid);
+  auto g = [&](int p) { return dim3(b->pgrid[p]); };
   if (what & FX_BATCH_ASSEMBLE)
   {
     if (b->nwork[kP1Asm])
-      hipLaunchKernelGGL(k_batch_p1_assemble, g, dim3(kP1Block), b->p1_lds, s, b->d_p1_asm,
+      hipLaunchKernelGGL(k_batch_p1_assemble, g(kP1Asm), dim3(kP1Block), b->p1_lds, s, b->d_p1_asm,
                          b->d_work[kP1Asm], b->nwork[kP1Asm]);
     if (b->nwork[kThAsm])
-      hipLaunchKernelGGL(k_batch_th_assemble, g, dim3(kThLanes * kThWaves), b->th_lds, s,
+      hipLaunchKernelGGL(k_batch_th_assemble, g(kThAsm), dim3(kThLanes * kThWaves), b->th_lds, s,
                          b->d_th_asm, b->d_work[kThAsm], b->nwork[kThAsm], b->th_tile);
     fx::check_launch("fx_batch: assembly");
   }
   if (what & FX_BATCH_VJP)
   {
     if (b->nwork[kP1Vjp])
-      hipLaunchKernelGGL(k_batch_p1_vjp, g, dim3(kP1Block), 0, s, b->d_p1_vjp, b->d_work[kP1Vjp],
-                         b->nwork[kP1Vjp]);
+      hipLaunchKernelGGL(k_batch_p1_vjp, g(kP1Vjp), dim3(kP1Block), 0, s, b->d_p1_vjp,
+                         b->d_work[kP1Vjp], b->nwork[kP1Vjp]);
     if (b->nwork[kThVjpElem])
-      hipLaunchKernelGGL(k_batch_th_vjp_elem, g, dim3(kThLanes), 0, s, b->d_th_vjp,
+      hipLaunchKernelGGL(k_batch_th_vjp_elem, g(kThVjpElem), dim3(kThLanes), 0, s, b->d_th_vjp,
                          b->d_work[kThVjpElem], b->nwork[kThVjpElem]);
     if (b->nwork[kThVjpGather])
-      hipLaunchKernelGGL(k_batch_vjp_gather, g, dim3(256), 0, s, b->d_th_gat,
+      hipLaunchKernelGGL(k_batch_vjp_gather, g(kThVjpGather), dim3(256), 0, s, b->d_th_gat,
                          b->d_work[kThVjpGather], b->nwork[kThVjpGather]);
     fx::check_launch("fx_batch: VJP");
   }
@@ -193,6 +196,7 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     int cus = 0;
     fx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device),
                   "hipDeviceGetAttribute");
+    b->cus    = cus;
     b->grid   = (std::max(1, cus * grid_per_cu) + 7) / 8 * 8; // xcd_walk needs a multiple of 8
     b->num_p1 = num_p1;
     b->num_th = num_th;
@@ -261,6 +265,20 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     b->th_lds  = sizeof(ThTensor) + static_cast<std::size_t>(max_blk_nnz) * 8 * kThWaves;
     fx::require(b->p1_lds <= 160 * 1024 && b->th_lds <= 160 * 1024, FX_ERR_UNSUPPORTED,
                 "fx_batch_create: LDS tile exceeds the workgroup limit");
+    // each phase's grid covers at least every workgroup slot its kernel can
+    // hold resident, so no phase runs below its occupancy
+    auto occ = [&](const void* k, int threads, std::size_t lds) {
+      int n = 0;
+      fx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, lds),
+                    "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+      return (std::max(b->grid, cus * n) + 7) / 8 * 8;
+    };
+    b->pgrid[kP1Asm]      = occ(reinterpret_cast<const void*>(k_batch_p1_assemble), kP1Block, b->p1_lds);
+    b->pgrid[kThAsm]      = occ(reinterpret_cast<const void*>(k_batch_th_assemble), kThLanes * kThWaves,
+                                b->th_lds);
+    b->pgrid[kP1Vjp]      = occ(reinterpret_cast<const void*>(k_batch_p1_vjp), kP1Block, 0);
+    b->pgrid[kThVjpElem]  = occ(reinterpret_cast<const void*>(k_batch_th_vjp_elem), kThLanes, 0);
+    b->pgrid[kThVjpGather] = occ(reinterpret_cast<const void*>(k_batch_vjp_gather), 256, 0);
     b->d_p1_asm = upload(pa);
     b->d_p1_vjp = upload(pv);
     b->d_th_asm = upload(ta);
