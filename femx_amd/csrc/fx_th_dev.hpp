@@ -386,35 +386,22 @@ __device__ __forceinline__ void th_velocity_rows(const ThTensor& T, const ThElem
   }
   Rx = 0.0, Ry = 0.0;
   const double* ta = T.ab[a];
-  // L_b = nu A_ab + C_ab for the six column nodes b: independent FMA chains
-  // advanced three at a time (k outside, b inside), each in the order
-  // stiffness, then the twelve convection terms.
-  double L[6];
 #pragma unroll
   for (int b = 0; b < 6; ++b)
   {
     const double* e = ta + 20 * b;
-    L[b]            = fma(G00, e[16], fma(G11, e[17], G01 * e[18]));
-  }
+    double        L = fma(G00, e[16], fma(G11, e[17], G01 * e[18]));
 #pragma unroll
-  for (int b0 = 0; b0 < 6; b0 += 3)
-#pragma unroll
-    for (int k = 0; k < 12; ++k)
-#pragma unroll
-      for (int b = b0; b < b0 + 3; ++b) L[b] = fma(V[k], ta[20 * b + k], L[b]);
-#pragma unroll
-  for (int b = 0; b < 6; ++b)
-  {
-    const double* e = ta + 20 * b;
-    double        M[4];
+    for (int k = 0; k < 12; ++k) L = fma(V[k], e[k], L);
+    double M[4];
 #pragma unroll
     for (int cd = 0; cd < 4; ++cd) M[cd] = fma(W[cd][0], e[12], fma(W[cd][1], e[13], W[cd][2] * e[14]));
-    Kx[2 * b]     = L[b] + M[0];
+    Kx[2 * b]     = L + M[0];
     Kx[2 * b + 1] = M[1];
     Ky[2 * b]     = M[2];
-    Ky[2 * b + 1] = L[b] + M[3];
-    Rx            = fma(L[b], s[2 * b], Rx);
-    Ry            = fma(L[b], s[2 * b + 1], Ry);
+    Ky[2 * b + 1] = L + M[3];
+    Rx            = fma(L, s[2 * b], Rx);
+    Ry            = fma(L, s[2 * b + 1], Ry);
   }
   const double* pa = T.pa[a];
 #pragma unroll
