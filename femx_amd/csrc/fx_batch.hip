@@ -98,7 +98,9 @@ __global__ __launch_bounds__(kP1Block) void k_batch_p1_vjp(const P1VjpArgs* __re
 // Each wave walks the work list on its own (xcd_walk with one walker per
 // wave) with its own row tile of `tile` doubles; the workgroup shares the
 // tables.
-__global__ __launch_bounds__(kThLanes* kThWaves) void k_batch_th_assemble(
+// At most 256 VGPRs (2 waves per SIMD): a register count past 256 would
+// halve the occupancy, which costs far more than a few spills.
+__global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_th_assemble(
     const ThAsmArgs* __restrict__ inst, const int2* __restrict__ work, int nwork, int tile)
 {
   extern __shared__ double smem[];

@@ -8,7 +8,9 @@ using namespace fxk;
 
 // kThWaves waves per workgroup, one row block each, consecutive row blocks
 // per XCD; each wave has a row tile of `tile` doubles after the tables.
-__global__ __launch_bounds__(kThLanes* kThWaves) void k_th_assemble(ThAsmArgs a, int num_blocks,
+// At most 256 VGPRs (2 waves per SIMD): a register count past 256 would
+// halve the occupancy, which costs far more than a few spills.
+__global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_th_assemble(ThAsmArgs a, int num_blocks,
                                                                     int tile)
 {
   extern __shared__ double smem[];
