@@ -30,6 +30,7 @@ _SIGS = {
     "fx_map_export_rowinc": ([_vp, _vp, _vp, _vp], C.c_int),
     "fx_map_export_rowinc15": ([_vp, _vp, _vp, _vp], C.c_int),
     "fx_map_th_blocks": ([_vp, _i32, _i32, _vp, _vp, _vp, _vp], C.c_int),
+    "fx_map_th_lanes": ([_vp, _i32, _vp, _vp, _vp], C.c_int),
     "fx_map_gls_plan": ([_vp, _i32] + [_vp] * 8, C.c_int),
     "fx_map_destroy": ([_vp], None),
     "fx_csr_transpose_map": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp], C.c_int),
@@ -53,7 +54,7 @@ _SIGS = {
     "fx_p1_assemble": ([_i32, _i32, _i32, _i32] + [_vp] * 4 + [C.c_int] + [_vp] * 7 + [_vp],
                        C.c_int),
     "fx_p1_vjp": ([_i32, _i32] + [_vp] * 11 + [_vp], C.c_int),
-    "fx_th_assemble": ([_i32] * 4 + [_vp] * 5 + [_f64] + [_vp] * 9 + [_vp], C.c_int),
+    "fx_th_assemble": ([_i32] * 4 + [_vp] * 5 + [_f64] + [_vp] * 10 + [_vp], C.c_int),
     "fx_csr_matvec_multi": ([_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32, _f64,
                              _f64, _vp], C.c_int),
     "fx_vec_div": ([_i32, _vp, _vp, _vp, _vp], C.c_int),
@@ -85,7 +86,7 @@ class THInstance(C.Structure):
                 + [("nu", _f64)]
                 + [(n, _vp) for n in ("nu_e", "inc_off", "inc_elem", "inc_meta", "blk_row",
                                       "blk_flags", "row_ptr", "vals", "res", "elem_inc", "lam",
-                                      "bc_mask", "jt", "g", "work")])
+                                      "bc_mask", "jt", "g", "work", "blk_lanes")])
 
 
 class MapInfo(C.Structure):

@@ -115,7 +115,7 @@ class THCase:
             ptr(g.vconn3), ptr(g.dofs15), ptr(self.state), self.nu, None, ptr(p.inc_off),
             ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row), ptr(p.blk_flags), ptr(rp),
             ptr(self.jac.vals), ptr(self.res), ptr(p.elem_inc), ptr(self.lam), ptr(self.bmap.d_mask),
-            ptr(self.jt), ptr(self.g), ptr(p.work))
+            ptr(self.jt), ptr(self.g), ptr(p.work), ptr(p.blk_lanes))
 
     def plain(self, what):
         op = ns.TaylorHoodNS(self.nu)

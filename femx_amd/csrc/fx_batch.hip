@@ -248,8 +248,8 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
                   FX_ERR_ARG, "fx_batch_create: Taylor-Hood instance lacks an array");
       max_blk_nnz = std::max(max_blk_nnz, q.max_block_nnz);
       ta.push_back({q.xv, q.yv, q.vconn3, q.dofs15, q.state, q.nu, q.nu_e, q.inc_off, q.inc_elem,
-                    reinterpret_cast<const uint4*>(q.inc_meta), q.blk_row, q.blk_flags, q.row_ptr,
-                    q.vals, q.res});
+                    reinterpret_cast<const uint4*>(q.inc_meta), q.blk_row, q.blk_flags, q.blk_lanes,
+                    q.row_ptr, q.vals, q.res});
       for (int k = 0; k < q.num_blocks; ++k) w[kThAsm].push_back(make_int2(i, k));
       const bool vjp = q.lam && q.jt && q.work && q.elem_inc;
       tv.push_back({q.num_elems, q.xv, q.yv, q.vconn3, q.dofs15, q.state, q.nu, q.nu_e, q.lam,

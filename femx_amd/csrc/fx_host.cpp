@@ -452,6 +452,45 @@ int fx_map_th_blocks(const fx_map* m, Index lanes, Index max_nnz, Index* num_blo
   });
 }
 
+int fx_map_th_lanes(const fx_map* m, Index num_blocks, const Index* blk_row, const Index* blk_flags,
+                    Index* blk_lanes)
+{
+  return fx::guard([&] {
+    fx::require(m && num_blocks >= 0 && (num_blocks == 0 || (blk_row && blk_flags && blk_lanes)),
+                FX_ERR_ARG, "fx_map_th_lanes: invalid arguments");
+    constexpr int L = 64;
+    std::vector<Index> deg;
+    for (Index b = 0; b < num_blocks; ++b)
+    {
+      const Index rs = blk_row[b], re = blk_row[b + 1], step = blk_flags[b] ? 2 : 1;
+      fx::require(rs >= 0 && re > rs && re <= m->num_res && (re - rs) % step == 0, FX_ERR_ARG,
+                  "fx_map_th_lanes: blk_row is not a block partition of the rows");
+      deg.clear();
+      for (Index r = rs; r < re; r += step) deg.push_back(m->inc_off[r + 1] - m->inc_off[r]);
+      fx::require(static_cast<Index>(deg.size()) <= L, FX_ERR_UNSUPPORTED,
+                  "fx_map_th_lanes: a block holds more than 64 items");
+      Index dmax = 1;
+      for (Index d : deg) dmax = std::max(dmax, d);
+      // fewest rounds whose lane demand fits the wave (R = dmax always fits)
+      Index R = 1;
+      for (;; ++R)
+      {
+        Index need = 0;
+        for (Index d : deg) need += std::max<Index>(1, (d + R - 1) / R);
+        if (need <= L || R >= dmax) break;
+      }
+      Index* out  = blk_lanes + static_cast<std::int64_t>(L) * b;
+      Index  lane = 0;
+      for (Index i = 0; i < static_cast<Index>(deg.size()); ++i)
+      {
+        const Index k = std::max<Index>(1, (deg[i] + R - 1) / R);
+        for (Index j = 0; j < k; ++j) out[lane++] = i | (j << 8) | (k << 16);
+      }
+      for (; lane < L; ++lane) out[lane] = -1;
+    }
+  });
+}
+
 // Node plan of the GLS P1/P1 kernel.  A node's three rows are read off the
 // element DOF tables (local rows 2k, 2k+1 and 6+k of local node k); every
 // element must agree, and every row must belong to exactly one node.  Blocks

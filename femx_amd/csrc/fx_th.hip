@@ -41,8 +41,8 @@ int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks, fx_in
                    const double* xv, const double* yv, const fx_index* vconn3,
                    const fx_index* dofs15, const double* state, double nu, const double* nu_e,
                    const fx_index* inc_off, const fx_index* inc_elem, const fx_index* inc_meta,
-                   const fx_index* blk_row, const fx_index* blk_flags, const fx_index* row_ptr,
-                   double* vals, double* res, void* stream)
+                   const fx_index* blk_row, const fx_index* blk_flags, const fx_index* blk_lanes,
+                   const fx_index* row_ptr, double* vals, double* res, void* stream)
 {
   return fx::guard([&] {
     fx::require(ndof >= 0 && num_elems >= 0 && num_blocks >= 0 && max_block_nnz >= 0, FX_ERR_ARG,
@@ -62,7 +62,7 @@ int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks, fx_in
     fx::require(lds <= 160 * 1024, FX_ERR_UNSUPPORTED, "fx_th_assemble: row tile exceeds LDS");
     const ThAsmArgs a{xv,      yv,       vconn3,   dofs15,   state,
                       nu,      nu_e,     inc_off,  inc_elem, reinterpret_cast<const uint4*>(inc_meta),
-                      blk_row, blk_flags, row_ptr, vals,     res};
+                      blk_row, blk_flags, blk_lanes, row_ptr, vals, res};
     hipLaunchKernelGGL(k_th_assemble, dim3(fx::blocks_for(num_blocks, kThWaves)),
                        dim3(kThLanes * kThWaves), lds, fx::as_stream(stream), a, num_blocks, tile);
     fx::check_launch("fx_th_assemble");

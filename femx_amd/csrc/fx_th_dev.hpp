@@ -457,7 +457,6 @@ __device__ __forceinline__ void th_add_rows(double* ax, double* ay, uint4 meta, 
   }
 }
 
-__device__ __forceinline__ double shfl1(double v) { return __shfl_xor(v, 1); }
 
 struct ThAsmArgs
 {
@@ -473,6 +472,7 @@ struct ThAsmArgs
   const uint4*  inc_meta;
   const int*    blk_row;
   const int*    blk_flags;
+  const int*    blk_lanes; // 64 per block (fx_map_th_lanes), or null: two lanes per item
   const int*    row_ptr;
   double*       vals;
   double*       res;
@@ -489,13 +489,16 @@ __device__ __forceinline__ ThRaw th_load_raw(const ThAsmArgs& a, const ThDofs& D
   return R;
 }
 
-// One row block per wave.  Two lanes per work item: lane role 0 takes the
-// item's incidences t0, t0+2, ..., role 1 takes t0+1, t0+3, ...; each round
-// role 0 adds its element rows into the tile, then role 1 adds its own, so
-// every CSR value receives its element contributions in ascending element
-// order, in program order, with no atomics.  The residual sums take the
-// partner's value by a cross-lane swap.  `acc` is the wave's own row tile;
-// the waves of a workgroup share only the tables.
+// One row block per wave.  A block's items share its 64 lanes by the lane
+// map (fx_map_th_lanes): item i has k_i consecutive lanes, and its lane of
+// role j takes the item's incidences t0+j, t0+j+k_i, ...  Each round the
+// roles add their element rows into the tile in sub-steps, role 0 first,
+// then role 1, ..., so every CSR value receives its element contributions in
+// ascending element order, in program order (the adds are LDS atomics only
+// to skip the read round trip; no two lanes add to one entry at once).  The
+// residual sums take the other roles' values by cross-lane reads, in the
+// same order.  Without a lane map an item has two lanes.  `acc` is the
+// wave's own row tile; the waves of a workgroup share only the tables.
 //
 // Loads are unconditional: a lane whose next incidence lies past the row's
 // end loads the row's last incidence instead (always valid) and discards the
@@ -511,9 +514,24 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
   for (int i = lane; i < bn; i += kThLanes) acc[i] = 0.0;
   wave_sync();
 
-  const int role   = lane & 1;
   const int nitems = pairs ? (re - rs) >> 1 : re - rs;
-  for (int item = lane >> 1; item < nitems; item += kThLanes / 2)
+  int       item, role, k, step;
+  if (a.blk_lanes)
+  {
+    const int lm = a.blk_lanes[kThLanes * b + lane];
+    item = lm < 0 ? nitems : lm & 0xff;
+    role = (lm >> 8) & 0xff;
+    k    = lm < 0 ? 1 : lm >> 16;
+    step = nitems; // one item per lane
+  }
+  else
+  {
+    item = lane >> 1, role = lane & 1, k = 2, step = kThLanes / 2;
+  }
+  int kmax = k; // roles of the widest item: the sub-steps of a round
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+  for (; item < nitems; item += step)
   {
     const int r  = rs + (pairs ? 2 * item : item);
     double*   ax = acc + (a.row_ptr[r] - base);
@@ -522,20 +540,20 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
     const int t0 = a.inc_off[r], t1 = a.inc_off[r + 1], tl = t1 - 1;
     if (t0 < t1)
     {
-      // software pipeline over this lane's incidences t0+role, t0+role+2, ...:
+      // software pipeline over this lane's incidences t0+role, t0+role+k, ...:
       // values of the next round and tables of the one after are in flight
       // while the current round computes
       ThIdx  ic = th_load_idx(a.inc_elem, a.inc_meta, min(t0 + role, tl));
-      ThIdx  in = th_load_idx(a.inc_elem, a.inc_meta, min(t0 + role + 2, tl));
+      ThIdx  in = th_load_idx(a.inc_elem, a.inc_meta, min(t0 + role + k, tl));
       ThRaw  ra = th_load_raw(a, th_load_dofs(a.dofs, a.vconn, ic.e), ic.e), rb;
       ThDofs dn = th_load_dofs(a.dofs, a.vconn, in.e);
-      // one round: compute and apply incidence t+role from `cur` while the
-      // next round's values land in `nxt` (the loop alternates the two
-      // buffers, so no values are copied between rounds)
+      // one round (incidences t .. t+k-1 of the item): compute this lane's
+      // from `cur` while the next round's values land in `nxt` (the loop
+      // alternates the two buffers, so no values are copied between rounds)
       auto round = [&](int t, const ThRaw& cur, ThRaw& nxt) {
         const int   tm   = t + role;
         const bool  have = tm < t1;
-        const ThIdx in2  = th_load_idx(a.inc_elem, a.inc_meta, min(tm + 4, tl));
+        const ThIdx in2  = th_load_idx(a.inc_elem, a.inc_meta, min(tm + 2 * k, tl));
         nxt              = th_load_raw(a, dn, in.e);
         double      K0[15], K1[15], R0, R1 = 0.0;
         const uint4 meta = ic.meta;
@@ -556,22 +574,26 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
           else
             th_pressure_row(T, E, li - 12, K0, R0);
         }
-        // Apply in element order: role 0 (incidence t) first, then role 1
-        // (incidence t+1), with the wave's LDS accesses kept in program order.
+        // Apply in element order: role 0 (incidence t), then role 1 (t+1),
+        // ..., with the wave's LDS accesses kept in program order.
+        for (int j = 0; j < kmax; ++j)
+        {
+          if (role == j && have && a.vals) th_add_rows(ax, ay, meta, K0, K1, pairs);
+          wave_sync();
+        }
         if (role == 0)
         {
           sx = sx + R0;
           if (pairs) sy = sy + R1;
-          if (a.vals) th_add_rows(ax, ay, meta, K0, K1, pairs);
         }
-        wave_sync();
-        if (role == 1 && have && a.vals) th_add_rows(ax, ay, meta, K0, K1, pairs);
-        wave_sync();
-        const double pR0 = shfl1(R0), pR1 = shfl1(R1);
-        if (role == 0 && t + 1 < t1)
+        for (int j = 1; j < kmax; ++j)
         {
-          sx = sx + pR0;
-          if (pairs) sy = sy + pR1;
+          const double pR0 = __shfl(R0, lane + j), pR1 = __shfl(R1, lane + j);
+          if (role == 0 && j < k && t + j < t1)
+          {
+            sx = sx + pR0;
+            if (pairs) sy = sy + pR1;
+          }
         }
         dn = th_load_dofs(a.dofs, a.vconn, in2.e);
         ic = in;
@@ -580,9 +602,9 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
       for (int t = t0;;)
       {
         round(t, ra, rb);
-        if ((t += 2) >= t1) break;
+        if ((t += k) >= t1) break;
         round(t, rb, ra);
-        if ((t += 2) >= t1) break;
+        if ((t += k) >= t1) break;
       }
     }
     if (role == 0)

@@ -327,7 +327,7 @@ struct DeviceAssemblyMap
 {
   fx_map_info                             info{};
   std::shared_ptr<const DeviceCsrPattern> pattern;
-  DeviceIndexVector                       inc_off, inc, inc_elem, inc_meta, blk_row, blk_flags;
+  DeviceIndexVector inc_off, inc, inc_elem, inc_meta, blk_row, blk_flags, blk_lanes;
   Index                                   num_blocks = 0, max_block_nnz = 0;
 };
 
@@ -355,11 +355,14 @@ inline void copy(const HostAssemblyMap& src, DeviceAssemblyMap& dst, HipContext&
     std::vector<Index> br(nb + 1), bf(nb);
     check(fx_map_th_blocks(src.handle(), FX_TH_BLOCK_ITEMS, FX_TH_TILE_NNZ, &nb, &mx, br.data(),
                            bf.data()));
+    std::vector<Index> bl(64 * static_cast<std::size_t>(nb));
+    check(fx_map_th_lanes(src.handle(), nb, br.data(), bf.data(), bl.data()));
     dst.inc_off.assign(off, ctx);
     dst.inc_elem.assign(el, ctx);
     dst.inc_meta.assign(me, ctx);
     dst.blk_row.assign(br, ctx);
     dst.blk_flags.assign(bf, ctx);
+    dst.blk_lanes.assign(bl, ctx);
     dst.num_blocks    = nb;
     dst.max_block_nnz = mx;
   }
@@ -535,7 +538,8 @@ inline void assemble(const TaylorHoodNS& op, const DeviceTHGeometry& geom, const
   check(fx_th_assemble(geom.ndof, geom.num_elems, map.num_blocks, map.max_block_nnz, geom.xv.data(),
                        geom.yv.data(), geom.vconn3.data(), geom.dofs15.data(), state.data(), op.nu,
                        op.nu_e, map.inc_off.data(), map.inc_elem.data(), map.inc_meta.data(),
-                       map.blk_row.data(), map.blk_flags.data(), map.pattern->rowPtrData(),
+                       map.blk_row.data(), map.blk_flags.data(), map.blk_lanes.data(),
+                       map.pattern->rowPtrData(),
                        jac ? jac->valsData() : nullptr, res.data(), ctx.raw()));
 }
 

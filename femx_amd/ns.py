@@ -87,6 +87,11 @@ class THPlan:
         _lib.call("fx_map_th_blocks", *args, C.byref(nb), C.byref(mx), ptr(blk_row), ptr(blk_flags))
         self.num_blocks, self.max_block_nnz = nb.value, mx.value
         self.h_blk_row, self.h_blk_flags = blk_row, blk_flags[: nb.value]
+        # each block's 64 lanes shared out so its items finish together
+        lanes = np.empty(max(64 * nb.value, 1), np.int32)
+        _lib.call("fx_map_th_lanes", amap._handle(), nb.value, ptr(blk_row), ptr(blk_flags),
+                  ptr(lanes))
+        self.h_blk_lanes = lanes[: 64 * nb.value]
         t = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
         self.inc_off, self.inc_elem = t(off), t(elem[: info.num_inc])
         self.inc_meta = t(meta[: 4 * info.num_inc])
@@ -95,6 +100,7 @@ class THPlan:
         self.elem_inc = t(slots[: 15 * amap.num_elems])
         self.work = torch.empty(15 * amap.num_elems, dtype=torch.float64, device=dev)
         self.blk_row, self.blk_flags = t(blk_row), t(self.h_blk_flags)
+        self.blk_lanes = t(self.h_blk_lanes)
 
 
 class TaylorHoodNS:
@@ -146,7 +152,8 @@ def assemble_ns(op: TaylorHoodNS, geom: THGeometry, amap: AssemblyMap, state, re
     _lib.call("fx_th_assemble", geom.ndof, geom.num_elems, p.num_blocks, p.max_block_nnz,
               ptr(geom.xv), ptr(geom.yv), ptr(geom.vconn3), ptr(geom.dofs15), ptr(state), op.nu,
               ptr(op.nu_e), ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row),
-              ptr(p.blk_flags), ptr(rp), ptr(jac.vals) if jac is not None else None, ptr(res),
+              ptr(p.blk_flags), ptr(p.blk_lanes), ptr(rp),
+              ptr(jac.vals) if jac is not None else None, ptr(res),
               stream_handle(stream))
     return res
 

@@ -120,6 +120,15 @@ int fx_map_export_rowinc15(const fx_map* map, fx_index* h_inc_off, fx_index* h_i
 #define FX_TH_TILE_NNZ 0     /* default tile bound (0: none; items bound the tile) */
 int fx_map_th_blocks(const fx_map* map, fx_index lanes, fx_index max_nnz, fx_index* num_blocks,
                      fx_index* max_block_nnz, fx_index* h_blk_row, fx_index* h_blk_flags);
+/* Lane map of those blocks (host): each block's 64 lanes shared out so its
+ * items finish together.  With d_i the incidence count of item i, the block
+ * runs R rounds, the smallest R with sum_i ceil(d_i / R) <= 64, and item i
+ * gets k_i = max(1, ceil(d_i / R)) consecutive lanes; its lane of role j
+ * (0 <= j < k_i) takes incidences j, j + k_i, ...  h_blk_lanes[64 *
+ * num_blocks]: item | role << 8 | k_i << 16 per lane, -1 for an idle lane.
+ * (The map that fx_th_assemble uses when given none is two lanes per item.) */
+int fx_map_th_lanes(const fx_map* map, fx_index num_blocks, const fx_index* h_blk_row,
+                    const fx_index* h_blk_flags, fx_index* h_blk_lanes);
 /* Node plan of the GLS P1/P1 Navier-Stokes kernel (fx_gls_assemble) over a
  * map whose elements have 9 local DOFs in NavierOperator order -- u0 v0 u1
  * v1 u2 v2 p0 p1 p2 (femx/model/ns/Components.hpp:247-256) -- for the mesh
@@ -302,7 +311,8 @@ int fx_p1_vjp(fx_index num_nodes, fx_index num_elems, const double* d_x, const d
  * element when non-null), vertex coordinates xv/yv, vconn3[3E] vertex ids,
  * dofs15[15E] element DOFs (fx_th_dofs order), tables from
  * fx_map_export_rowinc15 and fx_map_th_blocks (at most FX_TH_BLOCK_ITEMS
- * items per block).  vals may be NULL (residual only).
+ * items per block); blk_lanes from fx_map_th_lanes, or NULL for two lanes
+ * per item.  vals may be NULL (residual only).
  * Sums over elements in ascending element order, no atomics; arithmetic uses
  * fused multiply-adds. */
 int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks,
@@ -311,7 +321,8 @@ int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks,
                    double nu, const double* d_nu_e, const fx_index* d_inc_off,
                    const fx_index* d_inc_elem, const fx_index* d_inc_meta,
                    const fx_index* d_blk_row, const fx_index* d_blk_flags,
-                   const fx_index* d_row_ptr, double* d_vals, double* d_res, void* stream);
+                   const fx_index* d_blk_lanes, const fx_index* d_row_ptr, double* d_vals,
+                   double* d_res, void* stream);
 /* Navier-Stokes adjoint VJP: jt = J^T lam over the row-replaced Newton
  * Jacobian (lam zeroed on constrained rows, passed through there; bc_mask may
  * be NULL) and, when g is non-null, g[e] = lam~_e^T dR_e/dnu_e, the
@@ -418,6 +429,7 @@ typedef struct
   double*       jt;
   double*       g;
   double*       work; /* 15 * num_elems doubles */
+  const fx_index* blk_lanes; /* fx_map_th_lanes, or NULL for two lanes per item */
 } fx_th_instance;
 
 typedef struct fx_batch fx_batch;

@@ -58,6 +58,42 @@ def test_p1_assemble_bitwise_vs_oracle(n):
     assert close(host(jac3.vals), ov) and close(host(res3), orr)
 
 
+def scrambled_p1(n, seed):
+    """The structured P1 mesh with its nodes relabelled, its elements shuffled
+    and each element's vertex list rotated (orientation kept): every local
+    index meets every kind of neighbourhood, in no particular order."""
+    rng = np.random.default_rng(seed)
+    x, y, conn = O.mesh_p1(n)
+    conn = conn.reshape(-1, 3)
+    perm = rng.permutation(x.size)  # new label of each old node
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(x.size)
+    conn = perm[conn][rng.permutation(conn.shape[0])]
+    shift = rng.integers(0, 3, conn.shape[0])
+    conn = np.stack([np.roll(c, s) for c, s in zip(conn, shift)]).astype(np.int32)
+    return A.Geometry(x[inv], y[inv], conn, DEV)
+
+
+@pytest.mark.parametrize("n,seed", [(5, 1), (16, 2)])
+def test_p1_scrambled_mesh_bitwise_vs_oracle(n, seed):
+    """Assembly on a relabelled, shuffled, rotated mesh: the lane's own node
+    is at every local index of its elements, in every position of its row."""
+    geom = scrambled_p1(n, seed)
+    amap = A.make_assembly_map(geom.num_nodes, geom.num_nodes, geom.h_conn)
+    N, E = geom.num_nodes, geom.num_elems
+    kappa = np.exp(0.5 * RNG.standard_normal(E))
+    f = RNG.standard_normal(N)
+    u = RNG.standard_normal(N)
+    jac = CsrMatrix(amap.pattern, DEV)
+    res = torch.empty(N, dtype=torch.float64, device=DEV)
+    A.assemble(A.PoissonP1(dev(kappa), dev(f)), geom, amap, dev(u), res, jac)
+    _, jm = amap.jac()
+    ov, orr = O.p1_assemble(geom.h_x, geom.h_y, geom.h_conn.reshape(-1), kappa, f, u, jm,
+                            amap.pattern.nnz)
+    assert np.array_equal(host(jac.vals), ov)
+    assert np.array_equal(host(res), orr)
+
+
 @pytest.mark.parametrize("n", [2, 8])
 def test_p1_laplace_bitwise_vs_reference_fixture(n):
     """kappa = 1, f = 0: femx's own PoissonComponents assembly."""

@@ -53,6 +53,24 @@ def test_vjp_matches_oracle(n, with_bc):
     assert np.array_equal(host(jt2), host(jt)) and np.array_equal(host(g2), host(g))
 
 
+@pytest.mark.parametrize("seed", [1, 2])
+def test_vjp_scrambled_mesh_matches_oracle(seed):
+    """The VJP on a relabelled, shuffled, rotated mesh (each node at every
+    local index of its elements), with boundary conditions."""
+    from test_gpu_p1 import scrambled_p1
+    geom = scrambled_p1(12, seed)
+    x, y = geom.h_x, geom.h_y
+    dofs = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+    R = PoissonLogKappaResidual(geom, dofs, dev(RNG.standard_normal(dofs.size)))
+    N, E = geom.num_nodes, geom.num_elems
+    m = 0.5 * RNG.standard_normal(E)
+    u = RNG.standard_normal(N)
+    lam = RNG.standard_normal(N)
+    jt, g = R.vjp(dev(u), dev(m), dev(lam))
+    ojt, og = O.p1_vjp(x, y, geom.h_conn.reshape(-1), m, u, lam, R.bmap.bc_mask)
+    assert close(host(jt), ojt) and close(host(g), og)
+
+
 def test_vjp_full_size_matches_oracle():
     """Config 4 size: 1024^2 cells, 2.1M elements."""
     n = 1024

@@ -3,8 +3,9 @@ row-incidence export and the work-block plan."""
 import numpy as np
 import pytest
 
-from femx_amd import ns
+from femx_amd import _lib, ns
 from femx_amd.assembly import make_assembly_map
+from femx_amd.linalg import ptr
 from oracle import oracle as O
 
 
@@ -95,3 +96,44 @@ def test_fourier_state_and_cavity_bc():
     dofs, vals = ns.cavity_bc(g)
     assert np.all(np.diff(dofs) > 0) and dofs.size == 2 * 4 * 8
     assert vals.sum() == (2 * 4 + 1)
+
+
+@pytest.mark.parametrize("n", [1, 3, 8])
+def test_th_lane_map(n):
+    """fx_map_th_lanes: every item of a block gets k_i >= 1 consecutive lanes
+    with roles 0..k_i-1; R = max_i ceil(d_i / k_i) is the fewest rounds whose
+    lane demand fits 64; unused lanes are -1."""
+    g = ns.structured_th(n, "cpu")
+    amap = make_assembly_map(g.ndof, g.ndof, g.h_dofs15)
+    plan = ns.THPlan(amap, "cpu")
+    off = plan.inc_off.numpy()
+    lanes = plan.h_blk_lanes.reshape(-1, 64)
+    br, bf = plan.h_blk_row, plan.h_blk_flags
+    saved = 0
+    for b in range(plan.num_blocks):
+        step = 2 if bf[b] else 1
+        deg = np.diff(off)[br[b]:br[b + 1]:step]
+        lm = lanes[b]
+        live = lm[lm >= 0]
+        item, role, k = live & 0xFF, (live >> 8) & 0xFF, live >> 16
+        assert np.all(lm[live.size:] == -1)  # live lanes come first
+        ks = []
+        pos = 0
+        for i, d in enumerate(deg):
+            ki = int(k[pos])
+            assert ki >= 1 and np.all(item[pos:pos + ki] == i)
+            assert np.array_equal(role[pos:pos + ki], np.arange(ki)) and np.all(k[pos:pos + ki] == ki)
+            ks.append(ki)
+            pos += ki
+        assert pos == live.size <= 64
+        ks = np.array(ks)
+        R = int(np.max((deg + ks - 1) // ks))
+        need = lambda r: int(np.sum(np.maximum(1, (deg + r - 1) // r)))  # noqa: E731
+        assert need(R) <= 64 and (R == 1 or need(R - 1) > 64 or R >= deg.max())
+        assert np.array_equal(ks, np.maximum(1, (deg + R - 1) // R))
+        saved += int(np.max((deg + 1) // 2)) - R  # rounds saved over two lanes per item
+    assert saved > 0 or n == 1
+    bad = np.array([0, 5, 3], np.int32)
+    with pytest.raises(RuntimeError, match="block partition"):
+        _lib.call("fx_map_th_lanes", amap._handle(), 2, ptr(bad), ptr(np.zeros(2, np.int32)),
+                  ptr(np.empty(128, np.int32)))
