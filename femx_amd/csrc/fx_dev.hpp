@@ -26,4 +26,21 @@ __device__ __forceinline__ void lds_add(double* p, double v)
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
+// Copy n consecutive values of a stream read once (non-temporal loads) into
+// LDS, NT threads cooperating: each thread has four loads in flight before
+// their LDS stores, instead of one load, wait and store per step.
+template <int NT, class D, class S>
+__device__ __forceinline__ void stage_nt(D* __restrict__ dst, const S* __restrict__ src, int n)
+{
+  for (int i = threadIdx.x; i < n; i += 4 * NT)
+  {
+    S v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = i + u * NT < n ? __builtin_nontemporal_load(src + i + u * NT) : S(0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (i + u * NT < n) dst[i + u * NT] = static_cast<D>(v[u]);
+  }
+}
+
 } // namespace fxk

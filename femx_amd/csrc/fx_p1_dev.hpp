@@ -210,8 +210,7 @@ __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, d
   const bool  staged = ib1 - ib0 <= kIncStage;
   for (int i = threadIdx.x; i < bn; i += kP1Block) acc[i] = 0.0;
   if (staged)
-    for (int i = threadIdx.x; i < ib1 - ib0; i += kP1Block)
-      sinc[i] = __builtin_nontemporal_load(reinterpret_cast<const long long*>(a.inc) + ib0 + i);
+    stage_nt<kP1Block>(sinc, reinterpret_cast<const long long*>(a.inc) + ib0, ib1 - ib0);
   __syncthreads();
 
   const int r = r0 + threadIdx.x;
@@ -282,8 +281,7 @@ __device__ __forceinline__ void p1_vjp_block(int blk, const P1VjpArgs& a, long l
   const int  ib0 = a.inc_off[r0], ib1 = a.inc_off[r1];
   const bool staged = ib1 - ib0 <= kIncStage;
   if (staged)
-    for (int i = threadIdx.x; i < ib1 - ib0; i += kP1Block)
-      sinc[i] = __builtin_nontemporal_load(reinterpret_cast<const long long*>(a.inc) + ib0 + i);
+    stage_nt<kP1Block>(sinc, reinterpret_cast<const long long*>(a.inc) + ib0, ib1 - ib0);
   __syncthreads();
   const int j = r0 + threadIdx.x;
   if (j < a.num_rows)

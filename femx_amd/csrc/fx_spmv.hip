@@ -8,7 +8,7 @@
 // a block's rows are staged through LDS with coalesced loads.  The transpose apply gathers each column through the transpose
 // permutation in ascending row order -- the order femx's host matvecT adds
 // into y (MatrixHandler.cpp:177-198) -- instead of scattering with atomics.
-#include "fx_common.hpp"
+#include "fx_dev.hpp"
 
 namespace
 {
@@ -36,11 +36,8 @@ __global__ __launch_bounds__(kRowBlock) void k_spmv(int rows, const int* __restr
   const bool staged = kn <= kStage;
   if (staged)
   {
-    for (int i = threadIdx.x; i < kn; i += kRowBlock)
-    {
-      sv[i] = __builtin_nontemporal_load(vals + k0 + i);
-      sc[i] = __builtin_nontemporal_load(col + k0 + i);
-    }
+    fxk::stage_nt<kRowBlock>(sv, vals + k0, kn);
+    fxk::stage_nt<kRowBlock>(sc, col + k0, kn);
     __syncthreads();
   }
   const int r = r0 + threadIdx.x;
@@ -75,11 +72,8 @@ __global__ __launch_bounds__(kRowBlock) void k_spmv16(int rows, const int* __res
   const bool staged = kn <= kStage;
   if (staged)
   {
-    for (int i = threadIdx.x; i < kn; i += kRowBlock)
-    {
-      sv[i] = __builtin_nontemporal_load(vals + k0 + i);
-      so[i] = __builtin_nontemporal_load(off + k0 + i);
-    }
+    fxk::stage_nt<kRowBlock>(sv, vals + k0, kn);
+    fxk::stage_nt<kRowBlock>(so, off + k0, kn);
     __syncthreads();
   }
   const int r = r0 + threadIdx.x;
@@ -122,12 +116,11 @@ __global__ __launch_bounds__(kRowBlock) void k_spmv_multi(
   if (staged)
   {
     // stage the column index of each entry (rows of this block only)
-    for (int i = threadIdx.x; i < kn; i += kRowBlock)
-    {
-      sv[i] = __builtin_nontemporal_load(vals + k0 + i);
-      sc[i] = col ? __builtin_nontemporal_load(col + k0 + i)
-                  : static_cast<int>(__builtin_nontemporal_load(off + k0 + i));
-    }
+    fxk::stage_nt<kRowBlock>(sv, vals + k0, kn);
+    if (col)
+      fxk::stage_nt<kRowBlock>(sc, col + k0, kn);
+    else
+      fxk::stage_nt<kRowBlock>(sc, off + k0, kn);
     __syncthreads();
   }
   if (rr >= rows) return;

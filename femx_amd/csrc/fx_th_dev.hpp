@@ -732,7 +732,7 @@ __device__ __forceinline__ void vjp_gather_block(int blk, const GatherArgs& a, d
   const bool staged = ni <= kGatherStage;
   if (staged)
   {
-    for (int i = threadIdx.x; i < ni; i += 256) sval[i] = __builtin_nontemporal_load(a.cbuf + i0 + i);
+    stage_nt<256>(sval, a.cbuf + i0, ni);
     __syncthreads();
   }
   const int c = c0 + threadIdx.x;
