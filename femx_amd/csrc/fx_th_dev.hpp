@@ -12,16 +12,16 @@
 // Local DOF order: 2a+c for the 6 P2 velocity nodes, then 12+b for the 3
 // pressure vertices (FESpace::localDof, femx/fem/FESpace.cpp:96-101).
 //
-// Assembly design: two lanes per work item -- the x and y rows of one
-// velocity node (they share every column position and every quadrature-point
-// value) or one single row -- gathering over the row's incident elements in
-// ascending element order.  A 64-lane block owns a contiguous range of rows;
-// their CSR values are accumulated in an LDS tile and written back with
-// coalesced non-temporal stores.  Each CSR value is a sum over its elements
-// in ascending element order with no atomics, so repeated launches give
-// bitwise equal results.  Element geometry is recomputed from the three
-// vertex coordinates; the 7-point shape tables are built once per block in
-// LDS.  Arithmetic uses fused multiply-adds.
+// Assembly design: a work item is the x and y rows of one velocity node (they
+// share every column position and every quadrature-point value) or one
+// single row; its lanes (as many as the block's lane map gives it) gather
+// over the row's incident elements in ascending element order.  A 64-lane
+// block owns a contiguous range of rows; their CSR values are accumulated in
+// an LDS tile and written back with coalesced non-temporal stores.  Each CSR
+// value is a sum over its elements in ascending element order, term after
+// term, so repeated launches give bitwise equal results.  Element geometry is
+// recomputed from the three vertex coordinates; the element-row tensors are
+// staged once per workgroup in LDS.  Arithmetic uses fused multiply-adds.
 //
 // VJP design: two phases.  One lane per element contracts every column of
 // its element matrix with the boundary-masked adjoint through the adjoint's

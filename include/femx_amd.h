@@ -313,8 +313,8 @@ int fx_p1_vjp(fx_index num_nodes, fx_index num_elems, const double* d_x, const d
  * fx_map_export_rowinc15 and fx_map_th_blocks (at most FX_TH_BLOCK_ITEMS
  * items per block); blk_lanes from fx_map_th_lanes, or NULL for two lanes
  * per item.  vals may be NULL (residual only).
- * Sums over elements in ascending element order, no atomics; arithmetic uses
- * fused multiply-adds. */
+ * Each CSR value and residual entry is a sum over its elements in ascending
+ * element order (no global atomics); arithmetic uses fused multiply-adds. */
 int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks,
                    fx_index max_block_nnz, const double* d_xv, const double* d_yv,
                    const fx_index* d_vconn3, const fx_index* d_dofs15, const double* d_state,
