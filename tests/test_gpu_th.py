@@ -51,6 +51,28 @@ def test_th_assemble_matches_oracle(n, per_elem):
     assert np.array_equal(host(res2), host(res))
 
 
+def test_th_lane_map_and_two_lane_default_agree():
+    """fx_th_assemble with the block lane map (fx_map_th_lanes) and without
+    one (two lanes per item) add every entry's terms in the same order: the
+    results are bitwise equal."""
+    from femx_amd import _lib
+    from femx_amd.linalg import ptr, stream_handle
+    g, amap = setup(9)
+    s = dev(RNG.standard_normal(g.ndof))
+    p = ns.THPlan(amap, DEV)
+    rp, _ = amap.pattern.device_arrays(DEV)
+    out = []
+    for lanes in (p.blk_lanes, None):
+        jac = CsrMatrix(amap.pattern, DEV)
+        res = torch.empty(g.ndof, dtype=torch.float64, device=DEV)
+        _lib.call("fx_th_assemble", g.ndof, g.num_elems, p.num_blocks, p.max_block_nnz, ptr(g.xv),
+                  ptr(g.yv), ptr(g.vconn3), ptr(g.dofs15), ptr(s), 0.01, None, ptr(p.inc_off),
+                  ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row), ptr(p.blk_flags), ptr(lanes),
+                  ptr(rp), ptr(jac.vals), ptr(res), stream_handle())
+        out.append((host(jac.vals), host(res)))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_th_full_size_matches_oracle_and_is_deterministic():
     """Config 3: 512^2 cavity, Re = 100, Fourier state with lid."""
     g, amap = setup(512)

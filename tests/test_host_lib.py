@@ -216,6 +216,21 @@ def test_mesh_overflow_is_reported():
         _lib.call("fx_mesh_p1", 40000, C.c_void_p(8), C.c_void_p(8), C.c_void_p(8))
 
 
+def test_gather_offsets_are_checked_before_launch():
+    """The kernels gather through 32-bit byte offsets; arrays past 4 GiB are
+    refused with FX_ERR_OVERFLOW before anything is launched (the pointers
+    below are never dereferenced)."""
+    p = C.c_void_p(8)
+    big = 1 << 29  # 2^29 doubles = 4 GiB
+    with pytest.raises(RuntimeError, match="beyond 4 GiB"):
+        _lib.call("fx_p1_assemble", big, 8, 8, 7, p, p, p, p, 0, p, p, p, p, p, p, p, None)
+    with pytest.raises(RuntimeError, match="beyond 4 GiB"):
+        _lib.call("fx_p1_vjp", 8, big, p, p, p, p, p, p, None, p, p, p, p, None)
+    with pytest.raises(RuntimeError, match="beyond 4 GiB"):
+        _lib.call("fx_th_assemble", 8, (1 << 28) // 15 + 1, 1, 8, p, p, p, p, p, 0.01, None, p, p, p,
+                  p, p, None, p, p, p, None)
+
+
 @pytest.mark.parametrize("name", ["rect", "p1_n3"])
 def test_transpose_pattern_matches_reference(name):
     """MatrixHandler::transpose (compiled femx): the transposed pattern from the
