@@ -138,12 +138,6 @@ struct P1Raw
   double x[3], y[3], u[3], f[3], kv;
 };
 
-__device__ __forceinline__ int3 p1_conn(const int* __restrict__ conn, int e)
-{
-  const int* c = conn + 3 * static_cast<long>(e);
-  return make_int3(c[0], c[1], c[2]);
-}
-
 // The row's own node is vertex li of every element it meets, so its values
 // (o) are loaded once per row and only the element's two other vertices are
 // gathered per incidence; the three are then put back in element order.
