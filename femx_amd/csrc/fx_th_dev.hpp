@@ -521,8 +521,8 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
     const int lm = a.blk_lanes[kThLanes * b + lane];
     item = lm < 0 ? nitems : lm & 0xff;
     role = (lm >> 8) & 0xff;
-    k    = lm < 0 ? 1 : lm >> 16;
-    step = nitems; // one item per lane
+    k    = lm < 0 ? 1 : max(1, lm >> 16); // a stride of at least 1: every round loop ends
+    step = nitems;                        // one item per lane
   }
   else
   {
