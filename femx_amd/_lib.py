@@ -31,7 +31,7 @@ _SIGS = {
     "fx_map_export_rowinc15": ([_vp, _vp, _vp, _vp], C.c_int),
     "fx_map_th_blocks": ([_vp, _i32, _i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_map_th_lanes": ([_vp, _i32, _vp, _vp, _vp], C.c_int),
-    "fx_map_gls_plan": ([_vp, _i32] + [_vp] * 8, C.c_int),
+    "fx_map_gls_plan": ([_vp, _i32] + [_vp] * 9, C.c_int),
     "fx_map_destroy": ([_vp], None),
     "fx_csr_transpose_map": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp], C.c_int),
     "fx_bcmap_create": ([_i32, _vp, _vp, _i32, _vp, C.POINTER(_vp)], C.c_int),
@@ -60,7 +60,7 @@ _SIGS = {
     "fx_vec_div": ([_i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_vec_gather": ([_i32, _vp, _vp, _i32, _vp, _vp], C.c_int),
     "fx_vec_scatter": ([_i32, _vp, _vp, _vp, _i32, _vp], C.c_int),
-    "fx_gls_hist_vjp": ([_i32] * 3 + [_vp] * 5 + [_f64] * 3 + [_i32] * 3 + [_vp] * 6 + [_vp],
+    "fx_gls_hist_vjp": ([_i32] * 3 + [_vp] * 5 + [_f64] * 3 + [_i32] * 3 + [_vp] * 7 + [_vp],
                         C.c_int),
     "fx_gls_assemble": ([_i32] * 5 + [_vp] * 5 + [_f64] * 3 + [_i32] * 2 + [_vp] * 9 + [_vp],
                         C.c_int),

@@ -16,9 +16,15 @@ __global__ __launch_bounds__(64) void k_gls_assemble(GlsAsmArgs a)
   gls_assemble_block(fx::xcd_block(gridDim.x), a, tile);
 }
 
-__global__ __launch_bounds__(64) void k_gls_hist_vjp(GlsVjpArgs a)
+template <int LAG>
+__global__ __launch_bounds__(64) void k_gls_hist_vjp_elem(GlsVjpArgs a)
 {
-  gls_hist_vjp_block(fx::xcd_block(gridDim.x), a);
+  gls_hist_vjp_elem<LAG>(fx::xcd_block(gridDim.x) * 64 + threadIdx.x, a);
+}
+
+__global__ __launch_bounds__(256) void k_gls_hist_vjp_sum(GlsVjpArgs a)
+{
+  gls_hist_vjp_sum(fx::xcd_block(gridDim.x) * 256 + threadIdx.x, a);
 }
 
 } // namespace
@@ -88,8 +94,8 @@ int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                     const double* y, const fx_index* conn3, const fx_index* dofs9,
                     const fx_index* node_rows, double rho, double mu, double dt, fx_index step,
                     fx_index num_hist, fx_index lag, const double* hist, const double* nxt,
-                    const double* adj, const fx_index* inc_off, const fx_index* inc_elem,
-                    double* out, void* stream)
+                    const double* adj, const fx_index* inc_off, const fx_index* inc_slot,
+                    double* out, double* work, void* stream)
 {
   return fx::guard([&] {
     fx::require(num_nodes >= 0 && num_states >= 0 && num_elems >= 0, FX_ERR_ARG,
@@ -105,17 +111,33 @@ int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                 "Time assembly variable block is invalid");
     if (num_states == 0) return;
     fx::require(num_nodes > 0 && x && y && node_rows && hist && nxt && adj && inc_off && out
-                    && (num_elems == 0 || (conn3 && dofs9 && inc_elem)),
+                    && (num_elems == 0 || (conn3 && dofs9 && inc_slot && work)),
                 FX_ERR_ARG, "fx_gls_hist_vjp: null array");
     const std::size_t S = std::size_t(num_states) * 8, H = S * num_hist;
+    const std::size_t W = std::size_t(num_elems) * 6 * 8;
     fx::require(!fx::overlaps(out, S, hist, H) && !fx::overlaps(out, S, nxt, S)
-                    && !fx::overlaps(out, S, adj, S),
+                    && !fx::overlaps(out, S, adj, S) && !fx::overlaps(work, W, out, S)
+                    && !fx::overlaps(work, W, hist, H) && !fx::overlaps(work, W, nxt, S)
+                    && !fx::overlaps(work, W, adj, S),
                 FX_ERR_ALIAS, "Navier history VJP output must not alias its inputs");
-    const unsigned   nb = fx::blocks_for(num_nodes, kGlsBlock);
     const GlsVjpArgs a{num_nodes, num_states, x,   y,   conn3, dofs9,
                        node_rows, step,       num_hist, lag, hist,  nxt,
-                       adj,       gls_const(rho, mu, dt), inc_off, inc_elem, out};
-    hipLaunchKernelGGL(k_gls_hist_vjp, dim3(nb), dim3(64), 0, fx::as_stream(stream), a);
+                       adj,       gls_const(rho, mu, dt), inc_off, inc_slot, out, work,
+                       num_elems};
+    // two phases: per-element columns (partials once per element), then
+    // per-node sums in ascending element order
+    if (num_elems > 0)
+    {
+      const dim3 grid(fx::blocks_for(num_elems, 64));
+      if (lag == 0)
+        hipLaunchKernelGGL(k_gls_hist_vjp_elem<0>, grid, dim3(64), 0, fx::as_stream(stream), a);
+      else if (lag == 1)
+        hipLaunchKernelGGL(k_gls_hist_vjp_elem<1>, grid, dim3(64), 0, fx::as_stream(stream), a);
+      else
+        hipLaunchKernelGGL(k_gls_hist_vjp_elem<2>, grid, dim3(64), 0, fx::as_stream(stream), a);
+    }
+    hipLaunchKernelGGL(k_gls_hist_vjp_sum, dim3(fx::blocks_for(num_nodes, 256)), dim3(256), 0,
+                       fx::as_stream(stream), a);
     fx::check_launch("fx_gls_hist_vjp");
   });
 }

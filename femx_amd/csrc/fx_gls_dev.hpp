@@ -494,8 +494,8 @@ __device__ __forceinline__ void gls_assemble_block(int blk, const GlsAsmArgs& a,
 // for the current state (lag 0) and beta N_k dl/dadv for the previous one,
 // with (alpha, beta) = (1.5, -0.5) when the advection velocity is
 // extrapolated and (1, 0) otherwise; pressure columns are zero.  tau depends on
-// u through |u| and elemLength's streamline direction.  One lane per node
-// incidence forms the node's two velocity columns for both lags.
+// u through |u| and elemLength's streamline direction.  One lane per element
+// forms its three nodes' velocity columns; one lane per node then sums them.
 
 struct GlsVjpArgs
 {
@@ -514,17 +514,26 @@ struct GlsVjpArgs
   const double* adj;
   GlsConst      c;
   const int*    inc_off;
-  const int*    inc_elem;
+  const int*    inc_slot; // [3E]: slot of element e in its node k's list
   double*       out;
+  double*       work; // 2 per incidence slot t: work[2 t + c]
+  int           num_elems;
 };
 
-// Columns (k, 0..1) of node k = li for lag 0 (o0, o1) and lag 1 (p0, p1).
-__device__ __forceinline__ void gls_hvjp_cols(int li, const double (&gx)[3], const double (&gy)[3],
-                                              double jw, const double (&cur)[6],
-                                              const double (&prev)[6], bool extrap,
-                                              const double (&nx)[9], const double (&aj)[9],
-                                              const GlsConst& C, double& o0, double& o1,
-                                              double& p0, double& p1)
+// Columns (k, 0..1) of node k for lag 0 (o0, o1) and lag 1 (p0, p1).  The
+// partials at each quadrature point do not depend on the column's node:
+// gls_hvjp_nodes forms them once and contracts them with each of the
+// element's NK nodes ks[0..NK) (o: the current lag's columns, p: the
+// previous lag's), each node's sum running over the quadrature points in
+// order.
+template <int NK>
+__device__ __forceinline__ void gls_hvjp_nodes(const int (&ks)[NK], const double (&gx)[3],
+                                               const double (&gy)[3], double jw,
+                                               const double (&cur)[6], const double (&prev)[6],
+                                               bool extrap, const double (&nx)[9],
+                                               const double (&aj)[9], const GlsConst& C,
+                                               double (&o0)[NK], double (&o1)[NK],
+                                               double (&p0)[NK], double (&p1)[NK])
 {
   const double al = extrap ? 1.5 : 1.0, be = extrap ? -0.5 : 0.0;
   const double g[6] = {gx[0], gy[0], gx[1], gy[1], gx[2], gy[2]};
@@ -545,8 +554,14 @@ __device__ __forceinline__ void gls_hvjp_cols(int li, const double (&gx)[3], con
       Dpi[d] += g[2 * k + d] * aj[6 + k];
       Dq[d] += g[2 * k + d] * nx[6 + k];
     }
-  const double gkx = gsel3(gx[0], gx[1], gx[2], li), gky = gsel3(gy[0], gy[1], gy[2], li);
-  o0 = o1 = p0 = p1 = 0.0;
+  double gkx[NK], gky[NK];
+#pragma unroll
+  for (int m = 0; m < NK; ++m)
+  {
+    gkx[m] = gsel3(gx[0], gx[1], gx[2], ks[m]);
+    gky[m] = gsel3(gy[0], gy[1], gy[2], ks[m]);
+    o0[m] = o1[m] = p0[m] = p1[m] = 0.0;
+  }
 #pragma unroll
   for (int iq = 0; iq < 3; ++iq)
   {
@@ -657,101 +672,119 @@ __device__ __forceinline__ void gls_hvjp_cols(int li, const double (&gx)[3], con
     for (int c = 0; c < 2; ++c)
 #pragma unroll
       for (int d = 0; d < 2; ++d) PG[c][d] = 0.5 * C.mu * Dl[c][d] + Pag[c] * av[d];
-    const double Nk = gsel3(Nq[0], Nq[1], Nq[2], li);
-    const double d0 = Nk * Pu[0] + gkx * PG[0][0] + gky * PG[0][1] + al * Nk * Padv[0];
-    const double d1 = Nk * Pu[1] + gkx * PG[1][0] + gky * PG[1][1] + al * Nk * Padv[1];
-    o0 += jw * d0;
-    o1 += jw * d1;
-    p0 += jw * be * Nk * Padv[0];
-    p1 += jw * be * Nk * Padv[1];
+#pragma unroll
+    for (int m = 0; m < NK; ++m)
+    {
+      const double Nk = gsel3(Nq[0], Nq[1], Nq[2], ks[m]);
+      const double d0 = Nk * Pu[0] + gkx[m] * PG[0][0] + gky[m] * PG[0][1] + al * Nk * Padv[0];
+      const double d1 = Nk * Pu[1] + gkx[m] * PG[1][0] + gky[m] * PG[1][1] + al * Nk * Padv[1];
+      o0[m] += jw * d0;
+      o1[m] += jw * d1;
+      p0[m] += jw * be * Nk * Padv[0];
+      p1[m] += jw * be * Nk * Padv[1];
+    }
   }
 }
 
-// One block of kGlsBlock nodes, two lanes per node taking alternate
-// incidences; the node's two velocity entries are summed in ascending element
-// order (role 0's term, then role 1's, per round), its pressure entry is 0.
-__device__ __forceinline__ void gls_hist_vjp_block(int blk, const GlsVjpArgs& a)
+// Element inputs of the history VJP: geometry, the current, previous and
+// next states and the adjoint, gathered through the element DOF table.
+struct GlsVjpElem
 {
-  const int  lane   = threadIdx.x & 63;
-  const int  role   = lane & 1;
-  const int  n      = blk * kGlsBlock + (lane >> 1);
-  const bool live   = n < a.num_nodes && a.node_rows[3 * n] >= 0;
+  int    v[3];
+  double gx[3], gy[3], jw, cur[6], prev[6], nx[9], aj[9];
+};
+
+__device__ __forceinline__ void gls_vjp_load(GlsVjpElem& E, const GlsVjpArgs& a, int e, bool extrap)
+{
+  int    d[9];
+  double xe[3], ye[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) E.v[k] = a.conn[3 * e + k];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) d[k] = a.dofs[9 * e + k];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+  {
+    xe[k] = a.x[E.v[k]];
+    ye[k] = a.y[E.v[k]];
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+  {
+    E.cur[k]  = a.hist[d[k]];
+    E.prev[k] = extrap ? a.hist[a.num_states + d[k]] : 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 9; ++k)
+  {
+    E.nx[k] = a.nxt[d[k]];
+    E.aj[k] = a.adj[d[k]];
+  }
+  const double j00 = xe[1] - xe[0], j01 = xe[2] - xe[0], j10 = ye[1] - ye[0], j11 = ye[2] - ye[0];
+  const double det = j00 * j11 - j01 * j10, inv_det = 1.0 / det;
+  const double i00 = j11 * inv_det, i01 = -j01 * inv_det, i10 = -j10 * inv_det, i11 = j00 * inv_det;
+  E.gx[0] = -i00 - i10, E.gx[1] = i00, E.gx[2] = i10;
+  E.gy[0] = -i01 - i11, E.gy[1] = i01, E.gy[2] = i11;
+  E.jw    = fabs(det) * (1.0 / 6.0);
+}
+
+// Two-phase form, phase 1 (one lane per element): the partials once per
+// quadrature point, contracted with each of the element's three nodes; node
+// k's two columns for the requested lag go to the element's slot t =
+// inc_slot[3 e + k] in that node's incidence list, work[2 t + c], so that each
+// node's terms lie contiguous and in ascending element order.  (Evaluated
+// per node incidence instead, the same partials would be formed three times
+// per element.)
+// LAG: the requested lag, 0 or 1, fixed at compile time so that the other
+// lag's columns (and for lag 1 the state and gradient partials) are not
+// formed; 2 stands for any older lag, whose columns are zero.
+template <int LAG>
+__device__ __forceinline__ void gls_hist_vjp_elem(int e, const GlsVjpArgs& a)
+{
+  if (e >= a.num_elems) return;
   const bool extrap = a.step > 0 && a.num_hist > 1;
-  int        t0 = 0, t1 = 0;
-  if (live)
+  double     c[6]   = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  int        slot[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) slot[k] = a.inc_slot[3 * e + k];
+  if constexpr (LAG < 2)
   {
-    t0 = a.inc_off[n];
-    t1 = a.inc_off[n + 1];
-  }
-  double s0 = 0.0, s1 = 0.0;
-  int    rounds = (t1 - t0 + 1) >> 1;
+    GlsVjpElem E;
+    gls_vjp_load(E, a, e, extrap);
+    const int ks[3] = {0, 1, 2};
+    double    o0[3], o1[3], p0[3], p1[3];
+    gls_hvjp_nodes<3>(ks, E.gx, E.gy, E.jw, E.cur, E.prev, extrap, E.nx, E.aj, a.c, o0, o1, p0, p1);
 #pragma unroll
-  for (int off = 32; off >= 1; off >>= 1) rounds = max(rounds, __shfl_xor(rounds, off));
-  for (int r = 0; r < rounds; ++r)
-  {
-    const int  t    = t0 + 2 * r + role;
-    const bool have = t < t1;
-    double     c0 = 0.0, c1 = 0.0;
-    if (have && a.lag < 2)
+    for (int k = 0; k < 3; ++k)
     {
-      const int e = a.inc_elem[t];
-      int       vv[3], d[9];
-      double    xe[3], ye[3], cur[6], prev[6], nx[9], aj[9];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) vv[k] = a.conn[3 * e + k];
-      // the node's local index, read off the connectivity (no metadata load)
-      const int li = vv[0] == n ? 0 : (vv[1] == n ? 1 : 2);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) d[k] = a.dofs[9 * e + k];
-#pragma unroll
-      for (int k = 0; k < 3; ++k)
-      {
-        xe[k] = a.x[vv[k]];
-        ye[k] = a.y[vv[k]];
-      }
-#pragma unroll
-      for (int k = 0; k < 6; ++k)
-      {
-        cur[k]  = a.hist[d[k]];
-        prev[k] = extrap ? a.hist[a.num_states + d[k]] : 0.0;
-      }
-#pragma unroll
-      for (int k = 0; k < 9; ++k)
-      {
-        nx[k] = a.nxt[d[k]];
-        aj[k] = a.adj[d[k]];
-      }
-      const double j00 = xe[1] - xe[0], j01 = xe[2] - xe[0], j10 = ye[1] - ye[0],
-                   j11 = ye[2] - ye[0];
-      const double det = j00 * j11 - j01 * j10, inv_det = 1.0 / det;
-      const double i00 = j11 * inv_det, i01 = -j01 * inv_det, i10 = -j10 * inv_det,
-                   i11 = j00 * inv_det;
-      const double gx[3] = {-i00 - i10, i00, i10};
-      const double gy[3] = {-i01 - i11, i01, i11};
-      const double jw    = fabs(det) * (1.0 / 6.0);
-      double       o0, o1, p0, p1;
-      gls_hvjp_cols(li, gx, gy, jw, cur, prev, extrap, nx, aj, a.c, o0, o1, p0, p1);
-      c0 = a.lag == 0 ? o0 : p0;
-      c1 = a.lag == 0 ? o1 : p1;
-    }
-    const double q0 = __shfl_xor(c0, 1), q1 = __shfl_xor(c1, 1);
-    if (role == 0 && have)
-    {
-      s0 += c0;
-      s1 += c1;
-      if (t + 1 < t1)
-      {
-        s0 += q0;
-        s1 += q1;
-      }
+      c[2 * k]     = LAG == 0 ? o0[k] : p0[k];
+      c[2 * k + 1] = LAG == 0 ? o1[k] : p1[k];
     }
   }
-  if (live && role == 0)
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    *reinterpret_cast<double2*>(a.work + 2 * static_cast<long>(slot[k])) =
+        double2{c[2 * k], c[2 * k + 1]};
+}
+
+// Phase 2 (one lane per node): the node's two columns summed over its
+// incidence slots -- its elements in ascending order -- from zero, term after
+// term.  The pressure row's column is zero.
+__device__ __forceinline__ void gls_hist_vjp_sum(int n, const GlsVjpArgs& a)
+{
+  if (n >= a.num_nodes || a.node_rows[3 * n] < 0) return;
+  double         s0 = 0.0, s1 = 0.0;
+  const double2* w  = reinterpret_cast<const double2*>(a.work);
+  const int      t1 = a.inc_off[n + 1];
+  for (int t = a.inc_off[n]; t < t1; ++t)
   {
-    __builtin_nontemporal_store(s0, a.out + a.node_rows[3 * n]);
-    __builtin_nontemporal_store(s1, a.out + a.node_rows[3 * n + 1]);
-    __builtin_nontemporal_store(0.0, a.out + a.node_rows[3 * n + 2]);
+    const double2 wt = w[t];
+    s0 += wt.x;
+    s1 += wt.y;
   }
+  __builtin_nontemporal_store(s0, a.out + a.node_rows[3 * n]);
+  __builtin_nontemporal_store(s1, a.out + a.node_rows[3 * n + 1]);
+  __builtin_nontemporal_store(0.0, a.out + a.node_rows[3 * n + 2]);
 }
 
 } // namespace fxk

@@ -497,8 +497,8 @@ int fx_map_th_lanes(const fx_map* m, Index num_blocks, const Index* blk_row, con
 // of FX_GLS_BLOCK_NODES consecutive nodes list their rows as runs of
 // consecutive rows, whose CSR ranges are laid end to end in the LDS tile.
 int fx_map_gls_plan(const fx_map* m, Index num_nodes, const Index* conn3, Index* node_rows,
-                    Index* inc_off, Index* inc_elem, Index* inc_meta, Index* blk_runs,
-                    Index* node_tile, Index* max_tile)
+                    Index* inc_off, Index* inc_elem, Index* inc_meta, Index* inc_slot,
+                    Index* blk_runs, Index* node_tile, Index* max_tile)
 {
   return fx::guard([&] {
     fx::require(m && num_nodes >= 0 && node_rows && inc_off && blk_runs && node_tile && max_tile
@@ -556,6 +556,7 @@ int fx_map_gls_plan(const fx_map* m, Index num_nodes, const Index* conn3, Index*
           const Index t = fill[conn3[3 * e + k]]++;
           inc_elem[t]   = e;
           li_of[t]      = k;
+          if (inc_slot) inc_slot[3 * e + k] = t;
         }
     }
     std::vector<std::string> errs(1);

@@ -107,12 +107,13 @@ class GLSPlan:
                  inc_off=np.empty(N + 1, np.int32),
                  inc_elem=np.empty(max(3 * E, 1), np.int32),
                  inc_meta=np.empty(max(24 * E, 8), np.int32),
+                 inc_slot=np.empty(max(3 * E, 1), np.int32),
                  blk_runs=np.empty(max(4 * MAX_RUNS * nb, 4), np.int32),
                  node_tile=np.empty(max(3 * N, 1), np.int32))
         mx = C.c_int32()
         _lib.call("fx_map_gls_plan", amap._handle(), N, ptr(geom.h_conn), ptr(h["node_rows"]),
-                  ptr(h["inc_off"]), ptr(h["inc_elem"]), ptr(h["inc_meta"]), ptr(h["blk_runs"]),
-                  ptr(h["node_tile"]), C.byref(mx))
+                  ptr(h["inc_off"]), ptr(h["inc_elem"]), ptr(h["inc_meta"]), ptr(h["inc_slot"]),
+                  ptr(h["blk_runs"]), ptr(h["node_tile"]), C.byref(mx))
         self.max_tile = mx.value
         self.h = h
         for k, v in h.items():
@@ -188,7 +189,11 @@ def hist_vjp(op: NavierOperator, step, num_hist, lag, amap: AssemblyMap, hist, n
              out=None, stream=None):
     """(dR/dhist_lag)^T adj: TimeResidual::applyJacT with respect to a history
     block (femx/model/ns/NavierStokesModel.cpp:321-363), the product femx
-    takes with Enzyme, here hand-derived.  Returns out (allocated when None)."""
+    takes with Enzyme, here hand-derived.  Returns out (allocated when None).
+    Element columns go to the plan's workspace (6 doubles per element,
+    allocated on first use), then are summed per node.  Calls sharing a plan
+    share its workspace: history VJPs of one map on different streams must
+    not overlap."""
     geom = op.geom
     _require(num_hist > 0 and 0 <= lag < num_hist, "Time assembly variable block is invalid")
     _check_dev_vec(hist, num_hist * geom.num_states, torch.float64, "History VJP history")
@@ -198,10 +203,14 @@ def hist_vjp(op: NavierOperator, step, num_hist, lag, amap: AssemblyMap, hist, n
         out = torch.empty(geom.num_states, dtype=torch.float64, device=geom.device)
     _check_dev_vec(out, geom.num_states, torch.float64, "History VJP output")
     p = _plan(amap, geom)
+    if getattr(p, "vjp_work", None) is None:
+        p.vjp_work = torch.empty(max(6 * geom.num_elems, 1), dtype=torch.float64,
+                                 device=geom.device)
+    work = p.vjp_work
     _lib.call("fx_gls_hist_vjp", geom.num_nodes, geom.num_states, geom.num_elems, ptr(geom.x),
               ptr(geom.y), ptr(geom.conn), ptr(geom.dofs9), ptr(p.node_rows), op.fluid.rho,
               op.fluid.mu, op.dt, int(step), int(num_hist), int(lag), ptr(hist), ptr(nxt),
-              ptr(adj), ptr(p.inc_off), ptr(p.inc_elem), ptr(out),
+              ptr(adj), ptr(p.inc_off), ptr(p.inc_slot), ptr(out), ptr(work),
               stream_handle(stream))
     return out
 

@@ -548,8 +548,11 @@ namespace detail
 /** The GLS node plan (fx_map_gls_plan) on the device. */
 struct DeviceNavierPlan
 {
-  DeviceIndexVector node_rows, inc_off, inc_elem, inc_meta, blk_runs, node_tile;
+  DeviceIndexVector node_rows, inc_off, inc_elem, inc_meta, inc_slot, blk_runs, node_tile;
   Index             max_tile = 0;
+  // two-phase history VJP workspace (6 doubles per element), sized on first
+  // use; history VJPs of one plan must not overlap
+  mutable DeviceVector vjp_work;
 };
 } // namespace detail
 
@@ -589,15 +592,17 @@ inline void makeNavierGeometry(const std::vector<Real>& x, const std::vector<Rea
   out.dofs9.assign(d9, ctx);
   const std::size_t nb = (N + FX_GLS_BLOCK_NODES - 1) / FX_GLS_BLOCK_NODES;
   std::vector<Index> nr(3 * std::size_t(N)), io(N + 1), ie(3 * std::size_t(E)),
-      im(24 * std::size_t(E)), br(4 * FX_GLS_MAX_RUNS * nb), nt(3 * std::size_t(N));
+      im(24 * std::size_t(E)), is(3 * std::size_t(E)), br(4 * FX_GLS_MAX_RUNS * nb),
+      nt(3 * std::size_t(N));
   Index mx = 0;
   check(fx_map_gls_plan(map.handle(), N, conn3.data(), nr.data(), io.data(), ie.data(), im.data(),
-                        br.data(), nt.data(), &mx));
+                        is.data(), br.data(), nt.data(), &mx));
   auto& p = out.plan;
   p.node_rows.assign(nr, ctx);
   p.inc_off.assign(io, ctx);
   p.inc_elem.assign(ie, ctx);
   p.inc_meta.assign(im, ctx);
+  p.inc_slot.assign(is, ctx);
   p.blk_runs.assign(br, ctx);
   p.node_tile.assign(nt, ctx);
   p.max_tile = mx;
@@ -666,11 +671,12 @@ inline void applyNavierHistJacT(const NavierOperator& op, Index step, Index num_
               && adj.size() == g.num_states,
           "Navier history VJP dimensions do not match");
   if (out.size() != g.num_states) out.resize(g.num_states);
+  if (g.plan.vjp_work.size() != 6 * g.num_elems) g.plan.vjp_work.resize(6 * g.num_elems);
   check(fx_gls_hist_vjp(g.num_nodes, g.num_states, g.num_elems, g.x.data(), g.y.data(),
                         g.conn3.data(), g.dofs9.data(), g.plan.node_rows.data(), op.fluid.rho,
                         op.fluid.mu, op.dt, step, num_hist, lag, hist.data(), nxt.data(),
-                        adj.data(), g.plan.inc_off.data(), g.plan.inc_elem.data(), out.data(),
-                        ctx.raw()));
+                        adj.data(), g.plan.inc_off.data(), g.plan.inc_slot.data(), out.data(),
+                        g.plan.vjp_work.data(), ctx.raw()));
 }
 } // namespace model::ns
 

@@ -139,6 +139,9 @@ int fx_map_th_lanes(const fx_map* map, fx_index num_blocks, const fx_index* h_bl
  *   inc_meta[8 per incidence]   32 bytes: byte 9k+j is the position of local
  *                  column j in the node's row k (from row_ptr[row]), byte 27
  *                  the node's local index;
+ *   inc_slot[3E]   slot of element e in the list of its local node k:
+ *                  inc_elem[inc_slot[3e+k]] == e, within node conn3[3e+k]'s
+ *                  range (NULL: not written);
  *   blk_runs[4*FX_GLS_MAX_RUNS per block of FX_GLS_BLOCK_NODES nodes]: the
  *                  block's rows as runs of consecutive rows -- CSR begin,
  *                  CSR length, LDS tile offset, 0 (unused runs all zero);
@@ -151,8 +154,8 @@ int fx_map_th_lanes(const fx_map* map, fx_index num_blocks, const fx_index* h_bl
 #define FX_GLS_MAX_RUNS 4
 int fx_map_gls_plan(const fx_map* map, fx_index num_nodes, const fx_index* h_conn3,
                     fx_index* h_node_rows, fx_index* h_inc_off, fx_index* h_inc_elem,
-                    fx_index* h_inc_meta, fx_index* h_blk_runs, fx_index* h_node_tile,
-                    fx_index* max_tile);
+                    fx_index* h_inc_meta, fx_index* h_inc_slot, fx_index* h_blk_runs,
+                    fx_index* h_node_tile, fx_index* max_tile);
 void fx_map_destroy(fx_map* map);
 
 /* CSR transpose permutation (host), for the deterministic transpose apply:
@@ -367,14 +370,19 @@ int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems,
  * velocity and tau(u); pressure entries are zero.  adj has num_states
  * entries (the residual space).  Per state DOF, a sum over its elements in
  * ascending element order.  node_rows, inc_off and inc_elem from
- * fx_map_gls_plan. */
+ * fx_map_gls_plan, and inc_slot from the same call.  work (6 * num_elems
+ * doubles: two per incidence slot) receives each element's columns for its
+ * three nodes, formed from one evaluation of the quadrature-point partials
+ * per element and stored at the element's slot in each node's incidence
+ * list (inc_slot); a second pass sums each node's contiguous slots.  Calls
+ * sharing a work buffer must not overlap. */
 int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                     const double* d_x, const double* d_y, const fx_index* d_conn3,
                     const fx_index* d_dofs9, const fx_index* d_node_rows, double rho, double mu,
                     double dt, fx_index step, fx_index num_hist, fx_index lag,
                     const double* d_hist, const double* d_nxt, const double* d_adj,
-                    const fx_index* d_inc_off, const fx_index* d_inc_elem, double* d_out,
-                    void* stream);
+                    const fx_index* d_inc_off, const fx_index* d_inc_slot, double* d_out,
+                    double* d_work, void* stream);
 
 /* ======================================================================== */
 /* Batches (device).  Many P1 Poisson and Taylor-Hood instances of uneven size

@@ -158,6 +158,7 @@ def test_plan_tables(n):
         for t in range(off[v], off[v + 1]):
             e, ln = elem[t], meta[t, 27]
             assert conn[e, ln] == v
+            assert h["inc_slot"][3 * e + ln] == t
             for kk, lr in enumerate((2 * ln, 2 * ln + 1, 6 + ln)):
                 got = rp[nr[v, kk]] + meta[t, 9 * kk: 9 * kk + 9].astype(np.int64)
                 assert np.array_equal(got, jm[81 * e + 9 * lr: 81 * e + 9 * lr + 9])
@@ -280,7 +281,7 @@ def _lib_plan(amap, N, conn):
     import ctypes as C
     from femx_amd import _lib
     from femx_amd.linalg import ptr
-    buf = [np.empty(64, np.int32) for _ in range(7)]
+    buf = [np.empty(64, np.int32) for _ in range(8)]
     mx = C.c_int32()
     _lib.call("fx_map_gls_plan", amap._handle(), N, ptr(np.ascontiguousarray(conn, np.int32)),
               *[ptr(b) for b in buf], C.byref(mx))
