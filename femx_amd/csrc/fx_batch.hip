@@ -70,9 +70,9 @@ enum Phase
   kThVjpGather = 4
 };
 
-__global__ __launch_bounds__(kP1Block) void k_batch_p1_assemble(const P1AsmArgs* __restrict__ inst,
-                                                               const int2* __restrict__ work,
-                                                               int nwork)
+// capped at 96 VGPRs (5 waves/SIMD, as k_p1_assemble reaches uncapped)
+__global__ __launch_bounds__(kP1Block) __attribute__((amdgpu_waves_per_eu(5))) void
+k_batch_p1_assemble(const P1AsmArgs* __restrict__ inst, const int2* __restrict__ work, int nwork)
 {
   extern __shared__ double smem[];
   const fx::XcdWalk wk = fx::xcd_walk(nwork);
@@ -83,8 +83,9 @@ __global__ __launch_bounds__(kP1Block) void k_batch_p1_assemble(const P1AsmArgs*
   }
 }
 
-__global__ __launch_bounds__(kP1Block) void k_batch_p1_vjp(const P1VjpArgs* __restrict__ inst,
-                                                          const int2* __restrict__ work, int nwork)
+// capped at 96 VGPRs (5 waves/SIMD), as k_p1_vjp
+__global__ __launch_bounds__(kP1Block) __attribute__((amdgpu_waves_per_eu(5))) void
+k_batch_p1_vjp(const P1VjpArgs* __restrict__ inst, const int2* __restrict__ work, int nwork)
 {
   __shared__ long long sinc[kIncStage];
   const fx::XcdWalk wk = fx::xcd_walk(nwork);

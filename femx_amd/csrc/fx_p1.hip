@@ -13,7 +13,8 @@ __global__ __launch_bounds__(kP1Block) void k_p1_assemble(P1AsmArgs a)
   p1_assemble_block(fx::xcd_block(gridDim.x), a, smem);
 }
 
-__global__ __launch_bounds__(kP1Block) void k_p1_vjp(P1VjpArgs a)
+// capped at 96 VGPRs: 5 waves/SIMD instead of 4, no spills (74.6 -> 69.1 us at 1024^2)
+__global__ __launch_bounds__(kP1Block) __attribute__((amdgpu_waves_per_eu(5))) void k_p1_vjp(P1VjpArgs a)
 {
   __shared__ long long sinc[kIncStage];
   p1_vjp_block(fx::xcd_block(gridDim.x), a, sinc);
