@@ -22,9 +22,10 @@ __global__ __launch_bounds__(64) void k_gls_hist_vjp_elem(GlsVjpArgs a)
   gls_hist_vjp_elem<LAG>(fx::xcd_block(gridDim.x) * 64 + threadIdx.x, a);
 }
 
-__global__ __launch_bounds__(256) void k_gls_hist_vjp_sum(GlsVjpArgs a)
+__global__ __launch_bounds__(kGlsSumNodes) void k_gls_hist_vjp_sum(GlsVjpArgs a)
 {
-  gls_hist_vjp_sum(fx::xcd_block(gridDim.x) * 256 + threadIdx.x, a);
+  __shared__ double sw[kGlsSumStage];
+  gls_hist_vjp_sum_block(fx::xcd_block(gridDim.x), a, sw);
 }
 
 } // namespace
@@ -136,8 +137,8 @@ int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
       else
         hipLaunchKernelGGL(k_gls_hist_vjp_elem<2>, grid, dim3(64), 0, fx::as_stream(stream), a);
     }
-    hipLaunchKernelGGL(k_gls_hist_vjp_sum, dim3(fx::blocks_for(num_nodes, 256)), dim3(256), 0,
-                       fx::as_stream(stream), a);
+    hipLaunchKernelGGL(k_gls_hist_vjp_sum, dim3(fx::blocks_for(num_nodes, kGlsSumNodes)),
+                       dim3(kGlsSumNodes), 0, fx::as_stream(stream), a);
     fx::check_launch("fx_gls_hist_vjp");
   });
 }

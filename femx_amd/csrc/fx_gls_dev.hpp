@@ -34,7 +34,7 @@
 // positive tau argument), the comparisons are sign-blind at zero, and the
 // outputs are accumulated onto +0.0 as in femx.
 #pragma once
-#include "fx_common.hpp"
+#include "fx_dev.hpp"
 
 namespace fxk
 {
@@ -769,18 +769,40 @@ __device__ __forceinline__ void gls_hist_vjp_elem(int e, const GlsVjpArgs& a)
 
 // Phase 2 (one lane per node): the node's two columns summed over its
 // incidence slots -- its elements in ascending order -- from zero, term after
-// term.  The pressure row's column is zero.
-__device__ __forceinline__ void gls_hist_vjp_sum(int n, const GlsVjpArgs& a)
+// term.  The pressure row's column is zero.  A block's 256 nodes own one
+// contiguous slot range, staged in LDS with coalesced non-temporal loads when
+// it fits (kGlsSumStage doubles, 8 slots per node on average).
+constexpr int kGlsSumNodes = 256;
+constexpr int kGlsSumStage = 4096;
+
+__device__ __forceinline__ void gls_hist_vjp_sum_block(int blk, const GlsVjpArgs& a, double* sw)
 {
+  const int  n0 = blk * kGlsSumNodes, n1 = min(n0 + kGlsSumNodes, a.num_nodes);
+  const int  i0 = a.inc_off[n0], ni = a.inc_off[n1] - i0;
+  const bool staged = 2 * static_cast<long>(ni) <= kGlsSumStage;
+  if (staged)
+  {
+    stage_nt<kGlsSumNodes>(sw, a.work + 2 * static_cast<long>(i0), 2 * ni);
+    __syncthreads();
+  }
+  const int n = n0 + threadIdx.x;
   if (n >= a.num_nodes || a.node_rows[3 * n] < 0) return;
   double         s0 = 0.0, s1 = 0.0;
   const double2* w  = reinterpret_cast<const double2*>(a.work);
   const int      t1 = a.inc_off[n + 1];
   for (int t = a.inc_off[n]; t < t1; ++t)
   {
-    const double2 wt = w[t];
-    s0 += wt.x;
-    s1 += wt.y;
+    if (staged)
+    {
+      s0 += sw[2 * (t - i0)];
+      s1 += sw[2 * (t - i0) + 1];
+    }
+    else
+    {
+      const double2 wt = w[t];
+      s0 += wt.x;
+      s1 += wt.y;
+    }
   }
   __builtin_nontemporal_store(s0, a.out + a.node_rows[3 * n]);
   __builtin_nontemporal_store(s1, a.out + a.node_rows[3 * n + 1]);
