@@ -48,6 +48,7 @@ constexpr int kGlsRuns  = FX_GLS_MAX_RUNS;    // row runs per block
 struct GlsConst
 {
   double rho, mu, dt, nu, time, rdt, h5r, h5m;
+  double idt, irho; // 1/dt and 1/rho: the VJP multiplies where the residual divides
 };
 
 __host__ __device__ inline GlsConst gls_const(double rho, double mu, double dt)
@@ -62,6 +63,8 @@ __host__ __device__ inline GlsConst gls_const(double rho, double mu, double dt)
   c.rdt          = rho / dt;
   c.h5r          = 0.5 * rho;
   c.h5m          = 0.5 * mu;
+  c.idt          = 1.0 / dt;
+  c.irho         = 1.0 / rho;
   return c;
 }
 
@@ -614,18 +617,17 @@ __device__ __forceinline__ void gls_hvjp_nodes(const int (&ks)[NK], const double
         diff           = (4.0 * C.nu / (h * h)) * (4.0 * C.nu / (h * h));
         if (moving)
         {
-          const double S = speed * sum;
+          const double S   = speed * sum;
+          const double s4  = speed * speed * speed * speed;
+          const double is4 = 1.0 / s4, is6 = 1.0 / (s4 * speed * speed);
 #pragma unroll
           for (int c = 0; c < 2; ++c)
           {
             double dS = 0.0;
 #pragma unroll
             for (int k = 0; k < 3; ++k) dS += sgn[k] * g[2 * k + c];
-            dflow[c]        = 2.0 * S * dS;
-            const double s4 = speed * speed * speed * speed;
-            ddiff[c]        = C.nu * C.nu
-                       * (4.0 * S * S * S * dS / s4
-                          - 4.0 * S * S * S * S * u[c] / (s4 * speed * speed));
+            dflow[c] = 2.0 * S * dS;
+            ddiff[c] = C.nu * C.nu * (4.0 * S * S * S * dS * is4 - 4.0 * S * S * S * S * u[c] * is6);
           }
         }
         else
@@ -647,12 +649,12 @@ __device__ __forceinline__ void gls_hvjp_nodes(const int (&ks)[NK], const double
     }
 #pragma unroll
     for (int d = 0; d < 2; ++d)
-      l1 += Dpi[d] * (-u[d] / C.dt + 0.5 * ag[d] + v[d] / C.dt + 0.5 * z[d] + Dq[d] / C.rho);
+      l1 += Dpi[d] * (-u[d] * C.idt + 0.5 * ag[d] + v[d] * C.idt + 0.5 * z[d] + Dq[d] * C.irho);
     double Pu[2], Pag[2], Padv[2], PG[2][2];
 #pragma unroll
     for (int c = 0; c < 2; ++c)
     {
-      Pu[c]  = -C.rdt * lam[c] + tau * (-C.rdt * wv[c] - Dpi[c] / C.dt) + l1 * dtau[c];
+      Pu[c]  = -C.rdt * lam[c] + tau * (-C.rdt * wv[c] - Dpi[c] * C.idt) + l1 * dtau[c];
       Pag[c] = 0.5 * C.rho * lam[c] + tau * (0.5 * C.rho * wv[c] + 0.5 * Dpi[c]);
     }
 #pragma unroll

@@ -1228,6 +1228,7 @@ void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double 
   const int    extrap = step > 0 && num_hist > 1;
   const double al = extrap ? 1.5 : 1.0, be = extrap ? -0.5 : 0.0;
   const double nu = mu / rho, time = (2.0 / dt) * (2.0 / dt), rdt = rho / dt;
+  const double idt = 1.0 / dt, irho = 1.0 / rho; /* multiplied where the residual divides */
   double       Dl[2][2] = {{0, 0}, {0, 0}}, Dv[2][2] = {{0, 0}, {0, 0}}, G[2][2] = {{0, 0}, {0, 0}};
   double       Dpi[2] = {0, 0}, Dq[2] = {0, 0};
   for (int k = 0; k < 3; ++k)
@@ -1293,13 +1294,14 @@ void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double 
         {
           /* S = sum_k |u.g_k| = speed*sum: flow = S^2, diff = nu^2 S^4 / speed^4 */
           const double S = speed * sum;
+          const double s4 = speed * speed * speed * speed;
+          const double is4 = 1.0 / s4, is6 = 1.0 / (s4 * speed * speed);
           for (int c = 0; c < 2; ++c)
           {
             double dS = 0.0;
             for (int k = 0; k < 3; ++k) dS += sgn[k] * g[2 * k + c];
             dflow[c] = 2.0 * S * dS;
-            const double s4 = speed * speed * speed * speed;
-            ddiff[c] = nu * nu * (4.0 * S * S * S * dS / s4 - 4.0 * S * S * S * S * u[c] / (s4 * speed * speed));
+            ddiff[c] = nu * nu * (4.0 * S * S * S * dS * is4 - 4.0 * S * S * S * S * u[c] * is6);
           }
         }
         else
@@ -1315,11 +1317,11 @@ void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double 
       l1 += wv[c] * B[c];
     }
     for (int d = 0; d < 2; ++d)
-      l1 += Dpi[d] * (-u[d] / dt + 0.5 * ag[d] + v[d] / dt + 0.5 * z[d] + Dq[d] / rho);
+      l1 += Dpi[d] * (-u[d] * idt + 0.5 * ag[d] + v[d] * idt + 0.5 * z[d] + Dq[d] * irho);
     double Pu[2], Pag[2], Padv[2], PG[2][2];
     for (int c = 0; c < 2; ++c)
     {
-      Pu[c]  = -rdt * lam[c] + tau * (-rdt * wv[c] - Dpi[c] / dt) + l1 * dtau[c];
+      Pu[c]  = -rdt * lam[c] + tau * (-rdt * wv[c] - Dpi[c] * idt) + l1 * dtau[c];
       Pag[c] = 0.5 * rho * lam[c] + tau * (0.5 * rho * wv[c] + 0.5 * Dpi[c]);
     }
     for (int d = 0; d < 2; ++d)
