@@ -287,7 +287,12 @@ __device__ __forceinline__ void p1_vjp_block(int blk, const P1VjpArgs& a, long l
       // node j is vertex li of each of its elements: its values are loaded
       // once, the element's two other vertices are gathered per incidence
       // (their indices one incidence ahead)
-      auto masked = [&](int n) { return (a.bc_mask && gat32(a.bc_mask, n)) ? 0.0 : gat32(a.lam, n); };
+      // both gathers issued together (n < num_rows for both arrays), then a select
+      auto masked = [&](int n) {
+        const double l = gat32(a.lam, n);
+        const int    m = a.bc_mask ? gat32(a.bc_mask, n) : 0;
+        return m ? 0.0 : l;
+      };
       const double xo = a.x[j], yo = a.y[j], lo = masked(j), uo = a.u[j];
       int2         ic = inc_at(sinc, a.inc, staged, ta, ib0);
       int2         cc = p1_others(a.conn, ic.x, ic.y & 0xff);

@@ -659,7 +659,10 @@ __device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, c
   for (int k = 0; k < 15; ++k)
   {
     const int dk = D.d[k];
-    l[k]         = (a.bc_mask && gat32(a.bc_mask, dk)) ? 0.0 : gat32(a.lam, dk);
+    // both gathers issued together (dk < ndof for both arrays), then a select
+    const double lv = gat32(a.lam, dk);
+    const int    mk = a.bc_mask ? gat32(a.bc_mask, dk) : 0;
+    l[k]            = mk ? 0.0 : lv;
     c[k]         = 0.0;
   }
   double ge = 0.0;
