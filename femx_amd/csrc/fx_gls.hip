@@ -22,6 +22,13 @@ __global__ __launch_bounds__(64) void k_gls_hist_vjp_elem(GlsVjpArgs a)
   gls_hist_vjp_elem<LAG>(fx::xcd_block(gridDim.x) * 64 + threadIdx.x, a);
 }
 
+// lag 1 (advection partials only) held to 168 VGPRs: 3 waves/SIMD
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_gls_hist_vjp_elem1(
+    GlsVjpArgs a)
+{
+  gls_hist_vjp_elem<1>(fx::xcd_block(gridDim.x) * 64 + threadIdx.x, a);
+}
+
 __global__ __launch_bounds__(kGlsSumNodes) void k_gls_hist_vjp_sum(GlsVjpArgs a)
 {
   __shared__ double sw[kGlsSumStage];
@@ -133,7 +140,7 @@ int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
       if (lag == 0)
         hipLaunchKernelGGL(k_gls_hist_vjp_elem<0>, grid, dim3(64), 0, fx::as_stream(stream), a);
       else if (lag == 1)
-        hipLaunchKernelGGL(k_gls_hist_vjp_elem<1>, grid, dim3(64), 0, fx::as_stream(stream), a);
+        hipLaunchKernelGGL(k_gls_hist_vjp_elem1, grid, dim3(64), 0, fx::as_stream(stream), a);
       else
         hipLaunchKernelGGL(k_gls_hist_vjp_elem<2>, grid, dim3(64), 0, fx::as_stream(stream), a);
     }

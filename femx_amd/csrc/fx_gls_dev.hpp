@@ -597,8 +597,9 @@ __device__ __forceinline__ void gls_hvjp_nodes(const int (&ks)[NK], const double
       const bool moving = speed > 1.0e-10;
       if (moving)
       {
-        dir[0] = u[0] / speed;
-        dir[1] = u[1] / speed;
+        const double isp = 1.0 / speed;
+        dir[0]           = u[0] * isp;
+        dir[1]           = u[1] * isp;
       }
       else
         dir[0] = dir[1] = 1.0 / sqrt(2.0);
@@ -612,9 +613,10 @@ __device__ __forceinline__ void gls_hvjp_nodes(const int (&ks)[NK], const double
       double flow = 0.0, diff = 0.0, dflow[2] = {0.0, 0.0}, ddiff[2] = {0.0, 0.0};
       if (sum > 1.0e-14)
       {
-        const double h = 2.0 / sum;
-        flow           = (2.0 * speed / h) * (2.0 * speed / h);
-        diff           = (4.0 * C.nu / (h * h)) * (4.0 * C.nu / (h * h));
+        // with h = 2 / sum: 2 speed / h = speed sum and 4 nu / h^2 = nu sum^2
+        const double fs = speed * sum, dv = C.nu * sum * sum;
+        flow            = fs * fs;
+        diff            = dv * dv;
         if (moving)
         {
           const double S   = speed * sum;
@@ -633,7 +635,7 @@ __device__ __forceinline__ void gls_hvjp_nodes(const int (&ks)[NK], const double
         else
         {
 #pragma unroll
-          for (int c = 0; c < 2; ++c) dflow[c] = 8.0 / (h * h) * u[c];
+          for (int c = 0; c < 2; ++c) dflow[c] = 2.0 * sum * sum * u[c]; // 8 / h^2
         }
       }
       tau = 1.0 / sqrt(C.time + flow + diff);

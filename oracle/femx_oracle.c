@@ -1273,8 +1273,9 @@ void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double 
       const int moving = speed > 1.0e-10;
       if (moving)
       {
-        dir[0] = u[0] / speed;
-        dir[1] = u[1] / speed;
+        const double isp = 1.0 / speed;
+        dir[0]           = u[0] * isp;
+        dir[1]           = u[1] * isp;
       }
       else
         dir[0] = dir[1] = 1.0 / sqrt(2.0);
@@ -1287,9 +1288,10 @@ void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double 
       double flow = 0.0, diff = 0.0, dflow[2] = {0, 0}, ddiff[2] = {0, 0};
       if (sum > 1.0e-14)
       {
-        const double h = 2.0 / sum;
-        flow           = (2.0 * speed / h) * (2.0 * speed / h);
-        diff           = (4.0 * nu / (h * h)) * (4.0 * nu / (h * h));
+        /* with h = 2 / sum: 2 speed / h = speed sum and 4 nu / h^2 = nu sum^2 */
+        const double fs = speed * sum, dv = nu * sum * sum;
+        flow            = fs * fs;
+        diff            = dv * dv;
         if (moving)
         {
           /* S = sum_k |u.g_k| = speed*sum: flow = S^2, diff = nu^2 S^4 / speed^4 */
@@ -1305,7 +1307,7 @@ void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double 
           }
         }
         else
-          for (int c = 0; c < 2; ++c) dflow[c] = 8.0 / (h * h) * u[c];
+          for (int c = 0; c < 2; ++c) dflow[c] = 2.0 * sum * sum * u[c]; /* 8 / h^2 */
       }
       tau = 1.0 / sqrt(time + flow + diff);
       for (int c = 0; c < 2; ++c) dtau[c] = -0.5 * tau * tau * tau * (dflow[c] + ddiff[c]);
