@@ -369,8 +369,8 @@ int fx_gls_assemble(fx_index num_nodes, fx_index num_states, fx_index num_elems,
  * NavierOperator's residual through u, grad u, the extrapolated advection
  * velocity and tau(u); pressure entries are zero.  adj has num_states
  * entries (the residual space).  Per state DOF, a sum over its elements in
- * ascending element order.  node_rows, inc_off and inc_elem from
- * fx_map_gls_plan, and inc_slot from the same call.  work (6 * num_elems
+ * ascending element order.  node_rows, inc_off and inc_slot from
+ * fx_map_gls_plan.  work (6 * num_elems
  * doubles: two per incidence slot) receives each element's columns for its
  * three nodes, formed from one evaluation of the quadrature-point partials
  * per element and stored at the element's slot in each node's incidence
