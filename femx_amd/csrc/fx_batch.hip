@@ -331,7 +331,18 @@ int fx_batch_capture(fx_batch* b, int what)
       fx::check_hip(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
     fx::check_hip(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal),
                   "hipStreamBeginCapture");
-    launch_phases(b, what, b->cap_stream);
+    try
+    {
+      launch_phases(b, what, b->cap_stream);
+    }
+    catch (...)
+    {
+      // leave the private stream out of capture mode so a later capture works
+      hipGraph_t partial = nullptr;
+      if (hipStreamEndCapture(b->cap_stream, &partial) == hipSuccess && partial)
+        (void)hipGraphDestroy(partial);
+      throw;
+    }
     fx::check_hip(hipStreamEndCapture(b->cap_stream, &b->graph), "hipStreamEndCapture");
     fx::check_hip(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     b->graph_what = what;

@@ -78,6 +78,10 @@ int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* xv, const double*
   return fx::guard([&] {
     fx::require(ndof >= 0 && num_elems >= 0, FX_ERR_ARG, "fx_th_vjp: negative size");
     if (ndof == 0) return;
+    // state, lam and bc_mask are gathered through 32-bit byte offsets
+    fx::require(fx::fits32(ndof, 8) && fx::fits32(num_elems, 8)
+                    && fx::fits32(15 * static_cast<std::int64_t>(num_elems), 8),
+                FX_ERR_OVERFLOW, "fx_th_vjp: state, element or workspace arrays beyond 4 GiB");
     fx::require(xv && yv && state && lam && inc_off && jt
                     && (num_elems == 0 || (vconn3 && dofs15 && elem_inc && work)),
                 FX_ERR_ARG, "fx_th_vjp: null array");

@@ -181,7 +181,10 @@ def assemble(op: NavierOperator, step, num_hist, wrt, amap: AssemblyMap, hist, n
               ptr(p.blk_runs), ptr(p.node_tile), ptr(vals) if kind == "next" else None,
               ptr(res), stream_handle(stream))
     if vals is not None and kind != "next":
-        vals.zero_()
+        # a history-lag block of the Jacobian is zero; cleared on the launch
+        # stream so that later work queued there sees it
+        with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream()):
+            vals.zero_()
     return res
 
 

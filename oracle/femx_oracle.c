@@ -1209,7 +1209,11 @@ void or_gls_element_jvp(const double* xe, const double* ye, double rho, double m
  *   B_c = -rho/dt u_c + rho/2 ag_c + rho/dt v_c + rho/2 z_c + dq/dx_c.
  * u and G depend on lag 0, adv = 1.5 lag0 - 0.5 lag1 (step > 0, two lags) or
  * lag 0, tau on u through |u| and elemLength.  out is (9 per lag) the
- * element columns for both lags; pressure columns are zero. */
+ * element columns for both lags; pressure columns are zero.  The forward
+ * values (direction u/|u|, h = 2/sum, (2|u|/h)^2, (4 nu/h^2)^2, tau) are
+ * femx's own expressions (elemLength and evalQp, Components.hpp:286-394);
+ * a device kernel that rearranges them is checked against this within a
+ * tolerance, not bitwise. */
 void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double mu, double dt,
                          int step, int num_hist, const double* hist_e, const double* nxt_e,
                          const double* adj_e, double* out)
@@ -1228,7 +1232,6 @@ void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double 
   const int    extrap = step > 0 && num_hist > 1;
   const double al = extrap ? 1.5 : 1.0, be = extrap ? -0.5 : 0.0;
   const double nu = mu / rho, time = (2.0 / dt) * (2.0 / dt), rdt = rho / dt;
-  const double idt = 1.0 / dt, irho = 1.0 / rho; /* multiplied where the residual divides */
   double       Dl[2][2] = {{0, 0}, {0, 0}}, Dv[2][2] = {{0, 0}, {0, 0}}, G[2][2] = {{0, 0}, {0, 0}};
   double       Dpi[2] = {0, 0}, Dq[2] = {0, 0};
   for (int k = 0; k < 3; ++k)
@@ -1273,9 +1276,8 @@ void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double 
       const int moving = speed > 1.0e-10;
       if (moving)
       {
-        const double isp = 1.0 / speed;
-        dir[0]           = u[0] * isp;
-        dir[1]           = u[1] * isp;
+        dir[0] = u[0] / speed;
+        dir[1] = u[1] / speed;
       }
       else
         dir[0] = dir[1] = 1.0 / sqrt(2.0);
@@ -1288,26 +1290,24 @@ void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double 
       double flow = 0.0, diff = 0.0, dflow[2] = {0, 0}, ddiff[2] = {0, 0};
       if (sum > 1.0e-14)
       {
-        /* with h = 2 / sum: 2 speed / h = speed sum and 4 nu / h^2 = nu sum^2 */
-        const double fs = speed * sum, dv = nu * sum * sum;
-        flow            = fs * fs;
-        diff            = dv * dv;
+        const double h = 2.0 / sum;
+        flow           = (2.0 * speed / h) * (2.0 * speed / h);
+        diff           = (4.0 * nu / (h * h)) * (4.0 * nu / (h * h));
         if (moving)
         {
           /* S = sum_k |u.g_k| = speed*sum: flow = S^2, diff = nu^2 S^4 / speed^4 */
           const double S = speed * sum;
-          const double s4 = speed * speed * speed * speed;
-          const double is4 = 1.0 / s4, is6 = 1.0 / (s4 * speed * speed);
           for (int c = 0; c < 2; ++c)
           {
             double dS = 0.0;
             for (int k = 0; k < 3; ++k) dS += sgn[k] * g[2 * k + c];
             dflow[c] = 2.0 * S * dS;
-            ddiff[c] = nu * nu * (4.0 * S * S * S * dS * is4 - 4.0 * S * S * S * S * u[c] * is6);
+            const double s4 = speed * speed * speed * speed;
+            ddiff[c] = nu * nu * (4.0 * S * S * S * dS / s4 - 4.0 * S * S * S * S * u[c] / (s4 * speed * speed));
           }
         }
         else
-          for (int c = 0; c < 2; ++c) dflow[c] = 2.0 * sum * sum * u[c]; /* 8 / h^2 */
+          for (int c = 0; c < 2; ++c) dflow[c] = 8.0 / (h * h) * u[c];
       }
       tau = 1.0 / sqrt(time + flow + diff);
       for (int c = 0; c < 2; ++c) dtau[c] = -0.5 * tau * tau * tau * (dflow[c] + ddiff[c]);
@@ -1319,11 +1319,11 @@ void or_gls_element_hvjp(const double* xe, const double* ye, double rho, double 
       l1 += wv[c] * B[c];
     }
     for (int d = 0; d < 2; ++d)
-      l1 += Dpi[d] * (-u[d] * idt + 0.5 * ag[d] + v[d] * idt + 0.5 * z[d] + Dq[d] * irho);
+      l1 += Dpi[d] * (-u[d] / dt + 0.5 * ag[d] + v[d] / dt + 0.5 * z[d] + Dq[d] / rho);
     double Pu[2], Pag[2], Padv[2], PG[2][2];
     for (int c = 0; c < 2; ++c)
     {
-      Pu[c]  = -rdt * lam[c] + tau * (-rdt * wv[c] - Dpi[c] * idt) + l1 * dtau[c];
+      Pu[c]  = -rdt * lam[c] + tau * (-rdt * wv[c] - Dpi[c] / dt) + l1 * dtau[c];
       Pag[c] = 0.5 * rho * lam[c] + tau * (0.5 * rho * wv[c] + 0.5 * Dpi[c]);
     }
     for (int d = 0; d < 2; ++d)

@@ -148,9 +148,15 @@ def test_gls_1024_matches_oracle_and_invariants():
 
 @pytest.mark.parametrize("n,renumber", [(3, False), (40, True)])
 @pytest.mark.parametrize("step,nh", [(1, 2), (0, 1), (2, 3)])
-def test_gls_hist_vjp_matches_oracle_bitwise(n, renumber, step, nh):
+def test_gls_hist_vjp_matches_oracle(n, renumber, step, nh):
     """The history VJP (femx's Enzyme product, hand-derived here) against the
-    oracle's restatement: same operation order, bitwise equal, every lag."""
+    oracle's restatement, every lag, within 1e-12: the oracle forms tau with
+    femx's own expressions (elemLength's h = 2/sum, (2|u|/h)^2, (4 nu/h^2)^2,
+    u/|u|), the kernel with products that avoid the divisions.  Every lag
+    also has a few whole elements at rest (both velocity components of all
+    three nodes zero in every history lag), so that their quadrature points
+    take the at-rest branch (speed 0: fixed direction, no diffusive tau
+    derivative)."""
     print("case", n, renumber, step, nh, flush=True)
     x, y, conn = perturbed(n, 3 + n, renumber)
     geom = gls.GLSGeometry(x, y, conn, DEV)
@@ -158,19 +164,23 @@ def test_gls_hist_vjp_matches_oracle_bitwise(n, renumber, step, nh):
     nd = geom.num_states
     rng = np.random.default_rng(20 + step)
     hist = rng.standard_normal(nh * nd)
-    hist[: 2 * geom.num_nodes: 7] = 0.0  # a few nodes at rest
+    d9 = geom.h_dofs9.reshape(-1, 9)
+    rest = rng.choice(geom.num_elems, max(2, geom.num_elems // 16), replace=False)
+    vel = d9[rest, :6].reshape(-1)
+    for lag in range(nh):
+        hist[lag * nd + vel] = 0.0
     nxt, adj = rng.standard_normal(nd), rng.standard_normal(nd)
     op = gls.NavierOperator(geom, gls.FluidParams(1.2, 0.03), 0.05)
     for lag in range(nh):
         out = host(gls.hist_vjp(op, step, nh, lag, amap, dev(hist), dev(nxt), dev(adj)))
         ref = O.gls_hist_vjp(x, y, conn, geom.h_dofs9.reshape(-1), nd, 1.2, 0.03, 0.05, step, nh,
                              lag, hist, nxt, adj)
-        assert np.array_equal(out, ref), lag
+        assert close(out, ref), lag
         assert not out[2 * geom.num_nodes:].any()
 
 
 def test_gls_hist_vjp_1024_adjoint_identity():
-    """1024^2: bitwise equal to the oracle and run to run; and the adjoint
+    """1024^2: within 1e-12 of the oracle, bitwise equal run to run; and the adjoint
     identity dir^T VJP_lag(adj) = adj^T JVP_lag(dir) against the oracle's
     dual-number forward mode, an independent restatement (1e-12)."""
     n = 1024
@@ -188,7 +198,7 @@ def test_gls_hist_vjp_1024_adjoint_identity():
         out = host(gls.hist_vjp(op, 1, 2, lag, amap, h_d, n_d, a_d))
         assert np.array_equal(out, host(gls.hist_vjp(op, 1, 2, lag, amap, h_d, n_d, a_d)))
         ref = O.gls_hist_vjp(x, y, conn, d9, nd, 1.0, 1e-3, 0.01, 1, 2, lag, hist, nxt, adj)
-        assert np.array_equal(out, ref)
+        assert close(out, ref)
         jv = O.gls_hist_jvp(x, y, conn, d9, nd, 1.0, 1e-3, 0.01, 1, 2, lag, hist, nxt, dirv)
         lhs, rhs = dirv @ out, adj @ jv
         assert abs(lhs - rhs) <= 1e-12 * max(1.0, np.abs(dirv * out).sum()), (lag, lhs, rhs)

@@ -108,17 +108,21 @@ def test_p1_laplace_bitwise_vs_reference_fixture(n):
     assert np.array_equal(host(res), g["res"])
 
 
-def test_p1_full_size_bitwise_and_deterministic():
-    """Config 2 size (2048^2 cells, 8.4M elements): bitwise equal to the
-    oracle and to a second launch."""
+def test_p1_full_size_step_bitwise_and_deterministic():
+    """The whole 2048^2 step (8.4M elements) with the benchmark's inputs --
+    kappa_e = exp(0.5 z_e) (seed 2), f ~ N(0,1) (seed 3), u ~ N(0,1) (seed 4),
+    x_k ~ U(-1,1) (seeds 10..19): assembly, rhs = -res, prepareForwardSolve
+    with g = 0 on all 8 192 edge DOFs, and the 10-vector product.  Every
+    output is bitwise equal to the oracle (femx's host order), and a second
+    assembly launch is bitwise equal to the first."""
     n = 2048
     geom, amap = p1_setup(n)
     N, E = geom.num_nodes, geom.num_elems
     assert amap.pattern.nnz == 7 * n * n + 6 * n + 1
-    rng = np.random.default_rng(2)
-    kappa = np.exp(0.5 * rng.standard_normal(E))
-    f = rng.standard_normal(N)
-    u = rng.standard_normal(N)
+    kappa = np.exp(0.5 * np.random.default_rng(2).standard_normal(E))
+    f = np.random.default_rng(3).standard_normal(N)
+    u = np.random.default_rng(4).standard_normal(N)
+    xs = np.stack([np.random.default_rng(10 + k).uniform(-1.0, 1.0, N) for k in range(10)])
     op = A.PoissonP1(dev(kappa), dev(f))
     jac = CsrMatrix(amap.pattern, DEV)
     res = torch.empty(N, dtype=torch.float64, device=DEV)
@@ -128,8 +132,30 @@ def test_p1_full_size_bitwise_and_deterministic():
     assert np.array_equal(host(jac.vals), v1) and np.array_equal(host(res), r1)
     _, jm = amap.jac()
     ov, orr = O.p1_assemble(geom.h_x, geom.h_y, geom.h_conn.reshape(-1), kappa, f, u, jm,
-                            amap.pattern.nnz)
+                            amap.pattern.nnz, omp=True)
     assert np.array_equal(v1, ov) and np.array_equal(r1, orr)
+
+    x, y = geom.h_x, geom.h_y
+    dofs = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+    assert dofs.size == 4 * n
+    bmap = A.make_boundary_map(dofs, amap.pattern, DEV)
+    rhs = torch.empty(N, dtype=torch.float64, device=DEV)
+    VectorHandler().axpby(-1.0, res, 0.0, rhs)
+    A.prepare_forward_solve(bmap, jac, rhs, torch.zeros(dofs.size, dtype=torch.float64,
+                                                        device=DEV))
+    Y = torch.empty((10, N), dtype=torch.float64, device=DEV)
+    MatrixHandler().matvec_multi(jac, dev(xs), Y)
+    bc = dict(bc_rows=bmap.bc_rows, diag=bmap.diag, col_offsets=bmap.col_offsets,
+              col_entries=bmap.col_entries, col_rows=bmap.col_rows, bc_mask=bmap.bc_mask)
+    orhs = -orr
+    O.prepare_forward(amap.pattern.row_ptr, bc, ov, orhs, np.zeros(dofs.size))
+    assert np.array_equal(host(jac.vals), ov), "eliminated matrix"
+    assert np.array_equal(host(rhs), orhs), "eliminated right-hand side"
+    Yh = host(Y)
+    oy = np.empty(N)
+    for k in range(10):
+        O.matvec(amap.pattern.row_ptr, amap.pattern.col_ind, ov, xs[k], oy)
+        assert np.array_equal(Yh[k], oy), f"product {k}"
 
 
 def test_bc_ops_bitwise_vs_reference_fixture():

@@ -229,6 +229,11 @@ def test_gather_offsets_are_checked_before_launch():
     with pytest.raises(RuntimeError, match="beyond 4 GiB"):
         _lib.call("fx_th_assemble", 8, (1 << 28) // 15 + 1, 1, 8, p, p, p, p, p, 0.01, None, p, p, p,
                   p, p, None, p, p, p, None)
+    with pytest.raises(RuntimeError, match="beyond 4 GiB"):
+        _lib.call("fx_th_vjp", big, 8, p, p, p, p, p, 0.01, None, p, None, p, p, p, None, p, None)
+    with pytest.raises(RuntimeError, match="beyond 4 GiB"):
+        _lib.call("fx_th_vjp", 8, (1 << 29) // 15 + 1, p, p, p, p, p, 0.01, None, p, None, p, p, p,
+                  None, p, None)
 
 
 @pytest.mark.parametrize("name", ["rect", "p1_n3"])
