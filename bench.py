@@ -245,7 +245,7 @@ def main():
     amap = A.make_assembly_map(N, N, geom.h_conn)
     nnz = amap.pattern.nnz
     x, y = geom.h_x, geom.h_y
-    dofs = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+    dofs = A.unit_square_edge_nodes(x, y)
     bmap = A.make_boundary_map(dofs, amap.pattern, dev)
 
     kappa = np.exp(0.5 * np.random.default_rng(2).standard_normal(E))
@@ -294,7 +294,7 @@ def main():
             A.prepare_forward_solve(bmap, jac, rhs, g0)
         gv = A.structured_p1(1024, dev)
         xv, yv = gv.h_x, gv.h_y
-        vd = np.nonzero((xv == 0) | (xv == 1) | (yv == 0) | (yv == 1))[0].astype(np.int32)
+        vd = A.unit_square_edge_nodes(xv, yv)
         Rv = PoissonLogKappaResidual(gv, vd)
         m = t(0.5 * np.random.default_rng(6).standard_normal(gv.num_elems))
         lam = t(np.random.default_rng(7).standard_normal(gv.num_nodes))
@@ -369,7 +369,7 @@ def main():
         log(f"[bench] VJP P1 {nv}x{nv}")
         gv = A.structured_p1(nv, dev)
         xv, yv = gv.h_x, gv.h_y
-        vd = np.nonzero((xv == 0) | (xv == 1) | (yv == 0) | (yv == 1))[0].astype(np.int32)
+        vd = A.unit_square_edge_nodes(xv, yv)
         Rv = PoissonLogKappaResidual(gv, vd)
         m = t(0.5 * np.random.default_rng(6).standard_normal(gv.num_elems))
         lam = t(np.random.default_rng(7).standard_normal(gv.num_nodes))

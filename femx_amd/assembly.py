@@ -158,6 +158,24 @@ def structured_p1(n, device="cuda"):
     return Geometry(x, y, conn.reshape(E, 3), device)
 
 
+def on_unit_square_edge(x, y, tol=1e-12):
+    """Mask of the nodes on the boundary of [0,1]^2.  Structured coordinates
+    are x_min + i*dx as femx's Mesh::makeStructuredQuad forms them
+    (femx/fem/Mesh.cpp:19-29), so the last lattice coordinate can miss 1.0
+    by an ulp (n = 49: 49 * (1/49) = 0.9999999999999999); exact comparison
+    would drop the x = 1 and y = 1 edges for such n."""
+    x = np.asarray(x)
+    y = np.asarray(y)
+    return ((np.abs(x) <= tol) | (np.abs(x - 1.0) <= tol) | (np.abs(y) <= tol)
+            | (np.abs(y - 1.0) <= tol))
+
+
+def unit_square_edge_nodes(x, y):
+    """Ascending ids of the nodes on the boundary of [0,1]^2 (the order
+    DirichletBC::addBoundary adds them, femx/fem/DirichletBC.cpp:138-145)."""
+    return np.nonzero(on_unit_square_edge(x, y))[0].astype(np.int32)
+
+
 # ---------------------------------------------------------------------------
 # Boundary map
 

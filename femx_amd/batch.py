@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from . import _lib, ns
-from .assembly import make_assembly_map, make_boundary_map, structured_p1
+from .assembly import make_assembly_map, make_boundary_map, structured_p1, unit_square_edge_nodes
 from .inverse import PoissonLogKappaResidual
 from .linalg import CsrMatrix, ptr, stream_handle
 
@@ -47,7 +47,7 @@ class P1Case:
         self.geom = structured_p1(n, device)
         g = self.geom
         x, y = g.h_x, g.h_y
-        dofs = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+        dofs = unit_square_edge_nodes(x, y)
         rng = np.random.Generator(np.random.Philox(seed))
         self.m = t(0.5 * rng.standard_normal(g.num_elems))
         self.f = t(rng.standard_normal(g.num_nodes))

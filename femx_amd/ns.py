@@ -18,7 +18,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .assembly import AssemblyMap, make_assembly_map, make_boundary_map, replace_res, replace_rows
+from .assembly import (AssemblyMap, make_assembly_map, make_boundary_map, replace_res, replace_rows,
+                       unit_square_edge_nodes)
 from .linalg import CsrMatrix, _check_dev_vec, _require, ptr, stream_handle
 
 TH_ITEMS = 32  # work items per block (FX_TH_BLOCK_ITEMS)
@@ -183,10 +184,10 @@ def cavity_bc(geom: THGeometry, lid=1.0):
     components on every boundary P2 node; u_x = lid on y = 1 (corners
     included), 0 elsewhere."""
     x, y = geom.h_x2, geom.h_y2
-    nodes = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0]
+    nodes = unit_square_edge_nodes(x, y)
     dofs = np.stack([2 * nodes, 2 * nodes + 1], axis=1).reshape(-1).astype(np.int32)
     vals = np.zeros(dofs.size)
-    vals[0::2] = np.where(y[nodes] == 1.0, lid, 0.0)
+    vals[0::2] = np.where(np.abs(y[nodes] - 1.0) <= 1e-12, lid, 0.0)
     return dofs, vals
 
 
@@ -209,7 +210,7 @@ def fourier_state(geom: THGeometry, seed, lid=1.0):
     s[0:2 * geom.num_vel_nodes:2] = field(amp[0], x2, y2)
     s[1:2 * geom.num_vel_nodes:2] = field(amp[1], x2, y2)
     s[2 * geom.num_vel_nodes:] = field(amp[2], geom.h_xv, geom.h_yv)
-    top = np.nonzero(y2 == 1.0)[0]
+    top = np.nonzero(np.abs(y2 - 1.0) <= 1e-12)[0]
     s[2 * top] = lid
     return s
 

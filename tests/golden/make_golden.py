@@ -60,7 +60,8 @@ def boundary_dofs_p1(n):
     """Marker-based DirichletBC order: ascending node id (femx/fem/
     DirichletBC.cpp:138-145) over nodes on the unit-square boundary."""
     x, y, _ = O.mesh_p1(n)
-    on = (x == 0.0) | (x == 1.0) | (y == 0.0) | (y == 1.0)
+    tol = 1e-12  # x_min + i*dx can miss 1.0 by an ulp (femx/fem/Mesh.cpp:19-29)
+    on = (abs(x) <= tol) | (abs(x - 1.0) <= tol) | (abs(y) <= tol) | (abs(y - 1.0) <= tol)
     return np.nonzero(on)[0].astype(I32)
 
 

@@ -136,7 +136,7 @@ def test_p1_full_size_step_bitwise_and_deterministic():
     assert np.array_equal(v1, ov) and np.array_equal(r1, orr)
 
     x, y = geom.h_x, geom.h_y
-    dofs = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+    dofs = A.unit_square_edge_nodes(x, y)
     assert dofs.size == 4 * n
     bmap = A.make_boundary_map(dofs, amap.pattern, DEV)
     rhs = torch.empty(N, dtype=torch.float64, device=DEV)
@@ -312,7 +312,7 @@ def test_config1_pipeline_bitwise():
     N = geom.num_nodes
     x, y = geom.h_x, geom.h_y
     f = 2 * np.pi ** 2 * np.sin(np.pi * x) * np.sin(np.pi * y)
-    dofs = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+    dofs = A.unit_square_edge_nodes(x, y)
     assert dofs.size == 128
     b = A.make_boundary_map(dofs, amap.pattern, DEV)
     jac = CsrMatrix(amap.pattern, DEV)

@@ -47,7 +47,7 @@ def test_transpose_equals_reference(name):
 def poisson(n, seed=1):
     geom = A.structured_p1(n, DEV)
     x, y = geom.h_x, geom.h_y
-    bc = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+    bc = A.unit_square_edge_nodes(x, y)
     rng = np.random.default_rng(seed)
     f = dev(rng.standard_normal(geom.num_nodes))
     R = PoissonLogKappaResidual(geom, bc, f=f)

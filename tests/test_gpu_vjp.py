@@ -31,7 +31,7 @@ def host(t):
 def problem(n, with_bc=True, f=True):
     geom = A.structured_p1(n, DEV)
     x, y = geom.h_x, geom.h_y
-    dofs = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+    dofs = A.unit_square_edge_nodes(x, y)
     ff = dev(RNG.standard_normal(geom.num_nodes)) if f else None
     g = dev(RNG.standard_normal(dofs.size)) if with_bc else None
     R = PoissonLogKappaResidual(geom, dofs if with_bc else None, g, ff)
@@ -60,7 +60,7 @@ def test_vjp_scrambled_mesh_matches_oracle(seed):
     from test_gpu_p1 import scrambled_p1
     geom = scrambled_p1(12, seed)
     x, y = geom.h_x, geom.h_y
-    dofs = np.nonzero((x == 0) | (x == 1) | (y == 0) | (y == 1))[0].astype(np.int32)
+    dofs = A.unit_square_edge_nodes(x, y)
     R = PoissonLogKappaResidual(geom, dofs, dev(RNG.standard_normal(dofs.size)))
     N, E = geom.num_nodes, geom.num_elems
     m = 0.5 * RNG.standard_normal(E)

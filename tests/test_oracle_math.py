@@ -77,7 +77,8 @@ def _p1_problem(n, seed=0):
     N, E = x.size, conn.size // 3
     off = O.uniform_offsets(E, 3)
     rp, ci, jo, jm = O.map_build(N, N, off, conn, off, conn)
-    on = (x == 0) | (x == 1) | (y == 0) | (y == 1)
+    tol = 1e-12  # x_min + i*dx can miss 1.0 by an ulp
+    on = (abs(x) <= tol) | (abs(x - 1) <= tol) | (abs(y) <= tol) | (abs(y - 1) <= tol)
     dofs = np.nonzero(on)[0].astype(np.int32)
     return x, y, conn, rp, ci, jm, dofs
 

@@ -89,8 +89,8 @@ def test_fourier_state_and_cavity_bc():
     g = ns.structured_th(4, "cpu")
     s = ns.fourier_state(g, 5)
     assert s.shape == (g.ndof,)
-    top = np.nonzero(g.h_y2 == 1.0)[0]
-    assert np.all(s[2 * top] == 1.0)
+    top = np.nonzero(np.abs(g.h_y2 - 1.0) <= 1e-12)[0]
+    assert top.size == 2 * 4 + 1 and np.all(s[2 * top] == 1.0)
     s2 = ns.fourier_state(g, 5)
     assert np.array_equal(s, s2)
     dofs, vals = ns.cavity_bc(g)
@@ -137,3 +137,21 @@ def test_th_lane_map(n):
     with pytest.raises(RuntimeError, match="block partition"):
         _lib.call("fx_map_th_lanes", amap._handle(), 2, ptr(bad), ptr(np.zeros(2, np.int32)),
                   ptr(np.empty(128, np.int32)))
+
+
+@pytest.mark.parametrize("n", [49, 499])
+def test_lid_and_walls_found_when_the_last_coordinate_misses_one(n):
+    """femx's structured coordinates are x_min + i*dx (Mesh.cpp:19-29); for
+    these n the last one is 0.9999999999999999, and the walls at x = 1 and
+    y = 1 must still be found (they were dropped by exact comparison)."""
+    g = ns.structured_th(n, "cpu")
+    assert g.h_y2.max() < 1.0  # the case this test is about
+    dofs, vals = ns.cavity_bc(g)
+    assert dofs.size == 2 * 4 * (2 * n)  # both components of every edge node
+    assert np.count_nonzero(vals) == 2 * n + 1  # u_x = 1 on the whole lid, corners included
+    s = ns.fourier_state(g, 3)
+    top = np.argsort(g.h_y2)[-(2 * n + 1):]
+    assert np.all(s[2 * top] == 1.0)
+    from femx_amd.assembly import structured_p1, unit_square_edge_nodes
+    p = structured_p1(n, "cpu")
+    assert unit_square_edge_nodes(p.h_x, p.h_y).size == 4 * n
