@@ -131,8 +131,10 @@ def test_p1_full_size_step_bitwise_and_deterministic():
     A.assemble(op, geom, amap, dev(u), res, jac)
     assert np.array_equal(host(jac.vals), v1) and np.array_equal(host(res), r1)
     _, jm = amap.jac()
+    # the serial oracle: femx's host assembly order (the OpenMP one scatters
+    # with atomics, in no fixed order)
     ov, orr = O.p1_assemble(geom.h_x, geom.h_y, geom.h_conn.reshape(-1), kappa, f, u, jm,
-                            amap.pattern.nnz, omp=True)
+                            amap.pattern.nnz)
     assert np.array_equal(v1, ov) and np.array_equal(r1, orr)
 
     x, y = geom.h_x, geom.h_y
