@@ -65,6 +65,7 @@ _SIGS = {
     "fx_gls_assemble": ([_i32] * 5 + [_vp] * 5 + [_f64] * 3 + [_i32] * 2 + [_vp] * 9 + [_vp],
                         C.c_int),
     "fx_th_vjp": ([_i32] * 2 + [_vp] * 5 + [_f64] + [_vp] * 8 + [_vp], C.c_int),
+    "fx_th_vjp_noalloc": ([_i32] * 2 + [_vp] * 5 + [_f64] + [_vp] * 7 + [_vp], C.c_int),
     "fx_batch_create": ([_i32, _vp, _i32, _vp, _i32, C.POINTER(_vp)], C.c_int),
     "fx_batch_work": ([_vp, _vp, _vp], C.c_int),
     "fx_batch_run": ([_vp, C.c_int, _vp], C.c_int),

@@ -2,6 +2,10 @@
 // entry points.  The device code and its design notes are in fx_th_dev.hpp.
 #include "fx_th_dev.hpp"
 
+#include <atomic>
+#include <mutex>
+#include <string>
+
 namespace
 {
 using namespace fxk;
@@ -31,6 +35,79 @@ __global__ __launch_bounds__(256) void k_vjp_gather(GatherArgs a)
 {
   __shared__ double sval[kGatherStage];
   vjp_gather_block(fx::xcd_block(gridDim.x), a, sval);
+}
+
+// The two VJP phases with the element-column workspace `work` (15 doubles
+// per element); shared by fx_th_vjp (caller workspace) and
+// fx_th_vjp_noalloc (the library's fixed device workspace).
+void th_vjp_launch(fx_index ndof, fx_index num_elems, const double* xv, const double* yv,
+                   const fx_index* vconn3, const fx_index* dofs15, const double* state, double nu,
+                   const double* nu_e, const double* lam, const fx_index* bc_mask,
+                   const fx_index* inc_off, const fx_index* elem_inc, double* jt, double* g,
+                   double* work, void* stream, const char* who)
+{
+  const std::string w(who);
+  fx::require(ndof >= 0 && num_elems >= 0, FX_ERR_ARG, (w + ": negative size").c_str());
+  if (ndof == 0) return;
+  // state, lam and bc_mask are gathered through 32-bit byte offsets
+  fx::require(fx::fits32(ndof, 8) && fx::fits32(num_elems, 8)
+                  && fx::fits32(15 * static_cast<std::int64_t>(num_elems), 8),
+              FX_ERR_OVERFLOW, (w + ": state, element or workspace arrays beyond 4 GiB").c_str());
+  fx::require(xv && yv && state && lam && inc_off && jt
+                  && (num_elems == 0 || (vconn3 && dofs15 && elem_inc && work)),
+              FX_ERR_ARG, (w + ": null array").c_str());
+  const std::size_t nd = std::size_t(ndof) * 8, ne = std::size_t(num_elems) * 8;
+  fx::require(!fx::overlaps(jt, nd, lam, nd) && !fx::overlaps(jt, nd, state, nd)
+                  && !(g && fx::overlaps(g, ne, jt, nd)) && !fx::overlaps(work, 15 * ne, jt, nd)
+                  && !fx::overlaps(work, 15 * ne, lam, nd)
+                  && !(g && fx::overlaps(work, 15 * ne, g, ne)),
+              FX_ERR_ALIAS, (w + ": outputs must not alias inputs or each other").c_str());
+  if (num_elems > 0)
+  {
+    const ThVjpArgs a{num_elems, xv,      yv,       vconn3, dofs15, state, nu,
+                      nu_e,      lam,     bc_mask,  elem_inc, work, g};
+    hipLaunchKernelGGL(k_th_vjp_elem, dim3(fx::blocks_for(num_elems, kThLanes)), dim3(kThLanes), 0,
+                       fx::as_stream(stream), a);
+    fx::check_launch(who);
+  }
+  const GatherArgs ga{ndof, inc_off, work, lam, bc_mask, jt};
+  hipLaunchKernelGGL(k_vjp_gather, dim3(fx::blocks_for(ndof, 256)), dim3(256), 0,
+                     fx::as_stream(stream), ga);
+  fx::check_launch(who);
+}
+
+// Fixed device workspace of fx_th_vjp_noalloc: 15 doubles per element for
+// up to FX_TH_VJP_NOALLOC_MAX_ELEMS elements (64 MiB), one copy per GPU
+// held until the process ends.  In host code the array's name is not a
+// device address: kernels get the per-device address that
+// hipGetSymbolAddress returns, resolved once per device on first use.
+constexpr std::size_t kNoAllocWorkDoubles = 15 * static_cast<std::size_t>(FX_TH_VJP_NOALLOC_MAX_ELEMS);
+static __device__ __align__(256) double scratch[kNoAllocWorkDoubles];
+
+constexpr int                     kMaxDevices = 64;
+std::mutex                        g_scratch_mutex;
+std::atomic<double*>              g_scratch_ptr[kMaxDevices];
+
+double* scratch_on_current_device()
+{
+  int dev = 0;
+  fx::check_hip(hipGetDevice(&dev), "fx_th_vjp_noalloc: hipGetDevice");
+  fx::require(dev >= 0 && dev < kMaxDevices, FX_ERR_UNSUPPORTED,
+              "fx_th_vjp_noalloc: device ordinal beyond 63");
+  double* p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (p) return p;
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  p = g_scratch_ptr[dev].load(std::memory_order_relaxed);
+  if (!p)
+  {
+    void* addr = nullptr;
+    fx::check_hip(hipGetSymbolAddress(&addr, HIP_SYMBOL(scratch)),
+                  "fx_th_vjp_noalloc: hipGetSymbolAddress(scratch)");
+    fx::require(addr != nullptr, FX_ERR_HIP, "fx_th_vjp_noalloc: workspace has no device address");
+    p = static_cast<double*>(addr);
+    g_scratch_ptr[dev].store(p, std::memory_order_release);
+  }
+  return p;
 }
 
 } // namespace
@@ -76,32 +153,25 @@ int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* xv, const double*
               double* work, void* stream)
 {
   return fx::guard([&] {
-    fx::require(ndof >= 0 && num_elems >= 0, FX_ERR_ARG, "fx_th_vjp: negative size");
+    th_vjp_launch(ndof, num_elems, xv, yv, vconn3, dofs15, state, nu, nu_e, lam, bc_mask, inc_off,
+                  elem_inc, jt, g, work, stream, "fx_th_vjp");
+  });
+}
+
+int fx_th_vjp_noalloc(fx_index ndof, fx_index num_elems, const double* xv, const double* yv,
+                      const fx_index* vconn3, const fx_index* dofs15, const double* state,
+                      double nu, const double* nu_e, const double* lam, const fx_index* bc_mask,
+                      const fx_index* inc_off, const fx_index* elem_inc, double* jt, double* g,
+                      void* stream)
+{
+  return fx::guard([&] {
+    fx::require(num_elems >= 0 && num_elems <= FX_TH_VJP_NOALLOC_MAX_ELEMS, FX_ERR_ARG,
+                "fx_th_vjp_noalloc: num_elems exceeds FX_TH_VJP_NOALLOC_MAX_ELEMS (the fixed "
+                "64 MiB device workspace holds 15 doubles per element); use fx_th_vjp with a "
+                "caller workspace");
     if (ndof == 0) return;
-    // state, lam and bc_mask are gathered through 32-bit byte offsets
-    fx::require(fx::fits32(ndof, 8) && fx::fits32(num_elems, 8)
-                    && fx::fits32(15 * static_cast<std::int64_t>(num_elems), 8),
-                FX_ERR_OVERFLOW, "fx_th_vjp: state, element or workspace arrays beyond 4 GiB");
-    fx::require(xv && yv && state && lam && inc_off && jt
-                    && (num_elems == 0 || (vconn3 && dofs15 && elem_inc && work)),
-                FX_ERR_ARG, "fx_th_vjp: null array");
-    const std::size_t nd = std::size_t(ndof) * 8, ne = std::size_t(num_elems) * 8;
-    fx::require(!fx::overlaps(jt, nd, lam, nd) && !fx::overlaps(jt, nd, state, nd)
-                    && !(g && fx::overlaps(g, ne, jt, nd)) && !fx::overlaps(work, 15 * ne, jt, nd)
-                    && !fx::overlaps(work, 15 * ne, lam, nd) && !(g && fx::overlaps(work, 15 * ne, g, ne)),
-                FX_ERR_ALIAS, "fx_th_vjp: outputs must not alias inputs or each other");
-    if (num_elems > 0)
-    {
-      const ThVjpArgs a{num_elems, xv,      yv,       vconn3, dofs15, state, nu,
-                        nu_e,      lam,     bc_mask,  elem_inc, work, g};
-      hipLaunchKernelGGL(k_th_vjp_elem, dim3(fx::blocks_for(num_elems, kThLanes)), dim3(kThLanes), 0,
-                         fx::as_stream(stream), a);
-      fx::check_launch("fx_th_vjp");
-    }
-    const GatherArgs ga{ndof, inc_off, work, lam, bc_mask, jt};
-    hipLaunchKernelGGL(k_vjp_gather, dim3(fx::blocks_for(ndof, 256)), dim3(256), 0,
-                       fx::as_stream(stream), ga);
-    fx::check_launch("fx_th_vjp");
+    th_vjp_launch(ndof, num_elems, xv, yv, vconn3, dofs15, state, nu, nu_e, lam, bc_mask, inc_off,
+                  elem_inc, jt, g, scratch_on_current_device(), stream, "fx_th_vjp_noalloc");
   });
 }
 

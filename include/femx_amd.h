@@ -339,6 +339,22 @@ int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* d_xv, const doubl
               double nu, const double* d_nu_e, const double* d_lam, const fx_index* d_bc_mask,
               const fx_index* d_inc_off, const fx_index* d_elem_inc, double* d_jt, double* d_g,
               double* d_work, void* stream);
+/* fx_th_vjp without the work argument, for callers that cannot allocate
+ * device memory (code under hipGraph stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in femx.  The element
+ * columns go to a fixed 64 MiB workspace that the library keeps in device
+ * memory, one per GPU, until the process ends: num_elems may be at most
+ * FX_TH_VJP_NOALLOC_MAX_ELEMS (FX_ERR_ARG beyond; the 512 x 512 cavity,
+ * 524 288 elements, fits).  Calls this function makes allocate nothing.
+ * Every call on one GPU shares that GPU's workspace: calls that overlap in
+ * time on one GPU (other streams, other host threads) race, so serialise
+ * them.  Results are bitwise equal to fx_th_vjp's. */
+#define FX_TH_VJP_NOALLOC_MAX_ELEMS 559240
+int fx_th_vjp_noalloc(fx_index ndof, fx_index num_elems, const double* d_xv, const double* d_yv,
+                      const fx_index* d_vconn3, const fx_index* d_dofs15, const double* d_state,
+                      double nu, const double* d_nu_e, const double* d_lam,
+                      const fx_index* d_bc_mask, const fx_index* d_inc_off,
+                      const fx_index* d_elem_inc, double* d_jt, double* d_g, void* stream);
 
 /* ======================================================================== */
 /* GLS-stabilised P1/P1 Navier-Stokes time step (device): femx's

@@ -250,3 +250,15 @@ def test_transpose_pattern_matches_reference(name):
     _, _, perm = pat._host_transpose()
     assert np.array_equal(g[f"{name}_vals"][perm], g[f"{name}_t_vals"])
     assert pat.transposed() is tp  # cached
+
+
+def test_noalloc_vjp_rejects_sizes_beyond_its_workspace():
+    """fx_th_vjp_noalloc refuses more elements than its fixed workspace
+    holds, before touching the device (the pointers are never used)."""
+    p = C.c_void_p(8)
+    with pytest.raises(RuntimeError, match="FX_TH_VJP_NOALLOC_MAX_ELEMS"):
+        _lib.call("fx_th_vjp_noalloc", 8, 559241, p, p, p, p, p, 0.01, None, p, None, p, p, p,
+                  None, None)
+    with pytest.raises(RuntimeError, match="FX_TH_VJP_NOALLOC_MAX_ELEMS"):
+        _lib.call("fx_th_vjp_noalloc", 8, -1, p, p, p, p, p, 0.01, None, p, None, p, p, p, None,
+                  None)
