@@ -22,6 +22,7 @@ _SIGS = {
     "fx_mesh_p1": ([_i32, _vp, _vp, _vp], C.c_int),
     "fx_mesh_p2": ([_i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_th_dofs": ([_i32, _vp, _vp, _vp], C.c_int),
+    "fx_th_mesh": ([_i32, _i32, _vp, C.POINTER(_i32), _vp, _vp, _vp], C.c_int),
     "fx_map_create": ([_i32, _i32, _i32, _vp, _vp, _vp, _vp, C.POINTER(_vp)], C.c_int),
     "fx_map_info_get": ([_vp, _vp], C.c_int),
     "fx_map_export_pattern": ([_vp, _vp, _vp], C.c_int),

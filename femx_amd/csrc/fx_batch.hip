@@ -215,13 +215,14 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     for (int i = 0; i < num_p1; ++i)
     {
       const fx_p1_instance& q = p1[i];
-      fx::require(q.num_nodes > 0 && q.num_elems > 0 && q.max_row_len > 0 && q.max_row_len <= 32,
+      fx::require(q.num_nodes > 0 && q.num_elems > 0 && q.max_row_len > 0 && q.max_row_len <= 255,
                   FX_ERR_ARG, "fx_batch_create: invalid P1 instance sizes");
       fx::require(fx::fits32(q.num_nodes, 8) && fx::fits32(q.num_elems, 8), FX_ERR_OVERFLOW,
                   "fx_batch_create: P1 instance arrays beyond 4 GiB");
       fx::require(q.x && q.y && q.conn && q.inc_off && q.inc && q.row_ptr && q.res && q.vals,
                   FX_ERR_ARG, "fx_batch_create: P1 instance lacks an array");
-      max_row_len = std::max(max_row_len, static_cast<int>(q.max_row_len));
+      // rows past the tile width add into vals in global memory (fx_p1_dev.hpp)
+      max_row_len = std::max(max_row_len, std::min(static_cast<int>(q.max_row_len), kP1MaxRowLen));
       pa.push_back({q.num_nodes, q.x, q.y, q.conn, q.m, 1, q.f, q.u, q.inc_off,
                     reinterpret_cast<const int2*>(q.inc), q.row_ptr, q.vals, q.res, 0});
       pv.push_back({q.num_nodes, q.x, q.y, q.conn, q.m, q.u, q.lam, q.bc_mask, q.inc_off,

@@ -14,6 +14,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "fx_common.hpp"
@@ -157,6 +158,71 @@ int fx_th_dofs(Index n, const Index* conn6, const Index* vconn3, Index* dofs15)
         for (int c = 0; c < 2; ++c) d[2 * a + c] = 2 * conn6[6 * e + a] + c;
       for (int b = 0; b < 3; ++b) d[12 + b] = 2 * nv + vconn3[3 * e + b];
     }
+  });
+}
+
+int fx_th_mesh(Index num_vertices, Index num_elems, const Index* conn3, Index* num_p2_nodes,
+               Index* conn6, Index* edge_ends, Index* dofs15)
+{
+  return fx::guard([&] {
+    fx::require(num_vertices >= 0 && num_elems >= 0 && num_p2_nodes
+                    && (num_elems == 0 || conn3),
+                FX_ERR_ARG, "fx_th_mesh: invalid arguments");
+    // edge (a, b), a < b, -> its node id; first appearance numbers it
+    std::unordered_map<std::uint64_t, Index> edge_id;
+    std::vector<Index>                       uses, ends;
+    edge_id.reserve(static_cast<std::size_t>(num_elems) * 2);
+    std::vector<Index> c6(6 * static_cast<std::size_t>(num_elems));
+    for (Index e = 0; e < num_elems; ++e)
+    {
+      const Index* v = conn3 + 3 * static_cast<std::int64_t>(e);
+      for (int k = 0; k < 3; ++k)
+        fx::require(v[k] >= 0 && v[k] < num_vertices, FX_ERR_ARG,
+                    "fx_th_mesh: vertex index out of range");
+      fx::require(v[0] != v[1] && v[1] != v[2] && v[2] != v[0], FX_ERR_ARG,
+                  "fx_th_mesh: element with a repeated vertex");
+      for (int k = 0; k < 3; ++k) c6[6 * std::size_t(e) + k] = v[k];
+      for (int k = 0; k < 3; ++k)
+      {
+        const Index   a = v[k], b = v[(k + 1) % 3];
+        const Index   lo = std::min(a, b), hi = std::max(a, b);
+        const auto    key = (static_cast<std::uint64_t>(lo) << 32) | static_cast<std::uint32_t>(hi);
+        auto          it  = edge_id.find(key);
+        Index         id;
+        if (it == edge_id.end())
+        {
+          fx::require(static_cast<std::int64_t>(num_vertices) + uses.size()
+                          < std::numeric_limits<Index>::max() / 2,
+                      FX_ERR_OVERFLOW, "fx_th_mesh: P2 node count exceeds int32");
+          id = num_vertices + static_cast<Index>(uses.size());
+          edge_id.emplace(key, id);
+          uses.push_back(1);
+          ends.push_back(a);
+          ends.push_back(b);
+        }
+        else
+        {
+          id = it->second;
+          fx::require(++uses[id - num_vertices] <= 2, FX_ERR_UNSUPPORTED,
+                      "fx_th_mesh: an edge is shared by more than two elements");
+        }
+        c6[6 * std::size_t(e) + 3 + k] = id;
+      }
+    }
+    const Index np2 = num_vertices + static_cast<Index>(uses.size());
+    fx::require(2LL * np2 + num_vertices < std::numeric_limits<Index>::max(), FX_ERR_OVERFLOW,
+                "fx_th_mesh: Taylor-Hood DOF count exceeds int32");
+    *num_p2_nodes = np2;
+    if (conn6) std::copy(c6.begin(), c6.end(), conn6);
+    if (edge_ends) std::copy(ends.begin(), ends.end(), edge_ends);
+    if (dofs15)
+      for (Index e = 0; e < num_elems; ++e)
+      {
+        Index* d = dofs15 + 15 * static_cast<std::int64_t>(e);
+        for (int a = 0; a < 6; ++a)
+          for (int c = 0; c < 2; ++c) d[2 * a + c] = 2 * c6[6 * std::size_t(e) + a] + c;
+        for (int b = 0; b < 3; ++b) d[12 + b] = 2 * np2 + c6[6 * std::size_t(e) + b];
+      }
   });
 }
 
@@ -397,8 +463,9 @@ int fx_map_export_rowinc15(const fx_map* m, Index* inc_off, Index* inc_elem, Ind
 // when the two rows are interchangeable for the kernel -- same incident
 // elements in the same order, local rows li and li+1 with li even and < 12,
 // identical column positions -- and a single row otherwise.  Blocks hold
-// contiguous rows and, when max_nnz > 0, at most max_nnz CSR entries (a block
-// always takes at least one item).  blk_row[num_blocks+1] are block row
+// contiguous rows and at most min(max_nnz, FX_TH_TILE_NNZ_MAX) CSR entries,
+// max_nnz = 0 meaning FX_TH_TILE_NNZ_MAX (a block always takes at least one
+// item).  blk_row[num_blocks+1] are block row
 // starts, blk_flags the item kind (1 = pairs, 0 = single rows);
 // max_block_nnz bounds the LDS row tile.  Call with null arrays to get the
 // counts.
@@ -426,6 +493,9 @@ int fx_map_th_blocks(const fx_map* m, Index lanes, Index max_nnz, Index* num_blo
       }
       return true;
     };
+    // every block's tile fits the kernels' LDS (a single item always does:
+    // two rows of at most 255 entries)
+    const Index cap = (max_nnz == 0 || max_nnz > FX_TH_TILE_NNZ_MAX) ? FX_TH_TILE_NNZ_MAX : max_nnz;
     std::vector<Index> starts, flags;
     Index              r = 0, mx = 0;
     while (r < m->num_res)
@@ -436,7 +506,7 @@ int fx_map_th_blocks(const fx_map* m, Index lanes, Index max_nnz, Index* num_blo
       while (r < m->num_res && items < lanes && pairable(r) == pair)
       {
         const Index next = r + (pair ? 2 : 1);
-        if (items > 0 && max_nnz > 0 && m->row_ptr[next] - m->row_ptr[start] > max_nnz) break;
+        if (items > 0 && m->row_ptr[next] - m->row_ptr[start] > cap) break;
         r = next;
         ++items;
       }

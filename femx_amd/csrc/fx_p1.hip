@@ -3,6 +3,8 @@
 // fx_p1_dev.hpp.
 #include "fx_p1_dev.hpp"
 
+#include <algorithm>
+
 namespace
 {
 using namespace fxk;
@@ -37,15 +39,16 @@ int fx_p1_assemble(fx_index num_nodes, fx_index num_elems, fx_index nnz, fx_inde
                 FX_ERR_ARG, "fx_p1_assemble: null array");
     fx::require(fx::fits32(num_nodes, 8) && fx::fits32(num_elems, 8), FX_ERR_OVERFLOW,
                 "fx_p1_assemble: node or element arrays beyond 4 GiB");
-    fx::require(max_row_len >= 0 && max_row_len <= kP1MaxRowLen, FX_ERR_UNSUPPORTED,
-                "fx_p1_assemble: CSR rows longer than 32 entries");
+    fx::require(max_row_len >= 0 && max_row_len <= 255, FX_ERR_UNSUPPORTED,
+                "fx_p1_assemble: CSR rows longer than 255 entries");
     const std::size_t N = num_nodes;
     fx::require(!fx::overlaps(res, N * 8, vals, std::size_t(nnz) * 8)
                     && !(u && (fx::overlaps(u, N * 8, res, N * 8)
                                || fx::overlaps(u, N * 8, vals, std::size_t(nnz) * 8))),
                 FX_ERR_ALIAS, "Assembly state, residual, and matrix values must not alias");
     const unsigned nb      = fx::blocks_for(num_nodes, kP1Block);
-    const int      acc_cap = kP1Block * (max_row_len > 0 ? max_row_len : 1);
+    // blocks whose CSR range exceeds the tile add into vals in global memory
+    const int      acc_cap = kP1Block * std::min(std::max(max_row_len, 1), kP1MaxRowLen);
     const size_t   lds     = static_cast<size_t>(acc_cap) * 8 + kIncStage * 8;
     const P1AsmArgs a{num_nodes, x,  y,       conn, kappa, kappa_is_log, f, u, inc_off,
                       reinterpret_cast<const int2*>(inc), row_ptr, vals, res, acc_cap};

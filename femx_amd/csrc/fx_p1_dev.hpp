@@ -31,7 +31,12 @@ namespace fxk
 // 128 rows per workgroup: finer blocks keep more of the chip busy as the
 // launch drains (2% faster than 256 at 2048^2; 64 is slower again).
 constexpr int kP1Block = 128;
-constexpr int kP1MaxRowLen = 32; // LDS bound: 128 rows x 32 x 8 B = 32 KiB (+8 KiB staging)
+// LDS row tile: 128 rows x 32 x 8 B = 32 KiB (+8 KiB staging).  A block whose
+// CSR range is longer than its tile (rows of more than 32 entries) adds into
+// the values in global memory instead, each lane in its own row, in the same
+// order: rows of any length the 8-bit column positions allow (255) assemble,
+// bitwise as through the tile.
+constexpr int kP1MaxRowLen = 32;
 
 // Physical P1 gradients and |det J|*wt of an affine triangle, evaluated as
 // ElementValues::calcPhysicalValues/invJacobian do with the P1 reference
@@ -202,7 +207,9 @@ __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, d
   const int   bn   = a.vals ? a.row_ptr[r1] - base : 0;
   const int   ib0 = a.inc_off[r0], ib1 = a.inc_off[r1];
   const bool  staged = ib1 - ib0 <= kIncStage;
-  for (int i = threadIdx.x; i < bn; i += kP1Block) acc[i] = 0.0;
+  const bool  tiled  = bn <= a.acc_cap; // else: rows add straight into a.vals
+  if (tiled)
+    for (int i = threadIdx.x; i < bn; i += kP1Block) acc[i] = 0.0;
   if (staged)
     stage_nt<kP1Block>(sinc, reinterpret_cast<const long long*>(a.inc) + ib0, ib1 - ib0);
   __syncthreads();
@@ -210,9 +217,11 @@ __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, d
   const int r = r0 + threadIdx.x;
   if (r < a.num_rows)
   {
-    double*   racc = acc + (a.row_ptr[r] - base);
+    double*   racc = tiled ? acc + (a.row_ptr[r] - base) : a.vals + a.row_ptr[r];
     double    rsum = 0.0;
     const int ta = a.inc_off[r], t1 = a.inc_off[r + 1], tl = t1 - 1;
+    if (!tiled && a.vals)
+      for (int k = 0; k < a.row_ptr[r + 1] - a.row_ptr[r]; ++k) racc[k] = 0.0;
     if (ta < t1)
     {
       const P1Own o{a.x[r], a.y[r], a.u ? a.u[r] : 0.0, a.f ? a.f[r] : 0.0};
@@ -235,9 +244,22 @@ __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, d
         rsum = rsum + re;
         if (a.vals)
         {
-          lds_add(racc + ((meta >> 8) & 0xff), k0);
-          lds_add(racc + ((meta >> 16) & 0xff), k1);
-          lds_add(racc + (meta >> 24), k2);
+          double* p0 = racc + ((meta >> 8) & 0xff);
+          double* p1 = racc + ((meta >> 16) & 0xff);
+          double* p2 = racc + (meta >> 24);
+          if (tiled)
+          {
+            lds_add(p0, k0);
+            lds_add(p1, k1);
+            lds_add(p2, k2);
+          }
+          else
+          {
+            // the lane's own row in global memory: read, add, write, in order
+            *p0 = *p0 + k0;
+            *p1 = *p1 + k1;
+            *p2 = *p2 + k2;
+          }
         }
         ic = in;
       }
@@ -245,8 +267,9 @@ __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, d
     __builtin_nontemporal_store(rsum, a.res + r);
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < bn; i += kP1Block)
-    __builtin_nontemporal_store(acc[i], a.vals + base + i);
+  if (tiled)
+    for (int i = threadIdx.x; i < bn; i += kP1Block)
+      __builtin_nontemporal_store(acc[i], a.vals + base + i);
   __syncthreads();
 }
 

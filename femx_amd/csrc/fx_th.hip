@@ -26,6 +26,9 @@ __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per
   if (b < num_blocks) th_assemble_block(b, a, T, smem + kThTensorLen + w * tile);
 }
 
+static_assert(sizeof(ThTensor) + std::size_t(FX_TH_TILE_NNZ_MAX) * 8 * kThWaves <= 160 * 1024,
+              "the largest Taylor-Hood row tile must fit the workgroup's LDS");
+
 __global__ __launch_bounds__(kThLanes) void k_th_vjp_elem(ThVjpArgs a)
 {
   th_vjp_elem_block(fx::xcd_block(gridDim.x), a, kThTables);

@@ -66,6 +66,30 @@ def structured_th(n, device="cuda"):
     return geom
 
 
+def th_from_p1(x, y, conn3, device="cuda"):
+    """Taylor-Hood mesh over any conforming triangle mesh given as femx holds
+    a P1 mesh (vertex coordinates and 3 vertex ids per triangle, e.g. from
+    femx's GmshReader): P2 edge nodes at the edge midpoints, numbered after
+    the vertices in order of first appearance (fx_th_mesh)."""
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    conn3 = np.ascontiguousarray(conn3, np.int32).reshape(-1)
+    E = conn3.size // 3
+    np2 = C.c_int32()
+    _lib.call("fx_th_mesh", x.size, E, ptr(conn3), C.byref(np2), None, None, None)
+    ne = np2.value - x.size
+    c6 = np.empty(max(6 * E, 1), np.int32)
+    ends = np.empty(max(2 * ne, 1), np.int32)
+    d = np.empty(max(15 * E, 1), np.int32)
+    _lib.call("fx_th_mesh", x.size, E, ptr(conn3), C.byref(np2), ptr(c6), ptr(ends), ptr(d))
+    ends = ends[: 2 * ne].reshape(-1, 2)
+    geom = THGeometry(x, y, conn3, c6[: 6 * E], d[: 15 * E], np2.value, device)
+    geom.h_x2 = np.concatenate([x, 0.5 * (x[ends[:, 0]] + x[ends[:, 1]])])
+    geom.h_y2 = np.concatenate([y, 0.5 * (y[ends[:, 0]] + y[ends[:, 1]])])
+    geom.h_edge_ends = ends
+    return geom
+
+
 class THPlan:
     """Row-incidence tables, work blocks and the VJP workspace of the
     Taylor-Hood kernels for one assembly map, on one device.  Calls that share

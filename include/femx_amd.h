@@ -60,6 +60,21 @@ int fx_mesh_p2(fx_index n, double* h_x, double* h_y, fx_index* h_conn6, fx_index
  * 121-137). */
 int fx_th_dofs(fx_index n, const fx_index* h_conn6, const fx_index* h_vconn3,
                fx_index* h_dofs15);
+/* Taylor-Hood tables of any conforming triangle mesh (host), as femx's P1
+ * mesh (femx/fem/Mesh.hpp, e.g. from GmshReader) gives it: num_vertices
+ * vertices and conn3[3E].  P2 node ids: vertex v keeps id v; edge nodes
+ * follow, numbered num_vertices, num_vertices+1, ... in order of first
+ * appearance over the elements in order, local edges (0,1), (1,2), (2,0)
+ * (Gmsh 6-node order).  Writes *num_p2_nodes, and when the arrays are
+ * non-null conn6[6E], edge_ends[2*(num_p2_nodes - num_vertices)] (each edge
+ * node's two vertices, in first-appearance order) and dofs15[15E] (the
+ * MixedFESpace order of fx_th_dofs: 2*p2node+c, then 2*num_p2_nodes +
+ * vertex).  Call once with null arrays for the count.  FX_ERR_ARG for an
+ * out-of-range vertex or a degenerate element (repeated vertex);
+ * FX_ERR_UNSUPPORTED for an edge shared by more than two elements. */
+int fx_th_mesh(fx_index num_vertices, fx_index num_elems, const fx_index* h_conn3,
+               fx_index* num_p2_nodes, fx_index* h_conn6, fx_index* h_edge_ends,
+               fx_index* h_dofs15);
 
 /* ======================================================================== */
 /* Assembly map (host).  Replaces femx::assembly::makeAssemblyMap
@@ -109,15 +124,21 @@ int fx_map_export_rowinc(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc
 int fx_map_export_rowinc15(const fx_map* map, fx_index* h_inc_off, fx_index* h_inc_elem,
                            fx_index* h_inc_meta);
 /* Work blocks of the Taylor-Hood kernels: rows cut into contiguous blocks of
- * at most `lanes` work items and, when max_nnz > 0, at most max_nnz CSR
- * entries (a block always holds at least one item).  An item is a pair of
+ * at most `lanes` work items and at most min(max_nnz, FX_TH_TILE_NNZ_MAX)
+ * CSR entries, max_nnz = 0 meaning FX_TH_TILE_NNZ_MAX (a block always holds
+ * at least one item).  An item is a pair of
  * interchangeable velocity rows (r, r+1) -- same elements, local rows li and
  * li+1 (li even, < 12), same column positions -- or a single row.
  * blk_row[num_blocks+1], blk_flags[num_blocks] (1 = pairs); max_block_nnz
  * sizes the per-wave LDS row tile.  Pass null arrays to query num_blocks and
  * max_block_nnz. */
 #define FX_TH_BLOCK_ITEMS 32 /* work items per block the kernels are tuned for */
-#define FX_TH_TILE_NNZ 0     /* default tile bound (0: none; items bound the tile) */
+#define FX_TH_TILE_NNZ 0     /* default tile bound (0: the LDS capacity below) */
+/* The largest row tile the assembly kernels hold per wave (LDS: four waves'
+ * tiles and the reference tensors in 160 KiB).  max_nnz = 0 or above it
+ * means this bound, so that every block fits; a block still always holds at
+ * least one item. */
+#define FX_TH_TILE_NNZ_MAX 4928
 int fx_map_th_blocks(const fx_map* map, fx_index lanes, fx_index max_nnz, fx_index* num_blocks,
                      fx_index* max_block_nnz, fx_index* h_blk_row, fx_index* h_blk_flags);
 /* Lane map of those blocks (host): each block's 64 lanes shared out so its

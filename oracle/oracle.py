@@ -80,6 +80,8 @@ def ref():
                                   + [C.c_double, C.c_double])
         _REF.ref_poisson_p1.argtypes = ([C.c_int32, _pd, _pd, C.c_int32, _pi, _pd, C.c_int32,
                                          _pi, _pi, _pd, _pd])
+        _REF.ref_gmsh_read.argtypes = [C.c_char_p, C.c_int, _pi, _pd, _pd, _pi, _pi, _pi]
+        _REF.ref_gmsh_bc_dofs.argtypes = [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _pi, _pi]
         _REF.ref_ns_gls.argtypes = ([C.c_int32, _pd, _pd, C.c_int32, _pi] + [C.c_double] * 3
                                     + [C.c_int32, C.c_int32, _pd, _pd, C.c_int32]
                                     + [_pi, _pi, _pi, _pd, _pd])

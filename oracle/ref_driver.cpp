@@ -19,7 +19,9 @@
 #include <femx/assembly/BoundaryMap.hpp>
 #include <femx/fem/DofLayout.hpp>
 #include <femx/fem/ElementQuadratureData.hpp>
+#include <femx/fem/DirichletBC.hpp>
 #include <femx/fem/FESpace.hpp>
+#include <femx/fem/GmshReader.hpp>
 #include <femx/fem/GaussQuadrature.hpp>
 #include <femx/fem/Geometry.hpp>
 #include <femx/fem/Mesh.hpp>
@@ -411,6 +413,66 @@ int ref_ns_constrained(int query, Index num_nodes, const Real* x, const Real* y,
     cres.prepareLinearSolve(time, jac, rhs, ctx);
     std::memcpy(prep_vals, jac.valsData(), sizeof(Real) * dims[3]);
     for (Index i = 0; i < ns; ++i) prep_rhs[i] = rhs[i];
+  });
+}
+
+
+// femx::fem::GmshReader::read (femx/fem/GmshReader.cpp:463-538) of a 2-D
+// triangle mesh.  With query != 0 it writes dims = {num_nodes,
+// num_triangles, num_facets} only; otherwise it also writes the node
+// coordinates x, y, the triangles' node ids conn (3 per triangle, in the
+// mesh's element order) and, per boundary facet (2 nodes each), its
+// physical tag and node ids.
+int ref_gmsh_read(const char* path, int query, Index* dims, Real* x, Real* y, Index* conn,
+                  Index* facet_ptag, Index* facet_nids)
+{
+  return guarded([&] {
+    const femx::fem::Mesh mesh = femx::fem::GmshReader::read(path);
+    femx::require(mesh.dim() == 2, "expected a 2-D mesh");
+    dims[0] = mesh.numNodes();
+    dims[1] = mesh.numElems();
+    dims[2] = mesh.boundaryFacets().size();
+    if (query) return;
+    for (Index i = 0; i < mesh.numNodes(); ++i)
+    {
+      x[i] = mesh.node(i)[0];
+      y[i] = mesh.node(i)[1];
+    }
+    for (Index e = 0; e < mesh.numElems(); ++e)
+    {
+      femx::require(mesh.elem(e).numNodes() == 3, "expected triangles only");
+      const Index* n = mesh.elemNodeIds(e);
+      for (int k = 0; k < 3; ++k) conn[3 * e + k] = n[k];
+    }
+    const auto& facets = mesh.boundaryFacets();
+    for (Index f = 0; f < static_cast<Index>(facets.size()); ++f)
+    {
+      femx::require(facets[f].nids.size() == 2, "expected 2-node boundary facets");
+      facet_ptag[f]       = facets[f].ptag;
+      facet_nids[2 * f]     = facets[f].nids[0];
+      facet_nids[2 * f + 1] = facets[f].nids[1];
+    }
+  });
+}
+
+// femx::fem::DirichletBC::addBoundary(space, ptag, value, time, comp)
+// (femx/fem/DirichletBC.cpp:34-65) on the P1 space of the mesh read from
+// `path`, with ncomp components per node: writes the constrained DOFs in the
+// order femx adds them (count first; dofs may be null to query).
+int ref_gmsh_bc_dofs(const char* path, Index ptag, Index ncomp, Index comp, Index* count,
+                     Index* dofs)
+{
+  return guarded([&] {
+    const femx::fem::Mesh         mesh = femx::fem::GmshReader::read(path);
+    femx::fem::LagrangeTriangleP1 fe;
+    femx::fem::FESpace            space(&mesh, &fe, ncomp);
+    space.setup();
+    femx::fem::DirichletBC bc;
+    bc.addBoundary(space, ptag, 0.0, 0.0, comp);
+    const auto& d = bc.dofs();
+    *count        = static_cast<Index>(d.size());
+    if (dofs)
+      for (Index i = 0; i < *count; ++i) dofs[i] = d[i];
   });
 }
 

@@ -20,6 +20,10 @@ conventions, seeded values) and the reference's outputs:
   gls_constrained_n4.npz  NavierStokesModel residual under ConstrainedTimeResidual:
                           res, assembleNext, applyJacT(param), and
                           TimeIntegrator::solveStep's rhs/prepareLinearSolve
+  gmsh_*.npz              femx's own Gmsh meshes (data/meshes) through its
+                          GmshReader: makeAssemblyMap, PoissonComponents,
+                          makeBoundaryMap on physical groups +
+                          prepareForwardSolve, NavierOperator time assembly
 
     python tests/golden/make_golden.py
 """
@@ -260,7 +264,117 @@ def main():
                      f"{name}_t_col_ind": tci, f"{name}_t_vals": tv})
     np.savez_compressed(os.path.join(HERE, "transpose.npz"), **tout)
 
+    gmsh_fixtures(R)
     print("fixtures written to", HERE)
+
+
+MESHES = "/root/reference/data/meshes"
+
+
+def gmsh_read(R, path):
+    """femx's GmshReader::read: node coordinates, triangles, and boundary
+    facets (physical tag, 2 node ids)."""
+    dims = np.zeros(3, I32)
+    assert R.ref_gmsh_read(path.encode(), 1, p(dims), None, None, None, None, None) == 0, \
+        R.ref_last_error()
+    N, E, F = (int(v) for v in dims)
+    x, y = np.empty(N), np.empty(N)
+    conn, fp, fn = np.empty(3 * E, I32), np.empty(F, I32), np.empty(2 * F, I32)
+    assert R.ref_gmsh_read(path.encode(), 0, p(dims), p(x), p(y), p(conn), p(fp), p(fn)) == 0
+    return x, y, conn, fp, fn
+
+
+def gmsh_bc(R, path, ptag, ncomp, comp):
+    """DirichletBC::addBoundary(space, ptag, ...) on the P1 space with ncomp
+    components: the DOFs in femx's order."""
+    n = C.c_int32()
+    assert R.ref_gmsh_bc_dofs(path.encode(), ptag, ncomp, comp, C.byref(n), None) == 0, \
+        R.ref_last_error()
+    d = np.empty(n.value, I32)
+    assert R.ref_gmsh_bc_dofs(path.encode(), ptag, ncomp, comp, C.byref(n), p(d)) == 0
+    return d
+
+
+def ref_bcmap(R, rp, ci, dofs):
+    N = rp.size - 1
+    h = C.c_void_p()
+    assert R.ref_bc_build(N, p(rp), p(ci), dofs.size, p(dofs), C.byref(h)) == 0, R.ref_last_error()
+    nce = R.ref_bc_num_col_entries(h)
+    arrs = dict(bc_rows=np.empty(dofs.size, I32), diag=np.empty(dofs.size, I32),
+                col_offsets=np.empty(dofs.size + 1, I32), col_entries=np.empty(nce, I32),
+                col_rows=np.empty(nce, I32), bc_mask=np.empty(N, I32))
+    R.ref_bc_export(h, *(p(arrs[k]) for k in ("bc_rows", "diag", "col_offsets", "col_entries",
+                                              "col_rows", "bc_mask")))
+    R.ref_bc_free(h)
+    return arrs
+
+
+def ref_forward(R, rp, ci, dofs, vals, rhs, g):
+    """prepareForwardSolve (BoundaryMap.cpp:218-247) on copies."""
+    v, r = vals.copy(), rhs.copy()
+    assert R.ref_bc_apply(1, rp.size - 1, p(rp), p(ci), dofs.size, p(dofs), p(v), p(r), p(g),
+                          p(np.zeros(rp.size - 1)), 1.0) == 0, R.ref_last_error()
+    return v, r
+
+
+def gmsh_fixtures(R):
+    """femx's own unstructured meshes (data/meshes, Gmsh 4.1) read by femx's
+    GmshReader: makeAssemblyMap on the P1 DOF table, PoissonComponents
+    assembly (kappa = 1), makeBoundaryMap on physical groups with
+    prepareForwardSolve, and NavierOperator time assembly with the Navier-
+    Stokes model's boundary DOFs (velocity on walls and inlet, one pressure
+    DOF; ascending as femx's orderedBC leaves them, model/ns/Boundary.cpp:
+    19-33, 242-299) and prepareForwardSolve on that system.
+        gmsh_2d_square_res_low.npz  (walls: bottom 12, then top 9)
+        gmsh_2d_rectangle.npz       (wall 6; NS: wall 6 and inlet 4)
+        gmsh_2d_square_res_very_low.npz  (NS: all four walls)
+    and the mesh alone (coordinates, triangles, facets) of the next finer
+    square, for refinement studies:
+        gmsh_2d_square_res_mid.npz"""
+    rng = np.random.default_rng(20261204)
+    cases = {"2d_square_res_low": ([12, 9], None),
+             "2d_rectangle": ([6], [6, 4]),
+             "2d_square_res_very_low": ([12, 9], [9, 10, 11, 12])}
+    for name, (p1_tags, ns_tags) in cases.items():
+        path = os.path.join(MESHES, name + ".msh")
+        x, y, conn, fp, fn = gmsh_read(R, path)
+        N, E = x.size, conn.size // 3
+        out = dict(x=x, y=y, conn=conn, facet_ptag=fp, facet_nids=fn)
+        off = O.uniform_offsets(E, 3)
+        rp, ci, jo, jm = ref_map(R, N, N, off, conn)
+        out.update(row_ptr=rp, col_ind=ci, jac_offsets=jo, jac_map=jm)
+        u = rng.standard_normal(N)
+        vals = np.empty(ci.size)
+        res = np.empty(N)
+        rp2 = np.empty(N + 1, I32)
+        ci2 = np.empty(ci.size, I32)
+        assert R.ref_poisson_p1(N, p(x), p(y), E, p(conn), p(u), ci.size, p(rp2), p(ci2),
+                                p(vals), p(res)) == 0, R.ref_last_error()
+        assert np.array_equal(rp2, rp) and np.array_equal(ci2, ci)
+        out.update(state=u, vals=vals, res=res)
+        dofs = np.concatenate([gmsh_bc(R, path, t, 1, 0) for t in p1_tags]).astype(I32)
+        bc = ref_bcmap(R, rp, ci, dofs)
+        out.update(bc_dofs=dofs, **{"bc_" + k: v for k, v in bc.items()})
+        rhs0, g = rng.standard_normal(N), rng.standard_normal(dofs.size)
+        fv, fr = ref_forward(R, rp, ci, dofs, vals, rhs0, g)
+        out.update(fwd_rhs0=rhs0, fwd_g=g, fwd_vals=fv, fwd_rhs=fr)
+        if ns_tags is not None:
+            nd = 3 * N
+            hist, nxt = rng.standard_normal(2 * nd), rng.standard_normal(nd)
+            d9, grp, gci, gvals, gres = O.ref_ns_gls(x, y, conn, 1.0, 1e-3, 0.01, 1, 2, hist, nxt)
+            vel = [gmsh_bc(R, path, t, 2, c) for t in ns_tags for c in (0, 1)]
+            nsd = np.unique(np.concatenate(vel + [np.array([2 * N], I32)])).astype(I32)
+            gbc = ref_bcmap(R, grp, gci, nsd)
+            grhs0, gg = rng.standard_normal(nd), rng.standard_normal(nsd.size)
+            gfv, gfr = ref_forward(R, grp, gci, nsd, gvals, grhs0, gg)
+            out.update(ns_hist=hist, ns_nxt=nxt, ns_dofs9=d9, ns_row_ptr=grp, ns_col_ind=gci,
+                       ns_vals=gvals, ns_res=gres, ns_bc_dofs=nsd, ns_bc_mask=gbc["bc_mask"],
+                       ns_bc_diag=gbc["diag"], ns_fwd_rhs0=grhs0, ns_fwd_g=gg, ns_fwd_vals=gfv,
+                       ns_fwd_rhs=gfr)
+        np.savez_compressed(os.path.join(HERE, f"gmsh_{name}.npz"), **out)
+    x, y, conn, fp, fn = gmsh_read(R, os.path.join(MESHES, "2d_square_res_mid.msh"))
+    np.savez_compressed(os.path.join(HERE, "gmsh_2d_square_res_mid.npz"), x=x, y=y, conn=conn,
+                        facet_ptag=fp, facet_nids=fn)
 
 
 if __name__ == "__main__":
