@@ -59,6 +59,7 @@ _SIGS = {
     "fx_csr_matvec_multi": ([_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32, _f64,
                              _f64, _vp], C.c_int),
     "fx_vec_div": ([_i32, _vp, _vp, _vp, _vp], C.c_int),
+    "fx_vec_fill": ([_i32, _f64, _vp, _vp], C.c_int),
     "fx_vec_gather": ([_i32, _vp, _vp, _i32, _vp, _vp], C.c_int),
     "fx_vec_scatter": ([_i32, _vp, _vp, _vp, _i32, _vp], C.c_int),
     "fx_gls_hist_vjp": ([_i32] * 3 + [_vp] * 5 + [_f64] * 3 + [_i32] * 3 + [_vp] * 7 + [_vp],

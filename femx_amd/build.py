@@ -71,7 +71,7 @@ CPP_TEST = os.path.join(OBJ, "test_femx_hip")
 def build_cpp_tests():
     """The C++ interface tests (tests/cpp), linked against the library."""
     src = os.path.join(ROOT, "tests", "cpp", "test_femx_hip.cpp")
-    cmd = [hipcc(), src, "-o", CPP_TEST, "-O2", "-std=c++17", "-I" + os.path.join(HERE, "include"),
+    cmd = [hipcc(), src, "-o", CPP_TEST, "-O2", "-std=c++17", "--offload-arch=" + ARCH, "-I" + os.path.join(HERE, "include"),
            "-I" + os.path.join(ROOT, "include"), "-L" + HERE, "-lfemx_amd",
            "-Wl,-rpath,$ORIGIN/.."]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -221,6 +221,12 @@ __global__ __launch_bounds__(kRowBlock) void k_div(int n, const double* x, const
   if (i < n) z[i] = x[i] / d[i];
 }
 
+__global__ __launch_bounds__(kRowBlock) void k_fill(int n, double v, double* y)
+{
+  const int i = blockIdx.x * kRowBlock + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
 __global__ __launch_bounds__(kRowBlock) void k_scatter(int n, const int* __restrict__ idx,
                                                        const double* __restrict__ src,
                                                        double* __restrict__ dst)
@@ -414,6 +420,18 @@ int fx_vec_div(fx_index n, const double* x, const double* d, double* z, void* st
     hipLaunchKernelGGL(k_div, dim3(fx::blocks_for(n, kRowBlock)), dim3(kRowBlock), 0,
                        fx::as_stream(stream), n, x, d, z);
     fx::check_launch("fx_vec_div");
+  });
+}
+
+int fx_vec_fill(fx_index n, double v, double* y, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(n >= 0, FX_ERR_ARG, "fx_vec_fill: negative size");
+    if (n == 0) return;
+    fx::require(y, FX_ERR_ARG, "fx_vec_fill: null array");
+    hipLaunchKernelGGL(k_fill, dim3(fx::blocks_for(n, kRowBlock)), dim3(kRowBlock), 0,
+                       fx::as_stream(stream), n, v, y);
+    fx::check_launch("fx_vec_fill");
   });
 }
 

@@ -14,8 +14,14 @@
 //   assembly::assemble               femx/assembly/CudaAssembly.hpp:319-364
 //   linalg::HipMatrixHandler         femx/linalg/handler/MatrixHandler.hpp:193-378
 //   linalg::HipVectorHandler         femx/linalg/handler/VectorHandler.hpp:190-477
-//   state::Residual                  femx/state/Residual.hpp:18-60
+//   femx::VectorView                 femx/linalg/View.hpp:13-90
+//   state::Residual                  femx/state/Residual.hpp:16-60
 //   state::PoissonLogKappaResidual   a Residual for the P1 log-kappa inverse problem
+//   state::NavierStokesResidual      a Residual for the Taylor-Hood nu_e inverse problem
+//
+// femx_hip/time.hpp holds the time-residual contract, femx's Navier-Stokes
+// model residual, Dirichlet control and ConstrainedTimeResidual;
+// femx_hip/solver.hpp the LinearSolver contract and a device BiCGStab.
 //
 // Errors surface as std::runtime_error carrying the library's message, as
 // femx's require() does (femx/common/Checks.hpp:11-17).  Device work is
@@ -148,6 +154,71 @@ private:
 using DeviceVector      = DeviceArray<Real>;
 using DeviceIndexVector = DeviceArray<Index>;
 
+/** Non-owning view of contiguous device values (femx's VectorView<Device, T>,
+ * femx/linalg/View.hpp:13-90): a pointer and a size. */
+template <class T>
+class VectorView
+{
+public:
+  VectorView() = default;
+  VectorView(T* data, Index size) : data_(data), size_(size) {}
+  template <class U>
+  VectorView(const VectorView<U>& o) : data_(o.data()), size_(o.size())
+  {
+  }
+  template <class U>
+  VectorView(DeviceArray<U>& a) : data_(a.data()), size_(a.size())
+  {
+  }
+  template <class U>
+  VectorView(const DeviceArray<U>& a) : data_(a.data()), size_(a.size())
+  {
+  }
+  T*    data() const noexcept { return data_; }
+  Index size() const noexcept { return size_; }
+  bool  empty() const noexcept { return size_ == 0; }
+  bool  isValid() const noexcept { return size_ >= 0 && (size_ == 0 || data_ != nullptr); }
+  /** The sub-range [off, off + n) (femx's BlockVectorView::block). */
+  VectorView block(Index off, Index n) const
+  {
+    require(off >= 0 && n >= 0 && off + n <= size_, "VectorView block out of range");
+    return {data_ + off, n};
+  }
+
+private:
+  T*    data_ = nullptr;
+  Index size_ = 0;
+};
+
+using DeviceVectorView      = VectorView<Real>;
+using DeviceConstVectorView = VectorView<const Real>;
+using DeviceIndexView       = VectorView<Index>;
+using DeviceConstIndexView  = VectorView<const Index>;
+
+/** Views of a device array (femx's Vector::view). */
+template <class T>
+VectorView<T> view(DeviceArray<T>& a)
+{
+  return {a.data(), a.size()};
+}
+template <class T>
+VectorView<const T> view(const DeviceArray<T>& a)
+{
+  return {a.data(), a.size()};
+}
+
+namespace detail
+{
+inline void fill(Real* y, Index n, Real v, void* stream) { check(fx_vec_fill(n, v, y, stream)); }
+
+inline bool overlap(const void* a, std::size_t na, const void* b, std::size_t nb)
+{
+  if (!a || !b || na == 0 || nb == 0) return false;
+  auto pa = reinterpret_cast<std::uintptr_t>(a), pb = reinterpret_cast<std::uintptr_t>(b);
+  return pa < pb + nb && pb < pa + na;
+}
+} // namespace detail
+
 namespace detail
 {
 inline std::uint64_t newLayoutId()
@@ -203,8 +274,19 @@ public:
       m->t_row.assign(tr, ctx);
       m->t_perm.assign(tq, ctx);
       tmap_ = std::move(m);
+      HostCsrPattern t; // columns become rows, each row's columns ascending
+      t.rows = cols_, t.cols = rows_, t.row_ptr = std::move(tp), t.col_ind = std::move(tr);
+      tpat_ = std::make_shared<DeviceCsrPattern>(t, ctx);
     }
     return *tmap_;
+  }
+
+  /** The transposed pattern (a layout of its own), built with the
+   * transpose map and cached with it (MatrixHandler.cu:401-512). */
+  std::shared_ptr<const DeviceCsrPattern> transposed(HipContext& ctx) const
+  {
+    transposeMap(ctx);
+    return tpat_;
   }
 
   /** Cached 16-bit column offsets (col_ind - row) on the device, or null when
@@ -231,6 +313,7 @@ private:
   HostCsrPattern                                    host_;
   DeviceIndexVector                                 row_ptr_, col_ind_;
   mutable std::shared_ptr<TransposeMap>             tmap_;
+  mutable std::shared_ptr<const DeviceCsrPattern>   tpat_;
   mutable std::shared_ptr<DeviceArray<std::int16_t>> off16_;
   mutable bool                                      off16_tried_ = false;
 };
@@ -245,13 +328,15 @@ public:
     vals_.resize(pattern_->nnz());
   }
   const DeviceCsrPattern& pattern() const { return *pattern_; }
+  bool                    hasPattern() const noexcept { return pattern_ != nullptr; }
   std::shared_ptr<const DeviceCsrPattern> sharedPattern() const { return pattern_; }
   Index        rows() const { return pattern_->rows(); }
   Index        cols() const { return pattern_->cols(); }
   Index        nnz() const { return pattern_->nnz(); }
   Real*        valsData() { return vals_.data(); }
   const Real*  valsData() const { return vals_.data(); }
-  DeviceVector& vals() { return vals_; }
+  DeviceVector&       vals() { return vals_; }
+  const DeviceVector& vals() const { return vals_; }
 
 private:
   std::shared_ptr<const DeviceCsrPattern> pattern_;
@@ -328,7 +413,11 @@ struct DeviceAssemblyMap
   fx_map_info                             info{};
   std::shared_ptr<const DeviceCsrPattern> pattern;
   DeviceIndexVector inc_off, inc, inc_elem, inc_meta, blk_row, blk_flags, blk_lanes;
-  Index                                   num_blocks = 0, max_block_nnz = 0;
+  DeviceIndexVector elem_inc; // 15x15 maps: (element, local row) -> row incidence (the VJP)
+  Index             num_blocks = 0, max_block_nnz = 0;
+  // Taylor-Hood VJP workspace (15 doubles per element), sized on first use;
+  // VJPs of one map must not overlap
+  mutable DeviceVector vjp_work;
 };
 
 /** Host -> device copy (femx copy(map, map, ctx)); builds the row-incidence
@@ -365,6 +454,9 @@ inline void copy(const HostAssemblyMap& src, DeviceAssemblyMap& dst, HipContext&
     dst.blk_lanes.assign(bl, ctx);
     dst.num_blocks    = nb;
     dst.max_block_nnz = mx;
+    std::vector<Index> ei(15 * static_cast<std::size_t>(src.numElems()));
+    check(fx_map_export_rowinc(src.handle(), off.data(), nullptr, ei.data()));
+    dst.elem_inc.assign(ei, ctx);
   }
 }
 
@@ -442,8 +534,8 @@ inline void replaceRows(const DeviceBoundaryMap& map, DeviceCsrMatrix& jac, Real
                            jac.pattern().rowPtrData(), jac.valsData(), diag, ctx.raw()));
 }
 
-inline void replaceRes(const DeviceBoundaryMap& map, const DeviceVector& state,
-                       const DeviceVector& bc_vals, DeviceVector& res, HipContext& ctx)
+inline void replaceRes(const DeviceBoundaryMap& map, DeviceConstVectorView state,
+                       DeviceConstVectorView bc_vals, DeviceVectorView res, HipContext& ctx)
 {
   require(state.size() == map.rows && res.size() == map.rows && bc_vals.size() == map.num_bcs,
           "BoundaryMap residual vectors have incompatible sizes");
@@ -451,14 +543,14 @@ inline void replaceRes(const DeviceBoundaryMap& map, const DeviceVector& state,
                           ctx.raw()));
 }
 
-inline void zeroBoundary(const DeviceBoundaryMap& map, DeviceVector& vals, HipContext& ctx)
+inline void zeroBoundary(const DeviceBoundaryMap& map, DeviceVectorView vals, HipContext& ctx)
 {
   require(vals.size() == map.rows, "BoundaryMap vector has incompatible size");
   check(fx_bc_zero(map.num_bcs, map.bc_rows.data(), vals.data(), ctx.raw()));
 }
 
-inline void prepareForwardSolve(const DeviceBoundaryMap& map, DeviceCsrMatrix& mat, DeviceVector& rhs,
-                                const DeviceVector& bc_vals, HipContext& ctx)
+inline void prepareForwardSolve(const DeviceBoundaryMap& map, DeviceCsrMatrix& mat,
+                                DeviceVectorView rhs, DeviceConstVectorView bc_vals, HipContext& ctx)
 {
   checkLayout(map, mat);
   require(rhs.size() == map.rows && bc_vals.size() == map.num_bcs,
@@ -541,6 +633,29 @@ inline void assemble(const TaylorHoodNS& op, const DeviceTHGeometry& geom, const
                        map.blk_row.data(), map.blk_flags.data(), map.blk_lanes.data(),
                        map.pattern->rowPtrData(),
                        jac ? jac->valsData() : nullptr, res.data(), ctx.raw()));
+}
+
+/** Taylor-Hood adjoint VJP (fx_th_vjp): jt = J^T adj over the row-replaced
+ * Newton Jacobian (adj zeroed on the rows of bc, passed through there; bc
+ * may be null) and, when g is non-null, g[e] = adj~_e^T dR_e/dnu_e.  The
+ * Residual::applyParamJacT contract (femx/state/Residual.hpp:55-59) for a
+ * per-element viscosity. */
+inline void applyTaylorHoodJacT(const TaylorHoodNS& op, const DeviceTHGeometry& geom,
+                                const DeviceAssemblyMap& map, const DeviceVector& state,
+                                const DeviceVector& adj, const DeviceIndexVector* bc_mask,
+                                DeviceVector& jt, DeviceVector* g, HipContext& ctx)
+{
+  require(map.elem_inc.size() == 15 * geom.num_elems && geom.num_elems == map.info.num_elems,
+          "Taylor-Hood VJP needs the 15x15 map of the geometry");
+  require(state.size() == geom.ndof && adj.size() == geom.ndof,
+          "Taylor-Hood VJP vector sizes do not match");
+  if (jt.size() != geom.ndof) jt.resize(geom.ndof);
+  if (g && g->size() != geom.num_elems) g->resize(geom.num_elems);
+  if (map.vjp_work.size() != 15 * geom.num_elems) map.vjp_work.resize(15 * geom.num_elems);
+  check(fx_th_vjp(geom.ndof, geom.num_elems, geom.xv.data(), geom.yv.data(), geom.vconn3.data(),
+                  geom.dofs15.data(), state.data(), op.nu, op.nu_e, adj.data(),
+                  bc_mask ? bc_mask->data() : nullptr, map.inc_off.data(), map.elem_inc.data(),
+                  jt.data(), g ? g->data() : nullptr, map.vjp_work.data(), ctx.raw()));
 }
 
 namespace detail
@@ -663,8 +778,8 @@ struct NavierOperator
 /** applyNavierHistJacT (femx/model/ns/NavierStokesModel.cpp:321-363):
  * out = (dR/dhist_lag)^T adj, hand-derived where femx uses Enzyme. */
 inline void applyNavierHistJacT(const NavierOperator& op, Index step, Index num_hist, Index lag,
-                                const DeviceVector& hist, const DeviceVector& nxt,
-                                const DeviceVector& adj, DeviceVector& out, HipContext& ctx)
+                                DeviceConstVectorView hist, DeviceConstVectorView nxt,
+                                DeviceConstVectorView adj, DeviceVector& out, HipContext& ctx)
 {
   const auto& g = *op.geom;
   require(hist.size() == num_hist * g.num_states && nxt.size() == g.num_states
@@ -689,7 +804,7 @@ namespace assembly
  * it.  Outputs replaced. */
 inline void assemble(const model::ns::NavierOperator& op, Index step, Index num_hist,
                      state::VariableBlock wrt, const DeviceAssemblyMap& map,
-                     const DeviceVector& hist, const DeviceVector& nxt, DeviceVector& res,
+                     DeviceConstVectorView hist, DeviceConstVectorView nxt, DeviceVector& res,
                      DeviceCsrMatrix* jac, HipContext& ctx)
 {
   const auto& g = *op.geom;
@@ -711,9 +826,7 @@ inline void assemble(const model::ns::NavierOperator& op, Index step, Index num_
                         hist.data(), nxt.data(), g.plan.inc_off.data(), g.plan.inc_elem.data(),
                         g.plan.inc_meta.data(), g.plan.blk_runs.data(), g.plan.node_tile.data(),
                         jac && next ? jac->valsData() : nullptr, res.data(), ctx.raw()));
-  if (jac && !next && map.info.nnz > 0)
-    checkHip(hipMemsetAsync(jac->valsData(), 0, sizeof(Real) * map.info.nnz, ctx.stream()),
-             "hipMemsetAsync");
+  if (jac && !next) ::femx::detail::fill(jac->valsData(), map.info.nnz, 0.0, ctx.raw());
 }
 
 } // namespace assembly
@@ -721,51 +834,134 @@ inline void assemble(const model::ns::NavierOperator& op, Index step, Index num_
 namespace linalg
 {
 
-/** MatrixHandler<HipCsrBackend>: matvec / matvecT / zero / copy. */
+/** MatrixHandler<HipCsrBackend> (femx/linalg/handler/MatrixHandler.hpp:
+ * 193-378): CSR products on views, the explicit transpose with its pattern
+ * and permutation cached per source layout, copy, zero and finalize. */
 class HipMatrixHandler
 {
 public:
   explicit HipMatrixHandler(HipContext& ctx) : ctx_(ctx) {}
-  void matvec(const DeviceCsrMatrix& A, const Real* x, Index nx, Real* y, Index ny, Real alpha = 1.0,
-              Real beta = 0.0) const
+
+  /** y = alpha A x + beta y (MatrixHandler.hpp:270-275); x and y must not
+   * overlap each other or the values (MatrixHandler.cu:182-202). */
+  void matvec(const DeviceCsrMatrix& A, DeviceConstVectorView x, DeviceVectorView y,
+              Real alpha = 1.0, Real beta = 0.0) const
   {
-    require(nx == A.cols() && ny == A.rows(), "CSR matvec vector size mismatch");
+    checkApply(A, x, y, A.cols(), A.rows(), "CSR matvec");
     if (const auto* off = A.pattern().colOff16(ctx_)) // narrower index stream, same results
       check(fx_csr_matvec16(A.rows(), A.cols(), A.pattern().rowPtrData(), off->data(), A.valsData(),
-                            x, y, alpha, beta, ctx_.raw()));
+                            x.data(), y.data(), alpha, beta, ctx_.raw()));
     else
       check(fx_csr_matvec(A.rows(), A.cols(), A.pattern().rowPtrData(), A.pattern().colIndData(),
-                          A.valsData(), x, y, alpha, beta, ctx_.raw()));
+                          A.valsData(), x.data(), y.data(), alpha, beta, ctx_.raw()));
   }
-  void matvecT(const DeviceCsrMatrix& A, const Real* x, Index nx, Real* y, Index ny,
+  /** y = alpha A^T x + beta y (MatrixHandler.hpp:286-291), each column in
+   * ascending row order. */
+  void matvecT(const DeviceCsrMatrix& A, DeviceConstVectorView x, DeviceVectorView y,
                Real alpha = 1.0, Real beta = 0.0) const
   {
-    require(nx == A.rows() && ny == A.cols(), "CSR matvecT vector size mismatch");
+    checkApply(A, x, y, A.rows(), A.cols(), "CSR matvecT");
     const auto& t = A.pattern().transposeMap(ctx_);
     check(fx_csr_matvec_t(A.rows(), A.cols(), t.t_ptr.data(), t.t_row.data(), t.t_perm.data(),
-                          A.valsData(), x, y, alpha, beta, ctx_.raw()));
+                          A.valsData(), x.data(), y.data(), alpha, beta, ctx_.raw()));
   }
-  void zero(DeviceCsrMatrix& A) const
+  /** out = A x, out resized (MatrixHandler.hpp:303-306). */
+  void matvec(const DeviceCsrMatrix& A, DeviceConstVectorView x, DeviceVector& out) const
   {
-    if (A.nnz() > 0)
-      checkHip(hipMemsetAsync(A.valsData(), 0, sizeof(Real) * A.nnz(), ctx_.stream()), "hipMemsetAsync");
+    if (out.size() != A.rows()) out.resize(A.rows());
+    matvec(A, x, view(out));
   }
+  /** out = A^T x, out resized (MatrixHandler.hpp:318-321). */
+  void matvecT(const DeviceCsrMatrix& A, DeviceConstVectorView x, DeviceVector& out) const
+  {
+    if (out.size() != A.cols()) out.resize(A.cols());
+    matvecT(A, x, view(out));
+  }
+  /** dst = src^T (MatrixHandler.hpp:250-262): a changed source layout
+   * replaces dst's pattern (the transposed pattern is built once per source
+   * pattern and cached with it); values move through the cached
+   * permutation, a gather. */
+  void transpose(const DeviceCsrMatrix& src, DeviceCsrMatrix& dst) const
+  {
+    require(&src != &dst && (src.nnz() == 0 || src.valsData() != dst.valsData()),
+            "CSR transpose does not support in-place output");
+    const auto tp = src.pattern().transposed(ctx_);
+    if (!dst.hasPattern() || dst.pattern().layoutId() != tp->layoutId()) dst = DeviceCsrMatrix(tp);
+    const auto& t = src.pattern().transposeMap(ctx_);
+    check(fx_vec_gather(src.nnz(), t.t_perm.data(), src.valsData(), src.nnz(), dst.valsData(),
+                        ctx_.raw()));
+  }
+  /** Same-layout value copy (MatrixHandler.hpp:230-238). */
+  void copy(const DeviceCsrMatrix& src, DeviceCsrMatrix& dst) const
+  {
+    require(src.pattern().layoutId() == dst.pattern().layoutId(),
+            "CsrMatrix copy requires compatible source and destination graphs");
+    check(fx_vec_axpby(src.nnz(), 1.0, src.valsData(), 0.0, dst.valsData(), ctx_.raw()));
+  }
+  void zero(DeviceCsrMatrix& A) const { detail::fill(A.valsData(), A.nnz(), 0.0, ctx_.raw()); }
+  /** Nothing to do after device assembly (MatrixHandler.hpp:343-346). */
+  void finalize(DeviceCsrMatrix&) const noexcept {}
 
 private:
+  static void checkApply(const DeviceCsrMatrix& A, DeviceConstVectorView x, DeviceVectorView y,
+                         Index nx, Index ny, const char* what)
+  {
+    require(x.isValid() && y.isValid() && x.size() == nx && y.size() == ny,
+            std::string(what) + " vector size mismatch");
+    require(!detail::overlap(x.data(), sizeof(Real) * x.size(), y.data(), sizeof(Real) * y.size())
+                && !detail::overlap(y.data(), sizeof(Real) * y.size(), A.valsData(),
+                                    sizeof(Real) * A.nnz()),
+            "CSR matvec input and output must not alias");
+  }
   HipContext& ctx_;
 };
 
-/** VectorHandler<HipCsrBackend>: axpby and deterministic dot. */
+/** VectorHandler<HipCsrBackend> (femx/linalg/handler/VectorHandler.hpp:
+ * 190-477): axpby, copy, zero, gather/scatter and the deterministic dot. */
 class HipVectorHandler
 {
 public:
   explicit HipVectorHandler(HipContext& ctx) : ctx_(ctx), work_(fx_dot_work_len()) {}
-  void axpby(Real a, const DeviceVector& x, Real b, DeviceVector& y) const
+  /** y = a x + b y (VectorHandler.hpp:381-386). */
+  void axpby(Real a, DeviceConstVectorView x, Real b, DeviceVectorView y) const
   {
-    require(x.size() == y.size(), "axpby size mismatch");
+    require(x.isValid() && y.isValid() && x.size() == y.size(), "axpby size mismatch");
     check(fx_vec_axpby(y.size(), a, x.data(), b, y.data(), ctx_.raw()));
   }
-  void dot(const DeviceVector& x, const DeviceVector& y, DeviceVector& out)
+  /** dst = src (VectorHandler::copy). */
+  void copy(DeviceConstVectorView src, DeviceVectorView dst) const
+  {
+    require(src.size() == dst.size(), "Vector copy size mismatch");
+    check(fx_vec_axpby(dst.size(), 1.0, src.data(), 0.0, dst.data(), ctx_.raw()));
+  }
+  /** out = src, out resized. */
+  void copy(DeviceConstVectorView src, DeviceVector& out) const
+  {
+    if (out.size() != src.size()) out.resize(src.size());
+    copy(src, view(out));
+  }
+  void zero(DeviceVectorView v) const { detail::fill(v.data(), v.size(), 0.0, ctx_.raw()); }
+  /** out resized if needed, then zeroed (VectorHandler.hpp:282-300). */
+  void resizeOrZero(DeviceVector& out, Index n) const
+  {
+    if (out.size() != n) out.resize(n);
+    zero(view(out));
+  }
+  /** dst[i] = src[indices[i]] (VectorHandler.hpp:397-399). */
+  void gather(DeviceConstVectorView src, DeviceConstIndexView indices, DeviceVectorView dst) const
+  {
+    require(indices.size() == dst.size(), "Vector gather size mismatch");
+    check(fx_vec_gather(dst.size(), indices.data(), src.data(), src.size(), dst.data(), ctx_.raw()));
+  }
+  /** dst[indices[i]] = src[i] (VectorHandler.hpp:410-412); indices distinct. */
+  void scatter(DeviceConstVectorView src, DeviceConstIndexView indices, DeviceVectorView dst) const
+  {
+    require(indices.size() == src.size(), "Vector scatter size mismatch");
+    check(fx_vec_scatter(src.size(), indices.data(), src.data(), dst.data(), dst.size(), ctx_.raw()));
+  }
+  /** out[0] = x . y, a fixed-order two-stage reduction (VectorHandler.hpp:
+   * 423-425). */
+  void dot(DeviceConstVectorView x, DeviceConstVectorView y, DeviceVectorView out)
   {
     require(x.size() == y.size() && out.size() == 1, "dot size mismatch");
     check(fx_vec_dot(x.size(), x.data(), y.data(), work_.data(), out.data(), ctx_.raw()));
@@ -780,17 +976,40 @@ private:
 
 namespace state
 {
-/** Stationary residual contract over device storage (femx Residual<Backend>). */
+/** Sizes of a parameter-dependent stationary residual (femx/state/
+ * Residual.hpp:9-14). */
+struct Dimensions
+{
+  Index num_states{0};
+  Index num_param{0};
+  Index num_res{0};
+};
+
+/** Stationary residual contract over device storage (femx's
+ * Residual<Backend>, femx/state/Residual.hpp:16-60, for the HIP backend:
+ * Vec DeviceVector, Mat DeviceCsrMatrix, Pattern DeviceCsrPattern, Ctx
+ * HipContext). */
 class Residual
 {
 public:
-  virtual ~Residual()                                                                       = default;
-  virtual void res(const DeviceVector& state, const DeviceVector& prm, DeviceVector& out,
-                   HipContext& ctx) const                                                   = 0;
-  virtual void assembleStateJac(const DeviceVector& state, const DeviceVector& prm,
-                                DeviceCsrMatrix& out, HipContext& ctx) const                = 0;
-  virtual void applyParamJacT(const DeviceVector& state, const DeviceVector& prm,
-                              const DeviceVector& adj, DeviceVector& out, HipContext& ctx) const = 0;
+  using Vec     = DeviceVector;
+  using Mat     = DeviceCsrMatrix;
+  using Pattern = DeviceCsrPattern;
+  using Ctx     = HipContext;
+
+  virtual ~Residual() = default;
+  virtual Dimensions dims() const = 0;
+  /** The host pattern used for metadata (boundary maps, transposes). */
+  virtual const HostCsrPattern& hostPattern() const = 0;
+  /** The state-Jacobian pattern in device storage. */
+  virtual const Pattern& pattern() const = 0;
+  /** out = R(state, prm). */
+  virtual void res(const Vec& state, const Vec& prm, Vec& out, Ctx& ctx) const = 0;
+  /** out = dR/dstate at (state, prm). */
+  virtual void assembleStateJac(const Vec& state, const Vec& prm, Mat& out, Ctx& ctx) const = 0;
+  /** out = (dR/dprm)^T adj. */
+  virtual void applyParamJacT(const Vec& state, const Vec& prm, const Vec& adj, Vec& out,
+                              Ctx& ctx) const = 0;
 };
 
 /** P1 Poisson with a distributed control m_e = log kappa_e and Dirichlet
@@ -801,59 +1020,64 @@ public:
  * transposed products zero the adjoint on constrained rows first, as
  * ConstrainedTimeResidual::applyJacT does (femx/assembly/
  * ConstrainedTimeResidual.cpp:379-405).  The geometry, maps, load and
- * boundary values must outlive the residual. */
+ * boundary values must outlive the residual.  Calls on one residual share
+ * its scratch vectors: they must not overlap. */
 class PoissonLogKappaResidual : public Residual
 {
 public:
   PoissonLogKappaResidual(const assembly::DeviceGeometry& geom, const assembly::DeviceAssemblyMap& map,
                           const assembly::DeviceBoundaryMap* bc, const DeviceVector* bc_vals,
                           const Real* f)
-      : geom_(geom), map_(map), bc_(bc), bc_vals_(bc_vals), f_(f)
+      : geom_(geom), map_(map), bc_(bc), bc_vals_(bc_vals), f_(f), tmp_(map.info.num_res),
+        jt_(map.info.num_states), g_(geom.num_elems)
   {
     require(!bc || (bc_vals && bc_vals->size() == bc->num_bcs),
             "PoissonLogKappaResidual needs one boundary value per constrained row");
+    require(geom.num_elems == map.info.num_elems && map.info.max_res == 3 && map.info.max_state == 3,
+            "PoissonLogKappaResidual needs the P1 map of the geometry");
   }
 
-  void res(const DeviceVector& state, const DeviceVector& prm, DeviceVector& out,
-           HipContext& ctx) const override
+  Dimensions dims() const override
+  {
+    return {map_.info.num_states, geom_.num_elems, map_.info.num_res};
+  }
+  const HostCsrPattern& hostPattern() const override { return map_.pattern->host(); }
+  const Pattern&        pattern() const override { return *map_.pattern; }
+
+  void res(const Vec& state, const Vec& prm, Vec& out, Ctx& ctx) const override
   {
     checkPrm(prm);
+    if (out.size() != map_.info.num_res) out.resize(map_.info.num_res);
     assembly::assemble(assembly::PoissonP1{prm.data(), true, f_}, geom_, map_, &state, out, nullptr,
                        ctx);
     if (bc_) assembly::replaceRes(*bc_, state, *bc_vals_, out, ctx);
   }
 
-  void assembleStateJac(const DeviceVector& state, const DeviceVector& prm, DeviceCsrMatrix& out,
-                        HipContext& ctx) const override
+  void assembleStateJac(const Vec& state, const Vec& prm, Mat& out, Ctx& ctx) const override
   {
     checkPrm(prm);
-    DeviceVector tmp(map_.info.num_res);
-    assembly::assemble(assembly::PoissonP1{prm.data(), true, f_}, geom_, map_, &state, tmp, &out, ctx);
+    assembly::assemble(assembly::PoissonP1{prm.data(), true, f_}, geom_, map_, &state, tmp_, &out,
+                       ctx);
     if (bc_) assembly::replaceRows(*bc_, out, 1.0, ctx);
-    ctx.sync(); // tmp is released on return
   }
 
   /** out[e] = ((dR/dm)^T adj)_e (Residual::applyParamJacT, Residual.hpp:55-59). */
-  void applyParamJacT(const DeviceVector& state, const DeviceVector& prm, const DeviceVector& adj,
-                      DeviceVector& out, HipContext& ctx) const override
+  void applyParamJacT(const Vec& state, const Vec& prm, const Vec& adj, Vec& out,
+                      Ctx& ctx) const override
   {
-    DeviceVector jt(map_.info.num_states);
-    vjp(state, prm, adj, jt, out, ctx);
-    ctx.sync();
+    if (out.size() != geom_.num_elems) out.resize(geom_.num_elems);
+    vjp(state, prm, adj, jt_, out, ctx);
   }
 
   /** out = J^T adj over the row-replaced state Jacobian. */
-  void applyStateJacT(const DeviceVector& state, const DeviceVector& prm, const DeviceVector& adj,
-                      DeviceVector& out, HipContext& ctx) const
+  void applyStateJacT(const Vec& state, const Vec& prm, const Vec& adj, Vec& out, Ctx& ctx) const
   {
-    DeviceVector g(geom_.num_elems);
-    vjp(state, prm, adj, out, g, ctx);
-    ctx.sync();
+    if (out.size() != map_.info.num_states) out.resize(map_.info.num_states);
+    vjp(state, prm, adj, out, g_, ctx);
   }
 
   /** Both products in one launch (fx_p1_vjp). */
-  void vjp(const DeviceVector& state, const DeviceVector& prm, const DeviceVector& adj,
-           DeviceVector& jt, DeviceVector& g, HipContext& ctx) const
+  void vjp(const Vec& state, const Vec& prm, const Vec& adj, Vec& jt, Vec& g, Ctx& ctx) const
   {
     checkPrm(prm);
     require(state.size() == map_.info.num_states && adj.size() == map_.info.num_res
@@ -866,7 +1090,7 @@ public:
   }
 
 private:
-  void checkPrm(const DeviceVector& prm) const
+  void checkPrm(const Vec& prm) const
   {
     require(prm.size() == geom_.num_elems, "PoissonLogKappaResidual control size must be num_elems");
   }
@@ -876,6 +1100,79 @@ private:
   const assembly::DeviceBoundaryMap* bc_;
   const DeviceVector*                bc_vals_;
   const Real*                        f_;
+  mutable DeviceVector               tmp_, jt_, g_;
+};
+
+/** Steady Taylor-Hood Navier-Stokes with the per-element viscosity nu_e as
+ * parameter and Dirichlet rows replaced, the Residual contract of the
+ * north-star NS inverse problem:
+ *   R = R_NS(u; nu_e) on free rows,  R_r = u_r - g_r on constrained rows;
+ * the state Jacobian is the row-replaced Newton Jacobian, and applyParamJacT
+ * gives g_e = adj~_e^T dR_e/dnu_e with the adjoint zeroed on constrained rows
+ * (fx_th_vjp).  Inputs must outlive the residual; calls must not overlap. */
+class NavierStokesResidual : public Residual
+{
+public:
+  NavierStokesResidual(const assembly::DeviceTHGeometry& geom, const assembly::DeviceAssemblyMap& map,
+                       const assembly::DeviceBoundaryMap* bc, const DeviceVector* bc_vals)
+      : geom_(geom), map_(map), bc_(bc), bc_vals_(bc_vals), tmp_(map.info.num_res),
+        jt_(geom.ndof)
+  {
+    require(!bc || (bc_vals && bc_vals->size() == bc->num_bcs),
+            "NavierStokesResidual needs one boundary value per constrained row");
+    require(geom.num_elems == map.info.num_elems && map.info.num_res == geom.ndof
+                && map.elem_inc.size() == 15 * geom.num_elems,
+            "NavierStokesResidual needs the Taylor-Hood map of the geometry");
+  }
+
+  Dimensions dims() const override { return {geom_.ndof, geom_.num_elems, geom_.ndof}; }
+  const HostCsrPattern& hostPattern() const override { return map_.pattern->host(); }
+  const Pattern&        pattern() const override { return *map_.pattern; }
+
+  void res(const Vec& state, const Vec& nu_e, Vec& out, Ctx& ctx) const override
+  {
+    checkPrm(nu_e);
+    if (out.size() != geom_.ndof) out.resize(geom_.ndof);
+    assembly::assemble(assembly::TaylorHoodNS{0.0, nu_e.data()}, geom_, map_, state, out, nullptr,
+                       ctx);
+    if (bc_) assembly::replaceRes(*bc_, state, *bc_vals_, out, ctx);
+  }
+
+  void assembleStateJac(const Vec& state, const Vec& nu_e, Mat& out, Ctx& ctx) const override
+  {
+    checkPrm(nu_e);
+    assembly::assemble(assembly::TaylorHoodNS{0.0, nu_e.data()}, geom_, map_, state, tmp_, &out,
+                       ctx);
+    if (bc_) assembly::replaceRows(*bc_, out, 1.0, ctx);
+  }
+
+  void applyParamJacT(const Vec& state, const Vec& nu_e, const Vec& adj, Vec& out,
+                      Ctx& ctx) const override
+  {
+    checkPrm(nu_e);
+    assembly::applyTaylorHoodJacT(assembly::TaylorHoodNS{0.0, nu_e.data()}, geom_, map_, state, adj,
+                                  bc_ ? &bc_->bc_mask : nullptr, jt_, &out, ctx);
+  }
+
+  /** out = J^T adj over the row-replaced Newton Jacobian. */
+  void applyStateJacT(const Vec& state, const Vec& nu_e, const Vec& adj, Vec& out, Ctx& ctx) const
+  {
+    checkPrm(nu_e);
+    assembly::applyTaylorHoodJacT(assembly::TaylorHoodNS{0.0, nu_e.data()}, geom_, map_, state, adj,
+                                  bc_ ? &bc_->bc_mask : nullptr, out, nullptr, ctx);
+  }
+
+private:
+  void checkPrm(const Vec& nu_e) const
+  {
+    require(nu_e.size() == geom_.num_elems, "NavierStokesResidual viscosity size must be num_elems");
+  }
+
+  const assembly::DeviceTHGeometry&  geom_;
+  const assembly::DeviceAssemblyMap& map_;
+  const assembly::DeviceBoundaryMap* bc_;
+  const DeviceVector*                bc_vals_;
+  mutable DeviceVector               tmp_, jt_;
 };
 } // namespace state
 

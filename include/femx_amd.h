@@ -282,6 +282,10 @@ int fx_vec_gather(fx_index n, const fx_index* d_idx, const double* d_src, fx_ind
                   double* d_dst, void* stream);
 int fx_vec_scatter(fx_index n, const fx_index* d_idx, const double* d_src, double* d_dst,
                    fx_index dst_len, void* stream);
+/* y[i] = v, i < n (with v = 0, VectorHandler::zero and resizeOrZero,
+ * femx/linalg/handler/VectorHandler.hpp:282-300,365-371): a kernel, not a
+ * memset, so that it may be captured into a hipGraph. */
+int fx_vec_fill(fx_index n, double v, double* d_y, void* stream);
 /* z = x / d elementwise (the Jacobi step of the device iterative solver;
  * no counterpart in femx's VectorHandler).  z may alias x. */
 int fx_vec_div(fx_index n, const double* d_x, const double* d_d, double* d_z, void* stream);
