@@ -74,6 +74,9 @@ _SIGS = {
     "fx_batch_capture": ([_vp, C.c_int], C.c_int),
     "fx_batch_launch": ([_vp, _vp], C.c_int),
     "fx_batch_destroy": ([_vp], None),
+    "fx_batch_graph_create": ([_i32, _vp, C.c_int, C.POINTER(_vp)], C.c_int),
+    "fx_batch_graph_launch": ([_vp, _vp], C.c_int),
+    "fx_batch_graph_destroy": ([_vp], None),
 }
 
 

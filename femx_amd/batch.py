@@ -83,11 +83,18 @@ class P1Case:
         return [self.jac.vals, self.res, self.jt, self.g]
 
 
+# Taylor-Hood block layouts the autotuner chooses from: (work items per
+# block, row-tile bound in CSR entries; 0 = the LDS capacity)
+TH_LAYOUTS = ((32, 0), (24, 0), (16, 0), (32, 1500))
+# persistent workgroups per compute unit (0: one workgroup per work item)
+GRID_CANDIDATES = (0, 2, 8)
+
+
 class THCase:
     """One Taylor-Hood instance: lid-driven cavity, Re, Fourier state,
     adjoint, per-element viscosity sensitivities."""
 
-    def __init__(self, n, seed, Re, device, items=ns.TH_ITEMS):
+    def __init__(self, n, seed, Re, device, items=ns.TH_ITEMS, layouts=TH_LAYOUTS):
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
         self.geom = ns.structured_th(n, device)
         g = self.geom
@@ -97,7 +104,7 @@ class THCase:
         self.map = make_assembly_map(g.ndof, g.ndof, g.h_dofs15)
         dofs, _ = ns.cavity_bc(g)
         self.bmap = make_boundary_map(dofs, self.map.pattern, device)
-        self.plan = ns.THPlan(self.map, device, items)
+        self.plan = ns.THPlan(self.map, device, items, variants=layouts)
         setattr(self.map, "_th_plan_" + str(torch.device(device)), self.plan)
         self.map.release_host()
         self.jac = CsrMatrix(self.map.pattern, device)
@@ -129,6 +136,12 @@ class THCase:
         return [self.jac.vals, self.res, self.jt, self.g]
 
 
+def size_class(case, bounds=(8192, 131072)):
+    """0, 1 or 2 by element count (n < 64, n < 256, larger on the unit
+    square)."""
+    return sum(case.elements >= b for b in bounds)
+
+
 def build_cases(specs, device="cuda", items=ns.TH_ITEMS, log=None):
     cases = []
     for k, s in enumerate(specs):
@@ -144,7 +157,7 @@ def build_cases(specs, device="cuda", items=ns.TH_ITEMS, log=None):
 class Batch:
     """fx_batch over a list of cases."""
 
-    ASSEMBLE, VJP = 1, 2
+    ASSEMBLE, VJP, SERIAL = 1, 2, 4
 
     def __init__(self, cases, grid_per_cu=4):
         self.cases = cases
@@ -180,6 +193,29 @@ class Batch:
         _lib.call("fx_batch_launch", self._h, stream_handle(stream))
 
 
+class BatchGraph:
+    """One hipGraph over several batches (fx_batch_graph_create): every
+    chain of every batch is a branch, or (what & SERIAL) every phase runs in
+    turn over the batches."""
+
+    def __init__(self, batches, what=3):
+        self.batches = list(batches)
+        arr = (C.c_void_p * len(self.batches))(*[b._h.value for b in self.batches])
+        h = C.c_void_p()
+        _lib.call("fx_batch_graph_create", len(self.batches), C.cast(arr, C.c_void_p), int(what),
+                  C.byref(h))
+        self._h = h
+        self._destroy = _lib.lib().fx_batch_graph_destroy
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._destroy(self._h)
+            self._h = None
+
+    def launch(self, stream=None):
+        _lib.call("fx_batch_graph_launch", self._h, stream_handle(stream))
+
+
 def plain(cases, what=3):
     for c in cases:
         c.plain(what)
@@ -199,15 +235,66 @@ def time_fn(fn, reps, warmup=1):
     return a.elapsed_time(b) / reps * 1e-3, (time.perf_counter() - t0) / reps
 
 
-def autotune(cases, grid_candidates=(1, 2, 4, 8), reps=3):
-    """Pick the persistent grid multiplier (workgroups per compute unit) by
-    timing the captured graph for each candidate; returns (best, timings)."""
-    timings = {}
-    for gpc in grid_candidates:
-        b = Batch(cases, gpc)
-        b.capture(3)
-        t, _ = time_fn(b.launch, reps)
-        timings[gpc] = t
-        del b
-    best = min(timings, key=timings.get)
-    return best, timings
+def _graph_time(cases, grid_per_cu, reps):
+    """A group's phases in series, so that its shape is timed alone."""
+    b = Batch(cases, grid_per_cu)
+    b.capture(3 | Batch.SERIAL)
+    t, _ = time_fn(b.launch, reps)
+    return t
+
+
+def autotune(cases, layouts=TH_LAYOUTS, grids=GRID_CANDIDATES, reps=3, log=None):
+    """The launch shape of the batch, chosen by timing captured graphs.
+
+    Instances are grouped by kind and size class; each group becomes a batch
+    of its own (its own grid and Taylor-Hood row tile, so a small-instance
+    group does not inherit a large group's LDS tile), and all groups' chains
+    are branches of one graph.  Per group: for Taylor-Hood, the block layout
+    (work items per block -- the elements a workgroup's rows gather -- and
+    the row-tile bound) from `layouts`; then the persistent-grid multiplier
+    (or one workgroup per work item, 0) from `grids`.  Last, the per-group
+    batches and one batch over all instances (grid 8) are each captured with
+    the chains in series and as concurrent branches, and the fastest of the
+    four graphs is kept.  Returns (runner, report): runner.launch() replays
+    the tuned graph; the report holds every timing and choice."""
+    groups = {}
+    for c in cases:
+        groups.setdefault((isinstance(c, THCase), size_class(c)), []).append(c)
+    report = {"groups": {}}
+    batches = []
+    for (is_th, cls), gc in sorted(groups.items()):
+        key = f"{'th' if is_th else 'p1'}_class{cls}"
+        rec = {"instances": len(gc), "elements": int(sum(c.elements for c in gc))}
+        if is_th:
+            tl = {}
+            for lay in layouts:
+                for c in gc:
+                    c.plan.select(*lay)
+                tl[lay] = _graph_time(gc, 8, reps)
+            best_lay = min(tl, key=tl.get)
+            for c in gc:
+                c.plan.select(*best_lay)
+            rec["layout_ms"] = {f"{a}x{b}": v * 1e3 for (a, b), v in tl.items()}
+            rec["layout"] = {"items": best_lay[0], "tile_nnz": best_lay[1]}
+        tg = {g: _graph_time(gc, g, reps) for g in grids}
+        best_g = min(tg, key=tg.get)
+        rec["grid_ms"] = {str(g): v * 1e3 for g, v in tg.items()}
+        rec["grid_per_cu"] = best_g
+        report["groups"][key] = rec
+        if log:
+            log(f"[autotune] {key}: {rec}")
+        batches.append(Batch(gc, best_g))
+    single = Batch(cases, 8)
+    cands = {}
+    for mode, what in (("serial", 3 | Batch.SERIAL), ("concurrent", 3)):
+        cands[f"grouped_{mode}"] = BatchGraph(batches, what)
+        cands[f"single_{mode}"] = BatchGraph([single], what)
+    tf = {k: time_fn(g.launch, reps)[0] for k, g in cands.items()}
+    best = min(tf, key=tf.get)
+    report["graph_ms"] = {k: v * 1e3 for k, v in tf.items()}
+    report["graph"] = best
+    if log:
+        log(f"[autotune] graphs {report['graph_ms']} -> {best}")
+    runner = cands[best]
+    runner.keep = batches + [single]
+    return runner, report

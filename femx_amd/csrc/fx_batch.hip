@@ -11,9 +11,16 @@
 // -- each XCD's workgroups one contiguous eighth of it, so neighbouring
 // blocks share an L2 -- running the same per-block device code as the
 // single-instance kernels, so each instance's results are bitwise equal to
-// its plain launch.  The five phases (P1 assembly, Taylor-Hood assembly, P1
-// VJP, Taylor-Hood VJP element and gather phases) are captured once into a
-// hipGraph and replayed with a single hipGraphLaunch.
+// its plain launch.  The phases form four independent chains -- P1 assembly,
+// Taylor-Hood assembly, P1 VJP, and the Taylor-Hood VJP's element then gather
+// phase -- launched on private branch streams forked from the caller's
+// stream, so they run concurrently; captured, they become the branches of
+// one hipGraph, replayed with a single hipGraphLaunch.  Several batches (for
+// instance one per size class, each with its own grid and row tile) can be
+// captured into one graph together (fx_batch_graph_create).  With
+// grid_per_cu = 0 a phase launches one workgroup per work item instead of a
+// persistent grid, so that concurrent phases share the compute units as
+// their workgroups retire.
 #include <algorithm>
 #include <memory>
 #include <vector>
@@ -29,6 +36,7 @@ struct fx_batch
   int         cus    = 0;
   std::size_t p1_lds = 0, th_lds = 0;
   int         th_tile = 0; // row tile per wave of the Taylor-Hood assembly (doubles)
+  bool        p1_long = false; // some P1 row is longer than the LDS tile width
   // device memory
   fxk::P1AsmArgs*  d_p1_asm  = nullptr;
   fxk::P1VjpArgs*  d_p1_vjp  = nullptr;
@@ -42,18 +50,41 @@ struct fx_batch
   hipGraphExec_t   exec       = nullptr;
   int              graph_what = 0;
   int              device     = 0;
+  // independent chains (P1 assembly, TH assembly, P1 VJP, TH VJP) run on
+  // private branch streams forked from the caller's stream and joined back
+  hipStream_t branch[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t  fork      = nullptr;
+  hipEvent_t  done[4]   = {nullptr, nullptr, nullptr, nullptr};
 
   ~fx_batch()
   {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    for (auto& st : branch)
+      if (st) (void)hipStreamDestroy(st);
+    if (fork) (void)hipEventDestroy(fork);
+    for (auto& e : done)
+      if (e) (void)hipEventDestroy(e);
     for (void* p : {static_cast<void*>(d_p1_asm), static_cast<void*>(d_p1_vjp),
                     static_cast<void*>(d_th_asm), static_cast<void*>(d_th_vjp),
                     static_cast<void*>(d_th_gat)})
       if (p) (void)hipFree(p);
     for (auto* w : d_work)
       if (w) (void)hipFree(w);
+  }
+};
+
+struct fx_batch_graph
+{
+  hipStream_t    origin = nullptr;
+  hipGraph_t     graph  = nullptr;
+  hipGraphExec_t exec   = nullptr;
+  ~fx_batch_graph()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (origin) (void)hipStreamDestroy(origin);
   }
 };
 
@@ -71,6 +102,7 @@ enum Phase
 };
 
 // capped at 96 VGPRs (5 waves/SIMD, as k_p1_assemble reaches uncapped)
+template <bool kLong>
 __global__ __launch_bounds__(kP1Block) __attribute__((amdgpu_waves_per_eu(5))) void
 k_batch_p1_assemble(const P1AsmArgs* __restrict__ inst, const int2* __restrict__ work, int nwork)
 {
@@ -79,7 +111,7 @@ k_batch_p1_assemble(const P1AsmArgs* __restrict__ inst, const int2* __restrict__
   for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
-    p1_assemble_block(it.y, inst[it.x], smem);
+    p1_assemble_block<kLong>(it.y, inst[it.x], smem);
   }
 }
 
@@ -154,32 +186,123 @@ T* upload(const std::vector<T>& h)
   return d;
 }
 
-void launch_phases(fx_batch* b, int what, hipStream_t s)
+// The four chains of a batch, each on its own stream: P1 assembly, TH
+// assembly, P1 VJP, TH VJP (element then gather phase).  None reads another's
+// output, so they may run concurrently.
+void launch_chain(fx_batch* b, int chain, hipStream_t s)
 {
   auto g = [&](int p) { return dim3(b->pgrid[p]); };
-  if (what & FX_BATCH_ASSEMBLE)
+  switch (chain)
   {
-    if (b->nwork[kP1Asm])
-      hipLaunchKernelGGL(k_batch_p1_assemble, g(kP1Asm), dim3(kP1Block), b->p1_lds, s, b->d_p1_asm,
-                         b->d_work[kP1Asm], b->nwork[kP1Asm]);
-    if (b->nwork[kThAsm])
-      hipLaunchKernelGGL(k_batch_th_assemble, g(kThAsm), dim3(kThLanes * kThWaves), b->th_lds, s,
-                         b->d_th_asm, b->d_work[kThAsm], b->nwork[kThAsm], b->th_tile);
-    fx::check_launch("fx_batch: assembly");
-  }
-  if (what & FX_BATCH_VJP)
-  {
-    if (b->nwork[kP1Vjp])
-      hipLaunchKernelGGL(k_batch_p1_vjp, g(kP1Vjp), dim3(kP1Block), 0, s, b->d_p1_vjp,
-                         b->d_work[kP1Vjp], b->nwork[kP1Vjp]);
-    if (b->nwork[kThVjpElem])
-      hipLaunchKernelGGL(k_batch_th_vjp_elem, g(kThVjpElem), dim3(kThLanes), 0, s, b->d_th_vjp,
-                         b->d_work[kThVjpElem], b->nwork[kThVjpElem]);
-    if (b->nwork[kThVjpGather])
-      hipLaunchKernelGGL(k_batch_vjp_gather, g(kThVjpGather), dim3(256), 0, s, b->d_th_gat,
-                         b->d_work[kThVjpGather], b->nwork[kThVjpGather]);
+  case 0:
+    if (b->p1_long)
+      hipLaunchKernelGGL(k_batch_p1_assemble<true>, g(kP1Asm), dim3(kP1Block), b->p1_lds, s,
+                         b->d_p1_asm, b->d_work[kP1Asm], b->nwork[kP1Asm]);
+    else
+      hipLaunchKernelGGL(k_batch_p1_assemble<false>, g(kP1Asm), dim3(kP1Block), b->p1_lds, s,
+                         b->d_p1_asm, b->d_work[kP1Asm], b->nwork[kP1Asm]);
+    break;
+  case 1:
+    hipLaunchKernelGGL(k_batch_th_assemble, g(kThAsm), dim3(kThLanes * kThWaves), b->th_lds, s,
+                       b->d_th_asm, b->d_work[kThAsm], b->nwork[kThAsm], b->th_tile);
+    break;
+  case 2:
+    hipLaunchKernelGGL(k_batch_p1_vjp, g(kP1Vjp), dim3(kP1Block), 0, s, b->d_p1_vjp,
+                       b->d_work[kP1Vjp], b->nwork[kP1Vjp]);
+    break;
+  default:
+    hipLaunchKernelGGL(k_batch_th_vjp_elem, g(kThVjpElem), dim3(kThLanes), 0, s, b->d_th_vjp,
+                       b->d_work[kThVjpElem], b->nwork[kThVjpElem]);
     fx::check_launch("fx_batch: VJP");
+    hipLaunchKernelGGL(k_batch_vjp_gather, g(kThVjpGather), dim3(256), 0, s, b->d_th_gat,
+                       b->d_work[kThVjpGather], b->nwork[kThVjpGather]);
   }
+  fx::check_launch("fx_batch");
+}
+
+bool chain_active(const fx_batch* b, int chain, int what)
+{
+  switch (chain)
+  {
+  case 0: return (what & FX_BATCH_ASSEMBLE) && b->nwork[kP1Asm] > 0;
+  case 1: return (what & FX_BATCH_ASSEMBLE) && b->nwork[kThAsm] > 0;
+  case 2: return (what & FX_BATCH_VJP) && b->nwork[kP1Vjp] > 0;
+  default: return (what & FX_BATCH_VJP) && b->nwork[kThVjpElem] > 0;
+  }
+}
+
+void ensure_branches(fx_batch* b)
+{
+  if (b->fork) return;
+  for (auto& st : b->branch)
+    fx::check_hip(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+  fx::check_hip(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+  for (auto& e : b->done)
+    fx::check_hip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+}
+
+// Fork the active chains of the batches from stream s onto their branch
+// streams and join them back into s (also the shape a capture of s records:
+// one graph branch per chain); with FX_BATCH_SERIAL, launch them in order on s.
+void launch_batches(fx_batch* const* bs, int count, int what, hipStream_t s)
+{
+  if (what & FX_BATCH_SERIAL)
+  {
+    // every chain on s, one after another: no phase shares the chip
+    for (int c = 0; c < 4; ++c)
+      for (int i = 0; i < count; ++i)
+        if (chain_active(bs[i], c, what)) launch_chain(bs[i], c, s);
+    return;
+  }
+  for (int i = 0; i < count; ++i)
+  {
+    fx_batch* b = bs[i];
+    ensure_branches(b);
+    fx::check_hip(hipEventRecord(b->fork, s), "hipEventRecord");
+    for (int c = 0; c < 4; ++c)
+    {
+      if (!chain_active(b, c, what)) continue;
+      fx::check_hip(hipStreamWaitEvent(b->branch[c], b->fork, 0), "hipStreamWaitEvent");
+      launch_chain(b, c, b->branch[c]);
+      fx::check_hip(hipEventRecord(b->done[c], b->branch[c]), "hipEventRecord");
+    }
+  }
+  for (int i = 0; i < count; ++i)
+    for (int c = 0; c < 4; ++c)
+      if (chain_active(bs[i], c, what))
+        fx::check_hip(hipStreamWaitEvent(s, bs[i]->done[c], 0), "hipStreamWaitEvent");
+}
+
+// Capture launch_batches on `origin` into an instantiated graph; on failure
+// the stream leaves capture mode before the error propagates.
+hipGraphExec_t capture_batches(fx_batch* const* bs, int count, int what, hipStream_t origin,
+                               hipGraph_t* graph_out)
+{
+  for (int i = 0; i < count; ++i) ensure_branches(bs[i]); // no stream or event creation in capture
+  fx::check_hip(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal),
+                "hipStreamBeginCapture");
+  try
+  {
+    launch_batches(bs, count, what, origin);
+  }
+  catch (...)
+  {
+    hipGraph_t partial = nullptr;
+    if (hipStreamEndCapture(origin, &partial) == hipSuccess && partial)
+      (void)hipGraphDestroy(partial);
+    throw;
+  }
+  hipGraph_t graph = nullptr;
+  fx::check_hip(hipStreamEndCapture(origin, &graph), "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t st = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (st != hipSuccess)
+  {
+    (void)hipGraphDestroy(graph);
+    fx::check_hip(st, "hipGraphInstantiate");
+  }
+  *graph_out = graph;
+  return exec;
 }
 
 } // namespace
@@ -190,7 +313,7 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
                     const fx_th_instance* th, fx_index grid_per_cu, fx_batch** out)
 {
   return fx::guard([&] {
-    fx::require(out != nullptr && num_p1 >= 0 && num_th >= 0 && grid_per_cu > 0
+    fx::require(out != nullptr && num_p1 >= 0 && num_th >= 0 && grid_per_cu >= 0
                     && (num_p1 == 0 || p1) && (num_th == 0 || th),
                 FX_ERR_ARG, "fx_batch_create: invalid arguments");
     *out   = nullptr;
@@ -223,6 +346,7 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
                   FX_ERR_ARG, "fx_batch_create: P1 instance lacks an array");
       // rows past the tile width add into vals in global memory (fx_p1_dev.hpp)
       max_row_len = std::max(max_row_len, std::min(static_cast<int>(q.max_row_len), kP1MaxRowLen));
+      if (q.max_row_len > kP1MaxRowLen) b->p1_long = true;
       pa.push_back({q.num_nodes, q.x, q.y, q.conn, q.m, 1, q.f, q.u, q.inc_off,
                     reinterpret_cast<const int2*>(q.inc), q.row_ptr, q.vals, q.res, 0});
       pv.push_back({q.num_nodes, q.x, q.y, q.conn, q.m, q.u, q.lam, q.bc_mask, q.inc_off,
@@ -269,20 +393,34 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     b->th_lds  = sizeof(ThTensor) + static_cast<std::size_t>(max_blk_nnz) * 8 * kThWaves;
     fx::require(b->p1_lds <= 160 * 1024 && b->th_lds <= 160 * 1024, FX_ERR_UNSUPPORTED,
                 "fx_batch_create: LDS tile exceeds the workgroup limit");
-    // each phase's grid covers at least every workgroup slot its kernel can
-    // hold resident, so no phase runs below its occupancy
-    auto occ = [&](const void* k, int threads, std::size_t lds) {
+    // Persistent grids (grid_per_cu > 0) cover at least every workgroup slot
+    // the phase kernel can hold resident, so no phase runs below its
+    // occupancy; grid_per_cu = 0 gives one workgroup per work item (per four
+    // for the Taylor-Hood assembly, whose waves walk the list on their own),
+    // so that concurrent phases share the chip as their workgroups retire.
+    auto occ = [&](int p, const void* k, int threads, std::size_t lds) {
+      if (grid_per_cu == 0)
+      {
+        const std::int64_t items = w[p].size(), per = p == kThAsm ? kThWaves : 1;
+        return static_cast<int>(std::max<std::int64_t>(8, (items + per - 1) / per + 7) / 8 * 8);
+      }
       int n = 0;
       fx::check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, threads, lds),
                     "hipOccupancyMaxActiveBlocksPerMultiprocessor");
       return (std::max(b->grid, cus * n) + 7) / 8 * 8;
     };
-    b->pgrid[kP1Asm]      = occ(reinterpret_cast<const void*>(k_batch_p1_assemble), kP1Block, b->p1_lds);
-    b->pgrid[kThAsm]      = occ(reinterpret_cast<const void*>(k_batch_th_assemble), kThLanes * kThWaves,
-                                b->th_lds);
-    b->pgrid[kP1Vjp]      = occ(reinterpret_cast<const void*>(k_batch_p1_vjp), kP1Block, 0);
-    b->pgrid[kThVjpElem]  = occ(reinterpret_cast<const void*>(k_batch_th_vjp_elem), kThLanes, 0);
-    b->pgrid[kThVjpGather] = occ(reinterpret_cast<const void*>(k_batch_vjp_gather), 256, 0);
+    b->pgrid[kP1Asm] =
+        occ(kP1Asm,
+            b->p1_long ? reinterpret_cast<const void*>(k_batch_p1_assemble<true>)
+                       : reinterpret_cast<const void*>(k_batch_p1_assemble<false>),
+            kP1Block, b->p1_lds);
+    b->pgrid[kThAsm] = occ(kThAsm, reinterpret_cast<const void*>(k_batch_th_assemble),
+                           kThLanes * kThWaves, b->th_lds);
+    b->pgrid[kP1Vjp] = occ(kP1Vjp, reinterpret_cast<const void*>(k_batch_p1_vjp), kP1Block, 0);
+    b->pgrid[kThVjpElem] =
+        occ(kThVjpElem, reinterpret_cast<const void*>(k_batch_th_vjp_elem), kThLanes, 0);
+    b->pgrid[kThVjpGather] =
+        occ(kThVjpGather, reinterpret_cast<const void*>(k_batch_vjp_gather), 256, 0);
     b->d_p1_asm = upload(pa);
     b->d_p1_vjp = upload(pv);
     b->d_th_asm = upload(ta);
@@ -310,7 +448,7 @@ int fx_batch_run(fx_batch* b, int what, void* stream)
 {
   return fx::guard([&] {
     fx::require(b != nullptr, FX_ERR_ARG, "fx_batch_run: null batch");
-    launch_phases(b, what, fx::as_stream(stream));
+    launch_batches(&b, 1, what, fx::as_stream(stream));
   });
 }
 
@@ -330,22 +468,7 @@ int fx_batch_capture(fx_batch* b, int what)
     }
     if (!b->cap_stream)
       fx::check_hip(hipStreamCreateWithFlags(&b->cap_stream, hipStreamNonBlocking), "hipStreamCreate");
-    fx::check_hip(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal),
-                  "hipStreamBeginCapture");
-    try
-    {
-      launch_phases(b, what, b->cap_stream);
-    }
-    catch (...)
-    {
-      // leave the private stream out of capture mode so a later capture works
-      hipGraph_t partial = nullptr;
-      if (hipStreamEndCapture(b->cap_stream, &partial) == hipSuccess && partial)
-        (void)hipGraphDestroy(partial);
-      throw;
-    }
-    fx::check_hip(hipStreamEndCapture(b->cap_stream, &b->graph), "hipStreamEndCapture");
-    fx::check_hip(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    b->exec       = capture_batches(&b, 1, what, b->cap_stream, &b->graph);
     b->graph_what = what;
   });
 }
@@ -357,6 +480,36 @@ int fx_batch_launch(fx_batch* b, void* stream)
     fx::check_hip(hipGraphLaunch(b->exec, fx::as_stream(stream)), "hipGraphLaunch");
   });
 }
+
+int fx_batch_graph_create(fx_index count, fx_batch* const* batches, int what,
+                          fx_batch_graph** out)
+{
+  return fx::guard([&] {
+    fx::require(out && count > 0 && batches, FX_ERR_ARG, "fx_batch_graph_create: invalid arguments");
+    *out = nullptr;
+    for (fx_index i = 0; i < count; ++i)
+    {
+      fx::require(batches[i] != nullptr, FX_ERR_ARG, "fx_batch_graph_create: null batch");
+      for (fx_index j = 0; j < i; ++j)
+        fx::require(batches[j] != batches[i], FX_ERR_ARG,
+                    "fx_batch_graph_create: a batch is listed twice");
+    }
+    auto g = std::make_unique<fx_batch_graph>();
+    fx::check_hip(hipStreamCreateWithFlags(&g->origin, hipStreamNonBlocking), "hipStreamCreate");
+    g->exec = capture_batches(batches, static_cast<int>(count), what, g->origin, &g->graph);
+    *out    = g.release();
+  });
+}
+
+int fx_batch_graph_launch(fx_batch_graph* g, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(g && g->exec, FX_ERR_ARG, "fx_batch_graph_launch: null graph");
+    fx::check_hip(hipGraphLaunch(g->exec, fx::as_stream(stream)), "hipGraphLaunch");
+  });
+}
+
+void fx_batch_graph_destroy(fx_batch_graph* g) { delete g; }
 
 void fx_batch_destroy(fx_batch* b) { delete b; }
 

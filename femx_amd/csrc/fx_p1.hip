@@ -9,10 +9,11 @@ namespace
 {
 using namespace fxk;
 
+template <bool kLong>
 __global__ __launch_bounds__(kP1Block) void k_p1_assemble(P1AsmArgs a)
 {
   extern __shared__ double smem[];
-  p1_assemble_block(fx::xcd_block(gridDim.x), a, smem);
+  p1_assemble_block<kLong>(fx::xcd_block(gridDim.x), a, smem);
 }
 
 // capped at 96 VGPRs: 5 waves/SIMD instead of 4, no spills (74.6 -> 69.1 us at 1024^2)
@@ -52,7 +53,10 @@ int fx_p1_assemble(fx_index num_nodes, fx_index num_elems, fx_index nnz, fx_inde
     const size_t   lds     = static_cast<size_t>(acc_cap) * 8 + kIncStage * 8;
     const P1AsmArgs a{num_nodes, x,  y,       conn, kappa, kappa_is_log, f, u, inc_off,
                       reinterpret_cast<const int2*>(inc), row_ptr, vals, res, acc_cap};
-    hipLaunchKernelGGL(k_p1_assemble, dim3(nb), dim3(kP1Block), lds, fx::as_stream(stream), a);
+    if (max_row_len > kP1MaxRowLen)
+      hipLaunchKernelGGL(k_p1_assemble<true>, dim3(nb), dim3(kP1Block), lds, fx::as_stream(stream), a);
+    else
+      hipLaunchKernelGGL(k_p1_assemble<false>, dim3(nb), dim3(kP1Block), lds, fx::as_stream(stream), a);
     fx::check_launch("fx_p1_assemble");
   });
 }

@@ -197,6 +197,10 @@ __device__ __forceinline__ P1Raw p1_load_raw(const P1AsmArgs& a, int e, int2 n, 
 // updates are LDS atomic adds: the three positions of an incidence are
 // distinct entries of the lane's own row, so each becomes old + K in program
 // order, as the host loop adds them.
+// kLong: rows may be longer than the tile width (max_row_len > kP1MaxRowLen),
+// so a block whose CSR range does not fit the tile adds in global memory;
+// false compiles the tile path alone.
+template <bool kLong>
 __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, double* smem)
 {
   double*     acc  = smem;
@@ -207,7 +211,7 @@ __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, d
   const int   bn   = a.vals ? a.row_ptr[r1] - base : 0;
   const int   ib0 = a.inc_off[r0], ib1 = a.inc_off[r1];
   const bool  staged = ib1 - ib0 <= kIncStage;
-  const bool  tiled  = bn <= a.acc_cap; // else: rows add straight into a.vals
+  const bool  tiled  = !kLong || bn <= a.acc_cap; // else: rows add straight into a.vals
   if (tiled)
     for (int i = threadIdx.x; i < bn; i += kP1Block) acc[i] = 0.0;
   if (staged)

@@ -90,20 +90,13 @@ def th_from_p1(x, y, conn3, device="cuda"):
     return geom
 
 
-class THPlan:
-    """Row-incidence tables, work blocks and the VJP workspace of the
-    Taylor-Hood kernels for one assembly map, on one device.  Calls that share
-    a plan share its workspace: VJPs of one map on different streams must not
-    overlap."""
+class THBlocks:
+    """One block layout of the Taylor-Hood assembly (fx_map_th_blocks with
+    `items` work items per block and row tile bound `tile_nnz`, and its lane
+    map), on one device."""
 
-    def __init__(self, amap: AssemblyMap, device="cuda", items=TH_ITEMS, tile_nnz=TH_TILE_NNZ):
-        dev = torch.device(device)
-        info = amap.info
+    def __init__(self, amap: AssemblyMap, device, items, tile_nnz):
         self.items, self.tile_nnz = int(items), int(tile_nnz)
-        off = np.empty(amap.num_res + 1, np.int32)
-        elem = np.empty(max(info.num_inc, 1), np.int32)
-        meta = np.empty(max(4 * info.num_inc, 4), np.int32)
-        _lib.call("fx_map_export_rowinc15", amap._handle(), ptr(off), ptr(elem), ptr(meta))
         nb, mx = C.c_int32(), C.c_int32()
         args = (amap._handle(), self.items, self.tile_nnz)
         _lib.call("fx_map_th_blocks", *args, C.byref(nb), C.byref(mx), None, None)
@@ -117,6 +110,28 @@ class THPlan:
         _lib.call("fx_map_th_lanes", amap._handle(), nb.value, ptr(blk_row), ptr(blk_flags),
                   ptr(lanes))
         self.h_blk_lanes = lanes[: 64 * nb.value]
+        t = lambda a: torch.from_numpy(a).to(device)  # noqa: E731
+        self.blk_row, self.blk_flags = t(blk_row), t(self.h_blk_flags)
+        self.blk_lanes = t(self.h_blk_lanes)
+
+
+class THPlan:
+    """Row-incidence tables, work blocks and the VJP workspace of the
+    Taylor-Hood kernels for one assembly map, on one device.  Calls that share
+    a plan share its workspace: VJPs of one map on different streams must not
+    overlap.  `variants` lists extra (items, tile_nnz) block layouts to build
+    while the map's host tables exist; `select` switches among them (results
+    do not depend on the layout: every entry is summed in ascending element
+    order whatever the blocks)."""
+
+    def __init__(self, amap: AssemblyMap, device="cuda", items=TH_ITEMS, tile_nnz=TH_TILE_NNZ,
+                 variants=()):
+        dev = torch.device(device)
+        info = amap.info
+        off = np.empty(amap.num_res + 1, np.int32)
+        elem = np.empty(max(info.num_inc, 1), np.int32)
+        meta = np.empty(max(4 * info.num_inc, 4), np.int32)
+        _lib.call("fx_map_export_rowinc15", amap._handle(), ptr(off), ptr(elem), ptr(meta))
         t = lambda a: torch.from_numpy(a).to(dev)  # noqa: E731
         self.inc_off, self.inc_elem = t(off), t(elem[: info.num_inc])
         self.inc_meta = t(meta[: 4 * info.num_inc])
@@ -124,8 +139,19 @@ class THPlan:
         _lib.call("fx_map_export_rowinc", amap._handle(), ptr(off), None, ptr(slots))
         self.elem_inc = t(slots[: 15 * amap.num_elems])
         self.work = torch.empty(15 * amap.num_elems, dtype=torch.float64, device=dev)
-        self.blk_row, self.blk_flags = t(blk_row), t(self.h_blk_flags)
-        self.blk_lanes = t(self.h_blk_lanes)
+        self.layouts = {}
+        for it, tl in [(items, tile_nnz)] + list(variants):
+            if (int(it), int(tl)) not in self.layouts:
+                self.layouts[(int(it), int(tl))] = THBlocks(amap, dev, it, tl)
+        self.select(items, tile_nnz)
+
+    def select(self, items, tile_nnz):
+        """Use the block layout (items, tile_nnz), built at construction."""
+        b = self.layouts[(int(items), int(tile_nnz))]
+        self.items, self.tile_nnz = b.items, b.tile_nnz
+        self.num_blocks, self.max_block_nnz = b.num_blocks, b.max_block_nnz
+        self.h_blk_row, self.h_blk_flags, self.h_blk_lanes = b.h_blk_row, b.h_blk_flags, b.h_blk_lanes
+        self.blk_row, self.blk_flags, self.blk_lanes = b.blk_row, b.blk_flags, b.blk_lanes
 
 
 class TaylorHoodNS:

@@ -485,23 +485,40 @@ typedef struct fx_batch fx_batch;
 enum
 {
   FX_BATCH_ASSEMBLE = 1,
-  FX_BATCH_VJP      = 2
+  FX_BATCH_VJP      = 2,
+  FX_BATCH_SERIAL   = 4 /* with either: launch the chains one after another on one stream */
 };
 
 /* grid_per_cu: persistent workgroups per compute unit (the launch grid is
- * compute units x grid_per_cu).  Copies the descriptors to the device. */
+ * compute units x grid_per_cu, at least one full residency), or 0 for one
+ * workgroup per work item.  Copies the descriptors to the device. */
 int fx_batch_create(fx_index num_p1, const fx_p1_instance* h_p1, fx_index num_th,
                     const fx_th_instance* h_th, fx_index grid_per_cu, fx_batch** out);
 /* Work items per phase (P1 assembly, TH assembly, P1 VJP, TH VJP element,
  * TH VJP gather) and the persistent grid size. */
 int fx_batch_work(const fx_batch* batch, fx_index* h_counts5, fx_index* h_grid);
-/* Launch the phases in `what` (FX_BATCH_ASSEMBLE | FX_BATCH_VJP) on stream. */
+/* Launch the phases in `what` (FX_BATCH_ASSEMBLE | FX_BATCH_VJP) ordered
+ * after earlier work on stream, and make later work on stream wait for
+ * them.  The independent chains (P1 assembly, TH assembly, P1 VJP, TH VJP)
+ * run concurrently on the batch's private streams (default priority), or in
+ * that order on stream with FX_BATCH_SERIAL. */
 int fx_batch_run(fx_batch* batch, int what, void* stream);
-/* Capture the phases in `what` into a hipGraph (on a private stream). */
+/* Capture the phases in `what` into a hipGraph (on a private stream), one
+ * graph branch per chain (a single chain of nodes with FX_BATCH_SERIAL). */
 int fx_batch_capture(fx_batch* batch, int what);
 /* Replay the captured graph on stream (one launch). */
 int fx_batch_launch(fx_batch* batch, void* stream);
 void fx_batch_destroy(fx_batch* batch);
+/* One hipGraph over several batches (e.g. one per instance-size class, each
+ * with its own grid and Taylor-Hood row tile): every chain of every batch is
+ * a branch of the graph, or, with FX_BATCH_SERIAL, every phase in turn over
+ * the batches.  Replays on one graph, or on graphs sharing a
+ * batch, must not overlap.  The batches must outlive the graph. */
+typedef struct fx_batch_graph fx_batch_graph;
+int fx_batch_graph_create(fx_index count, fx_batch* const* batches, int what,
+                          fx_batch_graph** out);
+int fx_batch_graph_launch(fx_batch_graph* graph, void* stream);
+void fx_batch_graph_destroy(fx_batch_graph* graph);
 
 #ifdef __cplusplus
 }

@@ -60,8 +60,9 @@ def test_batch_assembly_only_and_autotune():
     torch.cuda.synchronize()
     got = [[t.detach().cpu().numpy() for t in c.outputs()[:2]] for c in cases]
     assert same(got, ref)
-    best, timings = B.autotune(cases, (1, 2), reps=2)
-    assert best in (1, 2) and all(t > 0 for t in timings.values())
+    runner, report = B.autotune(cases, layouts=B.TH_LAYOUTS[:2], grids=(1, 2), reps=2)
+    for g in report["groups"].values():
+        assert g["grid_per_cu"] in (1, 2) and all(t > 0 for t in g["grid_ms"].values())
 
 
 def test_instance_specs_follow_config5():
@@ -71,3 +72,44 @@ def test_instance_specs_follow_config5():
     ns = np.array([x["n"] for x in s])
     assert ns.min() >= 16 and ns.max() <= 512
     assert all(10.0 <= x["Re"] <= 400.0 for x in s if x["kind"] == "th")
+
+
+def test_grouped_graph_and_autotune_bitwise():
+    """The autotuner's product -- one hipGraph over per-size-class batches
+    or one batch, chains in series or concurrent, chosen block layouts and
+    grids (including one workgroup per work item) -- is bitwise equal to the
+    plain calls, replay after replay; every shape it tries is too."""
+    specs = B.instance_specs(count=12, seed=5, n_min=3, n_max=90)
+    cases = B.build_cases(specs, "cuda")
+    assert len({B.size_class(c) for c in cases}) >= 2
+    B.plain(cases, 3)
+    ref = snapshot(cases)
+    runner, report = B.autotune(cases, grids=(0, 3), reps=1)
+    assert report["groups"] and all("grid_per_cu" in g for g in report["groups"].values())
+    assert report["graph"] in report["graph_ms"] and len(report["graph_ms"]) == 4
+    for k in range(3):
+        clear(cases)
+        runner.launch()
+        assert same(snapshot(cases), ref), f"tuned graph replay {k}"
+        B.plain(cases[:1], 1)
+    # every candidate graph shape: per-class batches and one batch, chains
+    # in series and as concurrent branches
+    th = [c for c in cases if isinstance(c, B.THCase)]
+    p1 = [c for c in cases if isinstance(c, B.P1Case)]
+    parts = [B.Batch(p1, 2), B.Batch(th, 0)]
+    for what in (3, 3 | B.Batch.SERIAL):
+        for bs in (parts, [B.Batch(cases, 1)]):
+            g = B.BatchGraph(bs, what)
+            for k in range(2):
+                clear(cases)
+                g.launch()
+                assert same(snapshot(cases), ref), (what, len(bs), k)
+    for lay in B.TH_LAYOUTS:
+        for c in cases:
+            if isinstance(c, B.THCase):
+                c.plan.select(*lay)
+        for g, what in ((0, 3), (2, 3 | B.Batch.SERIAL)):
+            b = B.Batch(cases, g)
+            clear(cases)
+            b.run(what)
+            assert same(snapshot(cases), ref), (lay, g, what)

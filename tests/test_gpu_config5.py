@@ -88,6 +88,13 @@ def test_config5_full_size_batch_bitwise_and_oracle():
         b.launch()
         same_as_plain(f"graph replay {rep}")
         cases[0].plain(1)  # eager work between replays
+    # the benchmark's form: autotuned per-size-class batches in one graph
+    runner, _ = B.autotune(cases, reps=1)
+    for rep in range(2):
+        clear()
+        runner.launch()
+        same_as_plain(f"tuned graph replay {rep}")
+        cases[0].plain(1)
     del ref
 
     p1 = [c for c in cases if isinstance(c, B.P1Case)]
