@@ -68,6 +68,12 @@ _SIGS = {
                         C.c_int),
     "fx_th_vjp": ([_i32] * 2 + [_vp] * 5 + [_f64] + [_vp] * 8 + [_vp], C.c_int),
     "fx_th_vjp_noalloc": ([_i32] * 2 + [_vp] * 5 + [_f64] + [_vp] * 7 + [_vp], C.c_int),
+    "fx_th_patches_create": ([_i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, C.POINTER(_vp)],
+                             C.c_int),
+    "fx_th_patches_info_get": ([_vp, _vp], C.c_int),
+    "fx_th_patches_export": ([_vp] * 10, C.c_int),
+    "fx_th_patches_destroy": ([_vp], None),
+    "fx_th_vjp_patch": ([_i32] * 2 + [_vp] * 5 + [_f64] + [_vp] * 4 + [_vp] * 2 + [_vp], C.c_int),
     "fx_batch_create": ([_i32, _vp, _i32, _vp, _i32, C.POINTER(_vp)], C.c_int),
     "fx_batch_work": ([_vp, _vp, _vp], C.c_int),
     "fx_batch_run": ([_vp, C.c_int, _vp], C.c_int),
@@ -86,13 +92,27 @@ class P1Instance(C.Structure):
                                       "vals", "res", "lam", "bc_mask", "jt", "g")])
 
 
+
+
+class PatchInfo(C.Structure):
+    _fields_ = [(n, _i32) for n in ("num_patches", "num_ldofs", "num_seam", "num_parts")]
+
+
+class PatchPlan(C.Structure):
+    _fields_ = ([(n, _i32) for n in ("num_patches", "num_seam", "num_parts")]
+                + [(n, _vp) for n in ("patch_off", "patch_elem", "elem_slot", "ldof_off",
+                                      "ldof_dof", "ldof_part", "ldof_end", "seam_off", "seam_dof",
+                                      "parts")])
+
+
 class THInstance(C.Structure):
     _fields_ = ([(n, _i32) for n in ("ndof", "num_elems", "num_blocks", "max_block_nnz")]
                 + [(n, _vp) for n in ("xv", "yv", "vconn3", "dofs15", "state")]
                 + [("nu", _f64)]
                 + [(n, _vp) for n in ("nu_e", "inc_off", "inc_elem", "inc_meta", "blk_row",
-                                      "blk_flags", "row_ptr", "vals", "res", "elem_inc", "lam",
-                                      "bc_mask", "jt", "g", "work", "blk_lanes")])
+                                      "blk_flags", "row_ptr", "vals", "res", "lam", "bc_mask",
+                                      "jt", "g", "blk_lanes")]
+                + [("vjp", PatchPlan)])
 
 
 class MapInfo(C.Structure):

@@ -106,6 +106,8 @@ class THCase:
         self.bmap = make_boundary_map(dofs, self.map.pattern, device)
         self.plan = ns.THPlan(self.map, device, items, variants=layouts)
         setattr(self.map, "_th_plan_" + str(torch.device(device)), self.plan)
+        self.patches = ns.THPatches(g, device)
+        setattr(self.map, "_th_patches_" + str(torch.device(device)), self.patches)
         self.map.release_host()
         self.jac = CsrMatrix(self.map.pattern, device)
         self.res = torch.empty(g.ndof, dtype=torch.float64, device=device)
@@ -121,8 +123,8 @@ class THCase:
             g.ndof, g.num_elems, p.num_blocks, p.max_block_nnz, ptr(g.xv), ptr(g.yv),
             ptr(g.vconn3), ptr(g.dofs15), ptr(self.state), self.nu, None, ptr(p.inc_off),
             ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row), ptr(p.blk_flags), ptr(rp),
-            ptr(self.jac.vals), ptr(self.res), ptr(p.elem_inc), ptr(self.lam), ptr(self.bmap.d_mask),
-            ptr(self.jt), ptr(self.g), ptr(p.work), ptr(p.blk_lanes))
+            ptr(self.jac.vals), ptr(self.res), ptr(self.lam), ptr(self.bmap.d_mask),
+            ptr(self.jt), ptr(self.g), ptr(p.blk_lanes), self.patches.plan)
 
     def plain(self, what):
         op = ns.TaylorHoodNS(self.nu)

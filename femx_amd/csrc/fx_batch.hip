@@ -12,8 +12,8 @@
 // blocks share an L2 -- running the same per-block device code as the
 // single-instance kernels, so each instance's results are bitwise equal to
 // its plain launch.  The phases form four independent chains -- P1 assembly,
-// Taylor-Hood assembly, P1 VJP, and the Taylor-Hood VJP's element then gather
-// phase -- launched on private branch streams forked from the caller's
+// Taylor-Hood assembly, P1 VJP, and the Taylor-Hood VJP's patch then seam
+// pass -- launched on private branch streams forked from the caller's
 // stream, so they run concurrently; captured, they become the branches of
 // one hipGraph, replayed with a single hipGraphLaunch.  Several batches (for
 // instance one per size class, each with its own grid and row tile) can be
@@ -41,8 +41,9 @@ struct fx_batch
   fxk::P1AsmArgs*  d_p1_asm  = nullptr;
   fxk::P1VjpArgs*  d_p1_vjp  = nullptr;
   fxk::ThAsmArgs*  d_th_asm  = nullptr;
-  fxk::ThVjpArgs*  d_th_vjp  = nullptr;
-  fxk::GatherArgs* d_th_gat  = nullptr;
+  fxk::ThVjpArgs*   d_th_vjp  = nullptr;
+  fxk::ThPatchArgs* d_th_pat  = nullptr;
+  fxk::ThSeamArgs*  d_th_seam = nullptr;
   int2*            d_work[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   int              nwork[5]  = {0, 0, 0, 0, 0};
   hipStream_t      cap_stream = nullptr;
@@ -68,7 +69,7 @@ struct fx_batch
       if (e) (void)hipEventDestroy(e);
     for (void* p : {static_cast<void*>(d_p1_asm), static_cast<void*>(d_p1_vjp),
                     static_cast<void*>(d_th_asm), static_cast<void*>(d_th_vjp),
-                    static_cast<void*>(d_th_gat)})
+                    static_cast<void*>(d_th_pat), static_cast<void*>(d_th_seam)})
       if (p) (void)hipFree(p);
     for (auto* w : d_work)
       if (w) (void)hipFree(w);
@@ -97,8 +98,8 @@ enum Phase
   kP1Asm = 0,
   kThAsm = 1,
   kP1Vjp = 2,
-  kThVjpElem = 3,
-  kThVjpGather = 4
+  kThVjpPatch = 3,
+  kThVjpSeam = 4
 };
 
 // capped at 96 VGPRs (5 waves/SIMD, as k_p1_assemble reaches uncapped)
@@ -151,27 +152,28 @@ __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per
   }
 }
 
-__global__ __launch_bounds__(kThLanes) void k_batch_th_vjp_elem(const ThVjpArgs* __restrict__ inst,
-                                                               const int2* __restrict__ work,
-                                                               int nwork)
+__global__ __launch_bounds__(kThPatch) void k_batch_th_vjp_patch(const ThVjpArgs* __restrict__ inst,
+                                                                const ThPatchArgs* __restrict__ pat,
+                                                                const int2* __restrict__ work,
+                                                                int nwork)
 {
+  __shared__ double cs[15 * kThPatch];
   const fx::XcdWalk wk = fx::xcd_walk(nwork);
   for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
-    th_vjp_elem_block(it.y, inst[it.x], kThTables);
+    th_vjp_patch_block(it.y, inst[it.x], pat[it.x], kThTables, cs);
   }
 }
 
-__global__ __launch_bounds__(256) void k_batch_vjp_gather(const GatherArgs* __restrict__ inst,
-                                                         const int2* __restrict__ work, int nwork)
+__global__ __launch_bounds__(256) void k_batch_th_vjp_seam(const ThSeamArgs* __restrict__ inst,
+                                                          const int2* __restrict__ work, int nwork)
 {
-  __shared__ double sval[kGatherStage];
   const fx::XcdWalk wk = fx::xcd_walk(nwork);
   for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
-    vjp_gather_block(it.y, inst[it.x], sval);
+    th_vjp_seam_block(it.y, inst[it.x]);
   }
 }
 
@@ -211,11 +213,12 @@ void launch_chain(fx_batch* b, int chain, hipStream_t s)
                        b->d_work[kP1Vjp], b->nwork[kP1Vjp]);
     break;
   default:
-    hipLaunchKernelGGL(k_batch_th_vjp_elem, g(kThVjpElem), dim3(kThLanes), 0, s, b->d_th_vjp,
-                       b->d_work[kThVjpElem], b->nwork[kThVjpElem]);
+    hipLaunchKernelGGL(k_batch_th_vjp_patch, g(kThVjpPatch), dim3(kThPatch), 0, s, b->d_th_vjp,
+                       b->d_th_pat, b->d_work[kThVjpPatch], b->nwork[kThVjpPatch]);
     fx::check_launch("fx_batch: VJP");
-    hipLaunchKernelGGL(k_batch_vjp_gather, g(kThVjpGather), dim3(256), 0, s, b->d_th_gat,
-                       b->d_work[kThVjpGather], b->nwork[kThVjpGather]);
+    if (b->nwork[kThVjpSeam] > 0)
+      hipLaunchKernelGGL(k_batch_th_vjp_seam, g(kThVjpSeam), dim3(256), 0, s, b->d_th_seam,
+                         b->d_work[kThVjpSeam], b->nwork[kThVjpSeam]);
   }
   fx::check_launch("fx_batch");
 }
@@ -227,7 +230,7 @@ bool chain_active(const fx_batch* b, int chain, int what)
   case 0: return (what & FX_BATCH_ASSEMBLE) && b->nwork[kP1Asm] > 0;
   case 1: return (what & FX_BATCH_ASSEMBLE) && b->nwork[kThAsm] > 0;
   case 2: return (what & FX_BATCH_VJP) && b->nwork[kP1Vjp] > 0;
-  default: return (what & FX_BATCH_VJP) && b->nwork[kThVjpElem] > 0;
+  default: return (what & FX_BATCH_VJP) && b->nwork[kThVjpPatch] > 0;
   }
 }
 
@@ -331,7 +334,8 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     std::vector<P1VjpArgs>  pv;
     std::vector<ThAsmArgs>  ta;
     std::vector<ThVjpArgs>  tv;
-    std::vector<GatherArgs> tg;
+    std::vector<ThPatchArgs> tp;
+    std::vector<ThSeamArgs>  tsm;
     std::vector<int2>       w[5];
     int                     max_row_len = 1;
     fx_index                max_blk_nnz = 0;
@@ -377,16 +381,24 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
                     reinterpret_cast<const uint4*>(q.inc_meta), q.blk_row, q.blk_flags, q.blk_lanes,
                     q.row_ptr, q.vals, q.res});
       for (int k = 0; k < q.num_blocks; ++k) w[kThAsm].push_back(make_int2(i, k));
-      const bool vjp = q.lam && q.jt && q.work && q.elem_inc;
+      const fx_th_patch_plan& pv = q.vjp;
+      const bool vjp = q.lam && q.jt && pv.num_patches > 0 && pv.patch_off && pv.patch_elem
+                       && pv.elem_slot && pv.ldof_off && pv.ldof_dof && pv.ldof_part && pv.ldof_end
+                       && pv.seam_off && (pv.num_seam == 0 || pv.seam_dof)
+                       && (pv.num_parts == 0 || pv.parts);
+      fx::require(pv.num_patches >= 0 && pv.num_seam >= 0 && pv.num_parts >= 0
+                      && fx::fits32(pv.num_parts, 8),
+                  FX_ERR_ARG, "fx_batch_create: invalid Taylor-Hood VJP patch plan");
       tv.push_back({q.num_elems, q.xv, q.yv, q.vconn3, q.dofs15, q.state, q.nu, q.nu_e, q.lam,
-                    q.bc_mask, q.elem_inc, q.work, q.g});
-      tg.push_back({q.ndof, q.inc_off, q.work, q.lam, q.bc_mask, q.jt});
+                    q.bc_mask, nullptr, nullptr, q.g});
+      tp.push_back({pv.patch_off, pv.patch_elem, pv.elem_slot, pv.ldof_off, pv.ldof_dof,
+                    pv.ldof_part, pv.ldof_end, q.jt, pv.parts});
+      tsm.push_back({pv.num_seam, pv.seam_off, pv.seam_dof, pv.parts, q.lam, q.bc_mask, q.jt});
       if (vjp)
       {
-        const int ne = static_cast<int>(fx::blocks_for(q.num_elems, kThLanes));
-        const int nd = static_cast<int>(fx::blocks_for(q.ndof, 256));
-        for (int k = 0; k < ne; ++k) w[kThVjpElem].push_back(make_int2(i, k));
-        for (int k = 0; k < nd; ++k) w[kThVjpGather].push_back(make_int2(i, k));
+        const int ns = static_cast<int>(fx::blocks_for(pv.num_seam, 256));
+        for (int k = 0; k < pv.num_patches; ++k) w[kThVjpPatch].push_back(make_int2(i, k));
+        for (int k = 0; k < ns; ++k) w[kThVjpSeam].push_back(make_int2(i, k));
       }
     }
     b->th_tile = static_cast<int>(max_blk_nnz);
@@ -417,15 +429,16 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     b->pgrid[kThAsm] = occ(kThAsm, reinterpret_cast<const void*>(k_batch_th_assemble),
                            kThLanes * kThWaves, b->th_lds);
     b->pgrid[kP1Vjp] = occ(kP1Vjp, reinterpret_cast<const void*>(k_batch_p1_vjp), kP1Block, 0);
-    b->pgrid[kThVjpElem] =
-        occ(kThVjpElem, reinterpret_cast<const void*>(k_batch_th_vjp_elem), kThLanes, 0);
-    b->pgrid[kThVjpGather] =
-        occ(kThVjpGather, reinterpret_cast<const void*>(k_batch_vjp_gather), 256, 0);
+    b->pgrid[kThVjpPatch] =
+        occ(kThVjpPatch, reinterpret_cast<const void*>(k_batch_th_vjp_patch), kThPatch, 0);
+    b->pgrid[kThVjpSeam] =
+        occ(kThVjpSeam, reinterpret_cast<const void*>(k_batch_th_vjp_seam), 256, 0);
     b->d_p1_asm = upload(pa);
     b->d_p1_vjp = upload(pv);
     b->d_th_asm = upload(ta);
     b->d_th_vjp = upload(tv);
-    b->d_th_gat = upload(tg);
+    b->d_th_pat  = upload(tp);
+    b->d_th_seam = upload(tsm);
     for (int p = 0; p < 5; ++p)
     {
       b->nwork[p]  = static_cast<int>(w[p].size());

@@ -692,6 +692,205 @@ int fx_map_gls_plan(const fx_map* m, Index num_nodes, const Index* conn3, Index*
 
 void fx_map_destroy(fx_map* m) { delete m; }
 
+} // extern "C"
+
+// ---------------------------------------------------------------------------
+// Patch plan of the single-pass Taylor-Hood VJP.  Elements are ordered along
+// a Morton (Z-order) curve of their centroids and cut into patches of
+// consecutive elements, so a patch is a compact piece of the mesh; inside a
+// patch elements are in ascending id.  A patch's local DOFs (ascending global
+// id) each own a run of slots -- one per element of the patch holding the
+// DOF, in element order -- and elem_slot gives every (element, local k) its
+// slot.  A DOF held by elements of one patch only is finished by that patch;
+// any other DOF (on a seam between patches, or in no element) is summed from
+// its per-patch partials, in patch order, by the seam pass.
+
+struct fx_th_patches
+{
+  Index ndof = 0, num_elems = 0, num_parts = 0;
+  std::vector<Index>         patch_off, patch_elem, ldof_off, ldof_dof, ldof_part, seam_off, seam_dof;
+  std::vector<std::uint16_t> elem_slot, ldof_end;
+};
+
+namespace
+{
+// Spread the low 16 bits of v to the even bit positions.
+std::uint32_t spread16(std::uint32_t v)
+{
+  v &= 0xffffu;
+  v = (v | (v << 8)) & 0x00ff00ffu;
+  v = (v | (v << 4)) & 0x0f0f0f0fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+} // namespace
+
+extern "C" {
+
+int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const Index* vconn3,
+                         Index num_vertices, const double* xv, const double* yv, Index max_patch,
+                         fx_th_patches** out)
+{
+  return fx::guard([&] {
+    fx::require(out && ndof >= 0 && num_elems >= 0 && num_vertices >= 0
+                    && (num_elems == 0 || (dofs15 && vconn3 && xv && yv)),
+                FX_ERR_ARG, "fx_th_patches_create: invalid arguments");
+    fx::require(max_patch >= 1 && max_patch <= FX_TH_PATCH_MAX, FX_ERR_ARG,
+                "fx_th_patches_create: max_patch must be in [1, FX_TH_PATCH_MAX]");
+    *out        = nullptr;
+    auto        p = std::make_unique<fx_th_patches>();
+    const Index E = num_elems;
+    p->ndof       = ndof;
+    p->num_elems  = E;
+    for (Index e = 0; e < E; ++e)
+    {
+      for (int k = 0; k < 15; ++k)
+        fx::require(dofs15[15 * e + k] >= 0 && dofs15[15 * e + k] < ndof, FX_ERR_ARG,
+                    "fx_th_patches_create: DOF id out of range");
+      for (int k = 0; k < 3; ++k)
+        fx::require(vconn3[3 * e + k] >= 0 && vconn3[3 * e + k] < num_vertices, FX_ERR_ARG,
+                    "fx_th_patches_create: vertex id out of range");
+    }
+    // Morton order of the element centroids (ties by element id)
+    double lo[2] = {0.0, 0.0}, hi[2] = {0.0, 0.0};
+    std::vector<double> cx(E), cy(E);
+    for (Index e = 0; e < E; ++e)
+    {
+      const Index* v = vconn3 + 3 * e;
+      cx[e]          = (xv[v[0]] + xv[v[1]] + xv[v[2]]) / 3.0;
+      cy[e]          = (yv[v[0]] + yv[v[1]] + yv[v[2]]) / 3.0;
+      if (e == 0 || cx[e] < lo[0]) lo[0] = cx[e];
+      if (e == 0 || cy[e] < lo[1]) lo[1] = cy[e];
+      if (e == 0 || cx[e] > hi[0]) hi[0] = cx[e];
+      if (e == 0 || cy[e] > hi[1]) hi[1] = cy[e];
+    }
+    const double span = std::max(hi[0] - lo[0], hi[1] - lo[1]);
+    const double sc   = span > 0.0 ? 65535.0 / span : 0.0;
+    std::vector<std::uint64_t> key(E);
+    for (Index e = 0; e < E; ++e)
+    {
+      const auto qx = static_cast<std::uint32_t>(std::min(65535.0, std::max(0.0, (cx[e] - lo[0]) * sc)));
+      const auto qy = static_cast<std::uint32_t>(std::min(65535.0, std::max(0.0, (cy[e] - lo[1]) * sc)));
+      key[e] = (std::uint64_t(spread16(qx) | (spread16(qy) << 1)) << 32) | std::uint64_t(e);
+    }
+    std::sort(key.begin(), key.end());
+    const Index P = (E + max_patch - 1) / max_patch;
+    p->patch_off.resize(P + 1);
+    p->patch_elem.resize(E);
+    for (Index q = 0; q <= P; ++q) p->patch_off[q] = std::min(E, q * max_patch);
+    for (Index e = 0; e < E; ++e) p->patch_elem[e] = static_cast<Index>(key[e] & 0xffffffffu);
+    for (Index q = 0; q < P; ++q)
+      std::sort(p->patch_elem.begin() + p->patch_off[q], p->patch_elem.begin() + p->patch_off[q + 1]);
+
+    // patches touching each DOF, in patch order
+    std::vector<Index> npatch(ndof, 0), last(ndof, -1);
+    for (Index q = 0; q < P; ++q)
+      for (Index i = p->patch_off[q]; i < p->patch_off[q + 1]; ++i)
+        for (int k = 0; k < 15; ++k)
+        {
+          const Index d = dofs15[15 * p->patch_elem[i] + k];
+          if (last[d] != q) last[d] = q, ++npatch[d];
+        }
+    // seam DOFs: touched by several patches or by none; partials in patch order
+    std::vector<Index> part_next(ndof, -1);
+    p->seam_off.push_back(0);
+    for (Index d = 0; d < ndof; ++d)
+      if (npatch[d] != 1)
+      {
+        part_next[d] = p->seam_off.back();
+        p->seam_dof.push_back(d);
+        p->seam_off.push_back(p->seam_off.back() + npatch[d]);
+      }
+    p->num_parts = p->seam_off.back();
+
+    // local DOFs and slots of each patch
+    p->elem_slot.assign(16 * std::size_t(E), 0);
+    p->ldof_off.push_back(0);
+    std::vector<Index> local(ndof, -1), count, cursor, dofs;
+    for (Index q = 0; q < P; ++q)
+    {
+      const Index i0 = p->patch_off[q], i1 = p->patch_off[q + 1];
+      dofs.clear();
+      for (Index i = i0; i < i1; ++i)
+        for (int k = 0; k < 15; ++k)
+        {
+          const Index d = dofs15[15 * p->patch_elem[i] + k];
+          if (local[d] < 0) local[d] = 0, dofs.push_back(d);
+        }
+      std::sort(dofs.begin(), dofs.end());
+      const Index nl = static_cast<Index>(dofs.size());
+      for (Index j = 0; j < nl; ++j) local[dofs[j]] = j;
+      count.assign(nl, 0);
+      for (Index i = i0; i < i1; ++i)
+        for (int k = 0; k < 15; ++k)
+        {
+          const Index d = dofs15[15 * p->patch_elem[i] + k];
+          for (int kk = 0; kk < k; ++kk)
+            fx::require(dofs15[15 * p->patch_elem[i] + kk] != d, FX_ERR_UNSUPPORTED,
+                        "fx_th_patches_create: an element lists one DOF twice");
+          ++count[local[d]];
+        }
+      cursor.assign(nl, 0);
+      Index run = 0;
+      for (Index j = 0; j < nl; ++j)
+      {
+        cursor[j] = run;
+        run += count[j];
+        p->ldof_end.push_back(static_cast<std::uint16_t>(run));
+        const Index d = dofs[j];
+        p->ldof_dof.push_back(d);
+        p->ldof_part.push_back(npatch[d] == 1 ? Index(-1) : part_next[d]++);
+      }
+      for (Index i = i0; i < i1; ++i)
+        for (int k = 0; k < 15; ++k)
+        {
+          const Index d                   = dofs15[15 * p->patch_elem[i] + k];
+          p->elem_slot[16 * std::size_t(i) + k] = static_cast<std::uint16_t>(cursor[local[d]]++);
+        }
+      for (Index d : dofs) local[d] = -1;
+      p->ldof_off.push_back(p->ldof_off.back() + nl);
+    }
+    *out = p.release();
+  });
+}
+
+int fx_th_patches_info_get(const fx_th_patches* p, fx_th_patches_info* info)
+{
+  return fx::guard([&] {
+    fx::require(p && info, FX_ERR_ARG, "fx_th_patches_info_get: null argument");
+    info->num_patches = static_cast<Index>(p->patch_off.size()) - 1;
+    info->num_ldofs   = static_cast<Index>(p->ldof_dof.size());
+    info->num_seam    = static_cast<Index>(p->seam_dof.size());
+    info->num_parts   = p->num_parts;
+  });
+}
+
+int fx_th_patches_export(const fx_th_patches* p, Index* patch_off, Index* patch_elem,
+                         unsigned short* elem_slot, Index* ldof_off, Index* ldof_dof,
+                         Index* ldof_part, unsigned short* ldof_end, Index* seam_off,
+                         Index* seam_dof)
+{
+  return fx::guard([&] {
+    fx::require(p && patch_off && ldof_off && seam_off
+                    && (p->num_elems == 0 || (patch_elem && elem_slot))
+                    && (p->ldof_dof.empty() || (ldof_dof && ldof_part && ldof_end))
+                    && (p->seam_dof.empty() || seam_dof),
+                FX_ERR_ARG, "fx_th_patches_export: null array");
+    std::copy(p->patch_off.begin(), p->patch_off.end(), patch_off);
+    std::copy(p->patch_elem.begin(), p->patch_elem.end(), patch_elem);
+    std::copy(p->elem_slot.begin(), p->elem_slot.end(), elem_slot);
+    std::copy(p->ldof_off.begin(), p->ldof_off.end(), ldof_off);
+    std::copy(p->ldof_dof.begin(), p->ldof_dof.end(), ldof_dof);
+    std::copy(p->ldof_part.begin(), p->ldof_part.end(), ldof_part);
+    std::copy(p->ldof_end.begin(), p->ldof_end.end(), ldof_end);
+    std::copy(p->seam_off.begin(), p->seam_off.end(), seam_off);
+    std::copy(p->seam_dof.begin(), p->seam_dof.end(), seam_dof);
+  });
+}
+
+void fx_th_patches_destroy(fx_th_patches* p) { delete p; }
+
 // ---------------------------------------------------------------------------
 // CSR transpose permutation: counting sort by column, rows visited in order.
 

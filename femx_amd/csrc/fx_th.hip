@@ -40,6 +40,19 @@ __global__ __launch_bounds__(256) void k_vjp_gather(GatherArgs a)
   vjp_gather_block(fx::xcd_block(gridDim.x), a, sval);
 }
 
+static_assert(kThPatch == FX_TH_PATCH_MAX, "one lane per element of a patch");
+
+__global__ __launch_bounds__(kThPatch) void k_th_vjp_patch(ThVjpArgs a, ThPatchArgs q)
+{
+  __shared__ double cs[15 * kThPatch];
+  th_vjp_patch_block(fx::xcd_block(gridDim.x), a, q, kThTables, cs);
+}
+
+__global__ __launch_bounds__(256) void k_th_vjp_seam(ThSeamArgs a)
+{
+  th_vjp_seam_block(fx::xcd_block(gridDim.x), a);
+}
+
 // The two VJP phases with the element-column workspace `work` (15 doubles
 // per element); shared by fx_th_vjp (caller workspace) and
 // fx_th_vjp_noalloc (the library's fixed device workspace).
@@ -158,6 +171,54 @@ int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* xv, const double*
   return fx::guard([&] {
     th_vjp_launch(ndof, num_elems, xv, yv, vconn3, dofs15, state, nu, nu_e, lam, bc_mask, inc_off,
                   elem_inc, jt, g, work, stream, "fx_th_vjp");
+  });
+}
+
+int fx_th_vjp_patch(fx_index ndof, fx_index num_elems, const double* xv, const double* yv,
+                    const fx_index* vconn3, const fx_index* dofs15, const double* state, double nu,
+                    const double* nu_e, const double* lam, const fx_index* bc_mask,
+                    const fx_th_patch_plan* plan, double* jt, double* g, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(ndof >= 0 && num_elems >= 0 && plan, FX_ERR_ARG,
+                "fx_th_vjp_patch: negative size or null plan");
+    const fx_th_patch_plan& q = *plan;
+    fx::require(q.num_patches >= 0 && q.num_seam >= 0 && q.num_parts >= 0
+                    && (q.num_patches > 0) == (num_elems > 0),
+                FX_ERR_ARG, "fx_th_vjp_patch: plan sizes do not fit the mesh");
+    if (ndof == 0) return;
+    fx::require(fx::fits32(ndof, 8) && fx::fits32(num_elems, 8) && fx::fits32(q.num_parts, 8),
+                FX_ERR_OVERFLOW, "fx_th_vjp_patch: state, element or partial arrays beyond 4 GiB");
+    fx::require(xv && yv && state && lam && jt && q.patch_off && q.ldof_off && q.seam_off
+                    && (num_elems == 0
+                        || (vconn3 && dofs15 && q.patch_elem && q.elem_slot && q.ldof_dof
+                            && q.ldof_part && q.ldof_end))
+                    && (q.num_seam == 0 || q.seam_dof) && (q.num_parts == 0 || q.parts),
+                FX_ERR_ARG, "fx_th_vjp_patch: null array");
+    const std::size_t nd = std::size_t(ndof) * 8, ne = std::size_t(num_elems) * 8,
+                      np = std::size_t(q.num_parts) * 8;
+    fx::require(!fx::overlaps(jt, nd, lam, nd) && !fx::overlaps(jt, nd, state, nd)
+                    && !(g && fx::overlaps(g, ne, jt, nd)) && !fx::overlaps(q.parts, np, jt, nd)
+                    && !fx::overlaps(q.parts, np, lam, nd)
+                    && !(g && fx::overlaps(q.parts, np, g, ne)),
+                FX_ERR_ALIAS, "fx_th_vjp_patch: outputs must not alias inputs or each other");
+    if (q.num_patches > 0)
+    {
+      const ThVjpArgs   a{num_elems, xv, yv, vconn3, dofs15, state, nu, nu_e,
+                          lam,       bc_mask, nullptr, nullptr, g};
+      const ThPatchArgs pa{q.patch_off, q.patch_elem, q.elem_slot, q.ldof_off, q.ldof_dof,
+                           q.ldof_part, q.ldof_end,   jt,          q.parts};
+      hipLaunchKernelGGL(k_th_vjp_patch, dim3(static_cast<unsigned>(q.num_patches)),
+                         dim3(kThPatch), 0, fx::as_stream(stream), a, pa);
+      fx::check_launch("fx_th_vjp_patch");
+    }
+    if (q.num_seam > 0)
+    {
+      const ThSeamArgs sa{q.num_seam, q.seam_off, q.seam_dof, q.parts, lam, bc_mask, jt};
+      hipLaunchKernelGGL(k_th_vjp_seam, dim3(fx::blocks_for(q.num_seam, 256)), dim3(256), 0,
+                         fx::as_stream(stream), sa);
+      fx::check_launch("fx_th_vjp_patch");
+    }
   });
 }
 

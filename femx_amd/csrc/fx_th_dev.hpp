@@ -642,10 +642,9 @@ struct ThVjpArgs
 //   column (b,d): N_b (du_0/dx_d L_0 + du_1/dx_d L_1) + nu grad phi_b . grad L_d
 //                 + L_d (u . grad phi_b) - P d_d phi_b
 //   column p_b' : -psi_b' div L
-__device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, const ThTables& T)
+__device__ __forceinline__ void th_vjp_elem_cols(int e, const ThVjpArgs& a, const ThTables& T,
+                                                 double (&c)[15], double& ge)
 {
-  const int e = blk * kThLanes + threadIdx.x;
-  if (e >= a.num_elems) return;
   // the DOF row and vertices in vector loads, then one 32-bit-offset gather
   // per value (the pressure state does not enter J^T lam or dR/dnu)
   const ThDofs D = th_load_dofs(a.dofs, a.vconn, e);
@@ -654,7 +653,7 @@ __device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, c
   for (int k = 0; k < 12; ++k) E.s[k] = gat32(a.state, D.d[k]);
   E.g  = th_geom(a.xv, a.yv, D.v[0], D.v[1], D.v[2]);
   E.nu = a.nu_e ? a.nu_e[e] : a.nu;
-  double l[15], c[15];
+  double l[15];
 #pragma unroll
   for (int k = 0; k < 15; ++k)
   {
@@ -663,9 +662,9 @@ __device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, c
     const double lv = gat32(a.lam, dk);
     const int    mk = a.bc_mask ? gat32(a.bc_mask, dk) : 0;
     l[k]            = mk ? 0.0 : lv;
-    c[k]         = 0.0;
+    c[k]            = 0.0;
   }
-  double ge = 0.0;
+  ge = 0.0;
 #pragma unroll 1
   for (int q = 0; q < 7; ++q)
   {
@@ -702,12 +701,106 @@ __device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, c
     for (int b = 0; b < 3; ++b) c[12 + b] = fma(-T.psi[q][b] * (g00 + g11), wq, c[12 + b]);
     ge = fma(fma(g00, d00, g01 * d01) + fma(g10, d10, g11 * d11), wq, ge);
   }
+}
+
+__device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, const ThTables& T)
+{
+  const int e = blk * kThLanes + threadIdx.x;
+  if (e >= a.num_elems) return;
+  double c[15], ge;
+  th_vjp_elem_cols(e, a, T, c, ge);
   // each column's value goes to its slot in the row-incidence order, where
   // phase 2 finds a DOF's contributions contiguous
   const int* slot = a.elem_inc + 15 * static_cast<long>(e);
 #pragma unroll
   for (int k = 0; k < 15; ++k) a.cbuf[slot[k]] = c[k];
   if (a.gout) __builtin_nontemporal_store(ge, a.gout + e);
+}
+
+// ---------------------------------------------------------------------------
+// Single-pass VJP over element patches (fx_th_patches_create).  A workgroup
+// of kThPatch lanes takes one patch, one element per lane: it forms the
+// element's columns as above and stores each into its slot of the patch's
+// LDS slot array, where every local DOF owns a run of slots in element
+// order.  After a barrier the lanes sum the runs, each in slot order; a DOF
+// held by this patch alone is finished here (jt, with lam passed through on
+// constrained DOFs), any other gets this patch's partial in its seam slot.
+constexpr int kThPatch = 256;
+
+struct ThPatchArgs
+{
+  const int*            patch_off;
+  const int*            patch_elem;
+  const unsigned short* elem_slot; // 16 per element in patch order
+  const int*            ldof_off;
+  const int*            ldof_dof;
+  const int*            ldof_part;
+  const unsigned short* ldof_end;
+  double*               jt;
+  double*               parts;
+};
+
+__device__ __forceinline__ void th_vjp_patch_block(int p, const ThVjpArgs& a, const ThPatchArgs& q,
+                                                   const ThTables& T, double* cs)
+{
+  const int t  = threadIdx.x;
+  const int i0 = q.patch_off[p], ne = q.patch_off[p + 1] - i0;
+  if (t < ne)
+  {
+    const int i = i0 + t;
+    const int e = q.patch_elem[i];
+    double    c[15], ge;
+    th_vjp_elem_cols(e, a, T, c, ge);
+    typedef unsigned u4 __attribute__((ext_vector_type(4)));
+    const u4*      sp   = reinterpret_cast<const u4*>(q.elem_slot + 16 * static_cast<long>(i));
+    const u4       s0   = __builtin_nontemporal_load(sp), s1 = __builtin_nontemporal_load(sp + 1);
+    const unsigned w[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+#pragma unroll
+    for (int k = 0; k < 15; ++k) cs[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu] = c[k];
+    if (a.gout) __builtin_nontemporal_store(ge, a.gout + e);
+  }
+  __syncthreads();
+  const int l0 = q.ldof_off[p], nl = q.ldof_off[p + 1] - l0;
+  for (int j = t; j < nl; j += kThPatch)
+  {
+    const int beg = j == 0 ? 0 : q.ldof_end[l0 + j - 1], end = q.ldof_end[l0 + j];
+    double    v   = 0.0;
+    for (int s = beg; s < end; ++s) v = v + cs[s];
+    const int d = q.ldof_dof[l0 + j], part = q.ldof_part[l0 + j];
+    if (part < 0)
+    {
+      if (a.bc_mask && a.bc_mask[d]) v = v + a.lam[d];
+      __builtin_nontemporal_store(v, q.jt + d);
+    }
+    else
+      __builtin_nontemporal_store(v, q.parts + part);
+  }
+  __syncthreads(); // the slot array is reused by the next patch of a persistent walk
+}
+
+struct ThSeamArgs
+{
+  int           num_seam;
+  const int*    seam_off;
+  const int*    seam_dof;
+  const double* parts;
+  const double* lam;
+  const int*    bc_mask;
+  double*       jt;
+};
+
+// Seam pass (one lane per seam DOF): the DOF's per-patch partials in patch
+// order, then lam passed through on a constrained DOF.
+__device__ __forceinline__ void th_vjp_seam_block(int blk, const ThSeamArgs& a)
+{
+  const int s = blk * 256 + threadIdx.x;
+  if (s >= a.num_seam) return;
+  double    v  = 0.0;
+  const int t1 = a.seam_off[s + 1];
+  for (int t = a.seam_off[s]; t < t1; ++t) v = v + a.parts[t];
+  const int d = a.seam_dof[s];
+  if (a.bc_mask && a.bc_mask[d]) v = v + a.lam[d];
+  __builtin_nontemporal_store(v, a.jt + d);
 }
 
 

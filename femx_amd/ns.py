@@ -116,10 +116,10 @@ class THBlocks:
 
 
 class THPlan:
-    """Row-incidence tables, work blocks and the VJP workspace of the
-    Taylor-Hood kernels for one assembly map, on one device.  Calls that share
-    a plan share its workspace: VJPs of one map on different streams must not
-    overlap.  `variants` lists extra (items, tile_nnz) block layouts to build
+    """Row-incidence tables, work blocks and the two-phase VJP's workspace
+    of the Taylor-Hood kernels for one assembly map, on one device.  Calls
+    that share a plan share its workspace: VJPs of one map on different
+    streams must not overlap.  `variants` lists extra (items, tile_nnz) block layouts to build
     while the map's host tables exist; `select` switches among them (results
     do not depend on the layout: every entry is summed in ascending element
     order whatever the blocks)."""
@@ -138,12 +138,20 @@ class THPlan:
         slots = np.empty(max(15 * amap.num_elems, 1), np.int32)
         _lib.call("fx_map_export_rowinc", amap._handle(), ptr(off), None, ptr(slots))
         self.elem_inc = t(slots[: 15 * amap.num_elems])
-        self.work = torch.empty(15 * amap.num_elems, dtype=torch.float64, device=dev)
+        self._work, self._dev, self._num_elems = None, dev, amap.num_elems
         self.layouts = {}
         for it, tl in [(items, tile_nnz)] + list(variants):
             if (int(it), int(tl)) not in self.layouts:
                 self.layouts[(int(it), int(tl))] = THBlocks(amap, dev, it, tl)
         self.select(items, tile_nnz)
+
+    @property
+    def work(self):
+        """The two-phase VJP's workspace (fx_th_vjp: 15 doubles per element),
+        allocated on first use."""
+        if self._work is None:
+            self._work = torch.empty(15 * self._num_elems, dtype=torch.float64, device=self._dev)
+        return self._work
 
     def select(self, items, tile_nnz):
         """Use the block layout (items, tile_nnz), built at construction."""
@@ -152,6 +160,58 @@ class THPlan:
         self.num_blocks, self.max_block_nnz = b.num_blocks, b.max_block_nnz
         self.h_blk_row, self.h_blk_flags, self.h_blk_lanes = b.h_blk_row, b.h_blk_flags, b.h_blk_lanes
         self.blk_row, self.blk_flags, self.blk_lanes = b.blk_row, b.blk_flags, b.blk_lanes
+
+
+TH_PATCH = 256  # elements per VJP patch (FX_TH_PATCH_MAX)
+
+
+class THPatches:
+    """Patch plan of the single-pass VJP (fx_th_patches_create) on one
+    device, with its seam-partials workspace: VJPs that share it (one mesh)
+    on different streams must not overlap."""
+
+    def __init__(self, geom: "THGeometry", device, max_patch=TH_PATCH):
+        h = C.c_void_p()
+        _lib.call("fx_th_patches_create", geom.ndof, geom.num_elems, ptr(geom.h_dofs15),
+                  ptr(geom.h_vconn3), geom.num_vertices, ptr(geom.h_xv), ptr(geom.h_yv),
+                  int(max_patch), C.byref(h))
+        try:
+            info = _lib.PatchInfo()
+            _lib.call("fx_th_patches_info_get", h, C.byref(info))
+            P, L, S = info.num_patches, info.num_ldofs, info.num_seam
+            E = geom.num_elems
+            a = {"patch_off": np.empty(P + 1, np.int32), "patch_elem": np.empty(max(E, 1), np.int32),
+                 "elem_slot": np.empty(max(16 * E, 1), np.uint16),
+                 "ldof_off": np.empty(P + 1, np.int32), "ldof_dof": np.empty(max(L, 1), np.int32),
+                 "ldof_part": np.empty(max(L, 1), np.int32),
+                 "ldof_end": np.empty(max(L, 1), np.uint16),
+                 "seam_off": np.empty(S + 1, np.int32), "seam_dof": np.empty(max(S, 1), np.int32)}
+            _lib.call("fx_th_patches_export", h, *[ptr(a[k]) for k in (
+                "patch_off", "patch_elem", "elem_slot", "ldof_off", "ldof_dof", "ldof_part",
+                "ldof_end", "seam_off", "seam_dof")])
+        finally:
+            _lib.lib().fx_th_patches_destroy(h)
+        self.h = a
+        self.num_patches, self.num_ldofs, self.num_seam = P, L, S
+        self.num_parts = info.num_parts
+        dev = torch.device(device)
+        # uint16 tables travel as int16 bit patterns (torch has no uint16 copy path)
+        self.d = {k: torch.from_numpy(v.view(np.int16) if v.dtype == np.uint16 else v).to(dev)
+                  for k, v in a.items()}
+        self.parts = torch.empty(max(self.num_parts, 1), dtype=torch.float64, device=dev)
+        d = self.d
+        self.plan = _lib.PatchPlan(P, S, self.num_parts, *[ptr(d[k]) for k in (
+            "patch_off", "patch_elem", "elem_slot", "ldof_off", "ldof_dof", "ldof_part",
+            "ldof_end", "seam_off", "seam_dof")], ptr(self.parts))
+
+
+def _patches(geom, amap, device):
+    key = "_th_patches_" + str(torch.device(device))
+    p = getattr(amap, key, None)
+    if p is None:
+        p = THPatches(geom, device)
+        setattr(amap, key, p)
+    return p
 
 
 class TaylorHoodNS:
@@ -220,11 +280,10 @@ def ns_vjp(op: TaylorHoodNS, geom: THGeometry, amap: AssemblyMap, state, lam, bc
         jt_out = torch.empty(geom.ndof, dtype=torch.float64, device=dev)
     if g_out is None:
         g_out = torch.empty(geom.num_elems, dtype=torch.float64, device=dev)
-    p = _plan(amap, dev)
-    _lib.call("fx_th_vjp", geom.ndof, geom.num_elems, ptr(geom.xv), ptr(geom.yv),
+    pp = _patches(geom, amap, dev)
+    _lib.call("fx_th_vjp_patch", geom.ndof, geom.num_elems, ptr(geom.xv), ptr(geom.yv),
               ptr(geom.vconn3), ptr(geom.dofs15), ptr(state), op.nu, ptr(op.nu_e), ptr(lam),
-              ptr(bc_mask), ptr(p.inc_off), ptr(p.elem_inc), ptr(jt_out), ptr(g_out), ptr(p.work),
-              stream_handle(stream))
+              ptr(bc_mask), C.byref(pp.plan), ptr(jt_out), ptr(g_out), stream_handle(stream))
     return jt_out, g_out
 
 

@@ -381,6 +381,64 @@ int fx_th_vjp_noalloc(fx_index ndof, fx_index num_elems, const double* d_xv, con
                       const fx_index* d_bc_mask, const fx_index* d_inc_off,
                       const fx_index* d_elem_inc, double* d_jt, double* d_g, void* stream);
 
+/* Patch plan of fx_th_vjp_patch (host).  Elements are ordered along a Morton
+ * curve of their centroids and cut into patches of at most max_patch
+ * (<= FX_TH_PATCH_MAX) consecutive elements, ascending element id inside a
+ * patch.  A patch's local DOFs (ascending id) each own a run of slots, one
+ * per element of the patch holding the DOF, in element order; elem_slot[16 i
+ * + k] is the slot of local DOF k of the i-th element in patch order (entry
+ * 15 unused).  A DOF held by one patch only is finished by it; the others
+ * (on seams between patches, or in no element) are the seam DOFs, summed
+ * from per-patch partials (ldof_part, -1 for a finished DOF) in patch order.
+ * Array sizes: patch_off num_patches+1, patch_elem num_elems, elem_slot
+ * 16*num_elems, ldof_off num_patches+1, ldof_dof/ldof_part/ldof_end
+ * num_ldofs (ldof_end: each local DOF's run end within its patch),
+ * seam_off num_seam+1, seam_dof num_seam. */
+#define FX_TH_PATCH_MAX 256
+typedef struct fx_th_patches fx_th_patches;
+typedef struct
+{
+  fx_index num_patches, num_ldofs, num_seam, num_parts;
+} fx_th_patches_info;
+int fx_th_patches_create(fx_index ndof, fx_index num_elems, const fx_index* h_dofs15,
+                         const fx_index* h_vconn3, fx_index num_vertices, const double* h_xv,
+                         const double* h_yv, fx_index max_patch, fx_th_patches** out);
+int fx_th_patches_info_get(const fx_th_patches* patches, fx_th_patches_info* info);
+int fx_th_patches_export(const fx_th_patches* patches, fx_index* h_patch_off,
+                         fx_index* h_patch_elem, unsigned short* h_elem_slot,
+                         fx_index* h_ldof_off, fx_index* h_ldof_dof, fx_index* h_ldof_part,
+                         unsigned short* h_ldof_end, fx_index* h_seam_off, fx_index* h_seam_dof);
+void fx_th_patches_destroy(fx_th_patches* patches);
+
+/* The patch plan's arrays on the device, and the seam partials workspace
+ * (num_parts doubles). */
+typedef struct
+{
+  fx_index              num_patches, num_seam, num_parts;
+  const fx_index*       patch_off;
+  const fx_index*       patch_elem;
+  const unsigned short* elem_slot;
+  const fx_index*       ldof_off;
+  const fx_index*       ldof_dof;
+  const fx_index*       ldof_part;
+  const unsigned short* ldof_end;
+  const fx_index*       seam_off;
+  const fx_index*       seam_dof;
+  double*               parts;
+} fx_th_patch_plan;
+
+/* fx_th_vjp in one pass over element patches: a workgroup forms its
+ * patch's element columns (as fx_th_vjp's first phase), sums each local
+ * DOF's run in LDS in element order and writes jt for the DOFs it finishes;
+ * a second launch sums the seam DOFs' partials in patch order.  Only the
+ * seam partials go through device memory.  Results are deterministic; they
+ * differ from fx_th_vjp's only in the order of the per-DOF sums. */
+int fx_th_vjp_patch(fx_index ndof, fx_index num_elems, const double* d_xv, const double* d_yv,
+                    const fx_index* d_vconn3, const fx_index* d_dofs15, const double* d_state,
+                    double nu, const double* d_nu_e, const double* d_lam,
+                    const fx_index* d_bc_mask, const fx_th_patch_plan* h_plan, double* d_jt,
+                    double* d_g, void* stream);
+
 /* ======================================================================== */
 /* GLS-stabilised P1/P1 Navier-Stokes time step (device): femx's
  * NavierOperator (femx/model/ns/Components.hpp:400-577) assembled with
@@ -472,13 +530,12 @@ typedef struct
   const fx_index* row_ptr;
   double*       vals;
   double*       res;
-  const fx_index* elem_inc; /* fx_map_export_rowinc */
   const double* lam;
   const fx_index* bc_mask;
   double*       jt;
   double*       g;
-  double*       work; /* 15 * num_elems doubles */
   const fx_index* blk_lanes; /* fx_map_th_lanes, or NULL for two lanes per item */
+  fx_th_patch_plan vjp; /* the VJP's patch plan (fx_th_vjp_patch) */
 } fx_th_instance;
 
 typedef struct fx_batch fx_batch;

@@ -155,3 +155,89 @@ def test_lid_and_walls_found_when_the_last_coordinate_misses_one(n):
     from femx_amd.assembly import structured_p1, unit_square_edge_nodes
     p = structured_p1(n, "cpu")
     assert unit_square_edge_nodes(p.h_x, p.h_y).size == 4 * n
+
+
+def _check_patches(g, max_patch, rng):
+    """Invariants of the VJP patch plan, and the reduction it describes
+    applied to integer-valued element columns (exact in any order) against a
+    plain scatter-add."""
+    pp = ns.THPatches(g, "cpu", max_patch)
+    h = pp.h
+    E, P = g.num_elems, pp.num_patches
+    d15 = g.h_dofs15
+    po, pe = h["patch_off"], h["patch_elem"][:E]
+    assert po[0] == 0 and po[-1] == E and np.all(np.diff(po) >= 1) and np.all(np.diff(po) <= max_patch)
+    assert np.array_equal(np.sort(pe), np.arange(E))
+    lo, ld, lp, le = h["ldof_off"], h["ldof_dof"], h["ldof_part"], h["ldof_end"].astype(np.int64)
+    so, sd = h["seam_off"], h["seam_dof"][:pp.num_seam]
+    slot = h["elem_slot"].reshape(-1, 16)[:E, :15].astype(np.int64)
+    npatch = np.zeros(g.ndof, np.int64)
+    for p in range(P):
+        npatch[np.unique(d15[pe[po[p]:po[p + 1]]])] += 1
+    seam = np.flatnonzero(npatch != 1)
+    assert np.array_equal(sd, seam) and np.array_equal(np.diff(so), npatch[seam])
+    c = rng.integers(-1000, 1000, size=(E, 15)).astype(np.float64)
+    parts = np.full(pp.num_parts, np.nan)
+    jt = np.full(g.ndof, np.nan)
+    for p in range(P):
+        els = pe[po[p]:po[p + 1]]
+        assert np.all(np.diff(els) > 0)
+        dofs = ld[lo[p]:lo[p + 1]]
+        assert np.all(np.diff(dofs) > 0)
+        ends = le[lo[p]:lo[p + 1]]
+        begs = np.concatenate([[0], ends[:-1]])
+        owner = np.repeat(np.arange(dofs.size), ends - begs)
+        s = slot[po[p]:po[p + 1]]
+        assert np.array_equal(np.sort(s.reshape(-1)), np.arange(ends[-1]))
+        assert np.array_equal(dofs[owner[s]], d15[els])
+        # within a run, slots follow element order
+        el_of = np.empty(ends[-1], np.int64)
+        el_of[s.reshape(-1)] = np.repeat(np.arange(els.size), 15)
+        for j in range(dofs.size):
+            assert np.all(np.diff(el_of[begs[j]:ends[j]]) > 0)
+        cs = np.empty(ends[-1])
+        cs[s.reshape(-1)] = c[els].reshape(-1)
+        for j in range(dofs.size):
+            v = cs[begs[j]:ends[j]].sum()
+            if lp[lo[p] + j] < 0:
+                assert npatch[dofs[j]] == 1
+                jt[dofs[j]] = v
+            else:
+                assert np.isnan(parts[lp[lo[p] + j]])
+                parts[lp[lo[p] + j]] = v
+    assert not np.isnan(parts).any()
+    for k, dof in enumerate(sd):
+        jt[dof] = parts[so[k]:so[k + 1]].sum()
+    ref = np.zeros(g.ndof)
+    np.add.at(ref, d15.reshape(-1), c.reshape(-1))
+    assert np.array_equal(jt, ref)
+    return pp
+
+
+@pytest.mark.parametrize("n,max_patch", [(1, 256), (4, 7), (12, 64), (30, 256)])
+def test_vjp_patch_plan_structured(n, max_patch):
+    pp = _check_patches(ns.structured_th(n, "cpu"), max_patch, np.random.default_rng(n))
+    if n == 30:
+        # Morton patches are compact: most DOFs are finished inside a patch
+        assert pp.num_seam < 0.35 * ns.structured_th(n, "cpu").ndof
+
+
+def test_vjp_patch_plan_scrambled_and_unstructured():
+    rng = np.random.default_rng(5)
+    x, y, c3 = O.mesh_p1(9)
+    c3 = c3.reshape(-1, 3)[rng.permutation(2 * 81)]
+    g = ns.th_from_p1(x, y, c3, "cpu")
+    _check_patches(g, 32, rng)
+    # a mesh with an unused vertex: its pressure DOF is in no element
+    xs = np.append(x, 2.0)
+    ys = np.append(y, 2.0)
+    g2 = ns.th_from_p1(xs, ys, c3, "cpu")
+    pp = _check_patches(g2, 256, rng)
+    assert g2.ndof - 1 in pp.h["seam_dof"][:pp.num_seam]
+
+
+def test_vjp_patch_plan_rejects_bad_sizes():
+    g = ns.structured_th(2, "cpu")
+    for mp in (0, 257):
+        with pytest.raises(RuntimeError, match="max_patch"):
+            ns.THPatches(g, "cpu", mp)
