@@ -71,9 +71,9 @@ _SIGS = {
     "fx_th_patches_create": ([_i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, C.POINTER(_vp)],
                              C.c_int),
     "fx_th_patches_info_get": ([_vp, _vp], C.c_int),
-    "fx_th_patches_export": ([_vp] * 10, C.c_int),
+    "fx_th_patches_export": ([_vp] * 11, C.c_int),
     "fx_th_patches_destroy": ([_vp], None),
-    "fx_th_vjp_patch": ([_i32] * 2 + [_vp] * 5 + [_f64] + [_vp] * 4 + [_vp] * 2 + [_vp], C.c_int),
+    "fx_th_vjp_patch": ([_i32] * 2 + [_vp] * 3 + [_f64] + [_vp] * 4 + [_vp] * 2 + [_vp], C.c_int),
     "fx_batch_create": ([_i32, _vp, _i32, _vp, _i32, C.POINTER(_vp)], C.c_int),
     "fx_batch_work": ([_vp, _vp, _vp], C.c_int),
     "fx_batch_run": ([_vp, C.c_int, _vp], C.c_int),
@@ -95,12 +95,13 @@ class P1Instance(C.Structure):
 
 
 class PatchInfo(C.Structure):
-    _fields_ = [(n, _i32) for n in ("num_patches", "num_ldofs", "num_seam", "num_parts")]
+    _fields_ = [(n, _i32) for n in ("num_patches", "num_ldofs", "num_seam", "num_parts",
+                                    "max_patch_ldofs")]
 
 
 class PatchPlan(C.Structure):
-    _fields_ = ([(n, _i32) for n in ("num_patches", "num_seam", "num_parts")]
-                + [(n, _vp) for n in ("patch_off", "patch_elem", "elem_slot", "ldof_off",
+    _fields_ = ([(n, _i32) for n in ("num_patches", "num_seam", "num_parts", "max_patch_ldofs")]
+                + [(n, _vp) for n in ("patch_off", "patch_elem", "elem_vert", "elem_ldof", "ldof_off",
                                       "ldof_dof", "ldof_part", "ldof_end", "seam_off", "seam_dof",
                                       "parts")])
 

@@ -34,7 +34,8 @@ struct fx_batch
   int         grid   = 0;       // compute units x the tunable multiplier
   int         pgrid[5] = {0, 0, 0, 0, 0}; // per phase: at least one full residency
   int         cus    = 0;
-  std::size_t p1_lds = 0, th_lds = 0;
+  std::size_t p1_lds = 0, th_lds = 0, th_vjp_lds = 0;
+  int         th_max_ldofs = 0; // the most local DOFs of a VJP patch over the instances
   int         th_tile = 0; // row tile per wave of the Taylor-Hood assembly (doubles)
   bool        p1_long = false; // some P1 row is longer than the LDS tile width
   // device memory
@@ -152,17 +153,16 @@ __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per
   }
 }
 
-__global__ __launch_bounds__(kThPatch) void k_batch_th_vjp_patch(const ThVjpArgs* __restrict__ inst,
-                                                                const ThPatchArgs* __restrict__ pat,
-                                                                const int2* __restrict__ work,
-                                                                int nwork)
+__global__ __launch_bounds__(kThPatch) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_th_vjp_patch(
+    const ThVjpArgs* __restrict__ inst, const ThPatchArgs* __restrict__ pat,
+    const int2* __restrict__ work, int nwork, int max_patch_ldofs)
 {
-  __shared__ double cs[15 * kThPatch];
+  extern __shared__ __align__(16) unsigned char lds[];
   const fx::XcdWalk wk = fx::xcd_walk(nwork);
   for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
-    th_vjp_patch_block(it.y, inst[it.x], pat[it.x], kThTables, cs);
+    th_vjp_patch_block(it.y, inst[it.x], pat[it.x], kThTables, lds, max_patch_ldofs);
   }
 }
 
@@ -213,8 +213,8 @@ void launch_chain(fx_batch* b, int chain, hipStream_t s)
                        b->d_work[kP1Vjp], b->nwork[kP1Vjp]);
     break;
   default:
-    hipLaunchKernelGGL(k_batch_th_vjp_patch, g(kThVjpPatch), dim3(kThPatch), 0, s, b->d_th_vjp,
-                       b->d_th_pat, b->d_work[kThVjpPatch], b->nwork[kThVjpPatch]);
+    hipLaunchKernelGGL(k_batch_th_vjp_patch, g(kThVjpPatch), dim3(kThPatch), b->th_vjp_lds, s, b->d_th_vjp,
+                       b->d_th_pat, b->d_work[kThVjpPatch], b->nwork[kThVjpPatch], b->th_max_ldofs);
     fx::check_launch("fx_batch: VJP");
     if (b->nwork[kThVjpSeam] > 0)
       hipLaunchKernelGGL(k_batch_th_vjp_seam, g(kThVjpSeam), dim3(256), 0, s, b->d_th_seam,
@@ -383,17 +383,19 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
       for (int k = 0; k < q.num_blocks; ++k) w[kThAsm].push_back(make_int2(i, k));
       const fx_th_patch_plan& pv = q.vjp;
       const bool vjp = q.lam && q.jt && pv.num_patches > 0 && pv.patch_off && pv.patch_elem
-                       && pv.elem_slot && pv.ldof_off && pv.ldof_dof && pv.ldof_part && pv.ldof_end
+                       && pv.elem_vert && pv.elem_ldof && pv.ldof_off && pv.ldof_dof && pv.ldof_part && pv.ldof_end
                        && pv.seam_off && (pv.num_seam == 0 || pv.seam_dof)
                        && (pv.num_parts == 0 || pv.parts);
       fx::require(pv.num_patches >= 0 && pv.num_seam >= 0 && pv.num_parts >= 0
+                      && pv.max_patch_ldofs >= 0 && pv.max_patch_ldofs <= 15 * kThPatch
                       && fx::fits32(pv.num_parts, 8),
                   FX_ERR_ARG, "fx_batch_create: invalid Taylor-Hood VJP patch plan");
       tv.push_back({q.num_elems, q.xv, q.yv, q.vconn3, q.dofs15, q.state, q.nu, q.nu_e, q.lam,
                     q.bc_mask, nullptr, nullptr, q.g});
-      tp.push_back({pv.patch_off, pv.patch_elem, pv.elem_slot, pv.ldof_off, pv.ldof_dof,
-                    pv.ldof_part, pv.ldof_end, q.jt, pv.parts});
+      tp.push_back({pv.patch_off, pv.patch_elem, pv.elem_vert, pv.elem_ldof, pv.ldof_off,
+                    pv.ldof_dof, pv.ldof_part, pv.ldof_end, q.jt, pv.parts});
       tsm.push_back({pv.num_seam, pv.seam_off, pv.seam_dof, pv.parts, q.lam, q.bc_mask, q.jt});
+      if (vjp) b->th_max_ldofs = std::max(b->th_max_ldofs, static_cast<int>(pv.max_patch_ldofs));
       if (vjp)
       {
         const int ns = static_cast<int>(fx::blocks_for(pv.num_seam, 256));
@@ -429,8 +431,10 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     b->pgrid[kThAsm] = occ(kThAsm, reinterpret_cast<const void*>(k_batch_th_assemble),
                            kThLanes * kThWaves, b->th_lds);
     b->pgrid[kP1Vjp] = occ(kP1Vjp, reinterpret_cast<const void*>(k_batch_p1_vjp), kP1Block, 0);
+    b->th_vjp_lds = th_patch_lds(b->th_max_ldofs);
     b->pgrid[kThVjpPatch] =
-        occ(kThVjpPatch, reinterpret_cast<const void*>(k_batch_th_vjp_patch), kThPatch, 0);
+        occ(kThVjpPatch, reinterpret_cast<const void*>(k_batch_th_vjp_patch), kThPatch,
+            b->th_vjp_lds);
     b->pgrid[kThVjpSeam] =
         occ(kThVjpSeam, reinterpret_cast<const void*>(k_batch_th_vjp_seam), 256, 0);
     b->d_p1_asm = upload(pa);

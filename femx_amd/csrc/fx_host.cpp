@@ -707,9 +707,10 @@ void fx_map_destroy(fx_map* m) { delete m; }
 
 struct fx_th_patches
 {
-  Index ndof = 0, num_elems = 0, num_parts = 0;
-  std::vector<Index>         patch_off, patch_elem, ldof_off, ldof_dof, ldof_part, seam_off, seam_dof;
-  std::vector<std::uint16_t> elem_slot, ldof_end;
+  Index ndof = 0, num_elems = 0, num_parts = 0, max_patch_ldofs = 0;
+  std::vector<Index>         patch_off, patch_elem, elem_vert, ldof_off, ldof_dof, ldof_part, seam_off,
+      seam_dof;
+  std::vector<std::uint16_t> elem_ldof, ldof_end;
 };
 
 namespace
@@ -775,13 +776,39 @@ int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const
       key[e] = (std::uint64_t(spread16(qx) | (spread16(qy) << 1)) << 32) | std::uint64_t(e);
     }
     std::sort(key.begin(), key.end());
-    const Index P = (E + max_patch - 1) / max_patch;
-    p->patch_off.resize(P + 1);
+    // Cut the curve into patches: a patch closes at max_patch elements, or
+    // before an element that would give one of its DOFs a 17th element in
+    // the patch (a DOF's rank within its run has 4 bits).
+    std::vector<Index> stamp(ndof, -1), cnt(ndof, 0);
+    p->patch_off.push_back(0);
     p->patch_elem.resize(E);
-    for (Index q = 0; q <= P; ++q) p->patch_off[q] = std::min(E, q * max_patch);
-    for (Index e = 0; e < E; ++e) p->patch_elem[e] = static_cast<Index>(key[e] & 0xffffffffu);
+    for (Index i = 0; i < E; ++i)
+    {
+      const Index e  = static_cast<Index>(key[i] & 0xffffffffu);
+      const Index q  = static_cast<Index>(p->patch_off.size()) - 1;
+      bool        cut = i - p->patch_off.back() == max_patch;
+      for (int k = 0; k < 15 && !cut; ++k)
+      {
+        const Index d = dofs15[15 * e + k];
+        cut           = stamp[d] == q && cnt[d] == FX_TH_PATCH_RUN_MAX;
+      }
+      if (cut) p->patch_off.push_back(i);
+      const Index qq = static_cast<Index>(p->patch_off.size()) - 1;
+      for (int k = 0; k < 15; ++k)
+      {
+        const Index d = dofs15[15 * e + k];
+        if (stamp[d] != qq) stamp[d] = qq, cnt[d] = 0;
+        ++cnt[d];
+      }
+      p->patch_elem[i] = e;
+    }
+    if (E > 0) p->patch_off.push_back(E);
+    const Index P = static_cast<Index>(p->patch_off.size()) - 1;
     for (Index q = 0; q < P; ++q)
       std::sort(p->patch_elem.begin() + p->patch_off[q], p->patch_elem.begin() + p->patch_off[q + 1]);
+    p->elem_vert.resize(3 * std::size_t(E));
+    for (Index i = 0; i < E; ++i)
+      for (int k = 0; k < 3; ++k) p->elem_vert[3 * std::size_t(i) + k] = vconn3[3 * p->patch_elem[i] + k];
 
     // patches touching each DOF, in patch order
     std::vector<Index> npatch(ndof, 0), last(ndof, -1);
@@ -803,9 +830,11 @@ int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const
         p->seam_off.push_back(p->seam_off.back() + npatch[d]);
       }
     p->num_parts = p->seam_off.back();
+    fx::require(p->num_parts < FX_TH_PART_FIRST, FX_ERR_OVERFLOW,
+                "fx_th_patches_create: too many seam partials");
 
-    // local DOFs and slots of each patch
-    p->elem_slot.assign(16 * std::size_t(E), 0);
+    // local DOFs and runs of each patch
+    p->elem_ldof.assign(16 * std::size_t(E), 0);
     p->ldof_off.push_back(0);
     std::vector<Index> local(ndof, -1), count, cursor, dofs;
     for (Index q = 0; q < P; ++q)
@@ -820,6 +849,8 @@ int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const
         }
       std::sort(dofs.begin(), dofs.end());
       const Index nl = static_cast<Index>(dofs.size());
+      fx::require(nl <= 15 * FX_TH_PATCH_MAX, FX_ERR_UNSUPPORTED,
+                  "fx_th_patches_create: patch has too many local DOFs");
       for (Index j = 0; j < nl; ++j) local[dofs[j]] = j;
       count.assign(nl, 0);
       for (Index i = i0; i < i1; ++i)
@@ -831,25 +862,34 @@ int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const
                         "fx_th_patches_create: an element lists one DOF twice");
           ++count[local[d]];
         }
-      cursor.assign(nl, 0);
       Index run = 0;
       for (Index j = 0; j < nl; ++j)
       {
-        cursor[j] = run;
         run += count[j];
         p->ldof_end.push_back(static_cast<std::uint16_t>(run));
         const Index d = dofs[j];
         p->ldof_dof.push_back(d);
-        p->ldof_part.push_back(npatch[d] == 1 ? Index(-1) : part_next[d]++);
+        if (npatch[d] == 1)
+          p->ldof_part.push_back(-1);
+        else
+        {
+          const bool first = part_next[d] == p->seam_off[std::lower_bound(p->seam_dof.begin(),
+                                                                          p->seam_dof.end(), d)
+                                                         - p->seam_dof.begin()];
+          p->ldof_part.push_back(part_next[d]++ | (first ? FX_TH_PART_FIRST : 0));
+        }
       }
+      cursor.assign(nl, 0); // rank of the next element in each run
       for (Index i = i0; i < i1; ++i)
         for (int k = 0; k < 15; ++k)
         {
-          const Index d                   = dofs15[15 * p->patch_elem[i] + k];
-          p->elem_slot[16 * std::size_t(i) + k] = static_cast<std::uint16_t>(cursor[local[d]]++);
+          const Index j = local[dofs15[15 * p->patch_elem[i] + k]];
+          p->elem_ldof[16 * std::size_t(i) + k] =
+              static_cast<std::uint16_t>(j | (cursor[j]++ << 12));
         }
       for (Index d : dofs) local[d] = -1;
       p->ldof_off.push_back(p->ldof_off.back() + nl);
+      p->max_patch_ldofs = std::max(p->max_patch_ldofs, nl);
     }
     *out = p.release();
   });
@@ -863,23 +903,26 @@ int fx_th_patches_info_get(const fx_th_patches* p, fx_th_patches_info* info)
     info->num_ldofs   = static_cast<Index>(p->ldof_dof.size());
     info->num_seam    = static_cast<Index>(p->seam_dof.size());
     info->num_parts   = p->num_parts;
+    info->max_patch_ldofs = p->max_patch_ldofs;
   });
 }
 
 int fx_th_patches_export(const fx_th_patches* p, Index* patch_off, Index* patch_elem,
-                         unsigned short* elem_slot, Index* ldof_off, Index* ldof_dof,
+                         Index* elem_vert, unsigned short* elem_ldof, Index* ldof_off,
+                         Index* ldof_dof,
                          Index* ldof_part, unsigned short* ldof_end, Index* seam_off,
                          Index* seam_dof)
 {
   return fx::guard([&] {
     fx::require(p && patch_off && ldof_off && seam_off
-                    && (p->num_elems == 0 || (patch_elem && elem_slot))
+                    && (p->num_elems == 0 || (patch_elem && elem_vert && elem_ldof))
                     && (p->ldof_dof.empty() || (ldof_dof && ldof_part && ldof_end))
                     && (p->seam_dof.empty() || seam_dof),
                 FX_ERR_ARG, "fx_th_patches_export: null array");
     std::copy(p->patch_off.begin(), p->patch_off.end(), patch_off);
     std::copy(p->patch_elem.begin(), p->patch_elem.end(), patch_elem);
-    std::copy(p->elem_slot.begin(), p->elem_slot.end(), elem_slot);
+    std::copy(p->elem_vert.begin(), p->elem_vert.end(), elem_vert);
+    std::copy(p->elem_ldof.begin(), p->elem_ldof.end(), elem_ldof);
     std::copy(p->ldof_off.begin(), p->ldof_off.end(), ldof_off);
     std::copy(p->ldof_dof.begin(), p->ldof_dof.end(), ldof_dof);
     std::copy(p->ldof_part.begin(), p->ldof_part.end(), ldof_part);

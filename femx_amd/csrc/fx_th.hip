@@ -42,10 +42,11 @@ __global__ __launch_bounds__(256) void k_vjp_gather(GatherArgs a)
 
 static_assert(kThPatch == FX_TH_PATCH_MAX, "one lane per element of a patch");
 
-__global__ __launch_bounds__(kThPatch) void k_th_vjp_patch(ThVjpArgs a, ThPatchArgs q)
+__global__ __launch_bounds__(kThPatch) __attribute__((amdgpu_waves_per_eu(4))) void k_th_vjp_patch(ThVjpArgs a, ThPatchArgs q,
+                                                                                           int max_patch_ldofs)
 {
-  __shared__ double cs[15 * kThPatch];
-  th_vjp_patch_block(fx::xcd_block(gridDim.x), a, q, kThTables, cs);
+  extern __shared__ __align__(16) unsigned char lds[];
+  th_vjp_patch_block(fx::xcd_block(gridDim.x), a, q, kThTables, lds, max_patch_ldofs);
 }
 
 __global__ __launch_bounds__(256) void k_th_vjp_seam(ThSeamArgs a)
@@ -175,7 +176,7 @@ int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* xv, const double*
 }
 
 int fx_th_vjp_patch(fx_index ndof, fx_index num_elems, const double* xv, const double* yv,
-                    const fx_index* vconn3, const fx_index* dofs15, const double* state, double nu,
+                    const double* state, double nu,
                     const double* nu_e, const double* lam, const fx_index* bc_mask,
                     const fx_th_patch_plan* plan, double* jt, double* g, void* stream)
 {
@@ -184,6 +185,7 @@ int fx_th_vjp_patch(fx_index ndof, fx_index num_elems, const double* xv, const d
                 "fx_th_vjp_patch: negative size or null plan");
     const fx_th_patch_plan& q = *plan;
     fx::require(q.num_patches >= 0 && q.num_seam >= 0 && q.num_parts >= 0
+                    && q.max_patch_ldofs >= 0 && q.max_patch_ldofs <= 15 * kThPatch
                     && (q.num_patches > 0) == (num_elems > 0),
                 FX_ERR_ARG, "fx_th_vjp_patch: plan sizes do not fit the mesh");
     if (ndof == 0) return;
@@ -191,8 +193,8 @@ int fx_th_vjp_patch(fx_index ndof, fx_index num_elems, const double* xv, const d
                 FX_ERR_OVERFLOW, "fx_th_vjp_patch: state, element or partial arrays beyond 4 GiB");
     fx::require(xv && yv && state && lam && jt && q.patch_off && q.ldof_off && q.seam_off
                     && (num_elems == 0
-                        || (vconn3 && dofs15 && q.patch_elem && q.elem_slot && q.ldof_dof
-                            && q.ldof_part && q.ldof_end))
+                        || (q.patch_elem && q.elem_vert && q.elem_ldof
+                            && q.ldof_dof && q.ldof_part && q.ldof_end))
                     && (q.num_seam == 0 || q.seam_dof) && (q.num_parts == 0 || q.parts),
                 FX_ERR_ARG, "fx_th_vjp_patch: null array");
     const std::size_t nd = std::size_t(ndof) * 8, ne = std::size_t(num_elems) * 8,
@@ -204,12 +206,13 @@ int fx_th_vjp_patch(fx_index ndof, fx_index num_elems, const double* xv, const d
                 FX_ERR_ALIAS, "fx_th_vjp_patch: outputs must not alias inputs or each other");
     if (q.num_patches > 0)
     {
-      const ThVjpArgs   a{num_elems, xv, yv, vconn3, dofs15, state, nu, nu_e,
+      const ThVjpArgs   a{num_elems, xv, yv, nullptr, nullptr, state, nu, nu_e,
                           lam,       bc_mask, nullptr, nullptr, g};
-      const ThPatchArgs pa{q.patch_off, q.patch_elem, q.elem_slot, q.ldof_off, q.ldof_dof,
-                           q.ldof_part, q.ldof_end,   jt,          q.parts};
+      const ThPatchArgs pa{q.patch_off, q.patch_elem, q.elem_vert, q.elem_ldof, q.ldof_off,
+                           q.ldof_dof,  q.ldof_part,  q.ldof_end,  jt,          q.parts};
+      const int         mx = static_cast<int>(q.max_patch_ldofs);
       hipLaunchKernelGGL(k_th_vjp_patch, dim3(static_cast<unsigned>(q.num_patches)),
-                         dim3(kThPatch), 0, fx::as_stream(stream), a, pa);
+                         dim3(kThPatch), th_patch_lds(mx), fx::as_stream(stream), a, pa, mx);
       fx::check_launch("fx_th_vjp_patch");
     }
     if (q.num_seam > 0)

@@ -120,13 +120,6 @@ __device__ __forceinline__ ThGeom th_geom(const double* __restrict__ xv,
   return {j11 * id, -j01 * id, -j10 * id, j00 * id, fabs(det)};
 }
 
-struct ThElem
-{
-  double s[15];
-  ThGeom g;
-  double nu;
-};
-
 // Physical P2 gradients at one quadrature point.
 __device__ __forceinline__ void th_grads(const ThTables& T, int q, const ThGeom& g, double* dNx,
                                          double* dNy)
@@ -257,6 +250,15 @@ struct ThAff
 {
   double h00, h01, h10, h11, adet;
 };
+
+// th_geom from the three vertices' coordinates.
+__device__ __forceinline__ ThGeom th_geom_xy(const double (&x)[3], const double (&y)[3])
+{
+  const double j00 = x[1] - x[0], j01 = x[2] - x[0], j10 = y[1] - y[0], j11 = y[2] - y[0];
+  const double det = j00 * j11 - j01 * j10;
+  const double id  = 1.0 / det;
+  return {j11 * id, -j01 * id, -j10 * id, j00 * id, fabs(det)};
+}
 
 struct ThElemT
 {
@@ -642,18 +644,89 @@ struct ThVjpArgs
 //   column (b,d): N_b (du_0/dx_d L_0 + du_1/dx_d L_1) + nu grad phi_b . grad L_d
 //                 + L_d (u . grad phi_b) - P d_d phi_b
 //   column p_b' : -psi_b' div L
+// Element data of the VJP: the velocity state, the boundary-masked adjoint
+// (velocity, then pressure), the affine geometry and the viscosity.
+struct ThVjpElem
+{
+  double s[12];
+  double l[15];
+  ThGeom g;
+  double nu;
+};
+
+__device__ __forceinline__ void th_vjp_math(const ThVjpElem& E, const ThTables& T, double (&c)[15],
+                                            double& ge)
+{
+  const double(&l)[15] = E.l;
+#pragma unroll
+  for (int k = 0; k < 15; ++k) c[k] = 0.0;
+  ge = 0.0;
+  const ThGeom& G  = E.g;
+  const double  nu = E.nu;
+  // At each point the columns share three coefficients per component d:
+  //   c_(b,d) += w (N_b Y_d + d_x phi_b X_xd + d_y phi_b X_yd),
+  //   Y_d  = du_0/dx_d L_0 + du_1/dx_d L_1            (reaction)
+  //   X_ed = nu dL_d/dx_e + L_d u_e - P delta_ed      (viscous, convective,
+  //                                                    pressure)
+  // and with d_e phi_b = sum_r dN_b/dr_r Jinv_re the last two terms are
+  // dN_b/dr R_d + dN_b/ds S_d, R_d and S_d the rows of Jinv X: the shape
+  // tables enter only as coefficients, and every gradient is formed from
+  // the reference gradients once per point.
+#pragma unroll 1
+  for (int q = 0; q < 7; ++q)
+  {
+    const double* N  = T.N[q];
+    const double* dN = T.dNdr[q];
+    double u0 = 0.0, u1 = 0.0, L0 = 0.0, L1 = 0.0;
+    double ur0 = 0.0, us0 = 0.0, ur1 = 0.0, us1 = 0.0, lr0 = 0.0, ls0 = 0.0, lr1 = 0.0, ls1 = 0.0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+    {
+      const double na = N[a], dr = dN[2 * a], ds = dN[2 * a + 1];
+      const double sx = E.s[2 * a], sy = E.s[2 * a + 1], lx = l[2 * a], ly = l[2 * a + 1];
+      u0 = fma(na, sx, u0), u1 = fma(na, sy, u1), L0 = fma(na, lx, L0), L1 = fma(na, ly, L1);
+      ur0 = fma(dr, sx, ur0), us0 = fma(ds, sx, us0), ur1 = fma(dr, sy, ur1), us1 = fma(ds, sy, us1);
+      lr0 = fma(dr, lx, lr0), ls0 = fma(ds, lx, ls0), lr1 = fma(dr, ly, lr1), ls1 = fma(ds, ly, ls1);
+    }
+    double P = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) P = fma(T.psi[q][a], l[12 + a], P);
+    // physical gradients: d_cx = du_c/dx, d_cy = du_c/dy; g likewise for L
+    const double d00 = fma(G.i00, ur0, G.i10 * us0), d01 = fma(G.i01, ur0, G.i11 * us0);
+    const double d10 = fma(G.i00, ur1, G.i10 * us1), d11 = fma(G.i01, ur1, G.i11 * us1);
+    const double g00 = fma(G.i00, lr0, G.i10 * ls0), g01 = fma(G.i01, lr0, G.i11 * ls0);
+    const double g10 = fma(G.i00, lr1, G.i10 * ls1), g11 = fma(G.i01, lr1, G.i11 * ls1);
+    const double wq  = G.adet * T.w[q];
+    const double Y0 = wq * fma(d00, L0, d10 * L1), Y1 = wq * fma(d01, L0, d11 * L1);
+    const double Xx0 = fma(nu, g00, fma(L0, u0, -P)), Xy0 = fma(nu, g01, L0 * u1);
+    const double Xx1 = fma(nu, g10, L1 * u0), Xy1 = fma(nu, g11, fma(L1, u1, -P));
+    const double R0 = wq * fma(G.i00, Xx0, G.i01 * Xy0), S0 = wq * fma(G.i10, Xx0, G.i11 * Xy0);
+    const double R1 = wq * fma(G.i00, Xx1, G.i01 * Xy1), S1 = wq * fma(G.i10, Xx1, G.i11 * Xy1);
+#pragma unroll
+    for (int b = 0; b < 6; ++b)
+    {
+      const double nb = N[b], dr = dN[2 * b], ds = dN[2 * b + 1];
+      c[2 * b]        = fma(nb, Y0, fma(dr, R0, fma(ds, S0, c[2 * b])));
+      c[2 * b + 1]    = fma(nb, Y1, fma(dr, R1, fma(ds, S1, c[2 * b + 1])));
+    }
+    const double dv = -wq * (g00 + g11); // -psi_b' div L
+#pragma unroll
+    for (int b = 0; b < 3; ++b) c[12 + b] = fma(T.psi[q][b], dv, c[12 + b]);
+    ge = fma(fma(g00, d00, g01 * d01) + fma(g10, d10, g11 * d11), wq, ge);
+  }
+}
+
 __device__ __forceinline__ void th_vjp_elem_cols(int e, const ThVjpArgs& a, const ThTables& T,
                                                  double (&c)[15], double& ge)
 {
   // the DOF row and vertices in vector loads, then one 32-bit-offset gather
   // per value (the pressure state does not enter J^T lam or dR/dnu)
   const ThDofs D = th_load_dofs(a.dofs, a.vconn, e);
-  ThElem       E;
+  ThVjpElem    E;
 #pragma unroll
   for (int k = 0; k < 12; ++k) E.s[k] = gat32(a.state, D.d[k]);
   E.g  = th_geom(a.xv, a.yv, D.v[0], D.v[1], D.v[2]);
   E.nu = a.nu_e ? a.nu_e[e] : a.nu;
-  double l[15];
 #pragma unroll
   for (int k = 0; k < 15; ++k)
   {
@@ -661,46 +734,9 @@ __device__ __forceinline__ void th_vjp_elem_cols(int e, const ThVjpArgs& a, cons
     // both gathers issued together (dk < ndof for both arrays), then a select
     const double lv = gat32(a.lam, dk);
     const int    mk = a.bc_mask ? gat32(a.bc_mask, dk) : 0;
-    l[k]            = mk ? 0.0 : lv;
-    c[k]            = 0.0;
+    E.l[k]          = mk ? 0.0 : lv;
   }
-  ge = 0.0;
-#pragma unroll 1
-  for (int q = 0; q < 7; ++q)
-  {
-    double dNx[6], dNy[6];
-    th_grads(T, q, E.g, dNx, dNy);
-    const double wq = E.g.adet * T.w[q];
-    double u0 = 0.0, u1 = 0.0, d00 = 0.0, d01 = 0.0, d10 = 0.0, d11 = 0.0;
-    double L0 = 0.0, L1 = 0.0, g00 = 0.0, g01 = 0.0, g10 = 0.0, g11 = 0.0, P = 0.0;
-#pragma unroll
-    for (int a = 0; a < 6; ++a)
-    {
-      const double na = T.N[q][a], sx = E.s[2 * a], sy = E.s[2 * a + 1];
-      const double lx = l[2 * a], ly = l[2 * a + 1];
-      u0 = fma(na, sx, u0), u1 = fma(na, sy, u1);
-      d00 = fma(dNx[a], sx, d00), d01 = fma(dNy[a], sx, d01);
-      d10 = fma(dNx[a], sy, d10), d11 = fma(dNy[a], sy, d11);
-      L0 = fma(na, lx, L0), L1 = fma(na, ly, L1);
-      g00 = fma(dNx[a], lx, g00), g01 = fma(dNy[a], lx, g01);
-      g10 = fma(dNx[a], ly, g10), g11 = fma(dNy[a], ly, g11);
-    }
-#pragma unroll
-    for (int a = 0; a < 3; ++a) P = fma(T.psi[q][a], l[12 + a], P);
-#pragma unroll
-    for (int b = 0; b < 6; ++b)
-    {
-      const double nb = T.N[q][b], bx = dNx[b], by = dNy[b];
-      const double ub = fma(u0, bx, u1 * by);
-      const double c0 = fma(nb, fma(d00, L0, d10 * L1), fma(E.nu, fma(bx, g00, by * g01), fma(L0, ub, -P * bx)));
-      const double c1 = fma(nb, fma(d01, L0, d11 * L1), fma(E.nu, fma(bx, g10, by * g11), fma(L1, ub, -P * by)));
-      c[2 * b]        = fma(c0, wq, c[2 * b]);
-      c[2 * b + 1]    = fma(c1, wq, c[2 * b + 1]);
-    }
-#pragma unroll
-    for (int b = 0; b < 3; ++b) c[12 + b] = fma(-T.psi[q][b] * (g00 + g11), wq, c[12 + b]);
-    ge = fma(fma(g00, d00, g01 * d01) + fma(g10, d10, g11 * d11), wq, ge);
-  }
+  th_vjp_math(E, T, c, ge);
 }
 
 __device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, const ThTables& T)
@@ -719,10 +755,11 @@ __device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, c
 
 // ---------------------------------------------------------------------------
 // Single-pass VJP over element patches (fx_th_patches_create).  A workgroup
-// of kThPatch lanes takes one patch, one element per lane: it forms the
-// element's columns as above and stores each into its slot of the patch's
-// LDS slot array, where every local DOF owns a run of slots in element
-// order.  After a barrier the lanes sum the runs, each in slot order; a DOF
+// of kThPatch lanes takes one patch: it loads the patch's local DOF values
+// (state, masked adjoint) into LDS, one gather per local DOF, then, one
+// element per lane, forms the element's columns as above from them and
+// stores each into its slot of the patch's LDS slot array, where every local
+// DOF owns a run of slots in element order.  After a barrier the lanes sum the runs, each in slot order; a DOF
 // held by this patch alone is finished here (jt, with lam passed through on
 // constrained DOFs), any other gets this patch's partial in its seam slot.
 constexpr int kThPatch = 256;
@@ -731,7 +768,8 @@ struct ThPatchArgs
 {
   const int*            patch_off;
   const int*            patch_elem;
-  const unsigned short* elem_slot; // 16 per element in patch order
+  const int*            elem_vert; // 3 per element in patch order
+  const unsigned short* elem_ldof; // 16 per element in patch order: local DOF | run rank << 12
   const int*            ldof_off;
   const int*            ldof_dof;
   const int*            ldof_part;
@@ -740,42 +778,148 @@ struct ThPatchArgs
   double*               parts;
 };
 
-__device__ __forceinline__ void th_vjp_patch_block(int p, const ThVjpArgs& a, const ThPatchArgs& q,
-                                                   const ThTables& T, double* cs)
+// LDS of one patch workgroup, in bytes: the staged local DOF values (later
+// overwritten by the slot array), each run's start (u16, one past the last
+// local DOF too) and each local DOF's constraint flag (u8).
+__host__ __device__ constexpr int th_patch_lds_main(int max_patch_ldofs)
 {
-  const int t  = threadIdx.x;
-  const int i0 = q.patch_off[p], ne = q.patch_off[p + 1] - i0;
-  if (t < ne)
+  return 8 * (2 * max_patch_ldofs > 15 * kThPatch ? 2 * max_patch_ldofs : 15 * kThPatch);
+}
+__host__ __device__ constexpr int th_patch_lds(int max_patch_ldofs)
+{
+  return (th_patch_lds_main(max_patch_ldofs) + 3 * max_patch_ldofs + 2 + 15) / 16 * 16;
+}
+
+// Local DOFs a lane handles per round of the staging and summing loops: their
+// loads are all issued before the first is used.
+constexpr int kThPatchUnroll = 8;
+
+__device__ __forceinline__ void th_vjp_patch_block(int p, const ThVjpArgs& a, const ThPatchArgs& q,
+                                                   const ThTables& T, unsigned char* lds,
+                                                   int max_patch_ldofs)
+{
+  typedef unsigned u4 __attribute__((ext_vector_type(4)));
+  typedef int      i3 __attribute__((ext_vector_type(3), aligned(4)));
+  constexpr int    U = kThPatchUnroll;
+  const int        t  = threadIdx.x;
+  const int        i0 = q.patch_off[p], ne = q.patch_off[p + 1] - i0;
+  const int        l0 = q.ldof_off[p], nl = q.ldof_off[p + 1] - l0;
+  const bool       mine = t < ne;
+  const int        i    = i0 + t;
+  double*          sv   = reinterpret_cast<double*>(lds); // state, then the slot array
+  double*          lv   = sv + nl;                        // masked adjoint
+  unsigned short*  rs   = reinterpret_cast<unsigned short*>(lds + th_patch_lds_main(max_patch_ldofs));
+  unsigned char*   mk   = reinterpret_cast<unsigned char*>(rs + max_patch_ldofs + 1);
+  const u4*        lp   = reinterpret_cast<const u4*>(q.elem_ldof + 16 * static_cast<long>(i));
+  // the element's tables and geometry are requested first, so that their
+  // latency overlaps the staging below
+  int      e = 0;
+  unsigned w[8];
+  double   xg[3], yg[3], nu = a.nu;
+  if (mine)
   {
-    const int i = i0 + t;
-    const int e = q.patch_elem[i];
-    double    c[15], ge;
-    th_vjp_elem_cols(e, a, T, c, ge);
-    typedef unsigned u4 __attribute__((ext_vector_type(4)));
-    const u4*      sp   = reinterpret_cast<const u4*>(q.elem_slot + 16 * static_cast<long>(i));
-    const u4       s0   = __builtin_nontemporal_load(sp), s1 = __builtin_nontemporal_load(sp + 1);
-    const unsigned w[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
+    e           = q.patch_elem[i];
+    const u4 x0 = __builtin_nontemporal_load(lp), x1 = __builtin_nontemporal_load(lp + 1);
+    w[0] = x0.x, w[1] = x0.y, w[2] = x0.z, w[3] = x0.w, w[4] = x1.x, w[5] = x1.y, w[6] = x1.z,
+    w[7]       = x1.w;
+    const i3 v = *reinterpret_cast<const i3*>(q.elem_vert + 3 * static_cast<long>(i));
+    xg[0] = gat32(a.xv, v.x), xg[1] = gat32(a.xv, v.y), xg[2] = gat32(a.xv, v.z);
+    yg[0] = gat32(a.yv, v.x), yg[1] = gat32(a.yv, v.y), yg[2] = gat32(a.yv, v.z);
+    if (a.nu_e) nu = a.nu_e[e];
+  }
+  // the patch's local DOF values, each gathered once (state, masked
+  // adjoint), their constraint flags and the runs' starts
+  if (t == 0) rs[0] = 0;
+  for (int j0 = t; j0 < nl; j0 += U * kThPatch)
+  {
+    int d[U], re[U];
 #pragma unroll
-    for (int k = 0; k < 15; ++k) cs[(w[k >> 1] >> (16 * (k & 1))) & 0xffffu] = c[k];
+    for (int u = 0; u < U; ++u)
+    {
+      const int j = l0 + min(j0 + u * kThPatch, nl - 1);
+      d[u]        = q.ldof_dof[j];
+      re[u]       = q.ldof_end[j];
+    }
+    double sd[U], ld[U];
+    int    m[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      sd[u] = gat32(a.state, d[u]);
+      ld[u] = gat32(a.lam, d[u]);
+      m[u]  = a.bc_mask ? gat32(a.bc_mask, d[u]) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j0 + u * kThPatch < nl)
+      {
+        const int j = j0 + u * kThPatch;
+        sv[j]       = sd[u];
+        lv[j]       = m[u] ? 0.0 : ld[u];
+        rs[j + 1]   = static_cast<unsigned short>(re[u]);
+        mk[j]       = m[u] ? 1 : 0;
+      }
+  }
+  __syncthreads();
+  double c[15], ge = 0.0;
+  if (mine)
+  {
+    ThVjpElem E;
+#pragma unroll
+    for (int k = 0; k < 15; ++k)
+    {
+      const unsigned j = (w[k >> 1] >> (16 * (k & 1))) & 0xfffu;
+      if (k < 12) E.s[k] = sv[j];
+      E.l[k] = lv[j];
+    }
+    E.g  = th_geom_xy(xg, yg);
+    E.nu = nu;
+    th_vjp_math(E, T, c, ge);
+  }
+  __syncthreads(); // every lane has read the staged values: the slots overwrite them
+  double* cs = sv;
+  if (mine)
+  {
+    const u4       x0   = __builtin_nontemporal_load(lp), x1 = __builtin_nontemporal_load(lp + 1);
+    const unsigned o[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+    for (int k = 0; k < 15; ++k)
+    {
+      const unsigned h = (o[k >> 1] >> (16 * (k & 1))) & 0xffffu;
+      cs[rs[h & 0xfffu] + (h >> 12)] = c[k];
+    }
     if (a.gout) __builtin_nontemporal_store(ge, a.gout + e);
   }
   __syncthreads();
-  const int l0 = q.ldof_off[p], nl = q.ldof_off[p + 1] - l0;
-  for (int j = t; j < nl; j += kThPatch)
+  for (int j0 = t; j0 < nl; j0 += U * kThPatch)
   {
-    const int beg = j == 0 ? 0 : q.ldof_end[l0 + j - 1], end = q.ldof_end[l0 + j];
-    double    v   = 0.0;
-    for (int s = beg; s < end; ++s) v = v + cs[s];
-    const int d = q.ldof_dof[l0 + j], part = q.ldof_part[l0 + j];
-    if (part < 0)
+    int d[U], part[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
     {
-      if (a.bc_mask && a.bc_mask[d]) v = v + a.lam[d];
-      __builtin_nontemporal_store(v, q.jt + d);
+      const int j = l0 + min(j0 + u * kThPatch, nl - 1);
+      d[u]        = q.ldof_dof[j];
+      part[u]     = q.ldof_part[j];
     }
-    else
-      __builtin_nontemporal_store(v, q.parts + part);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const int j = j0 + u * kThPatch;
+      if (j >= nl) continue;
+      double    v = 0.0;
+      const int r1 = rs[j + 1];
+      for (int s = rs[j]; s < r1; ++s) v = v + cs[s];
+      // lam passes through on a constrained DOF: added by the patch that
+      // finishes it, or into the partial of its first patch
+      const bool pass = mk[j] && (part[u] < 0 || (part[u] & FX_TH_PART_FIRST));
+      if (pass) v = v + a.lam[d[u]];
+      if (part[u] < 0)
+        __builtin_nontemporal_store(v, q.jt + d[u]);
+      else
+        __builtin_nontemporal_store(v, q.parts + (part[u] & ~FX_TH_PART_FIRST));
+    }
   }
-  __syncthreads(); // the slot array is reused by the next patch of a persistent walk
+  __syncthreads(); // the LDS is reused by the next patch of a persistent walk
 }
 
 struct ThSeamArgs
@@ -790,16 +934,23 @@ struct ThSeamArgs
 };
 
 // Seam pass (one lane per seam DOF): the DOF's per-patch partials in patch
-// order, then lam passed through on a constrained DOF.
+// order (lam's pass-through is in the first), the first four loaded at once;
+// a DOF in no element gets lam's pass-through alone.
 __device__ __forceinline__ void th_vjp_seam_block(int blk, const ThSeamArgs& a)
 {
   const int s = blk * 256 + threadIdx.x;
   if (s >= a.num_seam) return;
-  double    v  = 0.0;
-  const int t1 = a.seam_off[s + 1];
-  for (int t = a.seam_off[s]; t < t1; ++t) v = v + a.parts[t];
-  const int d = a.seam_dof[s];
-  if (a.bc_mask && a.bc_mask[d]) v = v + a.lam[d];
+  const int t0 = a.seam_off[s], t1 = a.seam_off[s + 1], d = a.seam_dof[s];
+  const int n  = t1 - t0;
+  double    p[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) p[u] = n > 0 ? __builtin_nontemporal_load(a.parts + t0 + min(u, n - 1)) : 0.0;
+  double v = 0.0;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if (u < n) v = v + p[u];
+  for (int t = t0 + 4; t < t1; ++t) v = v + a.parts[t];
+  if (n == 0 && a.bc_mask && a.bc_mask[d]) v = a.lam[d];
   __builtin_nontemporal_store(v, a.jt + d);
 }
 

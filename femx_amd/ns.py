@@ -181,28 +181,29 @@ class THPatches:
             P, L, S = info.num_patches, info.num_ldofs, info.num_seam
             E = geom.num_elems
             a = {"patch_off": np.empty(P + 1, np.int32), "patch_elem": np.empty(max(E, 1), np.int32),
-                 "elem_slot": np.empty(max(16 * E, 1), np.uint16),
+                 "elem_vert": np.empty(max(3 * E, 1), np.int32),
+                 "elem_ldof": np.empty(max(16 * E, 1), np.uint16),
                  "ldof_off": np.empty(P + 1, np.int32), "ldof_dof": np.empty(max(L, 1), np.int32),
                  "ldof_part": np.empty(max(L, 1), np.int32),
                  "ldof_end": np.empty(max(L, 1), np.uint16),
                  "seam_off": np.empty(S + 1, np.int32), "seam_dof": np.empty(max(S, 1), np.int32)}
             _lib.call("fx_th_patches_export", h, *[ptr(a[k]) for k in (
-                "patch_off", "patch_elem", "elem_slot", "ldof_off", "ldof_dof", "ldof_part",
-                "ldof_end", "seam_off", "seam_dof")])
+                "patch_off", "patch_elem", "elem_vert", "elem_ldof", "ldof_off", "ldof_dof",
+                "ldof_part", "ldof_end", "seam_off", "seam_dof")])
         finally:
             _lib.lib().fx_th_patches_destroy(h)
         self.h = a
         self.num_patches, self.num_ldofs, self.num_seam = P, L, S
-        self.num_parts = info.num_parts
+        self.num_parts, self.max_patch_ldofs = info.num_parts, info.max_patch_ldofs
         dev = torch.device(device)
         # uint16 tables travel as int16 bit patterns (torch has no uint16 copy path)
         self.d = {k: torch.from_numpy(v.view(np.int16) if v.dtype == np.uint16 else v).to(dev)
                   for k, v in a.items()}
         self.parts = torch.empty(max(self.num_parts, 1), dtype=torch.float64, device=dev)
         d = self.d
-        self.plan = _lib.PatchPlan(P, S, self.num_parts, *[ptr(d[k]) for k in (
-            "patch_off", "patch_elem", "elem_slot", "ldof_off", "ldof_dof", "ldof_part",
-            "ldof_end", "seam_off", "seam_dof")], ptr(self.parts))
+        self.plan = _lib.PatchPlan(P, S, self.num_parts, self.max_patch_ldofs, *[ptr(d[k]) for k in (
+            "patch_off", "patch_elem", "elem_vert", "elem_ldof", "ldof_off", "ldof_dof",
+            "ldof_part", "ldof_end", "seam_off", "seam_dof")], ptr(self.parts))
 
 
 def _patches(geom, amap, device):
@@ -282,7 +283,7 @@ def ns_vjp(op: TaylorHoodNS, geom: THGeometry, amap: AssemblyMap, state, lam, bc
         g_out = torch.empty(geom.num_elems, dtype=torch.float64, device=dev)
     pp = _patches(geom, amap, dev)
     _lib.call("fx_th_vjp_patch", geom.ndof, geom.num_elems, ptr(geom.xv), ptr(geom.yv),
-              ptr(geom.vconn3), ptr(geom.dofs15), ptr(state), op.nu, ptr(op.nu_e), ptr(lam),
+              ptr(state), op.nu, ptr(op.nu_e), ptr(lam),
               ptr(bc_mask), C.byref(pp.plan), ptr(jt_out), ptr(g_out), stream_handle(stream))
     return jt_out, g_out
 

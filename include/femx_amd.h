@@ -383,30 +383,39 @@ int fx_th_vjp_noalloc(fx_index ndof, fx_index num_elems, const double* d_xv, con
 
 /* Patch plan of fx_th_vjp_patch (host).  Elements are ordered along a Morton
  * curve of their centroids and cut into patches of at most max_patch
- * (<= FX_TH_PATCH_MAX) consecutive elements, ascending element id inside a
- * patch.  A patch's local DOFs (ascending id) each own a run of slots, one
- * per element of the patch holding the DOF, in element order; elem_slot[16 i
- * + k] is the slot of local DOF k of the i-th element in patch order (entry
- * 15 unused).  A DOF held by one patch only is finished by it; the others
- * (on seams between patches, or in no element) are the seam DOFs, summed
- * from per-patch partials (ldof_part, -1 for a finished DOF) in patch order.
- * Array sizes: patch_off num_patches+1, patch_elem num_elems, elem_slot
- * 16*num_elems, ldof_off num_patches+1, ldof_dof/ldof_part/ldof_end
- * num_ldofs (ldof_end: each local DOF's run end within its patch),
- * seam_off num_seam+1, seam_dof num_seam. */
+ * (<= FX_TH_PATCH_MAX) consecutive elements -- a patch also closes before an
+ * element that would give one of its DOFs more than FX_TH_PATCH_RUN_MAX
+ * elements in it -- ascending element id inside a patch.  A patch's local
+ * DOFs (ascending id) each own a run of slots, one per element of the patch
+ * holding the DOF, in element order (ldof_end: each run's end within the
+ * patch).  elem_ldof[16 i + k] = j | r << 12 for local DOF k of the i-th
+ * element in patch order: j its index among the patch's local DOFs, r the
+ * element's rank in j's run (entry 15 unused); elem_vert[3 i + k] is vertex
+ * k of that element.  A DOF held by one patch only is finished by it
+ * (ldof_part -1); the others (on seams between patches, or in no element)
+ * are the seam DOFs, summed from per-patch partials in patch order:
+ * ldof_part is the partial's index, with FX_TH_PART_FIRST set on the DOF's
+ * first patch (which adds lam's pass-through on a constrained DOF).
+ * Array sizes: patch_off num_patches+1, patch_elem num_elems, elem_vert
+ * 3*num_elems, elem_ldof 16*num_elems, ldof_off num_patches+1,
+ * ldof_dof/ldof_part/ldof_end num_ldofs, seam_off num_seam+1, seam_dof
+ * num_seam. */
 #define FX_TH_PATCH_MAX 256
+#define FX_TH_PATCH_RUN_MAX 16
+#define FX_TH_PART_FIRST (1 << 30)
 typedef struct fx_th_patches fx_th_patches;
 typedef struct
 {
   fx_index num_patches, num_ldofs, num_seam, num_parts;
+  fx_index max_patch_ldofs; /* the most local DOFs of one patch */
 } fx_th_patches_info;
 int fx_th_patches_create(fx_index ndof, fx_index num_elems, const fx_index* h_dofs15,
                          const fx_index* h_vconn3, fx_index num_vertices, const double* h_xv,
                          const double* h_yv, fx_index max_patch, fx_th_patches** out);
 int fx_th_patches_info_get(const fx_th_patches* patches, fx_th_patches_info* info);
 int fx_th_patches_export(const fx_th_patches* patches, fx_index* h_patch_off,
-                         fx_index* h_patch_elem, unsigned short* h_elem_slot,
-                         fx_index* h_ldof_off, fx_index* h_ldof_dof, fx_index* h_ldof_part,
+                         fx_index* h_patch_elem, fx_index* h_elem_vert,
+                         unsigned short* h_elem_ldof, fx_index* h_ldof_off, fx_index* h_ldof_dof, fx_index* h_ldof_part,
                          unsigned short* h_ldof_end, fx_index* h_seam_off, fx_index* h_seam_dof);
 void fx_th_patches_destroy(fx_th_patches* patches);
 
@@ -414,10 +423,11 @@ void fx_th_patches_destroy(fx_th_patches* patches);
  * (num_parts doubles). */
 typedef struct
 {
-  fx_index              num_patches, num_seam, num_parts;
+  fx_index              num_patches, num_seam, num_parts, max_patch_ldofs;
   const fx_index*       patch_off;
   const fx_index*       patch_elem;
-  const unsigned short* elem_slot;
+  const fx_index*       elem_vert;
+  const unsigned short* elem_ldof;
   const fx_index*       ldof_off;
   const fx_index*       ldof_dof;
   const fx_index*       ldof_part;
@@ -427,15 +437,17 @@ typedef struct
   double*               parts;
 } fx_th_patch_plan;
 
-/* fx_th_vjp in one pass over element patches: a workgroup forms its
- * patch's element columns (as fx_th_vjp's first phase), sums each local
+/* fx_th_vjp in one pass over element patches: a workgroup loads its
+ * patch's local DOF values (state, masked adjoint) into LDS once, forms the
+ * element columns (as fx_th_vjp's first phase) from them, sums each local
  * DOF's run in LDS in element order and writes jt for the DOFs it finishes;
  * a second launch sums the seam DOFs' partials in patch order.  Only the
- * seam partials go through device memory.  Results are deterministic; they
- * differ from fx_th_vjp's only in the order of the per-DOF sums. */
+ * seam partials go through device memory; the element DOFs and vertices
+ * come from the plan (elem_ldof, ldof_dof, elem_vert).  Results are deterministic; they differ from
+ * fx_th_vjp's only in the order of the per-DOF sums. */
 int fx_th_vjp_patch(fx_index ndof, fx_index num_elems, const double* d_xv, const double* d_yv,
-                    const fx_index* d_vconn3, const fx_index* d_dofs15, const double* d_state,
-                    double nu, const double* d_nu_e, const double* d_lam,
+                    const double* d_state, double nu,
+                    const double* d_nu_e, const double* d_lam,
                     const fx_index* d_bc_mask, const fx_th_patch_plan* h_plan, double* d_jt,
                     double* d_g, void* stream);
 

@@ -48,8 +48,7 @@ def two_phase(g, s, nu_e, lam, mask):
 
 def patch_vjp(g, s, nu_e, lam, mask, pp):
     jt, ge = empties(g)
-    _lib.call("fx_th_vjp_patch", g.ndof, g.num_elems, ptr(g.xv), ptr(g.yv), ptr(g.vconn3),
-              ptr(g.dofs15), ptr(s), 0.0, ptr(nu_e), ptr(lam), ptr(mask), C.byref(pp.plan),
+    _lib.call("fx_th_vjp_patch", g.ndof, g.num_elems, ptr(g.xv), ptr(g.yv), ptr(s), 0.0, ptr(nu_e), ptr(lam), ptr(mask), C.byref(pp.plan),
               ptr(jt), ptr(ge), None)
     return host(jt), host(ge)
 
@@ -112,3 +111,19 @@ def test_patch_vjp_full_size_cavity():
     ojt, og = O.th_vjp(g.h_xv, g.h_yv, g.h_vconn3.reshape(-1), g.h_dofs15.reshape(-1), g.ndof,
                        0.0, s, lam, hmask, nu_e=nu_e)
     assert close(jt, ojt) and close(ge, og)
+
+
+def test_patch_vjp_fan_hub_cut_into_patches():
+    """24 elements around one hub vertex (runs capped at 16 per patch): the
+    hub's DOFs are seam DOFs over two or more patches, two of them
+    constrained.  (40 elements would give the hub a CSR row of more than 255
+    entries, beyond the two-phase VJP's row tables; the patch VJP has no row
+    limit.)"""
+    m = 24
+    ang = 2 * np.pi * np.arange(m) / m
+    x = np.concatenate([[0.0], np.cos(ang)])
+    y = np.concatenate([[0.0], np.sin(ang)])
+    c3 = np.array([[0, 1 + i, 1 + (i + 1) % m] for i in range(m)], np.int32)
+    g = ns.th_from_p1(x, y, c3, DEV)
+    hub = g.h_dofs15[0, [0, 12]]  # velocity x and pressure of vertex 0
+    check(g, hub, (256, 8), 4)

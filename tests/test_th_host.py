@@ -160,7 +160,7 @@ def test_lid_and_walls_found_when_the_last_coordinate_misses_one(n):
 def _check_patches(g, max_patch, rng):
     """Invariants of the VJP patch plan, and the reduction it describes
     applied to integer-valued element columns (exact in any order) against a
-    plain scatter-add."""
+    plain scatter-add, with lam's pass-through on constrained DOFs."""
     pp = ns.THPatches(g, "cpu", max_patch)
     h = pp.h
     E, P = g.num_elems, pp.num_patches
@@ -170,53 +170,69 @@ def _check_patches(g, max_patch, rng):
     assert np.array_equal(np.sort(pe), np.arange(E))
     lo, ld, lp, le = h["ldof_off"], h["ldof_dof"], h["ldof_part"], h["ldof_end"].astype(np.int64)
     so, sd = h["seam_off"], h["seam_dof"][:pp.num_seam]
-    slot = h["elem_slot"].reshape(-1, 16)[:E, :15].astype(np.int64)
+    packed = h["elem_ldof"].reshape(-1, 16)[:E, :15].astype(np.int64)
+    jloc, rank = packed & 0xfff, packed >> 12
+    FIRST = 1 << 30
     npatch = np.zeros(g.ndof, np.int64)
     for p in range(P):
         npatch[np.unique(d15[pe[po[p]:po[p + 1]]])] += 1
     seam = np.flatnonzero(npatch != 1)
     assert np.array_equal(sd, seam) and np.array_equal(np.diff(so), npatch[seam])
     c = rng.integers(-1000, 1000, size=(E, 15)).astype(np.float64)
+    lam = rng.integers(-1000, 1000, size=g.ndof).astype(np.float64)
+    mask = rng.random(g.ndof) < 0.2
     parts = np.full(pp.num_parts, np.nan)
     jt = np.full(g.ndof, np.nan)
+    firsts = 0
     for p in range(P):
         els = pe[po[p]:po[p + 1]]
         assert np.all(np.diff(els) > 0)
+        assert np.array_equal(h["elem_vert"].reshape(-1, 3)[po[p]:po[p + 1]], g.h_vconn3[els])
         dofs = ld[lo[p]:lo[p + 1]]
-        assert np.all(np.diff(dofs) > 0)
+        assert np.all(np.diff(dofs) > 0) and dofs.size <= pp.max_patch_ldofs
         ends = le[lo[p]:lo[p + 1]]
         begs = np.concatenate([[0], ends[:-1]])
-        owner = np.repeat(np.arange(dofs.size), ends - begs)
-        s = slot[po[p]:po[p + 1]]
-        assert np.array_equal(np.sort(s.reshape(-1)), np.arange(ends[-1]))
-        assert np.array_equal(dofs[owner[s]], d15[els])
-        # within a run, slots follow element order
-        el_of = np.empty(ends[-1], np.int64)
-        el_of[s.reshape(-1)] = np.repeat(np.arange(els.size), 15)
+        jl, rk = jloc[po[p]:po[p + 1]], rank[po[p]:po[p + 1]]
+        assert np.array_equal(dofs[jl], d15[els])
+        assert np.all(ends - begs <= 16)
+        # ranks number each run's elements 0, 1, ... in element order
         for j in range(dofs.size):
-            assert np.all(np.diff(el_of[begs[j]:ends[j]]) > 0)
+            r = np.sort(rk[jl == j])
+            assert np.array_equal(r, np.arange(ends[j] - begs[j]))
+            assert np.all(np.diff(rk[jl == j]) > 0)
+        slot = begs[jl] + rk
+        assert np.array_equal(np.sort(slot.reshape(-1)), np.arange(ends[-1]))
         cs = np.empty(ends[-1])
-        cs[s.reshape(-1)] = c[els].reshape(-1)
+        cs[slot.reshape(-1)] = c[els].reshape(-1)
         for j in range(dofs.size):
             v = cs[begs[j]:ends[j]].sum()
-            if lp[lo[p] + j] < 0:
-                assert npatch[dofs[j]] == 1
-                jt[dofs[j]] = v
+            d, part = dofs[j], lp[lo[p] + j]
+            if part < 0:
+                assert npatch[d] == 1
+                jt[d] = v + (lam[d] if mask[d] else 0.0)
             else:
-                assert np.isnan(parts[lp[lo[p] + j]])
-                parts[lp[lo[p] + j]] = v
+                first = bool(part & FIRST)
+                firsts += first
+                part &= ~FIRST
+                assert first == (part == so[np.searchsorted(sd, d)])
+                assert np.isnan(parts[part])
+                parts[part] = v + (lam[d] if first and mask[d] else 0.0)
+    assert firsts == np.count_nonzero(npatch[seam] > 0)
+    # a DOF in no element has no partial: the seam pass gives it lam's
+    # pass-through alone
+    empty = seam[npatch[seam] == 0]
     assert not np.isnan(parts).any()
     for k, dof in enumerate(sd):
-        jt[dof] = parts[so[k]:so[k + 1]].sum()
-    ref = np.zeros(g.ndof)
+        jt[dof] = parts[so[k]:so[k + 1]].sum() if so[k + 1] > so[k] else (lam[dof] if mask[dof] else 0.0)
+    ref = np.where(mask, lam, 0.0)
     np.add.at(ref, d15.reshape(-1), c.reshape(-1))
     assert np.array_equal(jt, ref)
-    return pp
+    return pp, empty
 
 
 @pytest.mark.parametrize("n,max_patch", [(1, 256), (4, 7), (12, 64), (30, 256)])
 def test_vjp_patch_plan_structured(n, max_patch):
-    pp = _check_patches(ns.structured_th(n, "cpu"), max_patch, np.random.default_rng(n))
+    pp, _ = _check_patches(ns.structured_th(n, "cpu"), max_patch, np.random.default_rng(n))
     if n == 30:
         # Morton patches are compact: most DOFs are finished inside a patch
         assert pp.num_seam < 0.35 * ns.structured_th(n, "cpu").ndof
@@ -232,8 +248,9 @@ def test_vjp_patch_plan_scrambled_and_unstructured():
     xs = np.append(x, 2.0)
     ys = np.append(y, 2.0)
     g2 = ns.th_from_p1(xs, ys, c3, "cpu")
-    pp = _check_patches(g2, 256, rng)
-    assert g2.ndof - 1 in pp.h["seam_dof"][:pp.num_seam]
+    pp, empty = _check_patches(g2, 256, rng)
+    unused = np.setdiff1d(np.arange(g2.ndof), g2.h_dofs15)
+    assert unused.size == 3 and g2.ndof - 1 in unused and np.array_equal(empty, unused)
 
 
 def test_vjp_patch_plan_rejects_bad_sizes():
@@ -241,3 +258,16 @@ def test_vjp_patch_plan_rejects_bad_sizes():
     for mp in (0, 257):
         with pytest.raises(RuntimeError, match="max_patch"):
             ns.THPatches(g, "cpu", mp)
+
+
+def test_vjp_patch_plan_cuts_runs_at_sixteen_elements():
+    """A hub vertex shared by 40 elements: its DOFs' runs are capped at 16
+    elements per patch, so the fan is cut into at least three patches."""
+    m = 40
+    ang = 2 * np.pi * np.arange(m) / m
+    x = np.concatenate([[0.0], np.cos(ang)])
+    y = np.concatenate([[0.0], np.sin(ang)])
+    c3 = np.array([[0, 1 + i, 1 + (i + 1) % m] for i in range(m)], np.int32)
+    g = ns.th_from_p1(x, y, c3, "cpu")
+    pp, empty = _check_patches(g, 256, np.random.default_rng(1))
+    assert pp.num_patches >= 3 and empty.size == 0
