@@ -55,6 +55,7 @@ _SIGS = {
     "fx_p1_assemble": ([_i32, _i32, _i32, _i32] + [_vp] * 4 + [C.c_int] + [_vp] * 7 + [_vp],
                        C.c_int),
     "fx_p1_vjp": ([_i32, _i32] + [_vp] * 11 + [_vp], C.c_int),
+    "fx_p1_vjp_patch": ([_i32, _i32] + [_vp] * 9 + [_vp], C.c_int),
     "fx_th_assemble": ([_i32] * 4 + [_vp] * 5 + [_f64] + [_vp] * 10 + [_vp], C.c_int),
     "fx_csr_matvec_multi": ([_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32, _f64,
                              _f64, _vp], C.c_int),
@@ -68,11 +69,11 @@ _SIGS = {
                         C.c_int),
     "fx_th_vjp": ([_i32] * 2 + [_vp] * 5 + [_f64] + [_vp] * 8 + [_vp], C.c_int),
     "fx_th_vjp_noalloc": ([_i32] * 2 + [_vp] * 5 + [_f64] + [_vp] * 7 + [_vp], C.c_int),
-    "fx_th_patches_create": ([_i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, C.POINTER(_vp)],
-                             C.c_int),
-    "fx_th_patches_info_get": ([_vp, _vp], C.c_int),
-    "fx_th_patches_export": ([_vp] * 11, C.c_int),
-    "fx_th_patches_destroy": ([_vp], None),
+    "fx_patches_create": ([_i32, _i32, _i32, _vp, _vp, _i32, _vp, _vp, _i32, C.POINTER(_vp)],
+                          C.c_int),
+    "fx_patches_info_get": ([_vp, _vp], C.c_int),
+    "fx_patches_export": ([_vp] * 11, C.c_int),
+    "fx_patches_destroy": ([_vp], None),
     "fx_th_vjp_patch": ([_i32] * 2 + [_vp] * 3 + [_f64] + [_vp] * 4 + [_vp] * 2 + [_vp], C.c_int),
     "fx_batch_create": ([_i32, _vp, _i32, _vp, _i32, C.POINTER(_vp)], C.c_int),
     "fx_batch_work": ([_vp, _vp, _vp], C.c_int),
@@ -86,17 +87,13 @@ _SIGS = {
 }
 
 
-class P1Instance(C.Structure):
-    _fields_ = ([(n, _i32) for n in ("num_nodes", "num_elems", "nnz", "max_row_len")]
-                + [(n, _vp) for n in ("x", "y", "conn", "m", "f", "u", "inc_off", "inc", "row_ptr",
-                                      "vals", "res", "lam", "bc_mask", "jt", "g")])
 
 
 
 
 class PatchInfo(C.Structure):
     _fields_ = [(n, _i32) for n in ("num_patches", "num_ldofs", "num_seam", "num_parts",
-                                    "max_patch_ldofs")]
+                                    "max_patch_ldofs", "ldof_stride")]
 
 
 class PatchPlan(C.Structure):
@@ -104,6 +101,13 @@ class PatchPlan(C.Structure):
                 + [(n, _vp) for n in ("patch_off", "patch_elem", "elem_vert", "elem_ldof", "ldof_off",
                                       "ldof_dof", "ldof_part", "ldof_end", "seam_off", "seam_dof",
                                       "parts")])
+
+
+class P1Instance(C.Structure):
+    _fields_ = ([(n, _i32) for n in ("num_nodes", "num_elems", "nnz", "max_row_len")]
+                + [(n, _vp) for n in ("x", "y", "conn", "m", "f", "u", "inc_off", "inc", "row_ptr",
+                                      "vals", "res", "lam", "bc_mask", "jt", "g")]
+                + [("vjp", PatchPlan)])
 
 
 class THInstance(C.Structure):

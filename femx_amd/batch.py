@@ -69,7 +69,8 @@ class P1Case:
             g.num_nodes, g.num_elems, m.pattern.nnz, m.max_row_len, ptr(g.x), ptr(g.y),
             ptr(g.conn), ptr(self.m), ptr(self.f), ptr(self.u), ptr(m.device("inc_off", g.device)),
             ptr(m.device("inc", g.device)), ptr(rp), ptr(self.jac.vals), ptr(self.res),
-            ptr(self.lam), ptr(self.R.bmap.d_mask), ptr(self.jt), ptr(self.g))
+            ptr(self.lam), ptr(self.R.bmap.d_mask), ptr(self.jt), ptr(self.g),
+            self.R.patches().plan)
 
     def plain(self, what):
         from .assembly import PoissonP1, assemble
@@ -180,7 +181,7 @@ class Batch:
             self._h = None
 
     def work(self):
-        counts = (C.c_int32 * 5)()
+        counts = (C.c_int32 * 6)()
         grid = C.c_int32()
         _lib.call("fx_batch_work", self._h, C.cast(counts, C.c_void_p), C.byref(grid))
         return list(counts), grid.value

@@ -12,8 +12,8 @@
 // blocks share an L2 -- running the same per-block device code as the
 // single-instance kernels, so each instance's results are bitwise equal to
 // its plain launch.  The phases form four independent chains -- P1 assembly,
-// Taylor-Hood assembly, P1 VJP, and the Taylor-Hood VJP's patch then seam
-// pass -- launched on private branch streams forked from the caller's
+// Taylor-Hood assembly, and the P1 and Taylor-Hood VJPs (each a patch then a
+// seam pass) -- launched on private branch streams forked from the caller's
 // stream, so they run concurrently; captured, they become the branches of
 // one hipGraph, replayed with a single hipGraphLaunch.  Several batches (for
 // instance one per size class, each with its own grid and row tile) can be
@@ -32,21 +32,25 @@ struct fx_batch
 {
   int         num_p1 = 0, num_th = 0;
   int         grid   = 0;       // compute units x the tunable multiplier
-  int         pgrid[5] = {0, 0, 0, 0, 0}; // per phase: at least one full residency
+  int         pgrid[6] = {0, 0, 0, 0, 0, 0}; // per phase: at least one full residency
   int         cus    = 0;
   std::size_t p1_lds = 0, th_lds = 0, th_vjp_lds = 0;
   int         th_max_ldofs = 0; // the most local DOFs of a VJP patch over the instances
+  int         p1_max_ldofs = 0;
+  std::size_t p1_vjp_lds   = 0;
   int         th_tile = 0; // row tile per wave of the Taylor-Hood assembly (doubles)
   bool        p1_long = false; // some P1 row is longer than the LDS tile width
   // device memory
   fxk::P1AsmArgs*  d_p1_asm  = nullptr;
   fxk::P1VjpArgs*  d_p1_vjp  = nullptr;
+  fxk::PatchArgs*  d_p1_pat  = nullptr;
+  fxk::SeamArgs*   d_p1_seam = nullptr;
   fxk::ThAsmArgs*  d_th_asm  = nullptr;
   fxk::ThVjpArgs*   d_th_vjp  = nullptr;
-  fxk::ThPatchArgs* d_th_pat  = nullptr;
-  fxk::ThSeamArgs*  d_th_seam = nullptr;
-  int2*            d_work[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
-  int              nwork[5]  = {0, 0, 0, 0, 0};
+  fxk::PatchArgs* d_th_pat  = nullptr;
+  fxk::SeamArgs*  d_th_seam = nullptr;
+  int2*            d_work[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  int              nwork[6]  = {0, 0, 0, 0, 0, 0};
   hipStream_t      cap_stream = nullptr;
   hipGraph_t       graph      = nullptr;
   hipGraphExec_t   exec       = nullptr;
@@ -70,7 +74,8 @@ struct fx_batch
       if (e) (void)hipEventDestroy(e);
     for (void* p : {static_cast<void*>(d_p1_asm), static_cast<void*>(d_p1_vjp),
                     static_cast<void*>(d_th_asm), static_cast<void*>(d_th_vjp),
-                    static_cast<void*>(d_th_pat), static_cast<void*>(d_th_seam)})
+                    static_cast<void*>(d_th_pat), static_cast<void*>(d_th_seam),
+                    static_cast<void*>(d_p1_pat), static_cast<void*>(d_p1_seam)})
       if (p) (void)hipFree(p);
     for (auto* w : d_work)
       if (w) (void)hipFree(w);
@@ -100,7 +105,8 @@ enum Phase
   kThAsm = 1,
   kP1Vjp = 2,
   kThVjpPatch = 3,
-  kThVjpSeam = 4
+  kThVjpSeam = 4,
+  kP1VjpSeam = 5
 };
 
 // capped at 96 VGPRs (5 waves/SIMD, as k_p1_assemble reaches uncapped)
@@ -117,16 +123,17 @@ k_batch_p1_assemble(const P1AsmArgs* __restrict__ inst, const int2* __restrict__
   }
 }
 
-// capped at 96 VGPRs (5 waves/SIMD), as k_p1_vjp
-__global__ __launch_bounds__(kP1Block) __attribute__((amdgpu_waves_per_eu(5))) void
-k_batch_p1_vjp(const P1VjpArgs* __restrict__ inst, const int2* __restrict__ work, int nwork)
+__global__ __launch_bounds__(kPatchLanes) void k_batch_p1_vjp_patch(const P1VjpArgs* __restrict__ inst,
+                                                                    const PatchArgs* __restrict__ pat,
+                                                                    const int2* __restrict__ work,
+                                                                    int nwork, int max_patch_ldofs)
 {
-  __shared__ long long sinc[kIncStage];
+  extern __shared__ __align__(16) unsigned char lds[];
   const fx::XcdWalk wk = fx::xcd_walk(nwork);
   for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
-    p1_vjp_block(it.y, inst[it.x], sinc);
+    p1_vjp_patch_block(it.y, inst[it.x], pat[it.x], lds, max_patch_ldofs);
   }
 }
 
@@ -153,8 +160,8 @@ __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per
   }
 }
 
-__global__ __launch_bounds__(kThPatch) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_th_vjp_patch(
-    const ThVjpArgs* __restrict__ inst, const ThPatchArgs* __restrict__ pat,
+__global__ __launch_bounds__(kPatchLanes) __attribute__((amdgpu_waves_per_eu(4))) void k_batch_th_vjp_patch(
+    const ThVjpArgs* __restrict__ inst, const PatchArgs* __restrict__ pat,
     const int2* __restrict__ work, int nwork, int max_patch_ldofs)
 {
   extern __shared__ __align__(16) unsigned char lds[];
@@ -166,14 +173,14 @@ __global__ __launch_bounds__(kThPatch) __attribute__((amdgpu_waves_per_eu(4))) v
   }
 }
 
-__global__ __launch_bounds__(256) void k_batch_th_vjp_seam(const ThSeamArgs* __restrict__ inst,
+__global__ __launch_bounds__(256) void k_batch_vjp_seam(const SeamArgs* __restrict__ inst,
                                                           const int2* __restrict__ work, int nwork)
 {
   const fx::XcdWalk wk = fx::xcd_walk(nwork);
   for (int w = wk.first; w < wk.end; w += wk.stride)
   {
     const int2 it = work[w];
-    th_vjp_seam_block(it.y, inst[it.x]);
+    patch_seam_block(it.y, inst[it.x]);
   }
 }
 
@@ -209,15 +216,20 @@ void launch_chain(fx_batch* b, int chain, hipStream_t s)
                        b->d_th_asm, b->d_work[kThAsm], b->nwork[kThAsm], b->th_tile);
     break;
   case 2:
-    hipLaunchKernelGGL(k_batch_p1_vjp, g(kP1Vjp), dim3(kP1Block), 0, s, b->d_p1_vjp,
-                       b->d_work[kP1Vjp], b->nwork[kP1Vjp]);
+    hipLaunchKernelGGL(k_batch_p1_vjp_patch, g(kP1Vjp), dim3(kPatchLanes), b->p1_vjp_lds, s,
+                       b->d_p1_vjp, b->d_p1_pat, b->d_work[kP1Vjp], b->nwork[kP1Vjp],
+                       b->p1_max_ldofs);
+    fx::check_launch("fx_batch: VJP");
+    if (b->nwork[kP1VjpSeam] > 0)
+      hipLaunchKernelGGL(k_batch_vjp_seam, g(kP1VjpSeam), dim3(256), 0, s, b->d_p1_seam,
+                         b->d_work[kP1VjpSeam], b->nwork[kP1VjpSeam]);
     break;
   default:
-    hipLaunchKernelGGL(k_batch_th_vjp_patch, g(kThVjpPatch), dim3(kThPatch), b->th_vjp_lds, s, b->d_th_vjp,
+    hipLaunchKernelGGL(k_batch_th_vjp_patch, g(kThVjpPatch), dim3(kPatchLanes), b->th_vjp_lds, s, b->d_th_vjp,
                        b->d_th_pat, b->d_work[kThVjpPatch], b->nwork[kThVjpPatch], b->th_max_ldofs);
     fx::check_launch("fx_batch: VJP");
     if (b->nwork[kThVjpSeam] > 0)
-      hipLaunchKernelGGL(k_batch_th_vjp_seam, g(kThVjpSeam), dim3(256), 0, s, b->d_th_seam,
+      hipLaunchKernelGGL(k_batch_vjp_seam, g(kThVjpSeam), dim3(256), 0, s, b->d_th_seam,
                          b->d_work[kThVjpSeam], b->nwork[kThVjpSeam]);
   }
   fx::check_launch("fx_batch");
@@ -334,9 +346,11 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     std::vector<P1VjpArgs>  pv;
     std::vector<ThAsmArgs>  ta;
     std::vector<ThVjpArgs>  tv;
-    std::vector<ThPatchArgs> tp;
-    std::vector<ThSeamArgs>  tsm;
-    std::vector<int2>       w[5];
+    std::vector<PatchArgs> tp;
+    std::vector<SeamArgs>  tsm;
+    std::vector<PatchArgs> pp;
+    std::vector<SeamArgs>  psm;
+    std::vector<int2>       w[6];
     int                     max_row_len = 1;
     fx_index                max_blk_nnz = 0;
     for (int i = 0; i < num_p1; ++i)
@@ -356,10 +370,25 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
       pv.push_back({q.num_nodes, q.x, q.y, q.conn, q.m, q.u, q.lam, q.bc_mask, q.inc_off,
                     reinterpret_cast<const int2*>(q.inc), q.jt, q.g});
       const int nb = static_cast<int>(fx::blocks_for(q.num_nodes, kP1Block));
-      for (int k = 0; k < nb; ++k)
+      for (int k = 0; k < nb; ++k) w[kP1Asm].push_back(make_int2(i, k));
+      const fx_patch_plan& pl = q.vjp;
+      fx::require(pl.num_patches >= 0 && pl.num_seam >= 0 && pl.num_parts >= 0
+                      && pl.max_patch_ldofs >= 0 && pl.max_patch_ldofs <= 3 * kPatchLanes
+                      && fx::fits32(pl.num_parts, 8),
+                  FX_ERR_ARG, "fx_batch_create: invalid P1 VJP patch plan");
+      const bool vjp = q.lam && q.jt && q.g && q.m && q.u && pl.num_patches > 0 && pl.patch_off
+                       && pl.patch_elem && pl.elem_ldof && pl.ldof_off && pl.ldof_dof
+                       && pl.ldof_part && pl.ldof_end && pl.seam_off
+                       && (pl.num_seam == 0 || pl.seam_dof) && (pl.num_parts == 0 || pl.parts);
+      pp.push_back({pl.patch_off, pl.patch_elem, pl.elem_vert, pl.elem_ldof, pl.ldof_off,
+                    pl.ldof_dof, pl.ldof_part, pl.ldof_end, q.jt, pl.parts});
+      psm.push_back({pl.num_seam, pl.seam_off, pl.seam_dof, pl.parts, q.lam, q.bc_mask, q.jt});
+      if (vjp)
       {
-        w[kP1Asm].push_back(make_int2(i, k));
-        if (q.lam && q.jt && q.g && q.m && q.u) w[kP1Vjp].push_back(make_int2(i, k));
+        b->p1_max_ldofs = std::max(b->p1_max_ldofs, static_cast<int>(pl.max_patch_ldofs));
+        const int ns = static_cast<int>(fx::blocks_for(pl.num_seam, 256));
+        for (int k = 0; k < pl.num_patches; ++k) w[kP1Vjp].push_back(make_int2(i, k));
+        for (int k = 0; k < ns; ++k) w[kP1VjpSeam].push_back(make_int2(i, k));
       }
     }
     const int acc_cap = kP1Block * max_row_len;
@@ -381,13 +410,13 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
                     reinterpret_cast<const uint4*>(q.inc_meta), q.blk_row, q.blk_flags, q.blk_lanes,
                     q.row_ptr, q.vals, q.res});
       for (int k = 0; k < q.num_blocks; ++k) w[kThAsm].push_back(make_int2(i, k));
-      const fx_th_patch_plan& pv = q.vjp;
+      const fx_patch_plan& pv = q.vjp;
       const bool vjp = q.lam && q.jt && pv.num_patches > 0 && pv.patch_off && pv.patch_elem
                        && pv.elem_vert && pv.elem_ldof && pv.ldof_off && pv.ldof_dof && pv.ldof_part && pv.ldof_end
                        && pv.seam_off && (pv.num_seam == 0 || pv.seam_dof)
                        && (pv.num_parts == 0 || pv.parts);
       fx::require(pv.num_patches >= 0 && pv.num_seam >= 0 && pv.num_parts >= 0
-                      && pv.max_patch_ldofs >= 0 && pv.max_patch_ldofs <= 15 * kThPatch
+                      && pv.max_patch_ldofs >= 0 && pv.max_patch_ldofs <= 15 * kPatchLanes
                       && fx::fits32(pv.num_parts, 8),
                   FX_ERR_ARG, "fx_batch_create: invalid Taylor-Hood VJP patch plan");
       tv.push_back({q.num_elems, q.xv, q.yv, q.vconn3, q.dofs15, q.state, q.nu, q.nu_e, q.lam,
@@ -430,20 +459,26 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
             kP1Block, b->p1_lds);
     b->pgrid[kThAsm] = occ(kThAsm, reinterpret_cast<const void*>(k_batch_th_assemble),
                            kThLanes * kThWaves, b->th_lds);
-    b->pgrid[kP1Vjp] = occ(kP1Vjp, reinterpret_cast<const void*>(k_batch_p1_vjp), kP1Block, 0);
+    b->p1_vjp_lds    = p1_patch_lds(b->p1_max_ldofs);
+    b->pgrid[kP1Vjp] = occ(kP1Vjp, reinterpret_cast<const void*>(k_batch_p1_vjp_patch), kPatchLanes,
+                           b->p1_vjp_lds);
+    b->pgrid[kP1VjpSeam] =
+        occ(kP1VjpSeam, reinterpret_cast<const void*>(k_batch_vjp_seam), 256, 0);
     b->th_vjp_lds = th_patch_lds(b->th_max_ldofs);
     b->pgrid[kThVjpPatch] =
-        occ(kThVjpPatch, reinterpret_cast<const void*>(k_batch_th_vjp_patch), kThPatch,
+        occ(kThVjpPatch, reinterpret_cast<const void*>(k_batch_th_vjp_patch), kPatchLanes,
             b->th_vjp_lds);
     b->pgrid[kThVjpSeam] =
-        occ(kThVjpSeam, reinterpret_cast<const void*>(k_batch_th_vjp_seam), 256, 0);
+        occ(kThVjpSeam, reinterpret_cast<const void*>(k_batch_vjp_seam), 256, 0);
     b->d_p1_asm = upload(pa);
     b->d_p1_vjp = upload(pv);
     b->d_th_asm = upload(ta);
     b->d_th_vjp = upload(tv);
     b->d_th_pat  = upload(tp);
     b->d_th_seam = upload(tsm);
-    for (int p = 0; p < 5; ++p)
+    b->d_p1_pat  = upload(pp);
+    b->d_p1_seam = upload(psm);
+    for (int p = 0; p < 6; ++p)
     {
       b->nwork[p]  = static_cast<int>(w[p].size());
       b->d_work[p] = upload(w[p]);
@@ -452,11 +487,11 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
   });
 }
 
-int fx_batch_work(const fx_batch* b, fx_index* counts5, fx_index* grid)
+int fx_batch_work(const fx_batch* b, fx_index* counts6, fx_index* grid)
 {
   return fx::guard([&] {
-    fx::require(b && counts5, FX_ERR_ARG, "fx_batch_work: null argument");
-    for (int p = 0; p < 5; ++p) counts5[p] = b->nwork[p];
+    fx::require(b && counts6, FX_ERR_ARG, "fx_batch_work: null argument");
+    for (int p = 0; p < 6; ++p) counts6[p] = b->nwork[p];
     if (grid) *grid = b->grid;
   });
 }

@@ -695,7 +695,7 @@ void fx_map_destroy(fx_map* m) { delete m; }
 } // extern "C"
 
 // ---------------------------------------------------------------------------
-// Patch plan of the single-pass Taylor-Hood VJP.  Elements are ordered along
+// Patch plan of the single-pass VJPs (Taylor-Hood and P1).  Elements are ordered along
 // a Morton (Z-order) curve of their centroids and cut into patches of
 // consecutive elements, so a patch is a compact piece of the mesh; inside a
 // patch elements are in ascending id.  A patch's local DOFs (ascending global
@@ -705,9 +705,9 @@ void fx_map_destroy(fx_map* m) { delete m; }
 // any other DOF (on a seam between patches, or in no element) is summed from
 // its per-patch partials, in patch order, by the seam pass.
 
-struct fx_th_patches
+struct fx_patches
 {
-  Index ndof = 0, num_elems = 0, num_parts = 0, max_patch_ldofs = 0;
+  Index ndof = 0, num_elems = 0, num_parts = 0, max_patch_ldofs = 0, stride = 16;
   std::vector<Index>         patch_off, patch_elem, elem_vert, ldof_off, ldof_dof, ldof_part, seam_off,
       seam_dof;
   std::vector<std::uint16_t> elem_ldof, ldof_end;
@@ -729,29 +729,31 @@ std::uint32_t spread16(std::uint32_t v)
 
 extern "C" {
 
-int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const Index* vconn3,
-                         Index num_vertices, const double* xv, const double* yv, Index max_patch,
-                         fx_th_patches** out)
+int fx_patches_create(Index ndof, Index num_elems, Index dofs_per_elem, const Index* dofs,
+                      const Index* vconn3, Index num_vertices, const double* xv, const double* yv,
+                      Index max_patch, fx_patches** out)
 {
   return fx::guard([&] {
-    fx::require(out && ndof >= 0 && num_elems >= 0 && num_vertices >= 0
-                    && (num_elems == 0 || (dofs15 && vconn3 && xv && yv)),
-                FX_ERR_ARG, "fx_th_patches_create: invalid arguments");
+    fx::require(out && ndof >= 0 && num_elems >= 0 && num_vertices >= 0 && dofs_per_elem >= 1
+                    && dofs_per_elem <= 16 && (num_elems == 0 || (dofs && vconn3 && xv && yv)),
+                FX_ERR_ARG, "fx_patches_create: invalid arguments");
+    const int per = static_cast<int>(dofs_per_elem), stride = (per + 3) / 4 * 4;
     fx::require(max_patch >= 1 && max_patch <= FX_TH_PATCH_MAX, FX_ERR_ARG,
-                "fx_th_patches_create: max_patch must be in [1, FX_TH_PATCH_MAX]");
+                "fx_patches_create: max_patch must be in [1, FX_TH_PATCH_MAX]");
     *out        = nullptr;
-    auto        p = std::make_unique<fx_th_patches>();
+    auto        p = std::make_unique<fx_patches>();
     const Index E = num_elems;
     p->ndof       = ndof;
+    p->stride     = stride;
     p->num_elems  = E;
     for (Index e = 0; e < E; ++e)
     {
-      for (int k = 0; k < 15; ++k)
-        fx::require(dofs15[15 * e + k] >= 0 && dofs15[15 * e + k] < ndof, FX_ERR_ARG,
-                    "fx_th_patches_create: DOF id out of range");
+      for (int k = 0; k < per; ++k)
+        fx::require(dofs[per * e + k] >= 0 && dofs[per * e + k] < ndof, FX_ERR_ARG,
+                    "fx_patches_create: DOF id out of range");
       for (int k = 0; k < 3; ++k)
         fx::require(vconn3[3 * e + k] >= 0 && vconn3[3 * e + k] < num_vertices, FX_ERR_ARG,
-                    "fx_th_patches_create: vertex id out of range");
+                    "fx_patches_create: vertex id out of range");
     }
     // Morton order of the element centroids (ties by element id)
     double lo[2] = {0.0, 0.0}, hi[2] = {0.0, 0.0};
@@ -787,16 +789,16 @@ int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const
       const Index e  = static_cast<Index>(key[i] & 0xffffffffu);
       const Index q  = static_cast<Index>(p->patch_off.size()) - 1;
       bool        cut = i - p->patch_off.back() == max_patch;
-      for (int k = 0; k < 15 && !cut; ++k)
+      for (int k = 0; k < per && !cut; ++k)
       {
-        const Index d = dofs15[15 * e + k];
+        const Index d = dofs[per * e + k];
         cut           = stamp[d] == q && cnt[d] == FX_TH_PATCH_RUN_MAX;
       }
       if (cut) p->patch_off.push_back(i);
       const Index qq = static_cast<Index>(p->patch_off.size()) - 1;
-      for (int k = 0; k < 15; ++k)
+      for (int k = 0; k < per; ++k)
       {
-        const Index d = dofs15[15 * e + k];
+        const Index d = dofs[per * e + k];
         if (stamp[d] != qq) stamp[d] = qq, cnt[d] = 0;
         ++cnt[d];
       }
@@ -814,9 +816,9 @@ int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const
     std::vector<Index> npatch(ndof, 0), last(ndof, -1);
     for (Index q = 0; q < P; ++q)
       for (Index i = p->patch_off[q]; i < p->patch_off[q + 1]; ++i)
-        for (int k = 0; k < 15; ++k)
+        for (int k = 0; k < per; ++k)
         {
-          const Index d = dofs15[15 * p->patch_elem[i] + k];
+          const Index d = dofs[per * p->patch_elem[i] + k];
           if (last[d] != q) last[d] = q, ++npatch[d];
         }
     // seam DOFs: touched by several patches or by none; partials in patch order
@@ -831,35 +833,35 @@ int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const
       }
     p->num_parts = p->seam_off.back();
     fx::require(p->num_parts < FX_TH_PART_FIRST, FX_ERR_OVERFLOW,
-                "fx_th_patches_create: too many seam partials");
+                "fx_patches_create: too many seam partials");
 
     // local DOFs and runs of each patch
-    p->elem_ldof.assign(16 * std::size_t(E), 0);
+    p->elem_ldof.assign(stride * std::size_t(E), 0);
     p->ldof_off.push_back(0);
-    std::vector<Index> local(ndof, -1), count, cursor, dofs;
+    std::vector<Index> local(ndof, -1), count, cursor, pdofs;
     for (Index q = 0; q < P; ++q)
     {
       const Index i0 = p->patch_off[q], i1 = p->patch_off[q + 1];
-      dofs.clear();
+      pdofs.clear();
       for (Index i = i0; i < i1; ++i)
-        for (int k = 0; k < 15; ++k)
+        for (int k = 0; k < per; ++k)
         {
-          const Index d = dofs15[15 * p->patch_elem[i] + k];
-          if (local[d] < 0) local[d] = 0, dofs.push_back(d);
+          const Index d = dofs[per * p->patch_elem[i] + k];
+          if (local[d] < 0) local[d] = 0, pdofs.push_back(d);
         }
-      std::sort(dofs.begin(), dofs.end());
-      const Index nl = static_cast<Index>(dofs.size());
-      fx::require(nl <= 15 * FX_TH_PATCH_MAX, FX_ERR_UNSUPPORTED,
-                  "fx_th_patches_create: patch has too many local DOFs");
-      for (Index j = 0; j < nl; ++j) local[dofs[j]] = j;
+      std::sort(pdofs.begin(), pdofs.end());
+      const Index nl = static_cast<Index>(pdofs.size());
+      fx::require(nl <= 4095 && nl <= per * FX_TH_PATCH_MAX, FX_ERR_UNSUPPORTED,
+                  "fx_patches_create: patch has too many local DOFs");
+      for (Index j = 0; j < nl; ++j) local[pdofs[j]] = j;
       count.assign(nl, 0);
       for (Index i = i0; i < i1; ++i)
-        for (int k = 0; k < 15; ++k)
+        for (int k = 0; k < per; ++k)
         {
-          const Index d = dofs15[15 * p->patch_elem[i] + k];
+          const Index d = dofs[per * p->patch_elem[i] + k];
           for (int kk = 0; kk < k; ++kk)
-            fx::require(dofs15[15 * p->patch_elem[i] + kk] != d, FX_ERR_UNSUPPORTED,
-                        "fx_th_patches_create: an element lists one DOF twice");
+            fx::require(dofs[per * p->patch_elem[i] + kk] != d, FX_ERR_UNSUPPORTED,
+                        "fx_patches_create: an element lists one DOF twice");
           ++count[local[d]];
         }
       Index run = 0;
@@ -867,7 +869,7 @@ int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const
       {
         run += count[j];
         p->ldof_end.push_back(static_cast<std::uint16_t>(run));
-        const Index d = dofs[j];
+        const Index d = pdofs[j];
         p->ldof_dof.push_back(d);
         if (npatch[d] == 1)
           p->ldof_part.push_back(-1);
@@ -881,13 +883,13 @@ int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const
       }
       cursor.assign(nl, 0); // rank of the next element in each run
       for (Index i = i0; i < i1; ++i)
-        for (int k = 0; k < 15; ++k)
+        for (int k = 0; k < per; ++k)
         {
-          const Index j = local[dofs15[15 * p->patch_elem[i] + k]];
-          p->elem_ldof[16 * std::size_t(i) + k] =
+          const Index j = local[dofs[per * p->patch_elem[i] + k]];
+          p->elem_ldof[stride * std::size_t(i) + k] =
               static_cast<std::uint16_t>(j | (cursor[j]++ << 12));
         }
-      for (Index d : dofs) local[d] = -1;
+      for (Index d : pdofs) local[d] = -1;
       p->ldof_off.push_back(p->ldof_off.back() + nl);
       p->max_patch_ldofs = std::max(p->max_patch_ldofs, nl);
     }
@@ -895,19 +897,20 @@ int fx_th_patches_create(Index ndof, Index num_elems, const Index* dofs15, const
   });
 }
 
-int fx_th_patches_info_get(const fx_th_patches* p, fx_th_patches_info* info)
+int fx_patches_info_get(const fx_patches* p, fx_patches_info* info)
 {
   return fx::guard([&] {
-    fx::require(p && info, FX_ERR_ARG, "fx_th_patches_info_get: null argument");
+    fx::require(p && info, FX_ERR_ARG, "fx_patches_info_get: null argument");
     info->num_patches = static_cast<Index>(p->patch_off.size()) - 1;
     info->num_ldofs   = static_cast<Index>(p->ldof_dof.size());
     info->num_seam    = static_cast<Index>(p->seam_dof.size());
     info->num_parts   = p->num_parts;
     info->max_patch_ldofs = p->max_patch_ldofs;
+    info->ldof_stride     = p->stride;
   });
 }
 
-int fx_th_patches_export(const fx_th_patches* p, Index* patch_off, Index* patch_elem,
+int fx_patches_export(const fx_patches* p, Index* patch_off, Index* patch_elem,
                          Index* elem_vert, unsigned short* elem_ldof, Index* ldof_off,
                          Index* ldof_dof,
                          Index* ldof_part, unsigned short* ldof_end, Index* seam_off,
@@ -918,7 +921,7 @@ int fx_th_patches_export(const fx_th_patches* p, Index* patch_off, Index* patch_
                     && (p->num_elems == 0 || (patch_elem && elem_vert && elem_ldof))
                     && (p->ldof_dof.empty() || (ldof_dof && ldof_part && ldof_end))
                     && (p->seam_dof.empty() || seam_dof),
-                FX_ERR_ARG, "fx_th_patches_export: null array");
+                FX_ERR_ARG, "fx_patches_export: null array");
     std::copy(p->patch_off.begin(), p->patch_off.end(), patch_off);
     std::copy(p->patch_elem.begin(), p->patch_elem.end(), patch_elem);
     std::copy(p->elem_vert.begin(), p->elem_vert.end(), elem_vert);
@@ -932,7 +935,7 @@ int fx_th_patches_export(const fx_th_patches* p, Index* patch_off, Index* patch_
   });
 }
 
-void fx_th_patches_destroy(fx_th_patches* p) { delete p; }
+void fx_patches_destroy(fx_patches* p) { delete p; }
 
 // ---------------------------------------------------------------------------
 // CSR transpose permutation: counting sort by column, rows visited in order.

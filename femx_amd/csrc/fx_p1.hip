@@ -23,6 +23,18 @@ __global__ __launch_bounds__(kP1Block) __attribute__((amdgpu_waves_per_eu(5))) v
   p1_vjp_block(fx::xcd_block(gridDim.x), a, sinc);
 }
 
+__global__ __launch_bounds__(kPatchLanes) void k_p1_vjp_patch(P1VjpArgs a, PatchArgs q,
+                                                              int max_patch_ldofs)
+{
+  extern __shared__ __align__(16) unsigned char lds[];
+  p1_vjp_patch_block(fx::xcd_block(gridDim.x), a, q, lds, max_patch_ldofs);
+}
+
+__global__ __launch_bounds__(256) void k_p1_vjp_seam(SeamArgs a)
+{
+  patch_seam_block(fx::xcd_block(gridDim.x), a);
+}
+
 } // namespace
 
 extern "C" {
@@ -84,6 +96,53 @@ int fx_p1_vjp(fx_index num_nodes, fx_index num_elems, const double* x, const dou
                       reinterpret_cast<const int2*>(inc), jt_lam, g};
     hipLaunchKernelGGL(k_p1_vjp, dim3(nb), dim3(kP1Block), 0, fx::as_stream(stream), a);
     fx::check_launch("fx_p1_vjp");
+  });
+}
+
+int fx_p1_vjp_patch(fx_index num_nodes, fx_index num_elems, const double* x, const double* y,
+                    const double* m, const double* u, const double* lam, const fx_index* bc_mask,
+                    const fx_patch_plan* plan, double* jt_lam, double* g, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(num_nodes >= 0 && num_elems >= 0 && plan, FX_ERR_ARG,
+                "fx_p1_vjp_patch: negative size or null plan");
+    const fx_patch_plan& q = *plan;
+    fx::require(q.num_patches >= 0 && q.num_seam >= 0 && q.num_parts >= 0
+                    && q.max_patch_ldofs >= 0 && q.max_patch_ldofs <= 3 * kPatchLanes
+                    && (q.num_patches > 0) == (num_elems > 0),
+                FX_ERR_ARG, "fx_p1_vjp_patch: plan sizes do not fit the mesh");
+    if (num_nodes == 0) return;
+    fx::require(fx::fits32(num_nodes, 8) && fx::fits32(num_elems, 8) && fx::fits32(q.num_parts, 8),
+                FX_ERR_OVERFLOW, "fx_p1_vjp_patch: node, element or partial arrays beyond 4 GiB");
+    fx::require(x && y && u && lam && jt_lam && q.patch_off && q.ldof_off && q.seam_off
+                    && (num_elems == 0
+                        || (m && g && q.patch_elem && q.elem_ldof && q.ldof_dof && q.ldof_part
+                            && q.ldof_end))
+                    && (q.num_seam == 0 || q.seam_dof) && (q.num_parts == 0 || q.parts),
+                FX_ERR_ARG, "fx_p1_vjp_patch: null array");
+    const std::size_t N = num_nodes, E = num_elems, np = std::size_t(q.num_parts) * 8;
+    fx::require(!fx::overlaps(jt_lam, N * 8, g, E * 8) && !fx::overlaps(jt_lam, N * 8, lam, N * 8)
+                    && !fx::overlaps(jt_lam, N * 8, u, N * 8) && !fx::overlaps(g, E * 8, m, E * 8)
+                    && !fx::overlaps(q.parts, np, jt_lam, N * 8)
+                    && !fx::overlaps(q.parts, np, lam, N * 8) && !fx::overlaps(q.parts, np, g, E * 8),
+                FX_ERR_ALIAS, "fx_p1_vjp_patch: outputs must not alias inputs or each other");
+    if (q.num_patches > 0)
+    {
+      const P1VjpArgs a{num_nodes, x, y, nullptr, m, u, lam, bc_mask, nullptr, nullptr, jt_lam, g};
+      const PatchArgs pa{q.patch_off, q.patch_elem, q.elem_vert, q.elem_ldof, q.ldof_off,
+                         q.ldof_dof,  q.ldof_part,  q.ldof_end,  jt_lam,      q.parts};
+      const int       mx = static_cast<int>(q.max_patch_ldofs);
+      hipLaunchKernelGGL(k_p1_vjp_patch, dim3(static_cast<unsigned>(q.num_patches)),
+                         dim3(kPatchLanes), p1_patch_lds(mx), fx::as_stream(stream), a, pa, mx);
+      fx::check_launch("fx_p1_vjp_patch");
+    }
+    if (q.num_seam > 0)
+    {
+      const SeamArgs sa{q.num_seam, q.seam_off, q.seam_dof, q.parts, lam, bc_mask, jt_lam};
+      hipLaunchKernelGGL(k_p1_vjp_seam, dim3(fx::blocks_for(q.num_seam, 256)), dim3(256), 0,
+                         fx::as_stream(stream), sa);
+      fx::check_launch("fx_p1_vjp_patch");
+    }
   });
 }
 

@@ -364,4 +364,136 @@ __device__ __forceinline__ void p1_vjp_block(int blk, const P1VjpArgs& a, long l
   __syncthreads();
 }
 
+
+// ---------------------------------------------------------------------------
+// The same VJP in one pass over element patches (fx_patches_create with the
+// nodes as DOFs): a workgroup stages its patch's local node values (x, y, u,
+// masked lam, constraint flag) in LDS with one gather each, then one lane per
+// element forms the element's three columns of K^T lam~ -- each exactly as
+// the row-gather lane above forms it -- and g_e, stores the columns into the
+// nodes' runs, and the runs are summed in element order.  A node inside one
+// patch gets the row-gather VJP's sum bit for bit; a seam node's partials
+// are summed by the seam pass.
+
+// LDS of one P1 patch workgroup, in bytes: the slots (3 per element), four
+// doubles per local node, the runs' starts (u16) and constraint flags (u8).
+__host__ __device__ constexpr int p1_patch_lds(int max_patch_ldofs)
+{
+  return (8 * 3 * kPatchLanes + 32 * max_patch_ldofs + 3 * max_patch_ldofs + 2 + 15) / 16 * 16;
+}
+
+__device__ __forceinline__ void p1_vjp_patch_block(int p, const P1VjpArgs& a, const PatchArgs& q,
+                                                   unsigned char* lds, int max_patch_ldofs)
+{
+  constexpr int   U  = 3; // a P1 patch has at most 3 * kPatchLanes local nodes
+  const int       t  = threadIdx.x;
+  const int       i0 = q.patch_off[p], ne = q.patch_off[p + 1] - i0;
+  const int       l0 = q.ldof_off[p], nl = q.ldof_off[p + 1] - l0;
+  const bool      mine = t < ne;
+  const int       i    = i0 + t;
+  double*         cs   = reinterpret_cast<double*>(lds);
+  double*         xs   = cs + 3 * kPatchLanes;
+  double*         ys   = xs + max_patch_ldofs;
+  double*         us   = ys + max_patch_ldofs;
+  double*         ls   = us + max_patch_ldofs;
+  unsigned short* rs   = reinterpret_cast<unsigned short*>(ls + max_patch_ldofs);
+  unsigned char*  mk   = reinterpret_cast<unsigned char*>(rs + max_patch_ldofs + 1);
+  // the element's tables and log-coefficient are requested first, so that
+  // their latency overlaps the staging below
+  int      e  = 0;
+  unsigned w0 = 0, w1 = 0;
+  double   me = 0.0;
+  if (mine)
+  {
+    e              = q.patch_elem[i];
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    const u2 lw    = __builtin_nontemporal_load(reinterpret_cast<const u2*>(q.elem_ldof + 4 * static_cast<long>(i)));
+    w0 = lw.x, w1 = lw.y;
+    me = gat32(a.m, e);
+  }
+  if (t == 0) rs[0] = 0;
+  for (int j0 = t; j0 < nl; j0 += U * kPatchLanes)
+  {
+    int  d[U], re[U];
+    bool on[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      on[u]        = j0 + u * kPatchLanes < nl;
+      const int  j = l0 + j0 + u * kPatchLanes;
+      d[u]         = on[u] ? q.ldof_dof[j] : 0;
+      re[u]        = on[u] ? q.ldof_end[j] : 0;
+    }
+    double xv[U], yv[U], uv[U], lv[U];
+    int    m[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      xv[u] = on[u] ? gat32(a.x, d[u]) : 0.0;
+      yv[u] = on[u] ? gat32(a.y, d[u]) : 0.0;
+      uv[u] = on[u] ? gat32(a.u, d[u]) : 0.0;
+      lv[u] = on[u] ? gat32(a.lam, d[u]) : 0.0;
+      m[u]  = on[u] && a.bc_mask ? gat32(a.bc_mask, d[u]) : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (j0 + u * kPatchLanes < nl)
+      {
+        const int j = j0 + u * kPatchLanes;
+        xs[j] = xv[u], ys[j] = yv[u], us[j] = uv[u];
+        ls[j]     = m[u] ? 0.0 : lv[u];
+        rs[j + 1] = static_cast<unsigned short>(re[u]);
+        mk[j]     = m[u] ? 1 : 0;
+      }
+  }
+  __syncthreads();
+  if (mine)
+  {
+    const unsigned h[3] = {w0 & 0xffffu, w0 >> 16, w1 & 0xffffu};
+    const unsigned j[3] = {h[0] & 0xfffu, h[1] & 0xfffu, h[2] & 0xfffu};
+    const P1Geom   g  = p1_geom(xs[j[0]], ys[j[0]], xs[j[1]], ys[j[1]], xs[j[2]], ys[j[2]]);
+    const double   ke = exp(me);
+    const double   le[3] = {ls[j[0]], ls[j[1]], ls[j[2]]};
+#pragma unroll
+    for (int li = 0; li < 3; ++li)
+    {
+      const double c = p1_k(g, 0, li, ke) * le[0] + p1_k(g, 1, li, ke) * le[1]
+                       + p1_k(g, 2, li, ke) * le[2];
+      cs[rs[j[li]] + (h[li] >> 12)] = c;
+    }
+    if (a.gout)
+    {
+      const double u0 = us[j[0]], u1 = us[j[1]], u2 = us[j[2]];
+      double       ge = 0.0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+      {
+        const double re = p1_k(g, r, 0, ke) * u0 + p1_k(g, r, 1, ke) * u1 + p1_k(g, r, 2, ke) * u2;
+        ge              = ge + le[r] * (re - 0.0);
+      }
+      __builtin_nontemporal_store(ge, a.gout + e);
+    }
+  }
+  __syncthreads();
+  for (int j0 = t; j0 < nl; j0 += U * kPatchLanes)
+  {
+    int d[U], part[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const bool on = j0 + u * kPatchLanes < nl;
+      const int  j  = l0 + j0 + u * kPatchLanes;
+      d[u]          = on ? q.ldof_dof[j] : 0;
+      part[u]       = on ? q.ldof_part[j] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      const int j = j0 + u * kPatchLanes;
+      if (j < nl) patch_finish(cs, rs[j], rs[j + 1], d[u], part[u], mk[j] != 0, a.lam, q.jt, q.parts);
+    }
+  }
+  __syncthreads(); // the LDS is reused by the next patch of a persistent walk
+}
+
 } // namespace fxk

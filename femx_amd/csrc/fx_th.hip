@@ -40,18 +40,18 @@ __global__ __launch_bounds__(256) void k_vjp_gather(GatherArgs a)
   vjp_gather_block(fx::xcd_block(gridDim.x), a, sval);
 }
 
-static_assert(kThPatch == FX_TH_PATCH_MAX, "one lane per element of a patch");
+static_assert(kPatchLanes == FX_TH_PATCH_MAX, "one lane per element of a patch");
 
-__global__ __launch_bounds__(kThPatch) __attribute__((amdgpu_waves_per_eu(4))) void k_th_vjp_patch(ThVjpArgs a, ThPatchArgs q,
+__global__ __launch_bounds__(kPatchLanes) __attribute__((amdgpu_waves_per_eu(4))) void k_th_vjp_patch(ThVjpArgs a, PatchArgs q,
                                                                                            int max_patch_ldofs)
 {
   extern __shared__ __align__(16) unsigned char lds[];
   th_vjp_patch_block(fx::xcd_block(gridDim.x), a, q, kThTables, lds, max_patch_ldofs);
 }
 
-__global__ __launch_bounds__(256) void k_th_vjp_seam(ThSeamArgs a)
+__global__ __launch_bounds__(256) void k_th_vjp_seam(SeamArgs a)
 {
-  th_vjp_seam_block(fx::xcd_block(gridDim.x), a);
+  patch_seam_block(fx::xcd_block(gridDim.x), a);
 }
 
 // The two VJP phases with the element-column workspace `work` (15 doubles
@@ -178,14 +178,14 @@ int fx_th_vjp(fx_index ndof, fx_index num_elems, const double* xv, const double*
 int fx_th_vjp_patch(fx_index ndof, fx_index num_elems, const double* xv, const double* yv,
                     const double* state, double nu,
                     const double* nu_e, const double* lam, const fx_index* bc_mask,
-                    const fx_th_patch_plan* plan, double* jt, double* g, void* stream)
+                    const fx_patch_plan* plan, double* jt, double* g, void* stream)
 {
   return fx::guard([&] {
     fx::require(ndof >= 0 && num_elems >= 0 && plan, FX_ERR_ARG,
                 "fx_th_vjp_patch: negative size or null plan");
-    const fx_th_patch_plan& q = *plan;
+    const fx_patch_plan& q = *plan;
     fx::require(q.num_patches >= 0 && q.num_seam >= 0 && q.num_parts >= 0
-                    && q.max_patch_ldofs >= 0 && q.max_patch_ldofs <= 15 * kThPatch
+                    && q.max_patch_ldofs >= 0 && q.max_patch_ldofs <= 15 * kPatchLanes
                     && (q.num_patches > 0) == (num_elems > 0),
                 FX_ERR_ARG, "fx_th_vjp_patch: plan sizes do not fit the mesh");
     if (ndof == 0) return;
@@ -208,16 +208,16 @@ int fx_th_vjp_patch(fx_index ndof, fx_index num_elems, const double* xv, const d
     {
       const ThVjpArgs   a{num_elems, xv, yv, nullptr, nullptr, state, nu, nu_e,
                           lam,       bc_mask, nullptr, nullptr, g};
-      const ThPatchArgs pa{q.patch_off, q.patch_elem, q.elem_vert, q.elem_ldof, q.ldof_off,
+      const PatchArgs pa{q.patch_off, q.patch_elem, q.elem_vert, q.elem_ldof, q.ldof_off,
                            q.ldof_dof,  q.ldof_part,  q.ldof_end,  jt,          q.parts};
       const int         mx = static_cast<int>(q.max_patch_ldofs);
       hipLaunchKernelGGL(k_th_vjp_patch, dim3(static_cast<unsigned>(q.num_patches)),
-                         dim3(kThPatch), th_patch_lds(mx), fx::as_stream(stream), a, pa, mx);
+                         dim3(kPatchLanes), th_patch_lds(mx), fx::as_stream(stream), a, pa, mx);
       fx::check_launch("fx_th_vjp_patch");
     }
     if (q.num_seam > 0)
     {
-      const ThSeamArgs sa{q.num_seam, q.seam_off, q.seam_dof, q.parts, lam, bc_mask, jt};
+      const SeamArgs sa{q.num_seam, q.seam_off, q.seam_dof, q.parts, lam, bc_mask, jt};
       hipLaunchKernelGGL(k_th_vjp_seam, dim3(fx::blocks_for(q.num_seam, 256)), dim3(256), 0,
                          fx::as_stream(stream), sa);
       fx::check_launch("fx_th_vjp_patch");

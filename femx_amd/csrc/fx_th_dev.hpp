@@ -755,52 +755,32 @@ __device__ __forceinline__ void th_vjp_elem_block(int blk, const ThVjpArgs& a, c
 
 // ---------------------------------------------------------------------------
 // Single-pass VJP over element patches (fx_th_patches_create).  A workgroup
-// of kThPatch lanes takes one patch: it loads the patch's local DOF values
+// of kPatchLanes lanes takes one patch: it loads the patch's local DOF values
 // (state, masked adjoint) into LDS, one gather per local DOF, then, one
 // element per lane, forms the element's columns as above from them and
 // stores each into its slot of the patch's LDS slot array, where every local
 // DOF owns a run of slots in element order.  After a barrier the lanes sum the runs, each in slot order; a DOF
 // held by this patch alone is finished here (jt, with lam passed through on
 // constrained DOFs), any other gets this patch's partial in its seam slot.
-constexpr int kThPatch = 256;
-
-struct ThPatchArgs
-{
-  const int*            patch_off;
-  const int*            patch_elem;
-  const int*            elem_vert; // 3 per element in patch order
-  const unsigned short* elem_ldof; // 16 per element in patch order: local DOF | run rank << 12
-  const int*            ldof_off;
-  const int*            ldof_dof;
-  const int*            ldof_part;
-  const unsigned short* ldof_end;
-  double*               jt;
-  double*               parts;
-};
-
 // LDS of one patch workgroup, in bytes: the staged local DOF values (later
 // overwritten by the slot array), each run's start (u16, one past the last
 // local DOF too) and each local DOF's constraint flag (u8).
 __host__ __device__ constexpr int th_patch_lds_main(int max_patch_ldofs)
 {
-  return 8 * (2 * max_patch_ldofs > 15 * kThPatch ? 2 * max_patch_ldofs : 15 * kThPatch);
+  return 8 * (2 * max_patch_ldofs > 15 * kPatchLanes ? 2 * max_patch_ldofs : 15 * kPatchLanes);
 }
 __host__ __device__ constexpr int th_patch_lds(int max_patch_ldofs)
 {
   return (th_patch_lds_main(max_patch_ldofs) + 3 * max_patch_ldofs + 2 + 15) / 16 * 16;
 }
 
-// Local DOFs a lane handles per round of the staging and summing loops: their
-// loads are all issued before the first is used.
-constexpr int kThPatchUnroll = 8;
-
-__device__ __forceinline__ void th_vjp_patch_block(int p, const ThVjpArgs& a, const ThPatchArgs& q,
+__device__ __forceinline__ void th_vjp_patch_block(int p, const ThVjpArgs& a, const PatchArgs& q,
                                                    const ThTables& T, unsigned char* lds,
                                                    int max_patch_ldofs)
 {
   typedef unsigned u4 __attribute__((ext_vector_type(4)));
   typedef int      i3 __attribute__((ext_vector_type(3), aligned(4)));
-  constexpr int    U = kThPatchUnroll;
+  constexpr int    U = kPatchUnroll;
   const int        t  = threadIdx.x;
   const int        i0 = q.patch_off[p], ne = q.patch_off[p + 1] - i0;
   const int        l0 = q.ldof_off[p], nl = q.ldof_off[p + 1] - l0;
@@ -830,30 +810,32 @@ __device__ __forceinline__ void th_vjp_patch_block(int p, const ThVjpArgs& a, co
   // the patch's local DOF values, each gathered once (state, masked
   // adjoint), their constraint flags and the runs' starts
   if (t == 0) rs[0] = 0;
-  for (int j0 = t; j0 < nl; j0 += U * kThPatch)
+  for (int j0 = t; j0 < nl; j0 += U * kPatchLanes)
   {
-    int d[U], re[U];
+    int  d[U], re[U];
+    bool on[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)
     {
-      const int j = l0 + min(j0 + u * kThPatch, nl - 1);
-      d[u]        = q.ldof_dof[j];
-      re[u]       = q.ldof_end[j];
+      on[u]        = j0 + u * kPatchLanes < nl;
+      const int  j = l0 + j0 + u * kPatchLanes;
+      d[u]         = on[u] ? q.ldof_dof[j] : 0;
+      re[u]        = on[u] ? q.ldof_end[j] : 0;
     }
     double sd[U], ld[U];
     int    m[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)
     {
-      sd[u] = gat32(a.state, d[u]);
-      ld[u] = gat32(a.lam, d[u]);
-      m[u]  = a.bc_mask ? gat32(a.bc_mask, d[u]) : 0;
+      sd[u] = on[u] ? gat32(a.state, d[u]) : 0.0;
+      ld[u] = on[u] ? gat32(a.lam, d[u]) : 0.0;
+      m[u]  = on[u] && a.bc_mask ? gat32(a.bc_mask, d[u]) : 0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (j0 + u * kThPatch < nl)
+      if (j0 + u * kPatchLanes < nl)
       {
-        const int j = j0 + u * kThPatch;
+        const int j = j0 + u * kPatchLanes;
         sv[j]       = sd[u];
         lv[j]       = m[u] ? 0.0 : ld[u];
         rs[j + 1]   = static_cast<unsigned short>(re[u]);
@@ -891,69 +873,26 @@ __device__ __forceinline__ void th_vjp_patch_block(int p, const ThVjpArgs& a, co
     if (a.gout) __builtin_nontemporal_store(ge, a.gout + e);
   }
   __syncthreads();
-  for (int j0 = t; j0 < nl; j0 += U * kThPatch)
+  for (int j0 = t; j0 < nl; j0 += U * kPatchLanes)
   {
     int d[U], part[U];
 #pragma unroll
     for (int u = 0; u < U; ++u)
     {
-      const int j = l0 + min(j0 + u * kThPatch, nl - 1);
-      d[u]        = q.ldof_dof[j];
-      part[u]     = q.ldof_part[j];
+      const bool on = j0 + u * kPatchLanes < nl;
+      const int  j  = l0 + j0 + u * kPatchLanes;
+      d[u]          = on ? q.ldof_dof[j] : 0;
+      part[u]       = on ? q.ldof_part[j] : 0;
     }
 #pragma unroll
     for (int u = 0; u < U; ++u)
     {
-      const int j = j0 + u * kThPatch;
-      if (j >= nl) continue;
-      double    v = 0.0;
-      const int r1 = rs[j + 1];
-      for (int s = rs[j]; s < r1; ++s) v = v + cs[s];
-      // lam passes through on a constrained DOF: added by the patch that
-      // finishes it, or into the partial of its first patch
-      const bool pass = mk[j] && (part[u] < 0 || (part[u] & FX_TH_PART_FIRST));
-      if (pass) v = v + a.lam[d[u]];
-      if (part[u] < 0)
-        __builtin_nontemporal_store(v, q.jt + d[u]);
-      else
-        __builtin_nontemporal_store(v, q.parts + (part[u] & ~FX_TH_PART_FIRST));
+      const int j = j0 + u * kPatchLanes;
+      if (j < nl) patch_finish(cs, rs[j], rs[j + 1], d[u], part[u], mk[j] != 0, a.lam, q.jt, q.parts);
     }
   }
   __syncthreads(); // the LDS is reused by the next patch of a persistent walk
 }
-
-struct ThSeamArgs
-{
-  int           num_seam;
-  const int*    seam_off;
-  const int*    seam_dof;
-  const double* parts;
-  const double* lam;
-  const int*    bc_mask;
-  double*       jt;
-};
-
-// Seam pass (one lane per seam DOF): the DOF's per-patch partials in patch
-// order (lam's pass-through is in the first), the first four loaded at once;
-// a DOF in no element gets lam's pass-through alone.
-__device__ __forceinline__ void th_vjp_seam_block(int blk, const ThSeamArgs& a)
-{
-  const int s = blk * 256 + threadIdx.x;
-  if (s >= a.num_seam) return;
-  const int t0 = a.seam_off[s], t1 = a.seam_off[s + 1], d = a.seam_dof[s];
-  const int n  = t1 - t0;
-  double    p[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) p[u] = n > 0 ? __builtin_nontemporal_load(a.parts + t0 + min(u, n - 1)) : 0.0;
-  double v = 0.0;
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-    if (u < n) v = v + p[u];
-  for (int t = t0 + 4; t < t1; ++t) v = v + a.parts[t];
-  if (n == 0 && a.bc_mask && a.bc_mask[d]) v = a.lam[d];
-  __builtin_nontemporal_store(v, a.jt + d);
-}
-
 
 constexpr int kGatherStage = 4096;
 

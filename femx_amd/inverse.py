@@ -11,10 +11,12 @@ one, femx/assembly/BoundaryMap.hpp:198-208).  `apply_param_jac_t` is the
 (dR/dm)^T adj that ReducedFunctional::gradAt subtracts from dJ/dm
 (femx/inverse/ReducedFunctional.hpp:94-117).
 
-`ResidualVJP` exposes R(u, m) to torch autograd: its backward is one launch
-of fx_p1_vjp giving J^T lam and (dR/dm)^T lam together.
+`ResidualVJP` exposes R(u, m) to torch autograd: its backward is one pass
+of fx_p1_vjp_patch giving J^T lam and (dR/dm)^T lam together.
 """
 from __future__ import annotations
+
+import ctypes as C
 
 import torch
 
@@ -22,6 +24,7 @@ from . import _lib
 from .assembly import (AssemblyMap, BoundaryMap, Geometry, PoissonP1, assemble, make_assembly_map,
                        make_boundary_map, replace_res, replace_rows)
 from .linalg import CsrMatrix, _check_dev_vec, ptr, stream_handle
+from .patches import Patches
 
 
 class PoissonLogKappaResidual:
@@ -43,6 +46,7 @@ class PoissonLogKappaResidual:
         else:
             self.bc_vals = None
         self.f = f
+        self._patches = None
 
     def dims(self):
         return dict(num_states=self.num_states, num_res=self.num_res, num_params=self.num_params)
@@ -66,7 +70,8 @@ class PoissonLogKappaResidual:
         return out
 
     def vjp(self, state, prm, adj, jt_out=None, g_out=None, stream=None):
-        """(J^T adj, (dR/dm)^T adj) in one pass (fx_p1_vjp)."""
+        """(J^T adj, (dR/dm)^T adj) in one pass over element patches
+        (fx_p1_vjp_patch)."""
         g = self.geom
         dev = g.device
         _check_dev_vec(state, self.num_states, torch.float64, "VJP state")
@@ -77,11 +82,19 @@ class PoissonLogKappaResidual:
         if g_out is None:
             g_out = torch.empty(self.num_params, dtype=torch.float64, device=dev)
         mask = self.bmap.d_mask if self.bmap is not None else None
-        _lib.call("fx_p1_vjp", g.num_nodes, g.num_elems, ptr(g.x), ptr(g.y), ptr(g.conn), ptr(prm),
-                  ptr(state), ptr(adj), ptr(mask), ptr(self.map.device("inc_off", dev)),
-                  ptr(self.map.device("inc", dev)), ptr(jt_out), ptr(g_out),
-                  stream_handle(stream))
+        _lib.call("fx_p1_vjp_patch", g.num_nodes, g.num_elems, ptr(g.x), ptr(g.y), ptr(prm),
+                  ptr(state), ptr(adj), ptr(mask), C.byref(self.patches().plan), ptr(jt_out),
+                  ptr(g_out), stream_handle(stream))
         return jt_out, g_out
+
+    def patches(self):
+        """The VJP's patch plan on the geometry's device (built on first use;
+        VJPs of this residual on different streams must not overlap)."""
+        if self._patches is None:
+            g = self.geom
+            self._patches = Patches(g.num_nodes, g.num_elems, 3, g.h_conn, g.h_conn, g.h_x, g.h_y,
+                                    g.device)
+        return self._patches
 
     def apply_param_jac_t(self, state, prm, adj, out=None, stream=None):
         """out = (dR/dprm)^T adj (Residual::applyParamJacT)."""

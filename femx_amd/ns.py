@@ -21,6 +21,7 @@ from . import _lib
 from .assembly import (AssemblyMap, make_assembly_map, make_boundary_map, replace_res, replace_rows,
                        unit_square_edge_nodes)
 from .linalg import CsrMatrix, _check_dev_vec, _require, ptr, stream_handle
+from .patches import Patches
 
 TH_ITEMS = 32  # work items per block (FX_TH_BLOCK_ITEMS)
 TH_TILE_NNZ = 0  # CSR entries per block row tile, 0 = bounded by items (FX_TH_TILE_NNZ)
@@ -165,45 +166,10 @@ class THPlan:
 TH_PATCH = 256  # elements per VJP patch (FX_TH_PATCH_MAX)
 
 
-class THPatches:
-    """Patch plan of the single-pass VJP (fx_th_patches_create) on one
-    device, with its seam-partials workspace: VJPs that share it (one mesh)
-    on different streams must not overlap."""
-
-    def __init__(self, geom: "THGeometry", device, max_patch=TH_PATCH):
-        h = C.c_void_p()
-        _lib.call("fx_th_patches_create", geom.ndof, geom.num_elems, ptr(geom.h_dofs15),
-                  ptr(geom.h_vconn3), geom.num_vertices, ptr(geom.h_xv), ptr(geom.h_yv),
-                  int(max_patch), C.byref(h))
-        try:
-            info = _lib.PatchInfo()
-            _lib.call("fx_th_patches_info_get", h, C.byref(info))
-            P, L, S = info.num_patches, info.num_ldofs, info.num_seam
-            E = geom.num_elems
-            a = {"patch_off": np.empty(P + 1, np.int32), "patch_elem": np.empty(max(E, 1), np.int32),
-                 "elem_vert": np.empty(max(3 * E, 1), np.int32),
-                 "elem_ldof": np.empty(max(16 * E, 1), np.uint16),
-                 "ldof_off": np.empty(P + 1, np.int32), "ldof_dof": np.empty(max(L, 1), np.int32),
-                 "ldof_part": np.empty(max(L, 1), np.int32),
-                 "ldof_end": np.empty(max(L, 1), np.uint16),
-                 "seam_off": np.empty(S + 1, np.int32), "seam_dof": np.empty(max(S, 1), np.int32)}
-            _lib.call("fx_th_patches_export", h, *[ptr(a[k]) for k in (
-                "patch_off", "patch_elem", "elem_vert", "elem_ldof", "ldof_off", "ldof_dof",
-                "ldof_part", "ldof_end", "seam_off", "seam_dof")])
-        finally:
-            _lib.lib().fx_th_patches_destroy(h)
-        self.h = a
-        self.num_patches, self.num_ldofs, self.num_seam = P, L, S
-        self.num_parts, self.max_patch_ldofs = info.num_parts, info.max_patch_ldofs
-        dev = torch.device(device)
-        # uint16 tables travel as int16 bit patterns (torch has no uint16 copy path)
-        self.d = {k: torch.from_numpy(v.view(np.int16) if v.dtype == np.uint16 else v).to(dev)
-                  for k, v in a.items()}
-        self.parts = torch.empty(max(self.num_parts, 1), dtype=torch.float64, device=dev)
-        d = self.d
-        self.plan = _lib.PatchPlan(P, S, self.num_parts, self.max_patch_ldofs, *[ptr(d[k]) for k in (
-            "patch_off", "patch_elem", "elem_vert", "elem_ldof", "ldof_off", "ldof_dof",
-            "ldof_part", "ldof_end", "seam_off", "seam_dof")], ptr(self.parts))
+def THPatches(geom: "THGeometry", device, max_patch=TH_PATCH):
+    """Patch plan of the single-pass Taylor-Hood VJP (15 DOFs per element)."""
+    return Patches(geom.ndof, geom.num_elems, 15, geom.h_dofs15, geom.h_vconn3, geom.h_xv,
+                   geom.h_yv, device, max_patch)
 
 
 def _patches(geom, amap, device):

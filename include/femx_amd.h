@@ -329,6 +329,17 @@ int fx_p1_vjp(fx_index num_nodes, fx_index num_elems, const double* d_x, const d
               const fx_index* d_conn, const double* d_m, const double* d_u,
               const double* d_lam, const fx_index* d_bc_mask, const fx_index* d_inc_off,
               const fx_index* d_inc, double* d_jt_lam, double* d_g, void* stream);
+/* fx_p1_vjp in one pass over element patches: plan from fx_patches_create
+ * with the nodes as DOFs (dofs = vconn3 = conn, coordinates x/y), on the
+ * device as fx_patch_plan (declared below).  A node whose elements lie in one
+ * patch gets fx_p1_vjp's value bit for bit (its elements in ascending order);
+ * a node on a patch seam sums per-patch partials, so it may differ from
+ * fx_p1_vjp in the last bits.  g is bitwise equal to fx_p1_vjp's. */
+typedef struct fx_patch_plan fx_patch_plan;
+int fx_p1_vjp_patch(fx_index num_nodes, fx_index num_elems, const double* d_x, const double* d_y,
+                    const double* d_m, const double* d_u, const double* d_lam,
+                    const fx_index* d_bc_mask, const fx_patch_plan* h_plan, double* d_jt_lam,
+                    double* d_g, void* stream);
 
 /* ======================================================================== */
 /* Taylor-Hood P2/P1 steady incompressible Navier-Stokes (device).  Not in
@@ -381,47 +392,52 @@ int fx_th_vjp_noalloc(fx_index ndof, fx_index num_elems, const double* d_xv, con
                       const fx_index* d_bc_mask, const fx_index* d_inc_off,
                       const fx_index* d_elem_inc, double* d_jt, double* d_g, void* stream);
 
-/* Patch plan of fx_th_vjp_patch (host).  Elements are ordered along a Morton
- * curve of their centroids and cut into patches of at most max_patch
- * (<= FX_TH_PATCH_MAX) consecutive elements -- a patch also closes before an
- * element that would give one of its DOFs more than FX_TH_PATCH_RUN_MAX
- * elements in it -- ascending element id inside a patch.  A patch's local
- * DOFs (ascending id) each own a run of slots, one per element of the patch
- * holding the DOF, in element order (ldof_end: each run's end within the
- * patch).  elem_ldof[16 i + k] = j | r << 12 for local DOF k of the i-th
- * element in patch order: j its index among the patch's local DOFs, r the
- * element's rank in j's run (entry 15 unused); elem_vert[3 i + k] is vertex
- * k of that element.  A DOF held by one patch only is finished by it
- * (ldof_part -1); the others (on seams between patches, or in no element)
- * are the seam DOFs, summed from per-patch partials in patch order:
- * ldof_part is the partial's index, with FX_TH_PART_FIRST set on the DOF's
- * first patch (which adds lam's pass-through on a constrained DOF).
+/* Patch plan of the single-pass VJPs (host): fx_th_vjp_patch (Taylor-Hood,
+ * 15 DOFs per element) and fx_p1_vjp_patch (P1, 3).  Elements are ordered
+ * along a Morton curve of their centroids (vertex ids vconn3 into xv/yv) and
+ * cut into patches of at most max_patch (<= FX_TH_PATCH_MAX) consecutive
+ * elements -- a patch also closes before an element that would give one of
+ * its DOFs more than FX_TH_PATCH_RUN_MAX elements in it -- ascending element
+ * id inside a patch.  A patch's local DOFs (ascending id, at most 4095) each
+ * own a run of slots, one per element of the patch holding the DOF, in
+ * element order (ldof_end: each run's end within the patch).
+ * elem_ldof[S i + k] = j | r << 12 for local DOF k of the i-th element in
+ * patch order, S = ldof_stride (dofs_per_elem rounded up to a multiple of
+ * 4; the tail entries are unused): j its index among the patch's local DOFs,
+ * r the element's rank in j's run; elem_vert[3 i + k] is vertex k of that
+ * element.  A DOF held by one patch only is finished by it (ldof_part -1);
+ * the others (on seams between patches, or in no element) are the seam
+ * DOFs, summed from per-patch partials in patch order: ldof_part is the
+ * partial's index, with FX_TH_PART_FIRST set on the DOF's first patch
+ * (which adds lam's pass-through on a constrained DOF).
  * Array sizes: patch_off num_patches+1, patch_elem num_elems, elem_vert
- * 3*num_elems, elem_ldof 16*num_elems, ldof_off num_patches+1,
+ * 3*num_elems, elem_ldof ldof_stride*num_elems, ldof_off num_patches+1,
  * ldof_dof/ldof_part/ldof_end num_ldofs, seam_off num_seam+1, seam_dof
  * num_seam. */
 #define FX_TH_PATCH_MAX 256
 #define FX_TH_PATCH_RUN_MAX 16
 #define FX_TH_PART_FIRST (1 << 30)
-typedef struct fx_th_patches fx_th_patches;
+typedef struct fx_patches fx_patches;
 typedef struct
 {
   fx_index num_patches, num_ldofs, num_seam, num_parts;
   fx_index max_patch_ldofs; /* the most local DOFs of one patch */
-} fx_th_patches_info;
-int fx_th_patches_create(fx_index ndof, fx_index num_elems, const fx_index* h_dofs15,
-                         const fx_index* h_vconn3, fx_index num_vertices, const double* h_xv,
-                         const double* h_yv, fx_index max_patch, fx_th_patches** out);
-int fx_th_patches_info_get(const fx_th_patches* patches, fx_th_patches_info* info);
-int fx_th_patches_export(const fx_th_patches* patches, fx_index* h_patch_off,
-                         fx_index* h_patch_elem, fx_index* h_elem_vert,
-                         unsigned short* h_elem_ldof, fx_index* h_ldof_off, fx_index* h_ldof_dof, fx_index* h_ldof_part,
-                         unsigned short* h_ldof_end, fx_index* h_seam_off, fx_index* h_seam_dof);
-void fx_th_patches_destroy(fx_th_patches* patches);
+  fx_index ldof_stride;     /* elem_ldof entries per element */
+} fx_patches_info;
+int fx_patches_create(fx_index ndof, fx_index num_elems, fx_index dofs_per_elem,
+                      const fx_index* h_dofs, const fx_index* h_vconn3, fx_index num_vertices,
+                      const double* h_xv, const double* h_yv, fx_index max_patch,
+                      fx_patches** out);
+int fx_patches_info_get(const fx_patches* patches, fx_patches_info* info);
+int fx_patches_export(const fx_patches* patches, fx_index* h_patch_off, fx_index* h_patch_elem,
+                      fx_index* h_elem_vert, unsigned short* h_elem_ldof, fx_index* h_ldof_off,
+                      fx_index* h_ldof_dof, fx_index* h_ldof_part, unsigned short* h_ldof_end,
+                      fx_index* h_seam_off, fx_index* h_seam_dof);
+void fx_patches_destroy(fx_patches* patches);
 
 /* The patch plan's arrays on the device, and the seam partials workspace
  * (num_parts doubles). */
-typedef struct
+struct fx_patch_plan
 {
   fx_index              num_patches, num_seam, num_parts, max_patch_ldofs;
   const fx_index*       patch_off;
@@ -435,7 +451,7 @@ typedef struct
   const fx_index*       seam_off;
   const fx_index*       seam_dof;
   double*               parts;
-} fx_th_patch_plan;
+};
 
 /* fx_th_vjp in one pass over element patches: a workgroup loads its
  * patch's local DOF values (state, masked adjoint) into LDS once, forms the
@@ -448,7 +464,7 @@ typedef struct
 int fx_th_vjp_patch(fx_index ndof, fx_index num_elems, const double* d_xv, const double* d_yv,
                     const double* d_state, double nu,
                     const double* d_nu_e, const double* d_lam,
-                    const fx_index* d_bc_mask, const fx_th_patch_plan* h_plan, double* d_jt,
+                    const fx_index* d_bc_mask, const fx_patch_plan* h_plan, double* d_jt,
                     double* d_g, void* stream);
 
 /* ======================================================================== */
@@ -500,10 +516,11 @@ int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
  * assembled and differentiated by persistent launches over one work list
  * per phase, optionally captured in a hipGraph and replayed with one launch.
  * femx has no batching; each instance's results are bitwise equal to its
- * single-instance calls (fx_p1_assemble with kappa_is_log = 1 and fx_p1_vjp;
- * fx_th_assemble and fx_th_vjp).  Instance arrays are caller-owned device
- * memory and must outlive the batch.  VJP pointers (lam, jt, g, work,
- * elem_inc) may be NULL to batch assembly only. */
+ * single-instance calls (fx_p1_assemble with kappa_is_log = 1 and
+ * fx_p1_vjp_patch; fx_th_assemble and fx_th_vjp_patch).  Instance arrays are
+ * caller-owned device memory and must outlive the batch.  VJP pointers (lam,
+ * jt, g, the patch plan) may be NULL (an empty plan) to batch assembly
+ * only. */
 typedef struct
 {
   fx_index      num_nodes, num_elems, nnz, max_row_len;
@@ -522,6 +539,7 @@ typedef struct
   const fx_index* bc_mask;
   double*       jt;
   double*       g;
+  fx_patch_plan vjp; /* the VJP's patch plan (fx_p1_vjp_patch) */
 } fx_p1_instance;
 
 typedef struct
@@ -547,7 +565,7 @@ typedef struct
   double*       jt;
   double*       g;
   const fx_index* blk_lanes; /* fx_map_th_lanes, or NULL for two lanes per item */
-  fx_th_patch_plan vjp; /* the VJP's patch plan (fx_th_vjp_patch) */
+  fx_patch_plan vjp; /* the VJP's patch plan (fx_th_vjp_patch) */
 } fx_th_instance;
 
 typedef struct fx_batch fx_batch;
@@ -563,9 +581,10 @@ enum
  * workgroup per work item.  Copies the descriptors to the device. */
 int fx_batch_create(fx_index num_p1, const fx_p1_instance* h_p1, fx_index num_th,
                     const fx_th_instance* h_th, fx_index grid_per_cu, fx_batch** out);
-/* Work items per phase (P1 assembly, TH assembly, P1 VJP, TH VJP element,
- * TH VJP gather) and the persistent grid size. */
-int fx_batch_work(const fx_batch* batch, fx_index* h_counts5, fx_index* h_grid);
+/* Work items per phase (P1 assembly, TH assembly, P1 VJP patches, TH VJP
+ * patches, TH VJP seam blocks, P1 VJP seam blocks) and the persistent grid
+ * size. */
+int fx_batch_work(const fx_batch* batch, fx_index* h_counts6, fx_index* h_grid);
 /* Launch the phases in `what` (FX_BATCH_ASSEMBLE | FX_BATCH_VJP) ordered
  * after earlier work on stream, and make later work on stream wait for
  * them.  The independent chains (P1 assembly, TH assembly, P1 VJP, TH VJP)
