@@ -114,8 +114,12 @@ def build_gls(n, dev, t):
     gls.assemble(op, 1, 2, gls.NEXT_STATE, amap, d_hist, d_nxt, res, jac)
     d_adj = t(np.random.default_rng(14).standard_normal(nd))
     vjp_out = torch.empty(nd, dtype=torch.float64, device=dev)
+    vjp_out1 = torch.empty(nd, dtype=torch.float64, device=dev)
     return {"hist_vjp": lambda: gls.hist_vjp(op, 1, 2, 0, amap, d_hist, d_nxt, d_adj, vjp_out),
-            "vjp_alg_bytes": gls_vjp_alg_bytes(geom.num_nodes, geom.num_elems),"assemble": lambda: gls.assemble(op, 1, 2, gls.NEXT_STATE, amap, d_hist, d_nxt, res,
+            "hist_vjp_lags": lambda: gls.hist_vjp_lags(op, 1, 2, amap, d_hist, d_nxt, d_adj,
+                                                       vjp_out, vjp_out1),
+            "vjp_alg_bytes": gls_vjp_alg_bytes(geom.num_nodes, geom.num_elems),
+            "assemble": lambda: gls.assemble(op, 1, 2, gls.NEXT_STATE, amap, d_hist, d_nxt, res,
                                              jac),
             "residual": lambda: gls.assemble_residual(op, 1, 2, amap, d_hist, d_nxt, res),
             "E": geom.num_elems, "N": geom.num_nodes, "states": nd, "nnz": amap.pattern.nnz,
@@ -409,6 +413,7 @@ def main():
         t_gl = time_kernel(gl["assemble"], reps)
         t_glr = time_kernel(gl["residual"], reps)
         t_glv = time_kernel(gl["hist_vjp"], reps)
+        t_glv2 = time_kernel(gl["hist_vjp_lags"], reps)
         inc = 3 * gl["E"]
         fl = GLS_FLOPS_PER_INCIDENCE * inc
         extra["gls_ns_%d" % n] = {
@@ -421,7 +426,8 @@ def main():
             "frac_fp64": fl / t_gl / 1e12 / FP64_PEAK_TFLOPS,
             "hist_vjp_us": t_glv * 1e6, "hist_vjp_elements_per_s": gl["E"] / t_glv,
             "hist_vjp_alg_bytes": gl["vjp_alg_bytes"],
-            "hist_vjp_GBs": gl["vjp_alg_bytes"] / t_glv / 1e9}
+            "hist_vjp_GBs": gl["vjp_alg_bytes"] / t_glv / 1e9,
+            "hist_vjp_both_lags_us": t_glv2 * 1e6}
         if not args.no_cpu:
             from oracle import oracle as O
             ns_ = 256  # bounded sample: the 256^2 mesh of the same problem

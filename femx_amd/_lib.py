@@ -65,6 +65,8 @@ _SIGS = {
     "fx_vec_scatter": ([_i32, _vp, _vp, _vp, _i32, _vp], C.c_int),
     "fx_gls_hist_vjp": ([_i32] * 3 + [_vp] * 5 + [_f64] * 3 + [_i32] * 3 + [_vp] * 7 + [_vp],
                         C.c_int),
+    "fx_gls_hist_vjp_patch": ([_i32] * 3 + [_vp] * 3 + [_f64] * 3 + [_i32] * 2 + [_vp] * 6 + [_vp],
+                              C.c_int),
     "fx_gls_assemble": ([_i32] * 5 + [_vp] * 5 + [_f64] * 3 + [_i32] * 2 + [_vp] * 9 + [_vp],
                         C.c_int),
     "fx_th_vjp": ([_i32] * 2 + [_vp] * 5 + [_f64] + [_vp] * 8 + [_vp], C.c_int),

@@ -35,6 +35,19 @@ __global__ __launch_bounds__(kGlsSumNodes) void k_gls_hist_vjp_sum(GlsVjpArgs a)
   gls_hist_vjp_sum_block(fx::xcd_block(gridDim.x), a, sw);
 }
 
+__global__ __launch_bounds__(kPatchLanes) void k_gls_hist_vjp_patch(GlsVjpArgs a, PatchArgs q,
+                                                                    GlsPatchOut o,
+                                                                    int max_patch_ldofs)
+{
+  extern __shared__ __align__(16) unsigned char lds[];
+  gls_vjp_patch_block(fx::xcd_block(gridDim.x), a, q, o, lds, max_patch_ldofs);
+}
+
+__global__ __launch_bounds__(256) void k_gls_hist_vjp_seam(GlsVjpArgs a, SeamArgs s, GlsPatchOut o)
+{
+  gls_vjp_seam_block(fx::xcd_block(gridDim.x), a, s, o);
+}
+
 } // namespace
 
 extern "C" {
@@ -147,6 +160,74 @@ int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
     hipLaunchKernelGGL(k_gls_hist_vjp_sum, dim3(fx::blocks_for(num_nodes, kGlsSumNodes)),
                        dim3(kGlsSumNodes), 0, fx::as_stream(stream), a);
     fx::check_launch("fx_gls_hist_vjp");
+  });
+}
+
+int fx_gls_hist_vjp_patch(fx_index num_nodes, fx_index num_states, fx_index num_elems,
+                          const double* x, const double* y, const fx_index* node_rows, double rho,
+                          double mu, double dt, fx_index step, fx_index num_hist,
+                          const double* hist, const double* nxt, const double* adj,
+                          const fx_patch_plan* plan, double* out_lag0, double* out_lag1,
+                          void* stream)
+{
+  return fx::guard([&] {
+    fx::require(num_nodes >= 0 && num_states >= 0 && num_elems >= 0 && plan, FX_ERR_ARG,
+                "fx_gls_hist_vjp_patch: negative size or null plan");
+    fx::require(dt > 0.0 && std::isfinite(dt), FX_ERR_ARG,
+                "NavierStokesModel requires a positive finite time step");
+    fx::require(std::isfinite(rho) && rho > 0.0, FX_ERR_ARG,
+                "NavierStokesModel requires positive finite density");
+    fx::require(std::isfinite(mu) && mu > 0.0, FX_ERR_ARG,
+                "NavierStokesModel requires positive finite viscosity");
+    // checkTimeAssemblyInputs' variable-block rule (femx/assembly/Assembly.hpp:100-103)
+    fx::require(num_hist > 0 && (out_lag0 || out_lag1) && (!out_lag1 || num_hist > 1), FX_ERR_ARG,
+                "Time assembly variable block is invalid");
+    const fx_patch_plan& q = *plan;
+    fx::require(q.num_patches >= 0 && q.num_seam >= 0 && q.num_parts >= 0
+                    && q.max_patch_ldofs >= 0 && q.max_patch_ldofs <= 3 * kPatchLanes
+                    && (q.num_patches > 0) == (num_elems > 0),
+                FX_ERR_ARG, "fx_gls_hist_vjp_patch: plan sizes do not fit the mesh");
+    if (num_states == 0) return;
+    fx::require(fx::fits32(num_states, 8 * num_hist) && fx::fits32(num_nodes, 12)
+                    && fx::fits32(q.num_parts, 32),
+                FX_ERR_OVERFLOW, "fx_gls_hist_vjp_patch: arrays beyond 4 GiB");
+    fx::require(num_nodes > 0 && x && y && node_rows && hist && nxt && adj && q.patch_off
+                    && q.ldof_off && q.seam_off
+                    && (num_elems == 0
+                        || (q.patch_elem && q.elem_ldof && q.ldof_dof && q.ldof_part && q.ldof_end))
+                    && (q.num_seam == 0 || q.seam_dof) && (q.num_parts == 0 || q.parts),
+                FX_ERR_ARG, "fx_gls_hist_vjp_patch: null array");
+    const std::size_t S = std::size_t(num_states) * 8, H = S * num_hist;
+    const std::size_t W = std::size_t(q.num_parts) * 32;
+    for (double* out : {out_lag0, out_lag1})
+      fx::require(!out || (!fx::overlaps(out, S, hist, H) && !fx::overlaps(out, S, nxt, S)
+                           && !fx::overlaps(out, S, adj, S) && !fx::overlaps(q.parts, W, out, S)),
+                  FX_ERR_ALIAS, "Navier history VJP output must not alias its inputs");
+    fx::require(!(out_lag0 && out_lag1 && fx::overlaps(out_lag0, S, out_lag1, S))
+                    && !fx::overlaps(q.parts, W, hist, H) && !fx::overlaps(q.parts, W, nxt, S)
+                    && !fx::overlaps(q.parts, W, adj, S),
+                FX_ERR_ALIAS, "Navier history VJP output must not alias its inputs");
+    const GlsVjpArgs  a{num_nodes, num_states, x,       y,        nullptr, nullptr,
+                       node_rows, step,       num_hist, 0,        hist,    nxt,
+                       adj,       gls_const(rho, mu, dt), nullptr, nullptr, nullptr, nullptr,
+                       num_elems};
+    const GlsPatchOut o{out_lag0, out_lag1};
+    if (q.num_patches > 0)
+    {
+      const PatchArgs pa{q.patch_off, q.patch_elem, q.elem_vert, q.elem_ldof, q.ldof_off,
+                         q.ldof_dof,  q.ldof_part,  q.ldof_end,  nullptr,     q.parts};
+      const int       mx = static_cast<int>(q.max_patch_ldofs);
+      hipLaunchKernelGGL(k_gls_hist_vjp_patch, dim3(static_cast<unsigned>(q.num_patches)),
+                         dim3(kPatchLanes), gls_patch_lds(mx), fx::as_stream(stream), a, pa, o, mx);
+      fx::check_launch("fx_gls_hist_vjp_patch");
+    }
+    if (q.num_seam > 0)
+    {
+      const SeamArgs sa{q.num_seam, q.seam_off, q.seam_dof, q.parts, nullptr, nullptr, nullptr};
+      hipLaunchKernelGGL(k_gls_hist_vjp_seam, dim3(fx::blocks_for(q.num_seam, 256)), dim3(256), 0,
+                         fx::as_stream(stream), a, sa, o);
+      fx::check_launch("fx_gls_hist_vjp_patch");
+    }
   });
 }
 

@@ -813,4 +813,200 @@ __device__ __forceinline__ void gls_hist_vjp_sum_block(int blk, const GlsVjpArgs
   __builtin_nontemporal_store(0.0, a.out + a.node_rows[3 * n + 2]);
 }
 
+
+// ---------------------------------------------------------------------------
+// Both lags' history VJPs in one pass over element patches
+// (fx_patches_create with the nodes as DOFs).  A workgroup stages its
+// patch's local node values -- coordinates, the two velocity entries of
+// lags 0 and 1, the three entries of the next state and of the adjoint --
+// in LDS with one gather each; one lane per element evaluates the partials
+// once (gls_hvjp_nodes, as the two-phase form) and stores its three nodes'
+// columns of both lags into the nodes' runs (the staged values' LDS, after a
+// barrier); the runs are summed in element order.  A node inside one patch
+// gets the two-phase VJP's sums bit for bit; a seam node's per-patch
+// partials (four values: lag 0 x, y, lag 1 x, y) are summed by
+// gls_vjp_seam_block in patch order.
+struct GlsPatchOut
+{
+  double* out0; // lag 0, or null
+  double* out1; // lag 1, or null
+};
+
+constexpr int kGlsNodeVals = 12; // x, y, cur (2), prev (2), nxt (3), adj (3)
+
+__host__ __device__ constexpr int gls_patch_lds_main(int max_patch_ldofs)
+{
+  return 8 * (kGlsNodeVals * max_patch_ldofs > 12 * kPatchLanes ? kGlsNodeVals * max_patch_ldofs
+                                                                 : 12 * kPatchLanes);
+}
+__host__ __device__ constexpr int gls_patch_lds(int max_patch_ldofs)
+{
+  return (gls_patch_lds_main(max_patch_ldofs) + 2 * max_patch_ldofs + 2 + 15) / 16 * 16;
+}
+
+__device__ __forceinline__ void gls_vjp_write(const GlsVjpArgs& a, const GlsPatchOut& o, int n,
+                                              const double (&v)[4])
+{
+  const int r0 = a.node_rows[3 * n], r1 = a.node_rows[3 * n + 1], r2 = a.node_rows[3 * n + 2];
+  if (o.out0)
+  {
+    __builtin_nontemporal_store(v[0], o.out0 + r0);
+    __builtin_nontemporal_store(v[1], o.out0 + r1);
+    __builtin_nontemporal_store(0.0, o.out0 + r2);
+  }
+  if (o.out1)
+  {
+    __builtin_nontemporal_store(v[2], o.out1 + r0);
+    __builtin_nontemporal_store(v[3], o.out1 + r1);
+    __builtin_nontemporal_store(0.0, o.out1 + r2);
+  }
+}
+
+__device__ __forceinline__ void gls_vjp_patch_block(int p, const GlsVjpArgs& a, const PatchArgs& q,
+                                                    const GlsPatchOut& o, unsigned char* lds,
+                                                    int max_patch_ldofs)
+{
+  constexpr int   U  = 3; // a patch has at most 3 * kPatchLanes local nodes
+  const int       t  = threadIdx.x;
+  const int       i0 = q.patch_off[p], ne = q.patch_off[p + 1] - i0;
+  const int       l0 = q.ldof_off[p], nl = q.ldof_off[p + 1] - l0;
+  const bool      mine   = t < ne;
+  const int       i      = i0 + t;
+  const bool      extrap = a.step > 0 && a.num_hist > 1;
+  double*         sv     = reinterpret_cast<double*>(lds); // node values, then the slots
+  unsigned short* rs = reinterpret_cast<unsigned short*>(lds + gls_patch_lds_main(max_patch_ldofs));
+  unsigned        w0 = 0, w1 = 0;
+  if (mine)
+  {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    const u2 lw = __builtin_nontemporal_load(reinterpret_cast<const u2*>(q.elem_ldof + 4 * static_cast<long>(i)));
+    w0 = lw.x, w1 = lw.y;
+  }
+  if (t == 0) rs[0] = 0;
+  for (int j0 = t; j0 < nl; j0 += U * kPatchLanes)
+  {
+    int  n[U], re[U], r[U][3];
+    bool on[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      on[u]       = j0 + u * kPatchLanes < nl;
+      const int j = l0 + j0 + u * kPatchLanes;
+      n[u]        = on[u] ? q.ldof_dof[j] : 0;
+      re[u]       = on[u] ? q.ldof_end[j] : 0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) r[u][k] = on[u] ? a.node_rows[3 * n[u] + k] : 0;
+    double v[U][kGlsNodeVals];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      v[u][0] = on[u] ? gat32(a.x, n[u]) : 0.0;
+      v[u][1] = on[u] ? gat32(a.y, n[u]) : 0.0;
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+      {
+        v[u][2 + c] = on[u] ? gat32(a.hist, r[u][c]) : 0.0;
+        v[u][4 + c] = on[u] && extrap ? gat32(a.hist + a.num_states, r[u][c]) : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+      {
+        v[u][6 + k] = on[u] ? gat32(a.nxt, r[u][k]) : 0.0;
+        v[u][9 + k] = on[u] ? gat32(a.adj, r[u][k]) : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (on[u])
+      {
+        const int j = j0 + u * kPatchLanes;
+#pragma unroll
+        for (int k = 0; k < kGlsNodeVals; ++k) sv[kGlsNodeVals * j + k] = v[u][k];
+        rs[j + 1] = static_cast<unsigned short>(re[u]);
+      }
+  }
+  __syncthreads();
+  double o0[3], o1[3], p0[3], p1[3];
+  if (mine)
+  {
+    const unsigned h[3] = {w0 & 0xffffu, w0 >> 16, w1 & 0xffffu};
+    double         xe[3], ye[3], cur[6], prev[6], nx[9], aj[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+      const double* nv = sv + kGlsNodeVals * (h[k] & 0xfffu);
+      xe[k] = nv[0], ye[k] = nv[1];
+      cur[2 * k] = nv[2], cur[2 * k + 1] = nv[3];
+      prev[2 * k] = nv[4], prev[2 * k + 1] = nv[5];
+      // next state and adjoint in the element DOF order: velocity (2k, 2k+1),
+      // pressure 6+k
+      nx[2 * k] = nv[6], nx[2 * k + 1] = nv[7], nx[6 + k] = nv[8];
+      aj[2 * k] = nv[9], aj[2 * k + 1] = nv[10], aj[6 + k] = nv[11];
+    }
+    const double j00 = xe[1] - xe[0], j01 = xe[2] - xe[0], j10 = ye[1] - ye[0], j11 = ye[2] - ye[0];
+    const double det = j00 * j11 - j01 * j10, inv_det = 1.0 / det;
+    const double i00 = j11 * inv_det, i01 = -j01 * inv_det, i10 = -j10 * inv_det, i11 = j00 * inv_det;
+    const double gx[3] = {-i00 - i10, i00, i10}, gy[3] = {-i01 - i11, i01, i11};
+    const double jw    = fabs(det) * (1.0 / 6.0);
+    const int    ks[3] = {0, 1, 2};
+    gls_hvjp_nodes<3>(ks, gx, gy, jw, cur, prev, extrap, nx, aj, a.c, o0, o1, p0, p1);
+  }
+  __syncthreads(); // every lane has read the staged values: the slots overwrite them
+  double* cs = sv;
+  if (mine)
+  {
+    const unsigned h[3] = {w0 & 0xffffu, w0 >> 16, w1 & 0xffffu};
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+    {
+      double* sl = cs + 4 * (rs[h[k] & 0xfffu] + (h[k] >> 12));
+      sl[0] = o0[k], sl[1] = o1[k], sl[2] = p0[k], sl[3] = p1[k];
+    }
+  }
+  __syncthreads();
+  for (int j = t; j < nl; j += kPatchLanes)
+  {
+    const int n = q.ldof_dof[l0 + j], part = q.ldof_part[l0 + j];
+    double    v[4] = {0.0, 0.0, 0.0, 0.0};
+    const int r1   = rs[j + 1];
+    for (int s = rs[j]; s < r1; ++s)
+    {
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[c] += cs[4 * s + c];
+    }
+    if (part < 0)
+      gls_vjp_write(a, o, n, v);
+    else
+    {
+      double* pp = q.parts + 4 * static_cast<long>(part & ~FX_TH_PART_FIRST);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) __builtin_nontemporal_store(v[c], pp + c);
+    }
+  }
+  __syncthreads(); // the LDS is reused by the next patch of a persistent walk
+}
+
+// Seam pass of the GLS history VJP (one lane per seam node): the node's
+// per-patch partials (4 values each) in patch order.  A node in no element
+// has no rows and is skipped.
+__device__ __forceinline__ void gls_vjp_seam_block(int blk, const GlsVjpArgs& a, const SeamArgs& s,
+                                                   const GlsPatchOut& o)
+{
+  const int k = blk * 256 + threadIdx.x;
+  if (k >= s.num_seam) return;
+  const int n = s.seam_dof[k];
+  if (a.node_rows[3 * n] < 0) return;
+  double    v[4] = {0.0, 0.0, 0.0, 0.0};
+  const int t1   = s.seam_off[k + 1];
+  for (int t = s.seam_off[k]; t < t1; ++t)
+  {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[c] += s.parts[4 * static_cast<long>(t) + c];
+  }
+  gls_vjp_write(a, o, n, v);
+}
+
 } // namespace fxk

@@ -28,6 +28,7 @@ import torch
 from . import _lib
 from .assembly import AssemblyMap, make_assembly_map
 from .linalg import CsrMatrix, _check_dev_vec, _require, ptr, stream_handle
+from .patches import Patches
 
 BLOCK_NODES = 32  # FX_GLS_BLOCK_NODES
 MAX_RUNS = 4  # FX_GLS_MAX_RUNS
@@ -188,15 +189,11 @@ def assemble(op: NavierOperator, step, num_hist, wrt, amap: AssemblyMap, hist, n
     return res
 
 
-def hist_vjp(op: NavierOperator, step, num_hist, lag, amap: AssemblyMap, hist, nxt, adj,
-             out=None, stream=None):
-    """(dR/dhist_lag)^T adj: TimeResidual::applyJacT with respect to a history
-    block (femx/model/ns/NavierStokesModel.cpp:321-363), the product femx
-    takes with Enzyme, here hand-derived.  Returns out (allocated when None).
-    Element columns go to the plan's workspace (6 doubles per element,
-    allocated on first use), then are summed per node.  Calls sharing a plan
-    share its workspace: history VJPs of one map on different streams must
-    not overlap."""
+def hist_vjp_two_phase(op: NavierOperator, step, num_hist, lag, amap: AssemblyMap, hist, nxt,
+                       adj, out=None, stream=None):
+    """hist_vjp through fx_gls_hist_vjp: element columns go to the plan's
+    workspace (6 doubles per element, allocated on first use), then are
+    summed per node in ascending element order."""
     geom = op.geom
     _require(num_hist > 0 and 0 <= lag < num_hist, "Time assembly variable block is invalid")
     _check_dev_vec(hist, num_hist * geom.num_states, torch.float64, "History VJP history")
@@ -216,6 +213,69 @@ def hist_vjp(op: NavierOperator, step, num_hist, lag, amap: AssemblyMap, hist, n
               ptr(adj), ptr(p.inc_off), ptr(p.inc_slot), ptr(out), ptr(work),
               stream_handle(stream))
     return out
+
+
+def _patches(amap, geom):
+    """The history VJP's patch plan (mesh nodes as DOFs, four doubles per
+    seam partial), cached with the map's GLS plan."""
+    p = _plan(amap, geom)
+    if getattr(p, "vjp_patches", None) is None:
+        p.vjp_patches = Patches(geom.num_nodes, geom.num_elems, 3, geom.h_conn, geom.h_conn,
+                                geom.h_x, geom.h_y, geom.device, part_width=4)
+    return p.vjp_patches
+
+
+def hist_vjp_lags(op: NavierOperator, step, num_hist, amap: AssemblyMap, hist, nxt, adj,
+                  out_lag0=None, out_lag1=None, stream=None):
+    """Both history lags' VJPs, (dR/dhist_0)^T adj and (dR/dhist_1)^T adj, in
+    one pass over element patches (fx_gls_hist_vjp_patch): the element
+    partials are evaluated once for both.  Lag 1 needs num_hist > 1; pass
+    out_lag1=False to skip it.  Returns (out_lag0, out_lag1).  Calls sharing
+    a map share its partials workspace: they must not overlap on different
+    streams."""
+    geom = op.geom
+    _require(num_hist > 0, "Time assembly variable block is invalid")
+    for v, n, what in ((hist, num_hist * geom.num_states, "History VJP history"),
+                       (nxt, geom.num_states, "History VJP next state"),
+                       (adj, geom.num_states, "History VJP adjoint")):
+        _check_dev_vec(v, n, torch.float64, what)
+    want1 = out_lag1 is not False and num_hist > 1
+    outs = []
+    for o, want in ((out_lag0, out_lag0 is not False), (out_lag1 if want1 else None, want1)):
+        if want and (o is None or o is True):
+            o = torch.empty(geom.num_states, dtype=torch.float64, device=geom.device)
+        if want:
+            _check_dev_vec(o, geom.num_states, torch.float64, "History VJP output")
+        outs.append(o if want else None)
+    pp = _patches(amap, geom)
+    p = _plan(amap, geom)  # every state row belongs to a mesh node (fx_map_gls_plan)
+    _lib.call("fx_gls_hist_vjp_patch", geom.num_nodes, geom.num_states, geom.num_elems,
+              ptr(geom.x), ptr(geom.y), ptr(p.node_rows), op.fluid.rho, op.fluid.mu, op.dt,
+              int(step), int(num_hist), ptr(hist), ptr(nxt), ptr(adj), C.byref(pp.plan),
+              ptr(outs[0]), ptr(outs[1]), stream_handle(stream))
+    return outs[0], outs[1]
+
+
+def hist_vjp(op: NavierOperator, step, num_hist, lag, amap: AssemblyMap, hist, nxt, adj,
+             out=None, stream=None):
+    """(dR/dhist_lag)^T adj: TimeResidual::applyJacT with respect to a history
+    block (femx/model/ns/NavierStokesModel.cpp:321-363), the product femx
+    takes with Enzyme, here hand-derived.  Returns out (allocated when None).
+    Lags 0 and 1 go through the single-pass patch VJP (hist_vjp_lags with the
+    other lag skipped); an older lag does not enter the operator and gets
+    zeros."""
+    geom = op.geom
+    _require(num_hist > 0 and 0 <= lag < num_hist, "Time assembly variable block is invalid")
+    if lag >= 2:
+        if out is None:
+            out = torch.empty(geom.num_states, dtype=torch.float64, device=geom.device)
+        _check_dev_vec(out, geom.num_states, torch.float64, "History VJP output")
+        with torch.cuda.stream(stream if stream is not None else torch.cuda.current_stream()):
+            out.zero_()
+        return out
+    if lag == 0:
+        return hist_vjp_lags(op, step, num_hist, amap, hist, nxt, adj, out, False, stream)[0]
+    return hist_vjp_lags(op, step, num_hist, amap, hist, nxt, adj, False, out, stream)[1]
 
 
 def assemble_residual(op: NavierOperator, step, num_hist, amap: AssemblyMap, hist, nxt, res,
@@ -281,6 +341,14 @@ class NavierResidual:
         _require(0 <= lag < self.NUM_HIST, "Navier residual history lag is out of range")
         return hist_vjp(self.op, time.step, self.NUM_HIST, lag, self.map, self._hist(time),
                         time.nxt, adj, out, stream)
+
+    def apply_jac_t_lags(self, time, adj, out_lag0=None, out_lag1=None, stream=None):
+        """apply_jac_t for both history lags at once (one pass over the
+        mesh, hist_vjp_lags): (dR/dhist_0)^T adj, (dR/dhist_1)^T adj."""
+        self._check(time)
+        _require(adj.numel() == self.geom.num_states, "Navier residual adjoint size mismatch")
+        return hist_vjp_lags(self.op, time.step, self.NUM_HIST, self.map, self._hist(time),
+                             time.nxt, adj, out_lag0, out_lag1, stream)
 
     def prepare_linear_solve(self, time, jac, rhs, stream=None):
         """TimeResidual's default: nothing to do (TimeResidual.hpp:229-238)."""

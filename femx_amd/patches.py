@@ -22,7 +22,7 @@ class Patches:
     share it (one mesh) on different streams must not overlap."""
 
     def __init__(self, ndof, num_elems, dofs_per_elem, h_dofs, h_vconn3, h_xv, h_yv, device,
-                 max_patch=256):
+                 max_patch=256, part_width=1):
         dofs = np.ascontiguousarray(h_dofs, np.int32)
         vconn = np.ascontiguousarray(h_vconn3, np.int32)
         xv = np.ascontiguousarray(h_xv, np.float64)
@@ -52,7 +52,9 @@ class Patches:
         # uint16 tables travel as int16 bit patterns (torch has no uint16 copy path)
         self.d = {k: torch.from_numpy(v.view(np.int16) if v.dtype == np.uint16 else v).to(dev)
                   for k, v in a.items()}
-        self.parts = torch.empty(max(self.num_parts, 1), dtype=torch.float64, device=dev)
+        # part_width doubles per seam partial (4 for the GLS history VJP's two lags)
+        self.parts = torch.empty(max(part_width * self.num_parts, 1), dtype=torch.float64,
+                                 device=dev)
         d = self.d
         self.plan = _lib.PatchPlan(P, S, self.num_parts, self.max_patch_ldofs,
                                    *[ptr(d[k]) for k in _KEYS], ptr(self.parts))

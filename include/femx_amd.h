@@ -510,6 +510,20 @@ int fx_gls_hist_vjp(fx_index num_nodes, fx_index num_states, fx_index num_elems,
                     const double* d_hist, const double* d_nxt, const double* d_adj,
                     const fx_index* d_inc_off, const fx_index* d_inc_slot, double* d_out,
                     double* d_work, void* stream);
+/* Both history lags' VJPs in one pass over element patches: out_lag0 =
+ * (dR/dhist_0)^T adj and out_lag1 = (dR/dhist_1)^T adj (either may be NULL;
+ * lag 1 needs num_hist > 1), from one evaluation of the element partials.
+ * Plan from fx_patches_create with the mesh nodes as DOFs (dofs = vconn3 =
+ * conn3, coordinates x/y) and a partials workspace of 4 * num_parts doubles
+ * in plan->parts.  A node inside one patch gets fx_gls_hist_vjp's sums bit
+ * for bit; a seam node sums per-patch partials, so it may differ from
+ * fx_gls_hist_vjp in the last bits. */
+int fx_gls_hist_vjp_patch(fx_index num_nodes, fx_index num_states, fx_index num_elems,
+                          const double* d_x, const double* d_y, const fx_index* d_node_rows,
+                          double rho, double mu, double dt, fx_index step, fx_index num_hist,
+                          const double* d_hist, const double* d_nxt, const double* d_adj,
+                          const fx_patch_plan* h_plan, double* d_out_lag0, double* d_out_lag1,
+                          void* stream);
 
 /* ======================================================================== */
 /* Batches (device).  Many P1 Poisson and Taylor-Hood instances of uneven size

@@ -230,3 +230,39 @@ def test_gls_hist_vjp_finite_difference(lag):
     fd = (vals[0] - vals[1]) / (2 * eps)
     vjp = dirv @ out
     assert abs(vjp - fd) <= 2e-6 * max(1.0, abs(vjp), abs(fd)), (vjp, fd)
+
+
+@pytest.mark.parametrize("step,nh", [(1, 2), (0, 1)])
+def test_gls_hist_vjp_both_lags_one_pass(step, nh):
+    """hist_vjp_lags (one pass over element patches, both lags from one
+    evaluation of the partials) against the two-phase fx_gls_hist_vjp per
+    lag: rows of nodes inside one patch bitwise equal, seam nodes within
+    1e-12, repeat launches bitwise; with one history lag only lag 0 exists."""
+    x, y, conn = perturbed(40, 7, True)
+    geom = gls.GLSGeometry(x, y, conn, DEV)
+    amap = gls.make_navier_map(geom)
+    nd = geom.num_states
+    rng = np.random.default_rng(31 + step)
+    hist = dev(rng.standard_normal(nh * nd))
+    nxt, adj = dev(rng.standard_normal(nd)), dev(rng.standard_normal(nd))
+    op = gls.NavierOperator(geom, gls.FluidParams(1.1, 0.02), 0.04)
+    o0, o1 = gls.hist_vjp_lags(op, step, nh, amap, hist, nxt, adj)
+    assert (o1 is None) == (nh == 1)
+    r0, r1 = gls.hist_vjp_lags(op, step, nh, amap, hist, nxt, adj)
+    pp = gls._patches(amap, geom)
+    seam = pp.h["seam_dof"][:pp.num_seam]
+    rows = gls._plan(amap, geom).h["node_rows"].reshape(-1, 3)
+    inner = np.ones(nd, bool)
+    inner[rows[seam].reshape(-1)] = False
+    for lag, out, again in ((0, o0, r0), (1, o1, r1)):
+        if lag >= nh:
+            continue
+        out, again = host(out), host(again)
+        assert np.array_equal(out, again)
+        two = gls.hist_vjp_two_phase(op, step, nh, lag, amap, hist, nxt, adj)
+        two = host(two)
+        ref = O.gls_hist_vjp(x, y, conn, geom.h_dofs9.reshape(-1), nd, 1.1, 0.02, 0.04, step, nh,
+                             lag, host(hist), host(nxt), host(adj))
+        assert close(out, ref), lag
+        assert np.array_equal(out[inner], two[inner]), lag
+        assert close(out, two), lag
