@@ -232,7 +232,8 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="for rocprofv3 counter runs: a 1 GiB device copy (counter "
                          "calibration), then 5 launches each of assembly, SpMV, elimination "
-                         "and VJP; prints no JSON")
+                         "and the VJPs (the GLS history VJP for lag 0, then for both lags); "
+                         "prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); "
@@ -325,6 +326,8 @@ def main():
                 gl["assemble"]()
             for _ in range(5):
                 gl["hist_vjp"]()
+            for _ in range(5):
+                gl["hist_vjp_lags"]()
             torch.cuda.synchronize()
         log("[bench] kernels-only done")
         return
