@@ -77,7 +77,7 @@ _SIGS = {
     "fx_patches_export": ([_vp] * 11, C.c_int),
     "fx_patches_destroy": ([_vp], None),
     "fx_th_vjp_patch": ([_i32] * 2 + [_vp] * 3 + [_f64] + [_vp] * 4 + [_vp] * 2 + [_vp], C.c_int),
-    "fx_batch_create": ([_i32, _vp, _i32, _vp, _i32, C.POINTER(_vp)], C.c_int),
+    "fx_batch_create": ([_i32, _vp, _i32, _vp, _i32, _vp, _i32, C.POINTER(_vp)], C.c_int),
     "fx_batch_work": ([_vp, _vp, _vp], C.c_int),
     "fx_batch_run": ([_vp, C.c_int, _vp], C.c_int),
     "fx_batch_capture": ([_vp, C.c_int], C.c_int),
@@ -119,6 +119,16 @@ class THInstance(C.Structure):
                 + [(n, _vp) for n in ("nu_e", "inc_off", "inc_elem", "inc_meta", "blk_row",
                                       "blk_flags", "row_ptr", "vals", "res", "lam", "bc_mask",
                                       "jt", "g", "blk_lanes")]
+                + [("vjp", PatchPlan)])
+
+
+class GLSInstance(C.Structure):
+    _fields_ = ([(n, _i32) for n in ("num_nodes", "num_states", "num_elems", "max_tile")]
+                + [(n, _vp) for n in ("x", "y", "conn3", "dofs9", "node_rows")]
+                + [(n, _f64) for n in ("rho", "mu", "dt")]
+                + [(n, _i32) for n in ("step", "num_hist")]
+                + [(n, _vp) for n in ("hist", "nxt", "inc_off", "inc_elem", "inc_meta", "blk_runs",
+                                      "node_tile", "vals", "res", "adj", "out_lag0", "out_lag1")]
                 + [("vjp", PatchPlan)])
 
 

@@ -1,6 +1,7 @@
-"""Batched assembly + adjoint VJP of many P1 Poisson and Taylor-Hood
-instances (fx_batch_* in include/femx_amd.h), the per-instance "plain" path
-it is compared with, and the autotuner for its launch shape.
+"""Batched assembly + adjoint VJP of many P1 Poisson, Taylor-Hood and GLS
+P1/P1 Navier-Stokes instances (fx_batch_* in include/femx_amd.h), the
+per-instance "plain" path it is compared with, and the autotuner for its
+launch shape.
 
 The batched path runs persistent launches over (instance, block) work lists
 and can be captured into one hipGraph; the plain path is femx's call pattern
@@ -16,7 +17,7 @@ import time
 import numpy as np
 import torch
 
-from . import _lib, ns
+from . import _lib, gls, ns
 from .assembly import make_assembly_map, make_boundary_map, structured_p1, unit_square_edge_nodes
 from .inverse import PoissonLogKappaResidual
 from .linalg import CsrMatrix, ptr, stream_handle
@@ -139,6 +140,58 @@ class THCase:
         return [self.jac.vals, self.res, self.jt, self.g]
 
 
+class GLSCase:
+    """One GLS P1/P1 Navier-Stokes time step on the structured unit square
+    (n x n cells, two triangles each): history, next state and adjoint drawn
+    from Philox(seed); assembly of the residual and Jacobian and the history
+    VJP of every lag (two lags: step 1, extrapolated advection; one lag:
+    step 0)."""
+
+    def __init__(self, n, seed, device, rho=1.0, mu=1e-3, dt=0.01, num_hist=2):
+        from .assembly import structured_p1
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+        p = structured_p1(n, "cpu")
+        self.geom = gls.GLSGeometry(p.h_x, p.h_y, p.h_conn, device)
+        g = self.geom
+        self.map = gls.make_navier_map(g)
+        self.op = gls.NavierOperator(g, gls.FluidParams(rho, mu), dt)
+        self.step, self.num_hist = (1, 2) if num_hist > 1 else (0, 1)
+        rng = np.random.Generator(np.random.Philox(seed))
+        S = g.num_states
+        self.hist = t(0.3 * rng.standard_normal(self.num_hist * S))
+        self.nxt = t(0.3 * rng.standard_normal(S))
+        self.adj = t(rng.standard_normal(S))
+        self.plan = gls._plan(self.map, g)
+        self.patches = gls._patches(self.map, g)
+        self.jac = CsrMatrix(self.map.pattern, device)
+        self.res = torch.empty(S, dtype=torch.float64, device=device)
+        self.out0 = torch.empty(S, dtype=torch.float64, device=device)
+        self.out1 = (torch.empty(S, dtype=torch.float64, device=device) if self.num_hist > 1
+                     else None)
+        self.elements = g.num_elems
+
+    def descriptor(self):
+        g, p, f = self.geom, self.plan, self.op.fluid
+        return _lib.GLSInstance(
+            g.num_nodes, g.num_states, g.num_elems, p.max_tile, ptr(g.x), ptr(g.y), ptr(g.conn),
+            ptr(g.dofs9), ptr(p.node_rows), f.rho, f.mu, self.op.dt, self.step, self.num_hist,
+            ptr(self.hist), ptr(self.nxt), ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta),
+            ptr(p.blk_runs), ptr(p.node_tile), ptr(self.jac.vals), ptr(self.res), ptr(self.adj),
+            ptr(self.out0), ptr(self.out1), self.patches.plan)
+
+    def plain(self, what):
+        if what & 1:
+            gls.assemble(self.op, self.step, self.num_hist, gls.NEXT_STATE, self.map, self.hist,
+                         self.nxt, self.res, self.jac)
+        if what & 2:
+            gls.hist_vjp_lags(self.op, self.step, self.num_hist, self.map, self.hist, self.nxt,
+                              self.adj, self.out0,
+                              self.out1 if self.out1 is not None else False)
+
+    def outputs(self):
+        return [t for t in (self.jac.vals, self.res, self.out0, self.out1) if t is not None]
+
+
 def size_class(case, bounds=(8192, 131072)):
     """0, 1 or 2 by element count (n < 64, n < 256, larger on the unit
     square)."""
@@ -152,6 +205,8 @@ def build_cases(specs, device="cuda", items=ns.TH_ITEMS, log=None):
             log(f"[batch] building instance {k}/{len(specs)}")
         if s["kind"] == "p1":
             cases.append(P1Case(s["n"], s["seed"], device))
+        elif s["kind"] == "gls":
+            cases.append(GLSCase(s["n"], s["seed"], device, num_hist=s.get("num_hist", 2)))
         else:
             cases.append(THCase(s["n"], s["seed"], s["Re"], device, items))
     return cases
@@ -166,11 +221,14 @@ class Batch:
         self.cases = cases
         p1 = [c.descriptor() for c in cases if isinstance(c, P1Case)]
         th = [c.descriptor() for c in cases if isinstance(c, THCase)]
+        gl = [c.descriptor() for c in cases if isinstance(c, GLSCase)]
         self._p1 = (_lib.P1Instance * max(len(p1), 1))(*p1)
         self._th = (_lib.THInstance * max(len(th), 1))(*th)
+        self._gls = (_lib.GLSInstance * max(len(gl), 1))(*gl)
         h = C.c_void_p()
         _lib.call("fx_batch_create", len(p1), C.cast(self._p1, C.c_void_p), len(th),
-                  C.cast(self._th, C.c_void_p), int(grid_per_cu), C.byref(h))
+                  C.cast(self._th, C.c_void_p), len(gl), C.cast(self._gls, C.c_void_p),
+                  int(grid_per_cu), C.byref(h))
         self._h = h
         self._destroy = _lib.lib().fx_batch_destroy
         self.grid_per_cu = grid_per_cu
@@ -181,7 +239,7 @@ class Batch:
             self._h = None
 
     def work(self):
-        counts = (C.c_int32 * 6)()
+        counts = (C.c_int32 * 9)()
         grid = C.c_int32()
         _lib.call("fx_batch_work", self._h, C.cast(counts, C.c_void_p), C.byref(grid))
         return list(counts), grid.value
@@ -262,11 +320,13 @@ def autotune(cases, layouts=TH_LAYOUTS, grids=GRID_CANDIDATES, reps=3, log=None)
     the tuned graph; the report holds every timing and choice."""
     groups = {}
     for c in cases:
-        groups.setdefault((isinstance(c, THCase), size_class(c)), []).append(c)
+        kind = "th" if isinstance(c, THCase) else "gls" if isinstance(c, GLSCase) else "p1"
+        groups.setdefault((kind, size_class(c)), []).append(c)
     report = {"groups": {}}
     batches = []
-    for (is_th, cls), gc in sorted(groups.items()):
-        key = f"{'th' if is_th else 'p1'}_class{cls}"
+    for (kind, cls), gc in sorted(groups.items()):
+        is_th = kind == "th"
+        key = f"{kind}_class{cls}"
         rec = {"instances": len(gc), "elements": int(sum(c.elements for c in gc))}
         if is_th:
             tl = {}

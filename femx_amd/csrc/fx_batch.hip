@@ -25,14 +25,15 @@
 #include <memory>
 #include <vector>
 
+#include "fx_gls_dev.hpp"
 #include "fx_p1_dev.hpp"
 #include "fx_th_dev.hpp"
 
 struct fx_batch
 {
-  int         num_p1 = 0, num_th = 0;
+  int         num_p1 = 0, num_th = 0, num_gls = 0;
   int         grid   = 0;       // compute units x the tunable multiplier
-  int         pgrid[6] = {0, 0, 0, 0, 0, 0}; // per phase: at least one full residency
+  int         pgrid[9] = {};    // per phase: at least one full residency
   int         cus    = 0;
   std::size_t p1_lds = 0, th_lds = 0, th_vjp_lds = 0;
   int         th_max_ldofs = 0; // the most local DOFs of a VJP patch over the instances
@@ -40,6 +41,8 @@ struct fx_batch
   std::size_t p1_vjp_lds   = 0;
   int         th_tile = 0; // row tile per wave of the Taylor-Hood assembly (doubles)
   bool        p1_long = false; // some P1 row is longer than the LDS tile width
+  std::size_t gls_lds = 0, gls_vjp_lds = 0;
+  int         gls_max_ldofs = 0;
   // device memory
   fxk::P1AsmArgs*  d_p1_asm  = nullptr;
   fxk::P1VjpArgs*  d_p1_vjp  = nullptr;
@@ -49,18 +52,24 @@ struct fx_batch
   fxk::ThVjpArgs*   d_th_vjp  = nullptr;
   fxk::PatchArgs* d_th_pat  = nullptr;
   fxk::SeamArgs*  d_th_seam = nullptr;
-  int2*            d_work[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-  int              nwork[6]  = {0, 0, 0, 0, 0, 0};
+  fxk::GlsAsmArgs*   d_gls_asm  = nullptr;
+  fxk::GlsVjpArgs*   d_gls_vjp  = nullptr;
+  fxk::PatchArgs*    d_gls_pat  = nullptr;
+  fxk::SeamArgs*     d_gls_seam = nullptr;
+  fxk::GlsPatchOut*  d_gls_out  = nullptr;
+  int2*            d_work[9] = {};
+  int              nwork[9]  = {};
   hipStream_t      cap_stream = nullptr;
   hipGraph_t       graph      = nullptr;
   hipGraphExec_t   exec       = nullptr;
   int              graph_what = 0;
   int              device     = 0;
-  // independent chains (P1 assembly, TH assembly, P1 VJP, TH VJP) run on
-  // private branch streams forked from the caller's stream and joined back
-  hipStream_t branch[4] = {nullptr, nullptr, nullptr, nullptr};
+  // independent chains (P1 assembly, TH assembly, P1 VJP, TH VJP, GLS
+  // assembly, GLS VJP) run on private branch streams forked from the
+  // caller's stream and joined back
+  hipStream_t branch[6] = {};
   hipEvent_t  fork      = nullptr;
-  hipEvent_t  done[4]   = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t  done[6]   = {};
 
   ~fx_batch()
   {
@@ -75,7 +84,10 @@ struct fx_batch
     for (void* p : {static_cast<void*>(d_p1_asm), static_cast<void*>(d_p1_vjp),
                     static_cast<void*>(d_th_asm), static_cast<void*>(d_th_vjp),
                     static_cast<void*>(d_th_pat), static_cast<void*>(d_th_seam),
-                    static_cast<void*>(d_p1_pat), static_cast<void*>(d_p1_seam)})
+                    static_cast<void*>(d_p1_pat), static_cast<void*>(d_p1_seam),
+                    static_cast<void*>(d_gls_asm), static_cast<void*>(d_gls_vjp),
+                    static_cast<void*>(d_gls_pat), static_cast<void*>(d_gls_seam),
+                    static_cast<void*>(d_gls_out)})
       if (p) (void)hipFree(p);
     for (auto* w : d_work)
       if (w) (void)hipFree(w);
@@ -106,8 +118,12 @@ enum Phase
   kP1Vjp = 2,
   kThVjpPatch = 3,
   kThVjpSeam = 4,
-  kP1VjpSeam = 5
+  kP1VjpSeam = 5,
+  kGlsAsm = 6,
+  kGlsVjpPatch = 7,
+  kGlsVjpSeam = 8
 };
+constexpr int kPhases = 9, kChains = 6;
 
 // capped at 96 VGPRs (5 waves/SIMD, as k_p1_assemble reaches uncapped)
 template <bool kLong>
@@ -184,6 +200,48 @@ __global__ __launch_bounds__(256) void k_batch_vjp_seam(const SeamArgs* __restri
   }
 }
 
+// GLS P1/P1 time-step assembly: one wave per 32-node block, as k_gls_assemble.
+__global__ __launch_bounds__(64) void k_batch_gls_assemble(const GlsAsmArgs* __restrict__ inst,
+                                                          const int2* __restrict__ work, int nwork)
+{
+  extern __shared__ double tile[];
+  const fx::XcdWalk wk = fx::xcd_walk(nwork);
+  for (int w = wk.first; w < wk.end; w += wk.stride)
+  {
+    const int2 it = work[w];
+    gls_assemble_block(it.y, inst[it.x], tile);
+    wave_sync(); // every lane has stored its tile values: the next block clears the tile
+  }
+}
+
+// GLS history VJP, both lags, over element patches, and its seam pass.
+__global__ __launch_bounds__(kPatchLanes) void k_batch_gls_vjp_patch(
+    const GlsVjpArgs* __restrict__ inst, const PatchArgs* __restrict__ pat,
+    const GlsPatchOut* __restrict__ out, const int2* __restrict__ work, int nwork,
+    int max_patch_ldofs)
+{
+  extern __shared__ __align__(16) unsigned char lds[];
+  const fx::XcdWalk wk = fx::xcd_walk(nwork);
+  for (int w = wk.first; w < wk.end; w += wk.stride)
+  {
+    const int2 it = work[w];
+    gls_vjp_patch_block(it.y, inst[it.x], pat[it.x], out[it.x], lds, max_patch_ldofs);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_batch_gls_vjp_seam(const GlsVjpArgs* __restrict__ inst,
+                                                           const SeamArgs* __restrict__ seam,
+                                                           const GlsPatchOut* __restrict__ out,
+                                                           const int2* __restrict__ work, int nwork)
+{
+  const fx::XcdWalk wk = fx::xcd_walk(nwork);
+  for (int w = wk.first; w < wk.end; w += wk.stride)
+  {
+    const int2 it = work[w];
+    gls_vjp_seam_block(it.y, inst[it.x], seam[it.x], out[it.x]);
+  }
+}
+
 template <class T>
 T* upload(const std::vector<T>& h)
 {
@@ -195,9 +253,9 @@ T* upload(const std::vector<T>& h)
   return d;
 }
 
-// The four chains of a batch, each on its own stream: P1 assembly, TH
-// assembly, P1 VJP, TH VJP (element then gather phase).  None reads another's
-// output, so they may run concurrently.
+// The six chains of a batch, each on its own stream: P1 assembly, TH
+// assembly, P1 VJP, TH VJP, GLS assembly, GLS VJP (each VJP a patch then a
+// seam pass).  None reads another's output, so they may run concurrently.
 void launch_chain(fx_batch* b, int chain, hipStream_t s)
 {
   auto g = [&](int p) { return dim3(b->pgrid[p]); };
@@ -224,13 +282,26 @@ void launch_chain(fx_batch* b, int chain, hipStream_t s)
       hipLaunchKernelGGL(k_batch_vjp_seam, g(kP1VjpSeam), dim3(256), 0, s, b->d_p1_seam,
                          b->d_work[kP1VjpSeam], b->nwork[kP1VjpSeam]);
     break;
-  default:
+  case 3:
     hipLaunchKernelGGL(k_batch_th_vjp_patch, g(kThVjpPatch), dim3(kPatchLanes), b->th_vjp_lds, s, b->d_th_vjp,
                        b->d_th_pat, b->d_work[kThVjpPatch], b->nwork[kThVjpPatch], b->th_max_ldofs);
     fx::check_launch("fx_batch: VJP");
     if (b->nwork[kThVjpSeam] > 0)
       hipLaunchKernelGGL(k_batch_vjp_seam, g(kThVjpSeam), dim3(256), 0, s, b->d_th_seam,
                          b->d_work[kThVjpSeam], b->nwork[kThVjpSeam]);
+    break;
+  case 4:
+    hipLaunchKernelGGL(k_batch_gls_assemble, g(kGlsAsm), dim3(64), b->gls_lds, s, b->d_gls_asm,
+                       b->d_work[kGlsAsm], b->nwork[kGlsAsm]);
+    break;
+  default:
+    hipLaunchKernelGGL(k_batch_gls_vjp_patch, g(kGlsVjpPatch), dim3(kPatchLanes), b->gls_vjp_lds, s,
+                       b->d_gls_vjp, b->d_gls_pat, b->d_gls_out, b->d_work[kGlsVjpPatch],
+                       b->nwork[kGlsVjpPatch], b->gls_max_ldofs);
+    fx::check_launch("fx_batch: VJP");
+    if (b->nwork[kGlsVjpSeam] > 0)
+      hipLaunchKernelGGL(k_batch_gls_vjp_seam, g(kGlsVjpSeam), dim3(256), 0, s, b->d_gls_vjp,
+                         b->d_gls_seam, b->d_gls_out, b->d_work[kGlsVjpSeam], b->nwork[kGlsVjpSeam]);
   }
   fx::check_launch("fx_batch");
 }
@@ -242,7 +313,9 @@ bool chain_active(const fx_batch* b, int chain, int what)
   case 0: return (what & FX_BATCH_ASSEMBLE) && b->nwork[kP1Asm] > 0;
   case 1: return (what & FX_BATCH_ASSEMBLE) && b->nwork[kThAsm] > 0;
   case 2: return (what & FX_BATCH_VJP) && b->nwork[kP1Vjp] > 0;
-  default: return (what & FX_BATCH_VJP) && b->nwork[kThVjpPatch] > 0;
+  case 3: return (what & FX_BATCH_VJP) && b->nwork[kThVjpPatch] > 0;
+  case 4: return (what & FX_BATCH_ASSEMBLE) && b->nwork[kGlsAsm] > 0;
+  default: return (what & FX_BATCH_VJP) && b->nwork[kGlsVjpPatch] > 0;
   }
 }
 
@@ -264,7 +337,7 @@ void launch_batches(fx_batch* const* bs, int count, int what, hipStream_t s)
   if (what & FX_BATCH_SERIAL)
   {
     // every chain on s, one after another: no phase shares the chip
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < kChains; ++c)
       for (int i = 0; i < count; ++i)
         if (chain_active(bs[i], c, what)) launch_chain(bs[i], c, s);
     return;
@@ -274,7 +347,7 @@ void launch_batches(fx_batch* const* bs, int count, int what, hipStream_t s)
     fx_batch* b = bs[i];
     ensure_branches(b);
     fx::check_hip(hipEventRecord(b->fork, s), "hipEventRecord");
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < kChains; ++c)
     {
       if (!chain_active(b, c, what)) continue;
       fx::check_hip(hipStreamWaitEvent(b->branch[c], b->fork, 0), "hipStreamWaitEvent");
@@ -283,7 +356,7 @@ void launch_batches(fx_batch* const* bs, int count, int what, hipStream_t s)
     }
   }
   for (int i = 0; i < count; ++i)
-    for (int c = 0; c < 4; ++c)
+    for (int c = 0; c < kChains; ++c)
       if (chain_active(bs[i], c, what))
         fx::check_hip(hipStreamWaitEvent(s, bs[i]->done[c], 0), "hipStreamWaitEvent");
 }
@@ -325,11 +398,12 @@ hipGraphExec_t capture_batches(fx_batch* const* bs, int count, int what, hipStre
 extern "C" {
 
 int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
-                    const fx_th_instance* th, fx_index grid_per_cu, fx_batch** out)
+                    const fx_th_instance* th, fx_index num_gls, const fx_gls_instance* gls,
+                    fx_index grid_per_cu, fx_batch** out)
 {
   return fx::guard([&] {
-    fx::require(out != nullptr && num_p1 >= 0 && num_th >= 0 && grid_per_cu >= 0
-                    && (num_p1 == 0 || p1) && (num_th == 0 || th),
+    fx::require(out != nullptr && num_p1 >= 0 && num_th >= 0 && num_gls >= 0 && grid_per_cu >= 0
+                    && (num_p1 == 0 || p1) && (num_th == 0 || th) && (num_gls == 0 || gls),
                 FX_ERR_ARG, "fx_batch_create: invalid arguments");
     *out   = nullptr;
     auto b = std::make_unique<fx_batch>();
@@ -341,6 +415,7 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     b->grid   = (std::max(1, cus * grid_per_cu) + 7) / 8 * 8; // xcd_walk needs a multiple of 8
     b->num_p1 = num_p1;
     b->num_th = num_th;
+    b->num_gls = num_gls;
 
     std::vector<P1AsmArgs>  pa;
     std::vector<P1VjpArgs>  pv;
@@ -350,7 +425,12 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     std::vector<SeamArgs>  tsm;
     std::vector<PatchArgs> pp;
     std::vector<SeamArgs>  psm;
-    std::vector<int2>       w[6];
+    std::vector<GlsAsmArgs>  ga;
+    std::vector<GlsVjpArgs>  gv;
+    std::vector<PatchArgs>   gp;
+    std::vector<SeamArgs>    gsm;
+    std::vector<GlsPatchOut> go;
+    std::vector<int2>       w[kPhases];
     int                     max_row_len = 1;
     fx_index                max_blk_nnz = 0;
     for (int i = 0; i < num_p1; ++i)
@@ -432,6 +512,65 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
         for (int k = 0; k < ns; ++k) w[kThVjpSeam].push_back(make_int2(i, k));
       }
     }
+    int gls_tile = 0;
+    for (int i = 0; i < num_gls; ++i)
+    {
+      const fx_gls_instance& q = gls[i];
+      // fx_gls_assemble's and fx_gls_hist_vjp_patch's checks
+      // (NavierStokesModel.cpp:35-46, Assembly.hpp:86-104)
+      fx::require(q.num_nodes > 0 && q.num_states > 0 && q.num_elems > 0 && q.max_tile >= 0
+                      && q.num_hist > 0,
+                  FX_ERR_ARG, "fx_batch_create: invalid GLS instance sizes");
+      fx::require(q.dt > 0.0 && std::isfinite(q.dt) && q.rho > 0.0 && std::isfinite(q.rho)
+                      && q.mu > 0.0 && std::isfinite(q.mu),
+                  FX_ERR_ARG, "fx_batch_create: NavierStokesModel requires positive finite "
+                              "time step, density and viscosity");
+      fx::require(fx::fits32(q.num_states, 8 * q.num_hist) && fx::fits32(q.num_nodes, 12),
+                  FX_ERR_OVERFLOW, "fx_batch_create: GLS instance arrays beyond 4 GiB");
+      fx::require(q.x && q.y && q.conn3 && q.dofs9 && q.node_rows && q.hist && q.nxt && q.inc_off
+                      && q.inc_elem && q.inc_meta && q.blk_runs && q.node_tile && q.res,
+                  FX_ERR_ARG, "fx_batch_create: GLS instance lacks an array");
+      fx::require(!q.vals || q.max_tile <= kGlsMaxTile, FX_ERR_UNSUPPORTED,
+                  "fx_batch_create: a GLS block's rows exceed the 64 KiB LDS tile");
+      if (q.vals) gls_tile = std::max(gls_tile, static_cast<int>(q.max_tile > 0 ? q.max_tile : 1));
+      const GlsConst c = gls_const(q.rho, q.mu, q.dt);
+      ga.push_back({static_cast<int>(q.num_nodes), static_cast<int>(q.num_states), q.x, q.y,
+                    q.conn3, q.dofs9, q.node_rows, static_cast<int>(q.step),
+                    static_cast<int>(q.num_hist), q.hist, q.nxt, c, q.inc_off, q.inc_elem,
+                    reinterpret_cast<const uint4*>(q.inc_meta),
+                    reinterpret_cast<const int4*>(q.blk_runs), q.node_tile, q.vals, q.res,
+                    static_cast<int>(q.max_tile)});
+      const int nb = static_cast<int>(fx::blocks_for(q.num_nodes, kGlsBlock));
+      for (int k = 0; k < nb; ++k) w[kGlsAsm].push_back(make_int2(i, k));
+      const fx_patch_plan& pl = q.vjp;
+      fx::require(pl.num_patches >= 0 && pl.num_seam >= 0 && pl.num_parts >= 0
+                      && pl.max_patch_ldofs >= 0 && pl.max_patch_ldofs <= 3 * kPatchLanes
+                      && fx::fits32(pl.num_parts, 32),
+                  FX_ERR_ARG, "fx_batch_create: invalid GLS VJP patch plan");
+      fx::require(!q.out_lag1 || q.num_hist > 1, FX_ERR_ARG,
+                  "fx_batch_create: Time assembly variable block is invalid");
+      const bool vjp = q.adj && (q.out_lag0 || q.out_lag1) && pl.num_patches > 0 && pl.patch_off
+                       && pl.patch_elem && pl.elem_ldof && pl.ldof_off && pl.ldof_dof
+                       && pl.ldof_part && pl.ldof_end && pl.seam_off
+                       && (pl.num_seam == 0 || pl.seam_dof) && (pl.num_parts == 0 || pl.parts);
+      gv.push_back({static_cast<int>(q.num_nodes), static_cast<int>(q.num_states), q.x, q.y,
+                    nullptr, nullptr, q.node_rows, static_cast<int>(q.step),
+                    static_cast<int>(q.num_hist), 0, q.hist, q.nxt, q.adj, c, nullptr, nullptr,
+                    nullptr, nullptr, static_cast<int>(q.num_elems)});
+      gp.push_back({pl.patch_off, pl.patch_elem, pl.elem_vert, pl.elem_ldof, pl.ldof_off,
+                    pl.ldof_dof, pl.ldof_part, pl.ldof_end, nullptr, pl.parts});
+      gsm.push_back({pl.num_seam, pl.seam_off, pl.seam_dof, pl.parts, nullptr, nullptr, nullptr});
+      go.push_back({q.out_lag0, q.out_lag1});
+      if (vjp)
+      {
+        b->gls_max_ldofs = std::max(b->gls_max_ldofs, static_cast<int>(pl.max_patch_ldofs));
+        const int ns = static_cast<int>(fx::blocks_for(pl.num_seam, 256));
+        for (int k = 0; k < pl.num_patches; ++k) w[kGlsVjpPatch].push_back(make_int2(i, k));
+        for (int k = 0; k < ns; ++k) w[kGlsVjpSeam].push_back(make_int2(i, k));
+      }
+    }
+    b->gls_lds     = static_cast<std::size_t>(gls_tile) * 8;
+    b->gls_vjp_lds = gls_patch_lds(b->gls_max_ldofs);
     b->th_tile = static_cast<int>(max_blk_nnz);
     b->th_lds  = sizeof(ThTensor) + static_cast<std::size_t>(max_blk_nnz) * 8 * kThWaves;
     fx::require(b->p1_lds <= 160 * 1024 && b->th_lds <= 160 * 1024, FX_ERR_UNSUPPORTED,
@@ -470,6 +609,12 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
             b->th_vjp_lds);
     b->pgrid[kThVjpSeam] =
         occ(kThVjpSeam, reinterpret_cast<const void*>(k_batch_vjp_seam), 256, 0);
+    b->pgrid[kGlsAsm] =
+        occ(kGlsAsm, reinterpret_cast<const void*>(k_batch_gls_assemble), 64, b->gls_lds);
+    b->pgrid[kGlsVjpPatch] = occ(kGlsVjpPatch, reinterpret_cast<const void*>(k_batch_gls_vjp_patch),
+                                 kPatchLanes, b->gls_vjp_lds);
+    b->pgrid[kGlsVjpSeam] =
+        occ(kGlsVjpSeam, reinterpret_cast<const void*>(k_batch_gls_vjp_seam), 256, 0);
     b->d_p1_asm = upload(pa);
     b->d_p1_vjp = upload(pv);
     b->d_th_asm = upload(ta);
@@ -478,7 +623,12 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     b->d_th_seam = upload(tsm);
     b->d_p1_pat  = upload(pp);
     b->d_p1_seam = upload(psm);
-    for (int p = 0; p < 6; ++p)
+    b->d_gls_asm  = upload(ga);
+    b->d_gls_vjp  = upload(gv);
+    b->d_gls_pat  = upload(gp);
+    b->d_gls_seam = upload(gsm);
+    b->d_gls_out  = upload(go);
+    for (int p = 0; p < kPhases; ++p)
     {
       b->nwork[p]  = static_cast<int>(w[p].size());
       b->d_work[p] = upload(w[p]);
@@ -487,11 +637,11 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
   });
 }
 
-int fx_batch_work(const fx_batch* b, fx_index* counts6, fx_index* grid)
+int fx_batch_work(const fx_batch* b, fx_index* counts9, fx_index* grid)
 {
   return fx::guard([&] {
-    fx::require(b && counts6, FX_ERR_ARG, "fx_batch_work: null argument");
-    for (int p = 0; p < 6; ++p) counts6[p] = b->nwork[p];
+    fx::require(b && counts9, FX_ERR_ARG, "fx_batch_work: null argument");
+    for (int p = 0; p < kPhases; ++p) counts9[p] = b->nwork[p];
     if (grid) *grid = b->grid;
   });
 }

@@ -17,6 +17,16 @@ __device__ __forceinline__ T gat32(const T* __restrict__ base, int i)
                                      + static_cast<unsigned>(i) * unsigned(sizeof(T)));
 }
 
+// Order the LDS accesses of one wave before and after this point (a wave's
+// LDS instructions execute in issue order; this keeps the compiler from
+// moving them across).
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // LDS floating-point atomic add whose result is not used (ds_add_f64): the
 // LDS unit performs old + v, rounded as a VALU add, and the issuing wave does
 // not wait for it.  One wave's LDS operations execute in issue order, so a

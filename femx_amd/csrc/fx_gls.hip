@@ -8,8 +8,6 @@ namespace
 {
 using namespace fxk;
 
-constexpr int kGlsMaxTile = 8192; // doubles of LDS row tile per wave (64 KiB)
-
 __global__ __launch_bounds__(64) void k_gls_assemble(GlsAsmArgs a)
 {
   extern __shared__ double tile[];

@@ -41,6 +41,7 @@ namespace fxk
 
 constexpr int kGlsBlock = FX_GLS_BLOCK_NODES; // nodes per wave
 constexpr int kGlsRuns  = FX_GLS_MAX_RUNS;    // row runs per block
+constexpr int kGlsMaxTile = 8192;             // doubles of LDS row tile per wave (64 KiB)
 
 // Loop-invariant factors of NavierOperator's expressions, formed once (on the
 // host, passed as kernel arguments) with the same operations femx applies at
@@ -326,13 +327,6 @@ __device__ __forceinline__ void gls_add_rows(double* tile, int tx, int ty, int t
   for (int k = 0; k < 9; ++k) v[k] = pp[meta_byte(m0, m1, 18 + k)];
 #pragma unroll
   for (int k = 0; k < 9; ++k) pp[meta_byte(m0, m1, 18 + k)] = v[k] + o.Kp[k];
-}
-
-__device__ __forceinline__ void wave_sync()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Element inputs of one incidence: geometry and the local history and next

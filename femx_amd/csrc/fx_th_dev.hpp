@@ -41,15 +41,6 @@ namespace fxk
 constexpr int kThLanes = 64;
 constexpr int kThWaves = 4; // waves per assembly workgroup, sharing one copy of the tables
 
-// Order the LDS accesses of one wave before and after this point (a wave's
-// LDS instructions execute in issue order; this keeps the compiler from
-// moving them across).
-__device__ __forceinline__ void wave_sync()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 struct ThTables
 {

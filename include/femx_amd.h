@@ -526,15 +526,16 @@ int fx_gls_hist_vjp_patch(fx_index num_nodes, fx_index num_states, fx_index num_
                           void* stream);
 
 /* ======================================================================== */
-/* Batches (device).  Many P1 Poisson and Taylor-Hood instances of uneven size
- * assembled and differentiated by persistent launches over one work list
- * per phase, optionally captured in a hipGraph and replayed with one launch.
- * femx has no batching; each instance's results are bitwise equal to its
- * single-instance calls (fx_p1_assemble with kappa_is_log = 1 and
- * fx_p1_vjp_patch; fx_th_assemble and fx_th_vjp_patch).  Instance arrays are
- * caller-owned device memory and must outlive the batch.  VJP pointers (lam,
- * jt, g, the patch plan) may be NULL (an empty plan) to batch assembly
- * only. */
+/* Batches (device).  Many P1 Poisson, Taylor-Hood and GLS P1/P1
+ * Navier-Stokes instances of uneven size assembled and differentiated by
+ * persistent launches over one work list per phase, optionally captured in a
+ * hipGraph and replayed with one launch.  femx has no batching; each
+ * instance's results are bitwise equal to its single-instance calls
+ * (fx_p1_assemble with kappa_is_log = 1 and fx_p1_vjp_patch; fx_th_assemble
+ * and fx_th_vjp_patch; fx_gls_assemble and fx_gls_hist_vjp_patch).  Instance
+ * arrays are caller-owned device memory and must outlive the batch.  VJP
+ * pointers (lam or adj, the outputs, the patch plan) may be NULL (an empty
+ * plan) to batch assembly only. */
 typedef struct
 {
   fx_index      num_nodes, num_elems, nnz, max_row_len;
@@ -582,6 +583,34 @@ typedef struct
   fx_patch_plan vjp; /* the VJP's patch plan (fx_th_vjp_patch) */
 } fx_th_instance;
 
+/* One GLS time step (fx_gls_assemble's arguments; vals NULL assembles the
+ * residual only) and its history VJP over both lags (fx_gls_hist_vjp_patch's
+ * arguments; out_lag1 needs num_hist > 1). */
+typedef struct
+{
+  fx_index      num_nodes, num_states, num_elems, max_tile;
+  const double* x;
+  const double* y;
+  const fx_index* conn3;
+  const fx_index* dofs9;
+  const fx_index* node_rows;
+  double        rho, mu, dt;
+  fx_index      step, num_hist;
+  const double* hist;
+  const double* nxt;
+  const fx_index* inc_off;
+  const fx_index* inc_elem;
+  const fx_index* inc_meta;
+  const fx_index* blk_runs;
+  const fx_index* node_tile;
+  double*       vals;
+  double*       res;
+  const double* adj;
+  double*       out_lag0;
+  double*       out_lag1;
+  fx_patch_plan vjp; /* the VJP's patch plan (fx_patches_create with 9 DOFs per element) */
+} fx_gls_instance;
+
 typedef struct fx_batch fx_batch;
 enum
 {
@@ -594,16 +623,17 @@ enum
  * compute units x grid_per_cu, at least one full residency), or 0 for one
  * workgroup per work item.  Copies the descriptors to the device. */
 int fx_batch_create(fx_index num_p1, const fx_p1_instance* h_p1, fx_index num_th,
-                    const fx_th_instance* h_th, fx_index grid_per_cu, fx_batch** out);
+                    const fx_th_instance* h_th, fx_index num_gls, const fx_gls_instance* h_gls,
+                    fx_index grid_per_cu, fx_batch** out);
 /* Work items per phase (P1 assembly, TH assembly, P1 VJP patches, TH VJP
- * patches, TH VJP seam blocks, P1 VJP seam blocks) and the persistent grid
- * size. */
-int fx_batch_work(const fx_batch* batch, fx_index* h_counts6, fx_index* h_grid);
+ * patches, TH VJP seam blocks, P1 VJP seam blocks, GLS assembly blocks, GLS
+ * VJP patches, GLS VJP seam blocks) and the persistent grid size. */
+int fx_batch_work(const fx_batch* batch, fx_index* h_counts9, fx_index* h_grid);
 /* Launch the phases in `what` (FX_BATCH_ASSEMBLE | FX_BATCH_VJP) ordered
  * after earlier work on stream, and make later work on stream wait for
- * them.  The independent chains (P1 assembly, TH assembly, P1 VJP, TH VJP)
- * run concurrently on the batch's private streams (default priority), or in
- * that order on stream with FX_BATCH_SERIAL. */
+ * them.  The independent chains (P1 assembly, TH assembly, P1 VJP, TH VJP,
+ * GLS assembly, GLS VJP) run concurrently on the batch's private streams
+ * (default priority), or in that order on stream with FX_BATCH_SERIAL. */
 int fx_batch_run(fx_batch* batch, int what, void* stream);
 /* Capture the phases in `what` into a hipGraph (on a private stream), one
  * graph branch per chain (a single chain of nodes with FX_BATCH_SERIAL). */

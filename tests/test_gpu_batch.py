@@ -34,7 +34,7 @@ def test_batch_matches_plain_bitwise(grid_per_cu):
     assert all(np.isfinite(x).all() for xa in ref for x in xa)
     b = B.Batch(cases, grid_per_cu)
     counts, grid = b.work()
-    assert grid > 0 and all(c > 0 for c in counts)
+    assert grid > 0 and all(c > 0 for c in counts[:6]) and counts[6:] == [0, 0, 0]
     clear(cases)
     b.run(3)
     assert same(snapshot(cases), ref)
@@ -113,3 +113,44 @@ def test_grouped_graph_and_autotune_bitwise():
             clear(cases)
             b.run(what)
             assert same(snapshot(cases), ref), (lay, g, what)
+
+
+def test_batch_with_gls_instances_bitwise():
+    """GLS P1/P1 Navier-Stokes time steps (assembly and the both-lags history
+    VJP; one instance with a single history lag, so step 0 and no lag-1
+    output) batched beside P1 and Taylor-Hood instances: bitwise equal to
+    fx_gls_assemble and fx_gls_hist_vjp_patch, through persistent launches,
+    one workgroup per item, graph replays and the autotuner's graph."""
+    specs = B.instance_specs(count=4, seed=9, n_min=3, n_max=20)
+    specs += [dict(kind="gls", n=n, seed=300 + n, num_hist=h)
+              for n, h in ((5, 2), (23, 2), (2, 1), (40, 2))]
+    cases = B.build_cases(specs, "cuda")
+    gl = [c for c in cases if isinstance(c, B.GLSCase)]
+    assert len(gl) == 4 and gl[2].out1 is None
+    B.plain(cases, 3)
+    ref = snapshot(cases)
+    assert all(np.isfinite(x).all() for xa in ref for x in xa)
+    for grid in (0, 2):
+        b = B.Batch(cases, grid)
+        counts, _ = b.work()
+        assert all(c > 0 for c in counts[6:8])
+        clear(cases)
+        b.run(3)
+        assert same(snapshot(cases), ref), grid
+        clear(cases)
+        b.run(3 | B.Batch.SERIAL)
+        assert same(snapshot(cases), ref), grid
+    b = B.Batch(gl, 1)
+    b.capture(3)
+    for k in range(3):
+        for c in gl:
+            for t in c.outputs():
+                t.fill_(float("nan"))
+        b.launch()
+        assert same(snapshot(gl), [ref[cases.index(c)] for c in gl]), f"graph replay {k}"
+        B.plain(cases[:1], 1)
+    runner, report = B.autotune(cases, layouts=B.TH_LAYOUTS[:1], grids=(0, 2), reps=1)
+    assert any(k.startswith("gls_") for k in report["groups"])
+    clear(cases)
+    runner.launch()
+    assert same(snapshot(cases), ref)
