@@ -56,6 +56,12 @@ _SIGS = {
                        C.c_int),
     "fx_p1_vjp": ([_i32, _i32] + [_vp] * 11 + [_vp], C.c_int),
     "fx_p1_vjp_patch": ([_i32, _i32] + [_vp] * 9 + [_vp], C.c_int),
+    "fx_p1_asm_destroy": ([_vp], None),
+    "fx_p1_asm_create": ([_i32, _i32] + [_vp] * 5 + [_i32, _i32, C.POINTER(_vp)], C.c_int),
+    "fx_p1_asm_info_get": ([_vp, _vp], C.c_int),
+    "fx_p1_asm_export": ([_vp] * 12, C.c_int),
+    "fx_p1_assemble_patch": ([_i32, _i32, _i32] + [_vp] * 4 + [C.c_int] + [_vp] * 8 + [_vp],
+                             C.c_int),
     "fx_th_assemble": ([_i32] * 4 + [_vp] * 5 + [_f64] + [_vp] * 10 + [_vp], C.c_int),
     "fx_csr_matvec_multi": ([_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32, _f64,
                              _f64, _vp], C.c_int),
@@ -130,6 +136,19 @@ class GLSInstance(C.Structure):
                 + [(n, _vp) for n in ("hist", "nxt", "inc_off", "inc_elem", "inc_meta", "blk_runs",
                                       "node_tile", "vals", "res", "adj", "out_lag0", "out_lag1")]
                 + [("vjp", PatchPlan)])
+
+
+class P1AsmInfo(C.Structure):
+    _fields_ = [(n, _i32) for n in ("num_patches", "num_entries", "num_ldofs", "num_runs",
+                                    "num_left", "max_patch_ldofs", "max_tile", "max_runs")]
+
+
+class P1AsmPlan(C.Structure):
+    _fields_ = ([(n, _i32) for n in ("num_patches", "num_left", "max_patch_ldofs", "max_tile",
+                                     "max_runs")]
+                + [(n, _vp) for n in ("patch_off", "patch_elem", "elem_ldof", "elem_pos",
+                                      "ldof_off", "ldof_dof", "ldof_tile", "patch_tile",
+                                      "run_off", "runs", "rows_left")])
 
 
 class MapInfo(C.Structure):

@@ -295,11 +295,44 @@ def _check_p1(geom: Geometry, amap: AssemblyMap):
     setattr(amap, key, geom)
 
 
+def p1_patches(geom: Geometry, amap: AssemblyMap):
+    """The element-patch plan of the P1 kernels on this mesh (fx_patches_create
+    with the nodes as DOFs), built on first use and kept with the map, per
+    device.  Calls that share it (its seam partials) must not overlap on
+    different streams."""
+    key = "_p1_patches_" + str(geom.device)
+    p = getattr(amap, key, None)
+    if p is None or p[0] is not geom:
+        from .patches import Patches
+        p = (geom, Patches(geom.num_nodes, geom.num_elems, 3, geom.h_conn, geom.h_conn, geom.h_x,
+                           geom.h_y, geom.device))
+        setattr(amap, key, p)
+    return p[1]
+
+
+def p1_asm_plan(geom: Geometry, amap: AssemblyMap):
+    """The P1 patch assembly's plan (fx_p1_asm_create), built on first use
+    and kept with the map, per device."""
+    key = "_p1_asm_" + str(geom.device)
+    p = getattr(amap, key, None)
+    if p is None or p[0] is not geom:
+        from .patches import P1AsmPlan
+        p = (geom, P1AsmPlan(geom.h_conn, geom.h_x, geom.h_y, amap.pattern, geom.device))
+        setattr(amap, key, p)
+    return p[1]
+
+
 def assemble(op, geom: Geometry, amap: AssemblyMap, state, res, jac: CsrMatrix = None,
-             stream=None):
+             stream=None, patches=True):
     """Assemble the residual into `res` and, when `jac` is given, the CSR
     Jacobian values into `jac.vals`; both outputs are replaced (femx
-    assembly::assemble, CudaAssembly.hpp:319-364).  Returns `res`."""
+    assembly::assemble, CudaAssembly.hpp:319-364).  Returns `res`.
+
+    With `patches` (the default) the mesh's element patches assemble the
+    rows of the nodes they own, each element evaluated once per patch
+    (fx_p1_assemble_patch; the plan is built on the first call); without, one
+    lane per row evaluates the row's elements (fx_p1_assemble).  The results
+    are bitwise equal."""
     if not isinstance(op, PoissonP1):
         raise RuntimeError(f"assemble: unsupported element operator {type(op).__name__}")
     _check_p1(geom, amap)
@@ -320,6 +353,14 @@ def assemble(op, geom: Geometry, amap: AssemblyMap, state, res, jac: CsrMatrix =
         _check_dev_vec(op.f, N, torch.float64, "PoissonP1 load")
     dev = geom.device
     rp, _ = amap.pattern.device_arrays(dev)
+    if patches and E > 0:
+        ap = p1_asm_plan(geom, amap)
+        _lib.call("fx_p1_assemble_patch", N, E, amap.pattern.nnz, ptr(geom.x), ptr(geom.y),
+                  ptr(geom.conn), ptr(coef), int(op.log_kappa is not None), ptr(op.f),
+                  ptr(state), ptr(amap.device("inc_off", dev)), ptr(amap.device("inc", dev)),
+                  ptr(rp), C.byref(ap.plan),
+                  ptr(jac.vals) if jac is not None else None, ptr(res), stream_handle(stream))
+        return res
     _lib.call("fx_p1_assemble", N, E, amap.pattern.nnz, amap.max_row_len, ptr(geom.x),
               ptr(geom.y), ptr(geom.conn), ptr(coef), int(op.log_kappa is not None), ptr(op.f),
               ptr(state),

@@ -937,6 +937,257 @@ int fx_patches_export(const fx_patches* p, Index* patch_off, Index* patch_elem,
 
 void fx_patches_destroy(fx_patches* p) { delete p; }
 
+} // extern "C"
+
+// P1 assembly plan: patches of at most max_patch elements in which every row
+// is finished by the one patch that owns its node.  Elements are visited in
+// Morton order of their centroids; a node is owned by the patch that meets it
+// first, and that patch takes all of the node's elements (its own and, as a
+// halo, its neighbours').  A node is ownable when its row has at most 16
+// entries and it has 1 to 16 elements; the others (and nodes in no element)
+// are left to the row-per-lane kernel.
+struct fx_p1_asm
+{
+  Index num_nodes = 0, max_patch_ldofs = 0, max_tile = 0, max_runs = 0;
+  std::vector<Index>         patch_off{0}, patch_elem, ldof_off{0}, ldof_dof, patch_tile, run_off{0},
+      runs, rows_left;
+  std::vector<std::uint16_t> elem_ldof, elem_pos, ldof_tile;
+};
+
+extern "C" {
+
+int fx_p1_asm_create(Index num_nodes, Index num_elems, const Index* conn, const double* x,
+                     const double* y, const Index* row_ptr, const Index* col_ind, Index max_patch,
+                     Index max_tile, fx_p1_asm** out)
+{
+  return fx::guard([&] {
+    fx::require(out && num_nodes >= 0 && num_elems >= 0 && row_ptr
+                    && (num_elems == 0 || (conn && x && y && col_ind)),
+                FX_ERR_ARG, "fx_p1_asm_create: invalid arguments");
+    fx::require(max_patch >= 1 && max_patch <= FX_TH_PATCH_MAX && max_tile >= 0
+                    && max_tile < 0xffff,
+                FX_ERR_ARG, "fx_p1_asm_create: max_patch must be in [1, FX_TH_PATCH_MAX], max_tile below 65535");
+    *out         = nullptr;
+    auto p       = std::make_unique<fx_p1_asm>();
+    const Index N = num_nodes, E = num_elems;
+    p->num_nodes = N;
+    for (Index i = 0; i < 3 * E; ++i)
+      fx::require(conn[i] >= 0 && conn[i] < N, FX_ERR_ARG, "fx_p1_asm_create: node id out of range");
+    // each node's elements, ascending (counting sort)
+    std::vector<Index> noff(N + 1, 0), nel(3 * std::size_t(E));
+    for (Index i = 0; i < 3 * E; ++i) ++noff[conn[i] + 1];
+    for (Index n = 0; n < N; ++n) noff[n + 1] += noff[n];
+    {
+      std::vector<Index> cur(noff.begin(), noff.end() - 1);
+      for (Index e = 0; e < E; ++e)
+        for (int k = 0; k < 3; ++k) nel[cur[conn[3 * e + k]]++] = e;
+    }
+    auto rank_of = [&](Index n, Index e) {
+      return static_cast<Index>(std::lower_bound(nel.begin() + noff[n], nel.begin() + noff[n + 1], e)
+                                - (nel.begin() + noff[n]));
+    };
+    auto rowlen = [&](Index n) { return row_ptr[n + 1] - row_ptr[n]; };
+    std::vector<char> ownable(N);
+    for (Index n = 0; n < N; ++n)
+    {
+      const Index ne = noff[n + 1] - noff[n];
+      ownable[n]     = ne >= 1 && ne <= 16 && rowlen(n) <= 16 && rowlen(n) + 0 <= max_tile;
+    }
+    // Morton order of the element centroids (ties by element id)
+    std::vector<std::uint64_t> key(E);
+    double lo[2] = {0.0, 0.0}, hi[2] = {0.0, 0.0};
+    std::vector<double> cx(E), cy(E);
+    for (Index e = 0; e < E; ++e)
+    {
+      cx[e] = (x[conn[3 * e]] + x[conn[3 * e + 1]] + x[conn[3 * e + 2]]) / 3.0;
+      cy[e] = (y[conn[3 * e]] + y[conn[3 * e + 1]] + y[conn[3 * e + 2]]) / 3.0;
+      if (e == 0 || cx[e] < lo[0]) lo[0] = cx[e];
+      if (e == 0 || cy[e] < lo[1]) lo[1] = cy[e];
+      if (e == 0 || cx[e] > hi[0]) hi[0] = cx[e];
+      if (e == 0 || cy[e] > hi[1]) hi[1] = cy[e];
+    }
+    const double span = std::max(hi[0] - lo[0], hi[1] - lo[1]);
+    const double sc   = span > 0.0 ? 65535.0 / span : 0.0;
+    for (Index e = 0; e < E; ++e)
+    {
+      const auto qx = static_cast<std::uint32_t>(std::min(65535.0, std::max(0.0, (cx[e] - lo[0]) * sc)));
+      const auto qy = static_cast<std::uint32_t>(std::min(65535.0, std::max(0.0, (cy[e] - lo[1]) * sc)));
+      key[e] = (std::uint64_t(spread16(qx) | (spread16(qy) << 1)) << 32) | std::uint64_t(e);
+    }
+    std::sort(key.begin(), key.end());
+
+    std::vector<Index> owner(N, -1), estamp(E, -1), local(N, -1);
+    std::vector<Index> ext, owned, nodes, fresh;
+    Index              tile = 0, q = 0;
+    auto close = [&]() {
+      if (owned.empty()) return;
+      std::sort(ext.begin(), ext.end());
+      nodes.clear();
+      for (Index e : ext)
+        for (int k = 0; k < 3; ++k)
+        {
+          const Index n = conn[3 * e + k];
+          if (local[n] < 0) local[n] = 0, nodes.push_back(n);
+        }
+      std::sort(nodes.begin(), nodes.end());
+      const Index nl = static_cast<Index>(nodes.size());
+      for (Index j = 0; j < nl; ++j) local[nodes[j]] = j;
+      // owned rows in ascending node order: tile offsets and runs of consecutive ids
+      Index t = 0, prev = -2, r0 = static_cast<Index>(p->runs.size() / 2);
+      for (Index n : nodes)
+      {
+        std::uint16_t tf = 0xffff;
+        if (owner[n] == q)
+        {
+          if (n != prev + 1) p->runs.push_back(t), p->runs.push_back(row_ptr[n]);
+          tf   = static_cast<std::uint16_t>(t);
+          t   += rowlen(n);
+          prev = n;
+        }
+        else
+          prev = -2;
+        p->ldof_tile.push_back(tf);
+        p->ldof_dof.push_back(n);
+      }
+      int steps = 0;
+      for (Index e : ext)
+      {
+        p->patch_elem.push_back(e);
+        for (int k = 0; k < 4; ++k)
+        {
+          std::uint16_t w = 0, pw = 0;
+          if (k < 3)
+          {
+            const Index n = conn[3 * e + k];
+            w             = static_cast<std::uint16_t>(local[n]);
+            if (owner[n] == q)
+            {
+              const Index rk = rank_of(n, e);
+              steps          = std::max(steps, static_cast<int>(rk) + 1);
+              w              = static_cast<std::uint16_t>(w | (rk << 12));
+              const Index* b = col_ind + row_ptr[n];
+              const Index* f = col_ind + row_ptr[n + 1];
+              for (int c = 0; c < 3; ++c)
+              {
+                const Index* at = std::lower_bound(b, f, conn[3 * e + c]);
+                fx::require(at != f && *at == conn[3 * e + c], FX_ERR_LAYOUT,
+                            "fx_p1_asm_create: an element column is missing from its row");
+                pw = static_cast<std::uint16_t>(pw | ((at - b) << (4 * c)));
+              }
+            }
+          }
+          p->elem_ldof.push_back(w);
+          p->elem_pos.push_back(pw);
+        }
+      }
+      for (Index n : nodes) local[n] = -1;
+      p->patch_off.push_back(static_cast<Index>(p->patch_elem.size()));
+      p->ldof_off.push_back(static_cast<Index>(p->ldof_dof.size()));
+      p->patch_tile.push_back(t | (steps << 20));
+      p->run_off.push_back(static_cast<Index>(p->runs.size() / 2));
+      p->max_patch_ldofs = std::max(p->max_patch_ldofs, nl);
+      p->max_tile        = std::max(p->max_tile, t);
+      p->max_runs        = std::max(p->max_runs, p->run_off.back() - r0);
+      ext.clear();
+      owned.clear();
+      tile = 0;
+      ++q;
+    };
+    for (Index i = 0; i < E; ++i)
+    {
+      const Index e = static_cast<Index>(key[i] & 0xffffffffu);
+      for (int attempt = 0; attempt < 2; ++attempt)
+      {
+        // nodes of e nobody owns yet, and the elements they would bring
+        fresh.clear();
+        Index add = 0, rows = 0;
+        for (int k = 0; k < 3; ++k)
+        {
+          const Index n = conn[3 * e + k]; // an element's three nodes are distinct
+          if (owner[n] >= 0 || !ownable[n]) continue;
+          fresh.push_back(n);
+          rows += rowlen(n);
+          for (Index t = noff[n]; t < noff[n + 1]; ++t)
+            if (estamp[nel[t]] != q && estamp[nel[t]] != -2 - i) estamp[nel[t]] = -2 - i, ++add;
+        }
+        // undo the trial marks
+        for (Index n : fresh)
+          for (Index t = noff[n]; t < noff[n + 1]; ++t)
+            if (estamp[nel[t]] == -2 - i) estamp[nel[t]] = -1;
+        if (fresh.empty()) break;
+        const bool fits = static_cast<Index>(ext.size()) + add <= max_patch && tile + rows <= max_tile;
+        if (!fits && !owned.empty() && attempt == 0)
+        {
+          close(); // and try again in a new, empty patch
+          continue;
+        }
+        if (!fits)
+        {
+          // too big for an empty patch: these nodes are left to the row kernel
+          for (Index n : fresh) ownable[n] = 0;
+          break;
+        }
+        for (Index n : fresh)
+        {
+          owner[n] = q;
+          owned.push_back(n);
+          for (Index t = noff[n]; t < noff[n + 1]; ++t)
+            if (estamp[nel[t]] != q) estamp[nel[t]] = q, ext.push_back(nel[t]);
+        }
+        tile += rows;
+        break;
+      }
+    }
+    close();
+    for (Index n = 0; n < N; ++n)
+      if (owner[n] < 0) p->rows_left.push_back(n);
+    *out = p.release();
+  });
+}
+
+int fx_p1_asm_info_get(const fx_p1_asm* p, fx_p1_asm_info* info)
+{
+  return fx::guard([&] {
+    fx::require(p && info, FX_ERR_ARG, "fx_p1_asm_info_get: null argument");
+    info->num_patches     = static_cast<Index>(p->patch_off.size()) - 1;
+    info->num_entries     = static_cast<Index>(p->patch_elem.size());
+    info->num_ldofs       = static_cast<Index>(p->ldof_dof.size());
+    info->num_runs        = static_cast<Index>(p->runs.size() / 2);
+    info->num_left        = static_cast<Index>(p->rows_left.size());
+    info->max_patch_ldofs = p->max_patch_ldofs;
+    info->max_tile        = p->max_tile;
+    info->max_runs        = p->max_runs;
+  });
+}
+
+int fx_p1_asm_export(const fx_p1_asm* p, Index* patch_off, Index* patch_elem,
+                     unsigned short* elem_ldof, unsigned short* elem_pos, Index* ldof_off,
+                     Index* ldof_dof, unsigned short* ldof_tile, Index* patch_tile, Index* run_off,
+                     Index* runs, Index* rows_left)
+{
+  return fx::guard([&] {
+    fx::require(p && patch_off && ldof_off && run_off
+                    && (p->patch_elem.empty() || (patch_elem && elem_ldof && elem_pos))
+                    && (p->ldof_dof.empty() || (ldof_dof && ldof_tile))
+                    && (p->patch_tile.empty() || patch_tile) && (p->runs.empty() || runs)
+                    && (p->rows_left.empty() || rows_left),
+                FX_ERR_ARG, "fx_p1_asm_export: null array");
+    std::copy(p->patch_off.begin(), p->patch_off.end(), patch_off);
+    std::copy(p->patch_elem.begin(), p->patch_elem.end(), patch_elem);
+    std::copy(p->elem_ldof.begin(), p->elem_ldof.end(), elem_ldof);
+    std::copy(p->elem_pos.begin(), p->elem_pos.end(), elem_pos);
+    std::copy(p->ldof_off.begin(), p->ldof_off.end(), ldof_off);
+    std::copy(p->ldof_dof.begin(), p->ldof_dof.end(), ldof_dof);
+    std::copy(p->ldof_tile.begin(), p->ldof_tile.end(), ldof_tile);
+    std::copy(p->patch_tile.begin(), p->patch_tile.end(), patch_tile);
+    std::copy(p->run_off.begin(), p->run_off.end(), run_off);
+    std::copy(p->runs.begin(), p->runs.end(), runs);
+    std::copy(p->rows_left.begin(), p->rows_left.end(), rows_left);
+  });
+}
+
+void fx_p1_asm_destroy(fx_p1_asm* p) { delete p; }
+
 // ---------------------------------------------------------------------------
 // CSR transpose permutation: counting sort by column, rows visited in order.
 

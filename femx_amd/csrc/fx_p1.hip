@@ -35,6 +35,20 @@ __global__ __launch_bounds__(256) void k_p1_vjp_seam(SeamArgs a)
   patch_seam_block(fx::xcd_block(gridDim.x), a);
 }
 
+__global__ __launch_bounds__(kPatchLanes) void k_p1_asm_patch(P1AsmArgs a, PatchArgs q, P1AsmPatch s,
+                                                              int max_patch_ldofs, int max_tile,
+                                                              int max_runs)
+{
+  extern __shared__ __align__(16) unsigned char lds[];
+  p1_asm_patch_block(fx::xcd_block(gridDim.x), a, q, s, lds, max_patch_ldofs, max_tile, max_runs);
+}
+
+__global__ __launch_bounds__(kP1Block) void k_p1_assemble_list(P1AsmArgs a, const int* rows, int nrows)
+{
+  __shared__ double smem[kP1Block * kP1ListStride];
+  p1_assemble_list_block(fx::xcd_block(gridDim.x), a, rows, nrows, smem);
+}
+
 } // namespace
 
 extern "C" {
@@ -142,6 +156,62 @@ int fx_p1_vjp_patch(fx_index num_nodes, fx_index num_elems, const double* x, con
       hipLaunchKernelGGL(k_p1_vjp_seam, dim3(fx::blocks_for(q.num_seam, 256)), dim3(256), 0,
                          fx::as_stream(stream), sa);
       fx::check_launch("fx_p1_vjp_patch");
+    }
+  });
+}
+
+int fx_p1_assemble_patch(fx_index num_nodes, fx_index num_elems, fx_index nnz, const double* x,
+                         const double* y, const fx_index* conn, const double* kappa,
+                         int kappa_is_log, const double* f, const double* u,
+                         const fx_index* inc_off, const fx_index* inc, const fx_index* row_ptr,
+                         const fx_p1_asm_plan* plan, double* vals, double* res, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(num_nodes >= 0 && num_elems >= 0 && nnz >= 0 && plan, FX_ERR_ARG,
+                "fx_p1_assemble_patch: negative size or null plan");
+    const fx_p1_asm_plan& s = *plan;
+    fx::require(s.num_patches >= 0 && s.max_patch_ldofs >= 0
+                    && s.max_patch_ldofs <= 3 * kPatchLanes && (num_elems > 0 || s.num_patches == 0)
+                    && s.num_left >= 0 && s.num_left <= num_nodes && s.max_tile >= 0
+                    && s.max_tile < 0xffff && s.max_runs >= 0 && s.max_runs <= s.max_tile,
+                FX_ERR_ARG, "fx_p1_assemble_patch: plan sizes do not fit the mesh");
+    if (num_nodes == 0) return;
+    fx::require(fx::fits32(num_nodes, 8) && fx::fits32(num_elems, 8), FX_ERR_OVERFLOW,
+                "fx_p1_assemble_patch: node or element arrays beyond 4 GiB");
+    fx::require(x && y && inc_off && row_ptr && res
+                    && (s.num_patches == 0
+                        || (s.patch_off && s.patch_elem && s.elem_ldof && s.elem_pos && s.ldof_off
+                            && s.ldof_dof && s.ldof_tile && s.patch_tile && s.run_off && s.runs))
+                    && (num_elems == 0 || (conn && inc)) && (s.num_left == 0 || s.rows_left),
+                FX_ERR_ARG, "fx_p1_assemble_patch: null array");
+    const std::size_t N = num_nodes;
+    fx::require(!fx::overlaps(res, N * 8, vals, std::size_t(nnz) * 8)
+                    && !(u && (fx::overlaps(u, N * 8, res, N * 8)
+                               || fx::overlaps(u, N * 8, vals, std::size_t(nnz) * 8))),
+                FX_ERR_ALIAS, "Assembly state, residual, and matrix values must not alias");
+    const P1AsmArgs a{num_nodes, x,  y,       conn, kappa, kappa_is_log, f, u, inc_off,
+                      reinterpret_cast<const int2*>(inc), row_ptr, vals, res, 0};
+    if (s.num_patches > 0)
+    {
+      const PatchArgs  pa{s.patch_off, s.patch_elem, nullptr, s.elem_ldof, s.ldof_off,
+                         s.ldof_dof,  nullptr,      nullptr, nullptr,     nullptr};
+      const P1AsmPatch ps{s.elem_pos, s.ldof_tile, s.patch_tile, s.run_off,
+                          reinterpret_cast<const int2*>(s.runs)};
+      const int        mx = static_cast<int>(s.max_patch_ldofs), mt = static_cast<int>(s.max_tile);
+      const int        mr = static_cast<int>(s.max_runs);
+      const std::size_t lds = p1_asm_patch_lds(mx, mt, mr);
+      fx::require(lds <= 160 * 1024, FX_ERR_UNSUPPORTED,
+                  "fx_p1_assemble_patch: a patch's tile and nodes exceed the LDS");
+      hipLaunchKernelGGL(k_p1_asm_patch, dim3(static_cast<unsigned>(s.num_patches)),
+                         dim3(kPatchLanes), lds, fx::as_stream(stream), a, pa, ps, mx, mt, mr);
+      fx::check_launch("fx_p1_assemble_patch");
+    }
+    if (s.num_left > 0)
+    {
+      hipLaunchKernelGGL(k_p1_assemble_list, dim3(fx::blocks_for(s.num_left, kP1Block)),
+                         dim3(kP1Block), 0, fx::as_stream(stream), a, s.rows_left,
+                         static_cast<int>(s.num_left));
+      fx::check_launch("fx_p1_assemble_patch");
     }
   });
 }

@@ -186,17 +186,70 @@ __device__ __forceinline__ P1Raw p1_load_raw(const P1AsmArgs& a, int e, int2 n, 
   return R;
 }
 
+// One row of the P1 assembly: the lane walks the row's incidences with the
+// next incidence's vertex indices in flight while the current one computes
+// (the lookahead past the row's last incidence re-reads that incidence,
+// always valid, and is discarded).  The row's own node is loaded once; per
+// incidence only the element's two other vertices and its diffusion value
+// are gathered.  racc is the row's first value: in an LDS tile (tiled; the
+// three positions of an incidence are distinct entries of the lane's own
+// row, so each LDS atomic add makes it old + K in program order, as the host
+// loop adds them) or in a.vals (read, add, write, in order).  The residual
+// is stored; the values stay in racc.
+__device__ __forceinline__ void p1_assemble_row(const P1AsmArgs& a, int r, double* racc, bool tiled,
+                                                const long long* sinc, bool staged, int ib0)
+{
+  double    rsum = 0.0;
+  const int ta = a.inc_off[r], t1 = a.inc_off[r + 1], tl = t1 - 1;
+  if (!tiled && a.vals)
+    for (int k = 0; k < a.row_ptr[r + 1] - a.row_ptr[r]; ++k) racc[k] = 0.0;
+  if (ta < t1)
+  {
+    const P1Own o{a.x[r], a.y[r], a.u ? a.u[r] : 0.0, a.f ? a.f[r] : 0.0};
+    int2        ic = inc_at(sinc, a.inc, staged, ta, ib0);
+    int2        cc = p1_others(a.conn, ic.x, ic.y & 0xff);
+    for (int t = ta; t < t1; ++t)
+    {
+      const int2     in   = inc_at(sinc, a.inc, staged, min(t + 1, tl), ib0);
+      const unsigned meta = static_cast<unsigned>(ic.y);
+      const int      li   = meta & 0xff;
+      const P1Raw    R    = p1_load_raw(a, ic.x, cc, li, o);
+      cc                  = p1_others(a.conn, in.x, in.y & 0xff);
+      const P1Geom   g    = p1_geom(R.x[0], R.y[0], R.x[1], R.y[1], R.x[2], R.y[2]);
+      const double   ke   = (a.kappa && a.kappa_is_log) ? exp(R.kv) : R.kv;
+      const double   k0 = p1_k(g, li, 0, ke), k1 = p1_k(g, li, 1, ke), k2 = p1_k(g, li, 2, ke);
+      double         re = 0.0;
+      if (a.u) re = k0 * R.u[0] + k1 * R.u[1] + k2 * R.u[2];
+      if (a.f) re = re - p1_load(g, li, R.f[0], R.f[1], R.f[2]);
+      else re = re - 0.0;
+      rsum = rsum + re;
+      if (a.vals)
+      {
+        double* p0 = racc + ((meta >> 8) & 0xff);
+        double* p1 = racc + ((meta >> 16) & 0xff);
+        double* p2 = racc + (meta >> 24);
+        if (tiled)
+        {
+          lds_add(p0, k0);
+          lds_add(p1, k1);
+          lds_add(p2, k2);
+        }
+        else
+        {
+          // the lane's own row in global memory: read, add, write, in order
+          *p0 = *p0 + k0;
+          *p1 = *p1 + k1;
+          *p2 = *p2 + k2;
+        }
+      }
+      ic = in;
+    }
+  }
+  __builtin_nontemporal_store(rsum, a.res + r);
+}
+
 // One row block of the P1 assembly (rows [blk*kP1Block, (blk+1)*kP1Block)); smem holds
 // acc_cap doubles of row tile followed by the incidence stage.
-//
-// Each lane walks its row's incidences with the next incidence's vertex
-// indices in flight while the current one computes (the lookahead past the
-// row's last incidence re-reads that incidence, always valid, and is
-// discarded).  The row's own node is loaded once; per incidence only the
-// element's two other vertices and its diffusion value are gathered.  Tile
-// updates are LDS atomic adds: the three positions of an incidence are
-// distinct entries of the lane's own row, so each becomes old + K in program
-// order, as the host loop adds them.
 // kLong: rows may be longer than the tile width (max_row_len > kP1MaxRowLen),
 // so a block whose CSR range does not fit the tile adds in global memory;
 // false compiles the tile path alone.
@@ -221,60 +274,209 @@ __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, d
   const int r = r0 + threadIdx.x;
   if (r < a.num_rows)
   {
-    double*   racc = tiled ? acc + (a.row_ptr[r] - base) : a.vals + a.row_ptr[r];
-    double    rsum = 0.0;
-    const int ta = a.inc_off[r], t1 = a.inc_off[r + 1], tl = t1 - 1;
-    if (!tiled && a.vals)
-      for (int k = 0; k < a.row_ptr[r + 1] - a.row_ptr[r]; ++k) racc[k] = 0.0;
-    if (ta < t1)
-    {
-      const P1Own o{a.x[r], a.y[r], a.u ? a.u[r] : 0.0, a.f ? a.f[r] : 0.0};
-      int2        ic = inc_at(sinc, a.inc, staged, ta, ib0);
-      int2        cc = p1_others(a.conn, ic.x, ic.y & 0xff);
-      for (int t = ta; t < t1; ++t)
-      {
-        const int2     in   = inc_at(sinc, a.inc, staged, min(t + 1, tl), ib0);
-        const unsigned meta = static_cast<unsigned>(ic.y);
-        const int      li   = meta & 0xff;
-        const P1Raw    R    = p1_load_raw(a, ic.x, cc, li, o);
-        cc                  = p1_others(a.conn, in.x, in.y & 0xff);
-        const P1Geom   g    = p1_geom(R.x[0], R.y[0], R.x[1], R.y[1], R.x[2], R.y[2]);
-        const double   ke   = (a.kappa && a.kappa_is_log) ? exp(R.kv) : R.kv;
-        const double   k0 = p1_k(g, li, 0, ke), k1 = p1_k(g, li, 1, ke), k2 = p1_k(g, li, 2, ke);
-        double         re = 0.0;
-        if (a.u) re = k0 * R.u[0] + k1 * R.u[1] + k2 * R.u[2];
-        if (a.f) re = re - p1_load(g, li, R.f[0], R.f[1], R.f[2]);
-        else re = re - 0.0;
-        rsum = rsum + re;
-        if (a.vals)
-        {
-          double* p0 = racc + ((meta >> 8) & 0xff);
-          double* p1 = racc + ((meta >> 16) & 0xff);
-          double* p2 = racc + (meta >> 24);
-          if (tiled)
-          {
-            lds_add(p0, k0);
-            lds_add(p1, k1);
-            lds_add(p2, k2);
-          }
-          else
-          {
-            // the lane's own row in global memory: read, add, write, in order
-            *p0 = *p0 + k0;
-            *p1 = *p1 + k1;
-            *p2 = *p2 + k2;
-          }
-        }
-        ic = in;
-      }
-    }
-    __builtin_nontemporal_store(rsum, a.res + r);
+    // two calls, so that each inlined copy knows its row's address space
+    if (tiled)
+      p1_assemble_row(a, r, acc + (a.row_ptr[r] - base), true, sinc, staged, ib0);
+    else
+      p1_assemble_row(a, r, a.vals + a.row_ptr[r], false, sinc, staged, ib0);
   }
   __syncthreads();
   if (tiled)
     for (int i = threadIdx.x; i < bn; i += kP1Block)
       __builtin_nontemporal_store(acc[i], a.vals + base + i);
   __syncthreads();
+}
+
+// Rows from a list (the rows an element-patch pass leaves: nodes on patch
+// seams), kP1Block per block, one lane per row, each lane with its own LDS
+// row of kP1ListRow values (rows longer than that add in global memory).
+constexpr int kP1ListRow = 16, kP1ListStride = 17; // odd stride: lanes' rows in distinct banks
+
+__device__ __forceinline__ void p1_assemble_list_block(int blk, const P1AsmArgs& a,
+                                                       const int* __restrict__ rows, int nrows,
+                                                       double* smem)
+{
+  const int k = blk * kP1Block + threadIdx.x;
+  if (k >= nrows) return;
+  const int r  = rows[k];
+  const int b0 = a.row_ptr[r], n = a.row_ptr[r + 1] - b0;
+  // two calls, so that each inlined copy knows its row's address space (LDS
+  // atomics for the tile, not flat ones)
+  if (n <= kP1ListRow)
+  {
+    double* racc = smem + threadIdx.x * kP1ListStride;
+    if (a.vals)
+      for (int i = 0; i < n; ++i) racc[i] = 0.0;
+    p1_assemble_row(a, r, racc, true, nullptr, false, 0);
+    if (a.vals)
+      for (int i = 0; i < n; ++i) a.vals[b0 + i] = racc[i];
+  }
+  else
+    p1_assemble_row(a, r, a.vals + b0, false, nullptr, false, 0);
+}
+
+// ---------------------------------------------------------------------------
+// P1 assembly over element patches (the VJP's Morton-ordered patches, their
+// elements in ascending order): one lane per element computes the element's
+// three matrix rows and residual entries once, from node values staged in
+// LDS once per patch.  A node all of whose elements lie in the patch
+// ("finished" here) has its CSR row in the patch's LDS tile and its residual
+// entry in an LDS slot; the lanes add into them in rank steps -- step k adds
+// the rows of every element that is its node's k-th element in ascending
+// element order -- so each value is still old + K in ascending element order,
+// as in p1_assemble_row, and the results are bitwise equal.  The tile holds
+// the finished rows in ascending node order, so consecutive node ids form
+// runs that are contiguous in the CSR values too: the write-back walks the
+// tile with consecutive lanes on consecutive values.  Rows on patch seams
+// (and any a patch cannot tile) are left to p1_assemble_list_block.
+struct P1AsmPatch
+{
+  const unsigned short* elem_pos;   // [4 per patch-ordered element]: word k holds the
+                                    // CSR positions (4 bits each) of the element's
+                                    // three columns in the row of its vertex k
+  const unsigned short* ldof_tile;  // [per local node]: tile offset of a finished row, or kP1NoTile
+  const int*            patch_tile; // [per patch]: tile length | rank steps << 20
+  const int*            run_off;    // [patches + 1]: the patch's runs in `runs`
+  const int2*           runs;       // tile offset and CSR offset of each run
+};
+
+constexpr unsigned kP1NoTile = 0xffffu;
+
+// LDS: the tile, x, y, u, f and the residual slot of each local node, the
+// runs (and an end marker), then each local node's tile offset.
+__host__ __device__ constexpr int p1_asm_patch_lds(int max_patch_ldofs, int max_tile, int max_runs)
+{
+  return (8 * max_tile + 40 * max_patch_ldofs + 8 * (max_runs + 1) + 2 * max_patch_ldofs + 15) / 16
+         * 16;
+}
+
+__device__ __forceinline__ void p1_asm_patch_block(int p, const P1AsmArgs& a, const PatchArgs& q,
+                                                   const P1AsmPatch& s, unsigned char* lds,
+                                                   int max_patch_ldofs, int max_tile, int max_runs)
+{
+  constexpr int   U  = 3; // a P1 patch has at most 3 * kPatchLanes local nodes
+  const int       t  = threadIdx.x;
+  const int       i0 = q.patch_off[p], ne = q.patch_off[p + 1] - i0;
+  const int       l0 = q.ldof_off[p], nl = q.ldof_off[p + 1] - l0;
+  const int       pt = s.patch_tile[p], tlen = pt & 0xfffff, nrank = pt >> 20;
+  const int       u0 = s.run_off[p], nr = s.run_off[p + 1] - u0;
+  const bool      mine = t < ne;
+  const int       i    = i0 + t;
+  double*         tile = reinterpret_cast<double*>(lds);
+  double*         xs   = tile + max_tile;
+  double*         ys   = xs + max_patch_ldofs;
+  double*         us   = ys + max_patch_ldofs;
+  double*         fs   = us + max_patch_ldofs;
+  double*         rt   = fs + max_patch_ldofs;
+  int2*           ru   = reinterpret_cast<int2*>(rt + max_patch_ldofs);
+  unsigned short* ts   = reinterpret_cast<unsigned short*>(ru + max_runs + 1);
+  // the element's tables and coefficient are requested first, so that their
+  // latency overlaps the staging below
+  unsigned w0 = 0, w1 = 0, c0 = 0, c1 = 0;
+  double   kv = 1.0;
+  if (mine)
+  {
+    typedef unsigned u2 __attribute__((ext_vector_type(2)));
+    const u2 lw = __builtin_nontemporal_load(reinterpret_cast<const u2*>(q.elem_ldof + 4 * static_cast<long>(i)));
+    const u2 pw = __builtin_nontemporal_load(reinterpret_cast<const u2*>(s.elem_pos + 4 * static_cast<long>(i)));
+    w0 = lw.x, w1 = lw.y, c0 = pw.x, c1 = pw.y;
+    if (a.kappa) kv = gat32(a.kappa, q.patch_elem[i]);
+  }
+  for (int k = t; k < tlen; k += kPatchLanes) tile[k] = 0.0;
+  for (int k = t; k < nr; k += kPatchLanes) ru[k] = s.runs[u0 + k];
+  if (t == 0) ru[nr] = make_int2(tlen, 0);
+  for (int j0 = t; j0 < nl; j0 += U * kPatchLanes)
+  {
+    int  d[U], tf[U];
+    bool on[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      on[u]       = j0 + u * kPatchLanes < nl;
+      const int j = l0 + j0 + u * kPatchLanes;
+      d[u]        = on[u] ? q.ldof_dof[j] : 0;
+      tf[u]       = on[u] ? s.ldof_tile[j] : kP1NoTile;
+    }
+    double xv[U], yv[U], uv[U], fv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+    {
+      xv[u] = on[u] ? gat32(a.x, d[u]) : 0.0;
+      yv[u] = on[u] ? gat32(a.y, d[u]) : 0.0;
+      uv[u] = on[u] && a.u ? gat32(a.u, d[u]) : 0.0;
+      fv[u] = on[u] && a.f ? gat32(a.f, d[u]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (on[u])
+      {
+        const int j = j0 + u * kPatchLanes;
+        xs[j] = xv[u], ys[j] = yv[u], us[j] = uv[u], fs[j] = fv[u], rt[j] = 0.0;
+        ts[j] = static_cast<unsigned short>(tf[u]);
+      }
+  }
+  __syncthreads();
+  const unsigned h[3]  = {w0 & 0xffffu, w0 >> 16, w1 & 0xffffu};
+  const unsigned cp[3] = {c0 & 0xffffu, c0 >> 16, c1 & 0xffffu};
+  const unsigned j[3]  = {h[0] & 0xfffu, h[1] & 0xfffu, h[2] & 0xfffu};
+  double         K[3][3], R[3];
+  unsigned       tof[3] = {kP1NoTile, kP1NoTile, kP1NoTile};
+  if (mine)
+  {
+    const P1Geom g  = p1_geom(xs[j[0]], ys[j[0]], xs[j[1]], ys[j[1]], xs[j[2]], ys[j[2]]);
+    const double ke = (a.kappa && a.kappa_is_log) ? exp(kv) : kv;
+    const double uu[3] = {us[j[0]], us[j[1]], us[j[2]]};
+    const double ff[3] = {fs[j[0]], fs[j[1]], fs[j[2]]};
+#pragma unroll
+    for (int li = 0; li < 3; ++li)
+    {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) K[li][c] = p1_k(g, li, c, ke);
+      double re = 0.0;
+      if (a.u) re = K[li][0] * uu[0] + K[li][1] * uu[1] + K[li][2] * uu[2];
+      if (a.f) re = re - p1_load(g, li, ff[0], ff[1], ff[2]);
+      else re = re - 0.0;
+      R[li]   = re;
+      tof[li] = ts[j[li]];
+    }
+  }
+  for (int rk = 0; rk < nrank; ++rk)
+  {
+    if (mine)
+    {
+#pragma unroll
+      for (int li = 0; li < 3; ++li)
+        if ((h[li] >> 12) == static_cast<unsigned>(rk) && tof[li] != kP1NoTile)
+        {
+          lds_add(rt + j[li], R[li]);
+          if (a.vals)
+          {
+            double* row = tile + tof[li];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) lds_add(row + ((cp[li] >> (4 * c)) & 0xfu), K[li][c]);
+          }
+        }
+    }
+    __syncthreads();
+  }
+  for (int k = t; k < nl; k += kPatchLanes)
+    if (ts[k] != kP1NoTile) __builtin_nontemporal_store(rt[k], a.res + q.ldof_dof[l0 + k]);
+  if (a.vals)
+  {
+    // value k of the tile lies in run m, the last run starting at or before k
+    for (int k = t; k < tlen; k += kPatchLanes)
+    {
+      int lo = 0, hi = nr - 1;
+      while (lo < hi)
+      {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ru[mid].x <= k) lo = mid;
+        else hi = mid - 1;
+      }
+      __builtin_nontemporal_store(tile[k], a.vals + ru[lo].y + (k - ru[lo].x));
+    }
+  }
+  __syncthreads(); // the LDS is reused by the next patch of a persistent walk
 }
 
 struct P1VjpArgs

@@ -22,9 +22,8 @@ import torch
 
 from . import _lib
 from .assembly import (AssemblyMap, BoundaryMap, Geometry, PoissonP1, assemble, make_assembly_map,
-                       make_boundary_map, replace_res, replace_rows)
+                       make_boundary_map, p1_patches, replace_res, replace_rows)
 from .linalg import CsrMatrix, _check_dev_vec, ptr, stream_handle
-from .patches import Patches
 
 
 class PoissonLogKappaResidual:
@@ -91,9 +90,7 @@ class PoissonLogKappaResidual:
         """The VJP's patch plan on the geometry's device (built on first use;
         VJPs of this residual on different streams must not overlap)."""
         if self._patches is None:
-            g = self.geom
-            self._patches = Patches(g.num_nodes, g.num_elems, 3, g.h_conn, g.h_conn, g.h_x, g.h_y,
-                                    g.device)
+            self._patches = p1_patches(self.geom, self.map)
         return self._patches
 
     def apply_param_jac_t(self, state, prm, adj, out=None, stream=None):

@@ -341,6 +341,78 @@ int fx_p1_vjp_patch(fx_index num_nodes, fx_index num_elems, const double* d_x, c
                     const fx_index* d_bc_mask, const fx_patch_plan* h_plan, double* d_jt_lam,
                     double* d_g, void* stream);
 
+/* fx_p1_assemble over element patches in which every row is finished by
+ * the one patch that owns its node, so no row is split between patches and
+ * none needs a second pass.  Elements are visited in Morton order of their
+ * centroids (vertex coordinates x/y); a node is owned by the patch that
+ * meets it first, and that patch takes all of the node's elements -- its own
+ * and, as a halo, elements of neighbouring patches, so an element near a
+ * patch border is evaluated by each patch that owns one of its nodes.  A
+ * workgroup loads its patch's node values into LDS once, evaluates each
+ * element once (one lane per element), and adds the rows of its owned nodes
+ * into an LDS tile in rank steps: step k adds every element that is its
+ * node's k-th in ascending element order, so each CSR value and residual
+ * entry is summed in ascending element order, as fx_p1_assemble sums it, and
+ * the results are bitwise equal.  The tile holds the owned rows in ascending
+ * node order and is written back along runs of consecutive node ids, which
+ * are contiguous CSR ranges.  Nodes a patch cannot own (rows of more than 16
+ * entries, more than 16 elements, nodes in no element) are left to
+ * fx_p1_assemble's row-per-lane kernel (rows_left).
+ * Plan (host, fx_p1_asm_create on the map's CSR pattern; max_patch <=
+ * FX_TH_PATCH_MAX elements per patch, tiles of at most max_tile values):
+ *   patch_off[P+1], patch_elem   the elements of each patch, ascending;
+ *   elem_ldof[4 i + k]           node k of the patch's i-th element: its local
+ *                                index | its rank among the node's elements
+ *                                << 12 (owned nodes);
+ *   elem_pos[4 i + k]            the CSR positions in node k's row of the
+ *                                element's three columns, 4 bits each
+ *                                (column j at bits 4j; owned nodes);
+ *   ldof_off[P+1], ldof_dof      each patch's local nodes, ascending;
+ *   ldof_tile                    tile offset of an owned node's row, 0xffff
+ *                                for the others;
+ *   patch_tile[P]                tile length | rank steps << 20;
+ *   run_off[P+1], runs           (tile offset, CSR offset) of each run;
+ *   rows_left                    the rows no patch owns, ascending. */
+typedef struct fx_p1_asm fx_p1_asm;
+typedef struct
+{
+  fx_index num_patches, num_entries, num_ldofs, num_runs, num_left;
+  fx_index max_patch_ldofs, max_tile, max_runs;
+} fx_p1_asm_info;
+int fx_p1_asm_create(fx_index num_nodes, fx_index num_elems, const fx_index* h_conn,
+                     const double* h_x, const double* h_y, const fx_index* h_row_ptr,
+                     const fx_index* h_col_ind, fx_index max_patch, fx_index max_tile,
+                     fx_p1_asm** out);
+int fx_p1_asm_info_get(const fx_p1_asm* plan, fx_p1_asm_info* info);
+int fx_p1_asm_export(const fx_p1_asm* plan, fx_index* h_patch_off, fx_index* h_patch_elem,
+                     unsigned short* h_elem_ldof, unsigned short* h_elem_pos,
+                     fx_index* h_ldof_off, fx_index* h_ldof_dof, unsigned short* h_ldof_tile,
+                     fx_index* h_patch_tile, fx_index* h_run_off, fx_index* h_runs,
+                     fx_index* h_rows_left);
+void fx_p1_asm_destroy(fx_p1_asm* plan);
+/* The plan's arrays on the device. */
+typedef struct
+{
+  fx_index              num_patches, num_left, max_patch_ldofs, max_tile, max_runs;
+  const fx_index*       patch_off;
+  const fx_index*       patch_elem;
+  const unsigned short* elem_ldof;
+  const unsigned short* elem_pos;
+  const fx_index*       ldof_off;
+  const fx_index*       ldof_dof;
+  const unsigned short* ldof_tile;
+  const fx_index*       patch_tile;
+  const fx_index*       run_off;
+  const fx_index*       runs;
+  const fx_index*       rows_left;
+} fx_p1_asm_plan;
+int fx_p1_assemble_patch(fx_index num_nodes, fx_index num_elems, fx_index nnz,
+                         const double* d_x, const double* d_y, const fx_index* d_conn,
+                         const double* d_kappa, int kappa_is_log, const double* d_f,
+                         const double* d_u, const fx_index* d_inc_off, const fx_index* d_inc,
+                         const fx_index* d_row_ptr, const fx_p1_asm_plan* h_plan, double* d_vals,
+                         double* d_res, void* stream);
+
 /* ======================================================================== */
 /* Taylor-Hood P2/P1 steady incompressible Navier-Stokes (device).  Not in
  * femx (its NS operator is GLS P1/P1, femx/model/ns/Components.hpp:400-577);
