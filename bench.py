@@ -3,7 +3,8 @@
 Step (the timed unit): on the P1 Poisson 2048x2048 structured triangle mesh
 (8,388,608 elements, 4,198,401 nodes, 29,372,417 CSR entries), FP64,
 kappa_e = exp(0.5 z_e) (seed 2), f ~ N(0,1) (seed 3), u ~ N(0,1) (seed 4):
-  1. assemble the CSR Jacobian and the residual K u - F   (fx_p1_assemble)
+  1. assemble the CSR Jacobian and the residual K u - F over the mesh's
+     owner-computes element patches (fx_p1_assemble_patch)
   2. rhs = -res; Dirichlet elimination on all edges, g = 0 (fx_bc_prepare_forward)
   3. 10 SpMVs y_k = A x_k with x_k ~ U(-1,1) (seeds 10..19), issued as one
      multi-vector product that streams A once (fx_csr_matvec_multi); each
@@ -12,7 +13,8 @@ All inputs are resident in HBM before timing.  `value` is elements per
 second over whole steps.  `roofline` is the step's longest kernel (the
 10-vector SpMV or the assembly, by measured mean duration): its algorithmic
 bytes per launch over its mean duration from HIP events on its stream, with
-the other kernel under `also`.  Assembly bytes: coords, conn, kappa, f, u,
+the other kernel under `also` (the assembly's time is its patch launch and
+its launch over the rows no patch owns, back to back).  Assembly bytes: coords, conn, kappa, f, u,
 femx's jac_map, CSR values, residual; SpMV bytes: CSR values, columns, row
 pointers, 10 x and 10 y vectors.  The
 `cpu_baseline` times the same step on the host with the oracle's OpenMP
@@ -24,6 +26,7 @@ atomic scatter) plus femx's serial host elimination and matvec.
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -279,10 +282,17 @@ def main():
     inc_off, inc = amap.device("inc_off", dev), amap.device("inc", dev)
     stream = stream_handle()
 
-    def assemble_launch():
+    def assemble_rows_launch():
         _lib.call("fx_p1_assemble", N, E, nnz, amap.max_row_len, ptr(geom.x), ptr(geom.y),
                   ptr(geom.conn), ptr(d_kappa), 0, ptr(d_f), ptr(d_u), ptr(inc_off), ptr(inc),
                   ptr(rp), ptr(jac.vals), ptr(res), stream)
+
+    asm_plan = A.p1_asm_plan(geom, amap)
+
+    def assemble_launch():
+        _lib.call("fx_p1_assemble_patch", N, E, nnz, ptr(geom.x), ptr(geom.y), ptr(geom.conn),
+                  ptr(d_kappa), 0, ptr(d_f), ptr(d_u), ptr(inc_off), ptr(inc), ptr(rp),
+                  C.byref(asm_plan.plan), ptr(jac.vals), ptr(res), stream)
 
     def step():
         A.assemble(op, geom, amap, d_u, res, jac)
@@ -295,6 +305,8 @@ def main():
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
+        for _ in range(5):
+            assemble_rows_launch()
         for _ in range(5):
             assemble_launch()
         for _ in range(5):
@@ -362,6 +374,7 @@ def main():
 
     reps = max(args.steps, 10)
     t_asm = time_kernel(assemble_launch, reps)
+    t_asm_rows = time_kernel(assemble_rows_launch, reps)
     t_spmv = time_kernel(lambda: mh.matvec(jac, d_xs[0], d_ys[0]), reps)
     t_spmv10 = time_kernel(lambda: mh.matvec_multi(jac, d_X, d_Y), reps)
     t_bc = time_kernel(lambda: A.prepare_forward_solve(bmap, jac, rhs, g0), reps)
@@ -369,7 +382,11 @@ def main():
     b_spmv = spmv_alg_bytes(N, nnz)
     kernels = {
         "p1_assemble": {"us": t_asm * 1e6, "alg_bytes": b_asm,
-                        "GBs": b_asm / t_asm / 1e9, "elements_per_s": E / t_asm},
+                        "GBs": b_asm / t_asm / 1e9, "elements_per_s": E / t_asm,
+                        "form": "owner-computes element patches + rows no patch owns",
+                        "patches": asm_plan.num_patches, "rows_left": asm_plan.num_left},
+        "p1_assemble_row_per_lane": {"us": t_asm_rows * 1e6, "alg_bytes": b_asm,
+                                     "GBs": b_asm / t_asm_rows / 1e9},
         "csr_spmv": {"us": t_spmv * 1e6, "alg_bytes": b_spmv, "GBs": b_spmv / t_spmv / 1e9},
         "csr_spmv_x10": {"us": t_spmv10 * 1e6, "alg_bytes": spmv_multi_alg_bytes(N, nnz, 10),
                          "GBs": spmv_multi_alg_bytes(N, nnz, 10) / t_spmv10 / 1e9},
@@ -501,7 +518,7 @@ def main():
     # the step and the Taylor-Hood and batched figures ride beside it.
     b_spmv10 = spmv_multi_alg_bytes(N, nnz, 10)
     step_kernels = {
-        "k_p1_assemble": (b_asm, t_asm),
+        "k_p1_asm_patch": (b_asm, t_asm),
         "k_spmv_multi<10>": (b_spmv10, t_spmv10),
     }
 
