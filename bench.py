@@ -378,6 +378,12 @@ def main():
     t_spmv = time_kernel(lambda: mh.matvec(jac, d_xs[0], d_ys[0]), reps)
     t_spmv10 = time_kernel(lambda: mh.matvec_multi(jac, d_X, d_Y), reps)
     t_bc = time_kernel(lambda: A.prepare_forward_solve(bmap, jac, rhs, g0), reps)
+    # the box's achievable HBM rate: a 1 GiB device-to-device copy (read + write)
+    csrc = torch.empty(2 ** 27, dtype=torch.float64, device=dev).fill_(1.0)
+    cdst = torch.empty_like(csrc)
+    t_copy = time_kernel(lambda: cdst.copy_(csrc), 10)
+    copy_GBs = 2 * csrc.numel() * 8 / t_copy / 1e9
+    del csrc, cdst
     b_asm = p1_alg_bytes(N, E, nnz)
     b_spmv = spmv_alg_bytes(N, nnz)
     kernels = {
@@ -534,6 +540,7 @@ def main():
                 "unit": "GB/s", "frac": head["frac"], "traffic": head["traffic"],
                 "kernel": dom, "us": head["us"], "alg_bytes": head["alg_bytes"],
                 "also": {k: roof(k) for k in step_kernels if k != dom},
+                "copy_GBs": copy_GBs, "frac_of_copy": head["achieved"] / copy_GBs,
                 "traffic_source": "profiles/hbm_traffic_p1_2048_th_512_gls_2048.md"}
     th_key = "th_ns_%d" % args.th_n
     if th_key in extra:

@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <memory>
 #include <string>
 #include <thread>
@@ -1216,6 +1217,61 @@ int fx_csr_transpose_map(Index rows, Index cols, const Index* row_ptr, const Ind
         t_row[d]      = r;
         t_perm[d]     = k;
       }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Stencil blocks of the multi-vector product: a full block of 64 rows whose
+// rows all have the same count S (1..max_len) of entries at the same column
+// offsets (col - row, entry by entry) gets the code (start << 5) | S, start
+// indexing its offsets in st_off (equal stencils share one copy); any other
+// block gets -1.
+
+int fx_csr_stencil_plan(Index rows, Index cols, const Index* row_ptr, const Index* col_ind,
+                        Index max_len, Index* num_blocks, Index* num_offsets, Index* blk_code,
+                        Index* st_off)
+{
+  return fx::guard([&] {
+    fx::require(rows >= 0 && cols >= 0 && row_ptr && num_blocks && num_offsets && max_len >= 1
+                    && max_len <= 31,
+                FX_ERR_ARG, "fx_csr_stencil_plan: invalid arguments");
+    fx::require(row_ptr[rows] == 0 || col_ind, FX_ERR_ARG, "fx_csr_stencil_plan: null column array");
+    const Index                     nb = (rows + 63) / 64;
+    std::map<std::vector<Index>, Index> seen;
+    std::vector<Index>              flat, off;
+    for (Index b = 0; b < nb; ++b)
+    {
+      const Index r0 = 64 * b;
+      Index       code = -1;
+      const Index S    = row_ptr[r0 + 1 <= rows ? r0 + 1 : rows] - row_ptr[r0];
+      if (r0 + 64 <= rows && S >= 1 && S <= max_len)
+      {
+        off.resize(static_cast<std::size_t>(S));
+        for (Index k = 0; k < S; ++k) off[k] = col_ind[row_ptr[r0] + k] - r0;
+        bool same = true;
+        for (Index r = r0 + 1; r < r0 + 64 && same; ++r)
+        {
+          same = row_ptr[r + 1] - row_ptr[r] == S;
+          for (Index k = 0; k < S && same; ++k) same = col_ind[row_ptr[r] + k] - r == off[k];
+        }
+        if (same)
+        {
+          auto it = seen.find(off);
+          if (it == seen.end())
+          {
+            fx::require(flat.size() + off.size() < (std::size_t(1) << 26), FX_ERR_UNSUPPORTED,
+                        "fx_csr_stencil_plan: too many distinct stencils");
+            it = seen.emplace(off, static_cast<Index>(flat.size())).first;
+            flat.insert(flat.end(), off.begin(), off.end());
+          }
+          code = (it->second << 5) | S;
+        }
+      }
+      if (blk_code) blk_code[b] = code;
+    }
+    if (st_off) std::copy(flat.begin(), flat.end(), st_off);
+    *num_blocks  = nb;
+    *num_offsets = static_cast<Index>(flat.size());
   });
 }
 

@@ -158,6 +158,135 @@ __global__ __launch_bounds__(kRowBlock) void k_spmv_multi(
   }
 }
 
+// The same products with stencil blocks (fx_csr_stencil_plan).  In a block
+// of 64 rows that share one column-offset stencil, entry k of every row
+// reads the vectors at row + off_k: a run of consecutive offsets (a band) is
+// one coalesced 64-value load per vector, at the band's first offset, and
+// each next offset in the band is the same registers moved down one lane
+// (DPP wave_shl:1: lane i takes lane i+1) with the one new value, at row 63
+// + offset, filled into lane 63.  Gathering instead, each entry is a
+// separate request per vector for lines the wave has just read (the L1 does
+// not keep them across the resident waves' entries).  One wave per 64-row
+// block; the block's matrix range (64 S contiguous values) is staged in the
+// wave's LDS.  Other blocks gather as k_spmv_multi does, unstaged.  Every
+// sum is formed in CSR order as in k_spmv_multi, so each product equals the
+// single-vector one bit for bit.  X's offsets fit 32 bits (checked by the
+// entry point).
+constexpr int kStWaves = 4;
+constexpr int kStMaxLen = FX_STENCIL_MAX_LEN; // longest stencil read as one (stage: 64 x 8 values per wave)
+
+__device__ __forceinline__ double wave_shl1(double v, double fill)
+{
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x130, 0xf,
+                                             0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x130, 0xf,
+                                             0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <int NV>
+__global__ __launch_bounds__(64 * kStWaves) void k_spmv_multi_st(
+    int rows, const int* __restrict__ row_ptr, const int* __restrict__ col,
+    const double* __restrict__ vals, const int* __restrict__ blk_code,
+    const int* __restrict__ st_off, const double* __restrict__ X, long ldx,
+    double* __restrict__ Y, long ldy, double alpha, double beta)
+{
+  __shared__ double sv_all[kStWaves][64 * kStMaxLen];
+  const int wv   = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int b    = fx::xcd_block(gridDim.x) * kStWaves + wv;
+  const int r0   = 64 * b;
+  if (r0 >= rows) return;
+  const int r    = r0 + lane;
+  const int code = blk_code[b];
+  double    acc[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) acc[j] = 0.0;
+  if (code >= 0 && (code & 31) <= kStMaxLen)
+  {
+    const int S = code & 31, so = code >> 5;
+    const int base = row_ptr[r0];
+    double*   sv   = sv_all[wv];
+    const int n    = 64 * S;
+    for (int i = lane; i < n; i += 4 * 64)
+    {
+      double v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = i + 64 * u < n ? __builtin_nontemporal_load(vals + base + i + 64 * u) : 0.0;
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (i + 64 * u < n) sv[i + 64 * u] = v[u];
+    }
+    fxk::wave_sync();
+    const double* m = sv + lane * S;
+    for (int k = 0; k < S;)
+    {
+      const int o = st_off[so + k];
+      int       w = 1;
+      while (k + w < S && st_off[so + k + w] == o + w) ++w;
+      double cur[NV];
+#pragma unroll
+      for (int j = 0; j < NV; ++j) cur[j] = fxk::gat32(X + j * ldx, r + o);
+#pragma unroll
+      for (int j = 0; j < NV; ++j) acc[j] = acc[j] + m[k] * cur[j];
+      for (int d = 1; d < w; ++d)
+      {
+        const int    t = r0 + 63 + o + d; // the value new to the band: lane 63's
+        const double mk = m[k + d];
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+        {
+          cur[j] = wave_shl1(cur[j], X[j * ldx + t]);
+          acc[j] = acc[j] + mk * cur[j];
+        }
+      }
+      k += w;
+    }
+  }
+  else if (r < rows)
+  {
+    const int a = row_ptr[r], e = row_ptr[r + 1];
+    for (int k = a; k < e; ++k)
+    {
+      const double v = vals[k];
+      const int    c = col[k];
+      double       xv[NV];
+#pragma unroll
+      for (int j = 0; j < NV; ++j) xv[j] = fxk::gat32(X + j * ldx, c);
+#pragma unroll
+      for (int j = 0; j < NV; ++j) acc[j] = acc[j] + v * xv[j];
+    }
+  }
+  if (r < rows)
+  {
+#pragma unroll
+    for (int j = 0; j < NV; ++j)
+    {
+      double* y = Y + j * ldy + r;
+      __builtin_nontemporal_store(beta == 0.0 ? alpha * acc[j] : alpha * acc[j] + beta * *y, y);
+    }
+  }
+}
+
+template <int NV = kMaxVec>
+void launch_multi_st_n(int n, int rows, const int* row_ptr, const int* col, const double* vals,
+                       const int* blk_code, const int* st_off, const double* X, long ldx, double* Y,
+                       long ldy, double alpha, double beta, hipStream_t st)
+{
+  if constexpr (NV >= 1)
+  {
+    if (n == NV)
+    {
+      const unsigned nb = fx::blocks_for(rows, 64 * kStWaves);
+      hipLaunchKernelGGL(k_spmv_multi_st<NV>, dim3(nb), dim3(64 * kStWaves), 0, st, rows, row_ptr,
+                         col, vals, blk_code, st_off, X, ldx, Y, ldy, alpha, beta);
+    }
+    else
+      launch_multi_st_n<NV - 1>(n, rows, row_ptr, col, vals, blk_code, st_off, X, ldx, Y, ldy,
+                                alpha, beta, st);
+  }
+}
+
 template <int NV>
 void launch_multi(int rows, const int* row_ptr, const int* col, const short* off, const double* vals,
                   const double* X, long ldx, double* Y, long ldy, double alpha, double beta,
@@ -377,6 +506,35 @@ int fx_csr_matvec_multi(fx_index rows, fx_index cols, const fx_index* row_ptr,
                      X + j0 * ldx, static_cast<long>(ldx), Y + j0 * ldy, static_cast<long>(ldy),
                      alpha, beta, fx::as_stream(stream));
       fx::check_launch("fx_csr_matvec_multi");
+    }
+  });
+}
+
+int fx_csr_matvec_multi_st(fx_index rows, fx_index cols, const fx_index* row_ptr,
+                           const fx_index* col_ind, const double* vals, const fx_index* blk_code,
+                           const fx_index* st_off, fx_index nv, const double* X, fx_index ldx,
+                           double* Y, fx_index ldy, double alpha, double beta, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(rows >= 0 && cols >= 0 && nv >= 0, FX_ERR_ARG,
+                "fx_csr_matvec_multi_st: negative size");
+    if (rows == 0 || nv == 0) return;
+    fx::require(row_ptr && col_ind && vals && blk_code && st_off && X && Y, FX_ERR_ARG,
+                "fx_csr_matvec_multi_st: null array");
+    fx::require(ldx >= cols && ldy >= rows, FX_ERR_ARG,
+                "fx_csr_matvec_multi_st: leading dimension smaller than the vector length");
+    fx::require(fx::fits32(cols, 8), FX_ERR_OVERFLOW,
+                "fx_csr_matvec_multi_st: a vector exceeds 4 GiB");
+    const std::size_t xs = std::size_t(ldx) * (nv - 1) + cols, ys = std::size_t(ldy) * (nv - 1) + rows;
+    fx::require(!fx::overlaps(X, xs * 8, Y, ys * 8), FX_ERR_ALIAS,
+                "CSR matvec input and output must not alias");
+    for (fx_index j0 = 0; j0 < nv; j0 += kMaxVec)
+    {
+      const int n = static_cast<int>(std::min<fx_index>(kMaxVec, nv - j0));
+      launch_multi_st_n(n, rows, row_ptr, col_ind, vals, blk_code, st_off, X + j0 * ldx,
+                        static_cast<long>(ldx), Y + j0 * ldy, static_cast<long>(ldy), alpha, beta,
+                        fx::as_stream(stream));
+      fx::check_launch("fx_csr_matvec_multi_st");
     }
   });
 }

@@ -10,6 +10,7 @@ arithmetic is the library's.
 """
 from __future__ import annotations
 
+import ctypes as C
 import itertools
 
 import numpy as np
@@ -86,6 +87,28 @@ class CsrPattern:
             off = np.empty(max(self.nnz, 1), np.int16)
             rc = _lib.lib().fx_csr_col_off16(self.rows, ptr(self.row_ptr), ptr(self.col_ind), ptr(off))
             self._off16[key] = torch.from_numpy(off[: self.nnz]).to(dev) if rc == 0 else None
+        return self._off16[key]
+
+    STENCIL_MAX_LEN = 8  # FX_STENCIL_MAX_LEN
+
+    def stencil_plan(self, device="cuda"):
+        """The 64-row stencil blocks of the multi-vector product
+        (fx_csr_stencil_plan), built once per device: (blk_code, st_off) device
+        tensors and the fraction of rows in stencil blocks."""
+        dev = torch.device(device)
+        key = "stencil:" + str(dev)
+        if key not in self._off16:
+            nb, no = C.c_int32(), C.c_int32()
+            args = (self.rows, self.cols, ptr(self.row_ptr), ptr(self.col_ind),
+                    self.STENCIL_MAX_LEN)
+            _lib.call("fx_csr_stencil_plan", *args, C.byref(nb), C.byref(no), None, None)
+            code = np.empty(max(nb.value, 1), np.int32)
+            off = np.empty(max(no.value, 1), np.int32)
+            _lib.call("fx_csr_stencil_plan", *args, C.byref(nb), C.byref(no), ptr(code), ptr(off))
+            code = code[: nb.value]
+            frac = 64.0 * np.count_nonzero(code >= 0) / max(self.rows, 1)
+            self._off16[key] = (torch.from_numpy(code).to(dev), torch.from_numpy(off).to(dev),
+                                frac, code)
         return self._off16[key]
 
     def _host_transpose(self):
@@ -180,16 +203,30 @@ class MatrixHandler:
                       ptr(y), float(alpha), float(beta), stream_handle(self.stream))
         return y
 
-    def matvec_multi(self, mat: CsrMatrix, X, Y, alpha=1.0, beta=0.0):
+    STENCIL_MIN_FRAC = 0.5  # the stencil kernel when at least this share of rows is in stencil blocks
+
+    def matvec_multi(self, mat: CsrMatrix, X, Y, alpha=1.0, beta=0.0, stencil=None):
         """Y[j] = alpha*A*X[j] + beta*Y[j] for each row j of the 2-D tensors
         X [nv, cols] and Y [nv, rows]: nv products that read the matrix once
-        (per 16 vectors); each equals matvec's result bit for bit."""
+        (per 16 vectors); each equals matvec's result bit for bit.  When at
+        least half of the rows lie in 64-row blocks that share one
+        column-offset stencil (structured meshes; the plan is built on first
+        use), those blocks read each band of the vectors once, coalesced
+        (fx_csr_matvec_multi_st); otherwise every row gathers its entries
+        (fx_csr_matvec_multi).  `stencil` True/False forces either kernel."""
         _require(X.dim() == 2 and Y.dim() == 2 and X.shape[0] == Y.shape[0]
                  and X.shape[1] == mat.cols and Y.shape[1] == mat.rows
                  and X.is_contiguous() and Y.is_contiguous() and X.dtype == torch.float64
                  and Y.dtype == torch.float64 and X.is_cuda and Y.is_cuda,
                  "CSR multi-vector product needs contiguous CUDA float64 [nv, n] tensors")
         rp, ci = mat.pattern.device_arrays(mat.device)
+        if stencil is not False and mat.rows > 0:
+            code, st_off, frac, _ = mat.pattern.stencil_plan(mat.device)
+            if stencil or frac >= self.STENCIL_MIN_FRAC:
+                _lib.call("fx_csr_matvec_multi_st", mat.rows, mat.cols, ptr(rp), ptr(ci),
+                          ptr(mat.vals), ptr(code), ptr(st_off), X.shape[0], ptr(X), mat.cols,
+                          ptr(Y), mat.rows, float(alpha), float(beta), stream_handle(self.stream))
+                return Y
         off = mat.pattern.col_off16(mat.device)
         _lib.call("fx_csr_matvec_multi", mat.rows, mat.cols, ptr(rp),
                   None if off is not None else ptr(ci), ptr(off) if off is not None else None,

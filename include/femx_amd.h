@@ -187,6 +187,17 @@ void fx_map_destroy(fx_map* map);
 int fx_csr_transpose_map(fx_index rows, fx_index cols, const fx_index* h_row_ptr,
                          const fx_index* h_col_ind, fx_index* h_t_ptr, fx_index* h_t_row,
                          fx_index* h_t_perm);
+/* Stencil blocks of fx_csr_matvec_multi_st (host; no counterpart in femx,
+ * whose products go one vector at a time): rows in blocks of 64,
+ * num_blocks = ceil(rows/64).  A full block whose rows all hold S entries
+ * (1 <= S <= max_len <= 31) at the same column offsets col - row gets
+ * blk_code = (start << 5) | S, its S offsets at st_off[start..]; equal
+ * stencils share one copy; other blocks get -1.  Call with null blk_code and
+ * st_off for num_blocks and num_offsets. */
+#define FX_STENCIL_MAX_LEN 8 /* the longest stencil fx_csr_matvec_multi_st reads as one */
+int fx_csr_stencil_plan(fx_index rows, fx_index cols, const fx_index* h_row_ptr,
+                        const fx_index* h_col_ind, fx_index max_len, fx_index* num_blocks,
+                        fx_index* num_offsets, fx_index* h_blk_code, fx_index* h_st_off);
 
 /* ======================================================================== */
 /* Boundary map (host).  Replaces femx::assembly::makeBoundaryMap
@@ -274,6 +285,17 @@ int fx_csr_matvec_multi(fx_index rows, fx_index cols, const fx_index* d_row_ptr,
                         const fx_index* d_col_ind, const int16_t* d_col_off, const double* d_vals,
                         fx_index nv, const double* d_X, fx_index ldx, double* d_Y, fx_index ldy,
                         double alpha, double beta, void* stream);
+/* The same nv products over the stencil blocks of fx_csr_stencil_plan (its
+ * blk_code and st_off on the device): in a stencil block each band of
+ * consecutive offsets is one coalesced load per vector, its other offsets
+ * the same values moved across lanes; other blocks (and codes with S above
+ * FX_STENCIL_MAX_LEN) gather per entry.  Each product equals
+ * fx_csr_matvec's bit for bit.  Vectors under 4 GiB (FX_ERR_OVERFLOW). */
+int fx_csr_matvec_multi_st(fx_index rows, fx_index cols, const fx_index* d_row_ptr,
+                           const fx_index* d_col_ind, const double* d_vals,
+                           const fx_index* d_blk_code, const fx_index* d_st_off, fx_index nv,
+                           const double* d_X, fx_index ldx, double* d_Y, fx_index ldy,
+                           double alpha, double beta, void* stream);
 /* gather: dst[i] = src[idx[i]]; scatter: dst[idx[i]] = src[i], i < n
  * (VectorHandler::gather/scatter, femx/linalg/handler/VectorHandler.cpp:
  * 46-80).  Indices must be in range (the host-side builders check them);

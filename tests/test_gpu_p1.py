@@ -289,10 +289,39 @@ def test_multi_vector_spmv_equals_single_products(bandwidth, nv):
     for alpha, beta in ((1.0, 0.0), (-0.7, 1.3)):
         Y = Y0.clone()
         mh.matvec_multi(m, X, Y, alpha, beta)
+        Ys = Y0.clone()  # the stencil kernel: no stencil block here, every row gathers
+        mh.matvec_multi(m, X, Ys, alpha, beta, stencil=True)
         for j in range(nv):
             yj = Y0[j].clone()
             mh.matvec(m, X[j], yj, alpha, beta)
             assert torch.equal(Y[j], yj), (j, alpha)
+            assert torch.equal(Ys[j], yj), ("stencil kernel", j, alpha)
+
+
+@pytest.mark.parametrize("n,nv", [(300, 10), (300, 3), (129, 17), (512, 1)])
+def test_multi_vector_spmv_stencil_blocks_equal_single_products(n, nv):
+    """The stencil-block kernel on structured P1 patterns (interior 7-point and
+    edge-row 5-point stencil blocks beside gathered blocks, a partial last
+    block; 16 + 1 vectors at nv = 17): every product bitwise equal to its
+    single-vector matvec, with alpha/beta, and to the gathering kernel."""
+    geom, amap = p1_setup(n)
+    pat = amap.pattern
+    code, _, frac, hcode = pat.stencil_plan(DEV)
+    assert frac > 0.3 and np.any(hcode >= 0) and np.any(hcode < 0)
+    rng = np.random.default_rng(40 + n + nv)
+    m = CsrMatrix(pat, DEV, dev(rng.standard_normal(pat.nnz)))
+    X = dev(rng.uniform(-1.0, 1.0, (nv, pat.cols)))
+    Y0 = dev(rng.standard_normal((nv, pat.rows)))
+    mh = MatrixHandler()
+    for alpha, beta in ((1.0, 0.0), (0.3, -2.0)):
+        Ys, Yg = Y0.clone(), Y0.clone()
+        mh.matvec_multi(m, X, Ys, alpha, beta, stencil=True)
+        mh.matvec_multi(m, X, Yg, alpha, beta, stencil=False)
+        assert torch.equal(Ys, Yg), alpha
+        for j in range(nv):
+            yj = Y0[j].clone()
+            mh.matvec(m, X[j], yj, alpha, beta)
+            assert torch.equal(Ys[j], yj), (j, alpha)
 
 
 def test_dot_is_deterministic_and_accurate():

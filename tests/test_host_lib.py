@@ -192,6 +192,65 @@ def test_col_off16():
     assert edge.col_off16("cpu").tolist() == [32767, 0]
 
 
+def test_stencil_plan_codes_reproduce_columns():
+    """fx_csr_stencil_plan on the P1 mesh (300 x 300 cells, 90601 rows): a
+    block gets a code exactly when its 64 rows share one offset stencil, and
+    the code's offsets rebuild every column; blocks inside a grid row share
+    the one 7-point stencil; a random pattern has no stencil blocks."""
+    n = 300
+    x = np.empty((n + 1) ** 2)
+    y = np.empty_like(x)
+    conn = np.empty(6 * n * n, np.int32)
+    _lib.call("fx_mesh_p1", n, ptr(x), ptr(y), ptr(conn))
+    pat = make_assembly_map(x.size, x.size, conn.reshape(-1, 3)).pattern
+    nb, no = C.c_int32(), C.c_int32()
+    args = (pat.rows, pat.cols, ptr(pat.row_ptr), ptr(pat.col_ind), 8)
+    _lib.call("fx_csr_stencil_plan", *args, C.byref(nb), C.byref(no), None, None)
+    code = np.empty(nb.value, np.int32)
+    off = np.empty(max(no.value, 1), np.int32)
+    _lib.call("fx_csr_stencil_plan", *args, C.byref(nb), C.byref(no), ptr(code), ptr(off))
+    assert nb.value == -(-pat.rows // 64)
+    rp, ci = pat.row_ptr, pat.col_ind
+    for b in range(nb.value):
+        r0 = 64 * b
+        rows = range(r0, min(r0 + 64, pat.rows))
+        stencils = {tuple(ci[rp[r]:rp[r + 1]] - r) for r in rows}
+        uniform = len(rows) == 64 and len(stencils) == 1 and 1 <= len(next(iter(stencils))) <= 8
+        assert (code[b] >= 0) == uniform, b
+        if code[b] >= 0:
+            S, start = code[b] & 31, code[b] >> 5
+            assert tuple(off[start:start + S]) == next(iter(stencils))
+    # the interior 7-point stencil holds most blocks; the bottom and top grid
+    # rows have 5-point stencils of their own
+    used = code[code >= 0]
+    vals_, counts_ = np.unique(used, return_counts=True)
+    top = vals_[np.argmax(counts_)]
+    assert top & 31 == 7
+    assert tuple(off[(top >> 5):(top >> 5) + 7]) == (-(n + 2), -(n + 1), -1, 0, 1, n + 1, n + 2)
+    assert {c & 31 for c in vals_} == {5, 7}
+    assert used.size >= nb.value // 2
+    # distinct stencils are stored once
+    starts = sorted({c >> 5 for c in code if c >= 0})
+    assert no.value == sum(int(code[np.nonzero((code >> 5) == s_)[0][0]] & 31) for s_ in starts)
+    rnd = _xs_rand_pattern()
+    _lib.call("fx_csr_stencil_plan", rnd.rows, rnd.cols, ptr(rnd.row_ptr), ptr(rnd.col_ind), 8,
+              C.byref(nb), C.byref(no), None, None)
+    code = np.empty(nb.value, np.int32)
+    _lib.call("fx_csr_stencil_plan", rnd.rows, rnd.cols, ptr(rnd.row_ptr), ptr(rnd.col_ind), 8,
+              C.byref(nb), C.byref(no), ptr(code), ptr(off))
+    assert np.all(code == -1) and no.value == 0
+
+
+def _xs_rand_pattern():
+    rng = np.random.default_rng(5)
+    rows, cols = 3000, 3100
+    counts = rng.integers(1, 9, rows)
+    ci = np.concatenate([np.sort(rng.choice(np.arange(max(0, r - 600), min(cols, r + 600)), c,
+                                            replace=False)) for r, c in enumerate(counts)])
+    rp = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    return CsrPattern(rows, cols, rp, ci.astype(np.int32))
+
+
 @pytest.mark.parametrize("n", [1, 3, 16])
 def test_meshes_match_oracle(n):
     N, E = (n + 1) ** 2, 2 * n * n
