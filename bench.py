@@ -3,18 +3,23 @@
 Step (the timed unit): on the P1 Poisson 2048x2048 structured triangle mesh
 (8,388,608 elements, 4,198,401 nodes, 29,372,417 CSR entries), FP64,
 kappa_e = exp(0.5 z_e) (seed 2), f ~ N(0,1) (seed 3), u ~ N(0,1) (seed 4):
-  1. assemble the CSR Jacobian and the residual K u - F over the mesh's
-     owner-computes element patches (fx_p1_assemble_patch)
+  1. assemble the CSR Jacobian and the residual K u - F: 97% of the rows lie
+     in 64-row blocks that share one incidence stencil, assembled from
+     coalesced loads of their columns' vertex data (fx_p1_assemble_stencil);
+     the rest gather per row
   2. rhs = -res; Dirichlet elimination on all edges, g = 0 (fx_bc_prepare_forward)
   3. 10 SpMVs y_k = A x_k with x_k ~ U(-1,1) (seeds 10..19), issued as one
-     multi-vector product that streams A once (fx_csr_matvec_multi); each
-     y_k equals the single-vector fx_csr_matvec result bit for bit
+     multi-vector product that streams A once, its 64-row stencil blocks
+     reading each band of the vectors with one coalesced load
+     (fx_csr_matvec_multi_st); each y_k equals the single-vector
+     fx_csr_matvec result bit for bit
 All inputs are resident in HBM before timing.  `value` is elements per
 second over whole steps.  `roofline` is the step's longest kernel (the
 10-vector SpMV or the assembly, by measured mean duration): its algorithmic
 bytes per launch over its mean duration from HIP events on its stream, with
-the other kernel under `also` (the assembly's time is its patch launch and
-its launch over the rows no patch owns, back to back).  Assembly bytes: coords, conn, kappa, f, u,
+the other kernel under `also`.  The general-mesh forms of both kernels
+(element-patch and row-per-lane assembly, gathering SpMV) are timed beside
+them under `kernels`.  Assembly bytes: coords, conn, kappa, f, u,
 femx's jac_map, CSR values, residual; SpMV bytes: CSR values, columns, row
 pointers, 10 x and 10 y vectors.  The
 `cpu_baseline` times the same step on the host with the oracle's OpenMP
@@ -288,11 +293,18 @@ def main():
                   ptr(rp), ptr(jac.vals), ptr(res), stream)
 
     asm_plan = A.p1_asm_plan(geom, amap)
+    st_plan = A.p1_stencil_plan(geom, amap)
 
-    def assemble_launch():
+    def assemble_patch_launch():
         _lib.call("fx_p1_assemble_patch", N, E, nnz, ptr(geom.x), ptr(geom.y), ptr(geom.conn),
                   ptr(d_kappa), 0, ptr(d_f), ptr(d_u), ptr(inc_off), ptr(inc), ptr(rp),
                   C.byref(asm_plan.plan), ptr(jac.vals), ptr(res), stream)
+
+    def assemble_launch():
+        _lib.call("fx_p1_assemble_stencil", N, E, nnz, amap.max_row_len, ptr(geom.x),
+                  ptr(geom.y), ptr(geom.conn), ptr(d_kappa), 0, ptr(d_f), ptr(d_u),
+                  ptr(inc_off), ptr(inc), ptr(rp), ptr(st_plan.blk_code), ptr(st_plan.words),
+                  ptr(jac.vals), ptr(res), stream)
 
     def step():
         A.assemble(op, geom, amap, d_u, res, jac)
@@ -308,9 +320,13 @@ def main():
         for _ in range(5):
             assemble_rows_launch()
         for _ in range(5):
+            assemble_patch_launch()
+        for _ in range(5):
             assemble_launch()
         for _ in range(5):
             mh.matvec(jac, d_xs[0], d_ys[0])
+        for _ in range(5):
+            mh.matvec_multi(jac, d_X, d_Y, stencil=False)
         for _ in range(5):
             mh.matvec_multi(jac, d_X, d_Y)
         for _ in range(5):
@@ -375,8 +391,10 @@ def main():
     reps = max(args.steps, 10)
     t_asm = time_kernel(assemble_launch, reps)
     t_asm_rows = time_kernel(assemble_rows_launch, reps)
+    t_asm_patch = time_kernel(assemble_patch_launch, reps)
     t_spmv = time_kernel(lambda: mh.matvec(jac, d_xs[0], d_ys[0]), reps)
     t_spmv10 = time_kernel(lambda: mh.matvec_multi(jac, d_X, d_Y), reps)
+    t_spmv10_g = time_kernel(lambda: mh.matvec_multi(jac, d_X, d_Y, stencil=False), reps)
     t_bc = time_kernel(lambda: A.prepare_forward_solve(bmap, jac, rhs, g0), reps)
     # the box's achievable HBM rate: a 1 GiB device-to-device copy (read + write)
     csrc = torch.empty(2 ** 27, dtype=torch.float64, device=dev).fill_(1.0)
@@ -389,13 +407,22 @@ def main():
     kernels = {
         "p1_assemble": {"us": t_asm * 1e6, "alg_bytes": b_asm,
                         "GBs": b_asm / t_asm / 1e9, "elements_per_s": E / t_asm,
-                        "form": "owner-computes element patches + rows no patch owns",
-                        "patches": asm_plan.num_patches, "rows_left": asm_plan.num_left},
+                        "form": "64-row stencil blocks (the rest row-per-lane)",
+                        "rows_in_stencil_blocks": st_plan.frac},
+        "p1_assemble_patches": {"us": t_asm_patch * 1e6, "alg_bytes": b_asm,
+                                "GBs": b_asm / t_asm_patch / 1e9,
+                                "form": "owner-computes element patches (any mesh)",
+                                "patches": asm_plan.num_patches, "rows_left": asm_plan.num_left},
         "p1_assemble_row_per_lane": {"us": t_asm_rows * 1e6, "alg_bytes": b_asm,
                                      "GBs": b_asm / t_asm_rows / 1e9},
         "csr_spmv": {"us": t_spmv * 1e6, "alg_bytes": b_spmv, "GBs": b_spmv / t_spmv / 1e9},
         "csr_spmv_x10": {"us": t_spmv10 * 1e6, "alg_bytes": spmv_multi_alg_bytes(N, nnz, 10),
-                         "GBs": spmv_multi_alg_bytes(N, nnz, 10) / t_spmv10 / 1e9},
+                         "GBs": spmv_multi_alg_bytes(N, nnz, 10) / t_spmv10 / 1e9,
+                         "form": "64-row stencil blocks, bands by lane shifts",
+                         "rows_in_stencil_blocks": jac.pattern.stencil_plan(dev)[2]},
+        "csr_spmv_x10_gathered": {"us": t_spmv10_g * 1e6,
+                                  "GBs": spmv_multi_alg_bytes(N, nnz, 10) / t_spmv10_g / 1e9,
+                                  "form": "every entry gathered (any pattern)"},
         "bc_prepare_forward": {"us": t_bc * 1e6},
     }
 
@@ -524,8 +551,8 @@ def main():
     # the step and the Taylor-Hood and batched figures ride beside it.
     b_spmv10 = spmv_multi_alg_bytes(N, nnz, 10)
     step_kernels = {
-        "k_p1_asm_patch": (b_asm, t_asm),
-        "k_spmv_multi<10>": (b_spmv10, t_spmv10),
+        "k_p1_asm_stencil": (b_asm, t_asm),
+        "k_spmv_multi_st<10>": (b_spmv10, t_spmv10),
     }
 
     def roof(name):

@@ -48,6 +48,9 @@ _SIGS = {
     "fx_csr_matvec": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _vp], C.c_int),
     "fx_csr_matvec16": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp, _f64, _f64, _vp], C.c_int),
     "fx_csr_col_off16": ([_i32, _vp, _vp, _vp], C.c_int),
+    "fx_p1_stencil_plan": ([_i32, _vp, _vp, _vp, _vp, _i32, _i32, _vp, _vp, _vp, _vp], C.c_int),
+    "fx_p1_assemble_stencil": ([_i32] * 4 + [_vp] * 3 + [_vp, C.c_int] + [_vp] * 9 + [_vp],
+                               C.c_int),
     "fx_csr_stencil_plan": ([_i32, _i32, _vp, _vp, _i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_csr_matvec_multi_st": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32,
                                 _f64, _f64, _vp], C.c_int),

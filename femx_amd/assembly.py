@@ -322,17 +322,59 @@ def p1_asm_plan(geom: Geometry, amap: AssemblyMap):
     return p[1]
 
 
+class P1StencilPlan:
+    """The P1 assembly's 64-row stencil blocks (fx_p1_stencil_plan) on one
+    device: `blk_code` and `words` device tensors, `frac` the share of rows in
+    uniform blocks."""
+
+    MAX, BAND = 8, 208  # FX_P1_STENCIL_MAX, FX_P1_STENCIL_BAND
+
+    def __init__(self, amap: "AssemblyMap", device):
+        inc_off, inc = amap.rowinc3()
+        rp, ci = amap.pattern.row_ptr, amap.pattern.col_ind
+        N = amap.pattern.rows
+        nb, nw = C.c_int32(), C.c_int32()
+        args = (N, ptr(rp), ptr(ci), ptr(inc_off), ptr(inc), self.MAX, self.BAND)
+        _lib.call("fx_p1_stencil_plan", *args, C.byref(nb), C.byref(nw), None, None)
+        code = np.empty(max(nb.value, 1), np.int32)
+        words = np.empty(max(nw.value, 1), np.int32)
+        _lib.call("fx_p1_stencil_plan", *args, C.byref(nb), C.byref(nw), ptr(code), ptr(words))
+        self.h_code, self.h_words = code[: nb.value], words[: nw.value]
+        self.frac = 64.0 * np.count_nonzero(self.h_code >= 0) / max(N, 1)
+        dev = torch.device(device)
+        self.blk_code = torch.from_numpy(code).to(dev)
+        self.words = torch.from_numpy(words).to(dev)
+
+
+def p1_stencil_plan(geom: Geometry, amap: AssemblyMap):
+    """The P1 assembly's stencil-block plan, built on first use and kept with
+    the map, per device."""
+    key = "_p1_stencil_" + str(geom.device)
+    p = getattr(amap, key, None)
+    if p is None or p[0] is not geom:
+        p = (geom, P1StencilPlan(amap, geom.device))
+        setattr(amap, key, p)
+    return p[1]
+
+
+STENCIL_MIN_FRAC = 0.5  # the stencil-block kernel when at least this share of rows is in uniform blocks
+
+
 def assemble(op, geom: Geometry, amap: AssemblyMap, state, res, jac: CsrMatrix = None,
-             stream=None, patches=True):
+             stream=None, patches=True, stencil=None):
     """Assemble the residual into `res` and, when `jac` is given, the CSR
     Jacobian values into `jac.vals`; both outputs are replaced (femx
     assembly::assemble, CudaAssembly.hpp:319-364).  Returns `res`.
 
-    With `patches` (the default) the mesh's element patches assemble the
-    rows of the nodes they own, each element evaluated once per patch
-    (fx_p1_assemble_patch; the plan is built on the first call); without, one
-    lane per row evaluates the row's elements (fx_p1_assemble).  The results
-    are bitwise equal."""
+    When at least half of the rows lie in 64-row blocks that share one
+    incidence stencil (lexicographically numbered structured meshes), those
+    blocks assemble from coalesced loads of their rows' vertex data with the
+    rows summed in registers (fx_p1_assemble_stencil; `stencil` True/False
+    forces it on or off).  Otherwise, with `patches` (the default), the
+    mesh's element patches assemble the rows of the nodes they own, each
+    element evaluated once per patch (fx_p1_assemble_patch); without, one
+    lane per row evaluates the row's elements (fx_p1_assemble).  The plans
+    are built on the first call; the results are bitwise equal."""
     if not isinstance(op, PoissonP1):
         raise RuntimeError(f"assemble: unsupported element operator {type(op).__name__}")
     _check_p1(geom, amap)
@@ -353,6 +395,17 @@ def assemble(op, geom: Geometry, amap: AssemblyMap, state, res, jac: CsrMatrix =
         _check_dev_vec(op.f, N, torch.float64, "PoissonP1 load")
     dev = geom.device
     rp, _ = amap.pattern.device_arrays(dev)
+    if stencil is not False and E > 0:
+        sp = p1_stencil_plan(geom, amap)
+        if stencil or sp.frac >= STENCIL_MIN_FRAC:
+            _lib.call("fx_p1_assemble_stencil", N, E, amap.pattern.nnz, amap.max_row_len,
+                      ptr(geom.x), ptr(geom.y), ptr(geom.conn), ptr(coef),
+                      int(op.log_kappa is not None), ptr(op.f), ptr(state),
+                      ptr(amap.device("inc_off", dev)), ptr(amap.device("inc", dev)), ptr(rp),
+                      ptr(sp.blk_code), ptr(sp.words),
+                      ptr(jac.vals) if jac is not None else None, ptr(res),
+                      stream_handle(stream))
+            return res
     if patches and E > 0:
         ap = p1_asm_plan(geom, amap)
         _lib.call("fx_p1_assemble_patch", N, E, amap.pattern.nnz, ptr(geom.x), ptr(geom.y),

@@ -27,6 +27,17 @@ __device__ __forceinline__ void wave_sync()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Lane i takes lane i+1's v and lane 63 takes fill (DPP wave_shl:1 on both
+// halves; every lane of the wave must be active).
+__device__ __forceinline__ double wave_shl1(double v, double fill)
+{
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x130, 0xf,
+                                             0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x130, 0xf,
+                                             0xf, false);
+  return __hiloint2double(hi, lo);
+}
+
 // LDS floating-point atomic add whose result is not used (ds_add_f64): the
 // LDS unit performs old + v, rounded as a VALU add, and the issuing wave does
 // not wait for it.  One wave's LDS operations execute in issue order, so a

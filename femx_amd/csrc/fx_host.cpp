@@ -1276,6 +1276,97 @@ int fx_csr_stencil_plan(Index rows, Index cols, const Index* row_ptr, const Inde
 }
 
 // ---------------------------------------------------------------------------
+// Stencil blocks of the P1 assembly.  A full block of 64 rows is uniform when
+// its rows have the same column offsets (S <= max_len entries), the same
+// number T <= max_len of incidences with the same local row and column
+// positions, incidence by incidence, and elements e_k(r) = A r + c_k for one
+// A and per-incidence constants c_k, and when its bands (runs of
+// consecutive offsets; a band of w offsets spans 63 + w values of a vector)
+// fit max_band values in all.  Its record, shared by equal blocks:
+// [S, T, A, offset_0 .. offset_(S-1), sidx_0 .. sidx_(S-1), (c_k, meta_k)
+// for k < T], meta as in the row incidence (li | p0 << 8 | p1 << 16 | p2 <<
+// 24) and sidx_q the index of slot q's lane-0 value in the block's band
+// buffer (bands packed in order); blk_code is the record's first word, or -1.
+
+int fx_p1_stencil_plan(Index num_nodes, const Index* row_ptr, const Index* col_ind,
+                       const Index* inc_off, const Index* inc, Index max_len, Index max_band,
+                       Index* num_blocks, Index* num_words, Index* blk_code, Index* words)
+{
+  return fx::guard([&] {
+    fx::require(num_nodes >= 0 && row_ptr && inc_off && num_blocks && num_words && max_len >= 1
+                    && max_len <= 255 && max_band >= 64,
+                FX_ERR_ARG, "fx_p1_stencil_plan: invalid arguments");
+    fx::require((row_ptr[num_nodes] == 0 || col_ind) && (inc_off[num_nodes] == 0 || inc),
+                FX_ERR_ARG, "fx_p1_stencil_plan: null array");
+    const Index                         nb = (num_nodes + 63) / 64;
+    std::map<std::vector<Index>, Index> seen;
+    std::vector<Index>                  flat, rec;
+    for (Index b = 0; b < nb; ++b)
+    {
+      const Index r0 = 64 * b;
+      Index       code = -1;
+      const Index S = row_ptr[r0 + 1] - row_ptr[r0], T = inc_off[r0 + 1] - inc_off[r0];
+      bool same = r0 + 64 <= num_nodes && S >= 1 && S <= max_len && T >= 1 && T <= max_len
+                  && inc_off[r0 + 2] - inc_off[r0 + 1] == T;
+      std::int64_t A = 0;
+      if (same)
+      {
+        const auto elem = [&](Index r, Index k) { return std::int64_t(inc[2 * (inc_off[r] + k)]); };
+        const auto meta = [&](Index r, Index k) { return inc[2 * (inc_off[r] + k) + 1]; };
+        A = elem(r0 + 1, 0) - elem(r0, 0);
+        rec.assign({S, T, 0});
+        for (Index k = 0; k < S; ++k) rec.push_back(col_ind[row_ptr[r0] + k] - r0);
+        // band buffer: a band of w consecutive offsets takes 63 + w values
+        Index used = 0;
+        for (Index q = 0; q < S;)
+        {
+          Index w = 1;
+          while (q + w < S && rec[3 + q + w] == rec[3 + q] + w) ++w;
+          for (Index d = 0; d < w; ++d) rec.push_back(used + d);
+          used += 63 + w;
+          q += w;
+        }
+        same = used <= max_band;
+        for (Index k = 0; k < T; ++k)
+        {
+          const std::int64_t c = elem(r0, k) - A * r0;
+          same = same && c >= std::numeric_limits<Index>::min()
+                 && c <= std::numeric_limits<Index>::max();
+          rec.push_back(static_cast<Index>(c));
+          rec.push_back(meta(r0, k));
+        }
+        same = same && A >= std::numeric_limits<Index>::min() && A <= std::numeric_limits<Index>::max();
+        for (Index r = r0 + 1; r < r0 + 64 && same; ++r)
+        {
+          same = row_ptr[r + 1] - row_ptr[r] == S && inc_off[r + 1] - inc_off[r] == T;
+          for (Index k = 0; k < S && same; ++k) same = col_ind[row_ptr[r] + k] - r == rec[3 + k];
+          for (Index k = 0; k < T && same; ++k)
+            same = elem(r, k) - A * r == rec[3 + 2 * S + 2 * k]
+                   && meta(r, k) == rec[4 + 2 * S + 2 * k];
+        }
+      }
+      if (same)
+      {
+        rec[2]  = static_cast<Index>(A);
+        auto it = seen.find(rec);
+        if (it == seen.end())
+        {
+          fx::require(flat.size() + rec.size() <= std::size_t(std::numeric_limits<Index>::max()),
+                      FX_ERR_UNSUPPORTED, "fx_p1_stencil_plan: too many distinct stencils");
+          it = seen.emplace(rec, static_cast<Index>(flat.size())).first;
+          flat.insert(flat.end(), rec.begin(), rec.end());
+        }
+        code = it->second;
+      }
+      if (blk_code) blk_code[b] = code;
+    }
+    if (words) std::copy(flat.begin(), flat.end(), words);
+    *num_blocks = nb;
+    *num_words  = static_cast<Index>(flat.size());
+  });
+}
+
+// ---------------------------------------------------------------------------
 // Boundary map.  Validation order and messages follow femx's makeBoundaryMap
 // (femx/assembly/BoundaryMap.cpp:59-139).
 

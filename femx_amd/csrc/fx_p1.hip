@@ -49,9 +49,49 @@ __global__ __launch_bounds__(kP1Block) void k_p1_assemble_list(P1AsmArgs a, cons
   p1_assemble_list_block(fx::xcd_block(gridDim.x), a, rows, nrows, smem);
 }
 
+__global__ __launch_bounds__(64 * kP1StWaves) void k_p1_asm_stencil(P1AsmArgs a,
+                                                                      const int* blk_code,
+                                                                      const int* words)
+{
+  __shared__ double bb[kP1StWaves][4 * kP1StBand];
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  p1_assemble_stencil_block(fx::xcd_block(gridDim.x) * kP1StWaves + wv, a, blk_code, words, bb[wv]);
+}
+
 } // namespace
 
 extern "C" {
+
+int fx_p1_assemble_stencil(fx_index num_nodes, fx_index num_elems, fx_index nnz,
+                           fx_index max_row_len, const double* x, const double* y,
+                           const fx_index* conn, const double* kappa, int kappa_is_log,
+                           const double* f, const double* u, const fx_index* inc_off,
+                           const fx_index* inc, const fx_index* row_ptr, const fx_index* blk_code,
+                           const fx_index* words, double* vals, double* res, void* stream)
+{
+  return fx::guard([&] {
+    fx::require(num_nodes >= 0 && num_elems >= 0 && nnz >= 0, FX_ERR_ARG,
+                "fx_p1_assemble_stencil: negative size");
+    if (num_nodes == 0) return;
+    fx::require(x && y && inc_off && row_ptr && res && blk_code && words
+                    && (num_elems == 0 || (conn && inc)),
+                FX_ERR_ARG, "fx_p1_assemble_stencil: null array");
+    fx::require(fx::fits32(num_nodes, 8) && fx::fits32(num_elems, 8), FX_ERR_OVERFLOW,
+                "fx_p1_assemble_stencil: node or element arrays beyond 4 GiB");
+    fx::require(max_row_len >= 0 && max_row_len <= 255, FX_ERR_UNSUPPORTED,
+                "fx_p1_assemble_stencil: CSR rows longer than 255 entries");
+    const std::size_t N = num_nodes;
+    fx::require(!fx::overlaps(res, N * 8, vals, std::size_t(nnz) * 8)
+                    && !(u && (fx::overlaps(u, N * 8, res, N * 8)
+                               || fx::overlaps(u, N * 8, vals, std::size_t(nnz) * 8))),
+                FX_ERR_ALIAS, "Assembly state, residual, and matrix values must not alias");
+    const P1AsmArgs a{num_nodes, x,  y,       conn, kappa, kappa_is_log, f, u, inc_off,
+                      reinterpret_cast<const int2*>(inc), row_ptr, vals, res, 0};
+    hipLaunchKernelGGL(k_p1_asm_stencil, dim3(fx::blocks_for(num_nodes, 64 * kP1StWaves)),
+                       dim3(64 * kP1StWaves), 0, fx::as_stream(stream), a, blk_code, words);
+    fx::check_launch("fx_p1_assemble_stencil");
+  });
+}
 
 int fx_p1_assemble(fx_index num_nodes, fx_index num_elems, fx_index nnz, fx_index max_row_len,
                    const double* x, const double* y, const fx_index* conn, const double* kappa,

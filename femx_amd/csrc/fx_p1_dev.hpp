@@ -287,6 +287,102 @@ __device__ __forceinline__ void p1_assemble_block(int blk, const P1AsmArgs& a, d
   __syncthreads();
 }
 
+// ---------------------------------------------------------------------------
+// P1 assembly over stencil blocks (fx_p1_stencil_plan): one wave per block
+// of 64 rows.  In a uniform block every row has the same column offsets and
+// the same incidences -- element A r + c_k, local row li_k, column positions
+// p0..p2 -- so the lane of row r finds vertex m of its k-th element at
+// column slot p_m.  The vertex data (x, y, u, f) of the block's columns are
+// loaded once into the wave's LDS band buffer, each band of consecutive
+// offsets as one coalesced run of 63 + w values per array (slot q of lane l
+// is then band value sidx_q + l), and each row accumulates its S values in
+// registers.  The arithmetic, and the order of every sum, are
+// p1_assemble_row's, so the results are bitwise equal; the block's 64 S
+// values leave through the same LDS with coalesced stores.  Other blocks
+// take p1_assemble_row, adding in global memory.
+constexpr int kP1StMax   = FX_P1_STENCIL_MAX;
+constexpr int kP1StBand  = FX_P1_STENCIL_BAND;
+constexpr int kP1StWaves = 4;
+static_assert(4 * kP1StBand >= 64 * kP1StMax, "the band buffer holds the write-back");
+
+__device__ __forceinline__ void p1_assemble_stencil_block(int b, const P1AsmArgs& a,
+                                                          const int* __restrict__ blk_code,
+                                                          const int* __restrict__ words,
+                                                          double* bb)
+{
+  const int lane = threadIdx.x & 63;
+  const int r0   = 64 * b;
+  if (r0 >= a.num_rows) return;
+  const int  r    = r0 + lane;
+  const int  code = blk_code[b];
+  const int* rec  = words + (code >= 0 ? code : 0);
+  if (code < 0 || rec[0] > kP1StMax || rec[1] > kP1StMax)
+  {
+    if (r < a.num_rows)
+      p1_assemble_row(a, r, a.vals ? a.vals + a.row_ptr[r] : nullptr, false, nullptr, false, 0);
+    return;
+  }
+  const int      S = rec[0], T = rec[1];
+  const unsigned A    = static_cast<unsigned>(rec[2]);
+  const int*     off  = rec + 3;
+  const int*     sidx = rec + 3 + S;
+  const int*     inc  = rec + 3 + 2 * S;
+  double*        bx = bb, *by = bb + kP1StBand, *bu = bb + 2 * kP1StBand, *bf = bb + 3 * kP1StBand;
+  for (int q = 0; q < S;)
+  {
+    const int o = off[q];
+    int       w = 1;
+    while (q + w < S && off[q + w] == o + w) ++w;
+    const int base = sidx[q], len = 63 + w;
+    // value i of the band: row r0 + i's column at offset o (i < 64), or row
+    // r0 + 63's at offset o + i - 63
+    for (int i = lane; i < len; i += 64)
+    {
+      const int    c  = r0 + o + i;
+      const double xv = gat32(a.x, c), yv = gat32(a.y, c);
+      const double uv = a.u ? gat32(a.u, c) : 0.0, fv = a.f ? gat32(a.f, c) : 0.0;
+      bx[base + i] = xv, by[base + i] = yv, bu[base + i] = uv, bf[base + i] = fv;
+    }
+    q += w;
+  }
+  wave_sync();
+  double acc[kP1StMax];
+#pragma unroll
+  for (int q = 0; q < kP1StMax; ++q) acc[q] = 0.0;
+  double rsum = 0.0;
+  for (int k = 0; k < T; ++k)
+  {
+    const int      e    = static_cast<int>(A * static_cast<unsigned>(r) + static_cast<unsigned>(inc[2 * k]));
+    const unsigned meta = static_cast<unsigned>(inc[2 * k + 1]);
+    const int      li = meta & 0xff, p0 = (meta >> 8) & 0xff, p1 = (meta >> 16) & 0xff, p2 = meta >> 24;
+    const int      s0 = sidx[p0] + lane, s1 = sidx[p1] + lane, s2 = sidx[p2] + lane;
+    const double   kv = a.kappa ? gat32(a.kappa, e) : 1.0;
+    const P1Geom   g  = p1_geom(bx[s0], by[s0], bx[s1], by[s1], bx[s2], by[s2]);
+    const double   ke = (a.kappa && a.kappa_is_log) ? exp(kv) : kv;
+    const double   k0 = p1_k(g, li, 0, ke), k1 = p1_k(g, li, 1, ke), k2 = p1_k(g, li, 2, ke);
+    double         re = 0.0;
+    if (a.u) re = k0 * bu[s0] + k1 * bu[s1] + k2 * bu[s2];
+    if (a.f) re = re - p1_load(g, li, bf[s0], bf[s1], bf[s2]);
+    else re = re - 0.0;
+    rsum    = rsum + re;
+    acc[p0] = acc[p0] + k0;
+    acc[p1] = acc[p1] + k1;
+    acc[p2] = acc[p2] + k2;
+  }
+  __builtin_nontemporal_store(rsum, a.res + r);
+  if (a.vals)
+  {
+    const int base = a.row_ptr[r0], n = 64 * S;
+    wave_sync(); // every lane has read the band buffer: it takes the row values now
+#pragma unroll
+    for (int q = 0; q < kP1StMax; ++q)
+      if (q < S) bb[lane * S + q] = acc[q];
+    wave_sync();
+    for (int i = lane; i < n; i += 64) __builtin_nontemporal_store(bb[i], a.vals + base + i);
+  }
+  wave_sync(); // the buffer is free for the next block
+}
+
 // Rows from a list (the rows an element-patch pass leaves: nodes on patch
 // seams), kP1Block per block, one lane per row, each lane with its own LDS
 // row of kP1ListRow values (rows longer than that add in global memory).

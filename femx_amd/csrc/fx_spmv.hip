@@ -175,15 +175,6 @@ __global__ __launch_bounds__(kRowBlock) void k_spmv_multi(
 constexpr int kStWaves = 4;
 constexpr int kStMaxLen = FX_STENCIL_MAX_LEN; // longest stencil read as one (stage: 64 x 8 values per wave)
 
-__device__ __forceinline__ double wave_shl1(double v, double fill)
-{
-  const int lo = __builtin_amdgcn_update_dpp(__double2loint(fill), __double2loint(v), 0x130, 0xf,
-                                             0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(fill), __double2hiint(v), 0x130, 0xf,
-                                             0xf, false);
-  return __hiloint2double(hi, lo);
-}
-
 template <int NV>
 __global__ __launch_bounds__(64 * kStWaves) void k_spmv_multi_st(
     int rows, const int* __restrict__ row_ptr, const int* __restrict__ col,
@@ -236,7 +227,7 @@ __global__ __launch_bounds__(64 * kStWaves) void k_spmv_multi_st(
 #pragma unroll
         for (int j = 0; j < NV; ++j)
         {
-          cur[j] = wave_shl1(cur[j], X[j * ldx + t]);
+          cur[j] = fxk::wave_shl1(cur[j], X[j * ldx + t]);
           acc[j] = acc[j] + mk * cur[j];
         }
       }

@@ -435,6 +435,40 @@ int fx_p1_assemble_patch(fx_index num_nodes, fx_index num_elems, fx_index nnz,
                          const fx_index* d_row_ptr, const fx_p1_asm_plan* h_plan, double* d_vals,
                          double* d_res, void* stream);
 
+/* Stencil blocks of the P1 assembly (host; no counterpart in femx): rows in
+ * blocks of 64, num_blocks = ceil(N/64).  A full block whose rows have the
+ * same column offsets (S entries) and the same incidences -- element
+ * A*row + c_k, local row and column positions as in fx_map_export_rowinc3,
+ * T incidences, S and T at most max_len -- and whose bands of consecutive
+ * offsets fit max_band values (63 + w for a band of w offsets) gets
+ * blk_code = the index in words of its record [S, T, A, offsets[S],
+ * sidx[S], (c_k, meta_k) for k < T], sidx being each offset's place in the
+ * band buffer (equal blocks share a record); other blocks get -1.  Call
+ * with null blk_code and words for num_blocks and num_words.
+ * (Lexicographically numbered structured meshes: all but the blocks holding
+ * boundary nodes.) */
+#define FX_P1_STENCIL_MAX 8    /* the largest S and T fx_p1_assemble_stencil reads as one */
+#define FX_P1_STENCIL_BAND 208 /* its band buffer, in values per vertex array */
+int fx_p1_stencil_plan(fx_index num_nodes, const fx_index* h_row_ptr, const fx_index* h_col_ind,
+                       const fx_index* h_inc_off, const fx_index* h_inc, fx_index max_len,
+                       fx_index max_band, fx_index* num_blocks, fx_index* num_words,
+                       fx_index* h_blk_code, fx_index* h_words);
+/* fx_p1_assemble over the stencil blocks of fx_p1_stencil_plan (blk_code,
+ * words on the device): a uniform block's lanes load the vertex data of
+ * their rows' columns once (each band of consecutive offsets one coalesced
+ * load per array, its other offsets by lane shifts) and sum their rows in
+ * registers; other blocks (and records with S or T above FX_P1_STENCIL_MAX)
+ * take the row-gather path.  The plan must have been built with max_len <=
+ * FX_P1_STENCIL_MAX and max_band <= FX_P1_STENCIL_BAND.  Arguments
+ * otherwise as fx_p1_assemble; the results are bitwise equal to it. */
+int fx_p1_assemble_stencil(fx_index num_nodes, fx_index num_elems, fx_index nnz,
+                           fx_index max_row_len, const double* d_x, const double* d_y,
+                           const fx_index* d_conn, const double* d_kappa, int kappa_is_log,
+                           const double* d_f, const double* d_u, const fx_index* d_inc_off,
+                           const fx_index* d_inc, const fx_index* d_row_ptr,
+                           const fx_index* d_blk_code, const fx_index* d_words, double* d_vals,
+                           double* d_res, void* stream);
+
 /* ======================================================================== */
 /* Taylor-Hood P2/P1 steady incompressible Navier-Stokes (device).  Not in
  * femx (its NS operator is GLS P1/P1, femx/model/ns/Components.hpp:400-577);

@@ -1,6 +1,7 @@
 """The P1 assembly over owner-computes element patches
-(fx_p1_assemble_patch) through the C-ABI against the row-per-lane
-fx_p1_assemble: CSR values and residuals bitwise equal (both sum every entry
+(fx_p1_assemble_patch) and over 64-row stencil blocks
+(fx_p1_assemble_stencil) through the C-ABI against the row-per-lane
+fx_p1_assemble: CSR values and residuals bitwise equal (all sum every entry
 in ascending element order) on structured, scrambled, unused-node and hub
 meshes, small patches and small tiles (rows left to the row kernel), log and
 plain kappa, without load or state, residual only, repeated launches, and
@@ -27,14 +28,14 @@ def host(t):
     return t.detach().cpu().numpy()
 
 
-def both(g, amap, op, u, jac=True):
+def both(g, amap, op, u, jac=True, paths=((False, False), (True, False))):
     out = []
-    for patches in (False, True):
+    for patches, stencil in paths:
         res = torch.full((g.num_nodes,), float("nan"), dtype=torch.float64, device=DEV)
         J = CsrMatrix(amap.pattern, DEV) if jac else None
         if J is not None:
             J.vals.fill_(float("nan"))
-        A.assemble(op, g, amap, u, res, J, patches=patches)
+        A.assemble(op, g, amap, u, res, J, patches=patches, stencil=stencil)
         out.append((host(J.vals) if jac else None, host(res)))
     return out
 
@@ -87,6 +88,42 @@ def test_patch_assembly_scrambled_unused_node_and_hub():
     check(xh, yh, ch, 5)
 
 
+STENCIL = ((False, False), (False, True))  # the row kernel, then the stencil-block kernel
+
+
+def check_stencil(n, seed):
+    """fx_p1_assemble_stencil against fx_p1_assemble on the structured n x n
+    mesh: uniform 7- and 5-point blocks beside row-gather blocks, log and
+    plain kappa, no load or state, residual only."""
+    print("stencil case", n, seed, flush=True)
+    g = A.structured_p1(n, DEV)
+    amap = A.make_assembly_map(g.num_nodes, g.num_nodes, g.h_conn)
+    sp = A.p1_stencil_plan(g, amap)
+    rng = np.random.default_rng(seed)
+    m = dev(0.5 * rng.standard_normal(g.num_elems))
+    kap = dev(np.exp(0.5 * rng.standard_normal(g.num_elems)))
+    f = dev(rng.standard_normal(g.num_nodes))
+    u = dev(rng.standard_normal(g.num_nodes))
+    for op, uu, jac in ((A.PoissonP1(f=f, log_kappa=m), u, True),
+                        (A.PoissonP1(f=f, kappa=kap), u, True),
+                        (A.PoissonP1(kappa=kap), None, True),
+                        (A.PoissonP1(f=f, log_kappa=m), u, False)):
+        (rv, rr), (sv, sr) = both(g, amap, op, uu, jac, STENCIL)
+        assert np.array_equal(rr, sr)
+        if jac:
+            assert np.array_equal(rv, sv)
+    return sp
+
+
+def test_stencil_assembly_matches_row_kernel():
+    # 300 and 129: interior and bottom/top-row stencil blocks, partial last
+    # block; 40: no uniform block (every block holds a boundary node)
+    for n, seed in ((300, 1), (129, 2)):
+        sp = check_stencil(n, seed)
+        assert 0.3 < sp.frac < 1.0
+    assert check_stencil(40, 3).frac == 0.0
+
+
 def test_patch_assembly_full_size_matches_row_kernel_and_oracle():
     """2048^2 (8.4 M elements): bitwise equal to fx_p1_assemble and to the
     oracle's element loop."""
@@ -106,3 +143,12 @@ def test_patch_assembly_full_size_matches_row_kernel_and_oracle():
     ov, orr = O.p1_assemble(g.h_x, g.h_y, g.h_conn.reshape(-1), hk, hf, hu, amap.jac()[1],
                             amap.pattern.nnz)
     assert np.array_equal(pv, ov) and np.array_equal(pr, orr)
+    # the stencil-block kernel (97% of the rows in uniform blocks here)
+    assert A.p1_stencil_plan(g, amap).frac > 0.9
+    (_, _), (sv, sr) = both(g, amap, A.PoissonP1(f=dev(hf), kappa=dev(hk)), dev(hu),
+                            paths=STENCIL)
+    assert np.array_equal(sv, ov) and np.array_equal(sr, orr)
+    (_, _), (sv, sr) = both(g, amap, A.PoissonP1(f=dev(hf), log_kappa=dev(hm)), dev(hu),
+                            paths=STENCIL)
+    (rv, rr), (_, _) = both(g, amap, A.PoissonP1(f=dev(hf), log_kappa=dev(hm)), dev(hu))
+    assert np.array_equal(sv, rv) and np.array_equal(sr, rr)

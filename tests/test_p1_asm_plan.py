@@ -168,3 +168,58 @@ def test_p1_asm_plan_hub_row_over_sixteen_entries():
     c3 = np.array([[0, 1 + i, 1 + (i + 1) % m] for i in range(m)], np.int32)
     ap = _check(x, y, c3, 256, seed=3)
     assert np.array_equal(ap.h["rows_left"][: ap.num_left], [0])
+
+
+@pytest.mark.parametrize("n,lo", [(70, 0.05), (129, 0.3)])
+def test_stencil_plan_records_reproduce_incidences(n, lo):
+    """fx_p1_stencil_plan: a 64-row block gets a record exactly when its rows
+    share column offsets and incidences (element A r + c_k, same local row and
+    positions) within the bounds; the record's offsets, band-buffer indices
+    (bands of consecutive offsets packed in order, 63 + w values each) and
+    (c_k, meta_k) rebuild every row's columns and incidences."""
+    from femx_amd.assembly import P1StencilPlan
+    g = structured_p1(n, "cpu")
+    amap = make_assembly_map(g.num_nodes, g.num_nodes, g.h_conn)
+    sp = P1StencilPlan(amap, "cpu")
+    rp, ci = amap.pattern.row_ptr, amap.pattern.col_ind
+    inc_off, inc = amap.rowinc3()
+    inc = inc.reshape(-1, 2)
+    N = g.num_nodes
+    code, w = sp.h_code, sp.h_words
+    assert code.size == -(-N // 64)
+    for b in range(code.size):
+        r0 = 64 * b
+        rows = range(r0, min(r0 + 64, N))
+        offs = {tuple(ci[rp[r]:rp[r + 1]] - r) for r in rows}
+        incs = [inc[inc_off[r]:inc_off[r + 1]] for r in rows]
+        A_ = int(incs[1][0, 0] - incs[0][0, 0]) if len(rows) > 1 and len(incs[1]) else 0
+        aff = {tuple((x[:, 0] - A_ * r).tolist() + x[:, 1].tolist()) for r, x in zip(rows, incs)}
+        S = len(next(iter(offs)))
+        T = len(incs[0])
+        if len(offs) == 1:
+            o = np.array(next(iter(offs)))
+            nb_ = 1 + np.count_nonzero(np.diff(o) != 1)
+            band = 63 * nb_ + S
+        uniform = (len(rows) == 64 and len(offs) == 1 and len(aff) == 1 and 1 <= S <= 8
+                   and 1 <= T <= 8 and band <= 208)
+        assert (code[b] >= 0) == uniform, b
+        if code[b] < 0:
+            continue
+        rec = w[code[b]:]
+        assert rec[0] == S and rec[1] == T and rec[2] == A_
+        off = rec[3:3 + S]
+        assert tuple(off) == next(iter(offs))
+        sidx = rec[3 + S:3 + 2 * S]
+        used, q = 0, 0
+        while q < S:
+            wd = 1
+            while q + wd < S and off[q + wd] == off[q] + wd:
+                wd += 1
+            assert list(sidx[q:q + wd]) == list(range(used, used + wd))
+            used += 63 + wd
+            q += wd
+        pairs = rec[3 + 2 * S:3 + 2 * S + 2 * T].reshape(-1, 2)
+        for r, x in zip(rows, incs):
+            assert np.array_equal(A_ * r + pairs[:, 0], x[:, 0])
+            assert np.array_equal(pairs[:, 1], x[:, 1])
+    assert lo < sp.frac < 1.0
