@@ -415,6 +415,9 @@ struct DeviceAssemblyMap
   DeviceIndexVector inc_off, inc, inc_elem, inc_meta, blk_row, blk_flags, blk_lanes;
   DeviceIndexVector elem_inc; // 15x15 maps: (element, local row) -> row incidence (the VJP)
   Index             num_blocks = 0, max_block_nnz = 0;
+  // 3x3 maps: the 64-row stencil blocks of the P1 assembly (fx_p1_stencil_plan)
+  DeviceIndexVector p1_blk_code, p1_words;
+  double            p1_stencil_frac = 0.0; // share of rows in uniform blocks
   // Taylor-Hood VJP workspace (15 doubles per element), sized on first use;
   // VJPs of one map must not overlap
   mutable DeviceVector vjp_work;
@@ -433,6 +436,23 @@ inline void copy(const HostAssemblyMap& src, DeviceAssemblyMap& dst, HipContext&
     check(fx_map_export_rowinc3(src.handle(), off.data(), inc.data()));
     dst.inc_off.assign(off, ctx);
     dst.inc.assign(inc, ctx);
+    const HostCsrPattern& hp = src.pattern();
+    Index                 nb = 0, nw = 0;
+    check(fx_p1_stencil_plan(src.numRes(), hp.row_ptr.data(), hp.col_ind.data(), off.data(),
+                             inc.data(), FX_P1_STENCIL_MAX, FX_P1_STENCIL_BAND, &nb, &nw, nullptr,
+                             nullptr));
+    std::vector<Index> code(nb), words(nw);
+    check(fx_p1_stencil_plan(src.numRes(), hp.row_ptr.data(), hp.col_ind.data(), off.data(),
+                             inc.data(), FX_P1_STENCIL_MAX, FX_P1_STENCIL_BAND, &nb, &nw,
+                             code.data(), words.data()));
+    std::size_t uniform = 0;
+    for (Index c : code) uniform += c >= 0;
+    dst.p1_stencil_frac = src.numRes() > 0 ? 64.0 * uniform / src.numRes() : 0.0;
+    if (uniform > 0)
+    {
+      dst.p1_blk_code.assign(code, ctx);
+      dst.p1_words.assign(words, ctx);
+    }
   }
   else if (src.maxRes() == 15 && src.maxState() == 15)
   {
@@ -599,13 +619,26 @@ inline void checkAssembly(const DeviceAssemblyMap& map, const DeviceVector* stat
 }
 
 /** assemble(op, geom, map, state, res, jac, ctx): residual and CSR Jacobian
- * values, both replaced (CudaAssembly.hpp:319-364).  jac may be null. */
+ * values, both replaced (CudaAssembly.hpp:319-364).  jac may be null.  When
+ * at least half of the map's rows lie in 64-row stencil blocks (built by
+ * copy(map)) the stencil-block kernel runs, else the row-per-lane one; the
+ * results are bitwise equal. */
 inline void assemble(const PoissonP1& op, const DeviceGeometry& geom, const DeviceAssemblyMap& map,
                      const DeviceVector* state, DeviceVector& res, DeviceCsrMatrix* jac,
                      HipContext& ctx)
 {
   checkAssembly(map, state, res, jac);
   require(geom.num_elems == map.info.num_elems, "Geometry and AssemblyMap have different element counts");
+  if (map.p1_stencil_frac >= 0.5)
+  {
+    check(fx_p1_assemble_stencil(
+        geom.num_nodes, geom.num_elems, map.info.nnz, map.info.max_row_len, geom.x.data(),
+        geom.y.data(), geom.conn.data(), op.kappa, op.log_kappa ? 1 : 0, op.f,
+        state ? state->data() : nullptr, map.inc_off.data(), map.inc.data(),
+        map.pattern->rowPtrData(), map.p1_blk_code.data(), map.p1_words.data(),
+        jac ? jac->valsData() : nullptr, res.data(), ctx.raw()));
+    return;
+  }
   check(fx_p1_assemble(geom.num_nodes, geom.num_elems, map.info.nnz, map.info.max_row_len,
                        geom.x.data(), geom.y.data(), geom.conn.data(), op.kappa, op.log_kappa ? 1 : 0,
                        op.f, state ? state->data() : nullptr, map.inc_off.data(), map.inc.data(),
