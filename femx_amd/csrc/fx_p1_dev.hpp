@@ -594,6 +594,66 @@ struct P1VjpArgs
 // One pass: each lane (node j) sums column j of its elements' matrices against
 // the boundary-masked adjoint; the lane holding an element's local row 0
 // also writes that element's control sensitivity g_e = lam~_e^T K^e u_e.
+// The VJP of row (node) j: its incidences from the block's LDS stage or,
+// unstaged, from memory.
+__device__ __forceinline__ void p1_vjp_row(const P1VjpArgs& a, int j, const long long* sinc,
+                                           bool staged, int ib0)
+{
+  double    acc = 0.0;
+  const int ta = a.inc_off[j], t1 = a.inc_off[j + 1], tl = t1 - 1;
+  if (ta < t1)
+  {
+    // node j is vertex li of each of its elements: its values are loaded
+    // once, the element's two other vertices are gathered per incidence
+    // (their indices one incidence ahead)
+    // both gathers issued together (n < num_rows for both arrays), then a select
+    auto masked = [&](int n) {
+      const double l = gat32(a.lam, n);
+      const int    m = a.bc_mask ? gat32(a.bc_mask, n) : 0;
+      return m ? 0.0 : l;
+    };
+    const double xo = a.x[j], yo = a.y[j], lo = masked(j), uo = a.u[j];
+    int2         ic = inc_at(sinc, a.inc, staged, ta, ib0);
+    int2         cc = p1_others(a.conn, ic.x, ic.y & 0xff);
+    for (int t = ta; t < t1; ++t)
+    {
+      const int2   in = inc_at(sinc, a.inc, staged, min(t + 1, tl), ib0);
+      const int    e  = ic.x;
+      const int    li = static_cast<unsigned>(ic.y) & 0xff;
+      const double xa = gat32(a.x, cc.x), xb = gat32(a.x, cc.y);
+      const double ya = gat32(a.y, cc.x), yb = gat32(a.y, cc.y);
+      const double la = masked(cc.x), lb = masked(cc.y);
+      const double me = gat32(a.m, e);
+      const int2   nx = cc;
+      cc              = p1_others(a.conn, in.x, in.y & 0xff);
+      const P1Geom g  = p1_geom(p1_place(li, 0, xo, xa, xb), p1_place(li, 0, yo, ya, yb),
+                                p1_place(li, 1, xo, xa, xb), p1_place(li, 1, yo, ya, yb),
+                                p1_place(li, 2, xo, xa, xb), p1_place(li, 2, yo, ya, yb));
+      const double ke = exp(me);
+      const double le[3] = {p1_place(li, 0, lo, la, lb), p1_place(li, 1, lo, la, lb),
+                            p1_place(li, 2, lo, la, lb)};
+      const double c0 = p1_k(g, 0, li, ke), c1 = p1_k(g, 1, li, ke), c2 = p1_k(g, 2, li, ke);
+      acc = acc + (c0 * le[0] + c1 * le[1] + c2 * le[2]);
+      if (li == 0)
+      {
+        // the element's vertex 0 is node j; vertices 1 and 2 are its others
+        const double u0 = uo, u1 = gat32(a.u, nx.x), u2 = gat32(a.u, nx.y);
+        double       ge = 0.0;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+        {
+          const double re = p1_k(g, r, 0, ke) * u0 + p1_k(g, r, 1, ke) * u1 + p1_k(g, r, 2, ke) * u2;
+          ge              = ge + le[r] * (re - 0.0);
+        }
+        a.gout[e] = ge;
+      }
+      ic = in;
+    }
+  }
+  if (a.bc_mask && a.bc_mask[j]) acc = acc + a.lam[j];
+  __builtin_nontemporal_store(acc, a.jt + j);
+}
+
 __device__ __forceinline__ void p1_vjp_block(int blk, const P1VjpArgs& a, long long* sinc)
 {
   const int  r0 = blk * kP1Block, r1 = min(r0 + kP1Block, a.num_rows);
@@ -603,62 +663,7 @@ __device__ __forceinline__ void p1_vjp_block(int blk, const P1VjpArgs& a, long l
     stage_nt<kP1Block>(sinc, reinterpret_cast<const long long*>(a.inc) + ib0, ib1 - ib0);
   __syncthreads();
   const int j = r0 + threadIdx.x;
-  if (j < a.num_rows)
-  {
-    double    acc = 0.0;
-    const int ta = a.inc_off[j], t1 = a.inc_off[j + 1], tl = t1 - 1;
-    if (ta < t1)
-    {
-      // node j is vertex li of each of its elements: its values are loaded
-      // once, the element's two other vertices are gathered per incidence
-      // (their indices one incidence ahead)
-      // both gathers issued together (n < num_rows for both arrays), then a select
-      auto masked = [&](int n) {
-        const double l = gat32(a.lam, n);
-        const int    m = a.bc_mask ? gat32(a.bc_mask, n) : 0;
-        return m ? 0.0 : l;
-      };
-      const double xo = a.x[j], yo = a.y[j], lo = masked(j), uo = a.u[j];
-      int2         ic = inc_at(sinc, a.inc, staged, ta, ib0);
-      int2         cc = p1_others(a.conn, ic.x, ic.y & 0xff);
-      for (int t = ta; t < t1; ++t)
-      {
-        const int2   in = inc_at(sinc, a.inc, staged, min(t + 1, tl), ib0);
-        const int    e  = ic.x;
-        const int    li = static_cast<unsigned>(ic.y) & 0xff;
-        const double xa = gat32(a.x, cc.x), xb = gat32(a.x, cc.y);
-        const double ya = gat32(a.y, cc.x), yb = gat32(a.y, cc.y);
-        const double la = masked(cc.x), lb = masked(cc.y);
-        const double me = gat32(a.m, e);
-        const int2   nx = cc;
-        cc              = p1_others(a.conn, in.x, in.y & 0xff);
-        const P1Geom g  = p1_geom(p1_place(li, 0, xo, xa, xb), p1_place(li, 0, yo, ya, yb),
-                                  p1_place(li, 1, xo, xa, xb), p1_place(li, 1, yo, ya, yb),
-                                  p1_place(li, 2, xo, xa, xb), p1_place(li, 2, yo, ya, yb));
-        const double ke = exp(me);
-        const double le[3] = {p1_place(li, 0, lo, la, lb), p1_place(li, 1, lo, la, lb),
-                              p1_place(li, 2, lo, la, lb)};
-        const double c0 = p1_k(g, 0, li, ke), c1 = p1_k(g, 1, li, ke), c2 = p1_k(g, 2, li, ke);
-        acc = acc + (c0 * le[0] + c1 * le[1] + c2 * le[2]);
-        if (li == 0)
-        {
-          // the element's vertex 0 is node j; vertices 1 and 2 are its others
-          const double u0 = uo, u1 = gat32(a.u, nx.x), u2 = gat32(a.u, nx.y);
-          double       ge = 0.0;
-#pragma unroll
-          for (int r = 0; r < 3; ++r)
-          {
-            const double re = p1_k(g, r, 0, ke) * u0 + p1_k(g, r, 1, ke) * u1 + p1_k(g, r, 2, ke) * u2;
-            ge              = ge + le[r] * (re - 0.0);
-          }
-          a.gout[e] = ge;
-        }
-        ic = in;
-      }
-    }
-    if (a.bc_mask && a.bc_mask[j]) acc = acc + a.lam[j];
-    __builtin_nontemporal_store(acc, a.jt + j);
-  }
+  if (j < a.num_rows) p1_vjp_row(a, j, sinc, staged, ib0);
   __syncthreads();
 }
 
