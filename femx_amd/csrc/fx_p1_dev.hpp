@@ -327,7 +327,17 @@ __device__ __forceinline__ void p1_assemble_stencil_block(int b, const P1AsmArgs
   const int*     off  = rec + 3;
   const int*     sidx = rec + 3 + S;
   const int*     inc  = rec + 3 + 2 * S;
-  double*        bx = bb, *by = bb + kP1StBand, *bu = bb + 2 * kP1StBand, *bf = bb + 3 * kP1StBand;
+  // the record's band buffer must fit this kernel's (a plan built with a
+  // larger max_band takes the row path)
+  int used = 0;
+  for (int q = 0; q < S; ++q)
+    if (q == S - 1 || off[q + 1] != off[q] + 1) used = sidx[q] + 64;
+  if (used > kP1StBand)
+  {
+    p1_assemble_row(a, r, a.vals ? a.vals + a.row_ptr[r] : nullptr, false, nullptr, false, 0);
+    return;
+  }
+  double* bx = bb, *by = bb + kP1StBand, *bu = bb + 2 * kP1StBand, *bf = bb + 3 * kP1StBand;
   for (int q = 0; q < S;)
   {
     const int o = off[q];

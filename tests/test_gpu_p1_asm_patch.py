@@ -10,8 +10,9 @@ import numpy as np
 import pytest
 import torch
 
+from femx_amd import _lib
 from femx_amd import assembly as A
-from femx_amd.linalg import CsrMatrix
+from femx_amd.linalg import CsrMatrix, ptr
 from femx_amd.patches import P1AsmPlan
 from oracle import oracle as O
 
@@ -122,6 +123,36 @@ def test_stencil_assembly_matches_row_kernel():
         sp = check_stencil(n, seed)
         assert 0.3 < sp.frac < 1.0
     assert check_stencil(40, 3).frac == 0.0
+
+
+def test_stencil_assembly_record_over_band_buffer_takes_row_path():
+    """A record whose band buffer would not fit the kernel's (here: every
+    record's band slots moved up by 100) sends its blocks down the row path:
+    the results are still bitwise equal to fx_p1_assemble."""
+    n = 129
+    g = A.structured_p1(n, DEV)
+    amap = A.make_assembly_map(g.num_nodes, g.num_nodes, g.h_conn)
+    sp = A.p1_stencil_plan(g, amap)
+    words = sp.h_words.copy()
+    for c in np.unique(sp.h_code[sp.h_code >= 0]):
+        S = words[c]
+        words[c + 3 + S:c + 3 + 2 * S] += 100
+    rng = np.random.default_rng(9)
+    kap = dev(np.exp(0.5 * rng.standard_normal(g.num_elems)))
+    f, u = dev(rng.standard_normal(g.num_nodes)), dev(rng.standard_normal(g.num_nodes))
+    rp, _ = amap.pattern.device_arrays(DEV)
+    out = []
+    for name, extra in (("fx_p1_assemble_stencil", (ptr(sp.blk_code), ptr(dev(words)))),
+                        ("fx_p1_assemble", ())):
+        J = CsrMatrix(amap.pattern, DEV)
+        J.vals.fill_(float("nan"))
+        res = torch.full((g.num_nodes,), float("nan"), dtype=torch.float64, device=DEV)
+        _lib.call(name, g.num_nodes, g.num_elems, amap.pattern.nnz, amap.max_row_len, ptr(g.x),
+                  ptr(g.y), ptr(g.conn), ptr(kap), 0, ptr(f), ptr(u),
+                  ptr(amap.device("inc_off", DEV)), ptr(amap.device("inc", DEV)), ptr(rp), *extra,
+                  ptr(J.vals), ptr(res), None)
+        out.append((host(J.vals), host(res)))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
 def test_patch_assembly_full_size_matches_row_kernel_and_oracle():
