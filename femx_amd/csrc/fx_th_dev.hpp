@@ -790,7 +790,7 @@ __device__ __forceinline__ void th_vjp_patch_block(int p, const ThVjpArgs& a, co
   if (mine)
   {
     e           = q.patch_elem[i];
-    const u4 x0 = __builtin_nontemporal_load(lp), x1 = __builtin_nontemporal_load(lp + 1);
+    const u4 x0 = lp[0], x1 = lp[1];
     w[0] = x0.x, w[1] = x0.y, w[2] = x0.z, w[3] = x0.w, w[4] = x1.x, w[5] = x1.y, w[6] = x1.z,
     w[7]       = x1.w;
     const i3 v = *reinterpret_cast<const i3*>(q.elem_vert + 3 * static_cast<long>(i));
@@ -853,7 +853,7 @@ __device__ __forceinline__ void th_vjp_patch_block(int p, const ThVjpArgs& a, co
   double* cs = sv;
   if (mine)
   {
-    const u4       x0   = __builtin_nontemporal_load(lp), x1 = __builtin_nontemporal_load(lp + 1);
+    const u4       x0   = lp[0], x1 = lp[1];
     const unsigned o[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
 #pragma unroll
     for (int k = 0; k < 15; ++k)
