@@ -305,6 +305,21 @@ constexpr int kP1StBand  = FX_P1_STENCIL_BAND;
 constexpr int kP1StWaves = 4;
 static_assert(4 * kP1StBand >= 64 * kP1StMax, "the band buffer holds the write-back");
 
+// Row li of an element's matrix and its residual entry, as p1_assemble_row
+// forms them (same operations, same order), for a compile-time li.
+template <int LI>
+__device__ __forceinline__ void p1_row_fixed(const P1AsmArgs& a, const P1Geom& g, double ke,
+                                             double u0, double u1, double u2, double f0,
+                                             double f1, double f2, double& k0, double& k1,
+                                             double& k2, double& re)
+{
+  k0 = p1_k(g, LI, 0, ke), k1 = p1_k(g, LI, 1, ke), k2 = p1_k(g, LI, 2, ke);
+  re = 0.0;
+  if (a.u) re = k0 * u0 + k1 * u1 + k2 * u2;
+  if (a.f) re = re - p1_load(g, LI, f0, f1, f2);
+  else re = re - 0.0;
+}
+
 __device__ __forceinline__ void p1_assemble_stencil_block(int b, const P1AsmArgs& a,
                                                           const int* __restrict__ blk_code,
                                                           const int* __restrict__ words,
@@ -369,11 +384,17 @@ __device__ __forceinline__ void p1_assemble_stencil_block(int b, const P1AsmArgs
     const double   kv = a.kappa ? gat32(a.kappa, e) : 1.0;
     const P1Geom   g  = p1_geom(bx[s0], by[s0], bx[s1], by[s1], bx[s2], by[s2]);
     const double   ke = (a.kappa && a.kappa_is_log) ? exp(kv) : kv;
-    const double   k0 = p1_k(g, li, 0, ke), k1 = p1_k(g, li, 1, ke), k2 = p1_k(g, li, 2, ke);
-    double         re = 0.0;
-    if (a.u) re = k0 * bu[s0] + k1 * bu[s1] + k2 * bu[s2];
-    if (a.f) re = re - p1_load(g, li, bf[s0], bf[s1], bf[s2]);
-    else re = re - 0.0;
+    const double   u0 = a.u ? bu[s0] : 0.0, u1 = a.u ? bu[s1] : 0.0, u2 = a.u ? bu[s2] : 0.0;
+    const double   f0 = a.f ? bf[s0] : 0.0, f1 = a.f ? bf[s1] : 0.0, f2 = a.f ? bf[s2] : 0.0;
+    double         k0, k1, k2, re;
+    // li is the same for every lane: one branch into a copy of the row
+    // arithmetic with li a constant (no per-lane selects)
+    switch (li)
+    {
+    case 0: p1_row_fixed<0>(a, g, ke, u0, u1, u2, f0, f1, f2, k0, k1, k2, re); break;
+    case 1: p1_row_fixed<1>(a, g, ke, u0, u1, u2, f0, f1, f2, k0, k1, k2, re); break;
+    default: p1_row_fixed<2>(a, g, ke, u0, u1, u2, f0, f1, f2, k0, k1, k2, re); break;
+    }
     rsum    = rsum + re;
     acc[p0] = acc[p0] + k0;
     acc[p1] = acc[p1] + k1;
