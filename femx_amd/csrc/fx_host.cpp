@@ -1243,7 +1243,7 @@ int fx_csr_stencil_plan(Index rows, Index cols, const Index* row_ptr, const Inde
     {
       const Index r0 = 64 * b;
       Index       code = -1;
-      const Index S    = row_ptr[r0 + 1 <= rows ? r0 + 1 : rows] - row_ptr[r0];
+      const Index S    = row_ptr[r0 + 1] - row_ptr[r0]; // r0 < rows
       if (r0 + 64 <= rows && S >= 1 && S <= max_len)
       {
         off.resize(static_cast<std::size_t>(S));

@@ -65,6 +65,7 @@ class CsrPattern:
         self._dev = {}
         self._tmap = {}
         self._off16 = {}
+        self._stencil = {}
 
     @property
     def nnz(self):
@@ -96,8 +97,8 @@ class CsrPattern:
         (fx_csr_stencil_plan), built once per device: (blk_code, st_off) device
         tensors and the fraction of rows in stencil blocks."""
         dev = torch.device(device)
-        key = "stencil:" + str(dev)
-        if key not in self._off16:
+        key = str(dev)
+        if key not in self._stencil:
             nb, no = C.c_int32(), C.c_int32()
             args = (self.rows, self.cols, ptr(self.row_ptr), ptr(self.col_ind),
                     self.STENCIL_MAX_LEN)
@@ -107,9 +108,9 @@ class CsrPattern:
             _lib.call("fx_csr_stencil_plan", *args, C.byref(nb), C.byref(no), ptr(code), ptr(off))
             code = code[: nb.value]
             frac = 64.0 * np.count_nonzero(code >= 0) / max(self.rows, 1)
-            self._off16[key] = (torch.from_numpy(code).to(dev), torch.from_numpy(off).to(dev),
-                                frac, code)
-        return self._off16[key]
+            self._stencil[key] = (torch.from_numpy(code).to(dev), torch.from_numpy(off).to(dev),
+                                  frac, code)
+        return self._stencil[key]
 
     def _host_transpose(self):
         if "host" not in self._tmap:
