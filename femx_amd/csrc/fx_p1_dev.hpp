@@ -353,35 +353,68 @@ __device__ __forceinline__ void p1_assemble_stencil_block(int b, const P1AsmArgs
     return;
   }
   double* bx = bb, *by = bb + kP1StBand, *bu = bb + 2 * kP1StBand, *bf = bb + 3 * kP1StBand;
+  // Buffer position v = lane + 64 j holds value i = v - sidx[q] of the band
+  // q it falls in: row r0 + i's column at offset off[q] (i < 64), or row
+  // r0 + 63's at offset off[q] + i - 63.  Every position's column is found
+  // first and all loads are issued before the first is waited on (one memory
+  // round trip per record, not one per band).
+  constexpr int kSlots = (kP1StBand + 63) / 64;
+  int           col[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j) col[j] = -1;
   for (int q = 0; q < S;)
   {
     const int o = off[q];
     int       w = 1;
     while (q + w < S && off[q + w] == o + w) ++w;
     const int base = sidx[q], len = 63 + w;
-    // value i of the band: row r0 + i's column at offset o (i < 64), or row
-    // r0 + 63's at offset o + i - 63
-    for (int i = lane; i < len; i += 64)
+#pragma unroll
+    for (int j = 0; j < kSlots; ++j)
     {
-      const int    c  = r0 + o + i;
-      const double xv = gat32(a.x, c), yv = gat32(a.y, c);
-      const double uv = a.u ? gat32(a.u, c) : 0.0, fv = a.f ? gat32(a.f, c) : 0.0;
-      bx[base + i] = xv, by[base + i] = yv, bu[base + i] = uv, bf[base + i] = fv;
+      const int i = lane + 64 * j - base;
+      if (i >= 0 && i < len) col[j] = r0 + o + i;
     }
     q += w;
   }
+  // positions outside every band load row r0's values (a valid node, one
+  // cache line for the wave) and are not stored; an absent u or f loads x
+  // and stores zero
+  const double* pu = a.u ? a.u : a.x;
+  const double* pf = a.f ? a.f : a.x;
+  double        xv[kSlots], yv[kSlots], uv[kSlots], fv[kSlots];
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j)
+  {
+    const int c = col[j] >= 0 ? col[j] : r0;
+    xv[j] = gat32(a.x, c), yv[j] = gat32(a.y, c), uv[j] = gat32(pu, c), fv[j] = gat32(pf, c);
+  }
+#pragma unroll
+  for (int j = 0; j < kSlots; ++j)
+    if (col[j] >= 0)
+    {
+      const int v = lane + 64 * j;
+      bx[v] = xv[j], by[v] = yv[j], bu[v] = a.u ? uv[j] : 0.0, bf[v] = a.f ? fv[j] : 0.0;
+    }
   wave_sync();
   double acc[kP1StMax];
 #pragma unroll
   for (int q = 0; q < kP1StMax; ++q) acc[q] = 0.0;
   double rsum = 0.0;
+  // kappa one incidence ahead: the next element's value is in flight while
+  // the current one computes
+  auto kappa_at = [&](int k) {
+    return a.kappa ? gat32(a.kappa, static_cast<int>(A * static_cast<unsigned>(r)
+                                                     + static_cast<unsigned>(inc[2 * k])))
+                   : 1.0;
+  };
+  double kv_next = kappa_at(0);
   for (int k = 0; k < T; ++k)
   {
-    const int      e    = static_cast<int>(A * static_cast<unsigned>(r) + static_cast<unsigned>(inc[2 * k]));
     const unsigned meta = static_cast<unsigned>(inc[2 * k + 1]);
     const int      li = meta & 0xff, p0 = (meta >> 8) & 0xff, p1 = (meta >> 16) & 0xff, p2 = meta >> 24;
     const int      s0 = sidx[p0] + lane, s1 = sidx[p1] + lane, s2 = sidx[p2] + lane;
-    const double   kv = a.kappa ? gat32(a.kappa, e) : 1.0;
+    const double   kv = kv_next;
+    if (k + 1 < T) kv_next = kappa_at(k + 1);
     const P1Geom   g  = p1_geom(bx[s0], by[s0], bx[s1], by[s1], bx[s2], by[s2]);
     const double   ke = (a.kappa && a.kappa_is_log) ? exp(kv) : kv;
     const double   u0 = a.u ? bu[s0] : 0.0, u1 = a.u ? bu[s1] : 0.0, u2 = a.u ? bu[s2] : 0.0;
