@@ -32,6 +32,7 @@ _SIGS = {
     "fx_map_export_rowinc15": ([_vp, _vp, _vp, _vp], C.c_int),
     "fx_map_th_blocks": ([_vp, _i32, _i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_map_th_lanes": ([_vp, _i32, _vp, _vp, _vp], C.c_int),
+    "fx_map_th_rounds": ([_i32] + [_vp] * 9, C.c_int),
     "fx_map_gls_plan": ([_vp, _i32] + [_vp] * 9, C.c_int),
     "fx_map_destroy": ([_vp], None),
     "fx_csr_transpose_map": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp], C.c_int),
@@ -69,6 +70,7 @@ _SIGS = {
     "fx_p1_assemble_patch": ([_i32, _i32, _i32] + [_vp] * 4 + [C.c_int] + [_vp] * 8 + [_vp],
                              C.c_int),
     "fx_th_assemble": ([_i32] * 4 + [_vp] * 5 + [_f64] + [_vp] * 10 + [_vp], C.c_int),
+    "fx_th_assemble_rounds": ([_i32] * 4 + [_vp] * 5 + [_f64] + [_vp] * 10 + [_vp], C.c_int),
     "fx_csr_matvec_multi": ([_i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp, _i32, _vp, _i32, _f64,
                              _f64, _vp], C.c_int),
     "fx_vec_div": ([_i32, _vp, _vp, _vp, _vp], C.c_int),

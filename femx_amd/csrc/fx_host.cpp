@@ -562,6 +562,79 @@ int fx_map_th_lanes(const fx_map* m, Index num_blocks, const Index* blk_row, con
   });
 }
 
+// Round tables of the Taylor-Hood assembly.  Block b runs R_b rounds, the
+// most incidences any of its lanes takes under the lane map; the incidence
+// that lane l takes in round m is entry rt_off[b] + 64 m + l of rt_elem /
+// rt_meta (four words, as fx_map_export_rowinc15 writes them).  An entry past
+// the lane's last incidence, or on a lane the map leaves idle, repeats a
+// valid incidence of the block -- the lane's last, else the block's first --
+// with bit 31 of its last meta word set: it is loaded, never added.  A block
+// whose rows have no incidence gets one such round (filled with incidence 0)
+// when the mesh has any incidence, so that a kernel walking several blocks
+// always finds entries; in a mesh without incidences every R_b is 0.
+int fx_map_th_rounds(Index num_blocks, const Index* blk_row, const Index* blk_flags,
+                     const Index* blk_lanes, const Index* inc_off, const Index* inc_elem,
+                     const Index* inc_meta, Index* rt_off, Index* rt_elem, Index* rt_meta)
+{
+  return fx::guard([&] {
+    fx::require(num_blocks >= 0 && rt_off
+                    && (num_blocks == 0 || (blk_row && blk_flags && blk_lanes && inc_off)),
+                FX_ERR_ARG, "fx_map_th_rounds: invalid arguments");
+    constexpr int      L = 64;
+    std::vector<Index> first(L), count(L), stride(L);
+    std::int64_t       total = 0;
+    rt_off[0]                = 0;
+    const bool any = num_blocks > 0 && inc_off[blk_row[num_blocks]] > inc_off[blk_row[0]];
+    for (Index b = 0; b < num_blocks; ++b)
+    {
+      const Index rs = blk_row[b], re = blk_row[b + 1], step = blk_flags[b] ? 2 : 1;
+      fx::require(re > rs && (re - rs) % step == 0, FX_ERR_ARG,
+                  "fx_map_th_rounds: blk_row is not a block partition of the rows");
+      const Index  nitems = (re - rs) / step;
+      const Index* lm     = blk_lanes + static_cast<std::int64_t>(L) * b;
+      Index        R = 0, fill = -1;
+      for (int l = 0; l < L; ++l)
+      {
+        count[l] = 0;
+        if (lm[l] < 0) continue;
+        const Index item = lm[l] & 0xff, role = (lm[l] >> 8) & 0xff,
+                    k = std::max<Index>(1, lm[l] >> 16);
+        fx::require(item < nitems && role < k, FX_ERR_ARG, "fx_map_th_rounds: lane map out of range");
+        const Index r = rs + step * item, t0 = inc_off[r], t1 = inc_off[r + 1];
+        // the lane's incidences are t0 + role + m k, m < count
+        first[l]  = t0 + role;
+        stride[l] = k;
+        count[l]  = t0 + role < t1 ? (t1 - t0 - role + k - 1) / k : 0;
+        R         = std::max(R, count[l]);
+        if (count[l] > 0 && fill < 0) fill = t0 + role;
+      }
+      if (R == 0 && any) R = 1, fill = inc_off[blk_row[0]];
+      total += static_cast<std::int64_t>(L) * R;
+      fx::require(total <= std::numeric_limits<Index>::max(), FX_ERR_OVERFLOW,
+                  "fx_map_th_rounds: round tables beyond 2^31 entries");
+      rt_off[b + 1] = static_cast<Index>(total);
+      if (!rt_elem) continue;
+      for (Index m = 0; m < R; ++m)
+        for (int l = 0; l < L; ++l)
+        {
+          const Index c = count[l];
+          Index       t;
+          bool        idle;
+          if (c > 0)
+            t = first[l] + std::min(m, c - 1) * stride[l], idle = m >= c;
+          else
+            t = fill, idle = true;
+          const std::int64_t q = rt_off[b] + static_cast<std::int64_t>(L) * m + l;
+          rt_elem[q]           = inc_elem[t];
+          std::memcpy(rt_meta + 4 * q, inc_meta + 4 * static_cast<std::int64_t>(t), 16);
+          if (idle)
+            rt_meta[4 * q + 3] =
+                static_cast<Index>(static_cast<std::uint32_t>(rt_meta[4 * q + 3]) | 0x80000000u);
+        }
+    }
+  });
+}
+
 // Node plan of the GLS P1/P1 kernel.  A node's three rows are read off the
 // element DOF tables (local rows 2k, 2k+1 and 6+k of local node k); every
 // element must agree, and every row must belong to exactly one node.  Blocks

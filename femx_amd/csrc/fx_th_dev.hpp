@@ -469,6 +469,10 @@ struct ThAsmArgs
   const int*    row_ptr;
   double*       vals;
   double*       res;
+  // round tables (fx_map_th_rounds), or null
+  const int*    rt_off;
+  const int*    rt_elem;
+  const uint4*  rt_meta;
 };
 
 __device__ __forceinline__ ThRaw th_load_raw(const ThAsmArgs& a, const ThDofs& D, int e)
@@ -609,6 +613,192 @@ __device__ __forceinline__ void th_assemble_block(int b, const ThAsmArgs& a, con
   wave_sync();
   for (int i = lane; i < bn; i += kThLanes) __builtin_nontemporal_store(acc[i], a.vals + base + i);
   wave_sync();
+}
+
+// The velocity rows (or single row) of one incidence.
+__device__ __forceinline__ void th_incidence_rows(const ThTensor& T, const ThRaw& cur, unsigned li,
+                                                  bool pairs, double* K0, double* K1, double& R0,
+                                                  double& R1)
+{
+  ThElemT E;
+  th_make_elem(E, cur);
+  R1 = 0.0;
+  if (li < 12)
+  {
+    th_velocity_rows(T, E, li >> 1, K0, K1, R0, R1);
+    if (!pairs && (li & 1))
+    {
+#pragma unroll
+      for (int j = 0; j < 15; ++j) K0[j] = K1[j];
+      R0 = R1;
+    }
+  }
+  else
+    th_pressure_row(T, E, li - 12, K0, R0);
+}
+
+// A position in a wave's walk over the round tables (fx_map_th_rounds):
+// block b (the wave takes blocks first, first + stride, ... < end), round m of
+// its R, whose entries start at q.  Every field is the same on all lanes.
+struct ThCur
+{
+  int b, m, R, q;
+};
+
+__device__ __forceinline__ ThCur th_cur_at(const ThAsmArgs& a, int b)
+{
+  const int q = a.rt_off[b];
+  return {b, 0, (a.rt_off[b + 1] - q) / kThLanes, q};
+}
+
+// The position after c; past the walk's last one it stays there (its loads
+// stay valid and are never used).
+__device__ __forceinline__ ThCur th_cur_next(const ThAsmArgs& a, const ThCur& c, int end,
+                                             int stride)
+{
+  if (c.m + 1 < c.R) return {c.b, c.m + 1, c.R, c.q};
+  if (c.b + stride < end) return th_cur_at(a, c.b + stride);
+  return c;
+}
+
+__device__ __forceinline__ ThIdx th_cur_idx(const ThAsmArgs& a, const ThCur& c, int lane)
+{
+  const int i = c.q + c.m * kThLanes + lane;
+  return {a.rt_elem[i], a.rt_meta[i]};
+}
+
+// A lane's part in one block: its item's row(s) in the tile, its role and
+// its item's lane count k (the lane map), and the block's widest k.
+struct ThLane
+{
+  double* ax;
+  double* ay;
+  int     r, role, k, kmax, base, bn;
+  bool    own, pairs;
+};
+
+// Set up block b: the lane's item and the zeroed tile.
+__device__ __forceinline__ ThLane th_lane_begin(const ThAsmArgs& a, int b, double* acc, int lane)
+{
+  ThLane    L;
+  const int rs = a.blk_row[b], re = a.blk_row[b + 1];
+  L.pairs          = a.blk_flags[b] != 0;
+  L.base           = a.row_ptr[rs];
+  L.bn             = a.vals ? a.row_ptr[re] - L.base : 0;
+  const int lm     = a.blk_lanes[kThLanes * b + lane];
+  const int nitems = L.pairs ? (re - rs) >> 1 : re - rs;
+  L.own            = lm >= 0 && (lm & 0xff) < nitems;
+  L.role           = (lm >> 8) & 0xff;
+  L.k              = L.own ? max(1, lm >> 16) : 1;
+  L.r              = L.own ? rs + (L.pairs ? 2 * (lm & 0xff) : lm & 0xff) : rs;
+  int kmax         = L.k;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
+  L.kmax = kmax;
+  L.ax   = acc + (a.row_ptr[L.r] - L.base);
+  L.ay   = L.pairs ? acc + (a.row_ptr[L.r + 1] - L.base) : nullptr;
+  for (int i = lane; i < L.bn; i += kThLanes) acc[i] = 0.0;
+  wave_sync();
+  return L;
+}
+
+// Finish a block: residuals of the lane's item, then the tile's write-back.
+__device__ __forceinline__ void th_lane_end(const ThAsmArgs& a, const ThLane& L, double sx, double sy,
+                                            const double* acc, int lane)
+{
+  if (L.own && L.role == 0)
+  {
+    __builtin_nontemporal_store(sx, a.res + L.r);
+    if (L.pairs) __builtin_nontemporal_store(sy, a.res + L.r + 1);
+  }
+  wave_sync();
+  for (int i = lane; i < L.bn; i += kThLanes)
+    __builtin_nontemporal_store(acc[i], a.vals + L.base + i);
+  wave_sync();
+}
+
+// th_assemble_block over the round tables, for a wave that walks blocks
+// first, first + stride, ... < end.  Lane l's incidence of round m of block b
+// is entry 64 m + l of the block's range, so the incidences of the next
+// rounds are read at addresses known from the walk alone, and the pipeline
+// (incidence two rounds ahead, element tables and values one round ahead)
+// runs on across block boundaries: the next block's first element loads are
+// in flight while this block's last round computes.  Every lane runs each
+// block's rounds, adding only what its entries do not mark idle.  The adds,
+// residual sums and write-back are th_assemble_block's, in the same order:
+// the results are bitwise equal to it.
+__device__ __forceinline__ void th_assemble_walk_rt(const ThAsmArgs& a, const ThTensor& T,
+                                                    double* acc, int first, int end, int stride)
+{
+  const int lane = threadIdx.x & (kThLanes - 1);
+  if (first >= end) return;
+  ThCur c0 = th_cur_at(a, first);
+  if (c0.R == 0)
+  {
+    // no incidences in the mesh (otherwise every block has a round)
+    for (int b = first; b < end; b += stride)
+      th_lane_end(a, th_lane_begin(a, b, acc, lane), 0.0, 0.0, acc, lane);
+    return;
+  }
+  ThCur  c1 = th_cur_next(a, c0, end, stride), c2 = th_cur_next(a, c1, end, stride);
+  ThIdx  ic = th_cur_idx(a, c0, lane), in = th_cur_idx(a, c1, lane);
+  ThRaw  ra = th_load_raw(a, th_load_dofs(a.dofs, a.vconn, ic.e), ic.e), rb;
+  ThDofs dn = th_load_dofs(a.dofs, a.vconn, in.e);
+  ThLane L  = th_lane_begin(a, c0.b, acc, lane);
+  double sx = 0.0, sy = 0.0;
+  bool   done  = false;
+  auto   round = [&](const ThRaw& cur, ThRaw& nxt) {
+    const ThIdx in2 = th_cur_idx(a, c2, lane);
+    nxt             = th_load_raw(a, dn, in.e);
+    const uint4 meta = ic.meta;
+    const bool  have = L.own && !(meta.w >> 31);
+    double      K0[15], K1[15], R0, R1;
+    th_incidence_rows(T, cur, (meta.w >> 24) & 0x7f, L.pairs, K0, K1, R0, R1);
+    for (int j = 0; j < L.kmax; ++j)
+    {
+      if (L.role == j && have && a.vals) th_add_rows(L.ax, L.ay, meta, K0, K1, L.pairs);
+      wave_sync();
+    }
+    if (L.role == 0 && have)
+    {
+      sx = sx + R0;
+      if (L.pairs) sy = sy + R1;
+    }
+    for (int j = 1; j < L.kmax; ++j)
+    {
+      const double pR0 = __shfl(R0, lane + j), pR1 = __shfl(R1, lane + j);
+      const int    ph  = __shfl(static_cast<int>(have), lane + j);
+      if (L.role == 0 && j < L.k && ph)
+      {
+        sx = sx + pR0;
+        if (L.pairs) sy = sy + pR1;
+      }
+    }
+    dn = th_load_dofs(a.dofs, a.vconn, in2.e);
+    ic = in;
+    in = in2;
+    if (c0.m + 1 >= c0.R)
+    {
+      th_lane_end(a, L, sx, sy, acc, lane);
+      if (c0.b + stride >= end)
+      {
+        done = true;
+        return;
+      }
+      sx = 0.0, sy = 0.0;
+      L  = th_lane_begin(a, c0.b + stride, acc, lane);
+    }
+    c0 = c1;
+    c1 = c2;
+    c2 = th_cur_next(a, c2, end, stride);
+  };
+  for (;;)
+  {
+    round(ra, rb);
+    if (done) break;
+    round(rb, ra);
+    if (done) break;
+  }
 }
 
 struct ThVjpArgs

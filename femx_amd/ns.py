@@ -96,7 +96,7 @@ class THBlocks:
     `items` work items per block and row tile bound `tile_nnz`, and its lane
     map), on one device."""
 
-    def __init__(self, amap: AssemblyMap, device, items, tile_nnz):
+    def __init__(self, amap: AssemblyMap, device, items, tile_nnz, rowinc=None):
         self.items, self.tile_nnz = int(items), int(tile_nnz)
         nb, mx = C.c_int32(), C.c_int32()
         args = (amap._handle(), self.items, self.tile_nnz)
@@ -114,6 +114,20 @@ class THBlocks:
         t = lambda a: torch.from_numpy(a).to(device)  # noqa: E731
         self.blk_row, self.blk_flags = t(blk_row), t(self.h_blk_flags)
         self.blk_lanes = t(self.h_blk_lanes)
+        # round tables (fx_map_th_rounds) from the exported row incidences
+        self.rt_off = self.rt_elem = self.rt_meta = None
+        if rowinc is not None:
+            off, elem, meta = rowinc
+            rt_off = np.empty(nb.value + 1, np.int32)
+            args = (nb.value, ptr(blk_row), ptr(self.h_blk_flags), ptr(self.h_blk_lanes), ptr(off),
+                    ptr(elem), ptr(meta))
+            _lib.call("fx_map_th_rounds", *args, ptr(rt_off), None, None)
+            n = int(rt_off[-1])
+            rt_elem = np.empty(max(n, 1), np.int32)
+            rt_meta = np.empty(max(4 * n, 4), np.int32)
+            _lib.call("fx_map_th_rounds", *args, ptr(rt_off), ptr(rt_elem), ptr(rt_meta))
+            self.h_rt_off, self.h_rt_elem, self.h_rt_meta = rt_off, rt_elem[:n], rt_meta[: 4 * n]
+            self.rt_off, self.rt_elem, self.rt_meta = t(rt_off), t(rt_elem), t(rt_meta)
 
 
 class THPlan:
@@ -141,9 +155,10 @@ class THPlan:
         self.elem_inc = t(slots[: 15 * amap.num_elems])
         self._work, self._dev, self._num_elems = None, dev, amap.num_elems
         self.layouts = {}
+        rowinc = (off, elem, meta)
         for it, tl in [(items, tile_nnz)] + list(variants):
             if (int(it), int(tl)) not in self.layouts:
-                self.layouts[(int(it), int(tl))] = THBlocks(amap, dev, it, tl)
+                self.layouts[(int(it), int(tl))] = THBlocks(amap, dev, it, tl, rowinc)
         self.select(items, tile_nnz)
 
     @property
@@ -161,6 +176,7 @@ class THPlan:
         self.num_blocks, self.max_block_nnz = b.num_blocks, b.max_block_nnz
         self.h_blk_row, self.h_blk_flags, self.h_blk_lanes = b.h_blk_row, b.h_blk_flags, b.h_blk_lanes
         self.blk_row, self.blk_flags, self.blk_lanes = b.blk_row, b.blk_flags, b.blk_lanes
+        self.rt_off, self.rt_elem, self.rt_meta = b.rt_off, b.rt_elem, b.rt_meta
 
 
 TH_PATCH = 256  # elements per VJP patch (FX_TH_PATCH_MAX)
@@ -212,9 +228,12 @@ def _check(geom: THGeometry, amap: AssemblyMap):
 
 
 def assemble_ns(op: TaylorHoodNS, geom: THGeometry, amap: AssemblyMap, state, res,
-                jac: CsrMatrix = None, stream=None):
+                jac: CsrMatrix = None, stream=None, rounds=True):
     """Residual into `res` and, when `jac` is given, the Newton Jacobian into
-    `jac.vals`; outputs replaced.  Returns `res`."""
+    `jac.vals`; outputs replaced.  Returns `res`.  `rounds` reads each block's
+    incidences through its round table (fx_th_assemble_rounds) instead of
+    through the lane map and row offsets (fx_th_assemble); the results are
+    the same bit for bit."""
     _check(geom, amap)
     _check_dev_vec(state, geom.ndof, torch.float64, "Assembly state")
     _check_dev_vec(res, geom.ndof, torch.float64, "Assembly residual")
@@ -227,12 +246,15 @@ def assemble_ns(op: TaylorHoodNS, geom: THGeometry, amap: AssemblyMap, state, re
         _check_dev_vec(op.nu_e, geom.num_elems, torch.float64, "Per-element viscosity")
     p = _plan(amap, geom.device)
     rp, _ = amap.pattern.device_arrays(geom.device)
-    _lib.call("fx_th_assemble", geom.ndof, geom.num_elems, p.num_blocks, p.max_block_nnz,
-              ptr(geom.xv), ptr(geom.yv), ptr(geom.vconn3), ptr(geom.dofs15), ptr(state), op.nu,
-              ptr(op.nu_e), ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row),
-              ptr(p.blk_flags), ptr(p.blk_lanes), ptr(rp),
-              ptr(jac.vals) if jac is not None else None, ptr(res),
-              stream_handle(stream))
+    out = (ptr(rp), ptr(jac.vals) if jac is not None else None, ptr(res), stream_handle(stream))
+    head = (geom.ndof, geom.num_elems, p.num_blocks, p.max_block_nnz, ptr(geom.xv), ptr(geom.yv),
+            ptr(geom.vconn3), ptr(geom.dofs15), ptr(state), op.nu, ptr(op.nu_e))
+    if rounds and p.rt_off is not None:
+        _lib.call("fx_th_assemble_rounds", *head, ptr(p.blk_row), ptr(p.blk_flags),
+                  ptr(p.blk_lanes), ptr(p.rt_off), ptr(p.rt_elem), ptr(p.rt_meta), *out)
+    else:
+        _lib.call("fx_th_assemble", *head, ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta),
+                  ptr(p.blk_row), ptr(p.blk_flags), ptr(p.blk_lanes), *out)
     return res
 
 

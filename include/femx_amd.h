@@ -150,6 +150,18 @@ int fx_map_th_blocks(const fx_map* map, fx_index lanes, fx_index max_nnz, fx_ind
  * (The map that fx_th_assemble uses when given none is two lanes per item.) */
 int fx_map_th_lanes(const fx_map* map, fx_index num_blocks, const fx_index* h_blk_row,
                     const fx_index* h_blk_flags, fx_index* h_blk_lanes);
+/* Round tables of those blocks (host): block b runs R_b rounds, the most
+ * incidences any of its lanes takes under the lane map, and the incidence
+ * its lane l takes in round m is entry h_rt_off[b] + 64 m + l of h_rt_elem
+ * and h_rt_meta (four words per entry, as fx_map_export_rowinc15 writes
+ * them, from its h_inc_off, h_inc_elem and h_inc_meta).  An entry past the
+ * lane's last incidence, or on an idle lane, repeats a valid incidence of
+ * the block with bit 31 of its last meta word set.  h_rt_off[num_blocks+1];
+ * pass null tables to get h_rt_off (the table length is its last entry). */
+int fx_map_th_rounds(fx_index num_blocks, const fx_index* h_blk_row, const fx_index* h_blk_flags,
+                     const fx_index* h_blk_lanes, const fx_index* h_inc_off,
+                     const fx_index* h_inc_elem, const fx_index* h_inc_meta, fx_index* h_rt_off,
+                     fx_index* h_rt_elem, fx_index* h_rt_meta);
 /* Node plan of the GLS P1/P1 Navier-Stokes kernel (fx_gls_assemble) over a
  * map whose elements have 9 local DOFs in NavierOperator order -- u0 v0 u1
  * v1 u2 v2 p0 p1 p2 (femx/model/ns/Components.hpp:247-256) -- for the mesh
@@ -490,6 +502,21 @@ int fx_th_assemble(fx_index ndof, fx_index num_elems, fx_index num_blocks,
                    const fx_index* d_blk_row, const fx_index* d_blk_flags,
                    const fx_index* d_blk_lanes, const fx_index* d_row_ptr, double* d_vals,
                    double* d_res, void* stream);
+/* fx_th_assemble over the blocks' round tables (fx_map_th_rounds, on the
+ * device; the lane map is required): each round's incidences are read at
+ * addresses known from the block alone, so a block's first element loads
+ * do not wait on its lane map and row offsets.  Drops in for the same
+ * assembly::assemble call (femx/assembly/CudaAssembly.hpp:319-364); the
+ * results equal fx_th_assemble's bit for bit. */
+int fx_th_assemble_rounds(fx_index ndof, fx_index num_elems, fx_index num_blocks,
+                          fx_index max_block_nnz, const double* d_xv, const double* d_yv,
+                          const fx_index* d_vconn3, const fx_index* d_dofs15,
+                          const double* d_state, double nu, const double* d_nu_e,
+                          const fx_index* d_blk_row, const fx_index* d_blk_flags,
+                          const fx_index* d_blk_lanes, const fx_index* d_rt_off,
+                          const fx_index* d_rt_elem, const fx_index* d_rt_meta,
+                          const fx_index* d_row_ptr, double* d_vals, double* d_res,
+                          void* stream);
 /* Navier-Stokes adjoint VJP: jt = J^T lam over the row-replaced Newton
  * Jacobian (lam zeroed on constrained rows, passed through there; bc_mask may
  * be NULL) and, when g is non-null, g[e] = lam~_e^T dR_e/dnu_e, the

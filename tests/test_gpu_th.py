@@ -73,6 +73,46 @@ def test_th_lane_map_and_two_lane_default_agree():
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("n,items,tile,relabel", [(9, 32, 0, None), (9, 16, 0, None),
+                                                  (9, 24, 0, None), (9, 32, 1500, None),
+                                                  (7, 32, 0, True), (1, 32, 0, None)])
+def test_th_round_tables_equal_lane_map_kernel(n, items, tile, relabel):
+    """fx_th_assemble_rounds (each round's incidences from the block's round
+    table) and fx_th_assemble (through the lane map and row offsets) add the
+    same terms in the same order: Jacobian and residual bitwise equal, with
+    and without the Jacobian, for several block layouts and a perturbed,
+    relabelled mesh."""
+    from femx_amd import _lib
+    from femx_amd.linalg import ptr, stream_handle
+    if relabel is None:
+        g, amap = setup(n)
+    else:
+        g = perturbed(n, 4, relabel)
+        amap = make_assembly_map(g.ndof, g.ndof, g.h_dofs15)
+    s = dev(RNG.standard_normal(g.ndof))
+    p = ns.THPlan(amap, DEV, items=items, tile_nnz=tile)
+    rp, _ = amap.pattern.device_arrays(DEV)
+    for with_jac in (True, False):
+        out = []
+        for rounds in (False, True):
+            jac = CsrMatrix(amap.pattern, DEV)
+            res = torch.empty(g.ndof, dtype=torch.float64, device=DEV)
+            head = (g.ndof, g.num_elems, p.num_blocks, p.max_block_nnz, ptr(g.xv), ptr(g.yv),
+                    ptr(g.vconn3), ptr(g.dofs15), ptr(s), 0.01, None)
+            tail = (ptr(rp), ptr(jac.vals) if with_jac else None, ptr(res), stream_handle())
+            if rounds:
+                _lib.call("fx_th_assemble_rounds", *head, ptr(p.blk_row), ptr(p.blk_flags),
+                          ptr(p.blk_lanes), ptr(p.rt_off), ptr(p.rt_elem), ptr(p.rt_meta), *tail)
+            else:
+                _lib.call("fx_th_assemble", *head, ptr(p.inc_off), ptr(p.inc_elem),
+                          ptr(p.inc_meta), ptr(p.blk_row), ptr(p.blk_flags), ptr(p.blk_lanes),
+                          *tail)
+            out.append((host(jac.vals), host(res)))
+        assert np.array_equal(out[0][1], out[1][1])
+        if with_jac:
+            assert np.array_equal(out[0][0], out[1][0])
+
+
 def test_th_full_size_matches_oracle_and_is_deterministic():
     """Config 3: 512^2 cavity, Re = 100, Fourier state with lid."""
     g, amap = setup(512)

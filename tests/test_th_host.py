@@ -139,6 +139,64 @@ def test_th_lane_map(n):
                   ptr(np.empty(128, np.int32)))
 
 
+@pytest.mark.parametrize("n,items,tile", [(1, 32, 0), (3, 32, 0), (8, 32, 0), (8, 16, 0),
+                                           (8, 32, 600)])
+def test_th_round_tables(n, items, tile):
+    """fx_map_th_rounds: block b's lane l takes in round m the incidence
+    t0 + role + m k of its lane-map item's row; past the lane's last
+    incidence, and on idle lanes, an entry repeats a valid incidence (the
+    lane's last, else the block's first) with bit 31 of its last meta word
+    set; every incidence of every row appears exactly once unflagged, and
+    R_b is the most incidences of any lane."""
+    g = ns.structured_th(n, "cpu")
+    amap = make_assembly_map(g.ndof, g.ndof, g.h_dofs15)
+    plan = ns.THPlan(amap, "cpu", items=items, tile_nnz=tile)
+    off, elem = plan.inc_off.numpy(), plan.inc_elem.numpy()
+    meta = plan.inc_meta.numpy().reshape(-1, 4).view(np.uint32)
+    rto, rte = plan.rt_off.numpy(), plan.rt_elem.numpy()
+    rtm = plan.rt_meta.numpy().reshape(-1, 4).view(np.uint32)
+    lanes = plan.h_blk_lanes.reshape(-1, 64)
+    br, bf = plan.h_blk_row, plan.h_blk_flags
+    assert rto[0] == 0 and np.all(np.diff(rto) % 64 == 0) and rto[-1] == rte.size
+    flag = np.uint32(1 << 31)
+    seen = np.zeros(off[-1], np.int64)
+    for b in range(plan.num_blocks):
+        step = 2 if bf[b] else 1
+        R = (rto[b + 1] - rto[b]) // 64
+        lists = {}
+        for lane, lm in enumerate(lanes[b]):
+            if lm < 0:
+                continue
+            item, role, k = lm & 0xFF, (lm >> 8) & 0xFF, max(1, lm >> 16)
+            r = br[b] + step * item
+            lists[lane] = list(range(off[r] + role, off[r + 1], k))
+        assert R == max([len(v) for v in lists.values()] + [1])  # at least one round
+        fill = next((v[0] for _, v in sorted(lists.items()) if v), off[br[0]])
+        for m in range(R):
+            for lane in range(64):
+                q = rto[b] + 64 * m + lane
+                ts = lists.get(lane, [])
+                if m < len(ts):
+                    t = ts[m]
+                    assert rte[q] == elem[t] and np.array_equal(rtm[q], meta[t])
+                    seen[t] += 1
+                else:
+                    t = ts[-1] if ts else fill
+                    assert rtm[q, 3] & flag and rte[q] == elem[t]
+                    assert np.array_equal(rtm[q, :3], meta[t, :3]) and rtm[q, 3] == meta[t, 3] | flag
+    # every row's incidences, each once (rows of a pair block: both rows' lists
+    # are the same elements, taken once through the item's first row)
+    for b in range(plan.num_blocks):
+        rows = range(br[b], br[b + 1], 2 if bf[b] else 1)
+        for r in rows:
+            assert np.all(seen[off[r]:off[r + 1]] == 1)
+    bad = np.array([0, 5, 3], np.int32)
+    with pytest.raises(RuntimeError, match="block partition"):
+        _lib.call("fx_map_th_rounds", 2, ptr(bad), ptr(np.zeros(2, np.int32)),
+                  ptr(np.full(128, -1, np.int32)), ptr(off), ptr(elem), ptr(plan.inc_meta.numpy()),
+                  ptr(np.empty(3, np.int32)), None, None)
+
+
 @pytest.mark.parametrize("n", [49, 499])
 def test_lid_and_walls_found_when_the_last_coordinate_misses_one(n):
     """femx's structured coordinates are x_min + i*dx (Mesh.cpp:19-29); for
