@@ -677,29 +677,70 @@ struct ThLane
   bool    own, pairs;
 };
 
-// Set up block b: the lane's item and the zeroed tile.
-__device__ __forceinline__ ThLane th_lane_begin(const ThAsmArgs& a, int b, double* acc, int lane)
+// What a lane reads of block b before it can add into the block's tile: the
+// block's row range and kind, its lane-map entry, and its item's row offsets.
+// The walk loads a block's part one block ahead (th_pre_lanes at the start
+// of the block before it, th_pre_rows after that block's first round).
+struct ThPre
+{
+  int rs, re, pairs, lm, rx, ry;
+};
+
+__device__ __forceinline__ ThPre th_pre_lanes(const ThAsmArgs& a, int b, int lane)
+{
+  ThPre p;
+  p.rs    = a.blk_row[b];
+  p.re    = a.blk_row[b + 1];
+  p.pairs = a.blk_flags[b] != 0;
+  p.lm    = a.blk_lanes[kThLanes * b + lane];
+  p.rx = p.ry = 0;
+  return p;
+}
+
+// The lane's item row (or the block's first row for a lane without one).
+__device__ __forceinline__ int th_pre_row(const ThPre& p)
+{
+  const int nitems = p.pairs ? (p.re - p.rs) >> 1 : p.re - p.rs;
+  const bool own   = p.lm >= 0 && (p.lm & 0xff) < nitems;
+  return own ? p.rs + (p.pairs ? 2 * (p.lm & 0xff) : p.lm & 0xff) : p.rs;
+}
+
+__device__ __forceinline__ void th_pre_rows(const ThAsmArgs& a, ThPre& p)
+{
+  const int r = th_pre_row(p);
+  p.rx        = a.row_ptr[r];
+  p.ry        = p.pairs ? a.row_ptr[r + 1] : 0;
+}
+
+// Set up a block from its preloaded part: the lane's item and the zeroed tile.
+__device__ __forceinline__ ThLane th_lane_begin(const ThAsmArgs& a, const ThPre& p, double* acc,
+                                                int lane)
 {
   ThLane    L;
-  const int rs = a.blk_row[b], re = a.blk_row[b + 1];
-  L.pairs          = a.blk_flags[b] != 0;
-  L.base           = a.row_ptr[rs];
-  L.bn             = a.vals ? a.row_ptr[re] - L.base : 0;
-  const int lm     = a.blk_lanes[kThLanes * b + lane];
-  const int nitems = L.pairs ? (re - rs) >> 1 : re - rs;
-  L.own            = lm >= 0 && (lm & 0xff) < nitems;
-  L.role           = (lm >> 8) & 0xff;
-  L.k              = L.own ? max(1, lm >> 16) : 1;
-  L.r              = L.own ? rs + (L.pairs ? 2 * (lm & 0xff) : lm & 0xff) : rs;
+  L.pairs          = p.pairs != 0;
+  L.base           = a.row_ptr[p.rs];
+  L.bn             = a.vals ? a.row_ptr[p.re] - L.base : 0;
+  const int nitems = L.pairs ? (p.re - p.rs) >> 1 : p.re - p.rs;
+  L.own            = p.lm >= 0 && (p.lm & 0xff) < nitems;
+  L.role           = (p.lm >> 8) & 0xff;
+  L.k              = L.own ? max(1, p.lm >> 16) : 1;
+  L.r              = th_pre_row(p);
   int kmax         = L.k;
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) kmax = max(kmax, __shfl_xor(kmax, off));
   L.kmax = kmax;
-  L.ax   = acc + (a.row_ptr[L.r] - L.base);
-  L.ay   = L.pairs ? acc + (a.row_ptr[L.r + 1] - L.base) : nullptr;
+  L.ax   = acc + (p.rx - L.base);
+  L.ay   = L.pairs ? acc + (p.ry - L.base) : nullptr;
   for (int i = lane; i < L.bn; i += kThLanes) acc[i] = 0.0;
   wave_sync();
   return L;
+}
+
+__device__ __forceinline__ ThLane th_lane_begin(const ThAsmArgs& a, int b, double* acc, int lane)
+{
+  ThPre p = th_pre_lanes(a, b, lane);
+  th_pre_rows(a, p);
+  return th_lane_begin(a, p, acc, lane);
 }
 
 // Finish a block: residuals of the lane's item, then the tile's write-back.
@@ -745,6 +786,9 @@ __device__ __forceinline__ void th_assemble_walk_rt(const ThAsmArgs& a, const Th
   ThRaw  ra = th_load_raw(a, th_load_dofs(a.dofs, a.vconn, ic.e), ic.e), rb;
   ThDofs dn = th_load_dofs(a.dofs, a.vconn, in.e);
   ThLane L  = th_lane_begin(a, c0.b, acc, lane);
+  // the next block's part, one block ahead
+  const bool more = first + stride < end;
+  ThPre      pn   = th_pre_lanes(a, more ? first + stride : first, lane);
   double sx = 0.0, sy = 0.0;
   bool   done  = false;
   auto   round = [&](const ThRaw& cur, ThRaw& nxt) {
@@ -777,6 +821,7 @@ __device__ __forceinline__ void th_assemble_walk_rt(const ThAsmArgs& a, const Th
     dn = th_load_dofs(a.dofs, a.vconn, in2.e);
     ic = in;
     in = in2;
+    if (c0.m == 0) th_pre_rows(a, pn); // its lane-map entry has arrived by now
     if (c0.m + 1 >= c0.R)
     {
       th_lane_end(a, L, sx, sy, acc, lane);
@@ -786,7 +831,9 @@ __device__ __forceinline__ void th_assemble_walk_rt(const ThAsmArgs& a, const Th
         return;
       }
       sx = 0.0, sy = 0.0;
-      L  = th_lane_begin(a, c0.b + stride, acc, lane);
+      L  = th_lane_begin(a, pn, acc, lane);
+      const int nb = c0.b + 2 * stride;
+      pn           = th_pre_lanes(a, nb < end ? nb : c0.b + stride, lane);
     }
     c0 = c1;
     c1 = c2;
