@@ -133,7 +133,8 @@ class THInstance(C.Structure):
                 + [(n, _vp) for n in ("nu_e", "inc_off", "inc_elem", "inc_meta", "blk_row",
                                       "blk_flags", "row_ptr", "vals", "res", "lam", "bc_mask",
                                       "jt", "g", "blk_lanes")]
-                + [("vjp", PatchPlan)])
+                + [("vjp", PatchPlan)]
+                + [(n, _vp) for n in ("rt_off", "rt_elem", "rt_meta")])
 
 
 class GLSInstance(C.Structure):

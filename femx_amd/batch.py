@@ -94,9 +94,11 @@ GRID_CANDIDATES = (0, 2, 8)
 
 class THCase:
     """One Taylor-Hood instance: lid-driven cavity, Re, Fourier state,
-    adjoint, per-element viscosity sensitivities."""
+    adjoint, per-element viscosity sensitivities.  With `rounds` the batch's
+    assembly phase walks the instance's round tables (fx_th_assemble_rounds'
+    walks, four blocks per work item) instead of taking one block per item."""
 
-    def __init__(self, n, seed, Re, device, items=ns.TH_ITEMS, layouts=TH_LAYOUTS):
+    def __init__(self, n, seed, Re, device, items=ns.TH_ITEMS, layouts=TH_LAYOUTS, rounds=True):
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
         self.geom = ns.structured_th(n, device)
         g = self.geom
@@ -116,6 +118,7 @@ class THCase:
         self.jt = torch.empty(g.ndof, dtype=torch.float64, device=device)
         self.g = torch.empty(g.num_elems, dtype=torch.float64, device=device)
         self.elements = g.num_elems
+        self.rounds = rounds  # the assembly phase over the round tables
 
     def descriptor(self):
         g, p = self.geom, self.plan
@@ -126,7 +129,9 @@ class THCase:
             ptr(g.vconn3), ptr(g.dofs15), ptr(self.state), self.nu, None, ptr(p.inc_off),
             ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row), ptr(p.blk_flags), ptr(rp),
             ptr(self.jac.vals), ptr(self.res), ptr(self.lam), ptr(self.bmap.d_mask),
-            ptr(self.jt), ptr(self.g), ptr(p.blk_lanes), self.patches.plan)
+            ptr(self.jt), ptr(self.g), ptr(p.blk_lanes), self.patches.plan,
+            ptr(p.rt_off) if self.rounds else None, ptr(p.rt_elem) if self.rounds else None,
+            ptr(p.rt_meta) if self.rounds else None)
 
     def plain(self, what):
         op = ns.TaylorHoodNS(self.nu)

@@ -41,6 +41,7 @@ struct fx_batch
   std::size_t p1_vjp_lds   = 0;
   int         th_tile = 0; // row tile per wave of the Taylor-Hood assembly (doubles)
   bool        p1_long = false; // some P1 row is longer than the LDS tile width
+  bool        th_rounds = false; // every Taylor-Hood instance has round tables
   std::size_t gls_lds = 0, gls_vjp_lds = 0;
   int         gls_max_ldofs = 0;
   // device memory
@@ -124,6 +125,8 @@ enum Phase
   kGlsVjpSeam = 8
 };
 constexpr int kPhases = 9, kChains = 6;
+// blocks per Taylor-Hood assembly work item when an instance has round tables
+constexpr int kThRtBatchWalk = 4;
 
 // capped at 96 VGPRs (5 waves/SIMD, as k_p1_assemble reaches uncapped)
 template <bool kLong>
@@ -158,6 +161,10 @@ __global__ __launch_bounds__(kPatchLanes) void k_batch_p1_vjp_patch(const P1VjpA
 // tables.
 // At most 256 VGPRs (2 waves per SIMD): a register count past 256 would
 // halve the occupancy, which costs far more than a few spills.
+// Rounds: every work item is a walk over an instance's round tables
+// (fx_th_assemble_rounds' walk), else one block (fx_th_assemble's); two
+// kernels, so that neither path's registers weigh on the other.
+template <bool Rounds>
 __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_batch_th_assemble(
     const ThAsmArgs* __restrict__ inst, const int2* __restrict__ work, int nwork, int tile)
 {
@@ -171,8 +178,12 @@ __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per
   const fx::XcdWalk wk = fx::xcd_walk(nwork, kThWaves, w);
   for (int i = wk.first; i < wk.end; i += wk.stride)
   {
-    const int2 it = work[i];
-    th_assemble_block(it.y, inst[it.x], T, smem + kThTensorLen + w * tile);
+    const int2       it = work[i];
+    const ThAsmArgs& a  = inst[it.x];
+    if (Rounds)
+      th_assemble_walk_rt(a, T, smem + kThTensorLen + w * tile, it.y, a.rt_blocks, a.rt_stride);
+    else
+      th_assemble_block(it.y, a, T, smem + kThTensorLen + w * tile);
   }
 }
 
@@ -270,8 +281,13 @@ void launch_chain(fx_batch* b, int chain, hipStream_t s)
                          b->d_p1_asm, b->d_work[kP1Asm], b->nwork[kP1Asm]);
     break;
   case 1:
-    hipLaunchKernelGGL(k_batch_th_assemble, g(kThAsm), dim3(kThLanes * kThWaves), b->th_lds, s,
-                       b->d_th_asm, b->d_work[kThAsm], b->nwork[kThAsm], b->th_tile);
+    if (b->th_rounds)
+      hipLaunchKernelGGL(k_batch_th_assemble<true>, g(kThAsm), dim3(kThLanes * kThWaves), b->th_lds,
+                         s, b->d_th_asm, b->d_work[kThAsm], b->nwork[kThAsm], b->th_tile);
+    else
+      hipLaunchKernelGGL(k_batch_th_assemble<false>, g(kThAsm), dim3(kThLanes * kThWaves),
+                         b->th_lds, s, b->d_th_asm, b->d_work[kThAsm], b->nwork[kThAsm],
+                         b->th_tile);
     break;
   case 2:
     hipLaunchKernelGGL(k_batch_p1_vjp_patch, g(kP1Vjp), dim3(kPatchLanes), b->p1_vjp_lds, s,
@@ -474,6 +490,9 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
     const int acc_cap = kP1Block * max_row_len;
     for (auto& a : pa) a.acc_cap = acc_cap;
     b->p1_lds = static_cast<std::size_t>(acc_cap) * 8 + kIncStage * 8;
+    b->th_rounds = num_th > 0;
+    for (int i = 0; i < num_th; ++i)
+      b->th_rounds = b->th_rounds && th[i].rt_off && th[i].rt_elem && th[i].rt_meta && th[i].blk_lanes;
     for (int i = 0; i < num_th; ++i)
     {
       const fx_th_instance& q = th[i];
@@ -486,10 +505,19 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
                       && q.inc_meta && q.blk_row && q.blk_flags && q.row_ptr && q.vals && q.res,
                   FX_ERR_ARG, "fx_batch_create: Taylor-Hood instance lacks an array");
       max_blk_nnz = std::max(max_blk_nnz, q.max_block_nnz);
+      // with round tables (on every instance) a work item is a walk over
+      // kThRtBatchWalk blocks of the instance (k, k + M, ... < num_blocks, M
+      // the number of walks), so that walks next to each other take
+      // neighbouring blocks
+      const bool rounds = b->th_rounds;
+      const int  walks  = rounds ? (q.num_blocks + kThRtBatchWalk - 1) / kThRtBatchWalk : 0;
       ta.push_back({q.xv, q.yv, q.vconn3, q.dofs15, q.state, q.nu, q.nu_e, q.inc_off, q.inc_elem,
                     reinterpret_cast<const uint4*>(q.inc_meta), q.blk_row, q.blk_flags, q.blk_lanes,
-                    q.row_ptr, q.vals, q.res});
-      for (int k = 0; k < q.num_blocks; ++k) w[kThAsm].push_back(make_int2(i, k));
+                    q.row_ptr, q.vals, q.res, rounds ? q.rt_off : nullptr,
+                    rounds ? q.rt_elem : nullptr,
+                    rounds ? reinterpret_cast<const uint4*>(q.rt_meta) : nullptr,
+                    static_cast<int>(q.num_blocks), walks});
+      for (int k = 0; k < (rounds ? walks : q.num_blocks); ++k) w[kThAsm].push_back(make_int2(i, k));
       const fx_patch_plan& pv = q.vjp;
       const bool vjp = q.lam && q.jt && pv.num_patches > 0 && pv.patch_off && pv.patch_elem
                        && pv.elem_vert && pv.elem_ldof && pv.ldof_off && pv.ldof_dof && pv.ldof_part && pv.ldof_end
@@ -596,7 +624,9 @@ int fx_batch_create(fx_index num_p1, const fx_p1_instance* p1, fx_index num_th,
             b->p1_long ? reinterpret_cast<const void*>(k_batch_p1_assemble<true>)
                        : reinterpret_cast<const void*>(k_batch_p1_assemble<false>),
             kP1Block, b->p1_lds);
-    b->pgrid[kThAsm] = occ(kThAsm, reinterpret_cast<const void*>(k_batch_th_assemble),
+    b->pgrid[kThAsm] = occ(kThAsm,
+                           b->th_rounds ? reinterpret_cast<const void*>(k_batch_th_assemble<true>)
+                                        : reinterpret_cast<const void*>(k_batch_th_assemble<false>),
                            kThLanes * kThWaves, b->th_lds);
     b->p1_vjp_lds    = p1_patch_lds(b->p1_max_ldofs);
     b->pgrid[kP1Vjp] = occ(kP1Vjp, reinterpret_cast<const void*>(k_batch_p1_vjp_patch), kPatchLanes,

@@ -211,7 +211,7 @@ int fx_th_assemble_rounds(fx_index ndof, fx_index num_elems, fx_index num_blocks
     const ThAsmArgs a{xv,       yv,        vconn3,    dofs15,  state,   nu,
                       nu_e,     nullptr,   nullptr,   nullptr, blk_row, blk_flags,
                       blk_lanes, row_ptr,  vals,      res,     rt_off,  rt_elem,
-                      reinterpret_cast<const uint4*>(rt_meta)};
+                      reinterpret_cast<const uint4*>(rt_meta), static_cast<int>(num_blocks), 1};
     // each wave walks about kThRtBlocksPerWave blocks: long enough for the
     // pipeline to run on across block boundaries, short enough that the
     // hardware still balances the walks (a multiple of 8 workgroups, for

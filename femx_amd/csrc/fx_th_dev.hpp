@@ -469,10 +469,12 @@ struct ThAsmArgs
   const int*    row_ptr;
   double*       vals;
   double*       res;
-  // round tables (fx_map_th_rounds), or null
+  // round tables (fx_map_th_rounds), or null; a batch's walks over them
+  // take blocks first, first + rt_stride, ... < rt_blocks
   const int*    rt_off;
   const int*    rt_elem;
   const uint4*  rt_meta;
+  int           rt_blocks, rt_stride;
 };
 
 __device__ __forceinline__ ThRaw th_load_raw(const ThAsmArgs& a, const ThDofs& D, int e)

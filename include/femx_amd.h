@@ -736,6 +736,11 @@ typedef struct
   double*       g;
   const fx_index* blk_lanes; /* fx_map_th_lanes, or NULL for two lanes per item */
   fx_patch_plan vjp; /* the VJP's patch plan (fx_th_vjp_patch) */
+  /* fx_map_th_rounds' tables (with blk_lanes), or NULL: the assembly phase
+   * then runs fx_th_assemble_rounds' walks instead of fx_th_assemble's blocks */
+  const fx_index* rt_off;
+  const fx_index* rt_elem;
+  const fx_index* rt_meta;
 } fx_th_instance;
 
 /* One GLS time step (fx_gls_assemble's arguments; vals NULL assembles the

@@ -25,10 +25,16 @@ def same(a, b):
     return all(np.array_equal(x, y, equal_nan=False) for xa, ya in zip(a, b) for x, y in zip(xa, ya))
 
 
-@pytest.mark.parametrize("grid_per_cu", [1, 3])
-def test_batch_matches_plain_bitwise(grid_per_cu):
+@pytest.mark.parametrize("grid_per_cu,rounds", [(1, True), (3, True), (1, False)])
+def test_batch_matches_plain_bitwise(grid_per_cu, rounds):
+    """The batch's outputs equal the plain calls' bit for bit, with its
+    Taylor-Hood assembly phase walking the round tables or taking one block
+    per work item."""
     specs = B.instance_specs(count=10, seed=9, n_min=3, n_max=40)
     cases = B.build_cases(specs, "cuda")
+    for c in cases:
+        if isinstance(c, B.THCase):
+            c.rounds = rounds
     B.plain(cases, 3)
     ref = snapshot(cases)
     assert all(np.isfinite(x).all() for xa in ref for x in xa)
