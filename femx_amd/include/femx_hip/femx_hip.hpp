@@ -414,6 +414,7 @@ struct DeviceAssemblyMap
   std::shared_ptr<const DeviceCsrPattern> pattern;
   DeviceIndexVector inc_off, inc, inc_elem, inc_meta, blk_row, blk_flags, blk_lanes;
   DeviceIndexVector elem_inc; // 15x15 maps: (element, local row) -> row incidence (the VJP)
+  DeviceIndexVector rt_off, rt_elem, rt_meta; // 15x15 maps: round tables (fx_map_th_rounds)
   Index             num_blocks = 0, max_block_nnz = 0;
   // 3x3 maps: the 64-row stencil blocks of the P1 assembly (fx_p1_stencil_plan)
   DeviceIndexVector p1_blk_code, p1_words;
@@ -466,6 +467,15 @@ inline void copy(const HostAssemblyMap& src, DeviceAssemblyMap& dst, HipContext&
                            bf.data()));
     std::vector<Index> bl(64 * static_cast<std::size_t>(nb));
     check(fx_map_th_lanes(src.handle(), nb, br.data(), bf.data(), bl.data()));
+    std::vector<Index> ro(nb + 1);
+    check(fx_map_th_rounds(nb, br.data(), bf.data(), bl.data(), off.data(), el.data(), me.data(),
+                           ro.data(), nullptr, nullptr));
+    std::vector<Index> re(std::max<Index>(ro[nb], 1)), rm(4 * re.size());
+    check(fx_map_th_rounds(nb, br.data(), bf.data(), bl.data(), off.data(), el.data(), me.data(),
+                           ro.data(), re.data(), rm.data()));
+    dst.rt_off.assign(ro, ctx);
+    dst.rt_elem.assign(re, ctx);
+    dst.rt_meta.assign(rm, ctx);
     dst.inc_off.assign(off, ctx);
     dst.inc_elem.assign(el, ctx);
     dst.inc_meta.assign(me, ctx);
@@ -660,12 +670,15 @@ inline void assemble(const TaylorHoodNS& op, const DeviceTHGeometry& geom, const
 {
   checkAssembly(map, &state, res, jac);
   require(geom.num_elems == map.info.num_elems, "Geometry and AssemblyMap have different element counts");
-  check(fx_th_assemble(geom.ndof, geom.num_elems, map.num_blocks, map.max_block_nnz, geom.xv.data(),
-                       geom.yv.data(), geom.vconn3.data(), geom.dofs15.data(), state.data(), op.nu,
-                       op.nu_e, map.inc_off.data(), map.inc_elem.data(), map.inc_meta.data(),
-                       map.blk_row.data(), map.blk_flags.data(), map.blk_lanes.data(),
-                       map.pattern->rowPtrData(),
-                       jac ? jac->valsData() : nullptr, res.data(), ctx.raw()));
+  // over the blocks' round tables (fx_th_assemble_rounds; bitwise equal to
+  // fx_th_assemble over the lane map)
+  check(fx_th_assemble_rounds(geom.ndof, geom.num_elems, map.num_blocks, map.max_block_nnz,
+                              geom.xv.data(), geom.yv.data(), geom.vconn3.data(),
+                              geom.dofs15.data(), state.data(), op.nu, op.nu_e,
+                              map.blk_row.data(), map.blk_flags.data(), map.blk_lanes.data(),
+                              map.rt_off.data(), map.rt_elem.data(), map.rt_meta.data(),
+                              map.pattern->rowPtrData(), jac ? jac->valsData() : nullptr,
+                              res.data(), ctx.raw()));
 }
 
 /** Taylor-Hood adjoint VJP (fx_th_vjp): jt = J^T adj over the row-replaced

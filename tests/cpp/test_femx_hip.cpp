@@ -364,10 +364,10 @@ int main(int argc, char** argv)
     expect(r1.toHost(ctx) == r2.toHost(ctx), "bitwise equal residuals");
   });
 
-  run("taylorHoodAssemblyIsDeterministic", [] {
+  run("taylorHoodAssemblyIsDeterministicAndEqualsLaneMapKernel", [] {
     needDevice();
     HipContext ctx;
-    const int  n = 3, E = 2 * n * n, Nv = (2 * n + 1) * (2 * n + 1), Np = (n + 1) * (n + 1);
+    const int  n = 9, E = 2 * n * n, Nv = (2 * n + 1) * (2 * n + 1), Np = (n + 1) * (n + 1);
     RV         x2(Nv), y2(Nv);
     IV         c6(6 * E), v3(3 * E), d15(15 * E);
     check(fx_mesh_p2(n, x2.data(), y2.data(), c6.data(), v3.data()));
@@ -400,6 +400,18 @@ int main(int argc, char** argv)
     const RV a = m1.vals().toHost(ctx), b = m2.vals().toHost(ctx);
     expect(a == b, "bitwise equal Jacobians");
     expect(r1.toHost(ctx) == r2.toHost(ctx) && r1.toHost(ctx) == r3.toHost(ctx), "bitwise equal residuals");
+    // assembly::assemble runs over the round tables; fx_th_assemble over the
+    // lane map gives the same bits
+    DeviceCsrMatrix m4(map.pattern);
+    DeviceVector    r4(ndof);
+    check(fx_th_assemble(geom.ndof, geom.num_elems, map.num_blocks, map.max_block_nnz,
+                         geom.xv.data(), geom.yv.data(), geom.vconn3.data(), geom.dofs15.data(),
+                         st.data(), op.nu, op.nu_e, map.inc_off.data(), map.inc_elem.data(),
+                         map.inc_meta.data(), map.blk_row.data(), map.blk_flags.data(),
+                         map.blk_lanes.data(), map.pattern->rowPtrData(), m4.valsData(), r4.data(),
+                         ctx.raw()));
+    expect(m4.vals().toHost(ctx) == a && r4.toHost(ctx) == r1.toHost(ctx),
+           "round tables bitwise equal to the lane-map kernel");
   });
 
   // tests/unit/model/ns/runNavierOperatorTests.cpp:223-290 (row assembly
