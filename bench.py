@@ -158,6 +158,7 @@ def build_th(n, dev, t):
     nu_e = torch.full((g.num_elems,), 0.01, dtype=torch.float64, device=dev)
     op = ns.TaylorHoodNS(0.01)
     return {"assemble": lambda: ns.assemble_ns(op, g, amap, s, res, jac),
+            "assemble_lane_map": lambda: ns.assemble_ns(op, g, amap, s, res, jac, rounds=False),
             "vjp": lambda: R.vjp(s, nu_e, lam, jt, ge),
             "E": g.num_elems, "ndof": g.ndof, "nnz": amap.pattern.nnz,
             "alg_bytes": th_alg_bytes(g.num_vertices, g.num_elems, g.ndof, amap.pattern.nnz),
@@ -353,6 +354,8 @@ def main():
         if not args.no_th:
             th = build_th(args.th_n, dev, t)
             for _ in range(5):
+                th["assemble_lane_map"]()
+            for _ in range(5):
                 th["assemble"]()
             for _ in range(5):
                 th["vjp"]()
@@ -479,13 +482,16 @@ def main():
         log(f"[bench] Taylor-Hood {args.th_n}x{args.th_n}")
         th = build_th(args.th_n, dev, t)
         t_th = time_kernel(th["assemble"], reps)
+        t_th_lm = time_kernel(th["assemble_lane_map"], reps)
         t_thv = time_kernel(th["vjp"], reps)
         b_th, b_thv = th["alg_bytes"], th["vjp_alg_bytes"]
         extra["th_ns_%d" % args.th_n] = {
             "elements": th["E"], "ndof": th["ndof"], "nnz": th["nnz"],
             "assemble_us": t_th * 1e6, "elements_per_s": th["E"] / t_th, "alg_bytes": b_th,
-            "traffic": profiled_traffic("k_th_assemble") if args.th_n == 512 else None,
+            "traffic": profiled_traffic("k_th_assemble_rt") if args.th_n == 512 else None,
             "GBs": b_th / t_th / 1e9, "frac": b_th / t_th / 1e9 / HBM_PEAK_GBS,
+            "form": "per-block round tables, four-block walks (fx_th_assemble_rounds)",
+            "lane_map_us": t_th_lm * 1e6, "lane_map_frac": b_th / t_th_lm / 1e9 / HBM_PEAK_GBS,
             "vjp_us": t_thv * 1e6, "VJPs_per_s": 1.0 / t_thv, "vjp_alg_bytes": b_thv,
             "vjp_GBs": b_thv / t_thv / 1e9, "vjp_frac": b_thv / t_thv / 1e9 / HBM_PEAK_GBS}
 

@@ -29,7 +29,9 @@ __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per
 // The same over the blocks' round tables: each wave walks a few blocks of its
 // XCD's share (xcd_walk), its pipeline running on across block boundaries.
 // Blocks per wave at 512^2 (A/B against fx_th_assemble's 290 us): all the
-// device holds at once (~20) 292 us, 1: 284, 2: 264, 4: 263, 8: 278.
+// device holds at once (~20) 292 us, 1: 284, 2: 264, 4: 263, 8: 278; at
+// 256^2 (about 5 blocks per resident wave in all) 4 gained nothing (88 vs
+// 87 us): the last walks' tail is as long as the rest.
 constexpr int kThRtBlocksPerWave = 4;
 
 __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_th_assemble_rt(ThAsmArgs a, int num_blocks,
@@ -212,12 +214,24 @@ int fx_th_assemble_rounds(fx_index ndof, fx_index num_elems, fx_index num_blocks
                       nu_e,     nullptr,   nullptr,   nullptr, blk_row, blk_flags,
                       blk_lanes, row_ptr,  vals,      res,     rt_off,  rt_elem,
                       reinterpret_cast<const uint4*>(rt_meta), static_cast<int>(num_blocks), 1};
-    // each wave walks about kThRtBlocksPerWave blocks: long enough for the
+    // each wave walks up to kThRtBlocksPerWave blocks: long enough for the
     // pipeline to run on across block boundaries, short enough that the
-    // hardware still balances the walks (a multiple of 8 workgroups, for
-    // xcd_walk)
-    const std::int64_t need = (static_cast<std::int64_t>(num_blocks) + kThWaves - 1) / kThWaves;
-    const int grid = static_cast<int>(((need + kThRtBlocksPerWave - 1) / kThRtBlocksPerWave + 7) / 8 * 8);
+    // hardware still balances the walks: at most a fifth of what each
+    // resident wave takes in all (the resident waves: the CUs times the
+    // workgroups that fit, at most two by the kernel's 2 waves/SIMD, by LDS),
+    // so that the last walks' tail stays short.  A multiple of 8 workgroups,
+    // for xcd_walk.
+    int dev = 0, cus = 0;
+    fx::check_hip(hipGetDevice(&dev), "hipGetDevice");
+    fx::check_hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev),
+                  "hipDeviceGetAttribute");
+    const std::int64_t resident =
+        static_cast<std::int64_t>(std::max(cus, 1)) * kThWaves
+        * std::max<std::int64_t>(1, std::min<std::int64_t>(2, (160 * 1024) / static_cast<std::int64_t>(lds)));
+    const std::int64_t bpw = std::max<std::int64_t>(
+        1, std::min<std::int64_t>(kThRtBlocksPerWave, num_blocks / (5 * std::max<std::int64_t>(resident, 1))));
+    const std::int64_t need = (static_cast<std::int64_t>(num_blocks) + kThWaves * bpw - 1) / (kThWaves * bpw);
+    const int grid = static_cast<int>((need + 7) / 8 * 8);
     hipLaunchKernelGGL(k_th_assemble_rt, dim3(grid), dim3(kThLanes * kThWaves), lds,
                        fx::as_stream(stream), a, num_blocks, tile);
     fx::check_launch("fx_th_assemble_rounds");
