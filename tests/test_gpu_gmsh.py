@@ -204,3 +204,12 @@ def test_taylor_hood_fan_and_tile_bound():
               ptr(p.inc_off), ptr(p.inc_elem), ptr(p.inc_meta), ptr(p.blk_row), ptr(p.blk_flags),
               ptr(p.blk_lanes), ptr(rp), ptr(jac.vals), ptr(res), stream_handle())
     assert np.array_equal(host(jac.vals), v) and np.array_equal(host(res), r)
+    # the round tables of the same small-block layout (rows of up to 213 entries, vertices of
+    # 30 elements: rounds with many idle lanes)
+    jac2 = CsrMatrix(amap.pattern, DEV)
+    res2 = torch.empty(g.ndof, dtype=torch.float64, device=DEV)
+    _lib.call("fx_th_assemble_rounds", g.ndof, g.num_elems, p.num_blocks, p.max_block_nnz,
+              ptr(g.xv), ptr(g.yv), ptr(g.vconn3), ptr(g.dofs15), ptr(d_s), 0.0, ptr(d_nu),
+              ptr(p.blk_row), ptr(p.blk_flags), ptr(p.blk_lanes), ptr(p.rt_off), ptr(p.rt_elem),
+              ptr(p.rt_meta), ptr(rp), ptr(jac2.vals), ptr(res2), stream_handle())
+    assert np.array_equal(host(jac2.vals), v) and np.array_equal(host(res2), r)
