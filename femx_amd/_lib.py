@@ -32,7 +32,7 @@ _SIGS = {
     "fx_map_export_rowinc15": ([_vp, _vp, _vp, _vp], C.c_int),
     "fx_map_th_blocks": ([_vp, _i32, _i32, _vp, _vp, _vp, _vp], C.c_int),
     "fx_map_th_lanes": ([_vp, _i32, _vp, _vp, _vp], C.c_int),
-    "fx_map_th_rounds": ([_i32] + [_vp] * 9, C.c_int),
+    "fx_map_th_rounds": ([_i32, _i32] + [_vp] * 9, C.c_int),
     "fx_map_gls_plan": ([_vp, _i32] + [_vp] * 9, C.c_int),
     "fx_map_destroy": ([_vp], None),
     "fx_csr_transpose_map": ([_i32, _i32, _vp, _vp, _vp, _vp, _vp], C.c_int),

@@ -572,14 +572,21 @@ int fx_map_th_lanes(const fx_map* m, Index num_blocks, const Index* blk_row, con
 // whose rows have no incidence gets one such round (filled with incidence 0)
 // when the mesh has any incidence, so that a kernel walking several blocks
 // always finds entries; in a mesh without incidences every R_b is 0.
-int fx_map_th_rounds(Index num_blocks, const Index* blk_row, const Index* blk_flags,
-                     const Index* blk_lanes, const Index* inc_off, const Index* inc_elem,
-                     const Index* inc_meta, Index* rt_off, Index* rt_elem, Index* rt_meta)
+int fx_map_th_rounds(Index num_rows, Index num_blocks, const Index* blk_row,
+                     const Index* blk_flags, const Index* blk_lanes, const Index* inc_off,
+                     const Index* inc_elem, const Index* inc_meta, Index* rt_off, Index* rt_elem,
+                     Index* rt_meta)
 {
   return fx::guard([&] {
-    fx::require(num_blocks >= 0 && rt_off
-                    && (num_blocks == 0 || (blk_row && blk_flags && blk_lanes && inc_off)),
+    fx::require(num_rows >= 0 && num_blocks >= 0 && rt_off
+                    && (num_blocks == 0 || (blk_row && blk_flags && blk_lanes && inc_off))
+                    && (!rt_elem || (rt_meta && inc_elem && inc_meta)),
                 FX_ERR_ARG, "fx_map_th_rounds: invalid arguments");
+    fx::require(num_blocks == 0 || (blk_row[0] >= 0 && blk_row[num_blocks] <= num_rows), FX_ERR_ARG,
+                "fx_map_th_rounds: blk_row is not a block partition of the rows");
+    for (Index r = 0; r < num_rows; ++r)
+      fx::require(inc_off[r] >= 0 && inc_off[r + 1] >= inc_off[r], FX_ERR_ARG,
+                  "fx_map_th_rounds: inc_off is not a row offset array");
     constexpr int      L = 64;
     std::vector<Index> first(L), count(L), stride(L);
     std::int64_t       total = 0;
@@ -588,7 +595,7 @@ int fx_map_th_rounds(Index num_blocks, const Index* blk_row, const Index* blk_fl
     for (Index b = 0; b < num_blocks; ++b)
     {
       const Index rs = blk_row[b], re = blk_row[b + 1], step = blk_flags[b] ? 2 : 1;
-      fx::require(re > rs && (re - rs) % step == 0, FX_ERR_ARG,
+      fx::require(re > rs && re <= num_rows && (re - rs) % step == 0, FX_ERR_ARG,
                   "fx_map_th_rounds: blk_row is not a block partition of the rows");
       const Index  nitems = (re - rs) / step;
       const Index* lm     = blk_lanes + static_cast<std::int64_t>(L) * b;

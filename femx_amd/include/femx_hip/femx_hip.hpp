@@ -468,10 +468,10 @@ inline void copy(const HostAssemblyMap& src, DeviceAssemblyMap& dst, HipContext&
     std::vector<Index> bl(64 * static_cast<std::size_t>(nb));
     check(fx_map_th_lanes(src.handle(), nb, br.data(), bf.data(), bl.data()));
     std::vector<Index> ro(nb + 1);
-    check(fx_map_th_rounds(nb, br.data(), bf.data(), bl.data(), off.data(), el.data(), me.data(),
+    check(fx_map_th_rounds(src.numRes(), nb, br.data(), bf.data(), bl.data(), off.data(), el.data(), me.data(),
                            ro.data(), nullptr, nullptr));
     std::vector<Index> re(std::max<Index>(ro[nb], 1)), rm(4 * re.size());
-    check(fx_map_th_rounds(nb, br.data(), bf.data(), bl.data(), off.data(), el.data(), me.data(),
+    check(fx_map_th_rounds(src.numRes(), nb, br.data(), bf.data(), bl.data(), off.data(), el.data(), me.data(),
                            ro.data(), re.data(), rm.data()));
     dst.rt_off.assign(ro, ctx);
     dst.rt_elem.assign(re, ctx);

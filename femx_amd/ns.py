@@ -119,8 +119,8 @@ class THBlocks:
         if rowinc is not None:
             off, elem, meta = rowinc
             rt_off = np.empty(nb.value + 1, np.int32)
-            args = (nb.value, ptr(blk_row), ptr(self.h_blk_flags), ptr(self.h_blk_lanes), ptr(off),
-                    ptr(elem), ptr(meta))
+            args = (amap.num_res, nb.value, ptr(blk_row), ptr(self.h_blk_flags),
+                    ptr(self.h_blk_lanes), ptr(off), ptr(elem), ptr(meta))
             _lib.call("fx_map_th_rounds", *args, ptr(rt_off), None, None)
             n = int(rt_off[-1])
             rt_elem = np.empty(max(n, 1), np.int32)

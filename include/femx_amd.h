@@ -156,12 +156,15 @@ int fx_map_th_lanes(const fx_map* map, fx_index num_blocks, const fx_index* h_bl
  * and h_rt_meta (four words per entry, as fx_map_export_rowinc15 writes
  * them, from its h_inc_off, h_inc_elem and h_inc_meta).  An entry past the
  * lane's last incidence, or on an idle lane, repeats a valid incidence of
- * the block with bit 31 of its last meta word set.  h_rt_off[num_blocks+1];
- * pass null tables to get h_rt_off (the table length is its last entry). */
-int fx_map_th_rounds(fx_index num_blocks, const fx_index* h_blk_row, const fx_index* h_blk_flags,
-                     const fx_index* h_blk_lanes, const fx_index* h_inc_off,
-                     const fx_index* h_inc_elem, const fx_index* h_inc_meta, fx_index* h_rt_off,
-                     fx_index* h_rt_elem, fx_index* h_rt_meta);
+ * the block with bit 31 of its last meta word set.  num_rows is the map's
+ * residual row count (h_inc_off has num_rows + 1 entries).
+ * h_rt_off[num_blocks+1]; pass null tables to get h_rt_off (the table length
+ * is its last entry). */
+int fx_map_th_rounds(fx_index num_rows, fx_index num_blocks, const fx_index* h_blk_row,
+                     const fx_index* h_blk_flags, const fx_index* h_blk_lanes,
+                     const fx_index* h_inc_off, const fx_index* h_inc_elem,
+                     const fx_index* h_inc_meta, fx_index* h_rt_off, fx_index* h_rt_elem,
+                     fx_index* h_rt_meta);
 /* Node plan of the GLS P1/P1 Navier-Stokes kernel (fx_gls_assemble) over a
  * map whose elements have 9 local DOFs in NavierOperator order -- u0 v0 u1
  * v1 u2 v2 p0 p1 p2 (femx/model/ns/Components.hpp:247-256) -- for the mesh

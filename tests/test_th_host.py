@@ -192,9 +192,19 @@ def test_th_round_tables(n, items, tile):
             assert np.all(seen[off[r]:off[r + 1]] == 1)
     bad = np.array([0, 5, 3], np.int32)
     with pytest.raises(RuntimeError, match="block partition"):
-        _lib.call("fx_map_th_rounds", 2, ptr(bad), ptr(np.zeros(2, np.int32)),
+        _lib.call("fx_map_th_rounds", off.size - 1, 2, ptr(bad), ptr(np.zeros(2, np.int32)),
                   ptr(np.full(128, -1, np.int32)), ptr(off), ptr(elem), ptr(plan.inc_meta.numpy()),
                   ptr(np.empty(3, np.int32)), None, None)
+    past = np.array([0, off.size], np.int32)  # a block beyond the last row
+    with pytest.raises(RuntimeError, match="block partition"):
+        _lib.call("fx_map_th_rounds", off.size - 1, 1, ptr(past), ptr(np.zeros(1, np.int32)),
+                  ptr(np.full(64, -1, np.int32)), ptr(off), ptr(elem), ptr(plan.inc_meta.numpy()),
+                  ptr(np.empty(2, np.int32)), None, None)
+    with pytest.raises(RuntimeError, match="invalid arguments"):
+        _lib.call("fx_map_th_rounds", off.size - 1, plan.num_blocks, ptr(br), ptr(bf),
+                  ptr(plan.h_blk_lanes), ptr(off), None, None, ptr(np.empty(plan.num_blocks + 1,
+                                                                         np.int32)),
+                  ptr(np.empty(64, np.int32)), ptr(np.empty(256, np.int32)))
 
 
 @pytest.mark.parametrize("n", [49, 499])
