@@ -181,7 +181,8 @@ __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per
     const int2       it = work[i];
     const ThAsmArgs& a  = inst[it.x];
     if (Rounds)
-      th_assemble_walk_rt(a, T, smem + kThTensorLen + w * tile, it.y, a.rt_blocks, a.rt_stride);
+      th_assemble_walk_rt(a, T, smem + kThTensorLen + w * tile, tile, it.y, a.rt_blocks,
+                          a.rt_stride);
     else
       th_assemble_block(it.y, a, T, smem + kThTensorLen + w * tile);
   }

@@ -685,7 +685,7 @@ struct ThLane
 // of the block before it, th_pre_rows after that block's first round).
 struct ThPre
 {
-  int rs, re, pairs, lm, rx, ry;
+  int rs, re, pairs, lm, rx, ry, base, bend;
 };
 
 __device__ __forceinline__ ThPre th_pre_lanes(const ThAsmArgs& a, int b, int lane)
@@ -695,6 +695,8 @@ __device__ __forceinline__ ThPre th_pre_lanes(const ThAsmArgs& a, int b, int lan
   p.re    = a.blk_row[b + 1];
   p.pairs = a.blk_flags[b] != 0;
   p.lm    = a.blk_lanes[kThLanes * b + lane];
+  p.base  = a.row_ptr[p.rs];
+  p.bend  = a.row_ptr[p.re];
   p.rx = p.ry = 0;
   return p;
 }
@@ -714,14 +716,15 @@ __device__ __forceinline__ void th_pre_rows(const ThAsmArgs& a, ThPre& p)
   p.ry        = p.pairs ? a.row_ptr[r + 1] : 0;
 }
 
-// Set up a block from its preloaded part: the lane's item and the zeroed tile.
+// Set up a block from its preloaded part: the lane's item, and (unless the
+// tile is known to be zero already) the zeroed tile.
 __device__ __forceinline__ ThLane th_lane_begin(const ThAsmArgs& a, const ThPre& p, double* acc,
-                                                int lane)
+                                                int lane, bool zero = true)
 {
   ThLane    L;
   L.pairs          = p.pairs != 0;
-  L.base           = a.row_ptr[p.rs];
-  L.bn             = a.vals ? a.row_ptr[p.re] - L.base : 0;
+  L.base           = p.base;
+  L.bn             = a.vals ? p.bend - L.base : 0;
   const int nitems = L.pairs ? (p.re - p.rs) >> 1 : p.re - p.rs;
   L.own            = p.lm >= 0 && (p.lm & 0xff) < nitems;
   L.role           = (p.lm >> 8) & 0xff;
@@ -733,8 +736,11 @@ __device__ __forceinline__ ThLane th_lane_begin(const ThAsmArgs& a, const ThPre&
   L.kmax = kmax;
   L.ax   = acc + (p.rx - L.base);
   L.ay   = L.pairs ? acc + (p.ry - L.base) : nullptr;
-  for (int i = lane; i < L.bn; i += kThLanes) acc[i] = 0.0;
-  wave_sync();
+  if (zero)
+  {
+    for (int i = lane; i < L.bn; i += kThLanes) acc[i] = 0.0;
+    wave_sync();
+  }
   return L;
 }
 
@@ -745,9 +751,10 @@ __device__ __forceinline__ ThLane th_lane_begin(const ThAsmArgs& a, int b, doubl
   return th_lane_begin(a, p, acc, lane);
 }
 
-// Finish a block: residuals of the lane's item, then the tile's write-back.
+// Finish a block: residuals of the lane's item, then the tile's write-back,
+// which leaves the tile's used range zero again.
 __device__ __forceinline__ void th_lane_end(const ThAsmArgs& a, const ThLane& L, double sx, double sy,
-                                            const double* acc, int lane)
+                                            double* acc, int lane)
 {
   if (L.own && L.role == 0)
   {
@@ -756,7 +763,10 @@ __device__ __forceinline__ void th_lane_end(const ThAsmArgs& a, const ThLane& L,
   }
   wave_sync();
   for (int i = lane; i < L.bn; i += kThLanes)
+  {
     __builtin_nontemporal_store(acc[i], a.vals + L.base + i);
+    acc[i] = 0.0;
+  }
   wave_sync();
 }
 
@@ -771,10 +781,15 @@ __device__ __forceinline__ void th_lane_end(const ThAsmArgs& a, const ThLane& L,
 // residual sums and write-back are th_assemble_block's, in the same order:
 // the results are bitwise equal to it.
 __device__ __forceinline__ void th_assemble_walk_rt(const ThAsmArgs& a, const ThTensor& T,
-                                                    double* acc, int first, int end, int stride)
+                                                    double* acc, int tile, int first, int end,
+                                                    int stride)
 {
   const int lane = threadIdx.x & (kThLanes - 1);
   if (first >= end) return;
+  // the whole tile zero once; every block's write-back leaves it zero again
+  if (a.vals)
+    for (int i = lane; i < tile; i += kThLanes) acc[i] = 0.0;
+  wave_sync();
   ThCur c0 = th_cur_at(a, first);
   if (c0.R == 0)
   {
@@ -787,7 +802,9 @@ __device__ __forceinline__ void th_assemble_walk_rt(const ThAsmArgs& a, const Th
   ThIdx  ic = th_cur_idx(a, c0, lane), in = th_cur_idx(a, c1, lane);
   ThRaw  ra = th_load_raw(a, th_load_dofs(a.dofs, a.vconn, ic.e), ic.e), rb;
   ThDofs dn = th_load_dofs(a.dofs, a.vconn, in.e);
-  ThLane L  = th_lane_begin(a, c0.b, acc, lane);
+  ThPre  p0 = th_pre_lanes(a, c0.b, lane);
+  th_pre_rows(a, p0);
+  ThLane L  = th_lane_begin(a, p0, acc, lane, false);
   // the next block's part, one block ahead
   const bool more = first + stride < end;
   ThPre      pn   = th_pre_lanes(a, more ? first + stride : first, lane);
@@ -833,7 +850,7 @@ __device__ __forceinline__ void th_assemble_walk_rt(const ThAsmArgs& a, const Th
         return;
       }
       sx = 0.0, sy = 0.0;
-      L  = th_lane_begin(a, pn, acc, lane);
+      L  = th_lane_begin(a, pn, acc, lane, false);
       const int nb = c0.b + 2 * stride;
       pn           = th_pre_lanes(a, nb < end ? nb : c0.b + stride, lane);
     }
