@@ -176,13 +176,14 @@ __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per
   // and the instance's arguments are loaded once per wave into scalar registers
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kThLanes);
   const fx::XcdWalk wk = fx::xcd_walk(nwork, kThWaves, w);
+  if (Rounds) th_tile_clear(smem + kThTensorLen + w * tile, tile);
   for (int i = wk.first; i < wk.end; i += wk.stride)
   {
     const int2       it = work[i];
     const ThAsmArgs& a  = inst[it.x];
     if (Rounds)
-      th_assemble_walk_rt(a, T, smem + kThTensorLen + w * tile, tile, it.y, a.rt_blocks,
-                          a.rt_stride);
+      th_assemble_walk_rt<false>(a, T, smem + kThTensorLen + w * tile, it.y, a.rt_blocks,
+                                 a.rt_stride);
     else
       th_assemble_block(it.y, a, T, smem + kThTensorLen + w * tile);
   }

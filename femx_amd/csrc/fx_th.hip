@@ -43,7 +43,8 @@ __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per
   __syncthreads();
   const int         w  = __builtin_amdgcn_readfirstlane(threadIdx.x / kThLanes);
   const fx::XcdWalk wk = fx::xcd_walk(num_blocks, kThWaves, w);
-  th_assemble_walk_rt(a, T, smem + kThTensorLen + w * tile, tile, wk.first, wk.end, wk.stride);
+  th_tile_clear(smem + kThTensorLen + w * tile, tile);
+  th_assemble_walk_rt(a, T, smem + kThTensorLen + w * tile, wk.first, wk.end, wk.stride);
 }
 
 static_assert(sizeof(ThTensor) + std::size_t(FX_TH_TILE_NNZ_MAX) * 8 * kThWaves <= 160 * 1024,

@@ -688,6 +688,10 @@ struct ThPre
   int rs, re, pairs, lm, rx, ry, base, bend;
 };
 
+// PreCsr: the block's CSR range is loaded here too (one block ahead in a
+// walk); the batch's walk kernel, short of scalar registers, loads it when
+// the block starts instead.
+template <bool PreCsr = true>
 __device__ __forceinline__ ThPre th_pre_lanes(const ThAsmArgs& a, int b, int lane)
 {
   ThPre p;
@@ -695,8 +699,8 @@ __device__ __forceinline__ ThPre th_pre_lanes(const ThAsmArgs& a, int b, int lan
   p.re    = a.blk_row[b + 1];
   p.pairs = a.blk_flags[b] != 0;
   p.lm    = a.blk_lanes[kThLanes * b + lane];
-  p.base  = a.row_ptr[p.rs];
-  p.bend  = a.row_ptr[p.re];
+  p.base  = PreCsr ? a.row_ptr[p.rs] : 0;
+  p.bend  = PreCsr ? a.row_ptr[p.re] : 0;
   p.rx = p.ry = 0;
   return p;
 }
@@ -718,13 +722,14 @@ __device__ __forceinline__ void th_pre_rows(const ThAsmArgs& a, ThPre& p)
 
 // Set up a block from its preloaded part: the lane's item, and (unless the
 // tile is known to be zero already) the zeroed tile.
+template <bool PreCsr = true>
 __device__ __forceinline__ ThLane th_lane_begin(const ThAsmArgs& a, const ThPre& p, double* acc,
                                                 int lane, bool zero = true)
 {
   ThLane    L;
   L.pairs          = p.pairs != 0;
-  L.base           = p.base;
-  L.bn             = a.vals ? p.bend - L.base : 0;
+  L.base           = PreCsr ? p.base : a.row_ptr[p.rs];
+  L.bn             = a.vals ? (PreCsr ? p.bend : a.row_ptr[p.re]) - L.base : 0;
   const int nitems = L.pairs ? (p.re - p.rs) >> 1 : p.re - p.rs;
   L.own            = p.lm >= 0 && (p.lm & 0xff) < nitems;
   L.role           = (p.lm >> 8) & 0xff;
@@ -780,16 +785,21 @@ __device__ __forceinline__ void th_lane_end(const ThAsmArgs& a, const ThLane& L,
 // block's rounds, adding only what its entries do not mark idle.  The adds,
 // residual sums and write-back are th_assemble_block's, in the same order:
 // the results are bitwise equal to it.
+// The wave's whole tile cleared (once per kernel: a walk takes a zero tile
+// and leaves it zero, every block's write-back clearing what it used).
+__device__ __forceinline__ void th_tile_clear(double* acc, int tile)
+{
+  for (int i = threadIdx.x & (kThLanes - 1); i < tile; i += kThLanes) acc[i] = 0.0;
+  wave_sync();
+}
+
+// The tile must be zero on entry (th_tile_clear); it is zero again on exit.
+template <bool PreCsr = true>
 __device__ __forceinline__ void th_assemble_walk_rt(const ThAsmArgs& a, const ThTensor& T,
-                                                    double* acc, int tile, int first, int end,
-                                                    int stride)
+                                                    double* acc, int first, int end, int stride)
 {
   const int lane = threadIdx.x & (kThLanes - 1);
   if (first >= end) return;
-  // the whole tile zero once; every block's write-back leaves it zero again
-  if (a.vals)
-    for (int i = lane; i < tile; i += kThLanes) acc[i] = 0.0;
-  wave_sync();
   ThCur c0 = th_cur_at(a, first);
   if (c0.R == 0)
   {
@@ -802,12 +812,12 @@ __device__ __forceinline__ void th_assemble_walk_rt(const ThAsmArgs& a, const Th
   ThIdx  ic = th_cur_idx(a, c0, lane), in = th_cur_idx(a, c1, lane);
   ThRaw  ra = th_load_raw(a, th_load_dofs(a.dofs, a.vconn, ic.e), ic.e), rb;
   ThDofs dn = th_load_dofs(a.dofs, a.vconn, in.e);
-  ThPre  p0 = th_pre_lanes(a, c0.b, lane);
+  ThPre  p0 = th_pre_lanes<PreCsr>(a, c0.b, lane);
   th_pre_rows(a, p0);
-  ThLane L  = th_lane_begin(a, p0, acc, lane, false);
+  ThLane L  = th_lane_begin<PreCsr>(a, p0, acc, lane, false);
   // the next block's part, one block ahead
   const bool more = first + stride < end;
-  ThPre      pn   = th_pre_lanes(a, more ? first + stride : first, lane);
+  ThPre      pn   = th_pre_lanes<PreCsr>(a, more ? first + stride : first, lane);
   double sx = 0.0, sy = 0.0;
   bool   done  = false;
   auto   round = [&](const ThRaw& cur, ThRaw& nxt) {
@@ -850,9 +860,9 @@ __device__ __forceinline__ void th_assemble_walk_rt(const ThAsmArgs& a, const Th
         return;
       }
       sx = 0.0, sy = 0.0;
-      L  = th_lane_begin(a, pn, acc, lane, false);
+      L  = th_lane_begin<PreCsr>(a, pn, acc, lane, false);
       const int nb = c0.b + 2 * stride;
-      pn           = th_pre_lanes(a, nb < end ? nb : c0.b + stride, lane);
+      pn           = th_pre_lanes<PreCsr>(a, nb < end ? nb : c0.b + stride, lane);
     }
     c0 = c1;
     c1 = c2;
