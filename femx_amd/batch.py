@@ -96,7 +96,7 @@ class THCase:
     """One Taylor-Hood instance: lid-driven cavity, Re, Fourier state,
     adjoint, per-element viscosity sensitivities.  With `rounds` the batch's
     assembly phase walks the instance's round tables (fx_th_assemble_rounds'
-    walks, four blocks per work item) instead of taking one block per item."""
+    walks, twelve blocks per work item) instead of taking one block per item."""
 
     def __init__(self, n, seed, Re, device, items=ns.TH_ITEMS, layouts=TH_LAYOUTS, rounds=True):
         t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731

@@ -125,8 +125,11 @@ enum Phase
   kGlsVjpSeam = 8
 };
 constexpr int kPhases = 9, kChains = 6;
-// blocks per Taylor-Hood assembly work item when an instance has round tables
-constexpr int kThRtBatchWalk = 4;
+// blocks per Taylor-Hood assembly work item when an instance has round
+// tables (256-instance assembly graph: 2: 5.82 ms, 4: 5.63, 6: 5.62, 8: 5.56,
+// 12: 5.49, 16: 5.52; capping an instance's walks at 1/64 of its blocks, so
+// that small instances spread over more waves, took 5.69)
+constexpr int kThRtBatchWalk = 12;
 
 // capped at 96 VGPRs (5 waves/SIMD, as k_p1_assemble reaches uncapped)
 template <bool kLong>
