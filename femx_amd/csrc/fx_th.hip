@@ -31,8 +31,10 @@ __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per
 // Blocks per wave at 512^2 (A/B against fx_th_assemble's 290 us): all the
 // device holds at once (~20) 292 us, 1: 284, 2: 264, 4: 263, 8: 278; at
 // 256^2 (about 5 blocks per resident wave in all) 4 gained nothing (88 vs
-// 87 us): the last walks' tail is as long as the rest.
-constexpr int kThRtBlocksPerWave = 4;
+// 87 us): the last walks' tail is as long as the rest.  With the next
+// block's data loaded one block ahead, longer walks pay: at 512^2 a cap of a
+// fifth of each resident wave's share (4 blocks) 247 us, a third (6) 233 us.
+constexpr int kThRtBlocksPerWave = 8;
 
 __global__ __launch_bounds__(kThLanes* kThWaves) __attribute__((amdgpu_waves_per_eu(2))) void k_th_assemble_rt(ThAsmArgs a, int num_blocks,
                                                                        int tile)
@@ -217,7 +219,7 @@ int fx_th_assemble_rounds(fx_index ndof, fx_index num_elems, fx_index num_blocks
                       reinterpret_cast<const uint4*>(rt_meta), static_cast<int>(num_blocks), 1};
     // each wave walks up to kThRtBlocksPerWave blocks: long enough for the
     // pipeline to run on across block boundaries, short enough that the
-    // hardware still balances the walks: at most a fifth of what each
+    // hardware still balances the walks: at most a third of what each
     // resident wave takes in all (the resident waves: the CUs times the
     // workgroups that fit, at most two by the kernel's 2 waves/SIMD, by LDS),
     // so that the last walks' tail stays short.  A multiple of 8 workgroups,
@@ -230,7 +232,7 @@ int fx_th_assemble_rounds(fx_index ndof, fx_index num_elems, fx_index num_blocks
         static_cast<std::int64_t>(std::max(cus, 1)) * kThWaves
         * std::max<std::int64_t>(1, std::min<std::int64_t>(2, (160 * 1024) / static_cast<std::int64_t>(lds)));
     const std::int64_t bpw = std::max<std::int64_t>(
-        1, std::min<std::int64_t>(kThRtBlocksPerWave, num_blocks / (5 * std::max<std::int64_t>(resident, 1))));
+        1, std::min<std::int64_t>(kThRtBlocksPerWave, num_blocks / (3 * std::max<std::int64_t>(resident, 1))));
     const std::int64_t need = (static_cast<std::int64_t>(num_blocks) + kThWaves * bpw - 1) / (kThWaves * bpw);
     const int grid = static_cast<int>((need + 7) / 8 * 8);
     hipLaunchKernelGGL(k_th_assemble_rt, dim3(grid), dim3(kThLanes * kThWaves), lds,
